@@ -1,0 +1,233 @@
+"""armando_amd -- MI355X (gfx950) WCSPH time step of the Armando code.
+
+The product is the C-ABI shared library `armando_amd/libarmando.so` built
+from `armando_amd/csrc` (declared in `include/armando.h`).  This module is a
+thin ctypes mirror of that interface whose methods carry the reference's
+routine names (link_list, density, eos, internal_force, av_vel, single_step,
+time_integration) so that scripts and tests read like the reference loop.
+
+There is no CPU fallback: if the library is missing or no HIP device is
+present, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import cases  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libarmando.so")
+
+ARM_OK = 0
+ERRORS = {-1: "ARM_ERR_ARG", -2: "ARM_ERR_HIP", -3: "ARM_ERR_CAPACITY", -4: "ARM_ERR_DOMAIN",
+          -5: "ARM_ERR_UNSUPPORTED", -6: "ARM_ERR_STATE"}
+ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY = 1, 2, 4, 8, 16
+FP64, FP32 = 0, 1
+PASSES = ("keys", "sort", "reorder", "density", "force")
+
+EXPORTS = ("arm_create", "arm_destroy", "arm_last_error", "arm_domain_violations", "arm_upload",
+           "arm_download", "arm_download_rates", "arm_link_list", "arm_get_pairs",
+           "arm_get_cells", "arm_density", "arm_eos", "arm_internal_force", "arm_av_vel",
+           "arm_single_step", "arm_predictor", "arm_corrector", "arm_time_integration",
+           "arm_conservation", "arm_pass_times", "arm_reset_pass_times", "arm_set_timing")
+
+
+class ArmParams(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int), ("flags", C.c_int), ("pa_sph", C.c_int),
+        ("density_method", C.c_int),
+        ("h", C.c_double), ("mass", C.c_double), ("rho0", C.c_double), ("c0", C.c_double),
+        ("pmin", C.c_double), ("av_alpha", C.c_double), ("av_beta", C.c_double),
+        ("av_eta", C.c_double), ("ah_g1", C.c_double), ("ah_g2", C.c_double),
+        ("ah_eta", C.c_double), ("xsph_eps", C.c_double), ("gx", C.c_double),
+        ("gy", C.c_double), ("dt", C.c_double), ("ox", C.c_double), ("oy", C.c_double),
+        ("cell", C.c_double), ("nx", C.c_int), ("ny", C.c_int),
+    ]
+
+
+class ArmandoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def load_library(path=LIB_PATH):
+    """Load the C-ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} not built: run __graft_entry__.build()")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    dpn = C.POINTER(C.c_double)
+    sig = {
+        "arm_create": (C.c_int, [C.POINTER(ArmParams), C.c_int, C.POINTER(vp)]),
+        "arm_destroy": (None, [vp]),
+        "arm_last_error": (C.c_char_p, [vp]),
+        "arm_domain_violations": (C.c_int, [vp]),
+        "arm_upload": (C.c_int, [vp, C.c_int, C.c_int, dpn, dpn, dpn, dpn]),
+        "arm_download": (C.c_int, [vp, dpn, dpn, dpn, dpn, dpn]),
+        "arm_download_rates": (C.c_int, [vp, dpn, dpn, dpn, dpn, dpn]),
+        "arm_link_list": (C.c_int, [vp, C.POINTER(C.c_int)]),
+        "arm_get_pairs": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), _ip, _ip]),
+        "arm_get_cells": (C.c_int, [vp, _ip, _ip, _ip]),
+        "arm_density": (C.c_int, [vp, C.c_int]),
+        "arm_eos": (C.c_int, [vp]),
+        "arm_internal_force": (C.c_int, [vp, C.c_int]),
+        "arm_av_vel": (C.c_int, [vp]),
+        "arm_single_step": (C.c_int, [vp]),
+        "arm_predictor": (C.c_int, [vp]),
+        "arm_corrector": (C.c_int, [vp, C.c_int]),
+        "arm_time_integration": (C.c_int, [vp, C.c_int, C.c_int]),
+        "arm_conservation": (C.c_int, [vp, _dp]),
+        "arm_pass_times": (C.c_int, [vp, C.c_int, dpn, C.POINTER(C.c_int)]),
+        "arm_reset_pass_times": (C.c_int, [vp]),
+        "arm_set_timing": (C.c_int, [vp, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def params_from(case):
+    P = ArmParams()
+    for k, _ in ArmParams._fields_:
+        setattr(P, k, case[k])
+    return P
+
+
+class Simulation:
+    """A device context holding one particle set (one per host thread)."""
+
+    def __init__(self, case, capacity=None):
+        self.L = load_library()
+        self.case = case
+        self.n = int(case["n"])
+        self.np = int(case["np"])
+        self.params = params_from(case)
+        h = C.c_void_p()
+        rc = self.L.arm_create(C.byref(self.params), int(capacity or self.n), C.byref(h))
+        self.ctx = h
+        self._check(rc)
+        X = np.ascontiguousarray(np.stack([case["x"], case["y"]], 1), np.float64)
+        V = np.ascontiguousarray(np.stack([case["vx"], case["vy"]], 1), np.float64)
+        self._check(self.L.arm_upload(self.ctx, self.np, self.n - self.np, _ptr(X), _ptr(V),
+                                      _ptr(np.ascontiguousarray(case["rho"], np.float64)),
+                                      _ptr(np.ascontiguousarray(case["u"], np.float64))))
+
+    def _check(self, rc):
+        if rc != ARM_OK:
+            msg = self.L.arm_last_error(self.ctx).decode() if self.ctx else ""
+            raise ArmandoError(rc, msg)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.arm_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- the reference's routines ------------------------------------------
+    def link_list(self, count=False):
+        niac = C.c_int(0)
+        self._check(self.L.arm_link_list(self.ctx, C.byref(niac) if count else None))
+        return niac.value if count else None
+
+    def density(self, method=0):
+        self._check(self.L.arm_density(self.ctx, method))
+
+    def eos(self):
+        self._check(self.L.arm_eos(self.ctx))
+
+    def internal_force(self, flags):
+        self._check(self.L.arm_internal_force(self.ctx, flags))
+
+    def av_vel(self):
+        self._check(self.L.arm_av_vel(self.ctx))
+
+    def single_step(self):
+        self._check(self.L.arm_single_step(self.ctx))
+
+    def predictor(self):
+        self._check(self.L.arm_predictor(self.ctx))
+
+    def corrector(self, its):
+        self._check(self.L.arm_corrector(self.ctx, its))
+
+    def time_integration(self, its0, nsteps):
+        self._check(self.L.arm_time_integration(self.ctx, its0, nsteps))
+
+    # ---- data ---------------------------------------------------------------
+    def state(self):
+        n = self.n
+        X = np.zeros((n, 2))
+        V = np.zeros((n, 2))
+        rho = np.zeros(n)
+        p = np.zeros(n)
+        u = np.zeros(n)
+        self._check(self.L.arm_download(self.ctx, _ptr(X), _ptr(V), _ptr(rho), _ptr(p), _ptr(u)))
+        return dict(x=X[:, 0].copy(), y=X[:, 1].copy(), vx=V[:, 0].copy(), vy=V[:, 1].copy(),
+                    rho=rho, p=p, u=u)
+
+    def rates(self):
+        n = self.n
+        dr = np.zeros(n)
+        A = np.zeros((n, 2))
+        du = np.zeros(n)
+        W = np.zeros((n, 2))
+        dv = np.zeros(n)
+        self._check(self.L.arm_download_rates(self.ctx, _ptr(dr), _ptr(A), _ptr(du), _ptr(W),
+                                              _ptr(dv)))
+        return dict(drho=dr, ax=A[:, 0].copy(), ay=A[:, 1].copy(), du=du, avx=W[:, 0].copy(),
+                    avy=W[:, 1].copy(), divv=dv)
+
+    def pairs(self, cap=None):
+        niac = C.c_int(0)
+        cap = cap or self.n * 16 + 64
+        pi = np.zeros(cap, np.int32)
+        pj = np.zeros(cap, np.int32)
+        self._check(self.L.arm_get_pairs(self.ctx, cap, C.byref(niac), pi, pj))
+        return pi[:niac.value].copy(), pj[:niac.value].copy()
+
+    def cells(self):
+        ncell = int(self.case["nx"]) * int(self.case["ny"])
+        k = np.zeros(self.n, np.int32)
+        s = np.zeros(self.n, np.int32)
+        cs = np.zeros(ncell + 1, np.int32)
+        self._check(self.L.arm_get_cells(self.ctx, k, s, cs))
+        return k, s, cs
+
+    def conservation(self):
+        out = np.zeros(5)
+        self._check(self.L.arm_conservation(self.ctx, out))
+        return out
+
+    def set_timing(self, on=True):
+        self._check(self.L.arm_set_timing(self.ctx, 1 if on else 0))
+
+    def reset_pass_times(self):
+        self._check(self.L.arm_reset_pass_times(self.ctx))
+
+    def pass_times(self):
+        ms = (C.c_double * len(PASSES))()
+        cnt = (C.c_int * len(PASSES))()
+        self._check(self.L.arm_pass_times(self.ctx, len(PASSES), ms, cnt))
+        return {PASSES[k]: (ms[k], cnt[k]) for k in range(len(PASSES))}

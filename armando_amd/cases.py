@@ -1,0 +1,161 @@
+"""Initial conditions and parameter sets for the WCSPH step.
+
+Each case is a dict with the particle arrays (real particles first, wall
+particles after them, as in CERNsource/time_integration.f:104) and every
+scalar of `arm_params`.
+
+  dam_break(...)   2D dam break: a water column in a tank with three layers
+                   of wall particles (ArmandoList/armando2Dcpu.cu:1790-1908
+                   is the reference's own dam break; sizes, h, c0, jitter and
+                   the hydrostatic start follow the 720k workload below).
+  DAM_BREAK_720K   the 720,000-fluid-particle workload: 0.6 x 1.2 m column,
+                   3.2 x 1.6 m tank, dx = 1e-3 m, jitter +-0.01 dx.
+"""
+import math
+
+import numpy as np
+
+# flags (include/armando.h)
+ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY = 1, 2, 4, 8, 16
+FP64, FP32 = 0, 1
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed, count):
+    """`count` successive outputs of splitmix64 started at `seed` (uint64)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed, count):
+    """Doubles in [0, 1) from the top 53 bits of splitmix64."""
+    return (splitmix64(seed, count) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def grid_for(x, y, cell, margin_cells=4, top_extra=0.0):
+    """Hash grid covering the particles with a margin (cell = 2h)."""
+    ox = float(np.min(x)) - margin_cells * cell
+    oy = float(np.min(y)) - margin_cells * cell
+    nx = int(math.ceil((float(np.max(x)) - ox) / cell)) + margin_cells + 1
+    ny = int(math.ceil((float(np.max(y)) + top_extra - oy) / cell)) + margin_cells + 1
+    return ox, oy, nx, ny
+
+
+def dam_break(col_w=0.6, col_h=1.2, tank_w=3.2, tank_h=1.6, dx=1e-3, jitter=0.01, seed=12345,
+              h_fac=1.3, layers=3, precision=FP64, alpha=0.01, beta=0.0, eps=0.5,
+              g=9.81, cfl=0.2, heat=True):
+    """2D dam break with dynamic (density-integrating) wall particles.
+
+    Fluid: x = (i+1/2)dx + U(-j,j)dx, y = (k+1/2)dx + U(-j,j)dx for i < col_w/dx,
+    k < col_h/dx; the uniform draws come from splitmix64(seed), consumed row by
+    row (k outer, i inner), x before y.  Walls: `layers` rows of particles at
+    spacing dx outside [0, tank_w] x [0, tank_h] (bottom, left and right; the
+    bottom rows run under the side walls).  Hydrostatic start
+    rho = rho0 (1 + rho0 g (col_h - y)/B)^(1/7), B = rho0 c0^2/7,
+    c0 = 10 sqrt(g col_h); dt = cfl h / c0.
+    """
+    ni = int(round(col_w / dx))
+    nk = int(round(col_h / dx))
+    nf = ni * nk
+    I, K = np.meshgrid(np.arange(ni), np.arange(nk))
+    I = I.ravel()
+    K = K.ravel()
+    x = (I + 0.5) * dx
+    y = (K + 0.5) * dx
+    if jitter:
+        u = uniform01(seed, 2 * nf).reshape(nf, 2)
+        x = x + (2.0 * u[:, 0] - 1.0) * jitter * dx
+        y = y + (2.0 * u[:, 1] - 1.0) * jitter * dx
+    nxw = int(round(tank_w / dx))
+    nyw = int(round(tank_h / dx))
+    wx, wy = [], []
+    for L in range(layers):  # bottom, spanning the side walls
+        xs = (np.arange(-layers, nxw + layers) + 0.5) * dx
+        wx.append(xs)
+        wy.append(np.full(xs.size, -(L + 0.5) * dx))
+    for L in range(layers):  # left and right
+        ys = (np.arange(0, nyw) + 0.5) * dx
+        wx.append(np.full(ys.size, -(L + 0.5) * dx))
+        wy.append(ys)
+        wx.append(np.full(ys.size, tank_w + (L + 0.5) * dx))
+        wy.append(ys)
+    wx = np.concatenate(wx)
+    wy = np.concatenate(wy)
+    rho0 = 1000.0
+    c0 = 10.0 * math.sqrt(g * col_h)
+    B = rho0 * c0 * c0 / 7.0
+    h = h_fac * dx
+
+    def hydro(yy):
+        return rho0 * (1.0 + rho0 * g * (col_h - yy) / B) ** (1.0 / 7.0)
+
+    rho_f = hydro(y)
+    under = (wx <= col_w) & (wy <= col_h)
+    rho_w = np.where(under, hydro(np.minimum(wy, col_h)), rho0)
+    X = np.concatenate([x, wx])
+    Y = np.concatenate([y, wy])
+    n = X.size
+    cell = 2.0 * h
+    ox, oy, nxg, nyg = grid_for(X, Y, cell, top_extra=0.25 * tank_h)
+    flags = DBC | GRAVITY | AV_VEL | (ART_VISC if alpha or beta else 0) | (ART_HEAT if heat else 0)
+    return dict(
+        name="dam_break", n=n, np=nf, nv=n - nf,
+        x=X, y=Y, vx=np.zeros(n), vy=np.zeros(n), rho=np.concatenate([rho_f, rho_w]),
+        u=np.zeros(n),
+        precision=precision, flags=flags, pa_sph=0, density_method=0,
+        kernel=0, h=h, mass=rho0 * dx * dx, rho0=rho0, c0=c0, pmin=0.0,
+        av_alpha=alpha, av_beta=beta, av_eta=0.1,
+        ah_g1=0.1, ah_g2=1.0, ah_eta=0.1,
+        xsph_eps=eps, gx=0.0, gy=-g, dt=cfl * h / c0,
+        ox=ox, oy=oy, cell=cell, nx=nxg, ny=nyg,
+    )
+
+
+def dam_break_720k(precision=FP64):
+    """The 720,000-fluid-particle dam break (seed 12345)."""
+    return dam_break(precision=precision)
+
+
+def dam_break_small(scale=0.05, precision=FP64, seed=12345, jitter=0.01, dx=1e-3):
+    """Same physics at a reduced size: column/tank dimensions times `scale`."""
+    return dam_break(col_w=0.6 * scale, col_h=1.2 * scale, tank_w=3.2 * scale,
+                     tank_h=1.6 * scale, dx=dx, precision=precision, seed=seed, jitter=jitter)
+
+
+def box_lattice(nx=20, ny=15, dx=0.01, jitter=0.05, seed=7, vel=0.0, precision=FP64):
+    """Periodic-free lattice block of real particles only (no walls), with
+    jittered positions and seeded velocities, for routine-level parity."""
+    I, K = np.meshgrid(np.arange(nx), np.arange(ny))
+    I = I.ravel()
+    K = K.ravel()
+    n = I.size
+    r = uniform01(seed, 6 * n).reshape(n, 6)
+    x = (I + 0.5) * dx + (2 * r[:, 0] - 1) * jitter * dx
+    y = (K + 0.5) * dx + (2 * r[:, 1] - 1) * jitter * dx
+    vx = vel * (2 * r[:, 2] - 1)
+    vy = vel * (2 * r[:, 3] - 1)
+    rho = 1000.0 * (1.0 + 0.01 * (2 * r[:, 4] - 1))
+    u = 1.0 + 0.5 * r[:, 5]
+    h = 1.3 * dx
+    cell = 2.0 * h
+    ox, oy, nxg, nyg = grid_for(x, y, cell)
+    return dict(
+        name="box", n=n, np=n, nv=0, x=x, y=y, vx=vx, vy=vy, rho=rho, u=u,
+        precision=precision, flags=ART_VISC | ART_HEAT | AV_VEL, pa_sph=0, density_method=0,
+        kernel=0, h=h, mass=1000.0 * dx * dx, rho0=1000.0, c0=10.0, pmin=-1e30,
+        av_alpha=1.0, av_beta=1.0, av_eta=0.1, ah_g1=0.1, ah_g2=1.0, ah_eta=0.1,
+        xsph_eps=0.3, gx=0.0, gy=0.0, dt=1e-5, ox=ox, oy=oy, cell=cell, nx=nxg, ny=nyg,
+    )
+
+
+PARAM_KEYS = ("precision", "flags", "pa_sph", "density_method", "h", "mass", "rho0", "c0",
+              "pmin", "av_alpha", "av_beta", "av_eta", "ah_g1", "ah_g2", "ah_eta", "xsph_eps",
+              "gx", "gy", "dt", "ox", "oy", "cell", "nx", "ny")

@@ -1,0 +1,829 @@
+// armando.hip -- C ABI (include/armando.h) over the gfx950 WCSPH kernels.
+//
+// One context owns the particle state in HBM (structure of arrays, sorted by
+// (cell, id) after every link_list), the sort workspace and one HIP stream.
+// The public entry points mirror the reference's routines; the fused
+// arm_time_integration runs the same kernels back to back on the stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/armando.h"
+#include "radix_sort.h"
+#include "sph_kernels.h"
+
+using namespace arm;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+};
+
+template <typename R> struct Layout {
+  static constexpr bool f32 = sizeof(R) == 4;
+};
+
+}  // namespace
+
+struct arm_ctx {
+  arm_params prm{};
+  KParams K{};
+  int cap = 0;
+  int n = 0, np = 0;
+  bool loaded = false, sorted = false, have_rates = false;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  // state double buffer (the active set is A)
+  St<double> A64{}, B64{};
+  St<float> A32{}, B32{};
+  // sort workspace
+  uint32_t *key_by_id = nullptr, *slot_by_id = nullptr;
+  uint32_t *sk[2] = {nullptr, nullptr}, *sv[2] = {nullptr, nullptr};
+  uint32_t* hist = nullptr;
+  int ntiles = 0, sort_bits = 0, sort_passes = 0;
+  uint32_t* skey = nullptr;  // sorted keys of the current order
+  int* cstart = nullptr;
+  int* d_err = nullptr;
+  double* d_part = nullptr;
+  int* d_cnt = nullptr;
+  int domain_bad = 0;
+  std::string err;
+  // timing
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double pass_ms[ARM_PASS_COUNT] = {0};
+  int pass_cnt[ARM_PASS_COUNT] = {0};
+};
+
+namespace {
+
+int fail(arm_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail((c), ARM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+int grid_of(int n) { return (n + BLOCK - 1) / BLOCK; }
+
+template <typename T> int dalloc(arm_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(&q, count * sizeof(T));
+  if (e != hipSuccess) return fail(c, ARM_ERR_HIP, "hipMalloc(%zu) failed: %s", count * sizeof(T), hipGetErrorString(e));
+  c->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return ARM_OK;
+}
+
+template <typename R> int alloc_state(arm_ctx* c, St<R>& S, int cap) {
+  int rc = ARM_OK;
+#define AL(f, cnt) \
+  if ((rc = dalloc(c, &S.f, (cnt))) != ARM_OK) return rc;
+  AL(id, cap);
+  AL(xy, cap);
+  if (sizeof(R) == 4) {
+    AL(cxy, cap);
+    AL(off, cap);
+    AL(uf, cap);
+  }
+  AL(v, cap);
+  AL(rho, cap);
+  AL(u, cap);
+  AL(p, cap);
+  AL(psi, cap);
+  AL(drho, cap);
+  AL(du, cap);
+  AL(a, cap);
+  AL(av, cap);
+  AL(divv, cap);
+  AL(rho_min, cap);
+  AL(u_min, cap);
+  AL(v_min, cap);
+#undef AL
+  return ARM_OK;
+}
+
+bool is32(const arm_ctx* c) { return c->prm.precision == ARM_FP32; }
+
+// ---- timing ---------------------------------------------------------------
+struct PassTimer {
+  arm_ctx* c;
+  int pass;
+  PassTimer(arm_ctx* c_, int p) : c(c_), pass(p) {
+    if (c->timing) (void)hipEventRecord(c->ev0, c->stream);
+  }
+  ~PassTimer() {
+    if (!c->timing) return;
+    (void)hipEventRecord(c->ev1, c->stream);
+    (void)hipEventSynchronize(c->ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->pass_ms[pass] += ms;
+    c->pass_cnt[pass] += 1;
+  }
+};
+
+// ---- sort -------------------------------------------------------------------
+// Sorts (key_by_id, slot_by_id) into (skey, sval); returns the value array.
+int run_sort(arm_ctx* c, const uint32_t** sval_out) {
+  const int n = c->n;
+  const int ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+  const uint32_t* kin = c->key_by_id;
+  const uint32_t* vin = c->slot_by_id;
+  int dst = 0;
+  for (int ps = 0; ps < c->sort_passes; ++ps) {
+    const int shift = ps * c->sort_bits;
+    const int bits = std::min(c->sort_bits, std::max(1, 32 - shift));
+    const int bins = 1 << bits;
+    hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(SORT_THREADS), 0, c->stream, kin, n,
+                       shift, bins, ntiles, c->hist);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, c->stream, c->hist, bins * ntiles);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, c->stream, kin,
+                       vin, c->sk[dst], c->sv[dst], n, shift, bits, ntiles, c->hist);
+    kin = c->sk[dst];
+    vin = c->sv[dst];
+    dst ^= 1;
+  }
+  HIPCHK(c, hipGetLastError());
+  c->skey = const_cast<uint32_t*>(kin);
+  *sval_out = vin;
+  return ARM_OK;
+}
+
+template <typename R> St<R>& stA(arm_ctx* c);
+template <> St<double>& stA<double>(arm_ctx* c) { return c->A64; }
+template <> St<float>& stA<float>(arm_ctx* c) { return c->A32; }
+template <typename R> St<R>& stB(arm_ctx* c);
+template <> St<double>& stB<double>(arm_ctx* c) { return c->B64; }
+template <> St<float>& stB<float>(arm_ctx* c) { return c->B32; }
+
+int check_domain(arm_ctx* c) {
+  int bad = 0;
+  HIPCHK(c, hipMemcpyAsync(&bad, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (bad) {
+    c->domain_bad = bad;
+    HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+    return fail(c, ARM_ERR_DOMAIN, "%d particle(s) outside the hash grid", bad);
+  }
+  return ARM_OK;
+}
+
+// keys of the current state -> sort -> permute (all fields) + cell ranges
+template <typename R> int link_list_t(arm_ctx* c) {
+  const int n = c->n;
+  St<R>& A = stA<R>(c);
+  St<R>& B = stB<R>(c);
+  {
+    PassTimer tm(c, ARM_PASS_KEYS);
+    hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                       c->key_by_id, c->slot_by_id, c->d_err);
+  }
+  const uint32_t* sval = nullptr;
+  {
+    PassTimer tm(c, ARM_PASS_SORT);
+    int rc = run_sort(c, &sval);
+    if (rc) return rc;
+  }
+  {
+    PassTimer tm(c, ARM_PASS_REORDER);
+    hipLaunchKernelGGL(k_reorder_all<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
+                       c->K, c->skey, sval, c->cstart);
+  }
+  HIPCHK(c, hipGetLastError());
+  std::swap(A, B);
+  c->sorted = true;
+  return check_domain(c);
+}
+
+template <typename R> int density_t(arm_ctx* c, int method) {
+  St<R>& A = stA<R>(c);
+  PassTimer tm(c, ARM_PASS_DENSITY);
+  if (method == 1)
+    hipLaunchKernelGGL((k_density<R, true>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A,
+                       c->K, c->skey, c->cstart);
+  else
+    hipLaunchKernelGGL((k_density<R, false>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                       A, c->K, c->skey, c->cstart);
+  HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+template <typename R> int eos_t(arm_ctx* c) {
+  St<R>& A = stA<R>(c);
+  PassTimer tm(c, ARM_PASS_DENSITY);
+  hipLaunchKernelGGL(k_eos<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K);
+  HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+template <typename R, bool P, bool AV, bool AH, bool XS> void launch_force(arm_ctx* c) {
+  St<R>& A = stA<R>(c);
+  hipLaunchKernelGGL((k_force<R, P, AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
+                     c->stream, A, c->K, c->skey, c->cstart);
+}
+
+// dispatch over the compile-time flag combination
+template <typename R> int force_t(arm_ctx* c, bool press, int flags, bool xs) {
+  const bool av = (flags & ARM_ART_VISC) != 0, ah = (flags & ARM_ART_HEAT) != 0;
+  PassTimer tm(c, ARM_PASS_FORCE);
+  const int code = (press ? 8 : 0) | (av ? 4 : 0) | (ah ? 2 : 0) | (xs ? 1 : 0);
+  switch (code) {
+    case 8: launch_force<R, true, false, false, false>(c); break;
+    case 9: launch_force<R, true, false, false, true>(c); break;
+    case 10: launch_force<R, true, false, true, false>(c); break;
+    case 11: launch_force<R, true, false, true, true>(c); break;
+    case 12: launch_force<R, true, true, false, false>(c); break;
+    case 13: launch_force<R, true, true, false, true>(c); break;
+    case 14: launch_force<R, true, true, true, false>(c); break;
+    case 15: launch_force<R, true, true, true, true>(c); break;
+    case 1: launch_force<R, false, false, false, true>(c); break;
+    default: return fail(c, ARM_ERR_ARG, "force pass with nothing to compute");
+  }
+  HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+template <typename R> int zero_av(arm_ctx* c) {
+  St<R>& A = stA<R>(c);
+  HIPCHK(c, hipMemsetAsync(A.av, 0, sizeof(typename V2<R>::type) * (size_t)c->n, c->stream));
+  return ARM_OK;
+}
+
+template <typename R> int single_step_t(arm_ctx* c) {
+  int rc;
+  if ((rc = density_t<R>(c, c->prm.density_method))) return rc;
+  if ((rc = eos_t<R>(c))) return rc;
+  const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
+  if ((rc = force_t<R>(c, true, c->prm.flags, xs))) return rc;
+  if (!xs && (rc = zero_av<R>(c))) return rc;
+  c->have_rates = true;
+  return ARM_OK;
+}
+
+template <typename R> int predictor_t(arm_ctx* c) {
+  St<R>& A = stA<R>(c);
+  PassTimer tm(c, ARM_PASS_KEYS);
+  hipLaunchKernelGGL(k_predictor<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K);
+  HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+template <typename R> int corrector_t(arm_ctx* c, int its, bool keys) {
+  St<R>& A = stA<R>(c);
+  PassTimer tm(c, ARM_PASS_KEYS);
+  hipLaunchKernelGGL(k_corrector<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                     its, keys ? c->key_by_id : nullptr, keys ? c->slot_by_id : nullptr,
+                     c->d_err);
+  HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+// time_integration.f:131-235, fused: per step
+//   [keys of the current positions] -> sort -> reorder + predictor
+//   -> density -> eos -> forces (+ XSPH) -> corrector + keys of the next step
+template <typename R> int time_integration_t(arm_ctx* c, int its0, int nsteps) {
+  const int n = c->n;
+  const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
+  int rc;
+  {
+    St<R>& A = stA<R>(c);
+    PassTimer tm(c, ARM_PASS_KEYS);
+    hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                       c->key_by_id, c->slot_by_id, c->d_err);
+  }
+  if (!xs && (rc = zero_av<R>(c))) return rc;
+  for (int its = its0; its < its0 + nsteps; ++its) {
+    const uint32_t* sval = nullptr;
+    {
+      PassTimer tm(c, ARM_PASS_SORT);
+      if ((rc = run_sort(c, &sval))) return rc;
+    }
+    {
+      St<R>& A = stA<R>(c);
+      St<R>& B = stB<R>(c);
+      PassTimer tm(c, ARM_PASS_REORDER);
+      hipLaunchKernelGGL(k_reorder_pred<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
+                         c->K, c->skey, sval, c->cstart, its != 1 ? 1 : 0);
+      std::swap(A, B);
+    }
+    if (!xs && (rc = zero_av<R>(c))) return rc;
+    c->sorted = true;
+    if ((rc = density_t<R>(c, c->prm.density_method))) return rc;
+    if ((rc = eos_t<R>(c))) return rc;
+    if ((rc = force_t<R>(c, true, c->prm.flags, xs))) return rc;
+    if ((rc = corrector_t<R>(c, its, true))) return rc;
+  }
+  c->have_rates = true;
+  HIPCHK(c, hipGetLastError());
+  return check_domain(c);
+}
+
+template <typename R> int pairs_t(arm_ctx* c, int cap, int* niac, int* pi, int* pj) {
+  St<R>& A = stA<R>(c);
+  const int n = c->n;
+  hipLaunchKernelGGL(k_pair_count<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                     c->skey, c->cstart, c->d_cnt, nullptr, nullptr, nullptr);
+  HIPCHK(c, hipGetLastError());
+  std::vector<int> cnt(n);
+  HIPCHK(c, hipMemcpyAsync(cnt.data(), c->d_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  long long total = 0;
+  std::vector<int> offs(n);
+  for (int i = 0; i < n; ++i) {
+    offs[i] = (int)total;
+    total += cnt[i];
+  }
+  if (niac) *niac = (int)total;
+  if (!pi) return ARM_OK;
+  if (total > cap) return fail(c, ARM_ERR_CAPACITY, "%lld pairs exceed cap %d", total, cap);
+  int *d_off = nullptr, *d_i = nullptr, *d_j = nullptr;
+  HIPCHK(c, hipMalloc(&d_off, sizeof(int) * std::max(n, 1)));
+  HIPCHK(c, hipMalloc(&d_i, sizeof(int) * std::max<long long>(total, 1)));
+  HIPCHK(c, hipMalloc(&d_j, sizeof(int) * std::max<long long>(total, 1)));
+  (void)hipMemcpyAsync(d_off, offs.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream);
+  hipLaunchKernelGGL(k_pair_count<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                     c->skey, c->cstart, nullptr, d_off, d_i, d_j);
+  std::vector<int> vi(total), vj(total);
+  (void)hipMemcpyAsync(vi.data(), d_i, sizeof(int) * total, hipMemcpyDeviceToHost, c->stream);
+  (void)hipMemcpyAsync(vj.data(), d_j, sizeof(int) * total, hipMemcpyDeviceToHost, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_off);
+  (void)hipFree(d_i);
+  (void)hipFree(d_j);
+  if (e != hipSuccess) return fail(c, ARM_ERR_HIP, "pair emission failed: %s", hipGetErrorString(e));
+  std::vector<unsigned long long> kk(total);
+  for (long long k = 0; k < total; ++k)
+    kk[k] = ((unsigned long long)(unsigned)vi[k] << 32) | (unsigned)vj[k];
+  std::sort(kk.begin(), kk.end());
+  for (long long k = 0; k < total; ++k) {
+    pi[k] = (int)(kk[k] >> 32);
+    pj[k] = (int)(kk[k] & 0xffffffffu);
+  }
+  return ARM_OK;
+}
+
+// ---- host <-> device --------------------------------------------------------
+template <typename R> int upload_t(arm_ctx* c, const double* x, const double* v, const double* rho,
+                                   const double* p, const double* u) {
+  using R2 = typename V2<R>::type;
+  const int n = c->n;
+  St<R>& A = stA<R>(c);
+  std::vector<int32_t> id(n);
+  std::vector<double2> xy(n);
+  std::vector<R2> vv(n);
+  std::vector<R> rr(n), pp(n);
+  std::vector<double> uu(n);
+  std::vector<float> uf(n);
+  for (int i = 0; i < n; ++i) {
+    id[i] = i;
+    xy[i] = make_double2(x[2 * i], x[2 * i + 1]);
+    vv[i].x = (R)(v ? v[2 * i] : 0.0);
+    vv[i].y = (R)(v ? v[2 * i + 1] : 0.0);
+    const double r = rho ? rho[i] : c->prm.rho0;
+    rr[i] = Layout<R>::f32 ? (R)(r - c->prm.rho0) : (R)r;
+    pp[i] = (R)(p ? p[i] : 0.0);
+    uu[i] = u ? u[i] : 0.0;
+    uf[i] = (float)uu[i];
+  }
+  auto cp = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+  };
+  HIPCHK(c, cp(A.id, id.data(), sizeof(int32_t) * n));
+  HIPCHK(c, cp(A.xy, xy.data(), sizeof(double2) * n));
+  HIPCHK(c, cp(A.v, vv.data(), sizeof(R2) * n));
+  HIPCHK(c, cp(A.rho, rr.data(), sizeof(R) * n));
+  HIPCHK(c, cp(A.p, pp.data(), sizeof(R) * n));
+  HIPCHK(c, cp(A.u, uu.data(), sizeof(double) * n));
+  if (Layout<R>::f32) HIPCHK(c, cp(A.uf, uf.data(), sizeof(float) * n));
+  // rates start at zero
+  HIPCHK(c, hipMemsetAsync(A.drho, 0, sizeof(R) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.du, 0, sizeof(R) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.a, 0, sizeof(R2) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.av, 0, sizeof(R2) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.divv, 0, sizeof(R) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.psi, 0, sizeof(R) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.rho_min, 0, sizeof(R) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.u_min, 0, sizeof(double) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.v_min, 0, sizeof(R2) * n, c->stream));
+  // FP32 form: cell + offset from the FP64 positions (the keys kernel writes them)
+  if (Layout<R>::f32) {
+    hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                       c->key_by_id, c->slot_by_id, c->d_err);
+    HIPCHK(c, hipGetLastError());
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_domain(c);
+}
+
+template <typename T> int d2h(arm_ctx* c, std::vector<T>& dst, const T* src, int n) {
+  dst.resize(n);
+  HIPCHK(c, hipMemcpyAsync(dst.data(), src, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+  return ARM_OK;
+}
+
+template <typename R> int download_t(arm_ctx* c, double* x, double* v, double* rho, double* p,
+                                     double* u) {
+  using R2 = typename V2<R>::type;
+  const int n = c->n;
+  St<R>& A = stA<R>(c);
+  std::vector<int32_t> id;
+  std::vector<double2> xy;
+  std::vector<R2> vv;
+  std::vector<R> rr, pp;
+  std::vector<double> uu;
+  int rc;
+  if ((rc = d2h(c, id, A.id, n))) return rc;
+  if (x && (rc = d2h(c, xy, A.xy, n))) return rc;
+  if (v && (rc = d2h(c, vv, A.v, n))) return rc;
+  if (rho && (rc = d2h(c, rr, A.rho, n))) return rc;
+  if (p && (rc = d2h(c, pp, A.p, n))) return rc;
+  if (u && (rc = d2h(c, uu, A.u, n))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < n; ++s) {
+    const int i = id[s];
+    if (x) {
+      x[2 * i] = xy[s].x;
+      x[2 * i + 1] = xy[s].y;
+    }
+    if (v) {
+      v[2 * i] = (double)vv[s].x;
+      v[2 * i + 1] = (double)vv[s].y;
+    }
+    if (rho) rho[i] = Layout<R>::f32 ? c->prm.rho0 + (double)rr[s] : (double)rr[s];
+    if (p) p[i] = (double)pp[s];
+    if (u) u[i] = uu[s];
+  }
+  return ARM_OK;
+}
+
+template <typename R> int download_rates_t(arm_ctx* c, double* drho, double* dvdt, double* dudt,
+                                           double* av, double* divv) {
+  using R2 = typename V2<R>::type;
+  const int n = c->n;
+  St<R>& A = stA<R>(c);
+  std::vector<int32_t> id;
+  std::vector<R> dr, du, dv;
+  std::vector<R2> aa, ww;
+  int rc;
+  if ((rc = d2h(c, id, A.id, n))) return rc;
+  if (drho && (rc = d2h(c, dr, A.drho, n))) return rc;
+  if (dudt && (rc = d2h(c, du, A.du, n))) return rc;
+  if (divv && (rc = d2h(c, dv, A.divv, n))) return rc;
+  if (dvdt && (rc = d2h(c, aa, A.a, n))) return rc;
+  if (av && (rc = d2h(c, ww, A.av, n))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < n; ++s) {
+    const int i = id[s];
+    if (drho) drho[i] = (double)dr[s];
+    if (dudt) dudt[i] = (double)du[s];
+    if (divv) divv[i] = (double)dv[s];
+    if (dvdt) {
+      dvdt[2 * i] = (double)aa[s].x;
+      dvdt[2 * i + 1] = (double)aa[s].y;
+    }
+    if (av) {
+      av[2 * i] = (double)ww[s].x;
+      av[2 * i + 1] = (double)ww[s].y;
+    }
+  }
+  return ARM_OK;
+}
+
+template <typename R> int conservation_t(arm_ctx* c, double out[5]) {
+  St<R>& A = stA<R>(c);
+  const int nb = grid_of(c->n);
+  hipLaunchKernelGGL(k_conserve<R>, dim3(nb), dim3(BLOCK), 0, c->stream, A, c->K, c->d_part);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> part((size_t)nb * 5);
+  HIPCHK(c, hipMemcpyAsync(part.data(), c->d_part, sizeof(double) * part.size(),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 5; ++k) out[k] = 0.0;
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 5; ++k) out[k] += part[(size_t)b * 5 + k];
+  return ARM_OK;
+}
+
+#define DISPATCH(c, call) (is32(c) ? call<float> : call<double>)
+
+int need_sorted(arm_ctx* c) {
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (!c->sorted) return fail(c, ARM_ERR_STATE, "arm_link_list has not run on this state");
+  return ARM_OK;
+}
+
+}  // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+extern "C" {
+
+int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
+  if (!prm || !out || capacity <= 0) return ARM_ERR_ARG;
+  *out = nullptr;
+  arm_ctx* c = new (std::nothrow) arm_ctx();
+  if (!c) return ARM_ERR_ARG;
+  c->prm = *prm;
+  const arm_params& P = *prm;
+  int rc = ARM_OK;
+  if (P.precision != ARM_FP64 && P.precision != ARM_FP32)
+    rc = fail(c, ARM_ERR_ARG, "precision must be ARM_FP64 or ARM_FP32");
+  else if (!(P.h > 0) || !(P.mass > 0) || !(P.rho0 > 0) || !(P.dt > 0))
+    rc = fail(c, ARM_ERR_ARG, "h, mass, rho0 and dt must be positive");
+  else if (!(P.cell >= 2.0 * P.h))
+    rc = fail(c, ARM_ERR_ARG, "cell (%g) must be at least the kernel support 2h (%g)", P.cell, 2 * P.h);
+  else if (P.nx < 1 || P.ny < 1 || (long long)P.nx * P.ny >= (1LL << 30))
+    rc = fail(c, ARM_ERR_ARG, "bad grid %d x %d", P.nx, P.ny);
+  else if (P.precision == ARM_FP32 && (P.nx > 65535 || P.ny > 65535))
+    rc = fail(c, ARM_ERR_ARG, "FP32 form packs cell coordinates in 16 bits");
+  else if (P.pa_sph != 0 && P.pa_sph != 1)
+    rc = fail(c, ARM_ERR_UNSUPPORTED, "pa_sph must be 0 or 1");
+  else if (P.density_method != 0 && P.density_method != 1)
+    rc = fail(c, ARM_ERR_UNSUPPORTED, "density_method must be 0 (continuity) or 1 (summation)");
+  if (rc) {
+    *out = c;  // so that arm_last_error can report
+    return rc;
+  }
+  KParams& K = c->K;
+  K.flags = P.flags;
+  K.all_rate = (P.flags & ARM_DBC) ? 1 : 0;
+  K.pa_sph = P.pa_sph;
+  K.density_method = P.density_method;
+  K.nx = P.nx;
+  K.ny = P.ny;
+  K.ncell = P.nx * P.ny;
+  K.h = P.h;
+  K.invh = 1.0 / P.h;
+  K.mass = P.mass;
+  K.rho0 = P.rho0;
+  K.c0 = P.c0;
+  K.pmin = P.pmin;
+  K.B = P.rho0 * P.c0 * P.c0 / 7.0;
+  K.av_alpha = P.av_alpha;
+  K.av_beta = P.av_beta;
+  K.av_e2 = (P.h * P.av_eta) * (P.h * P.av_eta);
+  K.ah_g1 = P.ah_g1;
+  K.ah_g2 = P.ah_g2;
+  K.ah_e2 = (P.h * P.ah_eta) * (P.h * P.ah_eta);
+  K.xsph_eps = P.xsph_eps;
+  K.gx = P.gx;
+  K.gy = P.gy;
+  K.dt = P.dt;
+  K.ox = P.ox;
+  K.oy = P.oy;
+  K.cell = P.cell;
+  K.cut2 = (2.0 * P.h) * (2.0 * P.h);
+  K.wnorm = 7.0 / (4.0 * M_PI * P.h * P.h);
+  K.C1 = 0.625 * K.wnorm / (P.h * P.h);
+  K.psi0 = P.ah_g1 * P.h * P.c0;
+  K.g2h2 = P.ah_g2 * P.h * P.h;
+  K.cellf = (float)P.cell;
+  K.cut2f = (float)K.cut2;
+
+  c->cap = capacity;
+  const int ncell = K.ncell;
+  int nbits = 1;
+  while (nbits < 31 && (1LL << nbits) < (long long)ncell) ++nbits;
+  c->sort_passes = (nbits + SORT_MAX_BITS - 1) / SORT_MAX_BITS;
+  c->sort_bits = (nbits + c->sort_passes - 1) / c->sort_passes;
+  c->ntiles = (capacity + SORT_TILE - 1) / SORT_TILE;
+
+  hipError_t e = hipStreamCreate(&c->stream);
+  if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  if (!rc) rc = is32(c) ? alloc_state(c, c->A32, capacity) : alloc_state(c, c->A64, capacity);
+  if (!rc) rc = is32(c) ? alloc_state(c, c->B32, capacity) : alloc_state(c, c->B64, capacity);
+  if (!rc) rc = dalloc(c, &c->key_by_id, capacity);
+  if (!rc) rc = dalloc(c, &c->slot_by_id, capacity);
+  for (int k = 0; k < 2 && !rc; ++k) {
+    rc = dalloc(c, &c->sk[k], capacity);
+    if (!rc) rc = dalloc(c, &c->sv[k], capacity);
+  }
+  if (!rc) rc = dalloc(c, &c->hist, (size_t)(1 << c->sort_bits) * c->ntiles);
+  if (!rc) rc = dalloc(c, &c->cstart, (size_t)ncell + 1);
+  if (!rc) rc = dalloc(c, &c->d_err, 1);
+  if (!rc) rc = dalloc(c, &c->d_part, (size_t)grid_of(capacity) * 5);
+  if (!rc) rc = dalloc(c, &c->d_cnt, capacity);
+  if (!rc) {
+    e = hipMemset(c->d_err, 0, sizeof(int));
+    if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipMemset: %s", hipGetErrorString(e));
+  }
+  if (!rc) {
+    e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return rc;
+}
+
+void arm_destroy(arm_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* arm_last_error(const arm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int arm_domain_violations(const arm_ctx* c) { return c ? c->domain_bad : 0; }
+
+int arm_upload(arm_ctx* c, int np, int nv, const double* x, const double* v, const double* rho,
+               const double* u) {
+  if (!c || !x || np < 0 || nv < 0) return ARM_ERR_ARG;
+  if (np + nv > c->cap) return fail(c, ARM_ERR_CAPACITY, "%d particles exceed capacity %d", np + nv, c->cap);
+  if (np + nv < 1) return fail(c, ARM_ERR_ARG, "no particles");
+  c->n = np + nv;
+  c->np = np;
+  c->K.n = c->n;
+  c->K.np = np;
+  c->sorted = false;
+  c->have_rates = false;
+  c->domain_bad = 0;
+  int rc = DISPATCH(c, upload_t)(c, x, v, rho, nullptr, u);
+  c->loaded = (rc == ARM_OK);
+  return rc;
+}
+
+int arm_download(arm_ctx* c, double* x, double* v, double* rho, double* p, double* u) {
+  if (!c) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  return DISPATCH(c, download_t)(c, x, v, rho, p, u);
+}
+
+int arm_download_rates(arm_ctx* c, double* drho, double* dvdt, double* dudt, double* av,
+                       double* divv) {
+  if (!c) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  return DISPATCH(c, download_rates_t)(c, drho, dvdt, dudt, av, divv);
+}
+
+int arm_link_list(arm_ctx* c, int* niac) {
+  if (!c) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  int rc = DISPATCH(c, link_list_t)(c);
+  if (rc) return rc;
+  if (niac) rc = DISPATCH(c, pairs_t)(c, 0, niac, nullptr, nullptr);
+  return rc;
+}
+
+int arm_get_pairs(arm_ctx* c, int cap, int* niac, int* pair_i, int* pair_j) {
+  if (!c || !pair_i || !pair_j || cap < 0) return ARM_ERR_ARG;
+  int rc = need_sorted(c);
+  if (rc) return rc;
+  return DISPATCH(c, pairs_t)(c, cap, niac, pair_i, pair_j);
+}
+
+int arm_get_cells(arm_ctx* c, int32_t* key_sorted, int32_t* id_sorted, int32_t* cell_start) {
+  if (!c) return ARM_ERR_ARG;
+  int rc = need_sorted(c);
+  if (rc) return rc;
+  const int32_t* dids = is32(c) ? c->A32.id : c->A64.id;
+  if (key_sorted)
+    HIPCHK(c, hipMemcpyAsync(key_sorted, c->skey, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+  if (id_sorted)
+    HIPCHK(c, hipMemcpyAsync(id_sorted, dids, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+  if (cell_start)
+    HIPCHK(c, hipMemcpyAsync(cell_start, c->cstart, sizeof(int32_t) * ((size_t)c->K.ncell + 1),
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ARM_OK;
+}
+
+int arm_density(arm_ctx* c, int method) {
+  if (!c) return ARM_ERR_ARG;
+  if (method != 0 && method != 1) return fail(c, ARM_ERR_UNSUPPORTED, "density method %d", method);
+  int rc = need_sorted(c);
+  if (rc) return rc;
+  rc = DISPATCH(c, density_t)(c, method);
+  if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int arm_eos(arm_ctx* c) {
+  if (!c) return ARM_ERR_ARG;
+  int rc = need_sorted(c);
+  if (rc) return rc;
+  rc = DISPATCH(c, eos_t)(c);
+  if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int arm_internal_force(arm_ctx* c, int flags) {
+  if (!c) return ARM_ERR_ARG;
+  int rc = need_sorted(c);
+  if (rc) return rc;
+  rc = DISPATCH(c, force_t)(c, true, flags, false);
+  if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int arm_av_vel(arm_ctx* c) {
+  if (!c) return ARM_ERR_ARG;
+  int rc = need_sorted(c);
+  if (rc) return rc;
+  rc = DISPATCH(c, force_t)(c, false, 0, true);
+  if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int arm_single_step(arm_ctx* c) {
+  if (!c) return ARM_ERR_ARG;
+  int rc = need_sorted(c);
+  if (rc) return rc;
+  rc = DISPATCH(c, single_step_t)(c);
+  if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int arm_predictor(arm_ctx* c) {
+  if (!c) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  int rc = DISPATCH(c, predictor_t)(c);
+  if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int arm_corrector(arm_ctx* c, int its) {
+  if (!c) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (!c->have_rates) return fail(c, ARM_ERR_STATE, "no rates: run arm_single_step first");
+  int rc = DISPATCH(c, corrector_t)(c, its, false);
+  if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
+  // positions moved: the cell order must be rebuilt before the next gather
+  if (!rc && is32(c)) {
+    // refresh cell + offset of the FP32 form from the FP64 accumulators
+    St<float>& A = c->A32;
+    hipLaunchKernelGGL(k_keys<float>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                       c->key_by_id, c->slot_by_id, c->d_err);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  c->sorted = false;
+  return rc ? rc : check_domain(c);
+}
+
+int arm_time_integration(arm_ctx* c, int its0, int nsteps) {
+  if (!c || its0 < 1 || nsteps < 0) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (nsteps == 0) return ARM_OK;
+  int rc = DISPATCH(c, time_integration_t)(c, its0, nsteps);
+  // the last corrector moved the particles after the last sort
+  c->sorted = false;
+  return rc;
+}
+
+int arm_conservation(arm_ctx* c, double out[5]) {
+  if (!c || !out) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  return DISPATCH(c, conservation_t)(c, out);
+}
+
+int arm_pass_times(arm_ctx* c, int n, double* ms, int* counts) {
+  if (!c || n < 0) return ARM_ERR_ARG;
+  for (int k = 0; k < n && k < ARM_PASS_COUNT; ++k) {
+    if (ms) ms[k] = c->pass_ms[k];
+    if (counts) counts[k] = c->pass_cnt[k];
+  }
+  return ARM_OK;
+}
+
+int arm_reset_pass_times(arm_ctx* c) {
+  if (!c) return ARM_ERR_ARG;
+  for (int k = 0; k < ARM_PASS_COUNT; ++k) {
+    c->pass_ms[k] = 0;
+    c->pass_cnt[k] = 0;
+  }
+  return ARM_OK;
+}
+
+int arm_set_timing(arm_ctx* c, int on) {
+  if (!c) return ARM_ERR_ARG;
+  c->timing = on != 0;
+  return ARM_OK;
+}
+
+}  // extern "C"

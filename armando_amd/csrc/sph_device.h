@@ -1,0 +1,141 @@
+// sph_device.h -- device-side state, parameters and pair physics of the
+// WCSPH step, for the FP64 plain form (R = double) and the FP32 chosen form
+// (R = float).  Included by armando.hip only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace arm {
+
+template <typename R> struct V2;
+template <> struct V2<double> { using type = double2; };
+template <> struct V2<float> { using type = float2; };
+
+// Scalar parameters, passed to every kernel by value.
+struct KParams {
+  int n, np;
+  int all_rate;  // 1: walls integrate density/energy too (dynamic boundary)
+  int flags, pa_sph, density_method;
+  int nx, ny, ncell;
+  double h, invh, mass, rho0, c0, pmin, B;  // B = rho0 c0^2 / 7
+  double av_alpha, av_beta, av_e2;          // av_e2 = (h eta)^2
+  double ah_g1, ah_g2, ah_e2;
+  double xsph_eps, gx, gy, dt;
+  double ox, oy, cell;
+  double cut2;   // (2h)*(2h)
+  double wnorm;  // Wendland 2D normalisation 7/(4 pi h^2)
+  double C1;     // grad W = C1 (q-2)^3 (x_i - x_j), C1 = 5 wnorm / (8 h^2)
+  double psi0;   // g1 h c0 (constant part of the heat coefficient)
+  double g2h2;   // g2 h h
+  float cellf, cut2f;
+};
+
+// Particle state in sorted (cell, id) order.  FP64 form: xy holds the
+// positions, rho the density.  FP32 form: xy holds the FP64 position
+// accumulators, (cxy, off) the integer cell and FP32 in-cell offset used by
+// the pair arithmetic, rho the density perturbation rho - rho0, uf a float
+// copy of the FP64 energy accumulator u.
+template <typename R> struct St {
+  using R2 = typename V2<R>::type;
+  int32_t* id;
+  double2* xy;
+  uint32_t* cxy;
+  float2* off;
+  R2* v;
+  R* rho;
+  double* u;
+  float* uf;
+  R* p;
+  R* psi;    // art_heat coefficient psi_i = g1 h c + g2 h^2 (|divv| - divv)
+  R* drho;   // density rate
+  R* du;     // energy rate
+  R2* a;     // acceleration
+  R2* av;    // XSPH correction
+  R* divv;   // velocity divergence
+  R* rho_min;
+  double* u_min;
+  R2* v_min;
+};
+
+__device__ __forceinline__ bool is_rate(const KParams& K, int id) {
+  return K.all_rate || id < K.np;
+}
+
+// ---- positions seen by the pair arithmetic ----------------------------
+struct PosD { double x, y; };
+struct PosF { int cx, cy; float ox, oy; };
+
+__device__ __forceinline__ PosD load_pos(const St<double>& S, int t) {
+  const double2 q = S.xy[t];
+  return {q.x, q.y};
+}
+__device__ __forceinline__ PosF load_pos(const St<float>& S, int t) {
+  const uint32_t c = S.cxy[t];
+  const float2 o = S.off[t];
+  return {int(c & 0xffffu), int(c >> 16), o.x, o.y};
+}
+
+// x_i - x_j.  The module is compiled without floating-point contraction,
+// so these are the exact operation sequences the oracle restates.
+__device__ __forceinline__ void delta(const PosD& a, const PosD& b, const KParams&, double& dx,
+                                      double& dy) {
+  dx = a.x - b.x;
+  dy = a.y - b.y;
+}
+__device__ __forceinline__ void delta(const PosF& a, const PosF& b, const KParams& K, float& dx,
+                                      float& dy) {
+  const float tx = a.ox - b.ox;
+  const float ux = (float)(a.cx - b.cx) * K.cellf;
+  dx = tx + ux;
+  const float ty = a.oy - b.oy;
+  const float uy = (float)(a.cy - b.cy) * K.cellf;
+  dy = ty + uy;
+}
+
+template <typename R> __device__ __forceinline__ R cutoff2(const KParams& K);
+template <> __device__ __forceinline__ double cutoff2<double>(const KParams& K) { return K.cut2; }
+template <> __device__ __forceinline__ float cutoff2<float>(const KParams& K) { return K.cut2f; }
+
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Density in the pair arithmetic: absolute (FP64 form) or rho0 + perturbation.
+__device__ __forceinline__ double dens(const KParams&, double r) { return r; }
+__device__ __forceinline__ float dens(const KParams& K, float r) { return (float)K.rho0 + r; }
+
+// Compensated (Kahan) accumulator for the FP32 form; plain for FP64.
+template <typename R> struct Acc {
+  R s = 0;
+  __device__ __forceinline__ void add(R x) { s = s + x; }
+  __device__ __forceinline__ R get() const { return s; }
+};
+template <> struct Acc<float> {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ float get() const { return s; }
+};
+
+// Tait pressure from the stored density (eos.f90:274).
+__device__ __forceinline__ double tait(const KParams& K, double rho) {
+  const double t = rho / K.rho0;
+  const double t2 = t * t;
+  const double t4 = t2 * t2;
+  return K.rho0 * K.c0 * K.c0 / 7.0 * (t4 * t2 * t - 1.0);
+}
+// FP32 form: (1+x)^7 - 1 expanded in x = rho'/rho0 so that small density
+// perturbations keep full relative precision.
+__device__ __forceinline__ float tait(const KParams& K, float rp) {
+  const float x = rp / (float)K.rho0;
+  const float poly = 7.f + x * (21.f + x * (35.f + x * (35.f + x * (21.f + x * (7.f + x)))));
+  return (float)K.B * (x * poly);
+}
+
+}  // namespace arm

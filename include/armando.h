@@ -1,0 +1,149 @@
+/*
+ * armando.h -- C ABI of the MI355X (gfx950) WCSPH time step.
+ *
+ * One extern "C" entry point per routine of the reference's step
+ *   link_list -> density -> eos -> internal_force + art_visc + art_heat
+ *   -> av_vel -> time_integration
+ * with the same argument meaning, operating on particle state that a
+ * context keeps resident in HBM.  Plain pointers and sizes only.
+ *
+ * Conventions (reference file:line in brackets):
+ *  - Particles 0..np-1 are real (fluid), np..np+nv-1 are virtual (wall)
+ *    particles appended after them [CERNsource/time_integration.f:104].
+ *  - Host arrays use the Fortran layout x(dim, n): x and y of particle i
+ *    are adjacent [CERNsource/time_integration.f:62].  Indices are 0-based.
+ *  - Scalars that the reference hard-codes as PARAMETERs (alpha, beta, eta
+ *    of art_visc.f:38-44; g1, g2, eta of art_heat.f:38-40; eps of
+ *    av_vel.f:35) are fields of arm_params.
+ *  - Every call returns ARM_OK (0) or a negative ARM_ERR_* code; the
+ *    reference's `stop` on overflow [CERNsource/link_list.f:63-67,169-173]
+ *    becomes ARM_ERR_CAPACITY / ARM_ERR_DOMAIN, never a silent truncation.
+ *  - A context is used from one host thread at a time; all work is ordered
+ *    on the context's HIP stream and every call returns after it finished.
+ */
+#ifndef ARMANDO_H
+#define ARMANDO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARM_OK 0
+#define ARM_ERR_ARG (-1)         /* bad argument */
+#define ARM_ERR_HIP (-2)         /* HIP runtime error */
+#define ARM_ERR_CAPACITY (-3)    /* a caller-sized output is too small */
+#define ARM_ERR_DOMAIN (-4)      /* a particle left the hash grid */
+#define ARM_ERR_UNSUPPORTED (-5) /* option outside what this build implements */
+#define ARM_ERR_STATE (-6)       /* call order violated (e.g. no link_list yet) */
+
+/* precision forms */
+#define ARM_FP64 0 /* plain form: every field and pair term in FP64 */
+#define ARM_FP32 1 /* chosen form: FP32 pair arithmetic, cell + FP32 offset
+                      positions, Kahan-compensated rate sums, FP64 position
+                      and energy accumulators */
+
+/* flags (single_step.f logicals) */
+#define ARM_ART_VISC 1 /* visc_artificial -> art_visc.f */
+#define ARM_ART_HEAT 2 /* heat_artificial -> art_heat.f */
+#define ARM_AV_VEL 4   /* average_velocity -> av_vel.f (XSPH) */
+#define ARM_DBC 8      /* walls integrate density and energy, velocity held
+                          [ArmandoList/armando2Dcpu.cu:832-907] */
+#define ARM_GRAVITY 16 /* body force (gx, gy) on real particles
+                          [ArmandoList/armando2Dcpu.cu:937-958] */
+
+typedef struct arm_params {
+  int precision;      /* ARM_FP64 or ARM_FP32 */
+  int flags;          /* ARM_* bit set */
+  int pa_sph;         /* 0: -(p_i+p_j)/(rho_i rho_j), 1: -(p_i/rho_i^2 + p_j/rho_j^2)
+                         [CERNsource/internal_force.f:193-335] */
+  int density_method; /* 0: con_density, 1: sum_density [single_step.f:108-117] */
+  double h;           /* smoothing length (uniform; sle = 0) */
+  double mass;        /* particle mass (uniform) */
+  double rho0, c0, pmin; /* Tait [FORTRANsource/eos.f90:255-283] */
+  double av_alpha, av_beta, av_eta;
+  double ah_g1, ah_g2, ah_eta;
+  double xsph_eps;
+  double gx, gy;
+  double dt;
+  /* hash grid: cell key = ix + iy*nx, ix = (int)((x - ox)/cell)
+     [ArmandoList/armando2Dcpu.cu:2080-2101]; cell >= 2h required */
+  double ox, oy, cell;
+  int nx, ny;
+} arm_params;
+
+typedef struct arm_ctx arm_ctx;
+
+/* Context with room for `capacity` particles on the current HIP device. */
+int arm_create(const arm_params* prm, int capacity, arm_ctx** out);
+void arm_destroy(arm_ctx* ctx);
+/* Text of the last error on this context ("" if none). */
+const char* arm_last_error(const arm_ctx* ctx);
+/* Number of particles outside the grid seen by the last failing call. */
+int arm_domain_violations(const arm_ctx* ctx);
+
+/* Upload the state; x, v are x(2, np+nv) (Fortran layout); rho, u have
+   np+nv entries.  Replaces [CERNsource/sph.f:104-108] allocation + input. */
+int arm_upload(arm_ctx* ctx, int np, int nv, const double* x, const double* v,
+               const double* rho, const double* u);
+/* Download in the original particle order.  Any pointer may be NULL. */
+int arm_download(arm_ctx* ctx, double* x, double* v, double* rho, double* p, double* u);
+/* Rates of the last single_step, original order; any pointer may be NULL.
+   dvdt and av are (2, n); divv is the velocity divergence of art_heat. */
+int arm_download_rates(arm_ctx* ctx, double* drho, double* dvdt, double* dudt, double* av,
+                       double* divv);
+
+/* link_list(nt, x, h, niac, pair_i, pair_j, skf) [CERNsource/link_list.f:1]:
+   hashes, sorts by (cell, particle) and builds cell ranges on the device.
+   niac (may be NULL) receives the number of unique pairs i<j within 2h. */
+int arm_link_list(arm_ctx* ctx, int* niac);
+/* Materialise the unique pairs (i<j, original ids), sorted by (i, j), for
+   parity checks; ARM_ERR_CAPACITY if more than cap. */
+int arm_get_pairs(arm_ctx* ctx, int cap, int* niac, int* pair_i, int* pair_j);
+/* Sorted cell keys, sorted particle ids and cell_start[nx*ny + 1]
+   (cell c holds sorted positions cell_start[c] .. cell_start[c+1]-1). */
+int arm_get_cells(arm_ctx* ctx, int32_t* key_sorted, int32_t* id_sorted, int32_t* cell_start);
+
+/* sum_density / con_density [CERNsource/density.f:1,166] (method 1 / 0). */
+int arm_density(arm_ctx* ctx, int method);
+/* Tait EOS on the current density [FORTRANsource/eos.f90:255-283]. */
+int arm_eos(arm_ctx* ctx);
+/* int_force [internal_force.f:1] plus, by flags, art_visc [art_visc.f:1]
+   and art_heat [art_heat.f:1], fused in one gather; adds gravity when
+   ARM_GRAVITY is set in the context flags. */
+int arm_internal_force(arm_ctx* ctx, int flags);
+/* av_vel [CERNsource/av_vel.f:1] (XSPH velocity correction). */
+int arm_av_vel(arm_ctx* ctx);
+/* single_step [CERNsource/single_step.f:1]: density, eos, forces, av_vel. */
+int arm_single_step(arm_ctx* ctx);
+/* The two halves of the leapfrog around single_step
+   [CERNsource/time_integration.f:136-154 and :186-217]. */
+int arm_predictor(arm_ctx* ctx);
+int arm_corrector(arm_ctx* ctx, int its);
+/* time_integration [CERNsource/time_integration.f:131-235] for steps
+   its0 .. its0+nsteps-1 (refresh_step = 1), fused on the device. */
+int arm_time_integration(arm_ctx* ctx, int its0, int nsteps);
+
+/* FP64 sums over real particles: mass, momentum x, momentum y, kinetic
+   energy, internal energy. */
+int arm_conservation(arm_ctx* ctx, double out[5]);
+
+/* Per-pass device time accumulated since the last reset, in ms, measured
+   with HIP events on the context stream; counts[] = launches per pass.
+   Pass indices: see ARM_PASS_*.  n is the array length. */
+#define ARM_PASS_KEYS 0    /* corrector/keys */
+#define ARM_PASS_SORT 1    /* radix sort */
+#define ARM_PASS_REORDER 2 /* permute + predictor + cell ranges */
+#define ARM_PASS_DENSITY 3 /* density + eos */
+#define ARM_PASS_FORCE 4   /* internal_force + art_visc + art_heat + av_vel */
+#define ARM_PASS_COUNT 5
+int arm_pass_times(arm_ctx* ctx, int n, double* ms, int* counts);
+int arm_reset_pass_times(arm_ctx* ctx);
+/* Enable (1) or disable (0) per-pass event timing. */
+int arm_set_timing(arm_ctx* ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

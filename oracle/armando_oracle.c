@@ -1,0 +1,1028 @@
+/*
+ * armando_oracle.c -- CPU restatement of Armando's 2D WCSPH time step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker of the HIP library.  Built with
+ * -ffp-contract=off so that every product and sum rounds on its own, as
+ * the reference's scalar code does.  Citations are to the reference tree.
+ */
+#define _DEFAULT_SOURCE
+#include "armando_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* options.inc:21 assigns a default-real literal to pi, so the Fortran
+   value is the REAL(4) rounding of pi widened to REAL(8). */
+static const double PI_F = 3.1415927410125732;
+/* kernel.f:41 writes 2./3. as a REAL(4) constant. */
+static const double TWO_THIRDS_F = 0.6666666865348816;
+
+/* ------------------------------------------------------------------ */
+/* kernel                                                               */
+/* ------------------------------------------------------------------ */
+
+/* Faithful per-component form of kernel.f (skf = 0) and the Wendland C2 of
+   armando2Dcpu.cu:116-172.  dwdx = grad_i W for dx = x_i - x_j. */
+static double kern(const orc_params* P, double r, double dx, double dy, double* dwx,
+                   double* dwy) {
+  const double h = P->h;
+  const double q = r / h;
+  if (P->kernel == ORC_K_WENDLAND2D) {
+    /* W = a (1 - q/2)^4 (1 + 2q), dW/dr = (a/h)(5/8) q (q-2)^3, a = 7/(4 pi h^2).
+       grad W = dW/dr * dx / r = (5a/(8h^2)) (q-2)^3 dx, regular at r = 0. */
+    if (q < 2.0) {
+      const double a = 7.0 / (4.0 * M_PI * h * h);
+      const double s = 1.0 - 0.5 * q;
+      const double s2 = s * s;
+      const double t = q - 2.0;
+      const double F = (0.625 * a / (h * h)) * (t * t * t);
+      *dwx = F * dx;
+      *dwy = F * dy;
+      return a * (s2 * s2) * (1.0 + 2.0 * q);
+    }
+    *dwx = 0.0;
+    *dwy = 0.0;
+    return 0.0;
+  }
+  /* kernel.f:31-56, cubic spline */
+  double factor;
+  if (P->kernel == ORC_K_CUBIC2D)
+    factor = 15.0 / (7.0 * PI_F * (h * h));
+  else
+    factor = 3.0 / (2.0 * PI_F * (h * h * h));
+  if (q >= 0.0 && q <= 1.0) {
+    const double g = factor * (-2.0 + 1.5 * q) / (h * h);
+    *dwx = g * dx;
+    *dwy = g * dy;
+    return factor * (TWO_THIRDS_F - q * q + (q * q * q) / 2.0);
+  } else if (q > 1.0 && q <= 2.0) {
+    const double g = -factor * 1.0 / 6.0 * 3.0 * ((2.0 - q) * (2.0 - q)) / h;
+    *dwx = g * (dx / r);
+    *dwy = g * (dy / r);
+    return factor * 1.0 / 6.0 * ((2.0 - q) * (2.0 - q) * (2.0 - q));
+  }
+  *dwx = 0.0;
+  *dwy = 0.0;
+  return 0.0;
+}
+
+double orc_kernel(const orc_params* P, double r, double* F) {
+  double dwx, dwy;
+  /* probe with dx = r so that dwdx = F * r */
+  double w = kern(P, r, r, 0.0, &dwx, &dwy);
+  *F = (r > 0.0) ? dwx / r : 0.0;
+  if (r == 0.0) {
+    double wx, wy;
+    kern(P, 0.0, 1.0, 0.0, &wx, &wy);
+    *F = wx;
+  }
+  return w;
+}
+
+/* ------------------------------------------------------------------ */
+/* neighbour search                                                     */
+/* ------------------------------------------------------------------ */
+
+int orc_link_list(const orc_params* P, int nt, const double* x, const double* y, int cap,
+                  int* niac, int* pair_i, int* pair_j) {
+  /* link_list.f:29-221 for dim = 2 and a uniform h (skf = 0: scale_k = 2) */
+  const int scale_k = 2;
+  double gmin[2] = {1.0e30, 1.0e30}, gmax[2] = {-1.0e30, -1.0e30};
+  double mh = 0.0;
+  for (int i = 0; i < nt; ++i) {
+    if (x[i] < gmin[0]) gmin[0] = x[i];
+    if (x[i] > gmax[0]) gmax[0] = x[i];
+    if (y[i] < gmin[1]) gmin[1] = y[i];
+    if (y[i] > gmax[1]) gmax[1] = y[i];
+    if (P->h > mh) mh = P->h;
+  }
+  int gn[2];
+  for (int d = 0; d < 2; ++d) {
+    gmin[d] = gmin[d] - mh;
+    gmax[d] = gmax[d] + mh;
+    gn[d] = (int)((gmax[d] - gmin[d]) / (scale_k * mh)) + 1;
+  }
+  const long gnt = (long)gn[0] * gn[1];
+  int* gindex = (int*)calloc((size_t)gnt + 1, sizeof(int));
+  int* gpart = (int*)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int));
+  int* gcell = (int*)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int));
+  if (!gindex || !gpart || !gcell) {
+    free(gindex); free(gpart); free(gcell);
+    return -2;
+  }
+  for (int i = 0; i < nt; ++i) {
+    int ix = (int)((x[i] - gmin[0]) / (scale_k * mh));
+    int iy = (int)((y[i] - gmin[1]) / (scale_k * mh));
+    gcell[i] = gn[0] * iy + ix;
+    gindex[gcell[i]]++;
+  }
+  /* running start of each cell; particles are placed in increasing i */
+  int counter = 0;
+  for (long g = 0; g < gnt; ++g) {
+    int c = gindex[g];
+    gindex[g] = counter;
+    counter += c;
+  }
+  gindex[gnt] = counter;
+  {
+    int* fill = (int*)malloc((size_t)gnt * sizeof(int) + sizeof(int));
+    memcpy(fill, gindex, (size_t)gnt * sizeof(int));
+    for (int i = 0; i < nt; ++i) gpart[fill[gcell[i]]++] = i;
+    free(fill);
+  }
+  const double cut2 = (scale_k * mh) * (scale_k * mh);
+  int n = 0;
+  int rc = 0;
+  for (int i = 0; i < nt - 1 && rc == 0; ++i) {
+    int ijk[2], lo[2], hi[2];
+    ijk[0] = (int)((x[i] - gmin[0]) / (scale_k * mh));
+    ijk[1] = (int)((y[i] - gmin[1]) / (scale_k * mh));
+    for (int d = 0; d < 2; ++d) {
+      lo[d] = ijk[d] - 1;
+      hi[d] = ijk[d] + 1;
+      if (lo[d] < 0) lo[d] = 0;
+      if (hi[d] >= gn[d]) hi[d] = gn[d] - 1;
+    }
+    for (int jg = lo[1]; jg <= hi[1] && rc == 0; ++jg) {
+      for (int ig = lo[0]; ig <= hi[0] && rc == 0; ++ig) {
+        const long g = (long)gn[0] * jg + ig;
+        for (int k = gindex[g]; k < gindex[g + 1]; ++k) {
+          const int j = gpart[k];
+          if (j > i) {
+            const double dx = x[i] - x[j];
+            const double dy = y[i] - y[j];
+            double dr2 = dx * dx;
+            dr2 = dr2 + dy * dy;
+            if (dr2 < cut2) {
+              if (n < cap) {
+                pair_i[n] = i;
+                pair_j[n] = j;
+                n++;
+              } else {
+                rc = -1;
+                break;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  *niac = n;
+  free(gindex);
+  free(gpart);
+  free(gcell);
+  return rc;
+}
+
+int orc_direct_find(const orc_params* P, int nt, const double* x, const double* y, int cap,
+                    int* niac, int* pair_i, int* pair_j) {
+  /* direct_find.f:36-60: all i < j with r^2 < (scale_k * mh)^2 */
+  const double cut2 = (2.0 * P->h) * (2.0 * P->h);
+  int n = 0;
+  for (int i = 0; i < nt - 1; ++i)
+    for (int j = i + 1; j < nt; ++j) {
+      const double dx = x[i] - x[j];
+      const double dy = y[i] - y[j];
+      double dr2 = dx * dx;
+      dr2 = dr2 + dy * dy;
+      if (dr2 < cut2) {
+        if (n >= cap) {
+          *niac = n;
+          return -1;
+        }
+        pair_i[n] = i;
+        pair_j[n] = j;
+        n++;
+      }
+    }
+  *niac = n;
+  return 0;
+}
+
+static int key_of(const orc_params* P, double x, double y) {
+  /* armando2Dcpu.cu:2080-2101 (updateHashHost) in FP64 */
+  const double tx = (x - P->ox) / P->cell;
+  const double ty = (y - P->oy) / P->cell;
+  if (!(tx >= 0.0) || !(ty >= 0.0)) return -1;
+  const int ix = (int)tx;
+  const int iy = (int)ty;
+  if (ix >= P->nx || iy >= P->ny) return -1;
+  return ix + iy * P->nx;
+}
+
+int orc_keys(const orc_params* P, int n, const double* x, const double* y, int32_t* key) {
+  int bad = 0;
+  for (int i = 0; i < n; ++i) {
+    key[i] = key_of(P, x[i], y[i]);
+    if (key[i] < 0) bad++;
+  }
+  return bad;
+}
+
+void orc_sort_cells(int n, const int32_t* key, int ncell, int32_t* sorted_id,
+                    int32_t* cell_start) {
+  /* counting sort; input is listed in id order so the result is ordered
+     by (cell, id), the order a stable radix sort of (key, id) produces */
+  memset(cell_start, 0, sizeof(int32_t) * (size_t)(ncell + 1));
+  for (int i = 0; i < n; ++i)
+    if (key[i] >= 0) cell_start[key[i] + 1]++;
+  for (int c = 0; c < ncell; ++c) cell_start[c + 1] += cell_start[c];
+  int32_t* fill = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncell > 0 ? ncell : 1));
+  memcpy(fill, cell_start, sizeof(int32_t) * (size_t)ncell);
+  for (int i = 0; i < n; ++i)
+    if (key[i] >= 0) sorted_id[fill[key[i]]++] = i;
+  free(fill);
+}
+
+/* ------------------------------------------------------------------ */
+/* faithful pair routines                                               */
+/* ------------------------------------------------------------------ */
+
+static inline void pair_geom(const orc_params* P, int i, int j, const double* x,
+                             const double* y, double* dx, double* dy, double* dr2,
+                             double* w, double* dwx, double* dwy) {
+  *dx = x[i] - x[j];
+  *dy = y[i] - y[j];
+  double r2 = (*dx) * (*dx);
+  r2 = r2 + (*dy) * (*dy);
+  *dr2 = r2;
+  *w = kern(P, sqrt(r2), *dx, *dy, dwx, dwy);
+}
+
+void orc_sum_density(const orc_params* P, int nrate, int niac, const int* pi, const int* pj,
+                     const double* x, const double* y, double* rho) {
+  /* density.f:1-62 */
+  double dwx, dwy;
+  const double selfdens = kern(P, 0.0, 0.0, 0.0, &dwx, &dwy);
+  for (int i = 0; i < nrate; ++i) rho[i] = selfdens * P->mass;
+  for (int k = 0; k < niac; ++k) {
+    const int i = pi[k], j = pj[k];
+    double dx, dy, dr2, w;
+    pair_geom(P, i, j, x, y, &dx, &dy, &dr2, &w, &dwx, &dwy);
+    if (i < nrate) rho[i] = rho[i] + P->mass * w;
+    if (j < nrate) rho[j] = rho[j] + P->mass * w;
+  }
+}
+
+void orc_con_density(const orc_params* P, int nrate, int niac, const int* pi, const int* pj,
+                     const double* x, const double* y, const double* vx, const double* vy,
+                     double* drho) {
+  /* density.f:166-223 */
+  for (int i = 0; i < nrate; ++i) drho[i] = 0.0;
+  for (int k = 0; k < niac; ++k) {
+    const int i = pi[k], j = pj[k];
+    double dx, dy, dr2, w, dwx, dwy;
+    pair_geom(P, i, j, x, y, &dx, &dy, &dr2, &w, &dwx, &dwy);
+    double vcc = 0.0;
+    vcc = vcc + (vx[i] - vx[j]) * dwx;
+    vcc = vcc + (vy[i] - vy[j]) * dwy;
+    if (i < nrate) drho[i] = drho[i] + P->mass * vcc;
+    if (j < nrate) drho[j] = drho[j] + P->mass * vcc;
+  }
+}
+
+void orc_int_force(const orc_params* P, int nt, int niac, const int* pi, const int* pj,
+                   const double* x, const double* y, const double* vx, const double* vy,
+                   const double* rho, const double* p, double* ax, double* ay, double* du) {
+  /* internal_force.f:57-71 (zeroing), :178-259 (pa_sph 0), :309-430
+     (pa_sph 1), :433-435 (du = tdsdt + du/2); no physical viscosity */
+  const double m = P->mass;
+  for (int i = 0; i < nt; ++i) {
+    ax[i] = 0.0;
+    ay[i] = 0.0;
+    du[i] = 0.0;
+  }
+  for (int k = 0; k < niac; ++k) {
+    const int i = pi[k], j = pj[k];
+    double dx, dy, dr2, w, dwx, dwy;
+    pair_geom(P, i, j, x, y, &dx, &dy, &dr2, &w, &dwx, &dwy);
+    double he = 0.0, hv;
+    if (P->pa_sph == 0) {
+      hv = -(p[i] + p[j]) * dwx;
+      he = he + (vx[j] - vx[i]) * hv;
+      hv = hv / (rho[i] * rho[j]);
+      ax[i] = ax[i] + m * hv;
+      ax[j] = ax[j] - m * hv;
+      hv = -(p[i] + p[j]) * dwy;
+      he = he + (vy[j] - vy[i]) * hv;
+      hv = hv / (rho[i] * rho[j]);
+      ay[i] = ay[i] + m * hv;
+      ay[j] = ay[j] - m * hv;
+      he = he / (rho[i] * rho[j]);
+    } else {
+      const double a = p[i] / (rho[i] * rho[i]) + p[j] / (rho[j] * rho[j]);
+      hv = -a * dwx;
+      he = he + (vx[j] - vx[i]) * hv;
+      ax[i] = ax[i] + m * hv;
+      ax[j] = ax[j] - m * hv;
+      hv = -a * dwy;
+      he = he + (vy[j] - vy[i]) * hv;
+      ay[i] = ay[i] + m * hv;
+      ay[j] = ay[j] - m * hv;
+    }
+    du[i] = du[i] + m * he;
+    du[j] = du[j] + m * he;
+  }
+  for (int i = 0; i < nt; ++i) du[i] = 0.0 + 0.5 * du[i];
+}
+
+void orc_art_visc(const orc_params* P, int niac, const int* pi, const int* pj,
+                  const double* x, const double* y, const double* vx, const double* vy,
+                  const double* rho, const double* c, double* ax, double* ay, double* du) {
+  /* art_visc.f:55-101 with alpha, beta, eta from the parameters */
+  const double m = P->mass;
+  for (int k = 0; k < niac; ++k) {
+    const int i = pi[k], j = pj[k];
+    double dx, dy, dr2, w, dwx, dwy;
+    pair_geom(P, i, j, x, y, &dx, &dy, &dr2, &w, &dwx, &dwy);
+    const double mh = P->h;
+    double vr = 0.0, vdwdx = 0.0, dv;
+    dv = vx[i] - vx[j];
+    vr = vr + dv * dx;
+    vdwdx = vdwdx + dv * dwx;
+    dv = vy[i] - vy[j];
+    vr = vr + dv * dy;
+    vdwdx = vdwdx + dv * dwy;
+    if (vr < 0.0) {
+      const double phi = (mh * vr) / (dr2 + (mh * P->av_eta) * (mh * P->av_eta));
+      const double mc = 0.5 * (c[i] + c[j]);
+      const double mrho = 0.5 * (rho[i] + rho[j]);
+      const double piv = (P->av_beta * phi - P->av_alpha * mc) * phi / mrho;
+      ax[i] = ax[i] - m * piv * dwx;
+      ax[j] = ax[j] + m * piv * dwx;
+      ay[i] = ay[i] - m * piv * dwy;
+      ay[j] = ay[j] + m * piv * dwy;
+      du[i] = du[i] + 0.5 * m * piv * vdwdx;
+      du[j] = du[j] + 0.5 * m * piv * vdwdx;
+    }
+  }
+}
+
+void orc_art_heat(const orc_params* P, int nt, int niac, const int* pi, const int* pj,
+                  const double* x, const double* y, const double* vx, const double* vy,
+                  const double* rho, const double* u, const double* c, double* du,
+                  double* divv_out) {
+  /* art_heat.f:42-97 with g1, g2, eta from the parameters */
+  const double m = P->mass;
+  double* divv = (double*)calloc((size_t)(nt > 0 ? nt : 1), sizeof(double));
+  for (int k = 0; k < niac; ++k) {
+    const int i = pi[k], j = pj[k];
+    double dx, dy, dr2, w, dwx, dwy;
+    pair_geom(P, i, j, x, y, &dx, &dy, &dr2, &w, &dwx, &dwy);
+    double vdwdx = 0.0;
+    vdwdx = vdwdx + (vx[j] - vx[i]) * dwx;
+    vdwdx = vdwdx + (vy[j] - vy[i]) * dwy;
+    divv[i] = divv[i] + m / rho[i] * vdwdx;
+    divv[j] = divv[j] + m / rho[j] * vdwdx;
+  }
+  for (int k = 0; k < niac; ++k) {
+    const int i = pi[k], j = pj[k];
+    double dx, dy, dr2, w, dwx, dwy;
+    pair_geom(P, i, j, x, y, &dx, &dy, &dr2, &w, &dwx, &dwy);
+    const double mh = P->h, h = P->h;
+    const double mrho = 0.5 * (rho[i] + rho[j]);
+    double rdwdx = 0.0;
+    rdwdx = rdwdx + dx * dwx;
+    rdwdx = rdwdx + dy * dwy;
+    const double psi_i = P->ah_g1 * h * c[i] + P->ah_g2 * h * h * (fabs(divv[i]) - divv[i]);
+    const double psi_j = P->ah_g1 * h * c[j] + P->ah_g2 * h * h * (fabs(divv[j]) - divv[j]);
+    const double psi_ij = 0.5 * (psi_i + psi_j);
+    const double psi =
+        (2.0 * psi_ij) / (mrho * (dr2 + (mh * P->ah_eta) * (mh * P->ah_eta))) * rdwdx;
+    du[i] = du[i] + m * psi * (u[i] - u[j]);
+    du[j] = du[j] + m * psi * (u[j] - u[i]);
+  }
+  if (divv_out) memcpy(divv_out, divv, sizeof(double) * (size_t)nt);
+  free(divv);
+}
+
+void orc_av_vel(const orc_params* P, int niac, const int* pi, const int* pj,
+                const double* x, const double* y, const double* vx, const double* vy,
+                const double* rho, double* avx, double* avy) {
+  /* av_vel.f:37-56 with eps from the parameters */
+  const double m = P->mass, eps = P->xsph_eps;
+  for (int k = 0; k < niac; ++k) {
+    const int i = pi[k], j = pj[k];
+    double dx, dy, dr2, w, dwx, dwy;
+    pair_geom(P, i, j, x, y, &dx, &dy, &dr2, &w, &dwx, &dwy);
+    const double mrho = (rho[i] + rho[j]) / 2.0;
+    double dv = vx[i] - vx[j];
+    avx[i] = avx[i] - eps * m / mrho * dv * w;
+    avx[j] = avx[j] + eps * m / mrho * dv * w;
+    dv = vy[i] - vy[j];
+    avy[i] = avy[i] - eps * m / mrho * dv * w;
+    avy[j] = avy[j] + eps * m / mrho * dv * w;
+  }
+}
+
+static inline double tait_p(const orc_params* P, double rho) {
+  /* eos.f90:274: p = rho0 c0^2 / 7 ((rho/rho0)^7 - 1) */
+  const double t = rho / P->rho0;
+  const double t2 = t * t;
+  const double t4 = t2 * t2;
+  return P->rho0 * P->c0 * P->c0 / 7.0 * (t4 * t2 * t - 1.0);
+}
+
+void orc_eos_tait(const orc_params* P, int n, double* rho, double* p, double* c) {
+  /* eos.f90:255-283 */
+  for (int i = 0; i < n; ++i) {
+    p[i] = tait_p(P, rho[i]);
+    c[i] = P->c0;
+    if (p[i] < P->pmin) {
+      p[i] = 0.0;
+      rho[i] = P->rho0;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* state, single step, integration                                      */
+/* ------------------------------------------------------------------ */
+
+orc_state* orc_state_new(int n, int np) {
+  orc_state* s = (orc_state*)calloc(1, sizeof(orc_state));
+  s->n = n;
+  s->np = np;
+  double** f[] = {&s->x,   &s->y,   &s->vx,  &s->vy,   &s->rho,     &s->u,
+                  &s->p,   &s->c,   &s->drho, &s->du,  &s->ax,      &s->ay,
+                  &s->avx, &s->avy, &s->divv, &s->rho_min, &s->u_min, &s->vx_min,
+                  &s->vy_min};
+  for (size_t k = 0; k < sizeof(f) / sizeof(f[0]); ++k)
+    *f[k] = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+  return s;
+}
+
+void orc_state_free(orc_state* s) {
+  if (!s) return;
+  double* f[] = {s->x,   s->y,   s->vx,  s->vy,   s->rho,     s->u,      s->p,
+                 s->c,   s->drho, s->du, s->ax,   s->ay,      s->avx,    s->avy,
+                 s->divv, s->rho_min, s->u_min, s->vx_min, s->vy_min};
+  for (size_t k = 0; k < sizeof(f) / sizeof(f[0]); ++k) free(f[k]);
+  free(s);
+}
+
+static int nrate_of(const orc_params* P, const orc_state* s) {
+  return (P->flags & ORC_DBC) ? s->n : s->np;
+}
+
+int orc_single_step(const orc_params* P, orc_state* s, int niac, const int* pi,
+                    const int* pj) {
+  /* single_step.f:92-234 */
+  const int n = s->n, np = s->np, nr = nrate_of(P, s);
+  double* avdv_x = (double*)calloc((size_t)n, sizeof(double));
+  double* avdv_y = (double*)calloc((size_t)n, sizeof(double));
+  double* avdu = (double*)calloc((size_t)n, sizeof(double));
+  double* ahdu = (double*)calloc((size_t)n, sizeof(double));
+  for (int i = 0; i < n; ++i) {
+    s->avx[i] = 0.0;
+    s->avy[i] = 0.0;
+    s->divv[i] = 0.0;
+  }
+  if (P->density_method == 0) {
+    for (int i = 0; i < n; ++i) s->drho[i] = 0.0;
+    orc_con_density(P, nr, niac, pi, pj, s->x, s->y, s->vx, s->vy, s->drho);
+  } else {
+    orc_sum_density(P, nr, niac, pi, pj, s->x, s->y, s->rho);
+  }
+  orc_eos_tait(P, nr, s->rho, s->p, s->c);
+  orc_int_force(P, n, niac, pi, pj, s->x, s->y, s->vx, s->vy, s->rho, s->p, s->ax, s->ay,
+                s->du);
+  if (P->flags & ORC_ART_VISC)
+    orc_art_visc(P, niac, pi, pj, s->x, s->y, s->vx, s->vy, s->rho, s->c, avdv_x, avdv_y,
+                 avdu);
+  if (P->flags & ORC_ART_HEAT)
+    orc_art_heat(P, n, niac, pi, pj, s->x, s->y, s->vx, s->vy, s->rho, s->u, s->c, ahdu,
+                 s->divv);
+  if (P->flags & ORC_AV_VEL)
+    orc_av_vel(P, niac, pi, pj, s->x, s->y, s->vx, s->vy, s->rho, s->avx, s->avy);
+  const int grav = (P->flags & ORC_GRAVITY) != 0;
+  for (int i = 0; i < n; ++i) {
+    const double ex = (grav && i < np) ? P->gx : 0.0;
+    const double ey = (grav && i < np) ? P->gy : 0.0;
+    s->ax[i] = s->ax[i] + ex + avdv_x[i];
+    s->ay[i] = s->ay[i] + ey + avdv_y[i];
+    s->du[i] = s->du[i] + avdu[i] + ahdu[i];
+  }
+  free(avdv_x);
+  free(avdv_y);
+  free(avdu);
+  free(ahdu);
+  return 0;
+}
+
+/* Cells of the fixed grid, built once per gather call. */
+typedef struct {
+  int32_t* key;
+  int32_t* sorted;
+  int32_t* start;
+  int ncell;
+} cells_t;
+
+static int build_cells(const orc_params* P, const orc_state* s, cells_t* C) {
+  C->ncell = P->nx * P->ny;
+  C->key = (int32_t*)malloc(sizeof(int32_t) * (size_t)s->n);
+  C->sorted = (int32_t*)malloc(sizeof(int32_t) * (size_t)s->n);
+  C->start = (int32_t*)malloc(sizeof(int32_t) * ((size_t)C->ncell + 1));
+  int bad = orc_keys(P, s->n, s->x, s->y, C->key);
+  if (bad) return bad;
+  orc_sort_cells(s->n, C->key, C->ncell, C->sorted, C->start);
+  return 0;
+}
+
+static void free_cells(cells_t* C) {
+  free(C->key);
+  free(C->sorted);
+  free(C->start);
+}
+
+int orc_single_step_gather(const orc_params* P, orc_state* s) {
+  const int n = s->n, np = s->np, nr = nrate_of(P, s);
+  const double m = P->mass, h = P->h;
+  const double cut2 = (2.0 * P->h) * (2.0 * P->h);
+  cells_t C;
+  int bad = build_cells(P, s, &C);
+  if (bad) {
+    free_cells(&C);
+    return bad;
+  }
+  const int nx = P->nx, ny = P->ny;
+  const double* x = s->x;
+  const double* y = s->y;
+  const double* vx = s->vx;
+  const double* vy = s->vy;
+  /* pass A: continuity (or summation) */
+  double w0dx, w0dy;
+  const double w0 = kern(P, 0.0, 0.0, 0.0, &w0dx, &w0dy);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    const int key = C.key[i];
+    const int cx = key % nx, cy = key / nx;
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < nx ? cx + 1 : nx - 1;
+    double dr = 0.0, rs = m * w0;
+    for (int ry = cy - 1; ry <= cy + 1; ++ry) {
+      if (ry < 0 || ry >= ny) continue;
+      const int t0 = C.start[ry * nx + x0], t1 = C.start[ry * nx + x1 + 1];
+      for (int t = t0; t < t1; ++t) {
+        const int j = C.sorted[t];
+        if (j == i) continue;
+        const double dx = x[i] - x[j], dy = y[i] - y[j];
+        double r2 = dx * dx;
+        r2 = r2 + dy * dy;
+        if (!(r2 < cut2)) continue;
+        double dwx, dwy;
+        const double w = kern(P, sqrt(r2), dx, dy, &dwx, &dwy);
+        double vcc = (vx[i] - vx[j]) * dwx;
+        vcc = vcc + (vy[i] - vy[j]) * dwy;
+        dr = dr + m * vcc;
+        rs = rs + m * w;
+      }
+    }
+    if (i < nr) {
+      if (P->density_method == 0)
+        s->drho[i] = dr;
+      else
+        s->rho[i] = rs;
+    } else {
+      s->drho[i] = 0.0;
+    }
+    s->divv[i] = -dr; /* divided by rho after the EOS */
+  }
+  orc_eos_tait(P, nr, s->rho, s->p, s->c);
+  double* psi = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const double dv = s->divv[i] / s->rho[i];
+    s->divv[i] = dv;
+    psi[i] = P->ah_g1 * h * s->c[i] + P->ah_g2 * h * h * (fabs(dv) - dv);
+  }
+  const int grav = (P->flags & ORC_GRAVITY) != 0;
+  const int f_av = (P->flags & ORC_ART_VISC) != 0;
+  const int f_ah = (P->flags & ORC_ART_HEAT) != 0;
+  const int f_xs = (P->flags & ORC_AV_VEL) != 0;
+  const double* rho = s->rho;
+  const double* p = s->p;
+  const double* c = s->c;
+  const double* u = s->u;
+  const double av_e2 = (h * P->av_eta) * (h * P->av_eta);
+  const double ah_e2 = (h * P->ah_eta) * (h * P->ah_eta);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    const int key = C.key[i];
+    const int cx = key % nx, cy = key / nx;
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < nx ? cx + 1 : nx - 1;
+    double fx = 0.0, fy = 0.0, due = 0.0, avfx = 0.0, avfy = 0.0, avdu = 0.0, ahdu = 0.0;
+    double xsx = 0.0, xsy = 0.0;
+    const double pr_i = p[i] / (rho[i] * rho[i]);
+    for (int ry = cy - 1; ry <= cy + 1; ++ry) {
+      if (ry < 0 || ry >= ny) continue;
+      const int t0 = C.start[ry * nx + x0], t1 = C.start[ry * nx + x1 + 1];
+      for (int t = t0; t < t1; ++t) {
+        const int j = C.sorted[t];
+        if (j == i) continue;
+        const double dx = x[i] - x[j], dy = y[i] - y[j];
+        double r2 = dx * dx;
+        r2 = r2 + dy * dy;
+        if (!(r2 < cut2)) continue;
+        double dwx, dwy;
+        const double w = kern(P, sqrt(r2), dx, dy, &dwx, &dwy);
+        const double dvx = vx[i] - vx[j], dvy = vy[i] - vy[j];
+        double vdw = dvx * dwx;
+        vdw = vdw + dvy * dwy;
+        double a;
+        if (P->pa_sph == 0)
+          a = (p[i] + p[j]) / (rho[i] * rho[j]);
+        else
+          a = pr_i + p[j] / (rho[j] * rho[j]);
+        fx = fx - m * a * dwx;
+        fy = fy - m * a * dwy;
+        due = due + m * a * vdw;
+        const double mrho = 0.5 * (rho[i] + rho[j]);
+        if (f_av) {
+          double vr = dvx * dx;
+          vr = vr + dvy * dy;
+          if (vr < 0.0) {
+            const double phi = (h * vr) / (r2 + av_e2);
+            const double mc = 0.5 * (c[i] + c[j]);
+            const double piv = (P->av_beta * phi - P->av_alpha * mc) * phi / mrho;
+            avfx = avfx - m * piv * dwx;
+            avfy = avfy - m * piv * dwy;
+            avdu = avdu + 0.5 * m * piv * vdw;
+          }
+        }
+        if (f_ah) {
+          double rdw = dx * dwx;
+          rdw = rdw + dy * dwy;
+          const double pij = 0.5 * (psi[i] + psi[j]);
+          const double ps = (2.0 * pij) / (mrho * (r2 + ah_e2)) * rdw;
+          ahdu = ahdu + m * ps * (u[i] - u[j]);
+        }
+        if (f_xs) {
+          xsx = xsx - P->xsph_eps * m / mrho * dvx * w;
+          xsy = xsy - P->xsph_eps * m / mrho * dvy * w;
+        }
+      }
+    }
+    const double ex = (grav && i < np) ? P->gx : 0.0;
+    const double ey = (grav && i < np) ? P->gy : 0.0;
+    s->ax[i] = fx + ex + avfx;
+    s->ay[i] = fy + ey + avfy;
+    s->du[i] = 0.5 * due + avdu + ahdu;
+    s->avx[i] = xsx;
+    s->avy[i] = xsy;
+  }
+  free(psi);
+  free_cells(&C);
+  return 0;
+}
+
+void orc_predictor(const orc_params* P, orc_state* s) {
+  /* time_integration.f:136-154; walls (DBC) integrate rho and u only */
+  const double dt = P->dt;
+  const int nr = nrate_of(P, s);
+  for (int i = 0; i < nr; ++i) {
+    s->u_min[i] = s->u[i];
+    s->u[i] = s->u[i] + (dt / 2.0) * s->du[i];
+    if (s->u[i] < 0.0) s->u[i] = 0.0;
+    if (P->density_method == 0) {
+      s->rho_min[i] = s->rho[i];
+      s->rho[i] = s->rho[i] + (dt / 2.0) * s->drho[i];
+    }
+    if (i < s->np) {
+      s->vx_min[i] = s->vx[i];
+      s->vx[i] = s->vx[i] + (dt / 2.0) * s->ax[i];
+      s->vy_min[i] = s->vy[i];
+      s->vy[i] = s->vy[i] + (dt / 2.0) * s->ay[i];
+    }
+  }
+}
+
+void orc_corrector(const orc_params* P, orc_state* s, int its) {
+  /* time_integration.f:186-217 */
+  const double dt = P->dt;
+  const int nr = nrate_of(P, s);
+  for (int i = 0; i < nr; ++i) {
+    const int fl = i < s->np;
+    if (its != 1) {
+      s->u[i] = s->u_min[i] + dt * s->du[i];
+      if (s->u[i] < 0.0) s->u[i] = 0.0;
+      if (P->density_method == 0) s->rho[i] = s->rho_min[i] + dt * s->drho[i];
+      if (fl) {
+        s->vx[i] = s->vx_min[i] + dt * s->ax[i] + s->avx[i];
+        s->x[i] = s->x[i] + dt * s->vx[i];
+        s->vy[i] = s->vy_min[i] + dt * s->ay[i] + s->avy[i];
+        s->y[i] = s->y[i] + dt * s->vy[i];
+      }
+    } else {
+      s->u[i] = s->u[i] + (dt / 2.0) * s->du[i];
+      if (s->u[i] < 0.0) s->u[i] = 0.0;
+      if (P->density_method == 0) s->rho[i] = s->rho[i] + (dt / 2.0) * s->drho[i];
+      if (fl) {
+        s->vx[i] = s->vx[i] + (dt / 2.0) * s->ax[i];
+        s->x[i] = s->x[i] + dt * s->vx[i];
+        s->vy[i] = s->vy[i] + (dt / 2.0) * s->ay[i];
+        s->y[i] = s->y[i] + dt * s->vy[i];
+      }
+    }
+  }
+}
+
+int orc_run(const orc_params* P, orc_state* s, int its0, int nsteps, int gather) {
+  int cap = 0, niac = 0;
+  int *pi = NULL, *pj = NULL;
+  if (!gather) {
+    cap = s->n * 16 + 64;
+    pi = (int*)malloc(sizeof(int) * (size_t)cap);
+    pj = (int*)malloc(sizeof(int) * (size_t)cap);
+  }
+  int rc = 0;
+  for (int its = its0; its < its0 + nsteps && rc == 0; ++its) {
+    if (its != 1) orc_predictor(P, s);
+    if (gather) {
+      rc = orc_single_step_gather(P, s);
+      if (rc) rc = -3;
+    } else {
+      rc = orc_link_list(P, s->n, s->x, s->y, cap, &niac, pi, pj);
+      if (rc == 0) rc = orc_single_step(P, s, niac, pi, pj);
+    }
+    if (rc == 0) orc_corrector(P, s, its);
+  }
+  free(pi);
+  free(pj);
+  return rc;
+}
+
+void orc_conservation(const orc_params* P, const orc_state* s, double out[5]) {
+  double mass = 0, mx = 0, my = 0, ke = 0, ie = 0;
+  for (int i = 0; i < s->np; ++i) {
+    mass += P->mass;
+    mx += P->mass * s->vx[i];
+    my += P->mass * s->vy[i];
+    ke += 0.5 * P->mass * (s->vx[i] * s->vx[i] + s->vy[i] * s->vy[i]);
+    ie += P->mass * s->u[i];
+  }
+  out[0] = mass;
+  out[1] = mx;
+  out[2] = my;
+  out[3] = ke;
+  out[4] = ie;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+}
+
+/* ------------------------------------------------------------------ */
+/* FP32 cell + offset criterion                                         */
+/* ------------------------------------------------------------------ */
+
+void orc_split_f32(const orc_params* P, int n, const double* x, const double* y,
+                   int32_t* cx, int32_t* cy, float* offx, float* offy) {
+  for (int i = 0; i < n; ++i) {
+    const int ix = (int)((x[i] - P->ox) / P->cell);
+    const int iy = (int)((y[i] - P->oy) / P->cell);
+    const double bx = P->ox + (double)ix * P->cell;
+    const double by = P->oy + (double)iy * P->cell;
+    cx[i] = ix;
+    cy[i] = iy;
+    offx[i] = (float)(x[i] - bx);
+    offy[i] = (float)(y[i] - by);
+  }
+}
+
+static int cmp_int(const void* a, const void* b) {
+  const int ia = *(const int*)a, ib = *(const int*)b;
+  return (ia > ib) - (ia < ib);
+}
+
+/* generic lexicographic half list over the fixed grid using a predicate */
+typedef int (*pair_pred)(const void* ctx, int i, int j);
+
+static int half_pairs(const orc_params* P, int n, const int32_t* cx, const int32_t* cy,
+                      pair_pred pred, const void* ctx, int cap, int* niac, int* pair_i,
+                      int* pair_j) {
+  const int nx = P->nx, ny = P->ny, ncell = nx * ny;
+  int32_t* key = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int32_t* sorted = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int32_t* start = (int32_t*)malloc(sizeof(int32_t) * ((size_t)ncell + 1));
+  for (int i = 0; i < n; ++i) {
+    key[i] = (cx[i] >= 0 && cx[i] < nx && cy[i] >= 0 && cy[i] < ny) ? cx[i] + cy[i] * nx : -1;
+  }
+  orc_sort_cells(n, key, ncell, sorted, start);
+  int* buf = (int*)malloc(sizeof(int) * 4096);
+  int cnt = 0, rc = 0;
+  for (int i = 0; i < n && rc == 0; ++i) {
+    if (key[i] < 0) continue;
+    int nb = 0;
+    for (int ry = cy[i] - 1; ry <= cy[i] + 1; ++ry) {
+      if (ry < 0 || ry >= ny) continue;
+      const int x0 = cx[i] > 0 ? cx[i] - 1 : 0, x1 = cx[i] + 1 < nx ? cx[i] + 1 : nx - 1;
+      for (int t = start[ry * nx + x0]; t < start[ry * nx + x1 + 1]; ++t) {
+        const int j = sorted[t];
+        if (j > i && pred(ctx, i, j) && nb < 4096) buf[nb++] = j;
+      }
+    }
+    qsort(buf, (size_t)nb, sizeof(int), cmp_int);
+    for (int k = 0; k < nb; ++k) {
+      if (cnt >= cap) {
+        rc = -1;
+        break;
+      }
+      pair_i[cnt] = i;
+      pair_j[cnt] = buf[k];
+      cnt++;
+    }
+  }
+  *niac = cnt;
+  free(buf);
+  free(key);
+  free(sorted);
+  free(start);
+  return rc;
+}
+
+typedef struct {
+  const int32_t *cx, *cy;
+  const float *ox, *oy;
+  float cellf, cut2f;
+} f32_ctx;
+
+static int pred_f32(const void* vctx, int i, int j) {
+  const f32_ctx* c = (const f32_ctx*)vctx;
+  const float tx = c->ox[i] - c->ox[j];
+  const float ux = (float)(c->cx[i] - c->cx[j]) * c->cellf;
+  const float dx = tx + ux;
+  const float ty = c->oy[i] - c->oy[j];
+  const float uy = (float)(c->cy[i] - c->cy[j]) * c->cellf;
+  const float dy = ty + uy;
+  const float a = dx * dx;
+  const float b = dy * dy;
+  const float r2 = a + b;
+  return r2 < c->cut2f;
+}
+
+int orc_pairs_f32(const orc_params* P, int n, const int32_t* cx, const int32_t* cy,
+                  const float* offx, const float* offy, int cap, int* niac, int* pair_i,
+                  int* pair_j) {
+  f32_ctx c;
+  c.cx = cx;
+  c.cy = cy;
+  c.ox = offx;
+  c.oy = offy;
+  c.cellf = (float)P->cell;
+  c.cut2f = (float)((2.0 * P->h) * (2.0 * P->h));
+  return half_pairs(P, n, cx, cy, pred_f32, &c, cap, niac, pair_i, pair_j);
+}
+
+typedef struct {
+  const double *x, *y;
+  double cut2;
+} f64_ctx;
+
+static int pred_f64(const void* vctx, int i, int j) {
+  const f64_ctx* c = (const f64_ctx*)vctx;
+  const double dx = c->x[i] - c->x[j], dy = c->y[i] - c->y[j];
+  double r2 = dx * dx;
+  r2 = r2 + dy * dy;
+  return r2 < c->cut2;
+}
+
+int orc_pairs_f64(const orc_params* P, int n, const double* x, const double* y, int cap,
+                  int* niac, int* pair_i, int* pair_j) {
+  int32_t* cx = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int32_t* cy = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) {
+    const double tx = (x[i] - P->ox) / P->cell, ty = (y[i] - P->oy) / P->cell;
+    cx[i] = (tx >= 0.0) ? (int)tx : -1;
+    cy[i] = (ty >= 0.0) ? (int)ty : -1;
+  }
+  f64_ctx c;
+  c.x = x;
+  c.y = y;
+  c.cut2 = (2.0 * P->h) * (2.0 * P->h);
+  int rc = half_pairs(P, n, cx, cy, pred_f64, &c, cap, niac, pair_i, pair_j);
+  free(cx);
+  free(cy);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* FP32 C/CUDA path restatements                                        */
+/* ------------------------------------------------------------------ */
+
+void orc_clist_f32(int n, const float* x, const float* y, float ox, float oy, float size,
+                   int nx, int ny, float smooth, int maxn, int32_t* sorted_idx,
+                   int32_t* list) {
+  /* updateHashHost (armando2Dcpu.cu:2080-2101), stable (key, index) sort
+     (CUDPP radix / qsort, :2177,:2197), sortArraysHost (:2049-2064),
+     updateSetsHost (:2103-2127), updateListHost (:2129-2160) */
+  /* The dump in ArmandoList/out.txt was written by the host code of an i686
+     build (lib/libcudpp_i686), where float expressions are evaluated in x87
+     extended precision (FLT_EVAL_METHOD 2); the cell index is therefore
+     computed in long double before truncation. */
+  const int ncell = nx * ny;
+  int32_t* key = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const long double tx = ((long double)x[i] - (long double)ox) / (long double)size;
+    const long double ty = ((long double)y[i] - (long double)oy) / (long double)size;
+    key[i] = (int)tx + (int)ty * nx;
+  }
+  int32_t* start = (int32_t*)malloc(sizeof(int32_t) * ((size_t)ncell + 1));
+  orc_sort_cells(n, key, ncell, sorted_idx, start);
+  float* sx = (float*)malloc(sizeof(float) * (size_t)n);
+  float* sy = (float*)malloc(sizeof(float) * (size_t)n);
+  for (int t = 0; t < n; ++t) {
+    sx[t] = x[sorted_idx[t]];
+    sy[t] = y[sorted_idx[t]];
+  }
+  for (int t = 0; t < n; ++t) {
+    for (int k = 0; k < maxn; ++k) list[(size_t)t * maxn + k] = t;
+    const int ic = key[sorted_idx[t]];
+    int np = 0;
+    for (int j = -1; j <= 1; ++j)
+      for (int i = -1; i <= 1; ++i) {
+        const int jc = ic + i + j * nx;
+        if (jc < 0 || jc >= ncell) continue;
+        for (int jp = start[jc]; jp < start[jc + 1]; ++jp) {
+          const float dx = sx[t] - sx[jp];
+          const float dy = sy[t] - sy[jp];
+          const float dr = sqrtf(dx * dx + dy * dy);
+          if (((double)dr < 2.0 * (double)smooth) && (np < maxn)) {
+            list[(size_t)t * maxn + np] = jp;
+            np++;
+          }
+        }
+      }
+  }
+  free(sx);
+  free(sy);
+  free(key);
+  free(start);
+}
+
+static float deriv_wendland_f32(float r, float h) {
+  /* kernelDerivWendland, armando2Dcpu.cu:147-172 (float with double
+     sub-expressions as written) */
+  const float q = r / h;
+  const float alpha = (float)(7.0 / (4.0 * M_PI * (double)h * (double)h));
+  float dwdr = 0.0f;
+  if (q < 2) {
+    dwdr = (float)(5.0 / 8.0 * (double)q * pow(((double)q - 2.0), 3));
+    dwdr *= alpha / h;
+  }
+  return dwdr;
+}
+
+void orc_kat_interaction_f32(int n, int maxn, const int32_t* list, const float* x,
+                             const float* y, const float* vx, const float* vy,
+                             const float* rho, const float* p, float smooth, float mass,
+                             float sound, float* drho, float* ax, float* ay) {
+  /* kerInteraction, armando2Dlist.cu:611-686 */
+  for (int ip = 0; ip < n; ++ip) {
+    float iDensityDot = 0.0f, iVelDotX = 0.0f, iVelDotY = 0.0f;
+    for (int il = 0; il < maxn; ++il) {
+      const int jp = list[(size_t)ip * maxn + il];
+      const float dx = x[ip] - x[jp];
+      const float dy = y[ip] - y[jp];
+      const float dz = 0.0f;
+      float dr = sqrtf(dx * dx + dy * dy + dz * dz);
+      if ((double)dr < 0.1 * (double)smooth) dr = (float)(100.0 * (double)smooth);
+      const float dwdr = deriv_wendland_f32(dr, smooth);
+      float dvr = 0.0f;
+      dvr += (x[ip] - x[jp]) * (vx[ip] - vx[jp]);
+      dvr += (y[ip] - y[jp]) * (vy[ip] - vy[jp]);
+      iDensityDot += mass * dvr * dwdr / dr;
+      float f = -(p[ip] + p[jp]) / (rho[ip] * rho[jp]);
+      iVelDotX += mass * f * dwdr * (x[ip] - x[jp]) / dr;
+      iVelDotY += mass * f * dwdr * (y[ip] - y[jp]) / dr;
+      f = rho[ip] - rho[jp];
+      f = (float)((double)f * (2.0 * (double)sound / (double)(rho[ip] + rho[jp])));
+      iDensityDot += mass * f * dwdr;
+      f = (dvr < 0) ? dvr : 0.0f;
+      f = (float)((double)f * ((double)smooth /
+                               ((double)(dr * dr) + 0.01 * (double)smooth * (double)smooth)));
+      f = (float)((double)f * (2. * (double)sound / (double)(rho[ip] + rho[jp])));
+      f = (float)((double)f * 0.03);
+      iVelDotX += mass * f * dwdr * (x[ip] - x[jp]) / dr;
+      iVelDotY += mass * f * dwdr * (y[ip] - y[jp]) / dr;
+    }
+    drho[ip] = iDensityDot;
+    ax[ip] = iVelDotX;
+    ay[ip] = iVelDotY;
+  }
+}

@@ -1,0 +1,208 @@
+/*
+ * armando_oracle.h -- CPU restatement of Armando's 2D WCSPH time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * library (armando_amd/csrc).  Only tests, the smoke check and the CPU
+ * baseline leg of the benchmark load it; the product path never does.
+ *
+ * Two families of routines:
+ *
+ *  1. "Faithful" FP64 routines that restate the CERN Fortran hot path
+ *     routine by routine, with the same half pair list (i < j), the same
+ *     pair order and the same scatter-to-both-ends accumulation:
+ *       link_list      CERNsource/link_list.f:1-223
+ *       kernel         CERNsource/kernel.f:1-107 (cubic) and the Wendland C2
+ *                      of ArmandoList/armando2Dcpu.cu:116-172
+ *       sum_density    CERNsource/density.f:1-62
+ *       con_density    CERNsource/density.f:166-223
+ *       int_force      CERNsource/internal_force.f:1-437 (pressure part)
+ *       art_visc       CERNsource/art_visc.f:1-103
+ *       art_heat       CERNsource/art_heat.f:1-99
+ *       av_vel         CERNsource/av_vel.f:1-58
+ *       eos (Tait)     FORTRANsource/eos.f90:91-102,255-283
+ *       time_integration / single_step
+ *                      CERNsource/time_integration.f:131-235,
+ *                      CERNsource/single_step.f:92-234
+ *     Walls follow the dynamic boundary of ArmandoList/armando2Dcpu.cu:
+ *     832-907 (density and energy integrated, velocity held) and gravity
+ *     follows updateForcesHost (:937-958).
+ *
+ *  2. A "gather" FP64 twin of single_step that visits, for every particle,
+ *     its full neighbour set in (cell, id) order (OpenMP parallel).  Same
+ *     mathematics, different summation order; used for large runs.
+ *
+ *  3. FP32 restatements of the C/CUDA benchmark path used only to pin
+ *     conventions against the reference's fixtures: the cell hash, stable
+ *     sort, start/stop sets and MAXN list (armando2Dcpu.cu:2080-2160) and the
+ *     continuity + pressure + shock-correction interaction of the known
+ *     answer test (ArmandoList/armando2Dlist.cu:611-686, 2451-2548).
+ *
+ *  4. The neighbour criterion of the FP32 "cell + offset" form of the HIP
+ *     library, restated in float arithmetic so that neighbour sets can be
+ *     compared bit for bit.
+ *
+ * All indices are 0-based.  Particles 0..np-1 are fluid, np..n-1 are wall
+ * (virtual) particles, as in time_integration.f:104 (np real, then nv).
+ */
+#ifndef ARMANDO_ORACLE_H
+#define ARMANDO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ORC_K_WENDLAND2D = 0, /* alpha = 7/(4 pi h^2), support 2h */
+  ORC_K_CUBIC2D = 1,    /* kernel.f skf=0, dim=2 factor, REAL(4) constants */
+  ORC_K_CUBIC3D = 2     /* kernel.f skf=0, dim=3 factor (for the as-shipped dim=3 build) */
+};
+
+enum {
+  ORC_ART_VISC = 1,  /* art_visc.f */
+  ORC_ART_HEAT = 2,  /* art_heat.f */
+  ORC_AV_VEL = 4,    /* av_vel.f (XSPH) */
+  ORC_DBC = 8,       /* walls integrate density and energy (armando2Dcpu.cu:832-907) */
+  ORC_GRAVITY = 16   /* body force on fluid (armando2Dcpu.cu:937-958) */
+};
+
+typedef struct {
+  int kernel;            /* ORC_K_* */
+  int flags;             /* ORC_* bit set */
+  int pa_sph;            /* 0: -(pi+pj)/(ri rj), 1: -(pi/ri^2 + pj/rj^2) */
+  int density_method;    /* 0: continuity, 1: summation */
+  double h;              /* uniform smoothing length */
+  double mass;           /* uniform particle mass */
+  double rho0, c0, pmin; /* Tait: p = rho0 c0^2/7 ((rho/rho0)^7 - 1), c = c0 */
+  double av_alpha, av_beta, av_eta; /* Monaghan artificial viscosity */
+  double ah_g1, ah_g2, ah_eta;      /* Noh artificial heat */
+  double xsph_eps;                  /* XSPH epsilon */
+  double gx, gy;                    /* body force on fluid particles */
+  double dt;
+  /* fixed hash grid (C path convention): key = ix + iy*nx,
+     ix = (int)((x - ox)/cell) */
+  double ox, oy, cell;
+  int nx, ny;
+} orc_params;
+
+/* Particle state, SoA, FP64. */
+typedef struct {
+  int n, np;
+  double *x, *y, *vx, *vy, *rho, *u, *p, *c;
+  /* rates of the last single_step */
+  double *drho, *du, *ax, *ay, *avx, *avy, *divv;
+  /* leapfrog save arrays */
+  double *rho_min, *u_min, *vx_min, *vy_min;
+} orc_state;
+
+/* ---- kernel ---------------------------------------------------------- */
+/* w and the factor F with grad_i W = F * (x_i - x_j) (kernel.f returns
+   dwdx = F*dx).  Returns w. */
+double orc_kernel(const orc_params* P, double r, double* F);
+
+/* ---- neighbour search ------------------------------------------------ */
+/* link_list.f restated for dim=2 (data-dependent grid, scale_k=2, half
+   list j>i, same visiting order).  Returns 0, or -1 if cap is exceeded. */
+int orc_link_list(const orc_params* P, int nt, const double* x, const double* y,
+                  int cap, int* niac, int* pair_i, int* pair_j);
+/* direct_find.f analogue: O(n^2) brute force, pairs (i<j) in lexicographic
+   order.  Returns 0 or -1 on cap overflow. */
+int orc_direct_find(const orc_params* P, int nt, const double* x, const double* y,
+                    int cap, int* niac, int* pair_i, int* pair_j);
+
+/* Fixed-grid cell keys (FP64 form).  Returns number of particles outside
+   the grid (their key is set to -1). */
+int orc_keys(const orc_params* P, int n, const double* x, const double* y, int32_t* key);
+/* Stable sort of particles by key, ties by id (input listed in id order):
+   sorted_id[t] = id at sorted position t; cell_start has ncell+1 entries
+   with cell_start[c] = first sorted position whose key >= c. */
+void orc_sort_cells(int n, const int32_t* key, int ncell, int32_t* sorted_id,
+                    int32_t* cell_start);
+
+/* ---- faithful pair routines (half list, scatter) --------------------- */
+void orc_sum_density(const orc_params* P, int nrate, int niac, const int* pi, const int* pj,
+                     const double* x, const double* y, double* rho);
+void orc_con_density(const orc_params* P, int nrate, int niac, const int* pi, const int* pj,
+                     const double* x, const double* y, const double* vx, const double* vy,
+                     double* drho);
+/* int_force pressure part; zeroes ax, ay, du for [0, nt) first, finishes
+   with du = 0.5*du (tdsdt = 0 without physical viscosity). */
+void orc_int_force(const orc_params* P, int nt, int niac, const int* pi, const int* pj,
+                   const double* x, const double* y, const double* vx, const double* vy,
+                   const double* rho, const double* p, double* ax, double* ay, double* du);
+/* art_visc accumulates into ax, ay, du. */
+void orc_art_visc(const orc_params* P, int niac, const int* pi, const int* pj,
+                  const double* x, const double* y, const double* vx, const double* vy,
+                  const double* rho, const double* c, double* ax, double* ay, double* du);
+/* art_heat accumulates into du; divv (length nt, may be NULL) receives pass 1. */
+void orc_art_heat(const orc_params* P, int nt, int niac, const int* pi, const int* pj,
+                  const double* x, const double* y, const double* vx, const double* vy,
+                  const double* rho, const double* u, const double* c, double* du,
+                  double* divv);
+/* av_vel accumulates into avx, avy. */
+void orc_av_vel(const orc_params* P, int niac, const int* pi, const int* pj,
+                const double* x, const double* y, const double* vx, const double* vy,
+                const double* rho, double* avx, double* avy);
+/* Tait EOS on [0, n): p, c, and the pmin clamp (p = 0, rho = rho0). */
+void orc_eos_tait(const orc_params* P, int n, double* rho, double* p, double* c);
+
+/* ---- state and step ---------------------------------------------------- */
+orc_state* orc_state_new(int n, int np);
+void orc_state_free(orc_state* s);
+/* single_step.f restated with the faithful half-list routines. */
+int orc_single_step(const orc_params* P, orc_state* s, int niac, const int* pi, const int* pj);
+/* Same rates, gather form over the fixed grid (OpenMP).  Returns the
+   number of particles outside the grid (0 on success). */
+int orc_single_step_gather(const orc_params* P, orc_state* s);
+/* time_integration.f predictor (its != 1) and corrector. */
+void orc_predictor(const orc_params* P, orc_state* s);
+void orc_corrector(const orc_params* P, orc_state* s, int its);
+/* nsteps of time_integration starting at step number its0 (1-based).
+   gather=0: faithful serial half-list path; gather=1: gather path.
+   Returns 0, or a negative error. */
+int orc_run(const orc_params* P, orc_state* s, int its0, int nsteps, int gather);
+/* FP64 conservation sums: mass, momentum x, momentum y, kinetic energy,
+   internal energy (fluid particles). */
+void orc_conservation(const orc_params* P, const orc_state* s, double out[5]);
+/* Number of threads OpenMP would use (1 without OpenMP). */
+int orc_num_threads(void);
+void orc_set_threads(int nthreads);
+
+/* ---- FP32 cell + offset form ----------------------------------------- */
+/* Split FP64 positions into integer cell coordinates and FP32 in-cell
+   offsets: ix = (int)((x-ox)/cell), off = (float)(x - (ox + ix*cell)). */
+void orc_split_f32(const orc_params* P, int n, const double* x, const double* y,
+                   int32_t* cx, int32_t* cy, float* offx, float* offy);
+/* Half pair list (i<j, ids) under the FP32 criterion
+   dx = (offx_i - offx_j) + (float)(cx_i - cx_j) * cellf, same for y,
+   r2 = dx*dx + dy*dy (no fused multiply-add), pair iff r2 < r2cut_f with
+   cellf = (float)cell and r2cut_f = (float)((2h)*(2h)).  Pairs are emitted
+   in lexicographic (i, j) order.  Returns 0 or -1 on cap overflow. */
+int orc_pairs_f32(const orc_params* P, int n, const int32_t* cx, const int32_t* cy,
+                  const float* offx, const float* offy, int cap, int* niac,
+                  int* pair_i, int* pair_j);
+/* Half pair list (i<j) under the FP64 criterion r2 = dx*dx + dy*dy <
+   (2h)*(2h), lexicographic order, computed with the fixed grid. */
+int orc_pairs_f64(const orc_params* P, int n, const double* x, const double* y,
+                  int cap, int* niac, int* pair_i, int* pair_j);
+
+/* ---- FP32 C/CUDA path restatements (fixtures) ------------------------ */
+/* updateHashHost + stable sort + updateSetsHost + updateListHost
+   (armando2Dcpu.cu:2080-2160) in float arithmetic.  Inputs are in the
+   original order; outputs are in sorted order: sorted_idx[t] = original
+   index, list[t*maxn + k] (padded with t). */
+void orc_clist_f32(int n, const float* x, const float* y, float ox, float oy, float size,
+                   int nx, int ny, float smooth, int maxn, int32_t* sorted_idx,
+                   int32_t* list);
+/* kerInteraction (armando2Dlist.cu:611-686) in float over a list built by
+   orc_clist_f32; arrays in sorted order.  Writes (not accumulates). */
+void orc_kat_interaction_f32(int n, int maxn, const int32_t* list, const float* x,
+                             const float* y, const float* vx, const float* vy,
+                             const float* rho, const float* p, float smooth, float mass,
+                             float sound, float* drho, float* ax, float* ay);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,241 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
+baseline leg of bench.py, never by the armando_amd package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liborc.so")
+REF_PATH = os.path.join(HERE, "_ref", "libarmref.so")
+
+K_WENDLAND2D, K_CUBIC2D, K_CUBIC3D = 0, 1, 2
+ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY = 1, 2, 4, 8, 16
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int), ("flags", C.c_int), ("pa_sph", C.c_int),
+        ("density_method", C.c_int),
+        ("h", C.c_double), ("mass", C.c_double),
+        ("rho0", C.c_double), ("c0", C.c_double), ("pmin", C.c_double),
+        ("av_alpha", C.c_double), ("av_beta", C.c_double), ("av_eta", C.c_double),
+        ("ah_g1", C.c_double), ("ah_g2", C.c_double), ("ah_eta", C.c_double),
+        ("xsph_eps", C.c_double), ("gx", C.c_double), ("gy", C.c_double),
+        ("dt", C.c_double),
+        ("ox", C.c_double), ("oy", C.c_double), ("cell", C.c_double),
+        ("nx", C.c_int), ("ny", C.c_int),
+    ]
+
+
+class State(C.Structure):
+    _fields_ = [("n", C.c_int), ("np", C.c_int)] + [
+        (name, C.POINTER(C.c_double)) for name in (
+            "x", "y", "vx", "vy", "rho", "u", "p", "c", "drho", "du", "ax", "ay",
+            "avx", "avy", "divv", "rho_min", "u_min", "vx_min", "vy_min")
+    ]
+
+
+STATE_FIELDS = ("x", "y", "vx", "vy", "rho", "u", "p", "c", "drho", "du", "ax", "ay",
+                "avx", "avy", "divv", "rho_min", "u_min", "vx_min", "vy_min")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_PP = C.POINTER(Params)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "orc_kernel": (C.c_double, [_PP, C.c_double, C.POINTER(C.c_double)]),
+        "orc_link_list": (C.c_int, [_PP, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_int), _ip, _ip]),
+        "orc_direct_find": (C.c_int, [_PP, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_int), _ip, _ip]),
+        "orc_keys": (C.c_int, [_PP, C.c_int, _dp, _dp, _ip]),
+        "orc_sort_cells": (None, [C.c_int, _ip, C.c_int, _ip, _ip]),
+        "orc_sum_density": (None, [_PP, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp]),
+        "orc_con_density": (None, [_PP, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp]),
+        "orc_int_force": (None, [_PP, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
+                                 _dp, _dp, _dp]),
+        "orc_art_visc": (None, [_PP, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                _dp]),
+        "orc_art_heat": (None, [_PP, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
+                                _dp, _dp, _dp]),
+        "orc_av_vel": (None, [_PP, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_eos_tait": (None, [_PP, C.c_int, _dp, _dp, _dp]),
+        "orc_state_new": (C.POINTER(State), [C.c_int, C.c_int]),
+        "orc_state_free": (None, [C.POINTER(State)]),
+        "orc_single_step": (C.c_int, [_PP, C.POINTER(State), C.c_int, _ip, _ip]),
+        "orc_single_step_gather": (C.c_int, [_PP, C.POINTER(State)]),
+        "orc_predictor": (None, [_PP, C.POINTER(State)]),
+        "orc_corrector": (None, [_PP, C.POINTER(State), C.c_int]),
+        "orc_run": (C.c_int, [_PP, C.POINTER(State), C.c_int, C.c_int, C.c_int]),
+        "orc_conservation": (None, [_PP, C.POINTER(State), _dp]),
+        "orc_num_threads": (C.c_int, []),
+        "orc_set_threads": (None, [C.c_int]),
+        "orc_split_f32": (None, [_PP, C.c_int, _dp, _dp, _ip, _ip, _fp, _fp]),
+        "orc_pairs_f32": (C.c_int, [_PP, C.c_int, _ip, _ip, _fp, _fp, C.c_int,
+                                    C.POINTER(C.c_int), _ip, _ip]),
+        "orc_pairs_f64": (C.c_int, [_PP, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_int), _ip,
+                                    _ip]),
+        "orc_clist_f32": (None, [C.c_int, _fp, _fp, C.c_float, C.c_float, C.c_float, C.c_int,
+                                 C.c_int, C.c_float, C.c_int, _ip, _ip]),
+        "orc_kat_interaction_f32": (None, [C.c_int, C.c_int, _ip, _fp, _fp, _fp, _fp, _fp, _fp,
+                                           C.c_float, C.c_float, C.c_float, _fp, _fp, _fp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def params(**kw):
+    """Params with every field defaulted to zero, then overridden."""
+    P = Params()
+    for k, v in kw.items():
+        setattr(P, k, v)
+    return P
+
+
+def params_from(case):
+    """Oracle Params from a case dictionary (see armando_amd.cases)."""
+    keys = [f for f, _ in Params._fields_]
+    return params(**{k: case[k] for k in keys if k in case})
+
+
+# ------------------------------------------------------------------ pairs
+def link_list(P, x, y, cap=None):
+    n = len(x)
+    cap = cap or max(64, n * 16)
+    pi = np.zeros(cap, np.int32)
+    pj = np.zeros(cap, np.int32)
+    niac = C.c_int(0)
+    rc = lib().orc_link_list(C.byref(P), n, np.ascontiguousarray(x, np.float64),
+                             np.ascontiguousarray(y, np.float64), cap, C.byref(niac), pi, pj)
+    if rc:
+        raise RuntimeError(f"orc_link_list rc={rc}")
+    return pi[:niac.value].copy(), pj[:niac.value].copy()
+
+
+def direct_find(P, x, y, cap=None):
+    n = len(x)
+    cap = cap or max(64, n * 16)
+    pi = np.zeros(cap, np.int32)
+    pj = np.zeros(cap, np.int32)
+    niac = C.c_int(0)
+    rc = lib().orc_direct_find(C.byref(P), n, np.ascontiguousarray(x, np.float64),
+                               np.ascontiguousarray(y, np.float64), cap, C.byref(niac), pi, pj)
+    if rc:
+        raise RuntimeError(f"orc_direct_find rc={rc}")
+    return pi[:niac.value].copy(), pj[:niac.value].copy()
+
+
+def pairs_f64(P, x, y, cap=None):
+    n = len(x)
+    cap = cap or max(64, n * 16)
+    pi = np.zeros(cap, np.int32)
+    pj = np.zeros(cap, np.int32)
+    niac = C.c_int(0)
+    rc = lib().orc_pairs_f64(C.byref(P), n, np.ascontiguousarray(x, np.float64),
+                             np.ascontiguousarray(y, np.float64), cap, C.byref(niac), pi, pj)
+    if rc:
+        raise RuntimeError(f"orc_pairs_f64 rc={rc}")
+    return pi[:niac.value].copy(), pj[:niac.value].copy()
+
+
+def split_f32(P, x, y):
+    n = len(x)
+    cx = np.zeros(n, np.int32)
+    cy = np.zeros(n, np.int32)
+    ox = np.zeros(n, np.float32)
+    oy = np.zeros(n, np.float32)
+    lib().orc_split_f32(C.byref(P), n, np.ascontiguousarray(x, np.float64),
+                        np.ascontiguousarray(y, np.float64), cx, cy, ox, oy)
+    return cx, cy, ox, oy
+
+
+def pairs_f32(P, cx, cy, offx, offy, cap=None):
+    n = len(cx)
+    cap = cap or max(64, n * 16)
+    pi = np.zeros(cap, np.int32)
+    pj = np.zeros(cap, np.int32)
+    niac = C.c_int(0)
+    rc = lib().orc_pairs_f32(C.byref(P), n, np.ascontiguousarray(cx, np.int32),
+                             np.ascontiguousarray(cy, np.int32),
+                             np.ascontiguousarray(offx, np.float32),
+                             np.ascontiguousarray(offy, np.float32), cap, C.byref(niac), pi, pj)
+    if rc:
+        raise RuntimeError(f"orc_pairs_f32 rc={rc}")
+    return pi[:niac.value].copy(), pj[:niac.value].copy()
+
+
+def keys(P, x, y):
+    n = len(x)
+    k = np.zeros(n, np.int32)
+    bad = lib().orc_keys(C.byref(P), n, np.ascontiguousarray(x, np.float64),
+                         np.ascontiguousarray(y, np.float64), k)
+    return k, bad
+
+
+def sort_cells(key, ncell):
+    n = len(key)
+    sid = np.zeros(n, np.int32)
+    start = np.zeros(ncell + 1, np.int32)
+    lib().orc_sort_cells(n, np.ascontiguousarray(key, np.int32), ncell, sid, start)
+    return sid, start
+
+
+# ------------------------------------------------------------------ state
+class OracleState:
+    """Owns an orc_state; fields are exposed as numpy views."""
+
+    def __init__(self, n, np_fluid):
+        self.ptr = lib().orc_state_new(n, np_fluid)
+        self.n = n
+        self.np = np_fluid
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().orc_state_free(self.ptr)
+            self.ptr = None
+
+    def field(self, name):
+        p = getattr(self.ptr.contents, name)
+        return np.ctypeslib.as_array(p, shape=(self.n,))
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            self.field(k)[:] = v
+
+    def get(self, *names):
+        return {k: self.field(k).copy() for k in names}
+
+
+def run(P, st, its0, nsteps, gather=True):
+    rc = lib().orc_run(C.byref(P), st.ptr, its0, nsteps, 1 if gather else 0)
+    if rc:
+        raise RuntimeError(f"orc_run rc={rc}")
+
+
+def single_step_gather(P, st):
+    rc = lib().orc_single_step_gather(C.byref(P), st.ptr)
+    if rc:
+        raise RuntimeError(f"orc_single_step_gather rc={rc}")
+
+
+def conservation(P, st):
+    out = np.zeros(5, np.float64)
+    lib().orc_conservation(C.byref(P), st.ptr, out)
+    return out

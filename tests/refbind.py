@@ -1,0 +1,145 @@
+"""Calls into oracle/_ref/libarmref.so: the reference's own CERNsource
+routines (kernel.f, link_list.f, density.f, internal_force.f, art_visc.f,
+art_heat.f, av_vel.f, direct_find.f) compiled unmodified by oracle/Makefile.
+
+The as-shipped options.inc has dim = 3, so positions and velocities are
+passed as x(3, n) with z = 0.  Their large local arrays need a big stack,
+so every call runs on a helper thread with a 512 MiB stack.
+"""
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+from oracle import orc
+
+_lib = None
+
+
+def available():
+    return os.path.exists(orc.REF_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(orc.REF_PATH)
+    return _lib
+
+
+def run_big_stack(fn, *a):
+    out = {}
+
+    def body():
+        try:
+            out["r"] = fn(*a)
+        except BaseException as e:  # re-raised on the caller thread
+            out["e"] = e
+
+    old = threading.stack_size(1 << 29)
+    try:
+        t = threading.Thread(target=body)
+        t.start()
+        t.join()
+    finally:
+        threading.stack_size(old)
+    if "e" in out:
+        raise out["e"]
+    return out["r"]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _i(v):
+    return C.byref(C.c_int(v))
+
+
+def vec3(a, b):
+    X = np.zeros((len(a), 3))
+    X[:, 0] = a
+    X[:, 1] = b
+    return X
+
+
+def link_list(x, y, h):
+    def f():
+        n = len(x)
+        X = vec3(x, y)
+        H = np.full(n, h)
+        cap = n * 64 + 64
+        pi = np.zeros(cap, np.int32)
+        pj = np.zeros(cap, np.int32)
+        niac = C.c_int(0)
+        lib().link_list_(_i(n), _p(X), _p(H), C.byref(niac), _p(pi), _p(pj), _i(0))
+        return pi[:niac.value] - 1, pj[:niac.value] - 1
+    return run_big_stack(f)
+
+
+def direct_find(x, y, h):
+    def f():
+        n = len(x)
+        X = vec3(x, y)
+        H = np.full(n, h)
+        cap = n * 64 + 64
+        pi = np.zeros(cap, np.int32)
+        pj = np.zeros(cap, np.int32)
+        niac = C.c_int(0)
+        lib().direct_find_(_i(n), _p(X), _p(H), C.byref(niac), _p(pi), _p(pj), _i(0))
+        return pi[:niac.value] - 1, pj[:niac.value] - 1
+    return run_big_stack(f)
+
+
+def pair_routines(x, y, vx, vy, rho, p, u, c, h, mass, pi, pj):
+    """Runs sum_density, con_density, int_force (pa_sph 0 and 1),
+    art_visc, art_heat and av_vel with every particle counted as real."""
+    def f():
+        n = len(x)
+        X = vec3(x, y)
+        V = vec3(vx, vy)
+        H = np.full(n, h)
+        M = np.full(n, mass)
+        P1 = np.ascontiguousarray(pi + 1, np.int32)
+        P2 = np.ascontiguousarray(pj + 1, np.int32)
+        niac = len(pi)
+        out = {}
+        rs = np.zeros(n)
+        lib().sum_density_(_i(n), _p(X), _p(H), _p(M), _p(rs), _i(niac), _p(P1), _p(P2), _i(0))
+        out["sum_rho"] = rs
+        dr = np.zeros(n)
+        lib().con_density_(_i(n), _p(X), _p(V), _p(H), _p(M), _p(dr), _i(niac), _p(P1),
+                           _p(P2), _i(0))
+        out["drho"] = dr
+        R = np.ascontiguousarray(rho, np.float64)
+        Pp = np.ascontiguousarray(p, np.float64)
+        U = np.ascontiguousarray(u, np.float64)
+        Cc = np.ascontiguousarray(c, np.float64)
+        mat = np.ones(n, np.int32)
+        eta = np.zeros(n)
+        dt = C.c_double(1e-4)
+        visc = C.c_int(0)
+        for pa in (0, 1):
+            dv = np.zeros((n, 3))
+            du = np.zeros(n)
+            tds = np.zeros(n)
+            lib().int_force_(_i(n), _p(X), _p(V), _p(mat), _p(H), _p(M), _p(R), _p(Pp), _p(eta),
+                             _p(dv), _p(du), _p(tds), _i(niac), _p(P1), _p(P2), C.byref(dt),
+                             _i(0), _i(pa), C.byref(visc))
+            out[f"if{pa}_ax"], out[f"if{pa}_ay"], out[f"if{pa}_du"] = dv[:, 0].copy(), dv[:, 1].copy(), du
+        dv = np.zeros((n, 3))
+        du = np.zeros(n)
+        lib().art_visc_(_i(n), _p(X), _p(V), _p(H), _p(M), _p(R), _p(Cc), _p(dv), _p(du),
+                        _i(niac), _p(P1), _p(P2), _i(0))
+        out["av_ax"], out["av_ay"], out["av_du"] = dv[:, 0].copy(), dv[:, 1].copy(), du
+        du = np.zeros(n)
+        lib().art_heat_(_i(n), _p(X), _p(V), _p(H), _p(M), _p(R), _p(U), _p(Cc), _p(du),
+                        _i(niac), _p(P1), _p(P2), _i(0))
+        out["ah_du"] = du
+        av = np.zeros((n, 3))
+        lib().av_vel_(_i(n), _p(X), _p(V), _p(H), _p(M), _p(R), _p(av), _i(niac), _p(P1),
+                      _p(P2), _i(0))
+        out["xs_x"], out["xs_y"] = av[:, 0].copy(), av[:, 1].copy()
+        return out
+    return run_big_stack(f)
