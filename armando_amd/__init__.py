@@ -24,13 +24,14 @@ ERRORS = {-1: "ARM_ERR_ARG", -2: "ARM_ERR_HIP", -3: "ARM_ERR_CAPACITY", -4: "ARM
           -5: "ARM_ERR_UNSUPPORTED", -6: "ARM_ERR_STATE"}
 ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY = 1, 2, 4, 8, 16
 FP64, FP32 = 0, 1
-PASSES = ("keys", "sort", "reorder", "density", "force")
+PASSES = ("keys", "sort", "reorder", "density", "eos", "force", "integrate")
 
 EXPORTS = ("arm_create", "arm_destroy", "arm_last_error", "arm_domain_violations", "arm_upload",
            "arm_download", "arm_download_rates", "arm_link_list", "arm_get_pairs",
            "arm_get_cells", "arm_density", "arm_eos", "arm_internal_force", "arm_av_vel",
            "arm_single_step", "arm_predictor", "arm_corrector", "arm_time_integration",
-           "arm_conservation", "arm_pass_times", "arm_reset_pass_times", "arm_set_timing")
+           "arm_conservation", "arm_pass_times", "arm_reset_pass_times", "arm_set_timing",
+           "arm_set_gather_path")
 
 
 class ArmParams(C.Structure):
@@ -90,6 +91,7 @@ def load_library(path=LIB_PATH):
         "arm_pass_times": (C.c_int, [vp, C.c_int, dpn, C.POINTER(C.c_int)]),
         "arm_reset_pass_times": (C.c_int, [vp]),
         "arm_set_timing": (C.c_int, [vp, C.c_int]),
+        "arm_set_gather_path": (C.c_int, [vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -219,6 +221,9 @@ class Simulation:
         out = np.zeros(5)
         self._check(self.L.arm_conservation(self.ctx, out))
         return out
+
+    def set_gather_path(self, tiled=True):
+        self._check(self.L.arm_set_gather_path(self.ctx, 1 if tiled else 0))
 
     def set_timing(self, on=True):
         self._check(self.L.arm_set_timing(self.ctx, 1 if on else 0))

@@ -130,7 +130,7 @@ def dam_break_small(scale=0.05, precision=FP64, seed=12345, jitter=0.01, dx=1e-3
                      tank_h=1.6 * scale, dx=dx, precision=precision, seed=seed, jitter=jitter)
 
 
-def box_lattice(nx=20, ny=15, dx=0.01, jitter=0.05, seed=7, vel=0.0, precision=FP64):
+def box_lattice(nx=20, ny=15, dx=0.01, jitter=0.05, seed=7, vel=0.0, precision=FP64, h_fac=1.3):
     """Periodic-free lattice block of real particles only (no walls), with
     jittered positions and seeded velocities, for routine-level parity."""
     I, K = np.meshgrid(np.arange(nx), np.arange(ny))
@@ -144,7 +144,7 @@ def box_lattice(nx=20, ny=15, dx=0.01, jitter=0.05, seed=7, vel=0.0, precision=F
     vy = vel * (2 * r[:, 3] - 1)
     rho = 1000.0 * (1.0 + 0.01 * (2 * r[:, 4] - 1))
     u = 1.0 + 0.5 * r[:, 5]
-    h = 1.3 * dx
+    h = h_fac * dx
     cell = 2.0 * h
     ox, oy, nxg, nyg = grid_for(x, y, cell)
     return dict(

@@ -18,6 +18,7 @@
 #include "../../include/armando.h"
 #include "radix_sort.h"
 #include "sph_kernels.h"
+#include "sph_tiled.h"
 
 using namespace arm;
 
@@ -39,6 +40,9 @@ struct arm_ctx {
   int cap = 0;
   int n = 0, np = 0;
   bool loaded = false, sorted = false, have_rates = false;
+  bool tiled = true;  // LDS-tiled gathers (false: direct global-memory gathers)
+  bool masks_valid = false;  // nbm holds the neighbour masks of the current order
+  uint4* nbm = nullptr;
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   // state double buffer (the active set is A)
@@ -48,6 +52,7 @@ struct arm_ctx {
   uint32_t *key_by_id = nullptr, *slot_by_id = nullptr;
   uint32_t *sk[2] = {nullptr, nullptr}, *sv[2] = {nullptr, nullptr};
   uint32_t* hist = nullptr;
+  uint32_t* tot = nullptr;
   int ntiles = 0, sort_bits = 0, sort_passes = 0;
   uint32_t* skey = nullptr;  // sorted keys of the current order
   int* cstart = nullptr;
@@ -56,9 +61,16 @@ struct arm_ctx {
   int* d_cnt = nullptr;
   int domain_bad = 0;
   std::string err;
-  // timing
+  // timing: (start, stop, pass) event triples recorded on the stream and
+  // resolved when the times are read, so timing adds no host synchronisation
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> ev_pool;
+  struct Rec {
+    hipEvent_t a, b;
+    int pass;
+  };
+  std::vector<Rec> pending;
   double pass_ms[ARM_PASS_COUNT] = {0};
   int pass_cnt[ARM_PASS_COUNT] = {0};
 };
@@ -125,20 +137,49 @@ template <typename R> int alloc_state(arm_ctx* c, St<R>& S, int cap) {
 bool is32(const arm_ctx* c) { return c->prm.precision == ARM_FP32; }
 
 // ---- timing ---------------------------------------------------------------
+hipEvent_t take_event(arm_ctx* c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Folds finished event pairs into the per-pass totals.
+void resolve_times(arm_ctx* c) {
+  if (c->pending.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (const auto& r : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      c->pass_ms[r.pass] += ms;
+      c->pass_cnt[r.pass] += 1;
+    }
+    c->ev_pool.push_back(r.a);
+    c->ev_pool.push_back(r.b);
+  }
+  c->pending.clear();
+}
+
 struct PassTimer {
   arm_ctx* c;
   int pass;
+  hipEvent_t a = nullptr;
   PassTimer(arm_ctx* c_, int p) : c(c_), pass(p) {
-    if (c->timing) (void)hipEventRecord(c->ev0, c->stream);
+    if (!c->timing) return;
+    if (c->pending.size() > 4096) resolve_times(c);
+    a = take_event(c);
+    if (a) (void)hipEventRecord(a, c->stream);
   }
   ~PassTimer() {
-    if (!c->timing) return;
-    (void)hipEventRecord(c->ev1, c->stream);
-    (void)hipEventSynchronize(c->ev1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    c->pass_ms[pass] += ms;
-    c->pass_cnt[pass] += 1;
+    if (!c->timing || !a) return;
+    hipEvent_t b = take_event(c);
+    if (!b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({a, b, pass});
   }
 };
 
@@ -156,9 +197,9 @@ int run_sort(arm_ctx* c, const uint32_t** sval_out) {
     const int bins = 1 << bits;
     hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(SORT_THREADS), 0, c->stream, kin, n,
                        shift, bins, ntiles, c->hist);
-    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, c->stream, c->hist, bins * ntiles);
+    hipLaunchKernelGGL(k_sort_scan, dim3(bins), dim3(64), 0, c->stream, c->hist, c->tot, ntiles);
     hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, c->stream, kin,
-                       vin, c->sk[dst], c->sv[dst], n, shift, bits, ntiles, c->hist);
+                       vin, c->sk[dst], c->sv[dst], n, shift, bits, ntiles, c->hist, c->tot);
     kin = c->sk[dst];
     vin = c->sv[dst];
     dst ^= 1;
@@ -212,25 +253,42 @@ template <typename R> int link_list_t(arm_ctx* c) {
   HIPCHK(c, hipGetLastError());
   std::swap(A, B);
   c->sorted = true;
+  c->masks_valid = false;
   return check_domain(c);
 }
 
-template <typename R> int density_t(arm_ctx* c, int method) {
+template <typename R> int nseg_of(const arm_ctx* c) {
+  return (c->K.nx + TileCfg<R>::CW - 1) / TileCfg<R>::CW;
+}
+
+template <typename R, bool SUM> void launch_density(arm_ctx* c) {
   St<R>& A = stA<R>(c);
+  using T = TileCfg<R>;
+  if (c->tiled) {
+    const int nseg = nseg_of<R>(c);
+    hipLaunchKernelGGL((k_density_tiled<R, SUM, T::BLK, T::CW, T::CAP>),
+                       dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
+                       c->cstart, nseg, c->nbm);
+  } else {
+    hipLaunchKernelGGL((k_density<R, SUM>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A,
+                       c->K, c->skey, c->cstart, c->nbm);
+  }
+  c->masks_valid = true;
+}
+
+template <typename R> int density_t(arm_ctx* c, int method) {
   PassTimer tm(c, ARM_PASS_DENSITY);
   if (method == 1)
-    hipLaunchKernelGGL((k_density<R, true>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A,
-                       c->K, c->skey, c->cstart);
+    launch_density<R, true>(c);
   else
-    hipLaunchKernelGGL((k_density<R, false>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
-                       A, c->K, c->skey, c->cstart);
+    launch_density<R, false>(c);
   HIPCHK(c, hipGetLastError());
   return ARM_OK;
 }
 
 template <typename R> int eos_t(arm_ctx* c) {
   St<R>& A = stA<R>(c);
-  PassTimer tm(c, ARM_PASS_DENSITY);
+  PassTimer tm(c, ARM_PASS_EOS);
   hipLaunchKernelGGL(k_eos<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K);
   HIPCHK(c, hipGetLastError());
   return ARM_OK;
@@ -238,8 +296,16 @@ template <typename R> int eos_t(arm_ctx* c) {
 
 template <typename R, bool P, bool AV, bool AH, bool XS> void launch_force(arm_ctx* c) {
   St<R>& A = stA<R>(c);
-  hipLaunchKernelGGL((k_force<R, P, AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
-                     c->stream, A, c->K, c->skey, c->cstart);
+  using T = TileCfg<R>;
+  if (c->tiled) {
+    const int nseg = nseg_of<R>(c);
+    hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, T::BLK, T::CW, T::CAP>),
+                       dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
+                       c->cstart, nseg, c->masks_valid ? c->nbm : nullptr);
+  } else {
+    hipLaunchKernelGGL((k_force<R, P, AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
+                       c->stream, A, c->K, c->skey, c->cstart);
+  }
 }
 
 // dispatch over the compile-time flag combination
@@ -282,7 +348,7 @@ template <typename R> int single_step_t(arm_ctx* c) {
 
 template <typename R> int predictor_t(arm_ctx* c) {
   St<R>& A = stA<R>(c);
-  PassTimer tm(c, ARM_PASS_KEYS);
+  PassTimer tm(c, ARM_PASS_INTEGRATE);
   hipLaunchKernelGGL(k_predictor<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K);
   HIPCHK(c, hipGetLastError());
   return ARM_OK;
@@ -290,7 +356,7 @@ template <typename R> int predictor_t(arm_ctx* c) {
 
 template <typename R> int corrector_t(arm_ctx* c, int its, bool keys) {
   St<R>& A = stA<R>(c);
-  PassTimer tm(c, ARM_PASS_KEYS);
+  PassTimer tm(c, ARM_PASS_INTEGRATE);
   hipLaunchKernelGGL(k_corrector<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
                      its, keys ? c->key_by_id : nullptr, keys ? c->slot_by_id : nullptr,
                      c->d_err);
@@ -325,6 +391,7 @@ template <typename R> int time_integration_t(arm_ctx* c, int its0, int nsteps) {
       hipLaunchKernelGGL(k_reorder_pred<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
                          c->K, c->skey, sval, c->cstart, its != 1 ? 1 : 0);
       std::swap(A, B);
+      c->masks_valid = false;
     }
     if (!xs && (rc = zero_av<R>(c))) return rc;
     c->sorted = true;
@@ -598,6 +665,8 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
   K.C1 = 0.625 * K.wnorm / (P.h * P.h);
   K.psi0 = P.ah_g1 * P.h * P.c0;
   K.g2h2 = P.ah_g2 * P.h * P.h;
+  K.eps_m = P.xsph_eps * P.mass;
+  K.same_e2 = (K.av_e2 == K.ah_e2) ? 1 : 0;
   K.cellf = (float)P.cell;
   K.cut2f = (float)K.cut2;
 
@@ -620,10 +689,12 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     if (!rc) rc = dalloc(c, &c->sv[k], capacity);
   }
   if (!rc) rc = dalloc(c, &c->hist, (size_t)(1 << c->sort_bits) * c->ntiles);
+  if (!rc) rc = dalloc(c, &c->tot, (size_t)(1 << c->sort_bits));
   if (!rc) rc = dalloc(c, &c->cstart, (size_t)ncell + 1);
   if (!rc) rc = dalloc(c, &c->d_err, 1);
   if (!rc) rc = dalloc(c, &c->d_part, (size_t)grid_of(capacity) * 5);
   if (!rc) rc = dalloc(c, &c->d_cnt, capacity);
+  if (!rc) rc = dalloc(c, &c->nbm, capacity);
   if (!rc) {
     e = hipMemset(c->d_err, 0, sizeof(int));
     if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipMemset: %s", hipGetErrorString(e));
@@ -640,6 +711,8 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
 void arm_destroy(arm_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  resolve_times(c);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -661,6 +734,7 @@ int arm_upload(arm_ctx* c, int np, int nv, const double* x, const double* v, con
   c->K.n = c->n;
   c->K.np = np;
   c->sorted = false;
+  c->masks_valid = false;
   c->have_rates = false;
   c->domain_bad = 0;
   int rc = DISPATCH(c, upload_t)(c, x, v, rho, nullptr, u);
@@ -773,7 +847,8 @@ int arm_corrector(arm_ctx* c, int its) {
   if (!c->have_rates) return fail(c, ARM_ERR_STATE, "no rates: run arm_single_step first");
   int rc = DISPATCH(c, corrector_t)(c, its, false);
   if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
-  // positions moved: the cell order must be rebuilt before the next gather
+  // positions moved: the cell order and masks must be rebuilt before the next gather
+  c->masks_valid = false;
   if (!rc && is32(c)) {
     // refresh cell + offset of the FP32 form from the FP64 accumulators
     St<float>& A = c->A32;
@@ -804,6 +879,7 @@ int arm_conservation(arm_ctx* c, double out[5]) {
 
 int arm_pass_times(arm_ctx* c, int n, double* ms, int* counts) {
   if (!c || n < 0) return ARM_ERR_ARG;
+  resolve_times(c);
   for (int k = 0; k < n && k < ARM_PASS_COUNT; ++k) {
     if (ms) ms[k] = c->pass_ms[k];
     if (counts) counts[k] = c->pass_cnt[k];
@@ -813,10 +889,17 @@ int arm_pass_times(arm_ctx* c, int n, double* ms, int* counts) {
 
 int arm_reset_pass_times(arm_ctx* c) {
   if (!c) return ARM_ERR_ARG;
+  resolve_times(c);
   for (int k = 0; k < ARM_PASS_COUNT; ++k) {
     c->pass_ms[k] = 0;
     c->pass_cnt[k] = 0;
   }
+  return ARM_OK;
+}
+
+int arm_set_gather_path(arm_ctx* c, int tiled) {
+  if (!c) return ARM_ERR_ARG;
+  c->tiled = tiled != 0;
   return ARM_OK;
 }
 

@@ -28,6 +28,8 @@ struct KParams {
   double C1;     // grad W = C1 (q-2)^3 (x_i - x_j), C1 = 5 wnorm / (8 h^2)
   double psi0;   // g1 h c0 (constant part of the heat coefficient)
   double g2h2;   // g2 h h
+  double eps_m;  // XSPH eps * mass
+  int same_e2;   // artificial viscosity and heat share (h eta)^2
   float cellf, cut2f;
 };
 

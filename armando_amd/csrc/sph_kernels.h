@@ -57,16 +57,37 @@ __global__ __launch_bounds__(BLOCK) void k_keys(St<R> S, KParams K, uint32_t* __
 
 // ------------------------------------------------------------------------
 // cell_start from sorted keys: cell_start[c] = first sorted position with
-// key >= c, for c in [0, ncell]; each lane fills the cells between its
-// predecessor's key and its own.
+// key >= c, for c in [0, ncell].  Sorted position t owns the cells
+// (key[t-1], key[t]].  Short gaps are filled by their own lane; long gaps
+// (empty regions of the tank) by the whole wavefront, 64 cells per store;
+// the cells before the first key and after the last one by every lane of
+// the grid, strided.  Must be called by all lanes of the wavefront.
 // ------------------------------------------------------------------------
-__device__ __forceinline__ void fill_cell_start(const uint32_t* __restrict__ skey, int t, int n,
-                                                int ncell, int* __restrict__ cstart) {
-  const int k = (int)skey[t];
-  const int kp = (t == 0) ? -1 : (int)skey[t - 1];
-  for (int c = kp + 1; c <= k; ++c) cstart[c] = t;
-  if (t == n - 1)
-    for (int c = k + 1; c <= ncell; ++c) cstart[c] = n;
+__device__ __forceinline__ void fill_cell_start(const uint32_t* __restrict__ skey, int t,
+                                                bool valid, int n, int ncell,
+                                                int* __restrict__ cstart) {
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
+  const int first = (int)skey[0];
+  const int last = (int)skey[n - 1];
+  for (int c = g; c <= first; c += stride) cstart[c] = 0;
+  for (int c = last + 1 + g; c <= ncell; c += stride) cstart[c] = n;
+  int lo = 1, hi = 0;
+  if (valid && t > 0) {
+    lo = (int)skey[t - 1] + 1;
+    hi = (int)skey[t];
+  }
+  const bool longgap = hi - lo >= 8;
+  if (!longgap)
+    for (int c = lo; c <= hi; ++c) cstart[c] = t;
+  uint64_t m = __ballot(longgap);
+  while (m) {
+    const int L = __ffsll((unsigned long long)m) - 1;
+    const int l0 = __shfl(lo, L, 64), h0 = __shfl(hi, L, 64), tv = __shfl(t, L, 64);
+    for (int c = l0 + lane; c <= h0; c += 64) cstart[c] = tv;
+    m &= m - 1;
+  }
 }
 
 // Permute every field from state A to state B (per-routine link_list).
@@ -76,7 +97,9 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_all(St<R> A, St<R> Bs, KParam
                                                        const uint32_t* __restrict__ sval,
                                                        int* __restrict__ cstart) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t >= K.n) return;
+  const bool valid = t < K.n;
+  fill_cell_start(skey, t, valid, K.n, K.ncell, cstart);
+  if (!valid) return;
   const int s = (int)sval[t];
   Bs.id[t] = A.id[s];
   Bs.xy[t] = A.xy[s];
@@ -98,7 +121,6 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_all(St<R> A, St<R> Bs, KParam
   Bs.rho_min[t] = A.rho_min[s];
   Bs.u_min[t] = A.u_min[s];
   Bs.v_min[t] = A.v_min[s];
-  fill_cell_start(skey, t, K.n, K.ncell, cstart);
 }
 
 // Fused reorder + leapfrog predictor (time_integration.f:136-154): the
@@ -110,7 +132,9 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
                                                         int* __restrict__ cstart, int predict) {
   using R2 = typename V2<R>::type;
   const int t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t >= K.n) return;
+  const bool valid = t < K.n;
+  fill_cell_start(skey, t, valid, K.n, K.ncell, cstart);
+  if (!valid) return;
   const int s = (int)sval[t];
   const int id = A.id[s];
   Bs.id[t] = id;
@@ -146,7 +170,6 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
   Bs.u[t] = u;
   if constexpr (sizeof(R) == 4) Bs.uf[t] = (float)u;
   Bs.p[t] = A.p[s];
-  fill_cell_start(skey, t, K.n, K.ncell, cstart);
 }
 
 // ------------------------------------------------------------------------
@@ -178,27 +201,183 @@ __device__ __forceinline__ Rows rows_of(const KParams& K, uint32_t key,
 }
 
 // ------------------------------------------------------------------------
-// density pass: con_density (density.f:166-223) and, for method 1,
-// sum_density (density.f:1-62), gathered.  Also the raw velocity
-// divergence sum of art_heat's first pass (art_heat.f:46-67): with a
-// uniform mass it is the negative of the continuity sum.
+// pair arithmetic shared by every gather path (same operations in the same
+// order, so the tiled and the direct paths give identical bits)
 // ------------------------------------------------------------------------
-template <typename R, bool SUMMATION>
-__global__ __launch_bounds__(BLOCK) void k_density(St<R> S, KParams K,
-                                                   const uint32_t* __restrict__ skey,
-                                                   const int* __restrict__ cstart) {
+
+// density pass: con_density (density.f:166-223) and, for method 1,
+// sum_density (density.f:1-62).  dvx = v_i.x - v_j.x.
+template <typename R, bool SUM>
+__device__ __forceinline__ void dens_pair(const KParams& K, R dx, R dy, R r2, R dvx, R dvy,
+                                          Acc<R>& sum, Acc<R>& rs) {
+  const R q = fsqrt(r2) * (R)K.invh;
+  const R tq = q - (R)2;
+  const R F = (R)K.C1 * (tq * tq * tq);
+  const R dwx = F * dx, dwy = F * dy;
+  R vcc = dvx * dwx;
+  vcc = vcc + dvy * dwy;
+  sum.add((R)K.mass * vcc);
+  if (SUM) {
+    const R s = (R)1 - (R)0.5 * q;
+    const R s2 = s * s;
+    rs.add((R)K.mass * ((R)K.wnorm * (s2 * s2) * ((R)1 + (R)2 * q)));
+  }
+}
+
+template <typename R> struct DensOwn {
+  Acc<R> sum, rs;
+};
+
+template <typename R, bool SUM>
+__device__ __forceinline__ void dens_begin(const KParams& K, DensOwn<R>& o) {
+  if (SUM) o.rs.add((R)K.mass * (R)K.wnorm);
+}
+
+// Writes drho (rate particles), rho (summation) and the raw divergence sum
+// of art_heat's first pass (art_heat.f:46-67): with a uniform mass it is the
+// negative of the continuity sum; the EOS divides it by rho.
+template <typename R, bool SUM>
+__device__ __forceinline__ void dens_end(St<R>& S, const KParams& K, int i, const DensOwn<R>& o) {
+  const bool rate = is_rate(K, S.id[i]);
+  if (!SUM) {
+    S.drho[i] = rate ? o.sum.get() : (R)0;
+  } else {
+    S.drho[i] = (R)0;
+    if (rate) {
+      if constexpr (sizeof(R) == 4)
+        S.rho[i] = o.rs.get() - (float)K.rho0;
+      else
+        S.rho[i] = o.rs.get();
+    }
+  }
+  S.divv[i] = -o.sum.get();
+}
+
+// force pass: int_force pressure part (internal_force.f:178-259, 309-430),
+// art_visc (art_visc.f:55-101), art_heat second pass (art_heat.f:69-97)
+// and av_vel (av_vel.f:37-56), rewritten around per-particle
+//   Q = 1/rho and A = p/rho:
+//   pa_sph 0: (p_i+p_j)/(rho_i rho_j) = A_i Q_j + A_j Q_i
+//   pa_sph 1: p_i/rho_i^2 + p_j/rho_j^2 = A_i Q_i + A_j Q_j
+//   1/rho_bar = 2/(rho_i+rho_j) = 2 Q_i Q_j / (Q_i + Q_j)
+// and grad_i W = F (x_i - x_j), so (v_i-v_j).grad W = F vr and
+// (x_i-x_j).grad W = F r^2.  Per pair:
+//   G   = m F (a + Pi_ij)                  (Pi_ij only when vr < 0)
+//   a_i -= G (x_i - x_j)
+//   du_i += G vr / 2 + m (psi_i+psi_j) F r^2 / (rho_bar (r^2+(h eta)^2)) (u_i-u_j)
+//   av_i -= eps m / rho_bar W (v_i - v_j)
+// The energy rate is one compensated (FP32) sum.
+template <typename R> struct ForceOwn {
   using R2 = typename V2<R>::type;
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= K.n) return;
+  R2 v;
+  R Q, A, psi, u;
+  R fx = 0, fy = 0, xsx = 0, xsy = 0;
+  Acc<R> du;
+};
+
+// Q = 1/rho and A = p/rho of a particle from its stored density.
+template <typename R>
+__device__ __forceinline__ void q_and_a(const KParams& K, R rho_s, R p, R& Q, R& A) {
+  Q = (R)1 / dens(K, rho_s);
+  A = p * Q;
+}
+
+template <typename R>
+__device__ __forceinline__ void force_begin(const St<R>& S, const KParams& K, int i, ForceOwn<R>& o) {
+  o.v = S.v[i];
+  q_and_a(K, S.rho[i], S.p[i], o.Q, o.A);
+  o.psi = S.psi[i];
+  if constexpr (sizeof(R) == 4) o.u = S.uf[i]; else o.u = (R)S.u[i];
+}
+
+template <typename R, bool PRESS, bool AV, bool AH, bool XS>
+__device__ __forceinline__ void force_pair(const KParams& K, ForceOwn<R>& o, R dx, R dy, R r2,
+                                           R vjx, R vjy, R Qj, R Aj, R psij, R uj) {
+  const R m = (R)K.mass;
+  const R q = fsqrt(r2) * (R)K.invh;
+  const R tq = q - (R)2;
+  const R F = (R)K.C1 * (tq * tq * tq);
+  const R dvx = o.v.x - vjx, dvy = o.v.y - vjy;
+  const R vr = dvx * dx + dvy * dy;
+  const R inv_mrho = (R)2 * o.Q * Qj * frcp(o.Q + Qj);
+  R coef = 0;
+  if (PRESS) coef = (K.pa_sph == 0) ? (o.A * Qj + Aj * o.Q) : (o.A * o.Q + Aj * Qj);
+  R inv_ea = 0;
+  if (AV) {
+    inv_ea = frcp(r2 + (R)K.av_e2);
+    if (vr < (R)0) {
+      const R phi = (R)K.h * vr * inv_ea;
+      coef = coef + ((R)K.av_beta * phi - (R)K.av_alpha * (R)K.c0) * phi * inv_mrho;
+    }
+  }
+  if (PRESS || AV) {
+    const R G = m * F * coef;
+    o.fx = o.fx - G * dx;
+    o.fy = o.fy - G * dy;
+    R dut = (R)0.5 * G * vr;
+    if (AH) {
+      const R inv_eh = (AV && K.same_e2) ? inv_ea : frcp(r2 + (R)K.ah_e2);
+      dut = dut + m * (o.psi + psij) * F * r2 * inv_mrho * inv_eh * (o.u - uj);
+    }
+    o.du.add(dut);
+  } else if (AH) {
+    const R inv_eh = frcp(r2 + (R)K.ah_e2);
+    o.du.add(m * (o.psi + psij) * F * r2 * inv_mrho * inv_eh * (o.u - uj));
+  }
+  if (XS) {
+    const R s = (R)1 - (R)0.5 * q;
+    const R s2 = s * s;
+    const R w = (R)K.wnorm * (s2 * s2) * ((R)1 + (R)2 * q);
+    const R wf = (R)K.eps_m * inv_mrho * w;
+    o.xsx = o.xsx - wf * dvx;
+    o.xsy = o.xsy - wf * dvy;
+  }
+}
+
+template <typename R, bool PRESS, bool XS>
+__device__ __forceinline__ void force_end(St<R>& S, const KParams& K, int i, const ForceOwn<R>& o) {
+  using R2 = typename V2<R>::type;
+  if (PRESS) {
+    const int id = S.id[i];
+    const bool grav = (K.flags & 16) && id < K.np;
+    const R ex = grav ? (R)K.gx : (R)0, ey = grav ? (R)K.gy : (R)0;
+    R2 acc;
+    acc.x = o.fx + ex;
+    acc.y = o.fy + ey;
+    S.a[i] = acc;
+    S.du[i] = o.du.get();
+  }
+  if (XS) {
+    R2 w;
+    w.x = o.xsx;
+    w.y = o.xsy;
+    S.av[i] = w;
+  }
+}
+
+template <typename R> __device__ __forceinline__ R u_of(const St<R>& S, int t) {
+  if constexpr (sizeof(R) == 4) return S.uf[t]; else return (R)S.u[t];
+}
+
+// ---- direct path: neighbour data straight from global memory ------------
+// Also records the neighbour masks of the three rows (bit b = candidate
+// t0 + b) for the force pass; w = 0 marks them incomplete (a row longer than
+// 32 candidates).
+template <typename R, bool SUM>
+__device__ __forceinline__ void density_direct(St<R>& S, const KParams& K, int i,
+                                               const uint32_t* __restrict__ skey,
+                                               const int* __restrict__ cstart,
+                                               uint4* __restrict__ nbm) {
   const auto pi = load_pos(S, i);
-  const R2 vi = S.v[i];
+  const auto vi = S.v[i];
   const Rows rw = rows_of(K, skey[i], cstart);
   const R cut2 = cutoff2<R>(K);
-  const R invh = (R)K.invh, C1 = (R)K.C1, m = (R)K.mass, wn = (R)K.wnorm;
-  Acc<R> sum;
-  Acc<R> rs;
-  if (SUMMATION) rs.add(m * wn);
+  DensOwn<R> o;
+  dens_begin<R, SUM>(K, o);
+  uint32_t rm[3] = {0u, 0u, 0u};
+  bool ok = true;
   for (int k = 0; k < 3; ++k) {
+    if (rw.t1[k] - rw.t0[k] > 32) ok = false;
     for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
       if (t == i) continue;
       const auto pj = load_pos(S, t);
@@ -206,34 +385,48 @@ __global__ __launch_bounds__(BLOCK) void k_density(St<R> S, KParams K,
       delta(pi, pj, K, dx, dy);
       const R r2 = dx * dx + dy * dy;
       if (!(r2 < cut2)) continue;
-      const R q = fsqrt(r2) * invh;
-      const R tq = q - (R)2;
-      const R F = C1 * (tq * tq * tq);
-      const R2 vj = S.v[t];
-      const R dwx = F * dx, dwy = F * dy;
-      R vcc = (vi.x - vj.x) * dwx;
-      vcc = vcc + (vi.y - vj.y) * dwy;
-      sum.add(m * vcc);
-      if (SUMMATION) {
-        const R s = (R)1 - (R)0.5 * q;
-        const R s2 = s * s;
-        rs.add(m * (wn * (s2 * s2) * ((R)1 + (R)2 * q)));
-      }
+      if (t - rw.t0[k] < 32) rm[k] |= 1u << (t - rw.t0[k]);
+      const auto vj = S.v[t];
+      dens_pair<R, SUM>(K, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
     }
   }
-  const int id = S.id[i];
-  const bool rate = is_rate(K, id);
-  if (!SUMMATION) S.drho[i] = rate ? sum.get() : (R)0;
-  else {
-    S.drho[i] = (R)0;
-    if (rate) {
-      if constexpr (sizeof(R) == 4)
-        S.rho[i] = rs.get() - (float)K.rho0;
-      else
-        S.rho[i] = rs.get();
+  if (nbm) nbm[i] = make_uint4(rm[0], rm[1], rm[2], ok ? 1u : 0u);
+  dens_end<R, SUM>(S, K, i, o);
+}
+
+template <typename R, bool PRESS, bool AV, bool AH, bool XS>
+__device__ __forceinline__ void force_direct(St<R>& S, const KParams& K, int i,
+                                             const uint32_t* __restrict__ skey,
+                                             const int* __restrict__ cstart) {
+  const auto pi = load_pos(S, i);
+  ForceOwn<R> o;
+  force_begin(S, K, i, o);
+  const Rows rw = rows_of(K, skey[i], cstart);
+  const R cut2 = cutoff2<R>(K);
+  for (int k = 0; k < 3; ++k)
+    for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
+      if (t == i) continue;
+      const auto pj = load_pos(S, t);
+      R dx, dy;
+      delta(pi, pj, K, dx, dy);
+      const R r2 = dx * dx + dy * dy;
+      if (!(r2 < cut2)) continue;
+      const auto vj = S.v[t];
+      R Qj, Aj;
+      q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
+      force_pair<R, PRESS, AV, AH, XS>(K, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t], u_of(S, t));
     }
-  }
-  S.divv[i] = -sum.get();
+  force_end<R, PRESS, XS>(S, K, i, o);
+}
+
+template <typename R, bool SUM>
+__global__ __launch_bounds__(BLOCK) void k_density(St<R> S, KParams K,
+                                                   const uint32_t* __restrict__ skey,
+                                                   const int* __restrict__ cstart,
+                                                   uint4* __restrict__ nbm) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= K.n) return;
+  density_direct<R, SUM>(S, K, i, skey, cstart, nbm);
 }
 
 // Tait EOS (eos.f90:255-283) on rate particles, then the art_heat
@@ -258,110 +451,13 @@ __global__ __launch_bounds__(BLOCK) void k_eos(St<R> S, KParams K) {
   S.psi[i] = (R)K.psi0 + (R)K.g2h2 * (ad - dv);
 }
 
-// ------------------------------------------------------------------------
-// force pass: int_force pressure part (internal_force.f:178-259, 309-430),
-// art_visc (art_visc.f:55-101), art_heat second pass (art_heat.f:69-97)
-// and av_vel (av_vel.f:37-56), fused over one neighbour walk.
-//   PRESS: pressure force and work, plus gravity on real particles
-//   AV, AH, XS: artificial viscosity, artificial heat, XSPH
-// Outputs are written (not accumulated): a and du when PRESS, av when XS.
-// ------------------------------------------------------------------------
 template <typename R, bool PRESS, bool AV, bool AH, bool XS>
 __global__ __launch_bounds__(BLOCK) void k_force(St<R> S, KParams K,
                                                  const uint32_t* __restrict__ skey,
                                                  const int* __restrict__ cstart) {
-  using R2 = typename V2<R>::type;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
-  const auto pi = load_pos(S, i);
-  const R2 vi = S.v[i];
-  const R rhoi = dens(K, S.rho[i]);
-  const R p_i = S.p[i];
-  const R psii = S.psi[i];
-  R ui;
-  if constexpr (sizeof(R) == 4) ui = S.uf[i]; else ui = (R)S.u[i];
-  const Rows rw = rows_of(K, skey[i], cstart);
-  const R cut2 = cutoff2<R>(K);
-  const R invh = (R)K.invh, C1 = (R)K.C1, m = (R)K.mass, wn = (R)K.wnorm, h = (R)K.h;
-  const R av_e2 = (R)K.av_e2, ah_e2 = (R)K.ah_e2, c0 = (R)K.c0;
-  const R alpha = (R)K.av_alpha, beta = (R)K.av_beta, eps = (R)K.xsph_eps;
-  const R pr_i = p_i / (rhoi * rhoi);
-  R fx = 0, fy = 0, avfx = 0, avfy = 0, xsx = 0, xsy = 0;
-  Acc<R> due, avdu, ahdu;
-  for (int k = 0; k < 3; ++k) {
-    for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
-      if (t == i) continue;
-      const auto pj = load_pos(S, t);
-      R dx, dy;
-      delta(pi, pj, K, dx, dy);
-      const R r2 = dx * dx + dy * dy;
-      if (!(r2 < cut2)) continue;
-      const R q = fsqrt(r2) * invh;
-      const R tq = q - (R)2;
-      const R F = C1 * (tq * tq * tq);
-      const R dwx = F * dx, dwy = F * dy;
-      const R2 vj = S.v[t];
-      const R dvx = vi.x - vj.x, dvy = vi.y - vj.y;
-      R vdw = dvx * dwx;
-      vdw = vdw + dvy * dwy;
-      const R rhoj = dens(K, S.rho[t]);
-      const R mrho = (R)0.5 * (rhoi + rhoj);
-      if (PRESS) {
-        const R pj_ = S.p[t];
-        R a;
-        if (K.pa_sph == 0)
-          a = (p_i + pj_) / (rhoi * rhoj);
-        else
-          a = pr_i + pj_ / (rhoj * rhoj);
-        fx = fx - m * a * dwx;
-        fy = fy - m * a * dwy;
-        due.add(m * a * vdw);
-      }
-      if (AV) {
-        R vr = dvx * dx;
-        vr = vr + dvy * dy;
-        if (vr < (R)0) {
-          const R phi = (h * vr) / (r2 + av_e2);
-          const R piv = (beta * phi - alpha * c0) * phi / mrho;
-          avfx = avfx - m * piv * dwx;
-          avfy = avfy - m * piv * dwy;
-          avdu.add((R)0.5 * m * piv * vdw);
-        }
-      }
-      if (AH) {
-        R rdw = dx * dwx;
-        rdw = rdw + dy * dwy;
-        const R pij = (R)0.5 * (psii + S.psi[t]);
-        const R ps = ((R)2 * pij) / (mrho * (r2 + ah_e2)) * rdw;
-        R uj;
-        if constexpr (sizeof(R) == 4) uj = S.uf[t]; else uj = (R)S.u[t];
-        ahdu.add(m * ps * (ui - uj));
-      }
-      if (XS) {
-        const R s = (R)1 - (R)0.5 * q;
-        const R s2 = s * s;
-        const R w = wn * (s2 * s2) * ((R)1 + (R)2 * q);
-        xsx = xsx - eps * m / mrho * dvx * w;
-        xsy = xsy - eps * m / mrho * dvy * w;
-      }
-    }
-  }
-  if (PRESS) {
-    const int id = S.id[i];
-    const bool grav = (K.flags & 16) && id < K.np;
-    const R ex = grav ? (R)K.gx : (R)0, ey = grav ? (R)K.gy : (R)0;
-    R2 acc;
-    acc.x = fx + ex + avfx;
-    acc.y = fy + ey + avfy;
-    S.a[i] = acc;
-    S.du[i] = (R)0.5 * due.get() + avdu.get() + ahdu.get();
-  }
-  if (XS) {
-    R2 w;
-    w.x = xsx;
-    w.y = xsy;
-    S.av[i] = w;
-  }
+  force_direct<R, PRESS, AV, AH, XS>(S, K, i, skey, cstart);
 }
 
 // ------------------------------------------------------------------------

@@ -1,0 +1,325 @@
+// sph_tiled.h -- LDS-tiled gathers of the density and force passes.
+//
+// A workgroup owns a segment of CW consecutive cells of one cell row.  Its
+// particles are one contiguous sorted range, and the candidates of all of
+// them lie in three contiguous sorted ranges (the cell row below, the row
+// itself and the row above, cells cx0-1 .. cx0+CW).  The workgroup copies
+// those three ranges, and the cell_start entries that delimit their cells,
+// into LDS with coalesced loads.  Each lane then handles one particle, row
+// by row:
+//   A  test the row's candidates (r^2 < (2h)^2) eight at a time, reading
+//      their positions from LDS with independent loads, into a 64-bit
+//      neighbour mask;
+//   B  run the pair arithmetic over the set bits in ascending order, so the
+//      wavefront does the expensive math only for real neighbours.
+// The visiting order (rows bottom to top, ascending sorted position) and the
+// arithmetic are those of the direct path, so both give identical bits.
+// Rows longer than 64 candidates are handled in 64-candidate chunks; when
+// the three ranges exceed the LDS capacity the workgroup falls back to the
+// direct path.
+#pragma once
+
+#include "sph_kernels.h"
+
+namespace arm {
+
+// ---- LDS tiles ------------------------------------------------------------
+template <typename R, int CAP> struct DensTile;
+template <int CAP> struct DensTile<float, CAP> {
+  float4 a[CAP];  // offx, offy, vx, vy
+  uint32_t c[CAP];
+  __device__ __forceinline__ void put(int l, const St<float>& S, const KParams&, int t) {
+    const float2 o = S.off[t];
+    const float2 v = S.v[t];
+    a[l] = make_float4(o.x, o.y, v.x, v.y);
+    c[l] = S.cxy[t];
+  }
+  __device__ __forceinline__ PosF pos(int l) const {
+    const uint32_t cc = c[l];
+    const float4 q = a[l];
+    return {int(cc & 0xffffu), int(cc >> 16), q.x, q.y};
+  }
+  __device__ __forceinline__ float2 vel(int l) const {
+    const float4 q = a[l];
+    return make_float2(q.z, q.w);
+  }
+};
+template <int CAP> struct DensTile<double, CAP> {
+  double2 x[CAP], v[CAP];
+  __device__ __forceinline__ void put(int l, const St<double>& S, const KParams&, int t) {
+    x[l] = S.xy[t];
+    v[l] = S.v[t];
+  }
+  __device__ __forceinline__ PosD pos(int l) const {
+    const double2 q = x[l];
+    return {q.x, q.y};
+  }
+  __device__ __forceinline__ double2 vel(int l) const { return v[l]; }
+};
+
+template <typename R, int CAP> struct ForceTile;
+template <int CAP> struct ForceTile<float, CAP> {
+  float4 a[CAP];  // offx, offy, vx, vy
+  float4 b[CAP];  // Q = 1/rho, A = p/rho, psi, u
+  uint32_t c[CAP];
+  __device__ __forceinline__ void put(int l, const St<float>& S, const KParams& K, int t) {
+    const float2 o = S.off[t];
+    const float2 v = S.v[t];
+    a[l] = make_float4(o.x, o.y, v.x, v.y);
+    float Q, A;
+    q_and_a(K, S.rho[t], S.p[t], Q, A);
+    b[l] = make_float4(Q, A, S.psi[t], S.uf[t]);
+    c[l] = S.cxy[t];
+  }
+  __device__ __forceinline__ PosF pos(int l) const {
+    const uint32_t cc = c[l];
+    const float4 q = a[l];
+    return {int(cc & 0xffffu), int(cc >> 16), q.x, q.y};
+  }
+  template <bool PRESS, bool AV, bool AH, bool XS>
+  __device__ __forceinline__ void pair(const KParams& K, ForceOwn<float>& o, int l, float dx,
+                                       float dy, float r2) const {
+    const float4 q = a[l];
+    const float4 w = b[l];
+    force_pair<float, PRESS, AV, AH, XS>(K, o, dx, dy, r2, q.z, q.w, w.x, w.y, w.z, w.w);
+  }
+};
+template <int CAP> struct ForceTile<double, CAP> {
+  double2 x[CAP], v[CAP], rp[CAP], pu[CAP];  // rp = (Q, A), pu = (psi, u)
+  __device__ __forceinline__ void put(int l, const St<double>& S, const KParams& K, int t) {
+    x[l] = S.xy[t];
+    v[l] = S.v[t];
+    double Q, A;
+    q_and_a(K, S.rho[t], S.p[t], Q, A);
+    rp[l] = make_double2(Q, A);
+    pu[l] = make_double2(S.psi[t], S.u[t]);
+  }
+  __device__ __forceinline__ PosD pos(int l) const {
+    const double2 q = x[l];
+    return {q.x, q.y};
+  }
+  template <bool PRESS, bool AV, bool AH, bool XS>
+  __device__ __forceinline__ void pair(const KParams& K, ForceOwn<double>& o, int l, double dx,
+                                       double dy, double r2) const {
+    const double2 vv = v[l];
+    const double2 a = rp[l];
+    const double2 b = pu[l];
+    force_pair<double, PRESS, AV, AH, XS>(K, o, dx, dy, r2, vv.x, vv.y, a.x, a.y, b.x, b.y);
+  }
+};
+
+// ---- segment geometry ------------------------------------------------------
+template <int CW> struct Segment {
+  int cy, slo, O0, O1;
+  int rs[3], rl[3], lo[3];
+  int total;
+};
+
+// cs[k][j] = cell_start of cell (slo + j) in row cy-1+k, j = 0 .. CW+2
+template <int CW>
+__device__ __forceinline__ Segment<CW> segment_of(const KParams& K, const int* __restrict__ cstart,
+                                                  int nseg, int (*cs)[CW + 3]) {
+  Segment<CW> g;
+  const int nx = K.nx;
+  g.cy = blockIdx.x / nseg;
+  const int cx0 = (blockIdx.x - g.cy * nseg) * CW;
+  const int cx1 = min(cx0 + CW, nx);
+  g.O0 = cstart[g.cy * nx + cx0];
+  g.O1 = cstart[g.cy * nx + cx1];
+  g.slo = cx0 > 0 ? cx0 - 1 : 0;
+  const int shi = cx1 < nx ? cx1 : nx - 1;
+  g.total = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int ry = g.cy - 1 + k;
+    if (ry < 0 || ry >= K.ny || g.O0 >= g.O1) {
+      g.rs[k] = 0;
+      g.rl[k] = 0;
+    } else {
+      g.rs[k] = cstart[ry * nx + g.slo];
+      g.rl[k] = cstart[ry * nx + shi + 1] - g.rs[k];
+      const int ncs = shi + 2 - g.slo;
+      for (int j = threadIdx.x; j < ncs; j += blockDim.x) cs[k][j] = cstart[ry * nx + g.slo + j];
+    }
+    g.lo[k] = g.total;
+    g.total += g.rl[k];
+  }
+  return g;
+}
+
+// Neighbour mask of candidates l0 .. l0+len-1 (len <= 64) of one row.
+template <typename R, typename Tile, typename Pos>
+__device__ __forceinline__ uint64_t row_mask(const KParams& K, const Tile& T, const Pos& pi,
+                                             int l0, int len, int self) {
+  const R cut2 = cutoff2<R>(K);
+  uint64_t m = 0;
+  for (int k0 = 0; k0 < len; k0 += 8) {
+    Pos pj[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) pj[q] = T.pos(l0 + min(k0 + q, len - 1));
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      R dx, dy;
+      delta(pi, pj[q], K, dx, dy);
+      const R r2 = dx * dx + dy * dy;
+      const int kk = k0 + q;
+      if (kk < len && r2 < cut2 && l0 + kk != self) m |= 1ull << kk;
+    }
+  }
+  return m;
+}
+
+// Calls visit(l) for every neighbour of lane particle i, rows bottom to top,
+// ascending sorted position within a row.  With rm != nullptr it also
+// returns the neighbour masks of the three rows (bit b = candidate ta_k + b)
+// and whether they are complete (every row at most 32 candidates).
+template <typename R, int CW, typename Tile, typename Pos, typename Visit>
+__device__ __forceinline__ void for_neighbours(const KParams& K, const Segment<CW>& g,
+                                               const int (*cs)[CW + 3], const Tile& T,
+                                               const Pos& pi, int i, uint32_t key,
+                                               Visit&& visit, uint32_t* rm = nullptr,
+                                               bool* ok = nullptr) {
+  const int nx = K.nx;
+  const int cx = (int)key - g.cy * nx;
+  const int a0 = (cx > 0 ? cx - 1 : 0) - g.slo;
+  const int a1 = (cx + 1 < nx ? cx + 1 : nx - 1) - g.slo + 1;
+  const int self = g.lo[1] + (i - g.rs[1]);
+  for (int k = 0; k < 3; ++k) {
+    if (g.rl[k] == 0) continue;
+    const int ta = cs[k][a0];
+    const int tb = cs[k][a1];
+    const int base = g.lo[k] - g.rs[k];
+    if (rm && tb - ta > 32) *ok = false;
+    for (int c0 = ta; c0 < tb; c0 += 64) {
+      const int len = min(64, tb - c0);
+      uint64_t m = row_mask<R>(K, T, pi, base + c0, len, self);
+      if (rm && c0 == ta) rm[k] = (uint32_t)m;
+      while (m) {
+        const int b = __builtin_ctzll(m);
+        m &= m - 1;
+        visit(base + c0 + b);
+      }
+    }
+  }
+}
+
+// Same walk driven by masks recorded by the density pass.
+template <int CW, typename Visit>
+__device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& g,
+                                           const int (*cs)[CW + 3], uint32_t key, uint4 nm,
+                                           Visit&& visit) {
+  const int nx = K.nx;
+  const int cx = (int)key - g.cy * nx;
+  const int a0 = (cx > 0 ? cx - 1 : 0) - g.slo;
+  const uint32_t rm[3] = {nm.x, nm.y, nm.z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (g.rl[k] == 0) continue;
+    const int l0 = g.lo[k] - g.rs[k] + cs[k][a0];
+    uint32_t m = rm[k];
+    while (m) {
+      const int b = __builtin_ctz(m);
+      m &= m - 1;
+      visit(l0 + b);
+    }
+  }
+}
+
+template <typename R, bool SUM, int BLK, int CW, int CAP>
+__global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
+                                                       const uint32_t* __restrict__ skey,
+                                                       const int* __restrict__ cstart, int nseg,
+                                                       uint4* __restrict__ nbm) {
+  __shared__ DensTile<R, CAP> T;
+  __shared__ int cs[3][CW + 3];
+  const Segment<CW> g = segment_of<CW>(K, cstart, nseg, cs);
+  if (g.O0 >= g.O1) return;
+  const bool tiled = g.total <= CAP;
+  if (tiled) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      for (int q = threadIdx.x; q < g.rl[k]; q += BLK) T.put(g.lo[k] + q, S, K, g.rs[k] + q);
+  }
+  __syncthreads();
+  for (int i0 = g.O0; i0 < g.O1; i0 += BLK) {
+    const int i = i0 + threadIdx.x;
+    if (i >= g.O1) continue;
+    if (!tiled) {
+      density_direct<R, SUM>(S, K, i, skey, cstart, nbm);
+      continue;
+    }
+    const int self = g.lo[1] + (i - g.rs[1]);
+    const auto pi = T.pos(self);
+    const auto vi = T.vel(self);
+    DensOwn<R> o;
+    dens_begin<R, SUM>(K, o);
+    uint32_t rm[3] = {0u, 0u, 0u};
+    bool ok = true;
+    for_neighbours<R, CW>(
+        K, g, cs, T, pi, i, skey[i],
+        [&](int l) {
+          R dx, dy;
+          delta(pi, T.pos(l), K, dx, dy);
+          const R r2 = dx * dx + dy * dy;
+          const auto vj = T.vel(l);
+          dens_pair<R, SUM>(K, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
+        },
+        rm, &ok);
+    nbm[i] = make_uint4(rm[0], rm[1], rm[2], ok ? 1u : 0u);
+    dens_end<R, SUM>(S, K, i, o);
+  }
+}
+
+template <typename R, bool PRESS, bool AV, bool AH, bool XS, int BLK, int CW, int CAP>
+__global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
+                                                     const uint32_t* __restrict__ skey,
+                                                     const int* __restrict__ cstart, int nseg,
+                                                     const uint4* __restrict__ nbm) {
+  __shared__ ForceTile<R, CAP> T;
+  __shared__ int cs[3][CW + 3];
+  const Segment<CW> g = segment_of<CW>(K, cstart, nseg, cs);
+  if (g.O0 >= g.O1) return;
+  const bool tiled = g.total <= CAP;
+  if (tiled) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      for (int q = threadIdx.x; q < g.rl[k]; q += BLK) T.put(g.lo[k] + q, S, K, g.rs[k] + q);
+  }
+  __syncthreads();
+  for (int i0 = g.O0; i0 < g.O1; i0 += BLK) {
+    const int i = i0 + threadIdx.x;
+    if (i >= g.O1) continue;
+    if (!tiled) {
+      force_direct<R, PRESS, AV, AH, XS>(S, K, i, skey, cstart);
+      continue;
+    }
+    const int self = g.lo[1] + (i - g.rs[1]);
+    const auto pi = T.pos(self);
+    ForceOwn<R> o;
+    force_begin(S, K, i, o);
+    auto visit = [&](int l) {
+      R dx, dy;
+      delta(pi, T.pos(l), K, dx, dy);
+      const R r2 = dx * dx + dy * dy;
+      T.template pair<PRESS, AV, AH, XS>(K, o, l, dx, dy, r2);
+    };
+    const uint32_t key = skey[i];
+    const uint4 nm = nbm ? nbm[i] : make_uint4(0u, 0u, 0u, 0u);
+    if (nm.w)
+      for_masked<CW>(K, g, cs, key, nm, visit);
+    else
+      for_neighbours<R, CW>(K, g, cs, T, pi, i, key, visit);
+    force_end<R, PRESS, XS>(S, K, i, o);
+  }
+}
+
+// Launch shapes: cells per segment, lanes, LDS record capacity.
+template <typename R> struct TileCfg;
+template <> struct TileCfg<float> {
+  static constexpr int BLK = 256, CW = 32, CAP = 1024;
+};
+template <> struct TileCfg<double> {
+  static constexpr int BLK = 128, CW = 16, CAP = 640;
+};
+
+}  // namespace arm

@@ -1,0 +1,204 @@
+"""Throughput of the WCSPH time step on one MI355X.
+
+Workload: the 2D dam break with 720,000 fluid particles (0.6 x 1.2 m column
+in a 3.2 x 1.6 m tank, dx = 1e-3 m, seeded jitter +-0.01 dx) plus 19,218 wall
+particles, Tait EOS, Monaghan viscosity (alpha 0.01, beta 0), artificial heat,
+XSPH (eps 0.5), leapfrog at dt = 0.2 h / c0.  One step is one pass of
+time_integration: keys, sort, reorder + predictor, density, EOS, force
+(+ artificial viscosity, artificial heat, XSPH), corrector.
+
+The headline `value` is the FP32 chosen form in particle-steps/s (walls
+included); the FP64 plain form runs the same steps for the FP32/FP64 speedup
+and for the accuracy comparison.  Inputs are resident in HBM before the timed
+region starts; the timed region is K whole steps.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp64]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s, 720k 2D dam break, FP32 vs FP64; adjoint/forward time ratio"
+
+# Algorithmic bytes per particle per launch: every particle-indexed array the
+# kernel reads once plus every one it writes once (neighbour re-reads,
+# cell-range arrays and uniform constants are not counted).
+BYTES = {
+    # force gather: reads skey, cell, offset, v, rho', p, psi, u, id;
+    # writes a, du, av
+    ("force", 1): 4 + 4 + 8 + 8 + 4 + 4 + 4 + 4 + 4 + (8 + 4 + 8),
+    # FP64: reads skey, xy, v, rho, p, psi, u, id; writes a, du, av
+    ("force", 0): 4 + 16 + 16 + 8 + 8 + 8 + 8 + 4 + (16 + 8 + 16),
+    # density gather: reads skey, pos, v, id; writes drho, divv
+    ("density", 1): 4 + 4 + 8 + 8 + 4 + (4 + 4),
+    ("density", 0): 4 + 16 + 16 + 4 + (8 + 8),
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 form")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="approximate CPU work for the baseline sample")
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="dam-break size factor (1.0 = the 720k workload)")
+    return ap.parse_args()
+
+
+def run_form(precision, args, case_fn):
+    import armando_amd
+
+    case = case_fn(precision)
+    sim = armando_amd.Simulation(case)
+    niac = sim.link_list(count=True)
+    # warmup steps (untimed), starting from the initial state
+    sim.time_integration(1, args.warmup)
+    its = 1 + args.warmup
+    # timed region: whole steps, no per-pass events
+    sim.set_timing(False)
+    t0 = time.perf_counter()
+    sim.time_integration(its, args.steps)  # returns after the stream finished
+    wall = time.perf_counter() - t0
+    its += args.steps
+    # per-pass device times from HIP events on the library stream, over a
+    # second window of the same length (events only bracket launches)
+    sim.set_timing(True)
+    sim.reset_pass_times()
+    nt = max(20, min(args.steps, 200))
+    sim.time_integration(its, nt)
+    passes = sim.pass_times()
+    sim.set_timing(False)
+    state = sim.state()
+    cons = sim.conservation()
+    sim.close()
+    return dict(case=case, niac=niac, wall=wall, passes=passes, timed_steps=nt, state=state,
+                conservation=cons)
+
+
+def cpu_baseline(case, seconds):
+    """The oracle's faithful serial restatement of the reference loop
+    (half pair list, scatter; one core) on the same seeded input."""
+    from oracle import orc
+
+    P = orc.params_from(case)
+    st = orc.OracleState(case["n"], case["np"])
+    st.set(**{k: case[k] for k in ("x", "y", "vx", "vy", "rho", "u")})
+    t0 = time.perf_counter()
+    orc.run(P, st, 1, 1, gather=False)
+    one = time.perf_counter() - t0
+    k = max(1, int(seconds / max(one, 1e-3)) - 1)
+    t0 = time.perf_counter()
+    orc.run(P, st, 2, k, gather=False)
+    dt = time.perf_counter() - t0
+    return dict(value=case["n"] * k / dt, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"steps 2..{k + 1} of the same 720k dam break, oracle half-list "
+                       f"restatement of CERNsource time_integration.f, 1 thread, "
+                       f"{dt:.1f} s")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the step is not "
+              f"sharded across GPUs in this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    from armando_amd import cases
+
+    if args.scale == 1.0:
+        case_fn = lambda prec: cases.dam_break_720k(precision=prec)  # noqa: E731
+    else:
+        case_fn = lambda prec: cases.dam_break_small(scale=args.scale, precision=prec)  # noqa: E731
+
+    r32 = run_form(cases.FP32, args, case_fn)
+    r64 = None if args.no_fp64 else run_form(cases.FP64, args, case_fn)
+    case = r32["case"]
+    n = case["n"]
+    ms_step = 1e3 * r32["wall"] / args.steps
+    value = n * args.steps / r32["wall"]
+
+    def pass_ms(r, name):
+        ms, cnt = r["passes"][name]
+        return ms / max(cnt, 1)
+
+    t_force = pass_ms(r32, "force") * 1e-3
+    bytes_force = n * BYTES[("force", 1)]
+    achieved = bytes_force / t_force / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"2D dam break, {case['np']} fluid + {case['nv']} wall particles, "
+                         f"FP32 chosen form (FP64 position/energy accumulators)"),
+            "n_particles": n,
+            "pairs_i_lt_j": r32["niac"],
+            "dt": case["dt"],
+            "h": case["h"],
+            "grid": [case["nx"], case["ny"]],
+            "parallelism": "single GPU",
+        },
+        "pair_interactions_per_s": 2 * r32["niac"] / t_force,
+        "passes_ms": {k: pass_ms(r32, k) for k in r32["passes"]},
+        "roofline": {
+            "kernel": "k_force<float,...> (internal_force + art_visc + art_heat + av_vel)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_particle": BYTES[("force", 1)],
+            "avg_launch_ms": t_force * 1e3,
+        },
+    }
+    if r64 is not None:
+        v64 = n * args.steps / r64["wall"]
+        s32, s64 = r32["state"], r64["state"]
+        acc = {}
+        for k in ("rho", "p", "vx", "vy", "u", "x", "y"):
+            d = s32[k] - s64[k]
+            nrm = np.linalg.norm(s64[k])
+            acc[k] = {"rel_l2": float(np.linalg.norm(d) / nrm) if nrm > 0 else float(np.linalg.norm(d)),
+                      "max_abs": float(np.max(np.abs(d)))}
+        out["fp64"] = {
+            "value": v64,
+            "ms_per_step": 1e3 * r64["wall"] / args.steps,
+            "speedup_fp32_over_fp64": value / v64,
+            "passes_ms": {k: pass_ms(r64, k) for k in r64["passes"]},
+            "force_achieved_GBs": n * BYTES[("force", 0)] / (pass_ms(r64, "force") * 1e-3) / 1e9,
+        }
+        out["accuracy_fp32_vs_fp64"] = {
+            "after_steps": args.warmup + args.steps + r32["timed_steps"], **acc}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -132,16 +132,22 @@ int arm_conservation(arm_ctx* ctx, double out[5]);
 /* Per-pass device time accumulated since the last reset, in ms, measured
    with HIP events on the context stream; counts[] = launches per pass.
    Pass indices: see ARM_PASS_*.  n is the array length. */
-#define ARM_PASS_KEYS 0    /* corrector/keys */
-#define ARM_PASS_SORT 1    /* radix sort */
-#define ARM_PASS_REORDER 2 /* permute + predictor + cell ranges */
-#define ARM_PASS_DENSITY 3 /* density + eos */
-#define ARM_PASS_FORCE 4   /* internal_force + art_visc + art_heat + av_vel */
-#define ARM_PASS_COUNT 5
+#define ARM_PASS_KEYS 0      /* cell keys of the current positions */
+#define ARM_PASS_SORT 1      /* radix sort (all digit passes) */
+#define ARM_PASS_REORDER 2   /* permute (+ predictor) + cell ranges */
+#define ARM_PASS_DENSITY 3   /* density gather */
+#define ARM_PASS_EOS 4       /* Tait EOS + art_heat coefficient */
+#define ARM_PASS_FORCE 5     /* internal_force + art_visc + art_heat + av_vel gather */
+#define ARM_PASS_INTEGRATE 6 /* predictor / corrector (+ keys of the next step) */
+#define ARM_PASS_COUNT 7
 int arm_pass_times(arm_ctx* ctx, int n, double* ms, int* counts);
 int arm_reset_pass_times(arm_ctx* ctx);
 /* Enable (1) or disable (0) per-pass event timing. */
 int arm_set_timing(arm_ctx* ctx, int on);
+/* Select the gather implementation: 1 (default) LDS-tiled cell segments,
+   0 direct per-particle gathers from global memory.  Both visit neighbours
+   in the same order with the same arithmetic and give identical results. */
+int arm_set_gather_path(arm_ctx* ctx, int tiled);
 
 #ifdef __cplusplus
 }

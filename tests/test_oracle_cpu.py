@@ -1,0 +1,452 @@
+"""CPU tests: the oracle pinned against the reference and the mathematics.
+
+  * oracle/_ref (the reference's own CERNsource routines, compiled unmodified
+    with dim = 3; run here with z = 0 and the cubic-spline kernel) against
+    the oracle's faithful restatement of the same routines;
+  * the reference's fixtures: the neighbour lists of the 6462-particle dam
+    break (ArmandoList/out.txt) and the known-answer test output
+    (ArmandoList/old.txt, reference.txt);
+  * closed forms (Wendland normalisation and derivative, Tait, leapfrog free
+    fall) and invariants (momentum, energy, antisymmetry).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from armando_amd import cases
+from oracle import orc
+
+from . import refbind
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+need_ref = pytest.mark.skipif(not refbind.available(), reason="oracle/_ref not built here")
+
+
+def jittered_box(nx=24, ny=18, dx=0.01, seed=3, vel=0.4):
+    c = cases.box_lattice(nx=nx, ny=ny, dx=dx, jitter=0.1, seed=seed, vel=vel)
+    return c
+
+
+# ---------------------------------------------------------------- reference
+@need_ref
+def test_link_list_equals_reference_fortran():
+    c = jittered_box()
+    P = orc.params_from(c)
+    ri, rj = refbind.link_list(c["x"], c["y"], c["h"])
+    oi, oj = orc.link_list(P, c["x"], c["y"])
+    assert len(ri) > 0
+    assert np.array_equal(ri, oi) and np.array_equal(rj, oj)
+
+
+@need_ref
+def test_direct_find_equals_reference_fortran():
+    c = jittered_box(nx=12, ny=10)
+    P = orc.params_from(c)
+    ri, rj = refbind.direct_find(c["x"], c["y"], c["h"])
+    oi, oj = orc.direct_find(P, c["x"], c["y"])
+    assert np.array_equal(ri, oi) and np.array_equal(rj, oj)
+
+
+@need_ref
+@pytest.mark.parametrize("seed", [3, 11])
+def test_pair_routines_equal_reference_fortran(seed):
+    """sum_density, con_density, int_force (pa_sph 0 and 1), art_visc,
+    art_heat and av_vel of the oracle against the reference's own routines,
+    on the same pair list and inputs, with kernel.f's cubic spline."""
+    c = jittered_box(seed=seed)
+    n = c["n"]
+    rng = np.random.default_rng(seed)
+    rho = 1000.0 * (1 + 0.02 * rng.standard_normal(n))
+    p = 1e3 * rng.standard_normal(n)
+    u = 1.0 + rng.random(n)
+    cs = np.full(n, 5.0)
+    P = orc.params_from(c)
+    P.kernel = orc.K_CUBIC3D
+    P.av_alpha, P.av_beta, P.av_eta = 1.0, 1.0, 0.1   # art_visc.f:38-44
+    P.ah_g1, P.ah_g2, P.ah_eta = 0.1, 1.0, 0.1         # art_heat.f:38-40
+    P.xsph_eps = 0.3                                   # av_vel.f:35
+    pi, pj = orc.link_list(P, c["x"], c["y"])
+    ref = refbind.pair_routines(c["x"], c["y"], c["vx"], c["vy"], rho, p, u, cs, c["h"],
+                                c["mass"], pi, pj)
+    L = orc.lib()
+    x = np.ascontiguousarray(c["x"])
+    y = np.ascontiguousarray(c["y"])
+    vx = np.ascontiguousarray(c["vx"])
+    vy = np.ascontiguousarray(c["vy"])
+    ni = len(pi)
+    out = {}
+    a = np.zeros(n)
+    L.orc_sum_density(C.byref(P), n, ni, pi, pj, x, y, a)
+    out["sum_rho"] = a
+    a = np.zeros(n)
+    L.orc_con_density(C.byref(P), n, ni, pi, pj, x, y, vx, vy, a)
+    out["drho"] = a
+    for pa in (0, 1):
+        P.pa_sph = pa
+        ax, ay, du = np.zeros(n), np.zeros(n), np.zeros(n)
+        L.orc_int_force(C.byref(P), n, ni, pi, pj, x, y, vx, vy, rho, p, ax, ay, du)
+        out[f"if{pa}_ax"], out[f"if{pa}_ay"], out[f"if{pa}_du"] = ax, ay, du
+    ax, ay, du = np.zeros(n), np.zeros(n), np.zeros(n)
+    L.orc_art_visc(C.byref(P), ni, pi, pj, x, y, vx, vy, rho, cs, ax, ay, du)
+    out["av_ax"], out["av_ay"], out["av_du"] = ax, ay, du
+    du = np.zeros(n)
+    L.orc_art_heat(C.byref(P), n, ni, pi, pj, x, y, vx, vy, rho, u, cs, du, np.zeros(n))
+    out["ah_du"] = du
+    ax, ay = np.zeros(n), np.zeros(n)
+    L.orc_av_vel(C.byref(P), ni, pi, pj, x, y, vx, vy, rho, ax, ay)
+    out["xs_x"], out["xs_y"] = ax, ay
+    for k, r in ref.items():
+        scale = max(np.max(np.abs(r)), 1e-300)
+        err = np.max(np.abs(out[k] - r)) / scale
+        assert err < 1e-13, (k, err)
+        assert np.max(np.abs(r)) > 0, k
+
+
+# ---------------------------------------------------------------- fixtures
+def dam6462_positions():
+    """initDamBreak of ArmandoList/armando2Dlist.cu:1713-1831 (float positions)."""
+    dr = 0.02
+    xs, ys = [], []
+    for j in range(0, 51):
+        for i in range(0, 101):
+            xs.append(i * dr + 0.5 * dr)
+            ys.append(j * dr + 0.5 * dr)
+    for j in range(-3, 0):
+        for i in range(-3, 272):
+            xs.append(i * dr)
+            ys.append(j * dr)
+    for j in range(0, 81):
+        for i in range(-3, 0):
+            xs.append(i * dr)
+            ys.append(j * dr)
+    for j in range(0, 81):
+        for i in range(269, 272):
+            xs.append(i * dr)
+            ys.append(j * dr)
+    return np.array(xs, np.float32), np.array(ys, np.float32)
+
+
+def test_neighbour_lists_match_reference_dump():
+    """Keys, stable (cell, index) order, start/stop sets and the MAXN = 21
+    list with self padding reproduce ArmandoList/out.txt row for row."""
+    x, y = dam6462_positions()
+    n = x.size
+    assert n == 6462
+    smooth = np.float32(1.2 * 0.02)
+    size = np.float32(2.0 * np.float64(smooth))
+    nx = int((6.0 + 1.0) / np.float64(size)) + 1
+    ny = int((4.0 + 1.0) / np.float64(size)) + 1
+    sidx = np.zeros(n, np.int32)
+    lst = np.zeros(n * 21, np.int32)
+    orc.lib().orc_clist_f32(n, x, y, np.float32(-1.0), np.float32(-1.0), size, nx, ny, smooth,
+                            21, sidx, lst)
+    ref = np.load(os.path.join(GOLDEN, "damlist6462.npz"))["list"]
+    assert np.array_equal(lst.reshape(n, 21), ref)
+
+
+def recover_kat_order(analytic):
+    """Lattice coordinates (i, j) of each row of reference.txt: rows come in
+    cell order, each cell listing its lattice rows j (ascending) with the same
+    run of i; the run length follows from the repeated y = -dvy/2 values."""
+    yv = np.round(-analytic[:, 2].astype(np.float64) / 2 * 100).astype(int)
+    n = len(yv)
+    t = 0
+    oi = np.empty(n, int)
+    oj = np.empty(n, int)
+    while t < n:
+        x0 = 0
+        rows = None
+        while x0 < 100:
+            y0 = yv[t]
+            k = 0
+            while t + k < n and yv[t + k] == y0:
+                k += 1
+            ys = []
+            s = t
+            while s < n and yv[s] not in ys and (rows is None or yv[s] in rows):
+                if s + k > n or not np.all(yv[s:s + k] == yv[s]):
+                    break
+                ys.append(yv[s])
+                s += k
+                if rows is not None and len(ys) == len(rows):
+                    break
+            if rows is None:
+                rows = ys
+            for r, jy in enumerate(ys):
+                oj[t + r * k:t + (r + 1) * k] = jy
+                oi[t + r * k:t + (r + 1) * k] = np.arange(x0, x0 + k)
+            t += k * len(ys)
+            x0 += k
+    return oi, oj
+
+
+def test_known_answer_test_matches_reference_output():
+    """check() of ArmandoList/armando2Dlist.cu:2451-2548: continuity, pressure
+    force, density diffusion and the 0.03 artificial viscosity over the MAXN
+    list, on the 100 x 100 lattice, against the reference GPU output old.txt."""
+    fx = np.load(os.path.join(GOLDEN, "kat_check.npz"))
+    gpu, ana = fx["gpu"], fx["analytic"]
+    ii, jj = recover_kat_order(ana)
+    assert len(set(zip(ii.tolist(), jj.tolist()))) == 10000
+    xs = (ii * 0.01).astype(np.float32)
+    ys = (jj * 0.01).astype(np.float32)
+    x64, y64 = xs.astype(np.float64), ys.astype(np.float64)
+    # the recovered order reproduces the analytic columns
+    assert np.array_equal((-2.0 * y64).astype(np.float32), ana[:, 2])
+    rdr = (-1000.0 * (10 * np.pi * np.cos(10 * np.pi * x64) + 2.0 * y64)).astype(np.float32)
+    assert np.max(np.abs(rdr - ana[:, 0])) < 1e-6
+    n = 10000
+    smooth = np.float32(1.2 * 0.01)
+    size = np.float32(2.0 * np.float64(smooth))
+    ox = np.float32(-0.5)
+    nxg = int(2.0 / np.float64(size)) + 1
+    sidx = np.zeros(n, np.int32)
+    lst = np.zeros(n * 21, np.int32)
+    L = orc.lib()
+    L.orc_clist_f32(n, xs, ys, ox, ox, size, nxg, nxg, smooth, 21, sidx, lst)
+    inv = np.empty(n, np.int32)
+    inv[sidx] = np.arange(n)
+    lst_rows = np.ascontiguousarray(sidx[lst.reshape(n, 21)[inv]], np.int32)
+    p = (1000.0 * (np.sin(10 * np.pi * x64) + (ys * ys).astype(np.float64))).astype(np.float32)
+    vx = np.sin(10 * np.pi * x64).astype(np.float32)
+    vy = (ys * ys).astype(np.float32)
+    rho = np.full(n, 1000.0, np.float32)
+    drho, ax, ay = (np.zeros(n, np.float32) for _ in range(3))
+    L.orc_kat_interaction_f32(n, 21, lst_rows, xs, ys, vx, vy, rho, p, smooth,
+                              np.float32(1000.0 * 0.01 * 0.01), np.float32(50.0), drho, ax, ay)
+    edge = (xs < size) | (ys < size) | (xs > 1.0 - size) | (ys > 1.0 - size)
+    mine = np.stack([drho, ax, ay], 1)
+    mine[edge] = ana[edge]
+    scale = np.abs(ana).max(axis=0)
+    err = np.abs(mine - gpu).max(axis=0) / scale
+    assert np.all(err < 1e-4), err
+    # the error metric printed by check() (percent), as the reference reports it
+    l1 = 100 * np.abs(mine - ana).mean(axis=0) / scale
+    l1_ref = 100 * np.abs(gpu - ana).mean(axis=0) / scale
+    assert np.allclose(l1, l1_ref, rtol=1e-3), (l1, l1_ref)
+    assert np.allclose(l1_ref, [2.868, 8.043, 1.730], atol=1e-3)
+
+
+# ---------------------------------------------------------------- closed forms
+def wendland_params(h=1.3e-3):
+    return orc.params(kernel=orc.K_WENDLAND2D, h=h, mass=1.0)
+
+
+def test_wendland_normalisation_and_support():
+    P = wendland_params()
+    h = P.h
+    F = C.c_double(0)
+    L = orc.lib()
+    r = np.linspace(0, 2 * h, 20001)
+    w = np.array([L.orc_kernel(C.byref(P), float(q), C.byref(F)) for q in r])
+    integral = np.trapezoid(2 * np.pi * r * w, r)
+    assert abs(integral - 1.0) < 1e-7
+    assert abs(w[0] - 7.0 / (4.0 * np.pi * h * h)) < 1e-9 * w[0]
+    assert L.orc_kernel(C.byref(P), 2 * h, C.byref(F)) == 0.0 and F.value == 0.0
+    assert L.orc_kernel(C.byref(P), 2.5 * h, C.byref(F)) == 0.0
+
+
+def test_wendland_gradient_is_derivative():
+    """F r = dW/dr (central differences) and dW/dr = -5 q (1-q/2)^3 a / h."""
+    P = wendland_params()
+    h = P.h
+    L = orc.lib()
+    a = 7.0 / (4.0 * np.pi * h * h)
+    F = C.c_double(0)
+    for q in (0.1, 0.5, 1.0, 1.5, 1.9):
+        r = q * h
+        eps = 1e-7 * h
+        wp = L.orc_kernel(C.byref(P), r + eps, C.byref(F))
+        wm = L.orc_kernel(C.byref(P), r - eps, C.byref(F))
+        L.orc_kernel(C.byref(P), r, C.byref(F))
+        d = F.value * r
+        assert abs(d - (wp - wm) / (2 * eps)) < 1e-6 * abs(d)
+        assert abs(d - (-5.0 * q * (1 - q / 2) ** 3 * a / h)) < 1e-12 * abs(d)
+
+
+def test_tait_eos():
+    P = orc.params(rho0=1000.0, c0=34.31, pmin=0.0)
+    L = orc.lib()
+    rho = np.array([1000.0, 1000.0 * (1 + 1e-6), 1010.0, 990.0])
+    p = np.zeros(4)
+    c = np.zeros(4)
+    r = rho.copy()
+    L.orc_eos_tait(C.byref(P), 4, r, p, c)
+    assert p[0] == 0.0
+    assert abs(p[1] / (rho[1] - 1000.0) - 34.31 ** 2) < 1e-3 * 34.31 ** 2   # dp/drho = c0^2
+    B = 1000.0 * 34.31 ** 2 / 7.0
+    assert abs(p[2] - B * (1.01 ** 7 - 1)) < 1e-9 * p[2]
+    assert p[3] == 0.0 and r[3] == 1000.0     # p < pmin: p = 0, rho = rho0 (eos.f90:278-281)
+    assert np.all(c == 34.31)
+
+
+# ---------------------------------------------------------------- neighbour sets
+def test_pair_lists_agree_and_match_brute_force():
+    c = jittered_box(nx=30, ny=22)
+    P = orc.params_from(c)
+    li, lj = orc.link_list(P, c["x"], c["y"])
+    di, dj = orc.direct_find(P, c["x"], c["y"])
+    gi, gj = orc.pairs_f64(P, c["x"], c["y"])
+    s1 = set(zip(li.tolist(), lj.tolist()))
+    assert s1 == set(zip(di.tolist(), dj.tolist())) == set(zip(gi.tolist(), gj.tolist()))
+    dx = c["x"][:, None] - c["x"][None, :]
+    dy = c["y"][:, None] - c["y"][None, :]
+    r2 = dx * dx + dy * dy
+    bi, bj = np.nonzero(np.triu(r2 < (2 * c["h"]) ** 2, 1))
+    assert s1 == set(zip(bi.tolist(), bj.tolist()))
+
+
+def test_fp32_criterion_matches_float32_restatement():
+    """orc_pairs_f32 against a numpy float32 evaluation of the same formula."""
+    c = cases.dam_break_small(scale=0.02)
+    P = orc.params_from(c)
+    cx, cy, ox, oy = orc.split_f32(P, c["x"], c["y"])
+    # the split reproduces the FP64 cell and rounds the offset once
+    kx = np.floor((c["x"] - P.ox) / P.cell).astype(np.int32)
+    assert np.array_equal(cx, kx)
+    assert np.array_equal(ox, (c["x"] - (P.ox + cx * P.cell)).astype(np.float32))
+    gi, gj = orc.pairs_f32(P, cx, cy, ox, oy)
+    cellf = np.float32(P.cell)
+    cut2f = np.float32((2.0 * P.h) * (2.0 * P.h))
+    n = c["n"]
+    i, j = np.triu_indices(n, 1)
+    near = (np.abs(cx[i] - cx[j]) <= 1) & (np.abs(cy[i] - cy[j]) <= 1)
+    i, j = i[near], j[near]
+    dxf = (ox[i] - ox[j]) + (cx[i] - cx[j]).astype(np.float32) * cellf
+    dyf = (oy[i] - oy[j]) + (cy[i] - cy[j]).astype(np.float32) * cellf
+    r2 = dxf * dxf + dyf * dyf
+    assert r2.dtype == np.float32
+    sel = r2 < cut2f
+    assert set(zip(gi.tolist(), gj.tolist())) == set(zip(i[sel].tolist(), j[sel].tolist()))
+
+
+def test_keys_and_sort_by_definition():
+    c = jittered_box(nx=40, ny=31, seed=5)
+    P = orc.params_from(c)
+    key, bad = orc.keys(P, c["x"], c["y"])
+    assert bad == 0
+    ix = np.floor((c["x"] - P.ox) / P.cell).astype(int)
+    iy = np.floor((c["y"] - P.oy) / P.cell).astype(int)
+    assert np.array_equal(key, ix + iy * P.nx)
+    ncell = P.nx * P.ny
+    sid, start = orc.sort_cells(key, ncell)
+    assert np.array_equal(sid, np.lexsort((np.arange(c["n"]), key)))
+    assert np.array_equal(start, np.searchsorted(key[sid], np.arange(ncell + 1), "left"))
+
+
+# ---------------------------------------------------------------- single step
+def test_gather_step_equals_faithful_half_list_step():
+    c = cases.dam_break_small(scale=0.04)
+    P = orc.params_from(c)
+    a = orc.OracleState(c["n"], c["np"])
+    b = orc.OracleState(c["n"], c["np"])
+    for s in (a, b):
+        s.set(**{k: c[k] for k in ("x", "y", "vx", "vy", "rho", "u")})
+    orc.run(P, a, 1, 5, gather=False)
+    orc.run(P, b, 1, 5, gather=True)
+    for k in ("x", "y", "vx", "vy", "rho", "u", "p", "drho", "du", "ax", "ay", "avx", "avy"):
+        ra, rb = a.field(k), b.field(k)
+        scale = max(np.max(np.abs(ra)), 1e-300)
+        assert np.max(np.abs(ra - rb)) / scale < 1e-11, k
+
+
+def single_step_rates(c, flags, gravity=False):
+    P = orc.params_from(c)
+    P.flags = flags | (orc.GRAVITY if gravity else 0)
+    st = orc.OracleState(c["n"], c["np"])
+    st.set(**{k: c[k] for k in ("x", "y", "vx", "vy", "rho", "u")})
+    orc.single_step_gather(P, st)
+    return P, st
+
+
+def test_momentum_and_energy_budget_of_pressure_and_viscosity():
+    """Pressure and artificial viscosity are pairwise antisymmetric: total
+    momentum rate is zero and the kinetic + internal energy rate is zero."""
+    c = jittered_box(nx=30, ny=26, vel=0.5)
+    P, st = single_step_rates(c, orc.ART_VISC)
+    m = P.mass
+    ax, ay, du = st.field("ax"), st.field("ay"), st.field("du")
+    vx, vy = st.field("vx"), st.field("vy")
+    scale = m * np.sum(np.abs(ax) + np.abs(ay))
+    assert abs(m * ax.sum()) < 1e-12 * scale and abs(m * ay.sum()) < 1e-12 * scale
+    de = m * np.sum(vx * ax + vy * ay) + m * np.sum(du)
+    assert abs(de) < 1e-11 * m * np.sum(np.abs(vx * ax) + np.abs(vy * ay))
+
+
+def test_artificial_heat_and_xsph_conserve():
+    c = jittered_box(nx=26, ny=20, vel=0.3)
+    c["u"] = 1.0 + np.sin(40 * c["x"]) * np.cos(30 * c["y"])
+    P, st = single_step_rates(c, orc.ART_HEAT | orc.AV_VEL)
+    P2, st2 = single_step_rates(c, orc.AV_VEL)
+    du_heat = st.field("du") - st2.field("du")
+    assert np.max(np.abs(du_heat)) > 0
+    assert abs(np.sum(du_heat)) < 1e-11 * np.sum(np.abs(du_heat))
+    avx = st.field("avx")
+    assert abs(np.sum(avx)) < 1e-12 * np.sum(np.abs(avx))
+    # uniform velocity and energy: no XSPH correction, no heat flux
+    c2 = dict(c)
+    c2["vx"] = np.full(c["n"], 0.7)
+    c2["vy"] = np.full(c["n"], -0.2)
+    c2["u"] = np.full(c["n"], 2.0)
+    P3, st3 = single_step_rates(c2, orc.ART_HEAT | orc.AV_VEL)
+    assert np.max(np.abs(st3.field("avx"))) < 1e-15
+    assert np.all(st3.field("drho") == 0.0)
+
+
+def test_leapfrog_free_fall_closed_form():
+    """Isolated particles under gravity: the leapfrog of time_integration.f
+    gives v_n = (n - 1/2) dt g and y_n = y_0 + g (n dt)^2 / 2."""
+    n = 5
+    x = np.arange(n) * 1.0
+    y = np.full(n, 10.0)
+    c = cases.box_lattice(nx=2, ny=2)
+    c.update(n=n, np=n, x=x, y=y, vx=np.zeros(n), vy=np.zeros(n),
+             rho=np.full(n, 1000.0), u=np.zeros(n))
+    c["ox"], c["oy"] = -2.0, -20.0
+    c["cell"] = 0.1
+    c["nx"], c["ny"] = 80, 400
+    P = orc.params_from(c)
+    P.flags = orc.GRAVITY
+    P.gy = -9.81
+    P.pmin = -1e30
+    st = orc.OracleState(n, n)
+    st.set(x=x, y=y, vx=np.zeros(n), vy=np.zeros(n), rho=np.full(n, 1000.0), u=np.zeros(n))
+    steps = 50
+    orc.run(P, st, 1, steps, gather=True)
+    t = steps * P.dt
+    assert np.allclose(st.field("y"), 10.0 - 0.5 * 9.81 * t * t, rtol=0, atol=1e-12)
+    assert np.allclose(st.field("vy"), -(steps - 0.5) * P.dt * 9.81, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- C ABI
+def test_library_exports_every_declared_symbol():
+    import re
+
+    import armando_amd
+
+    hdr = open(os.path.join(os.path.dirname(HERE), "include", "armando.h")).read()
+    names = set(re.findall(r"\b(arm_[a-z_]+)\s*\(", hdr))
+    assert names == set(armando_amd.EXPORTS), names ^ set(armando_amd.EXPORTS)
+    lib = C.CDLL(armando_amd.LIB_PATH)
+    for nm in names:
+        assert hasattr(lib, nm), nm
+
+
+def test_create_rejects_bad_parameters_without_a_device():
+    import armando_amd
+
+    L = armando_amd.load_library()
+    c = cases.box_lattice()
+    for field, val, code in (("precision", 7, -1), ("cell", c["h"], -1), ("pa_sph", 3, -5),
+                             ("density_method", 2, -5), ("h", -1.0, -1)):
+        P = armando_amd.params_from(c)
+        setattr(P, field, val)
+        h = C.c_void_p()
+        rc = L.arm_create(C.byref(P), 100, C.byref(h))
+        assert rc == code, (field, rc)
+        assert L.arm_last_error(h).decode()
+        L.arm_destroy(h)
