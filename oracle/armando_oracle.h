@@ -169,6 +169,15 @@ void orc_conservation(const orc_params* P, const orc_state* s, double out[5]);
 int orc_num_threads(void);
 void orc_set_threads(int nthreads);
 
+/* ---- discrete adjoint (armando_adjoint.c) ------------------------------ */
+/* nsteps of time_integration from s0 (its = 1 .. nsteps; s0 is not
+   modified), objective J = sum_steps dt sum_{objmask[i]} p_i^2, and its
+   gradients with respect to av_alpha and the initial velocities (the
+   neighbour list of every step is held fixed).  Returns 0, -5 for
+   summation density, or a link_list error. */
+int orc_adjoint(const orc_params* P, const orc_state* s0, int nsteps, const int* objmask,
+                double* J, double* dJ_dalpha, double* dJ_dvx0, double* dJ_dvy0);
+
 /* ---- FP32 cell + offset form ----------------------------------------- */
 /* Split FP64 positions into integer cell coordinates and FP32 in-cell
    offsets: ix = (int)((x-ox)/cell), off = (float)(x - (ox + ix*cell)). */

@@ -81,6 +81,8 @@ def lib():
         "orc_corrector": (None, [_PP, C.POINTER(State), C.c_int]),
         "orc_run": (C.c_int, [_PP, C.POINTER(State), C.c_int, C.c_int, C.c_int]),
         "orc_conservation": (None, [_PP, C.POINTER(State), _dp]),
+        "orc_adjoint": (C.c_int, [_PP, C.POINTER(State), C.c_int, _ip, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), _dp, _dp]),
         "orc_num_threads": (C.c_int, []),
         "orc_set_threads": (None, [C.c_int]),
         "orc_split_f32": (None, [_PP, C.c_int, _dp, _dp, _ip, _ip, _fp, _fp]),
@@ -239,3 +241,17 @@ def conservation(P, st):
     out = np.zeros(5, np.float64)
     lib().orc_conservation(C.byref(P), st.ptr, out)
     return out
+
+
+def adjoint(P, st, nsteps, objmask):
+    """J and dJ/d(alpha, vx0, vy0) by the oracle's discrete adjoint."""
+    n = st.n
+    J = C.c_double(0)
+    da = C.c_double(0)
+    gx = np.zeros(n)
+    gy = np.zeros(n)
+    rc = lib().orc_adjoint(C.byref(P), st.ptr, nsteps, np.ascontiguousarray(objmask, np.int32),
+                           C.byref(J), C.byref(da), gx, gy)
+    if rc:
+        raise RuntimeError(f"orc_adjoint rc={rc}")
+    return J.value, da.value, gx, gy
