@@ -138,10 +138,17 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
   const int s = (int)sval[t];
   const int id = A.id[s];
   Bs.id[t] = id;
-  Bs.xy[t] = A.xy[s];
+  const double2 q = A.xy[s];
+  Bs.xy[t] = q;
   if constexpr (sizeof(R) == 4) {
-    Bs.cxy[t] = A.cxy[s];
-    Bs.off[t] = A.off[s];
+    // cell + offset recomputed from the FP64 accumulator (as k_keys does)
+    int ix, iy;
+    bool ok;
+    cell_of(K, q.x, q.y, ix, iy, ok);
+    const double bx = K.ox + (double)ix * K.cell;
+    const double by = K.oy + (double)iy * K.cell;
+    Bs.cxy[t] = (uint32_t)ix | ((uint32_t)iy << 16);
+    Bs.off[t] = make_float2((float)(q.x - bx), (float)(q.y - by));
   }
   R2 v = A.v[s];
   R rho = A.rho[s];
@@ -169,7 +176,8 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
   Bs.rho[t] = rho;
   Bs.u[t] = u;
   if constexpr (sizeof(R) == 4) Bs.uf[t] = (float)u;
-  Bs.p[t] = A.p[s];
+  // the EOS recomputes p of every rate particle; others keep theirs
+  if (!is_rate(K, id)) Bs.p[t] = A.p[s];
 }
 
 // ------------------------------------------------------------------------
@@ -493,7 +501,7 @@ __global__ __launch_bounds__(BLOCK) void k_predictor(St<R> S, KParams K) {
 }
 
 // Corrector; with keys != nullptr it also hashes the updated position for
-// the next step's sort (and, FP32 form, refreshes cell + offset).
+// the next step's sort (the reorder then rebuilds the FP32 cell + offset).
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_corrector(St<R> S, KParams K, int its,
                                                      uint32_t* __restrict__ key_by_id,
@@ -541,12 +549,6 @@ __global__ __launch_bounds__(BLOCK) void k_corrector(St<R> S, KParams K, int its
     if (!ok) atomicAdd(err, 1);
     key_by_id[id] = (uint32_t)key;
     slot_by_id[id] = (uint32_t)i;
-    if constexpr (sizeof(R) == 4) {
-      const double bx = K.ox + (double)ix * K.cell;
-      const double by = K.oy + (double)iy * K.cell;
-      S.cxy[i] = (uint32_t)ix | ((uint32_t)iy << 16);
-      S.off[i] = make_float2((float)(q.x - bx), (float)(q.y - by));
-    }
   }
 }
 
