@@ -31,7 +31,7 @@ EXPORTS = ("arm_create", "arm_destroy", "arm_last_error", "arm_domain_violations
            "arm_get_cells", "arm_density", "arm_eos", "arm_internal_force", "arm_av_vel",
            "arm_single_step", "arm_predictor", "arm_corrector", "arm_time_integration",
            "arm_conservation", "arm_pass_times", "arm_reset_pass_times", "arm_set_timing",
-           "arm_set_gather_path")
+           "arm_set_gather_path", "arm_adjoint_rollout")
 
 
 class ArmParams(C.Structure):
@@ -92,6 +92,8 @@ def load_library(path=LIB_PATH):
         "arm_reset_pass_times": (C.c_int, [vp]),
         "arm_set_timing": (C.c_int, [vp, C.c_int]),
         "arm_set_gather_path": (C.c_int, [vp, C.c_int]),
+        "arm_adjoint_rollout": (C.c_int, [vp, C.c_int, C.c_int, _ip, C.c_int, dpn, dpn, dpn,
+                                          dpn, dpn]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -221,6 +223,19 @@ class Simulation:
         out = np.zeros(5)
         self._check(self.L.arm_conservation(self.ctx, out))
         return out
+
+    def adjoint_rollout(self, nsteps, ckpt_every, obj_ids):
+        """J, dJ/dalpha, dJ/dv0 (n, 2) and (forward, backward) device ms."""
+        ids = np.ascontiguousarray(obj_ids, np.int32)
+        J = C.c_double(0)
+        da = C.c_double(0)
+        g = np.zeros((self.n, 2))
+        tf = C.c_double(0)
+        tb = C.c_double(0)
+        self._check(self.L.arm_adjoint_rollout(self.ctx, nsteps, ckpt_every, ids, len(ids),
+                                               C.byref(J), C.byref(da), _ptr(g), C.byref(tf),
+                                               C.byref(tb)))
+        return J.value, da.value, g, (tf.value, tb.value)
 
     def set_gather_path(self, tiled=True):
         self._check(self.L.arm_set_gather_path(self.ctx, 1 if tiled else 0))

@@ -19,6 +19,7 @@
 #include "radix_sort.h"
 #include "sph_kernels.h"
 #include "sph_tiled.h"
+#include "sph_adjoint.h"
 
 using namespace arm;
 
@@ -130,6 +131,7 @@ template <typename R> int alloc_state(arm_ctx* c, St<R>& S, int cap) {
   AL(rho_min, cap);
   AL(u_min, cap);
   AL(v_min, cap);
+  AL(flag, cap);
 #undef AL
   return ARM_OK;
 }
@@ -367,41 +369,280 @@ template <typename R> int corrector_t(arm_ctx* c, int its, bool keys) {
 // time_integration.f:131-235, fused: per step
 //   [keys of the current positions] -> sort -> reorder + predictor
 //   -> density -> eos -> forces (+ XSPH) -> corrector + keys of the next step
-template <typename R> int time_integration_t(arm_ctx* c, int its0, int nsteps) {
+// Keys of the current positions, the start of every fused run of steps.
+template <typename R> int begin_steps(arm_ctx* c) {
+  St<R>& A = stA<R>(c);
+  PassTimer tm(c, ARM_PASS_KEYS);
+  hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                     c->key_by_id, c->slot_by_id, c->d_err);
+  HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+// One step of time_integration.f:131-235, fused:
+//   sort -> reorder + predictor -> density -> eos [-> objective] -> forces
+//   [-> tape] -> corrector + keys of the next step
+struct StepHooks {
+  const uint8_t* objmask = nullptr;  // objective: jpart += dt sum p^2
+  double* jpart = nullptr;
+  const Tape* tape = nullptr;        // record the step after the force pass
+};
+
+template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
   const int n = c->n;
   const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
   int rc;
+  const uint32_t* sval = nullptr;
+  {
+    PassTimer tm(c, ARM_PASS_SORT);
+    if ((rc = run_sort(c, &sval))) return rc;
+  }
   {
     St<R>& A = stA<R>(c);
-    PassTimer tm(c, ARM_PASS_KEYS);
-    hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
-                       c->key_by_id, c->slot_by_id, c->d_err);
+    St<R>& B = stB<R>(c);
+    PassTimer tm(c, ARM_PASS_REORDER);
+    hipLaunchKernelGGL(k_reorder_pred<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
+                       c->K, c->skey, sval, c->cstart, its != 1 ? 1 : 0);
+    std::swap(A, B);
+    c->masks_valid = false;
   }
   if (!xs && (rc = zero_av<R>(c))) return rc;
-  for (int its = its0; its < its0 + nsteps; ++its) {
-    const uint32_t* sval = nullptr;
-    {
-      PassTimer tm(c, ARM_PASS_SORT);
-      if ((rc = run_sort(c, &sval))) return rc;
-    }
-    {
-      St<R>& A = stA<R>(c);
-      St<R>& B = stB<R>(c);
-      PassTimer tm(c, ARM_PASS_REORDER);
-      hipLaunchKernelGGL(k_reorder_pred<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
-                         c->K, c->skey, sval, c->cstart, its != 1 ? 1 : 0);
-      std::swap(A, B);
-      c->masks_valid = false;
-    }
-    if (!xs && (rc = zero_av<R>(c))) return rc;
-    c->sorted = true;
-    if ((rc = density_t<R>(c, c->prm.density_method))) return rc;
-    if ((rc = eos_t<R>(c))) return rc;
-    if ((rc = force_t<R>(c, true, c->prm.flags, xs))) return rc;
-    if ((rc = corrector_t<R>(c, its, true))) return rc;
+  c->sorted = true;
+  if ((rc = density_t<R>(c, c->prm.density_method))) return rc;
+  if ((rc = eos_t<R>(c))) return rc;
+  if constexpr (sizeof(R) == 8) {
+    if (hk.jpart)
+      hipLaunchKernelGGL(k_objective, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, stA<R>(c),
+                         c->K, hk.objmask, hk.jpart);
   }
+  if ((rc = force_t<R>(c, true, c->prm.flags, xs))) return rc;
+  if constexpr (sizeof(R) == 8) {
+    if (hk.tape) {
+      const int nb = std::max(grid_of(n), grid_of(c->K.ncell + 1));
+      hipLaunchKernelGGL(k_tape, dim3(nb), dim3(BLOCK), 0, c->stream, stA<R>(c), c->K, c->skey,
+                         c->cstart, *hk.tape);
+    }
+  }
+  if ((rc = corrector_t<R>(c, its, true))) return rc;
+  return ARM_OK;
+}
+
+template <typename R> int time_integration_t(arm_ctx* c, int its0, int nsteps) {
+  int rc;
+  if ((rc = begin_steps<R>(c))) return rc;
+  if (!(c->prm.flags & ARM_AV_VEL) && (rc = zero_av<R>(c))) return rc;
+  const StepHooks none;
+  for (int its = its0; its < its0 + nsteps; ++its)
+    if ((rc = fused_step<R>(c, its, none))) return rc;
   c->have_rates = true;
   HIPCHK(c, hipGetLastError());
+  return check_domain(c);
+}
+
+// ---- adjoint rollout (FP64) ---------------------------------------------------
+// Device buffers owned by one call.
+struct Scratch {
+  arm_ctx* c;
+  std::vector<void*> ptrs;
+  explicit Scratch(arm_ctx* c_) : c(c_) {}
+  ~Scratch() {
+    for (void* q : ptrs) (void)hipFree(q);
+  }
+  template <typename T> int get(T** p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess)
+      return fail(c, ARM_ERR_HIP, "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
+    ptrs.push_back(q);
+    *p = static_cast<T*>(q);
+    return ARM_OK;
+  }
+};
+
+// the part of the state a step reads at its start
+struct Checkpoint {
+  int32_t* id;
+  double2 *xy, *v, *a;
+  double *rho, *u, *p, *du, *drho;
+};
+
+int alloc_checkpoint(Scratch& sc, Checkpoint& k, int n) {
+  int rc;
+  if ((rc = sc.get(&k.id, n)) || (rc = sc.get(&k.xy, n)) || (rc = sc.get(&k.v, n)) ||
+      (rc = sc.get(&k.a, n)) || (rc = sc.get(&k.rho, n)) || (rc = sc.get(&k.u, n)) ||
+      (rc = sc.get(&k.p, n)) || (rc = sc.get(&k.du, n)) || (rc = sc.get(&k.drho, n)))
+    return rc;
+  return ARM_OK;
+}
+
+int copy_checkpoint(arm_ctx* c, const St<double>& S, const Checkpoint& k, bool save) {
+  const size_t n = (size_t)c->n;
+  auto cp = [&](void* st, void* ck, size_t bytes) {
+    return save ? hipMemcpyAsync(ck, st, bytes, hipMemcpyDeviceToDevice, c->stream)
+                : hipMemcpyAsync(st, ck, bytes, hipMemcpyDeviceToDevice, c->stream);
+  };
+  HIPCHK(c, cp(S.id, k.id, n * sizeof(int32_t)));
+  HIPCHK(c, cp(S.xy, k.xy, n * sizeof(double2)));
+  HIPCHK(c, cp(S.v, k.v, n * sizeof(double2)));
+  HIPCHK(c, cp(S.a, k.a, n * sizeof(double2)));
+  HIPCHK(c, cp(S.rho, k.rho, n * sizeof(double)));
+  HIPCHK(c, cp(S.u, k.u, n * sizeof(double)));
+  HIPCHK(c, cp(S.p, k.p, n * sizeof(double)));
+  HIPCHK(c, cp(S.du, k.du, n * sizeof(double)));
+  HIPCHK(c, cp(S.drho, k.drho, n * sizeof(double)));
+  return ARM_OK;
+}
+
+int alloc_tape(Scratch& sc, Tape& t, int n, int ncell) {
+  int rc;
+  if ((rc = sc.get(&t.id, n)) || (rc = sc.get(&t.xy, n)) || (rc = sc.get(&t.v, n)) ||
+      (rc = sc.get(&t.rho, n)) || (rc = sc.get(&t.u, n)) || (rc = sc.get(&t.psi, n)) ||
+      (rc = sc.get(&t.divv, n)) || (rc = sc.get(&t.p, n)) || (rc = sc.get(&t.drho, n)) ||
+      (rc = sc.get(&t.du, n)) || (rc = sc.get(&t.a, n)) || (rc = sc.get(&t.av, n)) ||
+      (rc = sc.get(&t.rho_min, n)) || (rc = sc.get(&t.u_min, n)) || (rc = sc.get(&t.v_min, n)) ||
+      (rc = sc.get(&t.flag, n)) || (rc = sc.get(&t.skey, n)) || (rc = sc.get(&t.cstart, ncell + 1)))
+    return rc;
+  return ARM_OK;
+}
+
+template <bool AV, bool AH, bool XS>
+void launch_adj_force(arm_ctx* c, const Tape& T, const AdjScratch& W, double* apart) {
+  hipLaunchKernelGGL((k_adj_force<AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                     T, c->K, W, apart);
+}
+
+void adj_force(arm_ctx* c, const Tape& T, const AdjScratch& W, double* apart) {
+  const int f = c->prm.flags;
+  const int code = ((f & ARM_ART_VISC) ? 4 : 0) | ((f & ARM_ART_HEAT) ? 2 : 0) | ((f & ARM_AV_VEL) ? 1 : 0);
+  switch (code) {
+    case 0: launch_adj_force<false, false, false>(c, T, W, apart); break;
+    case 1: launch_adj_force<false, false, true>(c, T, W, apart); break;
+    case 2: launch_adj_force<false, true, false>(c, T, W, apart); break;
+    case 3: launch_adj_force<false, true, true>(c, T, W, apart); break;
+    case 4: launch_adj_force<true, false, false>(c, T, W, apart); break;
+    case 5: launch_adj_force<true, false, true>(c, T, W, apart); break;
+    case 6: launch_adj_force<true, true, false>(c, T, W, apart); break;
+    default: launch_adj_force<true, true, true>(c, T, W, apart); break;
+  }
+}
+
+int adjoint_rollout(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, int nobj,
+                    double* J, double* dJ_dalpha, double* dJ_dv0, double* ms_fwd, double* ms_bwd) {
+  const int n = c->n, ncell = c->K.ncell;
+  const int nb = grid_of(n);
+  Scratch sc(c);
+  int rc;
+  uint8_t* obj = nullptr;
+  double *jpart = nullptr, *apart = nullptr;
+  if ((rc = sc.get(&obj, n)) || (rc = sc.get(&jpart, nb)) || (rc = sc.get(&apart, nb))) return rc;
+  {
+    std::vector<uint8_t> h(n, 0);
+    for (int k = 0; k < nobj; ++k) {
+      if (obj_ids[k] < 0 || obj_ids[k] >= n) return fail(c, ARM_ERR_ARG, "objective id %d out of range", obj_ids[k]);
+      h[obj_ids[k]] = 1;
+    }
+    HIPCHK(c, hipMemcpyAsync(obj, h.data(), n, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipMemsetAsync(jpart, 0, sizeof(double) * nb, c->stream));
+  HIPCHK(c, hipMemsetAsync(apart, 0, sizeof(double) * nb, c->stream));
+  const int nck = (nsteps + ckpt - 1) / ckpt;
+  std::vector<Checkpoint> ck(nck + 1);
+  for (auto& k : ck)
+    if ((rc = alloc_checkpoint(sc, k, n))) return rc;
+  std::vector<Tape> tape(ckpt);
+  for (auto& t : tape)
+    if ((rc = alloc_tape(sc, t, n, ncell))) return rc;
+  AdjState L{};
+  double** lf[] = {&L.x,  &L.y,  &L.vx,  &L.vy,  &L.rho,  &L.u,   &L.bx,    &L.by,
+                   &L.bvx, &L.bvy, &L.brho, &L.bu, &L.cax, &L.cay, &L.cdrho, &L.cdu};
+  for (double** q : lf) {
+    if ((rc = sc.get(q, n))) return rc;
+    HIPCHK(c, hipMemsetAsync(*q, 0, sizeof(double) * n, c->stream));
+  }
+  AdjScratch W{};
+  double** wf[] = {&W.lax, &W.lay, &W.ldrho, &W.ldu, &W.lavx, &W.lavy, &W.lp, &W.lQ,  &W.lA,
+                   &W.lpsi, &W.lS, &W.lrhoh, &W.lu, &W.lvx, &W.lvy, &W.lx, &W.ly};
+  for (double** q : wf)
+    if ((rc = sc.get(q, n))) return rc;
+
+  hipEvent_t e0, e1, e2;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, hipEventCreate(&e2));
+  // forward with checkpoints and the objective
+  HIPCHK(c, hipEventRecord(e0, c->stream));
+  if ((rc = begin_steps<double>(c))) return rc;
+  if (!(c->prm.flags & ARM_AV_VEL) && (rc = zero_av<double>(c))) return rc;
+  StepHooks fwd;
+  fwd.objmask = obj;
+  fwd.jpart = jpart;
+  for (int its = 1; its <= nsteps; ++its) {
+    if ((its - 1) % ckpt == 0)
+      if ((rc = copy_checkpoint(c, c->A64, ck[(its - 1) / ckpt], true))) return rc;
+    if ((rc = fused_step<double>(c, its, fwd))) return rc;
+  }
+  if ((rc = copy_checkpoint(c, c->A64, ck[nck], true))) return rc;  // final state
+  HIPCHK(c, hipEventRecord(e1, c->stream));
+  // backward, one checkpoint segment at a time
+  for (int seg = nck - 1; seg >= 0; --seg) {
+    const int s0 = 1 + seg * ckpt, s1 = std::min(s0 + ckpt, nsteps + 1);
+    if ((rc = copy_checkpoint(c, c->A64, ck[seg], false))) return rc;
+    if ((rc = begin_steps<double>(c))) return rc;
+    for (int its = s0; its < s1; ++its) {
+      StepHooks rec;
+      rec.tape = &tape[its - s0];
+      if ((rc = fused_step<double>(c, its, rec))) return rc;
+    }
+    for (int its = s1 - 1; its >= s0; --its) {
+      const Tape& T = tape[its - s0];
+      hipLaunchKernelGGL(k_adj_corrector, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its, L, W,
+                         obj);
+      adj_force(c, T, W, apart);
+      hipLaunchKernelGGL(k_adj_eos, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
+      hipLaunchKernelGGL(k_adj_density, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
+      hipLaunchKernelGGL(k_adj_predictor, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its, L, W);
+      HIPCHK(c, hipGetLastError());
+      // the state adjoint before this step is the one after the previous step
+      std::swap(L.x, L.bx);
+      std::swap(L.y, L.by);
+      std::swap(L.vx, L.bvx);
+      std::swap(L.vy, L.bvy);
+      std::swap(L.rho, L.brho);
+      std::swap(L.u, L.bu);
+    }
+  }
+  HIPCHK(c, hipEventRecord(e2, c->stream));
+  // leave the context at the final forward state
+  if ((rc = copy_checkpoint(c, c->A64, ck[nck], false))) return rc;
+  std::vector<double> jp(nb), ap(nb), gx(n), gy(n);
+  HIPCHK(c, hipMemcpyAsync(jp.data(), jpart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ap.data(), apart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gx.data(), L.vx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gy.data(), L.vy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float f01 = 0.f, f12 = 0.f;
+  (void)hipEventElapsedTime(&f01, e0, e1);
+  (void)hipEventElapsedTime(&f12, e1, e2);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  double Js = 0, As = 0;
+  for (int b = 0; b < nb; ++b) {
+    Js += jp[b];
+    As += ap[b];
+  }
+  if (J) *J = Js;
+  if (dJ_dalpha) *dJ_dalpha = As;
+  if (dJ_dv0)
+    for (int i = 0; i < n; ++i) {
+      dJ_dv0[2 * i] = gx[i];
+      dJ_dv0[2 * i + 1] = gy[i];
+    }
+  if (ms_fwd) *ms_fwd = f01;
+  if (ms_bwd) *ms_bwd = f12;
+  c->sorted = false;
+  c->masks_valid = false;
+  c->have_rates = false;
   return check_domain(c);
 }
 
@@ -492,6 +733,7 @@ template <typename R> int upload_t(arm_ctx* c, const double* x, const double* v,
   HIPCHK(c, hipMemsetAsync(A.rho_min, 0, sizeof(R) * n, c->stream));
   HIPCHK(c, hipMemsetAsync(A.u_min, 0, sizeof(double) * n, c->stream));
   HIPCHK(c, hipMemsetAsync(A.v_min, 0, sizeof(R2) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.flag, 0, n, c->stream));
   // FP32 form: cell + offset from the FP64 positions (the keys kernel writes them)
   if (Layout<R>::f32) {
     hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
@@ -895,6 +1137,18 @@ int arm_reset_pass_times(arm_ctx* c) {
     c->pass_cnt[k] = 0;
   }
   return ARM_OK;
+}
+
+int arm_adjoint_rollout(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* obj_ids, int nobj,
+                        double* J, double* dJ_dalpha, double* dJ_dv0, double* ms_fwd,
+                        double* ms_bwd) {
+  if (!c || nsteps < 1 || ckpt_every < 1 || nobj < 0 || (nobj > 0 && !obj_ids)) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (is32(c)) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint runs in the FP64 form");
+  if (c->prm.density_method != 0)
+    return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint differentiates continuity density only");
+  return adjoint_rollout(c, nsteps, std::min(ckpt_every, nsteps), obj_ids, nobj, J, dJ_dalpha,
+                         dJ_dv0, ms_fwd, ms_bwd);
 }
 
 int arm_set_gather_path(arm_ctx* c, int tiled) {

@@ -58,6 +58,7 @@ template <typename R> struct St {
   R* rho_min;
   double* u_min;
   R2* v_min;
+  uint8_t* flag;  // bit 0: predictor clamped u at 0; bit 1: EOS clamp taken
 };
 
 __device__ __forceinline__ bool is_rate(const KParams& K, int id) {

@@ -121,6 +121,7 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_all(St<R> A, St<R> Bs, KParam
   Bs.rho_min[t] = A.rho_min[s];
   Bs.u_min[t] = A.u_min[s];
   Bs.v_min[t] = A.v_min[s];
+  Bs.flag[t] = A.flag[s];
 }
 
 // Fused reorder + leapfrog predictor (time_integration.f:136-154): the
@@ -153,13 +154,17 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
   R2 v = A.v[s];
   R rho = A.rho[s];
   double u = A.u[s];
+  uint8_t fl = 0;
   if (predict) {
     const double hdt = K.dt / 2.0;
     const bool rate = is_rate(K, id);
     if (rate) {
       Bs.u_min[t] = u;
       u = u + hdt * (double)A.du[s];
-      if (u < 0.0) u = 0.0;
+      if (u < 0.0) {
+        u = 0.0;
+        fl = 1u;
+      }
       if (K.density_method == 0) {
         Bs.rho_min[t] = rho;
         rho = rho + (R)hdt * A.drho[s];
@@ -176,6 +181,7 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
   Bs.rho[t] = rho;
   Bs.u[t] = u;
   if constexpr (sizeof(R) == 4) Bs.uf[t] = (float)u;
+  Bs.flag[t] = fl;
   // the EOS recomputes p of every rate particle; others keep theirs
   if (!is_rate(K, id)) Bs.p[t] = A.p[s];
 }
@@ -450,6 +456,7 @@ __global__ __launch_bounds__(BLOCK) void k_eos(St<R> S, KParams K) {
       p = (R)0;
       rho = (sizeof(R) == 4) ? (R)0 : (R)K.rho0;
       S.rho[i] = rho;
+      S.flag[i] |= 2u;
     }
     S.p[i] = p;
   }
@@ -477,11 +484,15 @@ __global__ __launch_bounds__(BLOCK) void k_predictor(St<R> S, KParams K) {
   if (i >= K.n) return;
   const int id = S.id[i];
   const double hdt = K.dt / 2.0;
+  S.flag[i] = 0;
   if (is_rate(K, id)) {
     double u = S.u[i];
     S.u_min[i] = u;
     u = u + hdt * (double)S.du[i];
-    if (u < 0.0) u = 0.0;
+    if (u < 0.0) {
+      u = 0.0;
+      S.flag[i] = 1u;
+    }
     S.u[i] = u;
     if constexpr (sizeof(R) == 4) S.uf[i] = (float)u;
     if (K.density_method == 0) {

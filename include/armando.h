@@ -125,6 +125,22 @@ int arm_corrector(arm_ctx* ctx, int its);
    its0 .. its0+nsteps-1 (refresh_step = 1), fused on the device. */
 int arm_time_integration(arm_ctx* ctx, int its0, int nsteps);
 
+/* Discrete adjoint of time_integration (no counterpart in the reference).
+   From the current state as step 1, runs nsteps forward, storing the state
+   every ckpt_every steps, with the objective
+       J = sum over steps of dt * sum_{i in obj_ids} p_i^2
+   (p of that step's single_step), then runs backward one checkpoint
+   segment at a time: the segment is recomputed forward onto a tape and the
+   hand-written adjoint kernels of the corrector, force, EOS, density and
+   predictor run in reverse.  The neighbour list of each step is that of the
+   forward step.  Outputs (any may be NULL): J, dJ/d av_alpha, dJ/dv0 as
+   (2, n) in the original order, and the forward / backward device times in
+   ms.  The context is left at the state after nsteps.  FP64 form and
+   continuity density only. */
+int arm_adjoint_rollout(arm_ctx* ctx, int nsteps, int ckpt_every, const int32_t* obj_ids,
+                        int nobj, double* J, double* dJ_dalpha, double* dJ_dv0, double* ms_fwd,
+                        double* ms_bwd);
+
 /* FP64 sums over real particles: mass, momentum x, momentum y, kinetic
    energy, internal energy. */
 int arm_conservation(arm_ctx* ctx, double out[5]);
