@@ -31,7 +31,8 @@ EXPORTS = ("arm_create", "arm_destroy", "arm_last_error", "arm_domain_violations
            "arm_get_cells", "arm_density", "arm_eos", "arm_internal_force", "arm_av_vel",
            "arm_single_step", "arm_predictor", "arm_corrector", "arm_time_integration",
            "arm_conservation", "arm_pass_times", "arm_reset_pass_times", "arm_set_timing",
-           "arm_set_gather_path", "arm_adjoint_rollout")
+           "arm_set_gather_path", "arm_adjoint_rollout", "arm_rollout_forward",
+           "arm_rollout_backward", "arm_update_params")
 
 
 class ArmParams(C.Structure):
@@ -94,6 +95,9 @@ def load_library(path=LIB_PATH):
         "arm_set_gather_path": (C.c_int, [vp, C.c_int]),
         "arm_adjoint_rollout": (C.c_int, [vp, C.c_int, C.c_int, _ip, C.c_int, dpn, dpn, dpn,
                                           dpn, dpn]),
+        "arm_rollout_forward": (C.c_int, [vp, C.c_int, C.c_int, _ip, C.c_int, dpn, dpn]),
+        "arm_rollout_backward": (C.c_int, [vp, C.c_double, dpn, dpn, dpn]),
+        "arm_update_params": (C.c_int, [vp, C.POINTER(ArmParams)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

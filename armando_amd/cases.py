@@ -156,6 +156,20 @@ def box_lattice(nx=20, ny=15, dx=0.01, jitter=0.05, seed=7, vel=0.0, precision=F
     )
 
 
+def dam_break_adjoint_100k(precision=FP64, seed=777, vstd=0.01):
+    """The 100k adjoint workload: a 0.3 x 0.6 m column, dx = 1.34e-3 m, in the
+    3.2 x 1.6 m tank, with initial velocities N(0, vstd) per fluid particle
+    from a seeded generator.  Returns (case, objective ids): the wall
+    particles of the left wall (x < 0) below y = 0.1 m."""
+    c = dam_break(col_w=0.3, col_h=0.6, dx=1.34e-3, precision=precision)
+    rng = np.random.default_rng(seed)
+    nf = c["np"]
+    c["vx"][:nf] = rng.normal(0.0, vstd, nf)
+    c["vy"][:nf] = rng.normal(0.0, vstd, nf)
+    obj = np.nonzero((c["x"] < 0) & (c["y"] < 0.1))[0].astype(np.int32)
+    return c, obj
+
+
 PARAM_KEYS = ("precision", "flags", "pa_sph", "density_method", "h", "mass", "rho0", "c0",
               "pmin", "av_alpha", "av_beta", "av_eta", "ah_g1", "ah_g2", "ah_eta", "xsph_eps",
               "gx", "gy", "dt", "ox", "oy", "cell", "nx", "ny")

@@ -42,6 +42,7 @@ struct arm_ctx {
   int n = 0, np = 0;
   bool loaded = false, sorted = false, have_rates = false;
   bool tiled = true;  // LDS-tiled gathers (false: direct global-memory gathers)
+  struct AdjRun* adj = nullptr;  // checkpoints of the last arm_rollout_forward
   bool masks_valid = false;  // nbm holds the neighbour masks of the current order
   uint4* nbm = nullptr;
   hipStream_t stream = nullptr;
@@ -526,78 +527,131 @@ void adj_force(arm_ctx* c, const Tape& T, const AdjScratch& W, double* apart) {
   }
 }
 
-int adjoint_rollout(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, int nobj,
-                    double* J, double* dJ_dalpha, double* dJ_dv0, double* ms_fwd, double* ms_bwd) {
-  const int n = c->n, ncell = c->K.ncell;
-  const int nb = grid_of(n);
-  Scratch sc(c);
-  int rc;
+}  // namespace
+
+// Checkpoints, tapes and adjoint arrays of one rollout, kept between
+// arm_rollout_forward and arm_rollout_backward.
+struct AdjRun {
+  Scratch sc;
+  int nsteps = 0, ckpt = 0, nck = 0;
   uint8_t* obj = nullptr;
   double *jpart = nullptr, *apart = nullptr;
-  if ((rc = sc.get(&obj, n)) || (rc = sc.get(&jpart, nb)) || (rc = sc.get(&apart, nb))) return rc;
+  std::vector<Checkpoint> ck;
+  std::vector<Tape> tape;
+  AdjState L{};
+  AdjScratch W{};
+  explicit AdjRun(arm_ctx* c) : sc(c) {}
+};
+
+namespace {
+
+void drop_adjoint(arm_ctx* c) {
+  delete c->adj;
+  c->adj = nullptr;
+}
+
+int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, int nobj, double* J,
+                    double* ms) {
+  const int n = c->n, ncell = c->K.ncell;
+  const int nb = grid_of(n);
+  drop_adjoint(c);
+  AdjRun* R = new AdjRun(c);
+  c->adj = R;
+  Scratch& sc = R->sc;
+  int rc;
+  R->nsteps = nsteps;
+  R->ckpt = ckpt;
+  R->nck = (nsteps + ckpt - 1) / ckpt;
+  if ((rc = sc.get(&R->obj, n)) || (rc = sc.get(&R->jpart, nb)) || (rc = sc.get(&R->apart, nb)))
+    return rc;
   {
     std::vector<uint8_t> h(n, 0);
     for (int k = 0; k < nobj; ++k) {
-      if (obj_ids[k] < 0 || obj_ids[k] >= n) return fail(c, ARM_ERR_ARG, "objective id %d out of range", obj_ids[k]);
+      if (obj_ids[k] < 0 || obj_ids[k] >= n)
+        return fail(c, ARM_ERR_ARG, "objective id %d out of range", obj_ids[k]);
       h[obj_ids[k]] = 1;
     }
-    HIPCHK(c, hipMemcpyAsync(obj, h.data(), n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(R->obj, h.data(), n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
   }
-  HIPCHK(c, hipMemsetAsync(jpart, 0, sizeof(double) * nb, c->stream));
-  HIPCHK(c, hipMemsetAsync(apart, 0, sizeof(double) * nb, c->stream));
-  const int nck = (nsteps + ckpt - 1) / ckpt;
-  std::vector<Checkpoint> ck(nck + 1);
-  for (auto& k : ck)
+  HIPCHK(c, hipMemsetAsync(R->jpart, 0, sizeof(double) * nb, c->stream));
+  R->ck.resize(R->nck + 1);
+  for (auto& k : R->ck)
     if ((rc = alloc_checkpoint(sc, k, n))) return rc;
-  std::vector<Tape> tape(ckpt);
-  for (auto& t : tape)
+  R->tape.resize(ckpt);
+  for (auto& t : R->tape)
     if ((rc = alloc_tape(sc, t, n, ncell))) return rc;
-  AdjState L{};
-  double** lf[] = {&L.x,  &L.y,  &L.vx,  &L.vy,  &L.rho,  &L.u,   &L.bx,    &L.by,
-                   &L.bvx, &L.bvy, &L.brho, &L.bu, &L.cax, &L.cay, &L.cdrho, &L.cdu};
-  for (double** q : lf) {
+  double** lf[] = {&R->L.x,   &R->L.y,   &R->L.vx,  &R->L.vy,  &R->L.rho,   &R->L.u,
+                   &R->L.bx,  &R->L.by,  &R->L.bvx, &R->L.bvy, &R->L.brho,  &R->L.bu,
+                   &R->L.cax, &R->L.cay, &R->L.cdrho, &R->L.cdu};
+  for (double** q : lf)
     if ((rc = sc.get(q, n))) return rc;
-    HIPCHK(c, hipMemsetAsync(*q, 0, sizeof(double) * n, c->stream));
-  }
-  AdjScratch W{};
-  double** wf[] = {&W.lax, &W.lay, &W.ldrho, &W.ldu, &W.lavx, &W.lavy, &W.lp, &W.lQ,  &W.lA,
-                   &W.lpsi, &W.lS, &W.lrhoh, &W.lu, &W.lvx, &W.lvy, &W.lx, &W.ly};
+  double** wf[] = {&R->W.lax, &R->W.lay, &R->W.ldrho, &R->W.ldu, &R->W.lavx, &R->W.lavy,
+                   &R->W.lp,  &R->W.lQ,  &R->W.lA,    &R->W.lpsi, &R->W.lS,  &R->W.lrhoh,
+                   &R->W.lu,  &R->W.lvx, &R->W.lvy,   &R->W.lx,  &R->W.ly};
   for (double** q : wf)
     if ((rc = sc.get(q, n))) return rc;
-
-  hipEvent_t e0, e1, e2;
+  hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
-  HIPCHK(c, hipEventCreate(&e2));
-  // forward with checkpoints and the objective
   HIPCHK(c, hipEventRecord(e0, c->stream));
   if ((rc = begin_steps<double>(c))) return rc;
   if (!(c->prm.flags & ARM_AV_VEL) && (rc = zero_av<double>(c))) return rc;
   StepHooks fwd;
-  fwd.objmask = obj;
-  fwd.jpart = jpart;
+  fwd.objmask = R->obj;
+  fwd.jpart = R->jpart;
   for (int its = 1; its <= nsteps; ++its) {
     if ((its - 1) % ckpt == 0)
-      if ((rc = copy_checkpoint(c, c->A64, ck[(its - 1) / ckpt], true))) return rc;
+      if ((rc = copy_checkpoint(c, c->A64, R->ck[(its - 1) / ckpt], true))) return rc;
     if ((rc = fused_step<double>(c, its, fwd))) return rc;
   }
-  if ((rc = copy_checkpoint(c, c->A64, ck[nck], true))) return rc;  // final state
+  if ((rc = copy_checkpoint(c, c->A64, R->ck[R->nck], true))) return rc;  // final state
   HIPCHK(c, hipEventRecord(e1, c->stream));
-  // backward, one checkpoint segment at a time
+  std::vector<double> jp(nb);
+  HIPCHK(c, hipMemcpyAsync(jp.data(), R->jpart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float f = 0.f;
+  (void)hipEventElapsedTime(&f, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  double Js = 0;
+  for (int k = 0; k < nb; ++k) Js += jp[k];
+  if (J) *J = Js;
+  if (ms) *ms = f;
+  c->sorted = false;
+  c->masks_valid = false;
+  c->have_rates = true;
+  return check_domain(c);
+}
+
+int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0, double* ms) {
+  AdjRun* R = c->adj;
+  const int n = c->n, nb = grid_of(n);
+  const int nsteps = R->nsteps, ckpt = R->ckpt, nck = R->nck;
+  AdjState& L = R->L;
+  AdjScratch& W = R->W;
+  int rc;
+  double** lf[] = {&L.x, &L.y, &L.vx, &L.vy, &L.rho, &L.u, &L.cax, &L.cay, &L.cdrho, &L.cdu};
+  for (double** q : lf) HIPCHK(c, hipMemsetAsync(*q, 0, sizeof(double) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(R->apart, 0, sizeof(double) * nb, c->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, hipEventRecord(e0, c->stream));
   for (int seg = nck - 1; seg >= 0; --seg) {
     const int s0 = 1 + seg * ckpt, s1 = std::min(s0 + ckpt, nsteps + 1);
-    if ((rc = copy_checkpoint(c, c->A64, ck[seg], false))) return rc;
+    if ((rc = copy_checkpoint(c, c->A64, R->ck[seg], false))) return rc;
     if ((rc = begin_steps<double>(c))) return rc;
     for (int its = s0; its < s1; ++its) {
       StepHooks rec;
-      rec.tape = &tape[its - s0];
+      rec.tape = &R->tape[its - s0];
       if ((rc = fused_step<double>(c, its, rec))) return rc;
     }
     for (int its = s1 - 1; its >= s0; --its) {
-      const Tape& T = tape[its - s0];
+      const Tape& T = R->tape[its - s0];
       hipLaunchKernelGGL(k_adj_corrector, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its, L, W,
-                         obj);
-      adj_force(c, T, W, apart);
+                         R->obj, seed);
+      adj_force(c, T, W, R->apart);
       hipLaunchKernelGGL(k_adj_eos, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
       hipLaunchKernelGGL(k_adj_density, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
       hipLaunchKernelGGL(k_adj_predictor, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its, L, W);
@@ -611,38 +665,29 @@ int adjoint_rollout(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, in
       std::swap(L.u, L.bu);
     }
   }
-  HIPCHK(c, hipEventRecord(e2, c->stream));
+  HIPCHK(c, hipEventRecord(e1, c->stream));
   // leave the context at the final forward state
-  if ((rc = copy_checkpoint(c, c->A64, ck[nck], false))) return rc;
-  std::vector<double> jp(nb), ap(nb), gx(n), gy(n);
-  HIPCHK(c, hipMemcpyAsync(jp.data(), jpart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(ap.data(), apart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  if ((rc = copy_checkpoint(c, c->A64, R->ck[nck], false))) return rc;
+  std::vector<double> ap(nb), gx(n), gy(n);
+  HIPCHK(c, hipMemcpyAsync(ap.data(), R->apart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(gx.data(), L.vx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(gy.data(), L.vy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  float f01 = 0.f, f12 = 0.f;
-  (void)hipEventElapsedTime(&f01, e0, e1);
-  (void)hipEventElapsedTime(&f12, e1, e2);
+  float f = 0.f;
+  (void)hipEventElapsedTime(&f, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(e2);
-  double Js = 0, As = 0;
-  for (int b = 0; b < nb; ++b) {
-    Js += jp[b];
-    As += ap[b];
-  }
-  if (J) *J = Js;
+  double As = 0;
+  for (int k = 0; k < nb; ++k) As += ap[k];
   if (dJ_dalpha) *dJ_dalpha = As;
   if (dJ_dv0)
     for (int i = 0; i < n; ++i) {
       dJ_dv0[2 * i] = gx[i];
       dJ_dv0[2 * i + 1] = gy[i];
     }
-  if (ms_fwd) *ms_fwd = f01;
-  if (ms_bwd) *ms_bwd = f12;
+  if (ms) *ms = f;
   c->sorted = false;
   c->masks_valid = false;
-  c->have_rates = false;
   return check_domain(c);
 }
 
@@ -835,6 +880,46 @@ template <typename R> int conservation_t(arm_ctx* c, double out[5]) {
 
 #define DISPATCH(c, call) (is32(c) ? call<float> : call<double>)
 
+void set_kparams(KParams& K, const arm_params& P) {
+  K.flags = P.flags;
+  K.all_rate = (P.flags & ARM_DBC) ? 1 : 0;
+  K.pa_sph = P.pa_sph;
+  K.density_method = P.density_method;
+  K.nx = P.nx;
+  K.ny = P.ny;
+  K.ncell = P.nx * P.ny;
+  K.h = P.h;
+  K.invh = 1.0 / P.h;
+  K.mass = P.mass;
+  K.rho0 = P.rho0;
+  K.c0 = P.c0;
+  K.pmin = P.pmin;
+  K.B = P.rho0 * P.c0 * P.c0 / 7.0;
+  K.av_alpha = P.av_alpha;
+  K.av_beta = P.av_beta;
+  K.av_e2 = (P.h * P.av_eta) * (P.h * P.av_eta);
+  K.ah_g1 = P.ah_g1;
+  K.ah_g2 = P.ah_g2;
+  K.ah_e2 = (P.h * P.ah_eta) * (P.h * P.ah_eta);
+  K.xsph_eps = P.xsph_eps;
+  K.gx = P.gx;
+  K.gy = P.gy;
+  K.dt = P.dt;
+  K.ox = P.ox;
+  K.oy = P.oy;
+  K.cell = P.cell;
+  K.cut2 = (2.0 * P.h) * (2.0 * P.h);
+  K.wnorm = 7.0 / (4.0 * M_PI * P.h * P.h);
+  K.C1 = 0.625 * K.wnorm / (P.h * P.h);
+  K.psi0 = P.ah_g1 * P.h * P.c0;
+  K.g2h2 = P.ah_g2 * P.h * P.h;
+  K.eps_m = P.xsph_eps * P.mass;
+  K.same_e2 = (K.av_e2 == K.ah_e2) ? 1 : 0;
+  K.cellf = (float)P.cell;
+  K.cut2f = (float)K.cut2;
+
+}
+
 int need_sorted(arm_ctx* c) {
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
   if (!c->sorted) return fail(c, ARM_ERR_STATE, "arm_link_list has not run on this state");
@@ -874,46 +959,10 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     *out = c;  // so that arm_last_error can report
     return rc;
   }
-  KParams& K = c->K;
-  K.flags = P.flags;
-  K.all_rate = (P.flags & ARM_DBC) ? 1 : 0;
-  K.pa_sph = P.pa_sph;
-  K.density_method = P.density_method;
-  K.nx = P.nx;
-  K.ny = P.ny;
-  K.ncell = P.nx * P.ny;
-  K.h = P.h;
-  K.invh = 1.0 / P.h;
-  K.mass = P.mass;
-  K.rho0 = P.rho0;
-  K.c0 = P.c0;
-  K.pmin = P.pmin;
-  K.B = P.rho0 * P.c0 * P.c0 / 7.0;
-  K.av_alpha = P.av_alpha;
-  K.av_beta = P.av_beta;
-  K.av_e2 = (P.h * P.av_eta) * (P.h * P.av_eta);
-  K.ah_g1 = P.ah_g1;
-  K.ah_g2 = P.ah_g2;
-  K.ah_e2 = (P.h * P.ah_eta) * (P.h * P.ah_eta);
-  K.xsph_eps = P.xsph_eps;
-  K.gx = P.gx;
-  K.gy = P.gy;
-  K.dt = P.dt;
-  K.ox = P.ox;
-  K.oy = P.oy;
-  K.cell = P.cell;
-  K.cut2 = (2.0 * P.h) * (2.0 * P.h);
-  K.wnorm = 7.0 / (4.0 * M_PI * P.h * P.h);
-  K.C1 = 0.625 * K.wnorm / (P.h * P.h);
-  K.psi0 = P.ah_g1 * P.h * P.c0;
-  K.g2h2 = P.ah_g2 * P.h * P.h;
-  K.eps_m = P.xsph_eps * P.mass;
-  K.same_e2 = (K.av_e2 == K.ah_e2) ? 1 : 0;
-  K.cellf = (float)P.cell;
-  K.cut2f = (float)K.cut2;
+  set_kparams(c->K, P);
 
   c->cap = capacity;
-  const int ncell = K.ncell;
+  const int ncell = c->K.ncell;
   int nbits = 1;
   while (nbits < 31 && (1LL << nbits) < (long long)ncell) ++nbits;
   c->sort_passes = (nbits + SORT_MAX_BITS - 1) / SORT_MAX_BITS;
@@ -954,6 +1003,7 @@ void arm_destroy(arm_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   resolve_times(c);
+  drop_adjoint(c);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1139,16 +1189,46 @@ int arm_reset_pass_times(arm_ctx* c) {
   return ARM_OK;
 }
 
-int arm_adjoint_rollout(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* obj_ids, int nobj,
-                        double* J, double* dJ_dalpha, double* dJ_dv0, double* ms_fwd,
-                        double* ms_bwd) {
+int arm_rollout_forward(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* obj_ids, int nobj,
+                        double* J, double* ms) {
   if (!c || nsteps < 1 || ckpt_every < 1 || nobj < 0 || (nobj > 0 && !obj_ids)) return ARM_ERR_ARG;
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
   if (is32(c)) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint runs in the FP64 form");
   if (c->prm.density_method != 0)
     return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint differentiates continuity density only");
-  return adjoint_rollout(c, nsteps, std::min(ckpt_every, nsteps), obj_ids, nobj, J, dJ_dalpha,
-                         dJ_dv0, ms_fwd, ms_bwd);
+  const int rc = rollout_forward(c, nsteps, std::min(ckpt_every, nsteps), obj_ids, nobj, J, ms);
+  if (rc) drop_adjoint(c);
+  return rc;
+}
+
+int arm_rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0, double* ms) {
+  if (!c) return ARM_ERR_ARG;
+  if (!c->adj) return fail(c, ARM_ERR_STATE, "no arm_rollout_forward to differentiate");
+  return rollout_backward(c, seed, dJ_dalpha, dJ_dv0, ms);
+}
+
+int arm_adjoint_rollout(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* obj_ids, int nobj,
+                        double* J, double* dJ_dalpha, double* dJ_dv0, double* ms_fwd,
+                        double* ms_bwd) {
+  int rc = arm_rollout_forward(c, nsteps, ckpt_every, obj_ids, nobj, J, ms_fwd);
+  if (rc) return rc;
+  return arm_rollout_backward(c, 1.0, dJ_dalpha, dJ_dv0, ms_bwd);
+}
+
+int arm_update_params(arm_ctx* c, const arm_params* prm) {
+  if (!c || !prm) return ARM_ERR_ARG;
+  const arm_params& P = *prm;
+  const arm_params& O = c->prm;
+  if (P.precision != O.precision || P.nx != O.nx || P.ny != O.ny || P.ox != O.ox ||
+      P.oy != O.oy || P.cell != O.cell || P.h != O.h || P.flags != O.flags ||
+      P.density_method != O.density_method || P.pa_sph != O.pa_sph)
+    return fail(c, ARM_ERR_ARG, "only the physical scalars can change after arm_create");
+  if (!(P.mass > 0) || !(P.rho0 > 0) || !(P.dt > 0))
+    return fail(c, ARM_ERR_ARG, "mass, rho0 and dt must be positive");
+  c->prm = P;
+  set_kparams(c->K, P);
+  drop_adjoint(c);
+  return ARM_OK;
 }
 
 int arm_set_gather_path(arm_ctx* c, int tiled) {

@@ -1,0 +1,152 @@
+// autograd.cpp -- torch::autograd::Function over the C ABI adjoint rollout.
+//
+// J(v0, alpha) = sum over steps of dt * sum_{i in obj} p_i^2 for nsteps of
+// time_integration started from a fixed initial state with initial
+// velocities v0 (n x 2) and artificial-viscosity alpha.  forward() runs
+// arm_rollout_forward (checkpoints kept in the device context); backward()
+// runs arm_rollout_backward with the incoming gradient as the adjoint seed
+// and returns dJ/dv0 and dJ/dalpha.  Host code only; the device work is in
+// libarmando.so.
+#include <torch/extension.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "armando.h"
+
+namespace {
+
+struct Rollout {
+  arm_ctx* ctx = nullptr;
+  arm_params prm{};
+  int np = 0, nv = 0;
+  std::vector<double> x, rho, u;  // x is (2, n): x and y of a particle adjacent
+  std::vector<int32_t> obj;
+  int nsteps = 0, ckpt = 0;
+};
+
+void check(const Rollout* r, int rc, const char* what) {
+  TORCH_CHECK(rc == ARM_OK, what, " failed (", rc, "): ",
+              r->ctx ? arm_last_error(r->ctx) : "no context");
+}
+
+Rollout* as_rollout(int64_t h) {
+  TORCH_CHECK(h != 0, "null rollout handle");
+  return reinterpret_cast<Rollout*>(h);
+}
+
+std::vector<double> to_vec(const torch::Tensor& t) {
+  auto c = t.detach().to(torch::kCPU, torch::kFloat64).contiguous();
+  return std::vector<double>(c.data_ptr<double>(), c.data_ptr<double>() + c.numel());
+}
+
+class RolloutFn : public torch::autograd::Function<RolloutFn> {
+ public:
+  static torch::Tensor forward(torch::autograd::AutogradContext* actx, torch::Tensor v0,
+                               torch::Tensor alpha, int64_t handle) {
+    Rollout* r = as_rollout(handle);
+    const int n = r->np + r->nv;
+    TORCH_CHECK(v0.dim() == 2 && v0.size(0) == n && v0.size(1) == 2, "v0 must be (n, 2)");
+    auto v = v0.detach().to(torch::kCPU, torch::kFloat64).contiguous();
+    arm_params P = r->prm;
+    P.av_alpha = alpha.detach().to(torch::kCPU, torch::kFloat64).item<double>();
+    check(r, arm_update_params(r->ctx, &P), "arm_update_params");
+    check(r, arm_upload(r->ctx, r->np, r->nv, r->x.data(), v.data_ptr<double>(), r->rho.data(),
+                        r->u.data()),
+          "arm_upload");
+    double J = 0.0;
+    check(r, arm_rollout_forward(r->ctx, r->nsteps, r->ckpt, r->obj.data(), (int)r->obj.size(),
+                                 &J, nullptr),
+          "arm_rollout_forward");
+    actx->saved_data["handle"] = handle;
+    actx->saved_data["v_device"] = v0.device().str();
+    return torch::tensor(J, torch::dtype(torch::kFloat64).device(alpha.device()));
+  }
+
+  static torch::autograd::tensor_list backward(torch::autograd::AutogradContext* actx,
+                                               torch::autograd::tensor_list grads) {
+    Rollout* r = as_rollout(actx->saved_data["handle"].toInt());
+    const std::string dev = actx->saved_data["v_device"].toStringRef();
+    const double seed = grads[0].detach().to(torch::kCPU, torch::kFloat64).item<double>();
+    const int n = r->np + r->nv;
+    auto dv = torch::zeros({n, 2}, torch::dtype(torch::kFloat64));
+    double da = 0.0;
+    check(r, arm_rollout_backward(r->ctx, seed, &da, dv.data_ptr<double>(), nullptr),
+          "arm_rollout_backward");
+    auto dalpha = torch::tensor(da, torch::dtype(torch::kFloat64).device(grads[0].device()));
+    return {dv.to(torch::Device(dev)), dalpha, torch::Tensor()};
+  }
+};
+
+int64_t make_rollout(py::dict p, torch::Tensor x, torch::Tensor rho, torch::Tensor u, int64_t np,
+                     int64_t nv, torch::Tensor obj, int64_t nsteps, int64_t ckpt) {
+  auto* r = new Rollout();
+  auto d = [&](const char* k) { return p[k].cast<double>(); };
+  auto i = [&](const char* k) { return p[k].cast<int>(); };
+  arm_params& P = r->prm;
+  P.precision = i("precision");
+  P.flags = i("flags");
+  P.pa_sph = i("pa_sph");
+  P.density_method = i("density_method");
+  P.h = d("h");
+  P.mass = d("mass");
+  P.rho0 = d("rho0");
+  P.c0 = d("c0");
+  P.pmin = d("pmin");
+  P.av_alpha = d("av_alpha");
+  P.av_beta = d("av_beta");
+  P.av_eta = d("av_eta");
+  P.ah_g1 = d("ah_g1");
+  P.ah_g2 = d("ah_g2");
+  P.ah_eta = d("ah_eta");
+  P.xsph_eps = d("xsph_eps");
+  P.gx = d("gx");
+  P.gy = d("gy");
+  P.dt = d("dt");
+  P.ox = d("ox");
+  P.oy = d("oy");
+  P.cell = d("cell");
+  P.nx = i("nx");
+  P.ny = i("ny");
+  r->np = (int)np;
+  r->nv = (int)nv;
+  r->x = to_vec(x);
+  r->rho = to_vec(rho);
+  r->u = to_vec(u);
+  auto o = obj.detach().to(torch::kCPU, torch::kInt32).contiguous();
+  r->obj.assign(o.data_ptr<int32_t>(), o.data_ptr<int32_t>() + o.numel());
+  r->nsteps = (int)nsteps;
+  r->ckpt = (int)ckpt;
+  const int n = (int)(np + nv);
+  TORCH_CHECK((int)r->x.size() == 2 * n && (int)r->rho.size() == n && (int)r->u.size() == n,
+              "x must be (n, 2), rho and u (n)");
+  const int rc = arm_create(&P, n, &r->ctx);
+  if (rc != ARM_OK) {
+    const std::string msg = r->ctx ? arm_last_error(r->ctx) : "";
+    arm_destroy(r->ctx);
+    delete r;
+    TORCH_CHECK(false, "arm_create failed (", rc, "): ", msg);
+  }
+  return reinterpret_cast<int64_t>(r);
+}
+
+void free_rollout(int64_t h) {
+  if (!h) return;
+  Rollout* r = as_rollout(h);
+  arm_destroy(r->ctx);
+  delete r;
+}
+
+torch::Tensor rollout_loss(int64_t h, torch::Tensor v0, torch::Tensor alpha) {
+  return RolloutFn::apply(v0, alpha, h);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
+  m.doc() = "torch.autograd over the MI355X WCSPH adjoint rollout";
+  m.def("make_rollout", &make_rollout, "create a rollout context (returns a handle)");
+  m.def("free_rollout", &free_rollout, "destroy a rollout context");
+  m.def("rollout_loss", &rollout_loss, "J(v0, alpha) with a hand-written adjoint backward");
+}

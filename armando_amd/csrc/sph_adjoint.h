@@ -103,10 +103,12 @@ __global__ __launch_bounds__(BLOCK) void k_objective(St<double> S, KParams K,
 
 // 1. corrector adjoint (time_integration.f:186-217): state adjoints after
 //    the step -> rate adjoints (sorted) and the direct part of the state
-//    adjoints before the step (id order); the objective seeds lp.
+//    adjoints before the step (id order); the objective, scaled by the
+//    incoming adjoint seed = dL/dJ, seeds lp.
 __global__ __launch_bounds__(BLOCK) void k_adj_corrector(Tape T, KParams K, int its,
                                                          AdjState L, AdjScratch W,
-                                                         const uint8_t* __restrict__ objmask) {
+                                                         const uint8_t* __restrict__ objmask,
+                                                         double seed) {
   const int k = blockIdx.x * BLOCK + threadIdx.x;
   if (k >= K.n) return;
   const int id = T.id[k];
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_corrector(Tape T, KParams K, int 
   W.ldu[k] = ldu;
   W.lavx[k] = lavx;
   W.lavy[k] = lavy;
-  W.lp[k] = objmask[id] ? 2.0 * dt * T.p[k] : 0.0;
+  W.lp[k] = objmask[id] ? seed * 2.0 * dt * T.p[k] : 0.0;
 }
 
 __device__ __forceinline__ void wendland_q(const KParams& K, double r2, double& q, double& F,

@@ -12,7 +12,11 @@ included); the FP64 plain form runs the same steps for the FP32/FP64 speedup
 and for the accuracy comparison.  Inputs are resident in HBM before the timed
 region starts; the timed region is K whole steps.
 
+The adjoint leg reports the adjoint / forward device-time ratio of the FP64
+rollout on the 100k dam break (400 steps, checkpoint every 20).
+
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp64]
+                  [--no-adjoint]
 """
 import argparse
 import json
@@ -50,6 +54,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 form")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work for the baseline sample")
     ap.add_argument("--scale", type=float, default=1.0,
@@ -85,6 +90,41 @@ def run_form(precision, args, case_fn):
     sim.close()
     return dict(case=case, niac=niac, wall=wall, passes=passes, timed_steps=nt, state=state,
                 conservation=cons)
+
+
+def adjoint_ratio(steps=400, ckpt=20):
+    """Adjoint / forward device time of the FP64 rollout on the 100k dam
+    break (400 steps, checkpoints every 20 steps)."""
+    import armando_amd
+    from armando_amd import cases
+
+    case, obj = cases.dam_break_adjoint_100k()
+    sim = armando_amd.Simulation(case)
+    sim.adjoint_rollout(5, 5, obj)  # warm up the kernels
+    sim.close()
+    sim = armando_amd.Simulation(case)
+    J, da, g, (t_fwd, t_bwd) = sim.adjoint_rollout(steps, ckpt, obj)
+    sim.close()
+    plain = armando_amd.Simulation(case)
+    plain.time_integration(1, 5)
+    plain.close()
+    plain = armando_amd.Simulation(case)
+    t0 = time.perf_counter()
+    plain.time_integration(1, steps)
+    t_plain = 1e3 * (time.perf_counter() - t0)
+    plain.close()
+    return {
+        "workload": f"dam break {case['np']} fluid + {case['nv']} wall particles, FP64, "
+                    f"{steps} steps, checkpoint every {ckpt}",
+        "ratio": t_bwd / t_fwd,
+        "forward_ms": t_fwd,
+        "backward_ms": t_bwd,
+        "plain_forward_ms": t_plain,
+        "ratio_vs_plain_forward": t_bwd / t_plain,
+        "J": J,
+        "dJ_dalpha": da,
+        "dJ_dv0_norm": float(np.linalg.norm(g)),
+    }
 
 
 def cpu_baseline(case, seconds):
@@ -195,6 +235,8 @@ def main():
         }
         out["accuracy_fp32_vs_fp64"] = {
             "after_steps": args.warmup + args.steps + r32["timed_steps"], **acc}
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_ratio()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out))

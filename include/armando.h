@@ -140,6 +140,17 @@ int arm_time_integration(arm_ctx* ctx, int its0, int nsteps);
 int arm_adjoint_rollout(arm_ctx* ctx, int nsteps, int ckpt_every, const int32_t* obj_ids,
                         int nobj, double* J, double* dJ_dalpha, double* dJ_dv0, double* ms_fwd,
                         double* ms_bwd);
+/* The two halves of arm_adjoint_rollout, for an autograd wrapper: the
+   forward keeps its checkpoints in the context; the backward scales the
+   objective adjoint by seed (= dL/dJ) and may be called again. */
+int arm_rollout_forward(arm_ctx* ctx, int nsteps, int ckpt_every, const int32_t* obj_ids,
+                        int nobj, double* J, double* ms);
+int arm_rollout_backward(arm_ctx* ctx, double seed, double* dJ_dalpha, double* dJ_dv0,
+                         double* ms);
+/* Replace the physical scalars (mass, rho0, c0, pmin, viscosity, heat, XSPH,
+   gravity, dt) of a context; grid, h, precision and flags must not change.
+   Drops the checkpoints of a previous rollout. */
+int arm_update_params(arm_ctx* ctx, const arm_params* prm);
 
 /* FP64 sums over real particles: mass, momentum x, momentum y, kinetic
    energy, internal energy. */

@@ -107,3 +107,23 @@ def test_adjoint_leaves_the_final_state():
     sb = b.state()
     for k in ("x", "y", "vx", "vy", "rho", "u"):
         assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_torch_autograd_function_backward_is_the_adjoint():
+    import torch
+
+    from armando_amd.autograd import RolloutLoss
+
+    c, obj = adjoint_case()
+    steps = 10
+    banner("torch autograd")
+    ref = gpu_rollout(c, obj, steps, 4)
+    loss = RolloutLoss(c, obj, steps, 4)
+    v0 = torch.tensor(np.stack([c["vx"], c["vy"]], 1), dtype=torch.float64, requires_grad=True)
+    alpha = torch.tensor(c["av_alpha"], dtype=torch.float64, requires_grad=True)
+    J = loss(v0, alpha)
+    assert J.item() == ref[0]
+    (3.0 * J).backward()
+    assert alpha.grad.item() == pytest.approx(3.0 * ref[1], rel=1e-12)
+    assert np.allclose(v0.grad.numpy(), 3.0 * ref[2], rtol=1e-12, atol=0)
+    loss.close()
