@@ -22,7 +22,8 @@ LIB_PATH = os.path.join(HERE, "libarmando.so")
 ARM_OK = 0
 ERRORS = {-1: "ARM_ERR_ARG", -2: "ARM_ERR_HIP", -3: "ARM_ERR_CAPACITY", -4: "ARM_ERR_DOMAIN",
           -5: "ARM_ERR_UNSUPPORTED", -6: "ARM_ERR_STATE"}
-ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY = 1, 2, 4, 8, 16
+ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC = 1, 2, 4, 8, 16, 32
+EOS_TAIT, EOS_LINEAR = 0, 1
 FP64, FP32 = 0, 1
 PASSES = ("keys", "sort", "reorder", "density", "eos", "force", "integrate")
 
@@ -45,7 +46,13 @@ class ArmParams(C.Structure):
         ("ah_eta", C.c_double), ("xsph_eps", C.c_double), ("gx", C.c_double),
         ("gy", C.c_double), ("dt", C.c_double), ("ox", C.c_double), ("oy", C.c_double),
         ("cell", C.c_double), ("nx", C.c_int), ("ny", C.c_int),
+        ("visc_eta", C.c_double), ("eos", C.c_int), ("eos_p0", C.c_double),
     ]
+
+
+# fields a case may leave out, with their defaults (no physical viscosity,
+# Tait EOS)
+PARAM_DEFAULTS = {"visc_eta": 0.0, "eos": 0, "eos_p0": 0.0}
 
 
 class ArmandoError(RuntimeError):
@@ -114,7 +121,7 @@ def _ptr(a):
 def params_from(case):
     P = ArmParams()
     for k, _ in ArmParams._fields_:
-        setattr(P, k, case[k])
+        setattr(P, k, case[k] if k in case else PARAM_DEFAULTS[k])
     return P
 
 
@@ -240,6 +247,25 @@ class Simulation:
                                                C.byref(J), C.byref(da), _ptr(g), C.byref(tf),
                                                C.byref(tb)))
         return J.value, da.value, g, (tf.value, tb.value)
+
+    def rollout_forward(self, nsteps, ckpt_every, obj_ids):
+        """J and the forward device ms; keeps the checkpoints for
+        rollout_backward."""
+        ids = np.ascontiguousarray(obj_ids, np.int32)
+        J = C.c_double(0)
+        tf = C.c_double(0)
+        self._check(self.L.arm_rollout_forward(self.ctx, nsteps, ckpt_every, ids, len(ids),
+                                               C.byref(J), C.byref(tf)))
+        return J.value, tf.value
+
+    def rollout_backward(self, seed=1.0):
+        """seed * (dJ/dalpha, dJ/dv0 (n, 2)) and the backward device ms."""
+        da = C.c_double(0)
+        g = np.zeros((self.n, 2))
+        tb = C.c_double(0)
+        self._check(self.L.arm_rollout_backward(self.ctx, seed, C.byref(da), _ptr(g),
+                                                C.byref(tb)))
+        return da.value, g, tb.value
 
     def set_gather_path(self, tiled=True):
         self._check(self.L.arm_set_gather_path(self.ctx, 1 if tiled else 0))

@@ -15,7 +15,7 @@ import sysconfig
 import numpy as np
 import torch
 
-from . import ArmParams, load_library
+from . import PARAM_DEFAULTS, ArmParams, load_library
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 _ext = None
@@ -39,7 +39,7 @@ def extension():
 class RolloutLoss:
     def __init__(self, case, obj_ids, nsteps, ckpt_every):
         ext = extension()
-        params = {k: case[k] for k, _ in ArmParams._fields_}
+        params = {k: case[k] if k in case else PARAM_DEFAULTS[k] for k, _ in ArmParams._fields_}
         x = torch.from_numpy(np.stack([case["x"], case["y"]], 1).astype(np.float64))
         rho = torch.from_numpy(np.asarray(case["rho"], np.float64))
         u = torch.from_numpy(np.asarray(case["u"], np.float64))

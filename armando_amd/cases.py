@@ -16,7 +16,8 @@ import math
 import numpy as np
 
 # flags (include/armando.h)
-ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY = 1, 2, 4, 8, 16
+ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC = 1, 2, 4, 8, 16, 32
+EOS_TAIT, EOS_LINEAR = 0, 1
 FP64, FP32 = 0, 1
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
@@ -38,6 +39,16 @@ def splitmix64(seed, count):
 def uniform01(seed, count):
     """Doubles in [0, 1) from the top 53 bits of splitmix64."""
     return (splitmix64(seed, count) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def normal01(seed, count):
+    """N(0, 1) doubles by Box-Muller on pairs of splitmix64 uniforms:
+    z0 = sqrt(-2 ln(1 - u0)) cos(2 pi u1), z1 = ... sin(2 pi u1), in order."""
+    m = (count + 1) // 2
+    u = uniform01(seed, 2 * m).reshape(m, 2)
+    r = np.sqrt(-2.0 * np.log1p(-u[:, 0]))
+    t = 2.0 * math.pi * u[:, 1]
+    return np.stack([r * np.cos(t), r * np.sin(t)], 1).ravel()[:count]
 
 
 def grid_for(x, y, cell, margin_cells=4, top_extra=0.0):
@@ -156,20 +167,70 @@ def box_lattice(nx=20, ny=15, dx=0.01, jitter=0.05, seed=7, vel=0.0, precision=F
     )
 
 
+def shear_cavity(m=41, dx=2.5e-5, lid=-1e-3, precision=FP64, h_fac=1.5, nwall=80, gap=1.0,
+                 eos_p0=-0.1, layers=1):
+    """Shear-driven cavity at Re ~ 1 (CERNsource/test_cases.f:577-761 with
+    the options of CERNsource/sph.f:298-316), on an m x m fluid lattice of
+    spacing dx whose outer rows lie `gap` dx inside the box walls; one layer
+    of nwall wall particles per side (`layers` > 1 adds rings outside),
+    evenly spaced along the perimeter with each corner once, the top side
+    moving at `lid` in x.  rho0 = 1000, dynamic viscosity 1e-3 Pa s, linear
+    EOS p = c^2 rho + eos_p0 with c = 0.01 m/s (eos_p0 = -c^2 rho0 is the
+    reference's eos_poly with a1 = 0.1), summation density, pa_sph 1, XSPH
+    eps 0.3 (av_vel.f:35), Wendland C2 with h = h_fac dx, dt = 5e-5 s.
+    Walls are frozen (no density or energy integration) and every particle
+    has the mass rho0 dx^2."""
+    L = (m - 1 + 2 * gap) * dx
+    I, K = np.meshgrid(np.arange(m), np.arange(m))
+    x = (I.ravel() + gap) * dx
+    y = (K.ravel() + gap) * dx
+    nf = x.size
+    wx, wy, wvx = [], [], []
+    for k in range(layers):
+        d = k * (L / nwall)          # layer k runs d outside the box
+        nk = nwall + 8 * k
+        t = np.arange(nk) * ((L + 2 * d) / nk)
+        wx += [t - d, np.full(nk, L + d), L + d - t, np.full(nk, -d)]
+        wy += [np.full(nk, -d), t - d, np.full(nk, L + d), L + d - t]
+        wvx += [np.zeros(nk), np.zeros(nk), np.full(nk, lid), np.zeros(nk)]
+    wx = np.concatenate(wx)
+    wy = np.concatenate(wy)
+    wvx = np.concatenate(wvx)
+    X = np.concatenate([x, wx])
+    Y = np.concatenate([y, wy])
+    n = X.size
+    vx = np.concatenate([np.zeros(nf), wvx])
+    h = h_fac * dx
+    cell = 2.0 * h
+    # the grid leaves room for the few particles that leak through the
+    # single wall layer at the lid corners
+    ox, oy, nxg, nyg = grid_for(X, Y, cell, margin_cells=40)
+    return dict(
+        name="shear_cavity", n=n, np=nf, nv=n - nf, x=X, y=Y, vx=vx, vy=np.zeros(n),
+        rho=np.full(n, 1000.0), u=np.zeros(n),
+        precision=precision, flags=VISC | AV_VEL, pa_sph=1, density_method=1, kernel=0,
+        h=h, mass=1000.0 * dx * dx, rho0=1000.0, c0=0.01, pmin=-1e30,
+        av_alpha=0.0, av_beta=0.0, av_eta=0.1, ah_g1=0.1, ah_g2=1.0, ah_eta=0.1,
+        xsph_eps=0.3, gx=0.0, gy=0.0, dt=5e-5, ox=ox, oy=oy, cell=cell, nx=nxg, ny=nyg,
+        visc_eta=1e-3, eos=EOS_LINEAR, eos_p0=eos_p0,
+    )
+
+
 def dam_break_adjoint_100k(precision=FP64, seed=777, vstd=0.01):
     """The 100k adjoint workload: a 0.3 x 0.6 m column, dx = 1.34e-3 m, in the
     3.2 x 1.6 m tank, with initial velocities N(0, vstd) per fluid particle
-    from a seeded generator.  Returns (case, objective ids): the wall
-    particles of the left wall (x < 0) below y = 0.1 m."""
+    (normal01(seed): vx of every fluid particle, then vy).  Returns (case,
+    objective ids): the wall particles of the left wall (x < 0) below
+    y = 0.1 m."""
     c = dam_break(col_w=0.3, col_h=0.6, dx=1.34e-3, precision=precision)
-    rng = np.random.default_rng(seed)
     nf = c["np"]
-    c["vx"][:nf] = rng.normal(0.0, vstd, nf)
-    c["vy"][:nf] = rng.normal(0.0, vstd, nf)
+    z = normal01(seed, 2 * nf)
+    c["vx"][:nf] = vstd * z[:nf]
+    c["vy"][:nf] = vstd * z[nf:]
     obj = np.nonzero((c["x"] < 0) & (c["y"] < 0.1))[0].astype(np.int32)
     return c, obj
 
 
 PARAM_KEYS = ("precision", "flags", "pa_sph", "density_method", "h", "mass", "rho0", "c0",
               "pmin", "av_alpha", "av_beta", "av_eta", "ah_g1", "ah_g2", "ah_eta", "xsph_eps",
-              "gx", "gy", "dt", "ox", "oy", "cell", "nx", "ny")
+              "gx", "gy", "dt", "ox", "oy", "cell", "nx", "ny", "visc_eta", "eos", "eos_p0")

@@ -133,6 +133,7 @@ template <typename R> int alloc_state(arm_ctx* c, St<R>& S, int cap) {
   AL(u_min, cap);
   AL(v_min, cap);
   AL(flag, cap);
+  AL(tau, 4 * (size_t)cap);
 #undef AL
   return ARM_OK;
 }
@@ -297,25 +298,43 @@ template <typename R> int eos_t(arm_ctx* c) {
   return ARM_OK;
 }
 
-template <typename R, bool P, bool AV, bool AH, bool XS> void launch_force(arm_ctx* c) {
+template <typename R, bool P, bool AV, bool AH, bool XS, bool VI = false>
+void launch_force(arm_ctx* c) {
   St<R>& A = stA<R>(c);
   using T = TileCfg<R>;
   if (c->tiled) {
     const int nseg = nseg_of<R>(c);
-    hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, T::BLK, T::CW, T::CAP>),
+    hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, T::BLK, T::CW, T::CAP>),
                        dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
                        c->cstart, nseg, c->masks_valid ? c->nbm : nullptr);
   } else {
-    hipLaunchKernelGGL((k_force<R, P, AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
+    hipLaunchKernelGGL((k_force<R, P, AV, AH, XS, VI>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
                        c->stream, A, c->K, c->skey, c->cstart);
+  }
+}
+
+// Shear tensor of the physical viscosity, after the EOS (tau of every particle).
+template <typename R> void launch_tensor(arm_ctx* c) {
+  St<R>& A = stA<R>(c);
+  using T = TileCfg<R>;
+  if (c->tiled) {
+    const int nseg = nseg_of<R>(c);
+    hipLaunchKernelGGL((k_tensor_tiled<R, T::BLK, T::CW, T::CAP>), dim3(c->K.ny * nseg),
+                       dim3(T::BLK), 0, c->stream, A, c->K, c->skey, c->cstart, nseg,
+                       c->masks_valid ? c->nbm : nullptr);
+  } else {
+    hipLaunchKernelGGL(k_tensor<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                       c->skey, c->cstart);
   }
 }
 
 // dispatch over the compile-time flag combination
 template <typename R> int force_t(arm_ctx* c, bool press, int flags, bool xs) {
   const bool av = (flags & ARM_ART_VISC) != 0, ah = (flags & ARM_ART_HEAT) != 0;
+  const bool vi = press && (flags & ARM_VISC) != 0;
   PassTimer tm(c, ARM_PASS_FORCE);
-  const int code = (press ? 8 : 0) | (av ? 4 : 0) | (ah ? 2 : 0) | (xs ? 1 : 0);
+  if (vi) launch_tensor<R>(c);
+  const int code = (vi ? 16 : 0) | (press ? 8 : 0) | (av ? 4 : 0) | (ah ? 2 : 0) | (xs ? 1 : 0);
   switch (code) {
     case 8: launch_force<R, true, false, false, false>(c); break;
     case 9: launch_force<R, true, false, false, true>(c); break;
@@ -325,6 +344,14 @@ template <typename R> int force_t(arm_ctx* c, bool press, int flags, bool xs) {
     case 13: launch_force<R, true, true, false, true>(c); break;
     case 14: launch_force<R, true, true, true, false>(c); break;
     case 15: launch_force<R, true, true, true, true>(c); break;
+    case 24: launch_force<R, true, false, false, false, true>(c); break;
+    case 25: launch_force<R, true, false, false, true, true>(c); break;
+    case 26: launch_force<R, true, false, true, false, true>(c); break;
+    case 27: launch_force<R, true, false, true, true, true>(c); break;
+    case 28: launch_force<R, true, true, false, false, true>(c); break;
+    case 29: launch_force<R, true, true, false, true, true>(c); break;
+    case 30: launch_force<R, true, true, true, false, true>(c); break;
+    case 31: launch_force<R, true, true, true, true, true>(c); break;
     case 1: launch_force<R, false, false, false, true>(c); break;
     default: return fail(c, ARM_ERR_ARG, "force pass with nothing to compute");
   }
@@ -555,6 +582,9 @@ int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, in
   const int n = c->n, ncell = c->K.ncell;
   const int nb = grid_of(n);
   drop_adjoint(c);
+  if ((c->prm.flags & ARM_VISC) || c->prm.eos != ARM_EOS_TAIT)
+    return fail(c, ARM_ERR_UNSUPPORTED,
+                "the adjoint covers the Tait EOS without physical viscosity");
   AdjRun* R = new AdjRun(c);
   c->adj = R;
   Scratch& sc = R->sc;
@@ -917,7 +947,11 @@ void set_kparams(KParams& K, const arm_params& P) {
   K.same_e2 = (K.av_e2 == K.ah_e2) ? 1 : 0;
   K.cellf = (float)P.cell;
   K.cut2f = (float)K.cut2;
-
+  K.eos = P.eos;
+  K.lin_c2 = P.c0 * P.c0;
+  K.lin_p0 = P.eos_p0;
+  K.lin_p0r = P.eos_p0 + K.lin_c2 * P.rho0;
+  K.eta = P.visc_eta;
 }
 
 int need_sorted(arm_ctx* c) {
@@ -955,6 +989,10 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     rc = fail(c, ARM_ERR_UNSUPPORTED, "pa_sph must be 0 or 1");
   else if (P.density_method != 0 && P.density_method != 1)
     rc = fail(c, ARM_ERR_UNSUPPORTED, "density_method must be 0 (continuity) or 1 (summation)");
+  else if (P.eos != ARM_EOS_TAIT && P.eos != ARM_EOS_LINEAR)
+    rc = fail(c, ARM_ERR_UNSUPPORTED, "eos must be ARM_EOS_TAIT or ARM_EOS_LINEAR");
+  else if ((P.flags & ARM_VISC) && !(P.visc_eta >= 0))
+    rc = fail(c, ARM_ERR_ARG, "visc_eta must be non-negative");
   if (rc) {
     *out = c;  // so that arm_last_error can report
     return rc;
@@ -1221,8 +1259,10 @@ int arm_update_params(arm_ctx* c, const arm_params* prm) {
   const arm_params& O = c->prm;
   if (P.precision != O.precision || P.nx != O.nx || P.ny != O.ny || P.ox != O.ox ||
       P.oy != O.oy || P.cell != O.cell || P.h != O.h || P.flags != O.flags ||
-      P.density_method != O.density_method || P.pa_sph != O.pa_sph)
+      P.density_method != O.density_method || P.pa_sph != O.pa_sph || P.eos != O.eos)
     return fail(c, ARM_ERR_ARG, "only the physical scalars can change after arm_create");
+  if ((P.flags & ARM_VISC) && !(P.visc_eta >= 0))
+    return fail(c, ARM_ERR_ARG, "visc_eta must be non-negative");
   if (!(P.mass > 0) || !(P.rho0 > 0) || !(P.dt > 0))
     return fail(c, ARM_ERR_ARG, "mass, rho0 and dt must be positive");
   c->prm = P;

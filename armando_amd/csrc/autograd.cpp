@@ -109,6 +109,9 @@ int64_t make_rollout(py::dict p, torch::Tensor x, torch::Tensor rho, torch::Tens
   P.cell = d("cell");
   P.nx = i("nx");
   P.ny = i("ny");
+  P.visc_eta = p.contains("visc_eta") ? d("visc_eta") : 0.0;
+  P.eos = p.contains("eos") ? i("eos") : ARM_EOS_TAIT;
+  P.eos_p0 = p.contains("eos_p0") ? d("eos_p0") : 0.0;
   r->np = (int)np;
   r->nv = (int)nv;
   r->x = to_vec(x);

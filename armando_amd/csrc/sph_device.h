@@ -31,6 +31,11 @@ struct KParams {
   double eps_m;  // XSPH eps * mass
   int same_e2;   // artificial viscosity and heat share (h eta)^2
   float cellf, cut2f;
+  int eos;         // 0 Tait, 1 linear
+  double lin_c2;   // linear EOS: p = lin_c2 rho + lin_p0
+  double lin_p0;
+  double lin_p0r;  // lin_p0 + lin_c2 rho0 (FP32 form: p = lin_c2 rho' + lin_p0r)
+  double eta;      // dynamic viscosity (flag 32)
 };
 
 // Particle state in sorted (cell, id) order.  FP64 form: xy holds the
@@ -59,6 +64,10 @@ template <typename R> struct St {
   double* u_min;
   R2* v_min;
   uint8_t* flag;  // bit 0: predictor clamped u at 0; bit 1: EOS clamp taken
+  // physical viscosity, per particle: 4 values (Wxx, Wyy, Wxy, tds) with
+  // W = eta T (pa_sph 0) or eta T / rho^2 (pa_sph 1), T the shear tensor
+  // and tds = eta/(2 rho) T:T
+  R* tau;
 };
 
 __device__ __forceinline__ bool is_rate(const KParams& K, int id) {
@@ -125,6 +134,14 @@ template <> struct Acc<float> {
   }
   __device__ __forceinline__ float get() const { return s; }
 };
+
+// Linear EOS p = c0^2 rho + p0 from the stored density.
+__device__ __forceinline__ double linear_p(const KParams& K, double rho) {
+  return K.lin_c2 * rho + K.lin_p0;
+}
+__device__ __forceinline__ float linear_p(const KParams& K, float rp) {
+  return (float)K.lin_c2 * rp + (float)K.lin_p0r;
+}
 
 // Tait pressure from the stored density (eos.f90:274).
 __device__ __forceinline__ double tait(const KParams& K, double rho) {

@@ -285,9 +285,20 @@ template <typename R> struct ForceOwn {
   using R2 = typename V2<R>::type;
   R2 v;
   R Q, A, psi, u;
+  R wxx = 0, wyy = 0, wxy = 0, tds = 0;  // physical viscosity (tau of i)
   R fx = 0, fy = 0, xsx = 0, xsy = 0;
   Acc<R> du;
 };
+
+// W of a neighbour for the viscous force (unused without viscosity).
+template <typename R> struct Tau3 {
+  R xx, yy, xy;
+};
+
+template <typename R> __device__ __forceinline__ Tau3<R> tau_of(const St<R>& S, int t) {
+  const R* w = S.tau + 4 * (size_t)t;
+  return {w[0], w[1], w[2]};
+}
 
 // Q = 1/rho and A = p/rho of a particle from its stored density.
 template <typename R>
@@ -296,17 +307,43 @@ __device__ __forceinline__ void q_and_a(const KParams& K, R rho_s, R p, R& Q, R&
   A = p * Q;
 }
 
-template <typename R>
+template <typename R, bool VI = false>
 __device__ __forceinline__ void force_begin(const St<R>& S, const KParams& K, int i, ForceOwn<R>& o) {
   o.v = S.v[i];
   q_and_a(K, S.rho[i], S.p[i], o.Q, o.A);
   o.psi = S.psi[i];
   if constexpr (sizeof(R) == 4) o.u = S.uf[i]; else o.u = (R)S.u[i];
+  if (VI) {
+    const R* w = S.tau + 4 * (size_t)i;
+    o.wxx = w[0];
+    o.wyy = w[1];
+    o.wxy = w[2];
+    o.tds = w[3];
+  }
 }
 
-template <typename R, bool PRESS, bool AV, bool AH, bool XS>
+// Viscous force of internal_force.f:196-430 (the `visc` branches):
+//   pa_sph 0: a_i += m (eta T_i + eta T_j) . grad W / (rho_i rho_j)
+//   pa_sph 1: a_i += m (eta T_i / rho_i^2 + eta T_j / rho_j^2) . grad W
+// with the per-particle W = eta T (pa_sph 0) or eta T / rho^2 (pa_sph 1).
+template <typename R>
+__device__ __forceinline__ void visc_pair(const KParams& K, ForceOwn<R>& o, R mF, R dx, R dy,
+                                          R Qj, const Tau3<R>& wj) {
+  R sxx = o.wxx + wj.xx, syy = o.wyy + wj.yy, sxy = o.wxy + wj.xy;
+  if (K.pa_sph == 0) {
+    const R qq = o.Q * Qj;
+    sxx = qq * sxx;
+    syy = qq * syy;
+    sxy = qq * sxy;
+  }
+  o.fx = o.fx + mF * (sxx * dx + sxy * dy);
+  o.fy = o.fy + mF * (sxy * dx + syy * dy);
+}
+
+template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI = false>
 __device__ __forceinline__ void force_pair(const KParams& K, ForceOwn<R>& o, R dx, R dy, R r2,
-                                           R vjx, R vjy, R Qj, R Aj, R psij, R uj) {
+                                           R vjx, R vjy, R Qj, R Aj, R psij, R uj,
+                                           const Tau3<R>& wj = Tau3<R>{}) {
   const R m = (R)K.mass;
   const R q = fsqrt(r2) * (R)K.invh;
   const R tq = q - (R)2;
@@ -324,6 +361,7 @@ __device__ __forceinline__ void force_pair(const KParams& K, ForceOwn<R>& o, R d
       coef = coef + ((R)K.av_beta * phi - (R)K.av_alpha * (R)K.c0) * phi * inv_mrho;
     }
   }
+  if (VI) visc_pair(K, o, m * F, dx, dy, Qj, wj);
   if (PRESS || AV) {
     const R G = m * F * coef;
     o.fx = o.fx - G * dx;
@@ -348,7 +386,7 @@ __device__ __forceinline__ void force_pair(const KParams& K, ForceOwn<R>& o, R d
   }
 }
 
-template <typename R, bool PRESS, bool XS>
+template <typename R, bool PRESS, bool XS, bool VI = false>
 __device__ __forceinline__ void force_end(St<R>& S, const KParams& K, int i, const ForceOwn<R>& o) {
   using R2 = typename V2<R>::type;
   if (PRESS) {
@@ -359,7 +397,8 @@ __device__ __forceinline__ void force_end(St<R>& S, const KParams& K, int i, con
     acc.x = o.fx + ex;
     acc.y = o.fy + ey;
     S.a[i] = acc;
-    S.du[i] = o.du.get();
+    // du = tds/dt + the pair sums (internal_force.f:433-435)
+    S.du[i] = VI ? o.du.get() + o.tds : o.du.get();
   }
   if (XS) {
     R2 w;
@@ -408,13 +447,13 @@ __device__ __forceinline__ void density_direct(St<R>& S, const KParams& K, int i
   dens_end<R, SUM>(S, K, i, o);
 }
 
-template <typename R, bool PRESS, bool AV, bool AH, bool XS>
+template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI = false>
 __device__ __forceinline__ void force_direct(St<R>& S, const KParams& K, int i,
                                              const uint32_t* __restrict__ skey,
                                              const int* __restrict__ cstart) {
   const auto pi = load_pos(S, i);
   ForceOwn<R> o;
-  force_begin(S, K, i, o);
+  force_begin<R, VI>(S, K, i, o);
   const Rows rw = rows_of(K, skey[i], cstart);
   const R cut2 = cutoff2<R>(K);
   for (int k = 0; k < 3; ++k)
@@ -428,9 +467,10 @@ __device__ __forceinline__ void force_direct(St<R>& S, const KParams& K, int i,
       const auto vj = S.v[t];
       R Qj, Aj;
       q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
-      force_pair<R, PRESS, AV, AH, XS>(K, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t], u_of(S, t));
+      force_pair<R, PRESS, AV, AH, XS, VI>(K, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t],
+                                           u_of(S, t), VI ? tau_of(S, t) : Tau3<R>{});
     }
-  force_end<R, PRESS, XS>(S, K, i, o);
+  force_end<R, PRESS, XS, VI>(S, K, i, o);
 }
 
 template <typename R, bool SUM>
@@ -443,15 +483,15 @@ __global__ __launch_bounds__(BLOCK) void k_density(St<R> S, KParams K,
   density_direct<R, SUM>(S, K, i, skey, cstart, nbm);
 }
 
-// Tait EOS (eos.f90:255-283) on rate particles, then the art_heat
-// coefficient from divv = (raw divergence sum) / rho.
+// Tait EOS (eos.f90:255-283) or the linear EOS on rate particles, then the
+// art_heat coefficient from divv = (raw divergence sum) / rho.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_eos(St<R> S, KParams K) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
   R rho = S.rho[i];
   if (is_rate(K, S.id[i])) {
-    R p = tait(K, rho);
+    R p = K.eos == 1 ? linear_p(K, rho) : tait(K, rho);
     if (p < (R)K.pmin) {
       p = (R)0;
       rho = (sizeof(R) == 4) ? (R)0 : (R)K.rho0;
@@ -466,13 +506,87 @@ __global__ __launch_bounds__(BLOCK) void k_eos(St<R> S, KParams K) {
   S.psi[i] = (R)K.psi0 + (R)K.g2h2 * (ad - dv);
 }
 
-template <typename R, bool PRESS, bool AV, bool AH, bool XS>
+template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI>
 __global__ __launch_bounds__(BLOCK) void k_force(St<R> S, KParams K,
                                                  const uint32_t* __restrict__ skey,
                                                  const int* __restrict__ cstart) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
-  force_direct<R, PRESS, AV, AH, XS>(S, K, i, skey, cstart);
+  force_direct<R, PRESS, AV, AH, XS, VI>(S, K, i, skey, cstart);
+}
+
+// ------------------------------------------------------------------------
+// shear tensor of the physical viscosity (internal_force.f:73-142), with
+// dv = v_j - v_i, grad_i W = F (x_i - x_j) and the post-EOS densities:
+//   T_xx = sum m/rho_j 2/3 (2 dvx dWx - dvy dWy)
+//   T_yy = sum m/rho_j 2/3 (2 dvy dWy - dvx dWx)
+//   T_xy = sum m/rho_j (dvx dWy + dvy dWx)
+// then tds/dt = eta/(2 rho) (Txx^2 + Tyy^2 + 2 Txy^2) (:144-169).
+// ------------------------------------------------------------------------
+template <typename R> struct TensOwn {
+  R xx = 0, yy = 0, xy = 0;
+};
+
+template <typename R>
+__device__ __forceinline__ void tens_pair(const KParams& K, TensOwn<R>& o, R dx, R dy, R r2,
+                                          R dvx, R dvy, R Qj) {
+  const R q = fsqrt(r2) * (R)K.invh;
+  const R tq = q - (R)2;
+  const R F = (R)K.C1 * (tq * tq * tq);
+  const R dwx = F * dx, dwy = F * dy;
+  const R mq = (R)K.mass * Qj;
+  const R a = dvx * dwx, b = dvy * dwy;
+  const R hxx = (R)(2.0 / 3.0) * ((R)2 * a - b);
+  const R hyy = (R)(2.0 / 3.0) * ((R)2 * b - a);
+  const R hxy = dvx * dwy + dvy * dwx;
+  o.xx = o.xx + mq * hxx;
+  o.yy = o.yy + mq * hyy;
+  o.xy = o.xy + mq * hxy;
+}
+
+template <typename R>
+__device__ __forceinline__ void tens_end(St<R>& S, const KParams& K, int i, R Qi,
+                                         const TensOwn<R>& o) {
+  const R eta = (R)K.eta;
+  const R tt = o.xx * o.xx + o.yy * o.yy + (R)2 * (o.xy * o.xy);
+  const R f = K.pa_sph == 0 ? eta : eta * Qi * Qi;
+  R* w = S.tau + 4 * (size_t)i;
+  w[0] = f * o.xx;
+  w[1] = f * o.yy;
+  w[2] = f * o.xy;
+  w[3] = (R)0.5 * eta * Qi * tt;
+}
+
+template <typename R>
+__device__ __forceinline__ void tensor_direct(St<R>& S, const KParams& K, int i,
+                                              const uint32_t* __restrict__ skey,
+                                              const int* __restrict__ cstart) {
+  const auto pi = load_pos(S, i);
+  const auto vi = S.v[i];
+  const Rows rw = rows_of(K, skey[i], cstart);
+  const R cut2 = cutoff2<R>(K);
+  TensOwn<R> o;
+  for (int k = 0; k < 3; ++k)
+    for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
+      if (t == i) continue;
+      const auto pj = load_pos(S, t);
+      R dx, dy;
+      delta(pi, pj, K, dx, dy);
+      const R r2 = dx * dx + dy * dy;
+      if (!(r2 < cut2)) continue;
+      const auto vj = S.v[t];
+      tens_pair(K, o, dx, dy, r2, vj.x - vi.x, vj.y - vi.y, (R)1 / dens(K, S.rho[t]));
+    }
+  tens_end(S, K, i, (R)1 / dens(K, S.rho[i]), o);
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_tensor(St<R> S, KParams K,
+                                                  const uint32_t* __restrict__ skey,
+                                                  const int* __restrict__ cstart) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= K.n) return;
+  tensor_direct(S, K, i, skey, cstart);
 }
 
 // ------------------------------------------------------------------------

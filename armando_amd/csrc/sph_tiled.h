@@ -57,11 +57,42 @@ template <int CAP> struct DensTile<double, CAP> {
   __device__ __forceinline__ double2 vel(int l) const { return v[l]; }
 };
 
-template <typename R, int CAP> struct ForceTile;
-template <int CAP> struct ForceTile<float, CAP> {
+// Neighbour W of the physical viscosity, staged only when VI.
+template <typename R, int CAP, bool VI> struct TauTile {
+  __device__ __forceinline__ void put(int, const St<R>&, int) {}
+  __device__ __forceinline__ Tau3<R> get(int) const { return Tau3<R>{}; }
+};
+template <int CAP> struct TauTile<float, CAP, true> {
+  float4 w[CAP];
+  __device__ __forceinline__ void put(int l, const St<float>& S, int t) {
+    const float4 q = reinterpret_cast<const float4*>(S.tau)[t];
+    w[l] = q;
+  }
+  __device__ __forceinline__ Tau3<float> get(int l) const {
+    const float4 q = w[l];
+    return {q.x, q.y, q.z};
+  }
+};
+template <int CAP> struct TauTile<double, CAP, true> {
+  double2 w01[CAP];
+  double w2[CAP];
+  __device__ __forceinline__ void put(int l, const St<double>& S, int t) {
+    const double* q = S.tau + 4 * (size_t)t;
+    w01[l] = make_double2(q[0], q[1]);
+    w2[l] = q[2];
+  }
+  __device__ __forceinline__ Tau3<double> get(int l) const {
+    const double2 q = w01[l];
+    return {q.x, q.y, w2[l]};
+  }
+};
+
+template <typename R, int CAP, bool VI = false> struct ForceTile;
+template <int CAP, bool VI> struct ForceTile<float, CAP, VI> {
   float4 a[CAP];  // offx, offy, vx, vy
   float4 b[CAP];  // Q = 1/rho, A = p/rho, psi, u
   uint32_t c[CAP];
+  TauTile<float, CAP, VI> tw;
   __device__ __forceinline__ void put(int l, const St<float>& S, const KParams& K, int t) {
     const float2 o = S.off[t];
     const float2 v = S.v[t];
@@ -70,6 +101,7 @@ template <int CAP> struct ForceTile<float, CAP> {
     q_and_a(K, S.rho[t], S.p[t], Q, A);
     b[l] = make_float4(Q, A, S.psi[t], S.uf[t]);
     c[l] = S.cxy[t];
+    tw.put(l, S, t);
   }
   __device__ __forceinline__ PosF pos(int l) const {
     const uint32_t cc = c[l];
@@ -81,11 +113,13 @@ template <int CAP> struct ForceTile<float, CAP> {
                                        float dy, float r2) const {
     const float4 q = a[l];
     const float4 w = b[l];
-    force_pair<float, PRESS, AV, AH, XS>(K, o, dx, dy, r2, q.z, q.w, w.x, w.y, w.z, w.w);
+    force_pair<float, PRESS, AV, AH, XS, VI>(K, o, dx, dy, r2, q.z, q.w, w.x, w.y, w.z, w.w,
+                                             tw.get(l));
   }
 };
-template <int CAP> struct ForceTile<double, CAP> {
+template <int CAP, bool VI> struct ForceTile<double, CAP, VI> {
   double2 x[CAP], v[CAP], rp[CAP], pu[CAP];  // rp = (Q, A), pu = (psi, u)
+  TauTile<double, CAP, VI> tw;
   __device__ __forceinline__ void put(int l, const St<double>& S, const KParams& K, int t) {
     x[l] = S.xy[t];
     v[l] = S.v[t];
@@ -93,6 +127,7 @@ template <int CAP> struct ForceTile<double, CAP> {
     q_and_a(K, S.rho[t], S.p[t], Q, A);
     rp[l] = make_double2(Q, A);
     pu[l] = make_double2(S.psi[t], S.u[t]);
+    tw.put(l, S, t);
   }
   __device__ __forceinline__ PosD pos(int l) const {
     const double2 q = x[l];
@@ -104,8 +139,20 @@ template <int CAP> struct ForceTile<double, CAP> {
     const double2 vv = v[l];
     const double2 a = rp[l];
     const double2 b = pu[l];
-    force_pair<double, PRESS, AV, AH, XS>(K, o, dx, dy, r2, vv.x, vv.y, a.x, a.y, b.x, b.y);
+    force_pair<double, PRESS, AV, AH, XS, VI>(K, o, dx, dy, r2, vv.x, vv.y, a.x, a.y, b.x, b.y,
+                                              tw.get(l));
   }
+};
+
+// Positions, velocities and Q = 1/rho for the shear tensor pass.
+template <typename R, int CAP> struct TensTile {
+  DensTile<R, CAP> d;
+  R q[CAP];
+  __device__ __forceinline__ void put(int l, const St<R>& S, const KParams& K, int t) {
+    d.put(l, S, K, t);
+    q[l] = (R)1 / dens(K, S.rho[t]);
+  }
+  __device__ __forceinline__ auto pos(int l) const { return d.pos(l); }
 };
 
 // ---- segment geometry ------------------------------------------------------
@@ -270,12 +317,12 @@ __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
   }
 }
 
-template <typename R, bool PRESS, bool AV, bool AH, bool XS, int BLK, int CW, int CAP>
-__global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
-                                                     const uint32_t* __restrict__ skey,
-                                                     const int* __restrict__ cstart, int nseg,
-                                                     const uint4* __restrict__ nbm) {
-  __shared__ ForceTile<R, CAP> T;
+template <typename R, int BLK, int CW, int CAP>
+__global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
+                                                      const uint32_t* __restrict__ skey,
+                                                      const int* __restrict__ cstart, int nseg,
+                                                      const uint4* __restrict__ nbm) {
+  __shared__ TensTile<R, CAP> T;
   __shared__ int cs[3][CW + 3];
   const Segment<CW> g = segment_of<CW>(K, cstart, nseg, cs);
   if (g.O0 >= g.O1) return;
@@ -290,13 +337,57 @@ __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
     const int i = i0 + threadIdx.x;
     if (i >= g.O1) continue;
     if (!tiled) {
-      force_direct<R, PRESS, AV, AH, XS>(S, K, i, skey, cstart);
+      tensor_direct(S, K, i, skey, cstart);
+      continue;
+    }
+    const int self = g.lo[1] + (i - g.rs[1]);
+    const auto pi = T.pos(self);
+    const auto vi = T.d.vel(self);
+    TensOwn<R> o;
+    auto visit = [&](int l) {
+      R dx, dy;
+      delta(pi, T.pos(l), K, dx, dy);
+      const R r2 = dx * dx + dy * dy;
+      const auto vj = T.d.vel(l);
+      tens_pair(K, o, dx, dy, r2, vj.x - vi.x, vj.y - vi.y, T.q[l]);
+    };
+    const uint32_t key = skey[i];
+    const uint4 nm = nbm ? nbm[i] : make_uint4(0u, 0u, 0u, 0u);
+    if (nm.w)
+      for_masked<CW>(K, g, cs, key, nm, visit);
+    else
+      for_neighbours<R, CW>(K, g, cs, T, pi, i, key, visit);
+    tens_end(S, K, i, T.q[self], o);
+  }
+}
+
+template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI, int BLK, int CW, int CAP>
+__global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
+                                                     const uint32_t* __restrict__ skey,
+                                                     const int* __restrict__ cstart, int nseg,
+                                                     const uint4* __restrict__ nbm) {
+  __shared__ ForceTile<R, CAP, VI> T;
+  __shared__ int cs[3][CW + 3];
+  const Segment<CW> g = segment_of<CW>(K, cstart, nseg, cs);
+  if (g.O0 >= g.O1) return;
+  const bool tiled = g.total <= CAP;
+  if (tiled) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      for (int q = threadIdx.x; q < g.rl[k]; q += BLK) T.put(g.lo[k] + q, S, K, g.rs[k] + q);
+  }
+  __syncthreads();
+  for (int i0 = g.O0; i0 < g.O1; i0 += BLK) {
+    const int i = i0 + threadIdx.x;
+    if (i >= g.O1) continue;
+    if (!tiled) {
+      force_direct<R, PRESS, AV, AH, XS, VI>(S, K, i, skey, cstart);
       continue;
     }
     const int self = g.lo[1] + (i - g.rs[1]);
     const auto pi = T.pos(self);
     ForceOwn<R> o;
-    force_begin(S, K, i, o);
+    force_begin<R, VI>(S, K, i, o);
     auto visit = [&](int l) {
       R dx, dy;
       delta(pi, T.pos(l), K, dx, dy);
@@ -309,7 +400,7 @@ __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
       for_masked<CW>(K, g, cs, key, nm, visit);
     else
       for_neighbours<R, CW>(K, g, cs, T, pi, i, key, visit);
-    force_end<R, PRESS, XS>(S, K, i, o);
+    force_end<R, PRESS, XS, VI>(S, K, i, o);
   }
 }
 

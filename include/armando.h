@@ -52,6 +52,15 @@ extern "C" {
                           [ArmandoList/armando2Dcpu.cu:832-907] */
 #define ARM_GRAVITY 16 /* body force (gx, gy) on real particles
                           [ArmandoList/armando2Dcpu.cu:937-958] */
+#define ARM_VISC 32    /* visc -> physical viscosity in int_force: shear
+                          tensor, viscous force, tds/dt
+                          [CERNsource/internal_force.f:73-171, 196-430] */
+
+/* equations of state */
+#define ARM_EOS_TAIT 0   /* p = rho0 c0^2/7 ((rho/rho0)^7 - 1), c = c0
+                            [FORTRANsource/eos.f90:255-283] */
+#define ARM_EOS_LINEAR 1 /* p = c0^2 rho + eos_p0, c = c0 (eos_poly of
+                            CERNsource/eos.f with only the a1 term) */
 
 typedef struct arm_params {
   int precision;      /* ARM_FP64 or ARM_FP32 */
@@ -71,6 +80,9 @@ typedef struct arm_params {
      [ArmandoList/armando2Dcpu.cu:2080-2101]; cell >= 2h required */
   double ox, oy, cell;
   int nx, ny;
+  double visc_eta; /* dynamic viscosity for ARM_VISC (eta of single_step.f:126-184) */
+  int eos;         /* ARM_EOS_*; both clamp p < pmin to p = 0, rho = rho0 */
+  double eos_p0;   /* ARM_EOS_LINEAR pressure offset */
 } arm_params;
 
 typedef struct arm_ctx arm_ctx;

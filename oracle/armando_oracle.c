@@ -289,13 +289,38 @@ void orc_con_density(const orc_params* P, int nrate, int niac, const int* pi, co
 void orc_int_force(const orc_params* P, int nt, int niac, const int* pi, const int* pj,
                    const double* x, const double* y, const double* vx, const double* vy,
                    const double* rho, const double* p, double* ax, double* ay, double* du) {
-  /* internal_force.f:57-71 (zeroing), :178-259 (pa_sph 0), :309-430
-     (pa_sph 1), :433-435 (du = tdsdt + du/2); no physical viscosity */
+  /* internal_force.f:57-71 (zeroing), :73-142 (shear tensor, visc), :144-169
+     (tdsdt), :178-259 (pa_sph 0), :309-430 (pa_sph 1), :433-435 (du) */
   const double m = P->mass;
+  const int visc = (P->flags & ORC_VISC) != 0;
+  const double eta = P->visc_eta;
+  double *txx = NULL, *tyy = NULL, *txy = NULL;
   for (int i = 0; i < nt; ++i) {
     ax[i] = 0.0;
     ay[i] = 0.0;
     du[i] = 0.0;
+  }
+  if (visc) {
+    txx = (double*)calloc((size_t)(nt > 0 ? nt : 1), sizeof(double));
+    tyy = (double*)calloc((size_t)(nt > 0 ? nt : 1), sizeof(double));
+    txy = (double*)calloc((size_t)(nt > 0 ? nt : 1), sizeof(double));
+    for (int k = 0; k < niac; ++k) {
+      const int i = pi[k], j = pj[k];
+      double dx, dy, dr2, w, dwx, dwy;
+      pair_geom(P, i, j, x, y, &dx, &dy, &dr2, &w, &dwx, &dwy);
+      const double dvx = vx[j] - vx[i], dvy = vy[j] - vy[i];
+      double hxx = 2.0 * dvx * dwx - dvy * dwy;
+      double hyy = 2.0 * dvy * dwy - dvx * dwx;
+      const double hxy = dvx * dwy + dvy * dwx;
+      hxx = 2.0 / 3.0 * hxx;
+      hyy = 2.0 / 3.0 * hyy;
+      txx[i] = txx[i] + m * hxx / rho[j];
+      tyy[i] = tyy[i] + m * hyy / rho[j];
+      txy[i] = txy[i] + m * hxy / rho[j];
+      txx[j] = txx[j] + m * hxx / rho[i];
+      tyy[j] = tyy[j] + m * hyy / rho[i];
+      txy[j] = txy[j] + m * hxy / rho[i];
+    }
   }
   for (int k = 0; k < niac; ++k) {
     const int i = pi[k], j = pj[k];
@@ -305,30 +330,51 @@ void orc_int_force(const orc_params* P, int nt, int niac, const int* pi, const i
     if (P->pa_sph == 0) {
       hv = -(p[i] + p[j]) * dwx;
       he = he + (vx[j] - vx[i]) * hv;
+      if (visc)
+        hv = hv + (eta * txx[i] + eta * txx[j]) * dwx + (eta * txy[i] + eta * txy[j]) * dwy;
       hv = hv / (rho[i] * rho[j]);
       ax[i] = ax[i] + m * hv;
       ax[j] = ax[j] - m * hv;
       hv = -(p[i] + p[j]) * dwy;
       he = he + (vy[j] - vy[i]) * hv;
+      if (visc)
+        hv = hv + (eta * txy[i] + eta * txy[j]) * dwx + (eta * tyy[i] + eta * tyy[j]) * dwy;
       hv = hv / (rho[i] * rho[j]);
       ay[i] = ay[i] + m * hv;
       ay[j] = ay[j] - m * hv;
       he = he / (rho[i] * rho[j]);
     } else {
-      const double a = p[i] / (rho[i] * rho[i]) + p[j] / (rho[j] * rho[j]);
+      const double ri2 = rho[i] * rho[i], rj2 = rho[j] * rho[j];
+      const double a = p[i] / ri2 + p[j] / rj2;
       hv = -a * dwx;
       he = he + (vx[j] - vx[i]) * hv;
+      if (visc)
+        hv = hv + (eta * txx[i] / ri2 + eta * txx[j] / rj2) * dwx +
+             (eta * txy[i] / ri2 + eta * txy[j] / rj2) * dwy;
       ax[i] = ax[i] + m * hv;
       ax[j] = ax[j] - m * hv;
       hv = -a * dwy;
       he = he + (vy[j] - vy[i]) * hv;
+      if (visc)
+        hv = hv + (eta * txy[i] / ri2 + eta * txy[j] / rj2) * dwx +
+             (eta * tyy[i] / ri2 + eta * tyy[j] / rj2) * dwy;
       ay[i] = ay[i] + m * hv;
       ay[j] = ay[j] - m * hv;
     }
     du[i] = du[i] + m * he;
     du[j] = du[j] + m * he;
   }
-  for (int i = 0; i < nt; ++i) du[i] = 0.0 + 0.5 * du[i];
+  for (int i = 0; i < nt; ++i) {
+    double tds = 0.0;
+    if (visc) {
+      tds = txx[i] * txx[i] + tyy[i] * tyy[i] + 2.0 * txy[i] * txy[i];
+      tds = 0.5 * eta / rho[i] * tds;
+    }
+    du[i] = tds + 0.5 * du[i];
+  }
+  free(txx);
+  free(tyy);
+  free(txy);
 }
 
 void orc_art_visc(const orc_params* P, int niac, const int* pi, const int* pj,
@@ -440,6 +486,24 @@ void orc_eos_tait(const orc_params* P, int n, double* rho, double* p, double* c)
   }
 }
 
+void orc_eos(const orc_params* P, int n, double* rho, double* p, double* c) {
+  if (P->eos != ORC_EOS_LINEAR) {
+    orc_eos_tait(P, n, rho, p, c);
+    return;
+  }
+  /* eos.f eos_poly with only a1 = rho0 c0^2 and an offset:
+     p = c0^2 rho + p0; the same clamp as the Tait EOS */
+  const double c2 = P->c0 * P->c0;
+  for (int i = 0; i < n; ++i) {
+    p[i] = c2 * rho[i] + P->eos_p0;
+    c[i] = P->c0;
+    if (p[i] < P->pmin) {
+      p[i] = 0.0;
+      rho[i] = P->rho0;
+    }
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* state, single step, integration                                      */
 /* ------------------------------------------------------------------ */
@@ -489,7 +553,7 @@ int orc_single_step(const orc_params* P, orc_state* s, int niac, const int* pi,
   } else {
     orc_sum_density(P, nr, niac, pi, pj, s->x, s->y, s->rho);
   }
-  orc_eos_tait(P, nr, s->rho, s->p, s->c);
+  orc_eos(P, nr, s->rho, s->p, s->c);
   orc_int_force(P, n, niac, pi, pj, s->x, s->y, s->vx, s->vy, s->rho, s->p, s->ax, s->ay,
                 s->du);
   if (P->flags & ORC_ART_VISC)
@@ -592,12 +656,49 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
     }
     s->divv[i] = -dr; /* divided by rho after the EOS */
   }
-  orc_eos_tait(P, nr, s->rho, s->p, s->c);
+  orc_eos(P, nr, s->rho, s->p, s->c);
   double* psi = (double*)malloc(sizeof(double) * (size_t)n);
   for (int i = 0; i < n; ++i) {
     const double dv = s->divv[i] / s->rho[i];
     s->divv[i] = dv;
     psi[i] = P->ah_g1 * h * s->c[i] + P->ah_g2 * h * h * (fabs(dv) - dv);
+  }
+  /* shear tensor (internal_force.f:73-142) with the post-EOS densities */
+  const int visc = (P->flags & ORC_VISC) != 0;
+  const double eta = P->visc_eta;
+  double* tt = visc ? (double*)calloc((size_t)n * 3, sizeof(double)) : NULL;
+  if (visc) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      const int key = C.key[i];
+      const int cx = key % nx, cy = key / nx;
+      const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < nx ? cx + 1 : nx - 1;
+      double txx = 0.0, tyy = 0.0, txy = 0.0;
+      for (int ry = cy - 1; ry <= cy + 1; ++ry) {
+        if (ry < 0 || ry >= ny) continue;
+        const int t0 = C.start[ry * nx + x0], t1 = C.start[ry * nx + x1 + 1];
+        for (int t = t0; t < t1; ++t) {
+          const int j = C.sorted[t];
+          if (j == i) continue;
+          const double dx = x[i] - x[j], dy = y[i] - y[j];
+          double r2 = dx * dx;
+          r2 = r2 + dy * dy;
+          if (!(r2 < cut2)) continue;
+          double dwx, dwy;
+          kern(P, sqrt(r2), dx, dy, &dwx, &dwy);
+          const double dvx = vx[j] - vx[i], dvy = vy[j] - vy[i];
+          const double hxx = 2.0 / 3.0 * (2.0 * dvx * dwx - dvy * dwy);
+          const double hyy = 2.0 / 3.0 * (2.0 * dvy * dwy - dvx * dwx);
+          const double hxy = dvx * dwy + dvy * dwx;
+          txx = txx + m * hxx / s->rho[j];
+          tyy = tyy + m * hyy / s->rho[j];
+          txy = txy + m * hxy / s->rho[j];
+        }
+      }
+      tt[3 * i] = txx;
+      tt[3 * i + 1] = tyy;
+      tt[3 * i + 2] = txy;
+    }
   }
   const int grav = (P->flags & ORC_GRAVITY) != 0;
   const int f_av = (P->flags & ORC_ART_VISC) != 0;
@@ -640,6 +741,24 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
         fx = fx - m * a * dwx;
         fy = fy - m * a * dwy;
         due = due + m * a * vdw;
+        if (visc) {
+          const double* ti = tt + 3 * i;
+          const double* tj = tt + 3 * j;
+          double sxx, syy, sxy;
+          if (P->pa_sph == 0) {
+            const double rr = rho[i] * rho[j];
+            sxx = eta * (ti[0] + tj[0]) / rr;
+            syy = eta * (ti[1] + tj[1]) / rr;
+            sxy = eta * (ti[2] + tj[2]) / rr;
+          } else {
+            const double ri2 = rho[i] * rho[i], rj2 = rho[j] * rho[j];
+            sxx = eta * ti[0] / ri2 + eta * tj[0] / rj2;
+            syy = eta * ti[1] / ri2 + eta * tj[1] / rj2;
+            sxy = eta * ti[2] / ri2 + eta * tj[2] / rj2;
+          }
+          fx = fx + m * (sxx * dwx + sxy * dwy);
+          fy = fy + m * (sxy * dwx + syy * dwy);
+        }
         const double mrho = 0.5 * (rho[i] + rho[j]);
         if (f_av) {
           double vr = dvx * dx;
@@ -670,10 +789,16 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
     const double ey = (grav && i < np) ? P->gy : 0.0;
     s->ax[i] = fx + ex + avfx;
     s->ay[i] = fy + ey + avfy;
-    s->du[i] = 0.5 * due + avdu + ahdu;
+    double tds = 0.0;
+    if (visc) {
+      const double* ti = tt + 3 * i;
+      tds = 0.5 * eta / rho[i] * (ti[0] * ti[0] + ti[1] * ti[1] + 2.0 * ti[2] * ti[2]);
+    }
+    s->du[i] = (tds + 0.5 * due) + avdu + ahdu;
     s->avx[i] = xsx;
     s->avy[i] = xsy;
   }
+  free(tt);
   free(psi);
   free_cells(&C);
   return 0;
@@ -734,7 +859,7 @@ int orc_run(const orc_params* P, orc_state* s, int its0, int nsteps, int gather)
   int cap = 0, niac = 0;
   int *pi = NULL, *pj = NULL;
   if (!gather) {
-    cap = s->n * 16 + 64;
+    cap = s->n * 64 + 64;
     pi = (int*)malloc(sizeof(int) * (size_t)cap);
     pj = (int*)malloc(sizeof(int) * (size_t)cap);
   }

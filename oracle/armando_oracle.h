@@ -64,7 +64,14 @@ enum {
   ORC_ART_HEAT = 2,  /* art_heat.f */
   ORC_AV_VEL = 4,    /* av_vel.f (XSPH) */
   ORC_DBC = 8,       /* walls integrate density and energy (armando2Dcpu.cu:832-907) */
-  ORC_GRAVITY = 16   /* body force on fluid (armando2Dcpu.cu:937-958) */
+  ORC_GRAVITY = 16,  /* body force on fluid (armando2Dcpu.cu:937-958) */
+  ORC_VISC = 32      /* physical viscosity: shear tensor, viscous force and
+                        tds/dt of internal_force.f:73-171, uniform eta */
+};
+
+enum {
+  ORC_EOS_TAIT = 0,  /* p = rho0 c0^2/7 ((rho/rho0)^7 - 1), eos.f90:274 */
+  ORC_EOS_LINEAR = 1 /* p = c0^2 rho + eos_p0 (the a1 term of eos_poly, eos.f) */
 };
 
 typedef struct {
@@ -84,6 +91,9 @@ typedef struct {
      ix = (int)((x - ox)/cell) */
   double ox, oy, cell;
   int nx, ny;
+  double visc_eta; /* dynamic viscosity (ORC_VISC) */
+  int eos;         /* ORC_EOS_* ; c = c0 for both, clamp p < pmin -> 0, rho0 */
+  double eos_p0;   /* ORC_EOS_LINEAR pressure offset */
 } orc_params;
 
 /* Particle state, SoA, FP64. */
@@ -126,8 +136,10 @@ void orc_sum_density(const orc_params* P, int nrate, int niac, const int* pi, co
 void orc_con_density(const orc_params* P, int nrate, int niac, const int* pi, const int* pj,
                      const double* x, const double* y, const double* vx, const double* vy,
                      double* drho);
-/* int_force pressure part; zeroes ax, ay, du for [0, nt) first, finishes
-   with du = 0.5*du (tdsdt = 0 without physical viscosity). */
+/* int_force; zeroes ax, ay, du for [0, nt) first.  With ORC_VISC: the
+   shear tensor sums (internal_force.f:73-142), tds/dt = eta/(2 rho) T:T
+   (:144-169) and the viscous force (eta_i T_i + eta_j T_j).grad W; finishes
+   with du = tdsdt + 0.5*du (:433-435). */
 void orc_int_force(const orc_params* P, int nt, int niac, const int* pi, const int* pj,
                    const double* x, const double* y, const double* vx, const double* vy,
                    const double* rho, const double* p, double* ax, double* ay, double* du);
@@ -146,6 +158,8 @@ void orc_av_vel(const orc_params* P, int niac, const int* pi, const int* pj,
                 const double* rho, double* avx, double* avy);
 /* Tait EOS on [0, n): p, c, and the pmin clamp (p = 0, rho = rho0). */
 void orc_eos_tait(const orc_params* P, int n, double* rho, double* p, double* c);
+/* The selected EOS (P->eos) with the same clamp. */
+void orc_eos(const orc_params* P, int n, double* rho, double* p, double* c);
 
 /* ---- state and step ---------------------------------------------------- */
 orc_state* orc_state_new(int n, int np);

@@ -13,7 +13,8 @@ LIB_PATH = os.path.join(HERE, "_build", "liborc.so")
 REF_PATH = os.path.join(HERE, "_ref", "libarmref.so")
 
 K_WENDLAND2D, K_CUBIC2D, K_CUBIC3D = 0, 1, 2
-ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY = 1, 2, 4, 8, 16
+ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC = 1, 2, 4, 8, 16, 32
+EOS_TAIT, EOS_LINEAR = 0, 1
 
 
 class Params(C.Structure):
@@ -28,6 +29,7 @@ class Params(C.Structure):
         ("dt", C.c_double),
         ("ox", C.c_double), ("oy", C.c_double), ("cell", C.c_double),
         ("nx", C.c_int), ("ny", C.c_int),
+        ("visc_eta", C.c_double), ("eos", C.c_int), ("eos_p0", C.c_double),
     ]
 
 
@@ -73,6 +75,7 @@ def lib():
                                 _dp, _dp, _dp]),
         "orc_av_vel": (None, [_PP, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "orc_eos_tait": (None, [_PP, C.c_int, _dp, _dp, _dp]),
+        "orc_eos": (None, [_PP, C.c_int, _dp, _dp, _dp]),
         "orc_state_new": (C.POINTER(State), [C.c_int, C.c_int]),
         "orc_state_free": (None, [C.POINTER(State)]),
         "orc_single_step": (C.c_int, [_PP, C.POINTER(State), C.c_int, _ip, _ip]),

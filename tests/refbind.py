@@ -143,3 +143,45 @@ def pair_routines(x, y, vx, vy, rho, p, u, c, h, mass, pi, pj):
         out["xs_x"], out["xs_y"] = av[:, 0].copy(), av[:, 1].copy()
         return out
     return run_big_stack(f)
+
+
+def int_force_visc(x, y, vx, vy, rho, p, h, mass, eta, pi, pj, pa_sph):
+    """internal_force.f with visc = .true. and a uniform eta, every particle
+    counted (nt = n): returns dvdt x, dvdt y and dudt."""
+    def f():
+        n = len(x)
+        X = vec3(x, y)
+        V = vec3(vx, vy)
+        H = np.full(n, h)
+        M = np.full(n, mass)
+        P1 = np.ascontiguousarray(pi + 1, np.int32)
+        P2 = np.ascontiguousarray(pj + 1, np.int32)
+        R = np.ascontiguousarray(rho, np.float64)
+        Pp = np.ascontiguousarray(p, np.float64)
+        mat = np.ones(n, np.int32)
+        E = np.full(n, eta)
+        dt = C.c_double(1e-4)
+        visc = C.c_int(1)
+        dv = np.zeros((n, 3))
+        du = np.zeros(n)
+        tds = np.zeros(n)
+        lib().int_force_(_i(n), _p(X), _p(V), _p(mat), _p(H), _p(M), _p(R), _p(Pp), _p(E),
+                         _p(dv), _p(du), _p(tds), _i(len(pi)), _p(P1), _p(P2), C.byref(dt),
+                         _i(0), _i(pa_sph), C.byref(visc))
+        return dv[:, 0].copy(), dv[:, 1].copy(), du, tds
+    return run_big_stack(f)
+
+
+def eos_poly(rho, u, rho0, a1, a2, a3, b0, b1, t1, t2, pmin):
+    """eos.f eos_poly per particle."""
+    def f():
+        p = np.zeros(len(rho))
+        for k in range(len(rho)):
+            r = C.c_double(rho[k])
+            e = C.c_double(u[k])
+            out = C.c_double(0.0)
+            args = [C.c_double(v) for v in (rho0, a1, a2, a3, b0, b1, t1, t2, pmin)]
+            lib().eos_poly_(C.byref(r), C.byref(e), C.byref(out), *[C.byref(a) for a in args])
+            p[k] = out.value
+        return p
+    return run_big_stack(f)

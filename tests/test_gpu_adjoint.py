@@ -42,6 +42,16 @@ def gpu_rollout(c, obj, nsteps, ckpt, alpha=None):
     return out
 
 
+def gpu_objective(c, obj, nsteps, ckpt, alpha=None):
+    c2 = dict(c)
+    if alpha is not None:
+        c2["av_alpha"] = alpha
+    sim = armando_amd.Simulation(c2)
+    J = sim.rollout_forward(nsteps, ckpt, obj)[0]
+    sim.close()
+    return J
+
+
 def test_adjoint_matches_oracle():
     c, obj = adjoint_case()
     steps = 20
@@ -127,3 +137,37 @@ def test_torch_autograd_function_backward_is_the_adjoint():
     assert alpha.grad.item() == pytest.approx(3.0 * ref[1], rel=1e-12)
     assert np.allclose(v0.grad.numpy(), 3.0 * ref[2], rtol=1e-12, atol=0)
     loss.close()
+
+
+def test_adjoint_100k_400_steps_against_central_differences_on_16_directions():
+    """The 100k adjoint workload at full size: 400 steps, checkpoint every 20.
+    dJ/dalpha and dJ/dv0 . d for 16 directions d ~ N(0, 1) per fluid particle
+    component (normal01 seeds 1001..1016) against central differences of the
+    GPU forward with eps = 1e-4 (relative for alpha, m/s for v0); tolerance
+    1e-3 relative."""
+    c, obj = cases.dam_break_adjoint_100k()
+    steps, ck = 400, 20
+    banner(f"100k adjoint n={c['n']} steps={steps} ckpt={ck}")
+    J, da, g, (tf, tb) = gpu_rollout(c, obj, steps, ck)
+    assert J > 0 and tb <= 3.0 * tf, (J, tf, tb)
+    a0 = c["av_alpha"]
+    e = 1e-4 * a0
+    fd = (gpu_objective(c, obj, steps, ck, alpha=a0 + e)
+          - gpu_objective(c, obj, steps, ck, alpha=a0 - e)) / (2 * e)
+    assert abs(fd - da) <= 1e-3 * abs(fd), (fd, da)
+    nf = c["np"]
+    e = 1e-4
+    worst = 0.0
+    for seed in range(1001, 1017):
+        z = cases.normal01(seed, 2 * nf)
+        d = np.zeros((c["n"], 2))
+        d[:nf, 0], d[:nf, 1] = z[:nf], z[nf:]
+        cp, cm = dict(c), dict(c)
+        cp["vx"], cp["vy"] = c["vx"] + e * d[:, 0], c["vy"] + e * d[:, 1]
+        cm["vx"], cm["vy"] = c["vx"] - e * d[:, 0], c["vy"] - e * d[:, 1]
+        fdv = (gpu_objective(cp, obj, steps, ck) - gpu_objective(cm, obj, steps, ck)) / (2 * e)
+        ad = float(np.sum(g * d))
+        rel = abs(fdv - ad) / abs(fdv)
+        worst = max(worst, rel)
+        assert rel <= 1e-3, (seed, fdv, ad)
+    banner(f"100k adjoint: alpha fd {fd:.6e} adj {da:.6e}; worst direction rel {worst:.2e}")
