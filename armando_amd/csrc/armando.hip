@@ -17,6 +17,7 @@
 
 #include "../../include/armando.h"
 #include "radix_sort.h"
+#include "cell_sort.h"
 #include "sph_kernels.h"
 #include "sph_tiled.h"
 #include "sph_adjoint.h"
@@ -56,6 +57,12 @@ struct arm_ctx {
   uint32_t* hist = nullptr;
   uint32_t* tot = nullptr;
   int ntiles = 0, sort_bits = 0, sort_passes = 0;
+  // counting sort over cells (grids of up to CS_TILE * CS_MAX_TILES cells)
+  bool counting = false;
+  int cs_tiles = 0;
+  int* cnt = nullptr;       // per-cell counters, zero between sorts
+  int* tsum = nullptr;      // per-tile sums of the counters
+  uint64_t* ent = nullptr;  // (id, slot) entries grouped by cell (cell in sk[1])
   uint32_t* skey = nullptr;  // sorted keys of the current order
   int* cstart = nullptr;
   int* d_err = nullptr;
@@ -189,8 +196,42 @@ struct PassTimer {
 
 // ---- sort -------------------------------------------------------------------
 // Sorts (key_by_id, slot_by_id) into (skey, sval); returns the value array.
-int run_sort(arm_ctx* c, const uint32_t** sval_out) {
+// Sorts (key_by_id, slot_by_id) into (cell, id) order: skey and *sval_out
+// hold the sorted keys and source slots.  The counting sort also writes
+// cell_start (*ranges_done = true); the radix sort leaves it to the reorder.
+// Where the hashing kernels put the keys for the next sort.
+KeyOut key_out(arm_ctx* c) {
+  if (c->counting) return KeyOut{nullptr, nullptr, c->key_by_id, c->cnt};
+  return KeyOut{c->key_by_id, c->slot_by_id, nullptr, nullptr};
+}
+
+// Clears the counters of the counting sort before a run of keyed steps.
+void zero_counts(arm_ctx* c) {
+  if (c->counting)
+    hipLaunchKernelGGL(k_cs_zero, dim3(std::min(1024, (c->K.ncell + CS_THREADS - 1) / CS_THREADS)),
+                       dim3(CS_THREADS), 0, c->stream, c->cnt, c->K.ncell);
+}
+
+int run_sort(arm_ctx* c, const int32_t* id_of, const uint32_t** sval_out, bool* ranges_done) {
   const int n = c->n;
+  if (c->counting) {
+    const int ncell = c->K.ncell;
+    const int nb = (n + CS_THREADS - 1) / CS_THREADS;
+    hipLaunchKernelGGL(k_cs_tile_sum, dim3(c->cs_tiles), dim3(CS_THREADS), 0, c->stream, c->cnt,
+                       ncell, c->tsum);
+    hipLaunchKernelGGL(k_cs_tile_scan, dim3(c->cs_tiles), dim3(CS_THREADS), 0, c->stream, c->cnt,
+                       ncell, n, c->tsum, c->cstart);
+    hipLaunchKernelGGL(k_cs_scatter, dim3(nb), dim3(CS_THREADS), 0, c->stream, c->key_by_id,
+                       id_of, n, c->cstart, c->cnt, c->ent, c->sk[1]);
+    hipLaunchKernelGGL(k_cs_rank, dim3(nb), dim3(CS_THREADS), 0, c->stream, c->ent, c->sk[1], n,
+                       c->cstart, c->sk[0], c->sv[0]);
+    HIPCHK(c, hipGetLastError());
+    c->skey = c->sk[0];
+    *sval_out = c->sv[0];
+    *ranges_done = true;
+    return ARM_OK;
+  }
+  *ranges_done = false;
   const int ntiles = (n + SORT_TILE - 1) / SORT_TILE;
   const uint32_t* kin = c->key_by_id;
   const uint32_t* vin = c->slot_by_id;
@@ -240,19 +281,21 @@ template <typename R> int link_list_t(arm_ctx* c) {
   St<R>& B = stB<R>(c);
   {
     PassTimer tm(c, ARM_PASS_KEYS);
+    zero_counts(c);
     hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
-                       c->key_by_id, c->slot_by_id, c->d_err);
+                       key_out(c), c->d_err);
   }
   const uint32_t* sval = nullptr;
+  bool ranges = false;
   {
     PassTimer tm(c, ARM_PASS_SORT);
-    int rc = run_sort(c, &sval);
+    int rc = run_sort(c, A.id, &sval, &ranges);
     if (rc) return rc;
   }
   {
     PassTimer tm(c, ARM_PASS_REORDER);
     hipLaunchKernelGGL(k_reorder_all<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
-                       c->K, c->skey, sval, c->cstart);
+                       c->K, c->skey, sval, c->cstart, ranges ? 0 : 1);
   }
   HIPCHK(c, hipGetLastError());
   std::swap(A, B);
@@ -388,8 +431,7 @@ template <typename R> int corrector_t(arm_ctx* c, int its, bool keys) {
   St<R>& A = stA<R>(c);
   PassTimer tm(c, ARM_PASS_INTEGRATE);
   hipLaunchKernelGGL(k_corrector<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
-                     its, keys ? c->key_by_id : nullptr, keys ? c->slot_by_id : nullptr,
-                     c->d_err);
+                     its, keys ? 1 : 0, keys ? key_out(c) : KeyOut{}, c->d_err);
   HIPCHK(c, hipGetLastError());
   return ARM_OK;
 }
@@ -401,8 +443,9 @@ template <typename R> int corrector_t(arm_ctx* c, int its, bool keys) {
 template <typename R> int begin_steps(arm_ctx* c) {
   St<R>& A = stA<R>(c);
   PassTimer tm(c, ARM_PASS_KEYS);
+  zero_counts(c);
   hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
-                     c->key_by_id, c->slot_by_id, c->d_err);
+                     key_out(c), c->d_err);
   HIPCHK(c, hipGetLastError());
   return ARM_OK;
 }
@@ -421,16 +464,17 @@ template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
   const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
   int rc;
   const uint32_t* sval = nullptr;
+  bool ranges = false;
   {
     PassTimer tm(c, ARM_PASS_SORT);
-    if ((rc = run_sort(c, &sval))) return rc;
+    if ((rc = run_sort(c, stA<R>(c).id, &sval, &ranges))) return rc;
   }
   {
     St<R>& A = stA<R>(c);
     St<R>& B = stB<R>(c);
     PassTimer tm(c, ARM_PASS_REORDER);
     hipLaunchKernelGGL(k_reorder_pred<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
-                       c->K, c->skey, sval, c->cstart, its != 1 ? 1 : 0);
+                       c->K, c->skey, sval, c->cstart, its != 1 ? 1 : 0, ranges ? 0 : 1);
     std::swap(A, B);
     c->masks_valid = false;
   }
@@ -812,7 +856,7 @@ template <typename R> int upload_t(arm_ctx* c, const double* x, const double* v,
   // FP32 form: cell + offset from the FP64 positions (the keys kernel writes them)
   if (Layout<R>::f32) {
     hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
-                       c->key_by_id, c->slot_by_id, c->d_err);
+                       KeyOut{}, c->d_err);
     HIPCHK(c, hipGetLastError());
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1019,6 +1063,17 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
   }
   if (!rc) rc = dalloc(c, &c->hist, (size_t)(1 << c->sort_bits) * c->ntiles);
   if (!rc) rc = dalloc(c, &c->tot, (size_t)(1 << c->sort_bits));
+  c->cs_tiles = (ncell + CS_TILE - 1) / CS_TILE;
+  c->counting = c->cs_tiles <= CS_MAX_TILES;
+  if (!rc && c->counting) {
+    rc = dalloc(c, &c->cnt, (size_t)ncell);
+    if (!rc) rc = dalloc(c, &c->tsum, (size_t)c->cs_tiles);
+    if (!rc) rc = dalloc(c, &c->ent, (size_t)capacity);
+    if (!rc) {
+      hipError_t ez = hipMemset(c->cnt, 0, sizeof(int) * (size_t)ncell);
+      if (ez != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipMemset: %s", hipGetErrorString(ez));
+    }
+  }
   if (!rc) rc = dalloc(c, &c->cstart, (size_t)ncell + 1);
   if (!rc) rc = dalloc(c, &c->d_err, 1);
   if (!rc) rc = dalloc(c, &c->d_part, (size_t)grid_of(capacity) * 5);
@@ -1183,7 +1238,7 @@ int arm_corrector(arm_ctx* c, int its) {
     // refresh cell + offset of the FP32 form from the FP64 accumulators
     St<float>& A = c->A32;
     hipLaunchKernelGGL(k_keys<float>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
-                       c->key_by_id, c->slot_by_id, c->d_err);
+                       KeyOut{}, c->d_err);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }

@@ -1,0 +1,193 @@
+// cell_sort.h -- counting sort of particles into hash cells for gfx950.
+//
+// Produces the same (cell, id) order as a stable sort of the cell keys
+// listed in id order, and the cell ranges as a by-product:
+//   emit_key        (in the kernels that hash positions) key_by_slot[t] and
+//                   count[key] += run, where a run is a group of adjacent
+//                   lanes of a wavefront with the same key: one atomic per
+//                   run; in sorted order most of a cell is one run
+//   k_cs_tile_sum   sums of CS_TILE consecutive counts
+//   k_cs_tile_scan  exclusive scan of the counts -> cell_start[0 .. ncell]
+//                   (each workgroup adds up the tile sums before its own)
+//   k_cs_scatter    every particle takes a slot of its cell range (count[key]
+//                   -= run restores the counters to zero); the order inside a
+//                   cell is arbitrary here
+//   k_cs_rank       every particle counts the ids of its cell below its own:
+//                   that rank is its place in (cell, id) order
+// The counters must be zero before the keys are emitted (k_cs_zero) and are
+// zero again after the scatter.  Grids of
+// up to CS_TILE * CS_MAX_TILES cells are handled.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace arm {
+
+constexpr int CS_THREADS = 256;
+constexpr int CS_PER_LANE = 8;
+constexpr int CS_TILE = CS_THREADS * CS_PER_LANE;  // cells per scan tile
+constexpr int CS_MAX_TILES = 4096;
+
+// Runs of equal keys in adjacent lanes of a wavefront.  All 64 lanes must
+// call it.  head: first lane of its run; len: run length (valid on heads);
+// hl: the head lane of this lane's run; rank: lane - hl.
+struct WaveRun {
+  bool head;
+  int len, hl, rank;
+};
+
+__device__ __forceinline__ WaveRun wave_run(uint32_t c, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t prev = __shfl_up(c, 1, 64);
+  const bool head = valid && (lane == 0 || prev != c);
+  const uint64_t hb = __ballot(head || !valid);
+  const uint64_t after = lane == 63 ? 0ull : (hb >> (lane + 1));
+  const int next = after ? lane + 1 + __builtin_ctzll(after) : 64;
+  const uint64_t upto = lane == 63 ? hb : (hb & ((2ull << lane) - 1ull));
+  WaveRun r;
+  r.head = head;
+  r.len = next - lane;
+  r.hl = upto ? 63 - __builtin_clzll(upto) : lane;
+  r.rank = lane - r.hl;
+  return r;
+}
+
+// Where the hashing kernels put the keys: indexed by id for the radix sort,
+// by slot (plus the per-cell counts) for the counting sort.
+struct KeyOut {
+  uint32_t* key_by_id;
+  uint32_t* slot_by_id;
+  uint32_t* key_by_slot;
+  int* count;
+};
+
+// Stores the key of slot t (particle id); nothing when every pointer of ko
+// is null.  All 64 lanes of the wavefront must call it; lanes with
+// valid = false only take part in the ballots.
+__device__ __forceinline__ void emit_key(const KeyOut& ko, bool valid, int t, int id,
+                                         uint32_t key) {
+  if (ko.key_by_slot) {
+    const WaveRun r = wave_run(key, valid);
+    if (valid) ko.key_by_slot[t] = key;
+    if (r.head) atomicAdd(&ko.count[key], r.len);
+  } else if (ko.key_by_id && valid) {
+    ko.key_by_id[id] = key;
+    ko.slot_by_id[id] = (uint32_t)t;
+  }
+}
+
+// Clears the per-cell counters (a fill kernel, so that it may be captured).
+__global__ __launch_bounds__(CS_THREADS) void k_cs_zero(int* __restrict__ count, int ncell) {
+  for (int c = blockIdx.x * CS_THREADS + threadIdx.x; c < ncell; c += gridDim.x * CS_THREADS)
+    count[c] = 0;
+}
+
+// Exclusive scan of one value per lane across the workgroup (LDS, one
+// wavefront-sized partial per wavefront).  Returns the lane's exclusive
+// prefix; *total receives the workgroup sum.
+template <int THREADS>
+__device__ __forceinline__ int block_exclusive_scan(int v, int* __restrict__ wsum, int* total) {
+  constexpr int NW = THREADS / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    int s = lane < NW ? wsum[lane] : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(s, d, 64);
+      if (lane >= d) s += y;
+    }
+    if (lane < NW) wsum[lane] = s;  // inclusive prefix of the wavefront sums
+  }
+  __syncthreads();
+  const int before = w > 0 ? wsum[w - 1] : 0;
+  *total = wsum[NW - 1];
+  return before + x - v;
+}
+
+__global__ __launch_bounds__(CS_THREADS) void k_cs_tile_sum(const int* __restrict__ count,
+                                                            int ncell, int* __restrict__ tsum) {
+  __shared__ int wsum[CS_THREADS / 64];
+  const int c0 = blockIdx.x * CS_TILE + threadIdx.x * CS_PER_LANE;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < CS_PER_LANE; ++k) s += (c0 + k < ncell) ? count[c0 + k] : 0;
+  int total;
+  block_exclusive_scan<CS_THREADS>(s, wsum, &total);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(CS_THREADS) void k_cs_tile_scan(const int* __restrict__ count,
+                                                             int ncell, int n,
+                                                             const int* __restrict__ tsum,
+                                                             int* __restrict__ cstart) {
+  __shared__ int wsum[CS_THREADS / 64];
+  const int c0 = blockIdx.x * CS_TILE + threadIdx.x * CS_PER_LANE;
+  int v[CS_PER_LANE];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < CS_PER_LANE; ++k) {
+    v[k] = (c0 + k < ncell) ? count[c0 + k] : 0;
+    s += v[k];
+  }
+  int part = 0;
+  for (int j = threadIdx.x; j < (int)blockIdx.x; j += CS_THREADS) part += tsum[j];
+  int base;
+  block_exclusive_scan<CS_THREADS>(part, wsum, &base);
+  __syncthreads();  // wsum is reused
+  int total;
+  int run = base + block_exclusive_scan<CS_THREADS>(s, wsum, &total);
+#pragma unroll
+  for (int k = 0; k < CS_PER_LANE; ++k) {
+    if (c0 + k < ncell) cstart[c0 + k] = run;
+    run += v[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
+}
+
+// entry = id << 32 | slot; entc = its cell.  Particles in slot order.
+__global__ __launch_bounds__(CS_THREADS) void k_cs_scatter(const uint32_t* __restrict__ key_by_slot,
+                                                           const int32_t* __restrict__ id_of,
+                                                           int n, const int* __restrict__ cstart,
+                                                           int* __restrict__ count,
+                                                           uint64_t* __restrict__ ent,
+                                                           uint32_t* __restrict__ entc) {
+  const int t = blockIdx.x * CS_THREADS + threadIdx.x;
+  const bool valid = t < n;
+  const uint32_t c = valid ? key_by_slot[t] : 0xffffffffu;
+  const WaveRun r = wave_run(c, valid);
+  int old = 0;
+  if (r.head) old = atomicSub(&count[c], r.len);
+  old = __shfl(old, r.hl, 64);
+  if (!valid) return;
+  const int pos = cstart[c] + old - 1 - r.rank;
+  ent[pos] = ((uint64_t)(uint32_t)id_of[t] << 32) | (uint64_t)(uint32_t)t;
+  entc[pos] = c;
+}
+
+__global__ __launch_bounds__(CS_THREADS) void k_cs_rank(const uint64_t* __restrict__ ent,
+                                                        const uint32_t* __restrict__ entc, int n,
+                                                        const int* __restrict__ cstart,
+                                                        uint32_t* __restrict__ skey,
+                                                        uint32_t* __restrict__ sval) {
+  const int p = blockIdx.x * CS_THREADS + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t c = entc[p];
+  const uint64_t e = ent[p];
+  const int a = cstart[c], b = cstart[c + 1];
+  const uint32_t id = (uint32_t)(e >> 32);
+  int rank = 0;
+  for (int q = a; q < b; ++q) rank += (uint32_t)(ent[q] >> 32) < id ? 1 : 0;
+  skey[a + rank] = c;
+  sval[a + rank] = (uint32_t)e;
+}
+
+}  // namespace arm

@@ -8,6 +8,7 @@
 // particle: no atomics, bitwise repeatable.
 #pragma once
 
+#include "cell_sort.h"
 #include "sph_device.h"
 
 namespace arm {
@@ -30,29 +31,32 @@ __device__ __forceinline__ int cell_of(const KParams& K, double x, double y, int
   return ix + iy * K.nx;
 }
 
-// Key of the particle in slot s, written at its id (the sort input is listed
-// in id order so that a stable sort yields (cell, id) order).  FP32 form:
-// also refreshes the integer cell and FP32 offset from the FP64 accumulator.
+// Key of the particle in slot s (emit_key: at its id for the radix sort, at
+// its slot with the per-cell counts for the counting sort; a stable sort of
+// the keys listed in id order yields (cell, id) order).  FP32 form: also
+// refreshes the integer cell and FP32 offset from the FP64 accumulator.
 template <typename R>
-__global__ __launch_bounds__(BLOCK) void k_keys(St<R> S, KParams K, uint32_t* __restrict__ key_by_id,
-                                                uint32_t* __restrict__ slot_by_id,
+__global__ __launch_bounds__(BLOCK) void k_keys(St<R> S, KParams K, KeyOut ko,
                                                 int* __restrict__ err) {
   const int s = blockIdx.x * BLOCK + threadIdx.x;
-  if (s >= K.n) return;
-  const double2 q = S.xy[s];
-  int ix, iy;
-  bool ok;
-  const int key = cell_of(K, q.x, q.y, ix, iy, ok);
-  if (!ok) atomicAdd(err, 1);
-  const int id = S.id[s];
-  key_by_id[id] = (uint32_t)key;
-  slot_by_id[id] = (uint32_t)s;
-  if constexpr (sizeof(R) == 4) {
-    const double bx = K.ox + (double)ix * K.cell;
-    const double by = K.oy + (double)iy * K.cell;
-    S.cxy[s] = (uint32_t)ix | ((uint32_t)iy << 16);
-    S.off[s] = make_float2((float)(q.x - bx), (float)(q.y - by));
+  const bool valid = s < K.n;
+  uint32_t key = 0;
+  int id = 0;
+  if (valid) {
+    const double2 q = S.xy[s];
+    int ix, iy;
+    bool ok;
+    key = (uint32_t)cell_of(K, q.x, q.y, ix, iy, ok);
+    if (!ok) atomicAdd(err, 1);
+    id = S.id[s];
+    if constexpr (sizeof(R) == 4) {
+      const double bx = K.ox + (double)ix * K.cell;
+      const double by = K.oy + (double)iy * K.cell;
+      S.cxy[s] = (uint32_t)ix | ((uint32_t)iy << 16);
+      S.off[s] = make_float2((float)(q.x - bx), (float)(q.y - by));
+    }
   }
+  emit_key(ko, valid, s, id, key);
 }
 
 // ------------------------------------------------------------------------
@@ -90,15 +94,16 @@ __device__ __forceinline__ void fill_cell_start(const uint32_t* __restrict__ ske
   }
 }
 
-// Permute every field from state A to state B (per-routine link_list).
+// Permute every field from state A to state B (per-routine link_list);
+// fill = 0 when the sort already wrote cell_start.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_reorder_all(St<R> A, St<R> Bs, KParams K,
                                                        const uint32_t* __restrict__ skey,
                                                        const uint32_t* __restrict__ sval,
-                                                       int* __restrict__ cstart) {
+                                                       int* __restrict__ cstart, int fill) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   const bool valid = t < K.n;
-  fill_cell_start(skey, t, valid, K.n, K.ncell, cstart);
+  if (fill) fill_cell_start(skey, t, valid, K.n, K.ncell, cstart);
   if (!valid) return;
   const int s = (int)sval[t];
   Bs.id[t] = A.id[s];
@@ -130,11 +135,12 @@ template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KParams K,
                                                         const uint32_t* __restrict__ skey,
                                                         const uint32_t* __restrict__ sval,
-                                                        int* __restrict__ cstart, int predict) {
+                                                        int* __restrict__ cstart, int predict,
+                                                        int fill) {
   using R2 = typename V2<R>::type;
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   const bool valid = t < K.n;
-  fill_cell_start(skey, t, valid, K.n, K.ncell, cstart);
+  if (fill) fill_cell_start(skey, t, valid, K.n, K.ncell, cstart);
   if (!valid) return;
   const int s = (int)sval[t];
   const int id = A.id[s];
@@ -625,56 +631,57 @@ __global__ __launch_bounds__(BLOCK) void k_predictor(St<R> S, KParams K) {
   }
 }
 
-// Corrector; with keys != nullptr it also hashes the updated position for
-// the next step's sort (the reorder then rebuilds the FP32 cell + offset).
+// Corrector; with keys it also hashes the updated position for the next
+// step's sort (the reorder then rebuilds the FP32 cell + offset).
 template <typename R>
-__global__ __launch_bounds__(BLOCK) void k_corrector(St<R> S, KParams K, int its,
-                                                     uint32_t* __restrict__ key_by_id,
-                                                     uint32_t* __restrict__ slot_by_id,
-                                                     int* __restrict__ err) {
+__global__ __launch_bounds__(BLOCK) void k_corrector(St<R> S, KParams K, int its, int keys,
+                                                     KeyOut ko, int* __restrict__ err) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= K.n) return;
-  const int id = S.id[i];
-  const double dt = K.dt, hdt = K.dt / 2.0;
-  double2 q = S.xy[i];
-  if (is_rate(K, id)) {
-    double u;
-    if (its != 1) {
-      u = S.u_min[i] + dt * (double)S.du[i];
-      if (K.density_method == 0) S.rho[i] = S.rho_min[i] + (R)dt * S.drho[i];
-    } else {
-      u = S.u[i] + hdt * (double)S.du[i];
-      if (K.density_method == 0) S.rho[i] = S.rho[i] + (R)hdt * S.drho[i];
-    }
-    if (u < 0.0) u = 0.0;
-    S.u[i] = u;
-    if constexpr (sizeof(R) == 4) S.uf[i] = (float)u;
-    if (id < K.np) {
-      auto v = S.v[i];
-      const auto a = S.a[i];
+  const bool valid = i < K.n;
+  uint32_t key = 0;
+  int id = 0;
+  if (valid) {
+    id = S.id[i];
+    const double dt = K.dt, hdt = K.dt / 2.0;
+    double2 q = S.xy[i];
+    if (is_rate(K, id)) {
+      double u;
       if (its != 1) {
-        const auto vm = S.v_min[i];
-        const auto av = S.av[i];
-        v.x = vm.x + (R)dt * a.x + av.x;
-        v.y = vm.y + (R)dt * a.y + av.y;
+        u = S.u_min[i] + dt * (double)S.du[i];
+        if (K.density_method == 0) S.rho[i] = S.rho_min[i] + (R)dt * S.drho[i];
       } else {
-        v.x = v.x + (R)hdt * a.x;
-        v.y = v.y + (R)hdt * a.y;
+        u = S.u[i] + hdt * (double)S.du[i];
+        if (K.density_method == 0) S.rho[i] = S.rho[i] + (R)hdt * S.drho[i];
       }
-      S.v[i] = v;
-      q.x = q.x + dt * (double)v.x;
-      q.y = q.y + dt * (double)v.y;
-      S.xy[i] = q;
+      if (u < 0.0) u = 0.0;
+      S.u[i] = u;
+      if constexpr (sizeof(R) == 4) S.uf[i] = (float)u;
+      if (id < K.np) {
+        auto v = S.v[i];
+        const auto a = S.a[i];
+        if (its != 1) {
+          const auto vm = S.v_min[i];
+          const auto av = S.av[i];
+          v.x = vm.x + (R)dt * a.x + av.x;
+          v.y = vm.y + (R)dt * a.y + av.y;
+        } else {
+          v.x = v.x + (R)hdt * a.x;
+          v.y = v.y + (R)hdt * a.y;
+        }
+        S.v[i] = v;
+        q.x = q.x + dt * (double)v.x;
+        q.y = q.y + dt * (double)v.y;
+        S.xy[i] = q;
+      }
+    }
+    if (keys) {
+      int ix, iy;
+      bool ok;
+      key = (uint32_t)cell_of(K, q.x, q.y, ix, iy, ok);
+      if (!ok) atomicAdd(err, 1);
     }
   }
-  if (key_by_id) {
-    int ix, iy;
-    bool ok;
-    const int key = cell_of(K, q.x, q.y, ix, iy, ok);
-    if (!ok) atomicAdd(err, 1);
-    key_by_id[id] = (uint32_t)key;
-    slot_by_id[id] = (uint32_t)i;
-  }
+  if (keys) emit_key(ko, valid, i, id, key);
 }
 
 // ------------------------------------------------------------------------

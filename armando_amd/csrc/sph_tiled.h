@@ -24,20 +24,66 @@
 namespace arm {
 
 // ---- LDS tiles ------------------------------------------------------------
+// Every tile has TPAD spare records after CAP so that the eight-wide
+// neighbour test may read up to seven records past a row's end (their bits
+// are masked off).  In the FP32 form a record keeps the cell column as a
+// float, and the cell-row difference of a candidate row is the constant
+// 1 - k of row k of the segment, so
+//   dx = (off_i.x - off_j.x) + (cx_i - cx_j) cell
+//   dy = (off_i.y - off_j.y) + (1 - k) cell
+// are the same operations, with the same roundings, as delta() on PosF.
+constexpr int TPAD = 8;
+
+// Byte offset 4 l of a 4-byte LDS record.  Computed out of the compiler's
+// sight: otherwise it derives this address from the 16-byte records' l << 4
+// with 64-bit integer multiply-adds, two of them per neighbour.
+__device__ __forceinline__ uint32_t off4(int l) {
+  uint32_t r;
+  asm("v_lshlrev_b32 %0, 2, %1" : "=v"(r) : "v"(l));
+  return r;
+}
+
+template <typename T> __device__ __forceinline__ T lds_at(const T* base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// Lane particle for tile deltas (FP32 form).
+struct LaneF {
+  float cxf, ox, oy;
+};
+
+// The cell-row term of row k: (float)(cy_i - cy_j) * cell with cy_i - cy_j = 1 - k.
+__device__ __forceinline__ float row_uy(const KParams& K, int k) {
+  return (float)(1 - k) * K.cellf;
+}
+
+__device__ __forceinline__ void tile_delta(const KParams& K, const LaneF& p, float cxj, float oxj,
+                                           float oyj, float uy, float& dx, float& dy) {
+  const float tx = p.ox - oxj;
+  const float ux = (p.cxf - cxj) * K.cellf;
+  dx = tx + ux;
+  const float ty = p.oy - oyj;
+  dy = ty + uy;
+}
+
 template <typename R, int CAP> struct DensTile;
 template <int CAP> struct DensTile<float, CAP> {
-  float4 a[CAP];  // offx, offy, vx, vy
-  uint32_t c[CAP];
+  float4 a[CAP + TPAD];  // offx, offy, vx, vy
+  float cx[CAP + TPAD];  // cell column as float
   __device__ __forceinline__ void put(int l, const St<float>& S, const KParams&, int t) {
     const float2 o = S.off[t];
     const float2 v = S.v[t];
     a[l] = make_float4(o.x, o.y, v.x, v.y);
-    c[l] = S.cxy[t];
+    cx[l] = (float)(S.cxy[t] & 0xffffu);
   }
-  __device__ __forceinline__ PosF pos(int l) const {
-    const uint32_t cc = c[l];
+  __device__ __forceinline__ LaneF lane(int l) const {
     const float4 q = a[l];
-    return {int(cc & 0xffffu), int(cc >> 16), q.x, q.y};
+    return {cx[l], q.x, q.y};
+  }
+  __device__ __forceinline__ void delta(const KParams& K, const LaneF& p, int l, float uy,
+                                        float& dx, float& dy) const {
+    const float4 q = a[l];
+    tile_delta(K, p, lds_at(cx, off4(l)), q.x, q.y, uy, dx, dy);
   }
   __device__ __forceinline__ float2 vel(int l) const {
     const float4 q = a[l];
@@ -45,14 +91,20 @@ template <int CAP> struct DensTile<float, CAP> {
   }
 };
 template <int CAP> struct DensTile<double, CAP> {
-  double2 x[CAP], v[CAP];
+  double2 x[CAP + TPAD], v[CAP + TPAD];
   __device__ __forceinline__ void put(int l, const St<double>& S, const KParams&, int t) {
     x[l] = S.xy[t];
     v[l] = S.v[t];
   }
-  __device__ __forceinline__ PosD pos(int l) const {
+  __device__ __forceinline__ PosD lane(int l) const {
     const double2 q = x[l];
     return {q.x, q.y};
+  }
+  __device__ __forceinline__ void delta(const KParams&, const PosD& p, int l, double, double& dx,
+                                        double& dy) const {
+    const double2 q = x[l];
+    dx = p.x - q.x;
+    dy = p.y - q.y;
   }
   __device__ __forceinline__ double2 vel(int l) const { return v[l]; }
 };
@@ -89,10 +141,10 @@ template <int CAP> struct TauTile<double, CAP, true> {
 
 template <typename R, int CAP, bool VI = false> struct ForceTile;
 template <int CAP, bool VI> struct ForceTile<float, CAP, VI> {
-  float4 a[CAP];  // offx, offy, vx, vy
-  float4 b[CAP];  // Q = 1/rho, A = p/rho, psi, u
-  uint32_t c[CAP];
-  TauTile<float, CAP, VI> tw;
+  float4 a[CAP + TPAD];  // offx, offy, vx, vy
+  float4 b[CAP + TPAD];  // Q = 1/rho, A = p/rho, psi, u
+  float cx[CAP + TPAD];  // cell column as float
+  TauTile<float, CAP + TPAD, VI> tw;
   __device__ __forceinline__ void put(int l, const St<float>& S, const KParams& K, int t) {
     const float2 o = S.off[t];
     const float2 v = S.v[t];
@@ -100,13 +152,17 @@ template <int CAP, bool VI> struct ForceTile<float, CAP, VI> {
     float Q, A;
     q_and_a(K, S.rho[t], S.p[t], Q, A);
     b[l] = make_float4(Q, A, S.psi[t], S.uf[t]);
-    c[l] = S.cxy[t];
+    cx[l] = (float)(S.cxy[t] & 0xffffu);
     tw.put(l, S, t);
   }
-  __device__ __forceinline__ PosF pos(int l) const {
-    const uint32_t cc = c[l];
+  __device__ __forceinline__ LaneF lane(int l) const {
     const float4 q = a[l];
-    return {int(cc & 0xffffu), int(cc >> 16), q.x, q.y};
+    return {cx[l], q.x, q.y};
+  }
+  __device__ __forceinline__ void delta(const KParams& K, const LaneF& p, int l, float uy,
+                                        float& dx, float& dy) const {
+    const float4 q = a[l];
+    tile_delta(K, p, lds_at(cx, off4(l)), q.x, q.y, uy, dx, dy);
   }
   template <bool PRESS, bool AV, bool AH, bool XS>
   __device__ __forceinline__ void pair(const KParams& K, ForceOwn<float>& o, int l, float dx,
@@ -118,8 +174,8 @@ template <int CAP, bool VI> struct ForceTile<float, CAP, VI> {
   }
 };
 template <int CAP, bool VI> struct ForceTile<double, CAP, VI> {
-  double2 x[CAP], v[CAP], rp[CAP], pu[CAP];  // rp = (Q, A), pu = (psi, u)
-  TauTile<double, CAP, VI> tw;
+  double2 x[CAP + TPAD], v[CAP + TPAD], rp[CAP + TPAD], pu[CAP + TPAD];  // rp = (Q, A), pu = (psi, u)
+  TauTile<double, CAP + TPAD, VI> tw;
   __device__ __forceinline__ void put(int l, const St<double>& S, const KParams& K, int t) {
     x[l] = S.xy[t];
     v[l] = S.v[t];
@@ -129,9 +185,15 @@ template <int CAP, bool VI> struct ForceTile<double, CAP, VI> {
     pu[l] = make_double2(S.psi[t], S.u[t]);
     tw.put(l, S, t);
   }
-  __device__ __forceinline__ PosD pos(int l) const {
+  __device__ __forceinline__ PosD lane(int l) const {
     const double2 q = x[l];
     return {q.x, q.y};
+  }
+  __device__ __forceinline__ void delta(const KParams&, const PosD& p, int l, double, double& dx,
+                                        double& dy) const {
+    const double2 q = x[l];
+    dx = p.x - q.x;
+    dy = p.y - q.y;
   }
   template <bool PRESS, bool AV, bool AH, bool XS>
   __device__ __forceinline__ void pair(const KParams& K, ForceOwn<double>& o, int l, double dx,
@@ -147,12 +209,17 @@ template <int CAP, bool VI> struct ForceTile<double, CAP, VI> {
 // Positions, velocities and Q = 1/rho for the shear tensor pass.
 template <typename R, int CAP> struct TensTile {
   DensTile<R, CAP> d;
-  R q[CAP];
+  R q[CAP + TPAD];
   __device__ __forceinline__ void put(int l, const St<R>& S, const KParams& K, int t) {
     d.put(l, S, K, t);
     q[l] = (R)1 / dens(K, S.rho[t]);
   }
-  __device__ __forceinline__ auto pos(int l) const { return d.pos(l); }
+  __device__ __forceinline__ auto lane(int l) const { return d.lane(l); }
+  template <typename LP>
+  __device__ __forceinline__ void delta(const KParams& K, const LP& p, int l, R uy, R& dx,
+                                        R& dy) const {
+    d.delta(K, p, l, uy, dx, dy);
+  }
 };
 
 // ---- segment geometry ------------------------------------------------------
@@ -194,36 +261,38 @@ __device__ __forceinline__ Segment<CW> segment_of(const KParams& K, const int* _
   return g;
 }
 
-// Neighbour mask of candidates l0 .. l0+len-1 (len <= 64) of one row.
-template <typename R, typename Tile, typename Pos>
-__device__ __forceinline__ uint64_t row_mask(const KParams& K, const Tile& T, const Pos& pi,
+// Neighbour mask of candidates l0 .. l0+len-1 (len <= 32) of row k: eight
+// independent tile reads and tests per batch, no per-candidate branch; the
+// bits past len and the lane's own bit are cleared afterwards.
+template <typename R, typename Tile, typename LP>
+__device__ __forceinline__ uint32_t row_mask(const KParams& K, const Tile& T, const LP& pi, R uy,
                                              int l0, int len, int self) {
   const R cut2 = cutoff2<R>(K);
-  uint64_t m = 0;
+  uint32_t m = 0;
   for (int k0 = 0; k0 < len; k0 += 8) {
-    Pos pj[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) pj[q] = T.pos(l0 + min(k0 + q, len - 1));
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       R dx, dy;
-      delta(pi, pj[q], K, dx, dy);
+      T.delta(K, pi, l0 + k0 + q, uy, dx, dy);
       const R r2 = dx * dx + dy * dy;
-      const int kk = k0 + q;
-      if (kk < len && r2 < cut2 && l0 + kk != self) m |= 1ull << kk;
+      m |= (r2 < cut2 ? 1u : 0u) << (k0 + q);
     }
   }
+  m &= len >= 32 ? 0xffffffffu : ((1u << len) - 1u);
+  const int sb = self - l0;
+  if (sb >= 0 && sb < 32) m &= ~(1u << sb);
   return m;
 }
 
-// Calls visit(l) for every neighbour of lane particle i, rows bottom to top,
-// ascending sorted position within a row.  With rm != nullptr it also
-// returns the neighbour masks of the three rows (bit b = candidate ta_k + b)
-// and whether they are complete (every row at most 32 candidates).
-template <typename R, int CW, typename Tile, typename Pos, typename Visit>
+// Calls visit(l, uy) for every neighbour of lane particle i, rows bottom to
+// top, ascending sorted position within a row (uy: the cell-row term of the
+// row).  With rm != nullptr it also returns the neighbour masks of the three
+// rows (bit b = candidate ta_k + b) and whether they are complete (every
+// row at most 32 candidates).
+template <typename R, int CW, typename Tile, typename LP, typename Visit>
 __device__ __forceinline__ void for_neighbours(const KParams& K, const Segment<CW>& g,
                                                const int (*cs)[CW + 3], const Tile& T,
-                                               const Pos& pi, int i, uint32_t key,
+                                               const LP& pi, int i, uint32_t key,
                                                Visit&& visit, uint32_t* rm = nullptr,
                                                bool* ok = nullptr) {
   const int nx = K.nx;
@@ -231,27 +300,29 @@ __device__ __forceinline__ void for_neighbours(const KParams& K, const Segment<C
   const int a0 = (cx > 0 ? cx - 1 : 0) - g.slo;
   const int a1 = (cx + 1 < nx ? cx + 1 : nx - 1) - g.slo + 1;
   const int self = g.lo[1] + (i - g.rs[1]);
+#pragma unroll
   for (int k = 0; k < 3; ++k) {
     if (g.rl[k] == 0) continue;
+    const R uy = (R)row_uy(K, k);
     const int ta = cs[k][a0];
     const int tb = cs[k][a1];
     const int base = g.lo[k] - g.rs[k];
     if (rm && tb - ta > 32) *ok = false;
-    for (int c0 = ta; c0 < tb; c0 += 64) {
-      const int len = min(64, tb - c0);
-      uint64_t m = row_mask<R>(K, T, pi, base + c0, len, self);
-      if (rm && c0 == ta) rm[k] = (uint32_t)m;
+    for (int c0 = ta; c0 < tb; c0 += 32) {
+      const int len = min(32, tb - c0);
+      uint32_t m = row_mask<R>(K, T, pi, uy, base + c0, len, self);
+      if (rm && c0 == ta) rm[k] = m;
       while (m) {
-        const int b = __builtin_ctzll(m);
+        const int b = __builtin_ctz(m);
         m &= m - 1;
-        visit(base + c0 + b);
+        visit(base + c0 + b, uy);
       }
     }
   }
 }
 
 // Same walk driven by masks recorded by the density pass.
-template <int CW, typename Visit>
+template <typename R, int CW, typename Visit>
 __device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& g,
                                            const int (*cs)[CW + 3], uint32_t key, uint4 nm,
                                            Visit&& visit) {
@@ -262,12 +333,13 @@ __device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& 
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     if (g.rl[k] == 0) continue;
+    const R uy = (R)row_uy(K, k);
     const int l0 = g.lo[k] - g.rs[k] + cs[k][a0];
     uint32_t m = rm[k];
     while (m) {
       const int b = __builtin_ctz(m);
       m &= m - 1;
-      visit(l0 + b);
+      visit(l0 + b, uy);
     }
   }
 }
@@ -296,7 +368,7 @@ __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
       continue;
     }
     const int self = g.lo[1] + (i - g.rs[1]);
-    const auto pi = T.pos(self);
+    const auto pi = T.lane(self);
     const auto vi = T.vel(self);
     DensOwn<R> o;
     dens_begin<R, SUM>(K, o);
@@ -304,9 +376,9 @@ __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
     bool ok = true;
     for_neighbours<R, CW>(
         K, g, cs, T, pi, i, skey[i],
-        [&](int l) {
+        [&](int l, R uy) {
           R dx, dy;
-          delta(pi, T.pos(l), K, dx, dy);
+          T.delta(K, pi, l, uy, dx, dy);
           const R r2 = dx * dx + dy * dy;
           const auto vj = T.vel(l);
           dens_pair<R, SUM>(K, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
@@ -341,12 +413,12 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
       continue;
     }
     const int self = g.lo[1] + (i - g.rs[1]);
-    const auto pi = T.pos(self);
+    const auto pi = T.lane(self);
     const auto vi = T.d.vel(self);
     TensOwn<R> o;
-    auto visit = [&](int l) {
+    auto visit = [&](int l, R uy) {
       R dx, dy;
-      delta(pi, T.pos(l), K, dx, dy);
+      T.delta(K, pi, l, uy, dx, dy);
       const R r2 = dx * dx + dy * dy;
       const auto vj = T.d.vel(l);
       tens_pair(K, o, dx, dy, r2, vj.x - vi.x, vj.y - vi.y, T.q[l]);
@@ -354,7 +426,7 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
     const uint32_t key = skey[i];
     const uint4 nm = nbm ? nbm[i] : make_uint4(0u, 0u, 0u, 0u);
     if (nm.w)
-      for_masked<CW>(K, g, cs, key, nm, visit);
+      for_masked<R, CW>(K, g, cs, key, nm, visit);
     else
       for_neighbours<R, CW>(K, g, cs, T, pi, i, key, visit);
     tens_end(S, K, i, T.q[self], o);
@@ -385,19 +457,19 @@ __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
       continue;
     }
     const int self = g.lo[1] + (i - g.rs[1]);
-    const auto pi = T.pos(self);
+    const auto pi = T.lane(self);
     ForceOwn<R> o;
     force_begin<R, VI>(S, K, i, o);
-    auto visit = [&](int l) {
+    auto visit = [&](int l, R uy) {
       R dx, dy;
-      delta(pi, T.pos(l), K, dx, dy);
+      T.delta(K, pi, l, uy, dx, dy);
       const R r2 = dx * dx + dy * dy;
       T.template pair<PRESS, AV, AH, XS>(K, o, l, dx, dy, r2);
     };
     const uint32_t key = skey[i];
     const uint4 nm = nbm ? nbm[i] : make_uint4(0u, 0u, 0u, 0u);
     if (nm.w)
-      for_masked<CW>(K, g, cs, key, nm, visit);
+      for_masked<R, CW>(K, g, cs, key, nm, visit);
     else
       for_neighbours<R, CW>(K, g, cs, T, pi, i, key, visit);
     force_end<R, PRESS, XS, VI>(S, K, i, o);
@@ -407,10 +479,10 @@ __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
 // Launch shapes: cells per segment, lanes, LDS record capacity.
 template <typename R> struct TileCfg;
 template <> struct TileCfg<float> {
-  static constexpr int BLK = 256, CW = 32, CAP = 1024;
+  static constexpr int BLK = 256, CW = 32, CAP = 720;
 };
 template <> struct TileCfg<double> {
-  static constexpr int BLK = 128, CW = 16, CAP = 640;
+  static constexpr int BLK = 128, CW = 16, CAP = 400;
 };
 
 }  // namespace arm

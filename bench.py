@@ -59,7 +59,21 @@ def parse():
                     help="approximate CPU work for the baseline sample")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="dam-break size factor (1.0 = the 720k workload)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB three times (a kernel of "
+                         "known traffic for scaling the memory counters of a profiler)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    import torch
+
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
 
 
 def run_form(precision, args, case_fn):
@@ -161,6 +175,8 @@ def main():
         sys.exit(1)
     from armando_amd import cases
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     if args.scale == 1.0:
         case_fn = lambda prec: cases.dam_break_720k(precision=prec)  # noqa: E731
     else:

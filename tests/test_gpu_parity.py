@@ -249,3 +249,29 @@ def test_medium_tiles_flush_lists():
     for k in ("drho", "ax", "ay", "du", "avx", "avy"):
         assert relmax(ra[k], st.field(k)) < 1e-10, k
     a.close()
+
+
+def test_radix_sort_path_on_a_large_grid():
+    """Grids above 2048 x 4096 cells take the radix sort instead of the
+    counting sort: same keys, permutation, ranges and pairs as the oracle."""
+    case = dict(cases.box_lattice(nx=30, ny=20, vel=0.3))
+    case["nx"], case["ny"] = 4200, 2100  # 8.8M cells, particles near the origin corner
+    banner(f"radix path grid {case['nx']}x{case['ny']}")
+    sim = armando_amd.Simulation(case)
+    niac = sim.link_list(count=True)
+    P = orc.params_from(case)
+    key, bad = orc.keys(P, case["x"], case["y"])
+    assert bad == 0
+    sid, start = orc.sort_cells(key, case["nx"] * case["ny"])
+    gk, gs, gc = sim.cells()
+    assert np.array_equal(gs, sid) and np.array_equal(gk, key[sid]) and np.array_equal(gc, start)
+    oi, oj = orc.pairs_f64(P, case["x"], case["y"])
+    gi, gj = sim.pairs()
+    assert niac == len(oi) and np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    sim.time_integration(1, 3)
+    st = oracle_state(case)
+    orc.run(P, st, 1, 3, gather=True)
+    s = sim.state()
+    for k in ("x", "y", "rho", "u"):
+        assert relmax(s[k], st.field(k)) < 1e-10, k
+    sim.close()
