@@ -308,27 +308,28 @@ template <typename R> int nseg_of(const arm_ctx* c) {
   return (c->K.nx + TileCfg<R>::CW - 1) / TileCfg<R>::CW;
 }
 
-template <typename R, bool SUM> void launch_density(arm_ctx* c) {
+template <typename R, bool SUM, bool EOS> void launch_density(arm_ctx* c) {
   St<R>& A = stA<R>(c);
   using T = TileCfg<R>;
   if (c->tiled) {
     const int nseg = nseg_of<R>(c);
-    hipLaunchKernelGGL((k_density_tiled<R, SUM, T::BLK, T::CW, T::CAP>),
+    hipLaunchKernelGGL((k_density_tiled<R, SUM, EOS, T::BLK, T::CW, T::CAP>),
                        dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
                        c->cstart, nseg, c->nbm);
   } else {
-    hipLaunchKernelGGL((k_density<R, SUM>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A,
-                       c->K, c->skey, c->cstart, c->nbm);
+    hipLaunchKernelGGL((k_density<R, SUM, EOS>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
+                       c->stream, A, c->K, c->skey, c->cstart, c->nbm);
   }
   c->masks_valid = true;
 }
 
-template <typename R> int density_t(arm_ctx* c, int method) {
+// density pass; with eos the equation of state is applied in the same kernel
+template <typename R> int density_t(arm_ctx* c, int method, bool eos = false) {
   PassTimer tm(c, ARM_PASS_DENSITY);
   if (method == 1)
-    launch_density<R, true>(c);
+    eos ? launch_density<R, true, true>(c) : launch_density<R, true, false>(c);
   else
-    launch_density<R, false>(c);
+    eos ? launch_density<R, false, true>(c) : launch_density<R, false, false>(c);
   HIPCHK(c, hipGetLastError());
   return ARM_OK;
 }
@@ -410,8 +411,7 @@ template <typename R> int zero_av(arm_ctx* c) {
 
 template <typename R> int single_step_t(arm_ctx* c) {
   int rc;
-  if ((rc = density_t<R>(c, c->prm.density_method))) return rc;
-  if ((rc = eos_t<R>(c))) return rc;
+  if ((rc = density_t<R>(c, c->prm.density_method, true))) return rc;
   const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
   if ((rc = force_t<R>(c, true, c->prm.flags, xs))) return rc;
   if (!xs && (rc = zero_av<R>(c))) return rc;
@@ -480,8 +480,7 @@ template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
   }
   if (!xs && (rc = zero_av<R>(c))) return rc;
   c->sorted = true;
-  if ((rc = density_t<R>(c, c->prm.density_method))) return rc;
-  if ((rc = eos_t<R>(c))) return rc;
+  if ((rc = density_t<R>(c, c->prm.density_method, true))) return rc;
   if constexpr (sizeof(R) == 8) {
     if (hk.jpart)
       hipLaunchKernelGGL(k_objective, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, stA<R>(c),
@@ -1177,7 +1176,7 @@ int arm_density(arm_ctx* c, int method) {
   if (method != 0 && method != 1) return fail(c, ARM_ERR_UNSUPPORTED, "density method %d", method);
   int rc = need_sorted(c);
   if (rc) return rc;
-  rc = DISPATCH(c, density_t)(c, method);
+  rc = DISPATCH(c, density_t)(c, method, false);
   if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
   return rc;
 }

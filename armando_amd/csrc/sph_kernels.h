@@ -253,24 +253,57 @@ __device__ __forceinline__ void dens_begin(const KParams& K, DensOwn<R>& o) {
   if (SUM) o.rs.add((R)K.mass * (R)K.wnorm);
 }
 
+// EOS of one particle from its stored density (Tait, eos.f90:255-283, or
+// linear) with the p < pmin clamp, then the art_heat coefficient from the
+// raw divergence sum (divv = sum / rho).  Shared by k_eos and the density
+// pass with the EOS folded in, so both give the same bits.
+template <typename R>
+__device__ __forceinline__ void eos_point(St<R>& S, const KParams& K, int i, bool rate, R rho,
+                                          R raw_divv) {
+  if (rate) {
+    R p = K.eos == 1 ? linear_p(K, rho) : tait(K, rho);
+    if (p < (R)K.pmin) {
+      p = (R)0;
+      rho = (sizeof(R) == 4) ? (R)0 : (R)K.rho0;
+      S.rho[i] = rho;
+      S.flag[i] |= 2u;
+    }
+    S.p[i] = p;
+  }
+  const R dv = raw_divv / dens(K, rho);
+  S.divv[i] = dv;
+  const R ad = dv < (R)0 ? -dv : dv;
+  S.psi[i] = (R)K.psi0 + (R)K.g2h2 * (ad - dv);
+}
+
 // Writes drho (rate particles), rho (summation) and the raw divergence sum
 // of art_heat's first pass (art_heat.f:46-67): with a uniform mass it is the
-// negative of the continuity sum; the EOS divides it by rho.
-template <typename R, bool SUM>
+// negative of the continuity sum; the EOS divides it by rho.  With EOS the
+// equation of state follows at once (single_step.f:119-186 comes right after
+// the density call).
+template <typename R, bool SUM, bool EOS = false>
 __device__ __forceinline__ void dens_end(St<R>& S, const KParams& K, int i, const DensOwn<R>& o) {
   const bool rate = is_rate(K, S.id[i]);
+  R rho = (R)0;
   if (!SUM) {
     S.drho[i] = rate ? o.sum.get() : (R)0;
+    if (EOS) rho = S.rho[i];
   } else {
     S.drho[i] = (R)0;
     if (rate) {
       if constexpr (sizeof(R) == 4)
-        S.rho[i] = o.rs.get() - (float)K.rho0;
+        rho = o.rs.get() - (float)K.rho0;
       else
-        S.rho[i] = o.rs.get();
+        rho = o.rs.get();
+      S.rho[i] = rho;
+    } else if (EOS) {
+      rho = S.rho[i];
     }
   }
-  S.divv[i] = -o.sum.get();
+  if (EOS)
+    eos_point(S, K, i, rate, rho, -o.sum.get());
+  else
+    S.divv[i] = -o.sum.get();
 }
 
 // force pass: int_force pressure part (internal_force.f:178-259, 309-430),
@@ -422,7 +455,7 @@ template <typename R> __device__ __forceinline__ R u_of(const St<R>& S, int t) {
 // Also records the neighbour masks of the three rows (bit b = candidate
 // t0 + b) for the force pass; w = 0 marks them incomplete (a row longer than
 // 32 candidates).
-template <typename R, bool SUM>
+template <typename R, bool SUM, bool EOS = false>
 __device__ __forceinline__ void density_direct(St<R>& S, const KParams& K, int i,
                                                const uint32_t* __restrict__ skey,
                                                const int* __restrict__ cstart,
@@ -450,7 +483,7 @@ __device__ __forceinline__ void density_direct(St<R>& S, const KParams& K, int i
     }
   }
   if (nbm) nbm[i] = make_uint4(rm[0], rm[1], rm[2], ok ? 1u : 0u);
-  dens_end<R, SUM>(S, K, i, o);
+  dens_end<R, SUM, EOS>(S, K, i, o);
 }
 
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI = false>
@@ -479,14 +512,14 @@ __device__ __forceinline__ void force_direct(St<R>& S, const KParams& K, int i,
   force_end<R, PRESS, XS, VI>(S, K, i, o);
 }
 
-template <typename R, bool SUM>
+template <typename R, bool SUM, bool EOS>
 __global__ __launch_bounds__(BLOCK) void k_density(St<R> S, KParams K,
                                                    const uint32_t* __restrict__ skey,
                                                    const int* __restrict__ cstart,
                                                    uint4* __restrict__ nbm) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
-  density_direct<R, SUM>(S, K, i, skey, cstart, nbm);
+  density_direct<R, SUM, EOS>(S, K, i, skey, cstart, nbm);
 }
 
 // Tait EOS (eos.f90:255-283) or the linear EOS on rate particles, then the
@@ -495,21 +528,7 @@ template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_eos(St<R> S, KParams K) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
-  R rho = S.rho[i];
-  if (is_rate(K, S.id[i])) {
-    R p = K.eos == 1 ? linear_p(K, rho) : tait(K, rho);
-    if (p < (R)K.pmin) {
-      p = (R)0;
-      rho = (sizeof(R) == 4) ? (R)0 : (R)K.rho0;
-      S.rho[i] = rho;
-      S.flag[i] |= 2u;
-    }
-    S.p[i] = p;
-  }
-  const R dv = S.divv[i] / dens(K, rho);
-  S.divv[i] = dv;
-  const R ad = dv < (R)0 ? -dv : dv;
-  S.psi[i] = (R)K.psi0 + (R)K.g2h2 * (ad - dv);
+  eos_point(S, K, i, is_rate(K, S.id[i]), S.rho[i], S.divv[i]);
 }
 
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI>

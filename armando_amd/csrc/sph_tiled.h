@@ -344,7 +344,7 @@ __device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& 
   }
 }
 
-template <typename R, bool SUM, int BLK, int CW, int CAP>
+template <typename R, bool SUM, bool EOS, int BLK, int CW, int CAP>
 __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
                                                        const uint32_t* __restrict__ skey,
                                                        const int* __restrict__ cstart, int nseg,
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
     const int i = i0 + threadIdx.x;
     if (i >= g.O1) continue;
     if (!tiled) {
-      density_direct<R, SUM>(S, K, i, skey, cstart, nbm);
+      density_direct<R, SUM, EOS>(S, K, i, skey, cstart, nbm);
       continue;
     }
     const int self = g.lo[1] + (i - g.rs[1]);
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
         },
         rm, &ok);
     nbm[i] = make_uint4(rm[0], rm[1], rm[2], ok ? 1u : 0u);
-    dens_end<R, SUM>(S, K, i, o);
+    dens_end<R, SUM, EOS>(S, K, i, o);
   }
 }
 
