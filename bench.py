@@ -46,6 +46,13 @@ BYTES = {
 }
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the FP32 force gather on the 720k workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE scaled by what the same counters
+# read for a plain 256 MiB copy (read = FETCH_SIZE x 2.0, written =
+# WRITE_SIZE x 1.0): 115.1 MB read + 15.1 MB written.  Measured on the
+# committed kernels; profiles/dam720k_pmc.md holds the runs.
+FORCE_TRAFFIC_BYTES = 130.2e6
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -228,7 +235,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": FORCE_TRAFFIC_BYTES if args.scale == 1.0 else None,
             "bytes_per_particle": BYTES[("force", 1)],
             "avg_launch_ms": t_force * 1e3,
         },

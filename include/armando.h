@@ -119,11 +119,15 @@ int arm_get_cells(arm_ctx* ctx, int32_t* key_sorted, int32_t* id_sorted, int32_t
 
 /* sum_density / con_density [CERNsource/density.f:1,166] (method 1 / 0). */
 int arm_density(arm_ctx* ctx, int method);
-/* Tait EOS on the current density [FORTRANsource/eos.f90:255-283]. */
+/* EOS on the current density: Tait [FORTRANsource/eos.f90:255-283] or the
+   linear EOS (prm.eos), with the p < pmin clamp; also the art_heat
+   coefficient from the density pass's velocity divergence. */
 int arm_eos(arm_ctx* ctx);
 /* int_force [internal_force.f:1] plus, by flags, art_visc [art_visc.f:1]
    and art_heat [art_heat.f:1], fused in one gather; adds gravity when
-   ARM_GRAVITY is set in the context flags. */
+   ARM_GRAVITY is set in the context flags.  With ARM_VISC in flags a shear
+   tensor pass runs first and the viscous force and tds/dt are added
+   [internal_force.f:73-171, 196-430]. */
 int arm_internal_force(arm_ctx* ctx, int flags);
 /* av_vel [CERNsource/av_vel.f:1] (XSPH velocity correction). */
 int arm_av_vel(arm_ctx* ctx);
@@ -172,11 +176,14 @@ int arm_conservation(arm_ctx* ctx, double out[5]);
    with HIP events on the context stream; counts[] = launches per pass.
    Pass indices: see ARM_PASS_*.  n is the array length. */
 #define ARM_PASS_KEYS 0      /* cell keys of the current positions */
-#define ARM_PASS_SORT 1      /* radix sort (all digit passes) */
+#define ARM_PASS_SORT 1      /* sort by (cell, id): counting sort over the cells
+                                (radix sort for grids above 8.4M cells) */
 #define ARM_PASS_REORDER 2   /* permute (+ predictor) + cell ranges */
-#define ARM_PASS_DENSITY 3   /* density gather */
-#define ARM_PASS_EOS 4       /* Tait EOS + art_heat coefficient */
-#define ARM_PASS_FORCE 5     /* internal_force + art_visc + art_heat + av_vel gather */
+#define ARM_PASS_DENSITY 3   /* density gather (with the EOS inside single_step and
+                                time_integration) */
+#define ARM_PASS_EOS 4       /* EOS + art_heat coefficient (arm_eos alone) */
+#define ARM_PASS_FORCE 5     /* [shear tensor +] internal_force + art_visc + art_heat
+                                + av_vel gather */
 #define ARM_PASS_INTEGRATE 6 /* predictor / corrector (+ keys of the next step) */
 #define ARM_PASS_COUNT 7
 int arm_pass_times(arm_ctx* ctx, int n, double* ms, int* counts);
