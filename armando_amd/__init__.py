@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "libarmando.so")
 ARM_OK = 0
 ERRORS = {-1: "ARM_ERR_ARG", -2: "ARM_ERR_HIP", -3: "ARM_ERR_CAPACITY", -4: "ARM_ERR_DOMAIN",
           -5: "ARM_ERR_UNSUPPORTED", -6: "ARM_ERR_STATE"}
-ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC = 1, 2, 4, 8, 16, 32
+ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
 EOS_TAIT, EOS_LINEAR = 0, 1
 FP64, FP32 = 0, 1
 PASSES = ("keys", "sort", "reorder", "density", "eos", "force", "integrate")
@@ -33,7 +33,8 @@ EXPORTS = ("arm_create", "arm_destroy", "arm_last_error", "arm_domain_violations
            "arm_single_step", "arm_predictor", "arm_corrector", "arm_time_integration",
            "arm_conservation", "arm_pass_times", "arm_reset_pass_times", "arm_set_timing",
            "arm_set_gather_path", "arm_adjoint_rollout", "arm_rollout_forward",
-           "arm_rollout_backward", "arm_update_params")
+           "arm_rollout_backward", "arm_update_params", "arm_set_objective",
+           "arm_rollout_param_grads")
 
 
 class ArmParams(C.Structure):
@@ -47,12 +48,16 @@ class ArmParams(C.Structure):
         ("gy", C.c_double), ("dt", C.c_double), ("ox", C.c_double), ("oy", C.c_double),
         ("cell", C.c_double), ("nx", C.c_int), ("ny", C.c_int),
         ("visc_eta", C.c_double), ("eos", C.c_int), ("eos_p0", C.c_double),
+        ("hs_q0", C.c_double), ("hs_x", C.c_double), ("hs_y", C.c_double),
+        ("hs_sigma", C.c_double),
     ]
 
 
 # fields a case may leave out, with their defaults (no physical viscosity,
 # Tait EOS)
-PARAM_DEFAULTS = {"visc_eta": 0.0, "eos": 0, "eos_p0": 0.0}
+PARAM_DEFAULTS = {"visc_eta": 0.0, "eos": 0, "eos_p0": 0.0, "hs_q0": 0.0, "hs_x": 0.0,
+                  "hs_y": 0.0, "hs_sigma": 0.0}
+OBJ_PRESSURE, OBJ_ENERGY = 0, 1
 
 
 class ArmandoError(RuntimeError):
@@ -105,6 +110,8 @@ def load_library(path=LIB_PATH):
         "arm_rollout_forward": (C.c_int, [vp, C.c_int, C.c_int, _ip, C.c_int, dpn, dpn]),
         "arm_rollout_backward": (C.c_int, [vp, C.c_double, dpn, dpn, dpn]),
         "arm_update_params": (C.c_int, [vp, C.POINTER(ArmParams)]),
+        "arm_set_objective": (C.c_int, [vp, C.c_int]),
+        "arm_rollout_param_grads": (C.c_int, [vp, dpn, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -266,6 +273,16 @@ class Simulation:
         self._check(self.L.arm_rollout_backward(self.ctx, seed, C.byref(da), _ptr(g),
                                                 C.byref(tb)))
         return da.value, g, tb.value
+
+    def set_objective(self, kind):
+        """OBJ_PRESSURE (J = sum dt sum p^2) or OBJ_ENERGY (u^2) for the next rollout."""
+        self._check(self.L.arm_set_objective(self.ctx, int(kind)))
+
+    def param_grads(self):
+        """(dJ/d av_alpha, dJ/d hs_q0) of the last rollout backward."""
+        out = np.zeros(2)
+        self._check(self.L.arm_rollout_param_grads(self.ctx, _ptr(out), 2))
+        return float(out[0]), float(out[1])
 
     def set_gather_path(self, tiled=True):
         self._check(self.L.arm_set_gather_path(self.ctx, 1 if tiled else 0))

@@ -37,7 +37,13 @@ def extension():
 
 
 class RolloutLoss:
-    def __init__(self, case, obj_ids, nsteps, ckpt_every):
+    """J(v0, alpha, q0) = sum over nsteps of dt * sum_{obj} f, f = p^2
+    (objective OBJ_PRESSURE) or u^2 (OBJ_ENERGY), as a torch function whose
+    backward is the hand-written adjoint (checkpoint every ckpt_every steps).
+    v0: (n, 2) initial velocities; alpha: artificial viscosity; q0: heat
+    source amplitude (used when the case has the heat source)."""
+
+    def __init__(self, case, obj_ids, nsteps, ckpt_every, objective=0):
         ext = extension()
         params = {k: case[k] if k in case else PARAM_DEFAULTS[k] for k, _ in ArmParams._fields_}
         x = torch.from_numpy(np.stack([case["x"], case["y"]], 1).astype(np.float64))
@@ -45,11 +51,14 @@ class RolloutLoss:
         u = torch.from_numpy(np.asarray(case["u"], np.float64))
         obj = torch.from_numpy(np.asarray(obj_ids, np.int32))
         self.n = int(case["n"])
+        self.q0 = float(params["hs_q0"])
         self.handle = ext.make_rollout(params, x, rho, u, int(case["np"]), int(case["nv"]), obj,
-                                       int(nsteps), int(ckpt_every))
+                                       int(nsteps), int(ckpt_every), int(objective))
 
-    def __call__(self, v0, alpha):
-        return extension().rollout_loss(self.handle, v0, alpha)
+    def __call__(self, v0, alpha, q0=None):
+        if q0 is None:
+            q0 = torch.tensor(self.q0, dtype=torch.float64)
+        return extension().rollout_loss(self.handle, v0, alpha, q0)
 
     def close(self):
         if getattr(self, "handle", 0):

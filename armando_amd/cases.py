@@ -16,7 +16,7 @@ import math
 import numpy as np
 
 # flags (include/armando.h)
-ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC = 1, 2, 4, 8, 16, 32
+ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
 EOS_TAIT, EOS_LINEAR = 0, 1
 FP64, FP32 = 0, 1
 
@@ -231,6 +231,16 @@ def dam_break_adjoint_100k(precision=FP64, seed=777, vstd=0.01):
     return c, obj
 
 
+def with_heat_source(case, q0, x, y, sigma):
+    """A copy of `case` with the volumetric Gaussian heat source
+    q0 exp(-|x - (x, y)|^2 / (2 sigma^2)) (W/m^3) on its real particles."""
+    c = dict(case)
+    c["flags"] = case["flags"] | HEAT_SOURCE
+    c["hs_q0"], c["hs_x"], c["hs_y"], c["hs_sigma"] = float(q0), float(x), float(y), float(sigma)
+    return c
+
+
 PARAM_KEYS = ("precision", "flags", "pa_sph", "density_method", "h", "mass", "rho0", "c0",
               "pmin", "av_alpha", "av_beta", "av_eta", "ah_g1", "ah_g2", "ah_eta", "xsph_eps",
-              "gx", "gy", "dt", "ox", "oy", "cell", "nx", "ny", "visc_eta", "eos", "eos_p0")
+              "gx", "gy", "dt", "ox", "oy", "cell", "nx", "ny", "visc_eta", "eos", "eos_p0",
+              "hs_q0", "hs_x", "hs_y", "hs_sigma")

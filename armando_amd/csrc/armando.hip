@@ -44,6 +44,8 @@ struct arm_ctx {
   bool loaded = false, sorted = false, have_rates = false;
   bool tiled = true;  // LDS-tiled gathers (false: direct global-memory gathers)
   struct AdjRun* adj = nullptr;  // checkpoints of the last arm_rollout_forward
+  int objkind = ARM_OBJ_PRESSURE;  // objective of the next rollout
+  double grads[ARM_NGRAD] = {0};   // parameter gradients of the last backward
   bool masks_valid = false;  // nbm holds the neighbour masks of the current order
   uint4* nbm = nullptr;
   hipStream_t stream = nullptr;
@@ -454,8 +456,9 @@ template <typename R> int begin_steps(arm_ctx* c) {
 //   sort -> reorder + predictor -> density -> eos [-> objective] -> forces
 //   [-> tape] -> corrector + keys of the next step
 struct StepHooks {
-  const uint8_t* objmask = nullptr;  // objective: jpart += dt sum p^2
+  const uint8_t* objmask = nullptr;  // objective: jpart += dt sum f (p^2 or u^2)
   double* jpart = nullptr;
+  int kind = ARM_OBJ_PRESSURE;
   const Tape* tape = nullptr;        // record the step after the force pass
 };
 
@@ -484,7 +487,7 @@ template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
   if constexpr (sizeof(R) == 8) {
     if (hk.jpart)
       hipLaunchKernelGGL(k_objective, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, stA<R>(c),
-                         c->K, hk.objmask, hk.jpart);
+                         c->K, hk.kind, hk.objmask, hk.jpart);
   }
   if ((rc = force_t<R>(c, true, c->prm.flags, xs))) return rc;
   if constexpr (sizeof(R) == 8) {
@@ -577,23 +580,24 @@ int alloc_tape(Scratch& sc, Tape& t, int n, int ncell) {
 }
 
 template <bool AV, bool AH, bool XS>
-void launch_adj_force(arm_ctx* c, const Tape& T, const AdjScratch& W, double* apart) {
+void launch_adj_force(arm_ctx* c, const Tape& T, const AdjScratch& W, double* apart,
+                      double* qpart) {
   hipLaunchKernelGGL((k_adj_force<AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
-                     T, c->K, W, apart);
+                     T, c->K, W, apart, qpart);
 }
 
-void adj_force(arm_ctx* c, const Tape& T, const AdjScratch& W, double* apart) {
+void adj_force(arm_ctx* c, const Tape& T, const AdjScratch& W, double* apart, double* qpart) {
   const int f = c->prm.flags;
   const int code = ((f & ARM_ART_VISC) ? 4 : 0) | ((f & ARM_ART_HEAT) ? 2 : 0) | ((f & ARM_AV_VEL) ? 1 : 0);
   switch (code) {
-    case 0: launch_adj_force<false, false, false>(c, T, W, apart); break;
-    case 1: launch_adj_force<false, false, true>(c, T, W, apart); break;
-    case 2: launch_adj_force<false, true, false>(c, T, W, apart); break;
-    case 3: launch_adj_force<false, true, true>(c, T, W, apart); break;
-    case 4: launch_adj_force<true, false, false>(c, T, W, apart); break;
-    case 5: launch_adj_force<true, false, true>(c, T, W, apart); break;
-    case 6: launch_adj_force<true, true, false>(c, T, W, apart); break;
-    default: launch_adj_force<true, true, true>(c, T, W, apart); break;
+    case 0: launch_adj_force<false, false, false>(c, T, W, apart, qpart); break;
+    case 1: launch_adj_force<false, false, true>(c, T, W, apart, qpart); break;
+    case 2: launch_adj_force<false, true, false>(c, T, W, apart, qpart); break;
+    case 3: launch_adj_force<false, true, true>(c, T, W, apart, qpart); break;
+    case 4: launch_adj_force<true, false, false>(c, T, W, apart, qpart); break;
+    case 5: launch_adj_force<true, false, true>(c, T, W, apart, qpart); break;
+    case 6: launch_adj_force<true, true, false>(c, T, W, apart, qpart); break;
+    default: launch_adj_force<true, true, true>(c, T, W, apart, qpart); break;
   }
 }
 
@@ -605,7 +609,8 @@ struct AdjRun {
   Scratch sc;
   int nsteps = 0, ckpt = 0, nck = 0;
   uint8_t* obj = nullptr;
-  double *jpart = nullptr, *apart = nullptr;
+  double *jpart = nullptr, *apart = nullptr, *qpart = nullptr;
+  int kind = ARM_OBJ_PRESSURE;
   std::vector<Checkpoint> ck;
   std::vector<Tape> tape;
   AdjState L{};
@@ -635,7 +640,9 @@ int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, in
   R->nsteps = nsteps;
   R->ckpt = ckpt;
   R->nck = (nsteps + ckpt - 1) / ckpt;
-  if ((rc = sc.get(&R->obj, n)) || (rc = sc.get(&R->jpart, nb)) || (rc = sc.get(&R->apart, nb)))
+  R->kind = c->objkind;
+  if ((rc = sc.get(&R->obj, n)) || (rc = sc.get(&R->jpart, nb)) || (rc = sc.get(&R->apart, nb)) ||
+      (rc = sc.get(&R->qpart, nb)))
     return rc;
   {
     std::vector<uint8_t> h(n, 0);
@@ -661,7 +668,7 @@ int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, in
     if ((rc = sc.get(q, n))) return rc;
   double** wf[] = {&R->W.lax, &R->W.lay, &R->W.ldrho, &R->W.ldu, &R->W.lavx, &R->W.lavy,
                    &R->W.lp,  &R->W.lQ,  &R->W.lA,    &R->W.lpsi, &R->W.lS,  &R->W.lrhoh,
-                   &R->W.lu,  &R->W.lvx, &R->W.lvy,   &R->W.lx,  &R->W.ly};
+                   &R->W.lu,  &R->W.lvx, &R->W.lvy,   &R->W.lx,  &R->W.ly, &R->W.lus};
   for (double** q : wf)
     if ((rc = sc.get(q, n))) return rc;
   hipEvent_t e0, e1;
@@ -673,6 +680,7 @@ int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, in
   StepHooks fwd;
   fwd.objmask = R->obj;
   fwd.jpart = R->jpart;
+  fwd.kind = R->kind;
   for (int its = 1; its <= nsteps; ++its) {
     if ((its - 1) % ckpt == 0)
       if ((rc = copy_checkpoint(c, c->A64, R->ck[(its - 1) / ckpt], true))) return rc;
@@ -707,6 +715,7 @@ int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0,
   double** lf[] = {&L.x, &L.y, &L.vx, &L.vy, &L.rho, &L.u, &L.cax, &L.cay, &L.cdrho, &L.cdu};
   for (double** q : lf) HIPCHK(c, hipMemsetAsync(*q, 0, sizeof(double) * n, c->stream));
   HIPCHK(c, hipMemsetAsync(R->apart, 0, sizeof(double) * nb, c->stream));
+  HIPCHK(c, hipMemsetAsync(R->qpart, 0, sizeof(double) * nb, c->stream));
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
@@ -723,8 +732,8 @@ int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0,
     for (int its = s1 - 1; its >= s0; --its) {
       const Tape& T = R->tape[its - s0];
       hipLaunchKernelGGL(k_adj_corrector, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its, L, W,
-                         R->obj, seed);
-      adj_force(c, T, W, R->apart);
+                         R->obj, R->kind, seed);
+      adj_force(c, T, W, R->apart, R->qpart);
       hipLaunchKernelGGL(k_adj_eos, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
       hipLaunchKernelGGL(k_adj_density, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
       hipLaunchKernelGGL(k_adj_predictor, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its, L, W);
@@ -741,8 +750,9 @@ int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0,
   HIPCHK(c, hipEventRecord(e1, c->stream));
   // leave the context at the final forward state
   if ((rc = copy_checkpoint(c, c->A64, R->ck[nck], false))) return rc;
-  std::vector<double> ap(nb), gx(n), gy(n);
+  std::vector<double> ap(nb), qp(nb), gx(n), gy(n);
   HIPCHK(c, hipMemcpyAsync(ap.data(), R->apart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(qp.data(), R->qpart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(gx.data(), L.vx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(gy.data(), L.vy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -750,8 +760,13 @@ int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0,
   (void)hipEventElapsedTime(&f, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  double As = 0;
-  for (int k = 0; k < nb; ++k) As += ap[k];
+  double As = 0, Qs = 0;
+  for (int k = 0; k < nb; ++k) {
+    As += ap[k];
+    Qs += qp[k];
+  }
+  c->grads[ARM_GRAD_ALPHA] = As;
+  c->grads[ARM_GRAD_Q0] = Qs;
   if (dJ_dalpha) *dJ_dalpha = As;
   if (dJ_dv0)
     for (int i = 0; i < n; ++i) {
@@ -995,6 +1010,10 @@ void set_kparams(KParams& K, const arm_params& P) {
   K.lin_p0 = P.eos_p0;
   K.lin_p0r = P.eos_p0 + K.lin_c2 * P.rho0;
   K.eta = P.visc_eta;
+  K.hs_q0 = P.hs_q0;
+  K.hs_x = P.hs_x;
+  K.hs_y = P.hs_y;
+  K.hs_k = P.hs_sigma > 0 ? 1.0 / (2.0 * P.hs_sigma * P.hs_sigma) : 0.0;
 }
 
 int need_sorted(arm_ctx* c) {
@@ -1036,6 +1055,8 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     rc = fail(c, ARM_ERR_UNSUPPORTED, "eos must be ARM_EOS_TAIT or ARM_EOS_LINEAR");
   else if ((P.flags & ARM_VISC) && !(P.visc_eta >= 0))
     rc = fail(c, ARM_ERR_ARG, "visc_eta must be non-negative");
+  else if ((P.flags & ARM_HEAT_SOURCE) && !(P.hs_sigma > 0))
+    rc = fail(c, ARM_ERR_ARG, "hs_sigma must be positive");
   if (rc) {
     *out = c;  // so that arm_last_error can report
     return rc;
@@ -1317,11 +1338,28 @@ int arm_update_params(arm_ctx* c, const arm_params* prm) {
     return fail(c, ARM_ERR_ARG, "only the physical scalars can change after arm_create");
   if ((P.flags & ARM_VISC) && !(P.visc_eta >= 0))
     return fail(c, ARM_ERR_ARG, "visc_eta must be non-negative");
+  if ((P.flags & ARM_HEAT_SOURCE) && !(P.hs_sigma > 0))
+    return fail(c, ARM_ERR_ARG, "hs_sigma must be positive");
   if (!(P.mass > 0) || !(P.rho0 > 0) || !(P.dt > 0))
     return fail(c, ARM_ERR_ARG, "mass, rho0 and dt must be positive");
   c->prm = P;
   set_kparams(c->K, P);
   drop_adjoint(c);
+  return ARM_OK;
+}
+
+int arm_set_objective(arm_ctx* c, int kind) {
+  if (!c) return ARM_ERR_ARG;
+  if (kind != ARM_OBJ_PRESSURE && kind != ARM_OBJ_ENERGY)
+    return fail(c, ARM_ERR_ARG, "objective kind %d", kind);
+  c->objkind = kind;
+  drop_adjoint(c);
+  return ARM_OK;
+}
+
+int arm_rollout_param_grads(arm_ctx* c, double* out, int n) {
+  if (!c || !out || n < 0) return ARM_ERR_ARG;
+  for (int k = 0; k < n && k < ARM_NGRAD; ++k) out[k] = c->grads[k];
   return ARM_OK;
 }
 

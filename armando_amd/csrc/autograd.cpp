@@ -1,12 +1,12 @@
 // autograd.cpp -- torch::autograd::Function over the C ABI adjoint rollout.
 //
-// J(v0, alpha) = sum over steps of dt * sum_{i in obj} p_i^2 for nsteps of
-// time_integration started from a fixed initial state with initial
-// velocities v0 (n x 2) and artificial-viscosity alpha.  forward() runs
-// arm_rollout_forward (checkpoints kept in the device context); backward()
-// runs arm_rollout_backward with the incoming gradient as the adjoint seed
-// and returns dJ/dv0 and dJ/dalpha.  Host code only; the device work is in
-// libarmando.so.
+// J(v0, alpha, q0) = sum over steps of dt * sum_{i in obj} f_i (f = p^2 or
+// u^2) for nsteps of time_integration started from a fixed initial state
+// with initial velocities v0 (n x 2), artificial-viscosity alpha and heat
+// source amplitude q0.  forward() runs arm_rollout_forward (checkpoints kept
+// in the device context); backward() runs arm_rollout_backward with the
+// incoming gradient as the adjoint seed and returns dJ/dv0, dJ/dalpha and
+// dJ/dq0.  Host code only; the device work is in libarmando.so.
 #include <torch/extension.h>
 
 #include <cstdint>
@@ -44,13 +44,14 @@ std::vector<double> to_vec(const torch::Tensor& t) {
 class RolloutFn : public torch::autograd::Function<RolloutFn> {
  public:
   static torch::Tensor forward(torch::autograd::AutogradContext* actx, torch::Tensor v0,
-                               torch::Tensor alpha, int64_t handle) {
+                               torch::Tensor alpha, torch::Tensor q0, int64_t handle) {
     Rollout* r = as_rollout(handle);
     const int n = r->np + r->nv;
     TORCH_CHECK(v0.dim() == 2 && v0.size(0) == n && v0.size(1) == 2, "v0 must be (n, 2)");
     auto v = v0.detach().to(torch::kCPU, torch::kFloat64).contiguous();
     arm_params P = r->prm;
     P.av_alpha = alpha.detach().to(torch::kCPU, torch::kFloat64).item<double>();
+    P.hs_q0 = q0.detach().to(torch::kCPU, torch::kFloat64).item<double>();
     check(r, arm_update_params(r->ctx, &P), "arm_update_params");
     check(r, arm_upload(r->ctx, r->np, r->nv, r->x.data(), v.data_ptr<double>(), r->rho.data(),
                         r->u.data()),
@@ -71,16 +72,19 @@ class RolloutFn : public torch::autograd::Function<RolloutFn> {
     const double seed = grads[0].detach().to(torch::kCPU, torch::kFloat64).item<double>();
     const int n = r->np + r->nv;
     auto dv = torch::zeros({n, 2}, torch::dtype(torch::kFloat64));
-    double da = 0.0;
-    check(r, arm_rollout_backward(r->ctx, seed, &da, dv.data_ptr<double>(), nullptr),
+    check(r, arm_rollout_backward(r->ctx, seed, nullptr, dv.data_ptr<double>(), nullptr),
           "arm_rollout_backward");
-    auto dalpha = torch::tensor(da, torch::dtype(torch::kFloat64).device(grads[0].device()));
-    return {dv.to(torch::Device(dev)), dalpha, torch::Tensor()};
+    double g[ARM_NGRAD] = {0.0, 0.0};
+    check(r, arm_rollout_param_grads(r->ctx, g, ARM_NGRAD), "arm_rollout_param_grads");
+    auto opts = torch::dtype(torch::kFloat64).device(grads[0].device());
+    return {dv.to(torch::Device(dev)), torch::tensor(g[ARM_GRAD_ALPHA], opts),
+            torch::tensor(g[ARM_GRAD_Q0], opts), torch::Tensor()};
   }
 };
 
 int64_t make_rollout(py::dict p, torch::Tensor x, torch::Tensor rho, torch::Tensor u, int64_t np,
-                     int64_t nv, torch::Tensor obj, int64_t nsteps, int64_t ckpt) {
+                     int64_t nv, torch::Tensor obj, int64_t nsteps, int64_t ckpt,
+                     int64_t objective) {
   auto* r = new Rollout();
   auto d = [&](const char* k) { return p[k].cast<double>(); };
   auto i = [&](const char* k) { return p[k].cast<int>(); };
@@ -112,6 +116,10 @@ int64_t make_rollout(py::dict p, torch::Tensor x, torch::Tensor rho, torch::Tens
   P.visc_eta = p.contains("visc_eta") ? d("visc_eta") : 0.0;
   P.eos = p.contains("eos") ? i("eos") : ARM_EOS_TAIT;
   P.eos_p0 = p.contains("eos_p0") ? d("eos_p0") : 0.0;
+  P.hs_q0 = p.contains("hs_q0") ? d("hs_q0") : 0.0;
+  P.hs_x = p.contains("hs_x") ? d("hs_x") : 0.0;
+  P.hs_y = p.contains("hs_y") ? d("hs_y") : 0.0;
+  P.hs_sigma = p.contains("hs_sigma") ? d("hs_sigma") : 0.0;
   r->np = (int)np;
   r->nv = (int)nv;
   r->x = to_vec(x);
@@ -124,12 +132,13 @@ int64_t make_rollout(py::dict p, torch::Tensor x, torch::Tensor rho, torch::Tens
   const int n = (int)(np + nv);
   TORCH_CHECK((int)r->x.size() == 2 * n && (int)r->rho.size() == n && (int)r->u.size() == n,
               "x must be (n, 2), rho and u (n)");
-  const int rc = arm_create(&P, n, &r->ctx);
+  int rc = arm_create(&P, n, &r->ctx);
+  if (rc == ARM_OK) rc = arm_set_objective(r->ctx, (int)objective);
   if (rc != ARM_OK) {
     const std::string msg = r->ctx ? arm_last_error(r->ctx) : "";
     arm_destroy(r->ctx);
     delete r;
-    TORCH_CHECK(false, "arm_create failed (", rc, "): ", msg);
+    TORCH_CHECK(false, "rollout context failed (", rc, "): ", msg);
   }
   return reinterpret_cast<int64_t>(r);
 }
@@ -141,8 +150,8 @@ void free_rollout(int64_t h) {
   delete r;
 }
 
-torch::Tensor rollout_loss(int64_t h, torch::Tensor v0, torch::Tensor alpha) {
-  return RolloutFn::apply(v0, alpha, h);
+torch::Tensor rollout_loss(int64_t h, torch::Tensor v0, torch::Tensor alpha, torch::Tensor q0) {
+  return RolloutFn::apply(v0, alpha, q0, h);
 }
 
 }  // namespace
@@ -151,5 +160,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.doc() = "torch.autograd over the MI355X WCSPH adjoint rollout";
   m.def("make_rollout", &make_rollout, "create a rollout context (returns a handle)");
   m.def("free_rollout", &free_rollout, "destroy a rollout context");
-  m.def("rollout_loss", &rollout_loss, "J(v0, alpha) with a hand-written adjoint backward");
+  m.def("rollout_loss", &rollout_loss, "J(v0, alpha, q0) with a hand-written adjoint backward");
 }

@@ -15,7 +15,8 @@
 // per-step reordering); per-step scratch lives in the step's sorted order.
 // Pair terms are gathered per particle: a particle accumulates only the
 // derivatives of its own inputs, each pair being visited from both ends, so
-// no atomics are needed.  The objective is J = sum_steps dt sum_obj p^2.
+// no atomics are needed.  The objective is J = sum_steps dt sum_obj f,
+// f = p^2 or u^2; the heat source (flag 64) adds dJ/d hs_q0.
 #pragma once
 
 #include "sph_kernels.h"
@@ -56,6 +57,7 @@ struct AdjState {
 struct AdjScratch {
   double *lax, *lay, *ldrho, *ldu, *lavx, *lavy, *lp;
   double *lQ, *lA, *lpsi, *lS, *lrhoh, *lu, *lvx, *lvy, *lx, *ly;
+  double* lus;  // objective seed of the energy (ARM_OBJ_ENERGY)
 };
 
 __global__ __launch_bounds__(BLOCK) void k_tape(St<double> S, KParams K,
@@ -84,14 +86,18 @@ __global__ __launch_bounds__(BLOCK) void k_tape(St<double> S, KParams K,
   T.skey[t] = skey[t];
 }
 
-// objective: jpart[block] += dt sum_{obj} p^2 over the block's particles
-__global__ __launch_bounds__(BLOCK) void k_objective(St<double> S, KParams K,
+// objective: jpart[block] += dt sum_{obj} f over the block's particles,
+// f = p^2 (kind 0) or u^2 (kind 1)
+__global__ __launch_bounds__(BLOCK) void k_objective(St<double> S, KParams K, int kind,
                                                      const uint8_t* __restrict__ objmask,
                                                      double* __restrict__ jpart) {
   __shared__ double sh[BLOCK];
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   double v = 0.0;
-  if (i < K.n && objmask[S.id[i]]) v = K.dt * S.p[i] * S.p[i];
+  if (i < K.n && objmask[S.id[i]]) {
+    const double f = kind == 1 ? S.u[i] : S.p[i];
+    v = K.dt * f * f;
+  }
   sh[threadIdx.x] = v;
   __syncthreads();
   for (int s = BLOCK / 2; s > 0; s >>= 1) {
@@ -108,7 +114,7 @@ __global__ __launch_bounds__(BLOCK) void k_objective(St<double> S, KParams K,
 __global__ __launch_bounds__(BLOCK) void k_adj_corrector(Tape T, KParams K, int its,
                                                          AdjState L, AdjScratch W,
                                                          const uint8_t* __restrict__ objmask,
-                                                         double seed) {
+                                                         int kind, double seed) {
   const int k = blockIdx.x * BLOCK + threadIdx.x;
   if (k >= K.n) return;
   const int id = T.id[k];
@@ -155,7 +161,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_corrector(Tape T, KParams K, int 
   W.ldu[k] = ldu;
   W.lavx[k] = lavx;
   W.lavy[k] = lavy;
-  W.lp[k] = objmask[id] ? seed * 2.0 * dt * T.p[k] : 0.0;
+  W.lp[k] = (objmask[id] && kind == 0) ? seed * 2.0 * dt * T.p[k] : 0.0;
+  W.lus[k] = (objmask[id] && kind == 1) ? seed * 2.0 * dt * T.u[k] : 0.0;
 }
 
 __device__ __forceinline__ void wendland_q(const KParams& K, double r2, double& q, double& F,
@@ -175,8 +182,10 @@ __device__ __forceinline__ Rows tape_rows(const KParams& K, const Tape& T, int k
 //    neighbour l (as j); accumulates k's own input adjoints.
 template <bool AV, bool AH, bool XS>
 __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape T, KParams K, AdjScratch W,
-                                                     double* __restrict__ alpha_part) {
+                                                     double* __restrict__ alpha_part,
+                                                     double* __restrict__ q0_part) {
   __shared__ double sh[BLOCK];
+  double lq0 = 0.0;
   const int k = blockIdx.x * BLOCK + threadIdx.x;
   double lal = 0.0;
   if (k < K.n) {
@@ -268,6 +277,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape T, KParams K, AdjScrat
         lvx += ldvx;
         lvy += ldvy;
       }
+    // heat source du_k += q0 E_k Q_k, E = exp(-hs_k |x_k - c|^2)
+    if ((K.flags & 64) && T.id[k] < K.np) {
+      const double E = heat_shape(K, xk.x, xk.y);
+      lq0 = lduk * E * Qk;
+      lQ += lduk * K.hs_q0 * E;
+      const double lE = lduk * K.hs_q0 * Qk * E * (-2.0 * K.hs_k);
+      lx += lE * (xk.x - K.hs_x);
+      ly += lE * (xk.y - K.hs_y);
+    }
     W.lx[k] = lx;
     W.ly[k] = ly;
     W.lvx[k] = lvx;
@@ -275,7 +293,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape T, KParams K, AdjScrat
     W.lQ[k] = lQ;
     W.lA[k] = lA;
     W.lpsi[k] = lpsi;
-    W.lu[k] = lu;
+    W.lu[k] = lu + W.lus[k];
   }
   // each pair is visited from both ends: half of its alpha derivative each
   sh[threadIdx.x] = 0.5 * lal;
@@ -285,6 +303,14 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape T, KParams K, AdjScrat
     __syncthreads();
   }
   if (threadIdx.x == 0) alpha_part[blockIdx.x] += sh[0];
+  __syncthreads();
+  sh[threadIdx.x] = lq0;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) q0_part[blockIdx.x] += sh[0];
 }
 
 // 3. EOS and art_heat coefficient adjoint (per particle)

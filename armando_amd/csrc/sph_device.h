@@ -36,7 +36,14 @@ struct KParams {
   double lin_p0;
   double lin_p0r;  // lin_p0 + lin_c2 rho0 (FP32 form: p = lin_c2 rho' + lin_p0r)
   double eta;      // dynamic viscosity (flag 32)
+  double hs_q0, hs_x, hs_y, hs_k;  // heat source (flag 64): q0 exp(-k |x - c|^2) / rho
 };
+
+// Gaussian heat source factor exp(-k |x - c|^2) at a position.
+__device__ __forceinline__ double heat_shape(const KParams& K, double x, double y) {
+  const double dx = x - K.hs_x, dy = y - K.hs_y;
+  return exp(-K.hs_k * (dx * dx + dy * dy));
+}
 
 // Particle state in sorted (cell, id) order.  FP64 form: xy holds the
 // positions, rho the density.  FP32 form: xy holds the FP64 position

@@ -436,8 +436,13 @@ __device__ __forceinline__ void force_end(St<R>& S, const KParams& K, int i, con
     acc.x = o.fx + ex;
     acc.y = o.fy + ey;
     S.a[i] = acc;
-    // du = tds/dt + the pair sums (internal_force.f:433-435)
-    S.du[i] = VI ? o.du.get() + o.tds : o.du.get();
+    // du = tds/dt + the pair sums (internal_force.f:433-435) [+ heat source]
+    R du = VI ? o.du.get() + o.tds : o.du.get();
+    if ((K.flags & 64) && id < K.np) {
+      const double2 q = S.xy[i];
+      du = du + (R)(K.hs_q0 * heat_shape(K, q.x, q.y)) * o.Q;
+    }
+    S.du[i] = du;
   }
   if (XS) {
     R2 w;

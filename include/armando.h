@@ -55,6 +55,10 @@ extern "C" {
 #define ARM_VISC 32    /* visc -> physical viscosity in int_force: shear
                           tensor, viscous force, tds/dt
                           [CERNsource/internal_force.f:73-171, 196-430] */
+#define ARM_HEAT_SOURCE 64 /* volumetric Gaussian heat source on real particles:
+                              du += hs_q0 exp(-|x - (hs_x, hs_y)|^2 / (2 hs_sigma^2)) / rho
+                              (the external heating slot exdudt of
+                              CERNsource/single_step.f:217-222) */
 
 /* equations of state */
 #define ARM_EOS_TAIT 0   /* p = rho0 c0^2/7 ((rho/rho0)^7 - 1), c = c0
@@ -83,6 +87,7 @@ typedef struct arm_params {
   double visc_eta; /* dynamic viscosity for ARM_VISC (eta of single_step.f:126-184) */
   int eos;         /* ARM_EOS_*; both clamp p < pmin to p = 0, rho = rho0 */
   double eos_p0;   /* ARM_EOS_LINEAR pressure offset */
+  double hs_q0, hs_x, hs_y, hs_sigma; /* ARM_HEAT_SOURCE amplitude (W/m^3), centre, width */
 } arm_params;
 
 typedef struct arm_ctx arm_ctx;
@@ -151,8 +156,9 @@ int arm_time_integration(arm_ctx* ctx, int its0, int nsteps);
    predictor run in reverse.  The neighbour list of each step is that of the
    forward step.  Outputs (any may be NULL): J, dJ/d av_alpha, dJ/dv0 as
    (2, n) in the original order, and the forward / backward device times in
-   ms.  The context is left at the state after nsteps.  FP64 form and
-   continuity density only. */
+   ms (dJ/d hs_q0: arm_rollout_param_grads).  The context is left at the
+   state after nsteps.  FP64 form, continuity density, Tait EOS, no physical
+   viscosity. */
 int arm_adjoint_rollout(arm_ctx* ctx, int nsteps, int ckpt_every, const int32_t* obj_ids,
                         int nobj, double* J, double* dJ_dalpha, double* dJ_dv0, double* ms_fwd,
                         double* ms_bwd);
@@ -164,9 +170,22 @@ int arm_rollout_forward(arm_ctx* ctx, int nsteps, int ckpt_every, const int32_t*
 int arm_rollout_backward(arm_ctx* ctx, double seed, double* dJ_dalpha, double* dJ_dv0,
                          double* ms);
 /* Replace the physical scalars (mass, rho0, c0, pmin, viscosity, heat, XSPH,
-   gravity, dt) of a context; grid, h, precision and flags must not change.
-   Drops the checkpoints of a previous rollout. */
+   gravity, dt, heat source) of a context; grid, h, precision and flags must
+   not change.  Drops the checkpoints of a previous rollout. */
 int arm_update_params(arm_ctx* ctx, const arm_params* prm);
+/* Objective of the next rollout: J = sum over steps of dt * sum_{obj} f_i
+   with f = p^2 (ARM_OBJ_PRESSURE, the default) or u^2 (ARM_OBJ_ENERGY), taken
+   after the EOS of each step.  No counterpart in the reference. */
+#define ARM_OBJ_PRESSURE 0
+#define ARM_OBJ_ENERGY 1
+int arm_set_objective(arm_ctx* ctx, int kind);
+/* Parameter gradients of the last arm_rollout_backward (scaled by its seed):
+   out[ARM_GRAD_ALPHA] = dJ/d av_alpha, out[ARM_GRAD_Q0] = dJ/d hs_q0.  n is
+   the length of out.  No counterpart in the reference. */
+#define ARM_GRAD_ALPHA 0
+#define ARM_GRAD_Q0 1
+#define ARM_NGRAD 2
+int arm_rollout_param_grads(arm_ctx* ctx, double* out, int n);
 
 /* FP64 sums over real particles: mass, momentum x, momentum y, kinetic
    energy, internal energy. */

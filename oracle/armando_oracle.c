@@ -474,6 +474,13 @@ static inline double tait_p(const orc_params* P, double rho) {
   return P->rho0 * P->c0 * P->c0 / 7.0 * (t4 * t2 * t - 1.0);
 }
 
+/* Gaussian heat source rate per unit mass at a particle. */
+static inline double heat_source(const orc_params* P, double x, double y, double rho) {
+  const double dx = x - P->hs_x, dy = y - P->hs_y;
+  const double k = 1.0 / (2.0 * P->hs_sigma * P->hs_sigma);
+  return P->hs_q0 * exp(-k * (dx * dx + dy * dy)) / rho;
+}
+
 void orc_eos_tait(const orc_params* P, int n, double* rho, double* p, double* c) {
   /* eos.f90:255-283 */
   for (int i = 0; i < n; ++i) {
@@ -571,6 +578,8 @@ int orc_single_step(const orc_params* P, orc_state* s, int niac, const int* pi,
     s->ax[i] = s->ax[i] + ex + avdv_x[i];
     s->ay[i] = s->ay[i] + ey + avdv_y[i];
     s->du[i] = s->du[i] + avdu[i] + ahdu[i];
+    if ((P->flags & ORC_HEAT_SOURCE) && i < np)
+      s->du[i] = s->du[i] + heat_source(P, s->x[i], s->y[i], s->rho[i]);
   }
   free(avdv_x);
   free(avdv_y);
@@ -795,6 +804,7 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
       tds = 0.5 * eta / rho[i] * (ti[0] * ti[0] + ti[1] * ti[1] + 2.0 * ti[2] * ti[2]);
     }
     s->du[i] = (tds + 0.5 * due) + avdu + ahdu;
+    if ((P->flags & ORC_HEAT_SOURCE) && i < np) s->du[i] += heat_source(P, x[i], y[i], rho[i]);
     s->avx[i] = xsx;
     s->avy[i] = xsy;
   }

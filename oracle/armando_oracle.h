@@ -65,8 +65,10 @@ enum {
   ORC_AV_VEL = 4,    /* av_vel.f (XSPH) */
   ORC_DBC = 8,       /* walls integrate density and energy (armando2Dcpu.cu:832-907) */
   ORC_GRAVITY = 16,  /* body force on fluid (armando2Dcpu.cu:937-958) */
-  ORC_VISC = 32      /* physical viscosity: shear tensor, viscous force and
+  ORC_VISC = 32,     /* physical viscosity: shear tensor, viscous force and
                         tds/dt of internal_force.f:73-171, uniform eta */
+  ORC_HEAT_SOURCE = 64 /* du += hs_q0 exp(-|x - c|^2/(2 hs_sigma^2)) / rho on real
+                          particles (the exdudt slot of single_step.f:217-222) */
 };
 
 enum {
@@ -94,6 +96,7 @@ typedef struct {
   double visc_eta; /* dynamic viscosity (ORC_VISC) */
   int eos;         /* ORC_EOS_* ; c = c0 for both, clamp p < pmin -> 0, rho0 */
   double eos_p0;   /* ORC_EOS_LINEAR pressure offset */
+  double hs_q0, hs_x, hs_y, hs_sigma; /* ORC_HEAT_SOURCE */
 } orc_params;
 
 /* Particle state, SoA, FP64. */

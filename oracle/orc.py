@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "_build", "liborc.so")
 REF_PATH = os.path.join(HERE, "_ref", "libarmref.so")
 
 K_WENDLAND2D, K_CUBIC2D, K_CUBIC3D = 0, 1, 2
-ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC = 1, 2, 4, 8, 16, 32
+ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
 EOS_TAIT, EOS_LINEAR = 0, 1
 
 
@@ -30,6 +30,8 @@ class Params(C.Structure):
         ("ox", C.c_double), ("oy", C.c_double), ("cell", C.c_double),
         ("nx", C.c_int), ("ny", C.c_int),
         ("visc_eta", C.c_double), ("eos", C.c_int), ("eos_p0", C.c_double),
+        ("hs_q0", C.c_double), ("hs_x", C.c_double), ("hs_y", C.c_double),
+        ("hs_sigma", C.c_double),
     ]
 
 

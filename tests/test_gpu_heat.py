@@ -1,0 +1,144 @@
+"""The volumetric Gaussian heat source on the GPU (forward against the
+oracle) and its adjoint: dJ/d hs_q0 for an energy objective against central
+differences of the GPU forward, the pressure objective's exact zero, and the
+torch autograd Function's third input."""
+import numpy as np
+import pytest
+
+import armando_amd
+from armando_amd import cases
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+FIELDS_IN = ("x", "y", "vx", "vy", "rho", "u")
+
+
+def banner(msg):
+    print(f"[gpu-heat] {msg}", flush=True)
+
+
+def relmax(a, b):
+    scale = max(float(np.max(np.abs(b))), 1e-300)
+    return float(np.max(np.abs(a - b))) / scale
+
+
+def heated_dam(scale=0.02, q0=2.0e8, precision=cases.FP64):
+    c = cases.dam_break_small(scale=scale, precision=precision)
+    w, hgt = 0.6 * scale, 1.2 * scale
+    c = cases.with_heat_source(c, q0, 0.5 * w, 0.3 * hgt, 0.15 * w)
+    nf = c["np"]
+    z = cases.normal01(777, 2 * nf)
+    c["vx"] = c["vx"].copy()
+    c["vy"] = c["vy"].copy()
+    c["vx"][:nf] = 0.01 * z[:nf]
+    c["vy"][:nf] = 0.01 * z[nf:]
+    obj = np.nonzero((c["x"][:nf] > 0.25 * w) & (c["x"][:nf] < 0.75 * w)
+                     & (c["y"][:nf] < 0.6 * hgt))[0].astype(np.int32)
+    return c, obj
+
+
+def test_heat_source_step_matches_oracle():
+    c, _ = heated_dam()
+    banner(f"heat source single step + 10 steps n={c['n']}")
+    sim = armando_amd.Simulation(c)
+    sim.link_list()
+    sim.single_step()
+    g = sim.rates()
+    P = orc.params_from(c)
+    st = orc.OracleState(c["n"], c["np"])
+    st.set(**{k: c[k] for k in FIELDS_IN})
+    orc.single_step_gather(P, st)
+    assert relmax(g["du"], st.field("du")) < 1e-10
+    sim.close()
+    sim = armando_amd.Simulation(c)
+    sim.time_integration(1, 10)
+    s = sim.state()
+    st = orc.OracleState(c["n"], c["np"])
+    st.set(**{k: c[k] for k in FIELDS_IN})
+    orc.run(P, st, 1, 10, gather=True)
+    for k in ("u", "rho", "x", "y"):
+        assert relmax(s[k], st.field(k)) < 1e-10, k
+    assert np.max(s["u"]) > 0
+    sim.close()
+
+
+def test_heat_source_fp32_close_to_fp64():
+    a = armando_amd.Simulation(heated_dam()[0])
+    b = armando_amd.Simulation(heated_dam(precision=cases.FP32)[0])
+    a.time_integration(1, 40)
+    b.time_integration(1, 40)
+    ua, ub = a.state()["u"], b.state()["u"]
+    rel = np.linalg.norm(ub - ua) / np.linalg.norm(ua)
+    banner(f"fp32 vs fp64 u rel-L2 {rel:.2e}")
+    assert rel < 1e-4
+    a.close()
+    b.close()
+
+
+def rollout(c, obj, steps, ckpt, kind, q0=None, backward=True):
+    c2 = dict(c)
+    if q0 is not None:
+        c2["hs_q0"] = q0
+    sim = armando_amd.Simulation(c2)
+    sim.set_objective(kind)
+    J = sim.rollout_forward(steps, ckpt, obj)[0]
+    out = (J,)
+    if backward:
+        da, g, _ = sim.rollout_backward(1.0)
+        out = (J, sim.param_grads(), g)
+    sim.close()
+    return out
+
+
+def test_heat_source_adjoint_against_central_differences():
+    c, obj = heated_dam()
+    steps, ck = 20, 6
+    banner(f"dJ/dq0, energy objective, n={c['n']} steps={steps}")
+    J, (da, dq), g = rollout(c, obj, steps, ck, armando_amd.OBJ_ENERGY)
+    q0 = c["hs_q0"]
+    e = 1e-4 * q0
+    Jp = rollout(c, obj, steps, ck, armando_amd.OBJ_ENERGY, q0=q0 + e, backward=False)[0]
+    Jm = rollout(c, obj, steps, ck, armando_amd.OBJ_ENERGY, q0=q0 - e, backward=False)[0]
+    fd = (Jp - Jm) / (2 * e)
+    banner(f"  fd {fd:.8e} adjoint {dq:.8e}")
+    assert J > 0 and abs(fd - dq) <= 1e-6 * abs(fd)
+    # velocities, energy objective
+    nf = c["np"]
+    for seed in (1001, 1002):
+        z = cases.normal01(seed, 2 * nf)
+        d = np.zeros((c["n"], 2))
+        d[:nf, 0], d[:nf, 1] = z[:nf], z[nf:]
+        ev = 1e-6
+        cp, cm = dict(c), dict(c)
+        cp["vx"], cp["vy"] = c["vx"] + ev * d[:, 0], c["vy"] + ev * d[:, 1]
+        cm["vx"], cm["vy"] = c["vx"] - ev * d[:, 0], c["vy"] - ev * d[:, 1]
+        fdv = (rollout(cp, obj, steps, ck, 1, backward=False)[0]
+               - rollout(cm, obj, steps, ck, 1, backward=False)[0]) / (2 * ev)
+        ad = float(np.sum(g * d))
+        assert abs(fdv - ad) <= 1e-5 * abs(fdv), (seed, fdv, ad)
+    # the pressure objective does not see the internal energy
+    _, (da_p, dq_p), _ = rollout(c, obj, steps, ck, armando_amd.OBJ_PRESSURE)
+    assert dq_p == 0.0 and da_p != 0.0
+
+
+def test_torch_autograd_with_heat_source():
+    import torch
+
+    from armando_amd.autograd import RolloutLoss
+
+    c, obj = heated_dam()
+    steps = 8
+    banner("torch autograd (v0, alpha, q0)")
+    ref = rollout(c, obj, steps, 4, armando_amd.OBJ_ENERGY)
+    loss = RolloutLoss(c, obj, steps, 4, objective=armando_amd.OBJ_ENERGY)
+    v0 = torch.tensor(np.stack([c["vx"], c["vy"]], 1), dtype=torch.float64, requires_grad=True)
+    alpha = torch.tensor(c["av_alpha"], dtype=torch.float64, requires_grad=True)
+    q0 = torch.tensor(c["hs_q0"], dtype=torch.float64, requires_grad=True)
+    J = loss(v0, alpha, q0)
+    assert J.item() == ref[0]
+    J.backward()
+    assert q0.grad.item() == pytest.approx(ref[1][1], rel=1e-12)
+    assert alpha.grad.item() == pytest.approx(ref[1][0], rel=1e-12)
+    assert np.allclose(v0.grad.numpy(), ref[2], rtol=1e-12, atol=0)
+    loss.close()

@@ -192,6 +192,31 @@ def test_gather_with_viscosity_equals_faithful_half_list():
     assert np.max(np.abs(a.field("vx")[:c["np"]])) > 1e-6  # the lid drags the fluid
 
 
+def test_heat_source_total_power_is_the_gaussian_integral():
+    """Fluid at rest on a regular lattice at rho0 (p = 0, no pair terms): the
+    heat source's total power sum m du = q0 dx^2 sum exp(-r^2/2s^2) must be
+    q0 * 2 pi s^2 per unit depth (the lattice sum of a Gaussian much wider
+    than the spacing equals its integral to round-off)."""
+    dx, sigma, q0 = 1e-3, 5e-3, 3.0e8
+    c = dict(cases.box_lattice(nx=90, ny=90, dx=dx, jitter=0.0, vel=0.0))
+    c["rho"] = np.full(c["n"], 1000.0)
+    c["u"] = np.zeros(c["n"])
+    c = cases.with_heat_source(c, q0, 45 * dx, 45 * dx, sigma)
+    P = orc.params_from(c)
+    st = orc.OracleState(c["n"], c["np"])
+    st.set(**{k: c[k] for k in ("x", "y", "vx", "vy", "rho", "u")})
+    orc.single_step_gather(P, st)
+    du = st.field("du").copy()
+    total = c["mass"] * du.sum()
+    assert abs(total / (q0 * 2.0 * np.pi * sigma ** 2) - 1.0) < 1e-9
+    # the faithful half-list step adds the same source
+    st2 = orc.OracleState(c["n"], c["np"])
+    st2.set(**{k: c[k] for k in ("x", "y", "vx", "vy", "rho", "u")})
+    pi, pj = orc.link_list(P, c["x"], c["y"])
+    assert orc.lib().orc_single_step(C.byref(P), st2.ptr, len(pi), pi, pj) == 0
+    assert np.max(np.abs(st2.field("du") - du)) <= 1e-12 * np.max(np.abs(du))
+
+
 # ---------------------------------------------------------------- fixtures
 def dam6462_positions():
     """initDamBreak of ArmandoList/armando2Dlist.cu:1713-1831 (float positions)."""
