@@ -459,7 +459,7 @@ struct StepHooks {
   const uint8_t* objmask = nullptr;  // objective: jpart += dt sum f (p^2 or u^2)
   double* jpart = nullptr;
   int kind = ARM_OBJ_PRESSURE;
-  const Tape* tape = nullptr;        // record the step after the force pass
+  const void* tape = nullptr;        // Tape<R>: record the step after the force pass
 };
 
 template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
@@ -484,18 +484,14 @@ template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
   if (!xs && (rc = zero_av<R>(c))) return rc;
   c->sorted = true;
   if ((rc = density_t<R>(c, c->prm.density_method, true))) return rc;
-  if constexpr (sizeof(R) == 8) {
-    if (hk.jpart)
-      hipLaunchKernelGGL(k_objective, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, stA<R>(c),
-                         c->K, hk.kind, hk.objmask, hk.jpart);
-  }
+  if (hk.jpart)
+    hipLaunchKernelGGL(k_objective<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, stA<R>(c),
+                       c->K, hk.kind, hk.objmask, hk.jpart);
   if ((rc = force_t<R>(c, true, c->prm.flags, xs))) return rc;
-  if constexpr (sizeof(R) == 8) {
-    if (hk.tape) {
-      const int nb = std::max(grid_of(n), grid_of(c->K.ncell + 1));
-      hipLaunchKernelGGL(k_tape, dim3(nb), dim3(BLOCK), 0, c->stream, stA<R>(c), c->K, c->skey,
-                         c->cstart, *hk.tape);
-    }
+  if (hk.tape) {
+    const int nb = std::max(grid_of(n), grid_of(c->K.ncell + 1));
+    hipLaunchKernelGGL(k_tape<R>, dim3(nb), dim3(BLOCK), 0, c->stream, stA<R>(c), c->K, c->skey,
+                       c->cstart, *static_cast<const Tape<R>*>(hk.tape));
   }
   if ((rc = corrector_t<R>(c, its, true))) return rc;
   return ARM_OK;
@@ -534,13 +530,16 @@ struct Scratch {
 };
 
 // the part of the state a step reads at its start
-struct Checkpoint {
+template <typename R> struct Checkpoint {
+  using R2 = typename V2<R>::type;
   int32_t* id;
-  double2 *xy, *v, *a;
-  double *rho, *u, *p, *du, *drho;
+  double2* xy;
+  R2 *v, *a;
+  R *rho, *p, *du, *drho;
+  double* u;
 };
 
-int alloc_checkpoint(Scratch& sc, Checkpoint& k, int n) {
+template <typename R> int alloc_checkpoint(Scratch& sc, Checkpoint<R>& k, int n) {
   int rc;
   if ((rc = sc.get(&k.id, n)) || (rc = sc.get(&k.xy, n)) || (rc = sc.get(&k.v, n)) ||
       (rc = sc.get(&k.a, n)) || (rc = sc.get(&k.rho, n)) || (rc = sc.get(&k.u, n)) ||
@@ -549,7 +548,9 @@ int alloc_checkpoint(Scratch& sc, Checkpoint& k, int n) {
   return ARM_OK;
 }
 
-int copy_checkpoint(arm_ctx* c, const St<double>& S, const Checkpoint& k, bool save) {
+template <typename R>
+int copy_checkpoint(arm_ctx* c, const St<R>& S, const Checkpoint<R>& k, bool save) {
+  using R2 = typename V2<R>::type;
   const size_t n = (size_t)c->n;
   auto cp = [&](void* st, void* ck, size_t bytes) {
     return save ? hipMemcpyAsync(ck, st, bytes, hipMemcpyDeviceToDevice, c->stream)
@@ -557,18 +558,21 @@ int copy_checkpoint(arm_ctx* c, const St<double>& S, const Checkpoint& k, bool s
   };
   HIPCHK(c, cp(S.id, k.id, n * sizeof(int32_t)));
   HIPCHK(c, cp(S.xy, k.xy, n * sizeof(double2)));
-  HIPCHK(c, cp(S.v, k.v, n * sizeof(double2)));
-  HIPCHK(c, cp(S.a, k.a, n * sizeof(double2)));
-  HIPCHK(c, cp(S.rho, k.rho, n * sizeof(double)));
+  HIPCHK(c, cp(S.v, k.v, n * sizeof(R2)));
+  HIPCHK(c, cp(S.a, k.a, n * sizeof(R2)));
+  HIPCHK(c, cp(S.rho, k.rho, n * sizeof(R)));
   HIPCHK(c, cp(S.u, k.u, n * sizeof(double)));
-  HIPCHK(c, cp(S.p, k.p, n * sizeof(double)));
-  HIPCHK(c, cp(S.du, k.du, n * sizeof(double)));
-  HIPCHK(c, cp(S.drho, k.drho, n * sizeof(double)));
+  HIPCHK(c, cp(S.p, k.p, n * sizeof(R)));
+  HIPCHK(c, cp(S.du, k.du, n * sizeof(R)));
+  HIPCHK(c, cp(S.drho, k.drho, n * sizeof(R)));
   return ARM_OK;
 }
 
-int alloc_tape(Scratch& sc, Tape& t, int n, int ncell) {
+template <typename R> int alloc_tape(Scratch& sc, Tape<R>& t, int n, int ncell) {
   int rc;
+  t.cxy = nullptr;
+  t.off = nullptr;
+  if (sizeof(R) == 4 && ((rc = sc.get(&t.cxy, n)) || (rc = sc.get(&t.off, n)))) return rc;
   if ((rc = sc.get(&t.id, n)) || (rc = sc.get(&t.xy, n)) || (rc = sc.get(&t.v, n)) ||
       (rc = sc.get(&t.rho, n)) || (rc = sc.get(&t.u, n)) || (rc = sc.get(&t.psi, n)) ||
       (rc = sc.get(&t.divv, n)) || (rc = sc.get(&t.p, n)) || (rc = sc.get(&t.drho, n)) ||
@@ -579,25 +583,27 @@ int alloc_tape(Scratch& sc, Tape& t, int n, int ncell) {
   return ARM_OK;
 }
 
-template <bool AV, bool AH, bool XS>
-void launch_adj_force(arm_ctx* c, const Tape& T, const AdjScratch& W, double* apart,
+template <typename R, bool AV, bool AH, bool XS>
+void launch_adj_force(arm_ctx* c, const Tape<R>& T, const AdjScratch<R>& W, double* apart,
                       double* qpart) {
-  hipLaunchKernelGGL((k_adj_force<AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
-                     T, c->K, W, apart, qpart);
+  hipLaunchKernelGGL((k_adj_force<R, AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
+                     c->stream, T, c->K, W, apart, qpart);
 }
 
-void adj_force(arm_ctx* c, const Tape& T, const AdjScratch& W, double* apart, double* qpart) {
+template <typename R>
+void adj_force(arm_ctx* c, const Tape<R>& T, const AdjScratch<R>& W, double* apart,
+               double* qpart) {
   const int f = c->prm.flags;
   const int code = ((f & ARM_ART_VISC) ? 4 : 0) | ((f & ARM_ART_HEAT) ? 2 : 0) | ((f & ARM_AV_VEL) ? 1 : 0);
   switch (code) {
-    case 0: launch_adj_force<false, false, false>(c, T, W, apart, qpart); break;
-    case 1: launch_adj_force<false, false, true>(c, T, W, apart, qpart); break;
-    case 2: launch_adj_force<false, true, false>(c, T, W, apart, qpart); break;
-    case 3: launch_adj_force<false, true, true>(c, T, W, apart, qpart); break;
-    case 4: launch_adj_force<true, false, false>(c, T, W, apart, qpart); break;
-    case 5: launch_adj_force<true, false, true>(c, T, W, apart, qpart); break;
-    case 6: launch_adj_force<true, true, false>(c, T, W, apart, qpart); break;
-    default: launch_adj_force<true, true, true>(c, T, W, apart, qpart); break;
+    case 0: launch_adj_force<R, false, false, false>(c, T, W, apart, qpart); break;
+    case 1: launch_adj_force<R, false, false, true>(c, T, W, apart, qpart); break;
+    case 2: launch_adj_force<R, false, true, false>(c, T, W, apart, qpart); break;
+    case 3: launch_adj_force<R, false, true, true>(c, T, W, apart, qpart); break;
+    case 4: launch_adj_force<R, true, false, false>(c, T, W, apart, qpart); break;
+    case 5: launch_adj_force<R, true, false, true>(c, T, W, apart, qpart); break;
+    case 6: launch_adj_force<R, true, true, false>(c, T, W, apart, qpart); break;
+    default: launch_adj_force<R, true, true, true>(c, T, W, apart, qpart); break;
   }
 }
 
@@ -605,16 +611,27 @@ void adj_force(arm_ctx* c, const Tape& T, const AdjScratch& W, double* apart, do
 
 // Checkpoints, tapes and adjoint arrays of one rollout, kept between
 // arm_rollout_forward and arm_rollout_backward.
+template <typename R> struct AdjArrays {
+  std::vector<Checkpoint<R>> ck;
+  std::vector<Tape<R>> tape;
+  AdjScratch<R> W{};
+};
+
 struct AdjRun {
   Scratch sc;
   int nsteps = 0, ckpt = 0, nck = 0;
   uint8_t* obj = nullptr;
   double *jpart = nullptr, *apart = nullptr, *qpart = nullptr;
   int kind = ARM_OBJ_PRESSURE;
-  std::vector<Checkpoint> ck;
-  std::vector<Tape> tape;
   AdjState L{};
-  AdjScratch W{};
+  AdjArrays<double> a64;
+  AdjArrays<float> a32;
+  template <typename R> AdjArrays<R>& arrays() {
+    if constexpr (sizeof(R) == 8)
+      return a64;
+    else
+      return a32;
+  }
   explicit AdjRun(arm_ctx* c) : sc(c) {}
 };
 
@@ -625,6 +642,7 @@ void drop_adjoint(arm_ctx* c) {
   c->adj = nullptr;
 }
 
+template <typename Rf>
 int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, int nobj, double* J,
                     double* ms) {
   const int n = c->n, ncell = c->K.ncell;
@@ -655,38 +673,39 @@ int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, in
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   HIPCHK(c, hipMemsetAsync(R->jpart, 0, sizeof(double) * nb, c->stream));
-  R->ck.resize(R->nck + 1);
-  for (auto& k : R->ck)
+  AdjArrays<Rf>& Ar = R->arrays<Rf>();
+  Ar.ck.resize(R->nck + 1);
+  for (auto& k : Ar.ck)
     if ((rc = alloc_checkpoint(sc, k, n))) return rc;
-  R->tape.resize(ckpt);
-  for (auto& t : R->tape)
+  Ar.tape.resize(ckpt);
+  for (auto& t : Ar.tape)
     if ((rc = alloc_tape(sc, t, n, ncell))) return rc;
   double** lf[] = {&R->L.x,   &R->L.y,   &R->L.vx,  &R->L.vy,  &R->L.rho,   &R->L.u,
                    &R->L.bx,  &R->L.by,  &R->L.bvx, &R->L.bvy, &R->L.brho,  &R->L.bu,
                    &R->L.cax, &R->L.cay, &R->L.cdrho, &R->L.cdu};
   for (double** q : lf)
     if ((rc = sc.get(q, n))) return rc;
-  double** wf[] = {&R->W.lax, &R->W.lay, &R->W.ldrho, &R->W.ldu, &R->W.lavx, &R->W.lavy,
-                   &R->W.lp,  &R->W.lQ,  &R->W.lA,    &R->W.lpsi, &R->W.lS,  &R->W.lrhoh,
-                   &R->W.lu,  &R->W.lvx, &R->W.lvy,   &R->W.lx,  &R->W.ly, &R->W.lus};
-  for (double** q : wf)
+  AdjScratch<Rf>& W = Ar.W;
+  Rf** wf[] = {&W.lax, &W.lay, &W.ldrho, &W.ldu, &W.lavx, &W.lavy, &W.lp,  &W.lQ,  &W.lA,
+               &W.lpsi, &W.lS, &W.lrhoh, &W.lu,  &W.lvx,  &W.lvy,  &W.lx,  &W.ly,  &W.lus};
+  for (Rf** q : wf)
     if ((rc = sc.get(q, n))) return rc;
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
   HIPCHK(c, hipEventRecord(e0, c->stream));
-  if ((rc = begin_steps<double>(c))) return rc;
-  if (!(c->prm.flags & ARM_AV_VEL) && (rc = zero_av<double>(c))) return rc;
+  if ((rc = begin_steps<Rf>(c))) return rc;
+  if (!(c->prm.flags & ARM_AV_VEL) && (rc = zero_av<Rf>(c))) return rc;
   StepHooks fwd;
   fwd.objmask = R->obj;
   fwd.jpart = R->jpart;
   fwd.kind = R->kind;
   for (int its = 1; its <= nsteps; ++its) {
     if ((its - 1) % ckpt == 0)
-      if ((rc = copy_checkpoint(c, c->A64, R->ck[(its - 1) / ckpt], true))) return rc;
-    if ((rc = fused_step<double>(c, its, fwd))) return rc;
+      if ((rc = copy_checkpoint(c, stA<Rf>(c), Ar.ck[(its - 1) / ckpt], true))) return rc;
+    if ((rc = fused_step<Rf>(c, its, fwd))) return rc;
   }
-  if ((rc = copy_checkpoint(c, c->A64, R->ck[R->nck], true))) return rc;  // final state
+  if ((rc = copy_checkpoint(c, stA<Rf>(c), Ar.ck[R->nck], true))) return rc;  // final state
   HIPCHK(c, hipEventRecord(e1, c->stream));
   std::vector<double> jp(nb);
   HIPCHK(c, hipMemcpyAsync(jp.data(), R->jpart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
@@ -705,12 +724,14 @@ int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, in
   return check_domain(c);
 }
 
+template <typename Rf>
 int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0, double* ms) {
   AdjRun* R = c->adj;
   const int n = c->n, nb = grid_of(n);
   const int nsteps = R->nsteps, ckpt = R->ckpt, nck = R->nck;
   AdjState& L = R->L;
-  AdjScratch& W = R->W;
+  AdjArrays<Rf>& Ar = R->arrays<Rf>();
+  AdjScratch<Rf>& W = Ar.W;
   int rc;
   double** lf[] = {&L.x, &L.y, &L.vx, &L.vy, &L.rho, &L.u, &L.cax, &L.cay, &L.cdrho, &L.cdu};
   for (double** q : lf) HIPCHK(c, hipMemsetAsync(*q, 0, sizeof(double) * n, c->stream));
@@ -722,21 +743,22 @@ int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0,
   HIPCHK(c, hipEventRecord(e0, c->stream));
   for (int seg = nck - 1; seg >= 0; --seg) {
     const int s0 = 1 + seg * ckpt, s1 = std::min(s0 + ckpt, nsteps + 1);
-    if ((rc = copy_checkpoint(c, c->A64, R->ck[seg], false))) return rc;
-    if ((rc = begin_steps<double>(c))) return rc;
+    if ((rc = copy_checkpoint(c, stA<Rf>(c), Ar.ck[seg], false))) return rc;
+    if ((rc = begin_steps<Rf>(c))) return rc;
     for (int its = s0; its < s1; ++its) {
       StepHooks rec;
-      rec.tape = &R->tape[its - s0];
-      if ((rc = fused_step<double>(c, its, rec))) return rc;
+      rec.tape = &Ar.tape[its - s0];
+      if ((rc = fused_step<Rf>(c, its, rec))) return rc;
     }
     for (int its = s1 - 1; its >= s0; --its) {
-      const Tape& T = R->tape[its - s0];
-      hipLaunchKernelGGL(k_adj_corrector, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its, L, W,
-                         R->obj, R->kind, seed);
-      adj_force(c, T, W, R->apart, R->qpart);
-      hipLaunchKernelGGL(k_adj_eos, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
-      hipLaunchKernelGGL(k_adj_density, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
-      hipLaunchKernelGGL(k_adj_predictor, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its, L, W);
+      const Tape<Rf>& T = Ar.tape[its - s0];
+      hipLaunchKernelGGL(k_adj_corrector<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its,
+                         L, W, R->obj, R->kind, seed);
+      adj_force<Rf>(c, T, W, R->apart, R->qpart);
+      hipLaunchKernelGGL(k_adj_eos<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
+      hipLaunchKernelGGL(k_adj_density<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
+      hipLaunchKernelGGL(k_adj_predictor<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its,
+                         L, W);
       HIPCHK(c, hipGetLastError());
       // the state adjoint before this step is the one after the previous step
       std::swap(L.x, L.bx);
@@ -749,7 +771,7 @@ int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0,
   }
   HIPCHK(c, hipEventRecord(e1, c->stream));
   // leave the context at the final forward state
-  if ((rc = copy_checkpoint(c, c->A64, R->ck[nck], false))) return rc;
+  if ((rc = copy_checkpoint(c, stA<Rf>(c), Ar.ck[nck], false))) return rc;
   std::vector<double> ap(nb), qp(nb), gx(n), gy(n);
   HIPCHK(c, hipMemcpyAsync(ap.data(), R->apart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(qp.data(), R->qpart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
@@ -1306,10 +1328,11 @@ int arm_rollout_forward(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* o
                         double* J, double* ms) {
   if (!c || nsteps < 1 || ckpt_every < 1 || nobj < 0 || (nobj > 0 && !obj_ids)) return ARM_ERR_ARG;
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
-  if (is32(c)) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint runs in the FP64 form");
   if (c->prm.density_method != 0)
     return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint differentiates continuity density only");
-  const int rc = rollout_forward(c, nsteps, std::min(ckpt_every, nsteps), obj_ids, nobj, J, ms);
+  const int ck = std::min(ckpt_every, nsteps);
+  const int rc = is32(c) ? rollout_forward<float>(c, nsteps, ck, obj_ids, nobj, J, ms)
+                         : rollout_forward<double>(c, nsteps, ck, obj_ids, nobj, J, ms);
   if (rc) drop_adjoint(c);
   return rc;
 }
@@ -1317,7 +1340,8 @@ int arm_rollout_forward(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* o
 int arm_rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0, double* ms) {
   if (!c) return ARM_ERR_ARG;
   if (!c->adj) return fail(c, ARM_ERR_STATE, "no arm_rollout_forward to differentiate");
-  return rollout_backward(c, seed, dJ_dalpha, dJ_dv0, ms);
+  return is32(c) ? rollout_backward<float>(c, seed, dJ_dalpha, dJ_dv0, ms)
+                 : rollout_backward<double>(c, seed, dJ_dalpha, dJ_dv0, ms);
 }
 
 int arm_adjoint_rollout(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* obj_ids, int nobj,

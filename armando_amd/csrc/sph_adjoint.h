@@ -1,4 +1,8 @@
-// sph_adjoint.h -- discrete adjoint of the FP64 step (reverse mode).
+// sph_adjoint.h -- discrete adjoint of the step (reverse mode), for the FP64
+// plain form (R = double) and the FP32 chosen form (R = float: pair
+// arithmetic in FP32 on the cell + offset positions, as the forward; the
+// adjoint of the state is accumulated in FP64 like the forward's position
+// and energy accumulators).
 //
 // Differentiates the step exactly as the forward kernels compute it:
 //   density   drho_i = sum_j m F vr,  S_i = -sum_j m F vr (divv_i = S_i/rho_i)
@@ -25,26 +29,39 @@ namespace arm {
 
 // one recorded step: the sorted state after the force pass (before the
 // corrector), the keys and the cell ranges of that step
-struct Tape {
+template <typename R> struct Tape {
+  using R2 = typename V2<R>::type;
   int32_t* id;
   double2* xy;
-  double2* v;   // predicted (half-step) velocity
-  double* rho;  // density after the EOS clamp
-  double* u;    // predicted energy
-  double* psi;
-  double* divv;
-  double* p;
-  double* drho;
-  double* du;
-  double2* a;
-  double2* av;
-  double* rho_min;
+  uint32_t* cxy;  // FP32 form: cell and in-cell offset of the pair arithmetic
+  float2* off;
+  R2* v;      // predicted (half-step) velocity
+  R* rho;     // density after the EOS clamp (FP32 form: rho - rho0)
+  double* u;  // predicted energy
+  R* psi;
+  R* divv;
+  R* p;
+  R* drho;
+  R* du;
+  R2* a;
+  R2* av;
+  R* rho_min;
   double* u_min;
-  double2* v_min;
+  R2* v_min;
   uint8_t* flag;
   uint32_t* skey;
   int* cstart;
 };
+
+__device__ __forceinline__ PosD tape_pos(const Tape<double>& T, int t) {
+  const double2 q = T.xy[t];
+  return {q.x, q.y};
+}
+__device__ __forceinline__ PosF tape_pos(const Tape<float>& T, int t) {
+  const uint32_t c = T.cxy[t];
+  const float2 o = T.off[t];
+  return {int(c & 0xffffu), int(c >> 16), o.x, o.y};
+}
 
 // adjoint variables in id order
 struct AdjState {
@@ -54,21 +71,26 @@ struct AdjState {
 };
 
 // per-step scratch in sorted order
-struct AdjScratch {
-  double *lax, *lay, *ldrho, *ldu, *lavx, *lavy, *lp;
-  double *lQ, *lA, *lpsi, *lS, *lrhoh, *lu, *lvx, *lvy, *lx, *ly;
-  double* lus;  // objective seed of the energy (ARM_OBJ_ENERGY)
+template <typename R> struct AdjScratch {
+  R *lax, *lay, *ldrho, *ldu, *lavx, *lavy, *lp;
+  R *lQ, *lA, *lpsi, *lS, *lrhoh, *lu, *lvx, *lvy, *lx, *ly;
+  R* lus;  // objective seed of the energy (ARM_OBJ_ENERGY)
 };
 
-__global__ __launch_bounds__(BLOCK) void k_tape(St<double> S, KParams K,
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_tape(St<R> S, KParams K,
                                                 const uint32_t* __restrict__ skey,
-                                                const int* __restrict__ cstart, Tape T) {
+                                                const int* __restrict__ cstart, Tape<R> T) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   const int stride = gridDim.x * BLOCK;
   for (int c = t; c <= K.ncell; c += stride) T.cstart[c] = cstart[c];
   if (t >= K.n) return;
   T.id[t] = S.id[t];
   T.xy[t] = S.xy[t];
+  if constexpr (sizeof(R) == 4) {
+    T.cxy[t] = S.cxy[t];
+    T.off[t] = S.off[t];
+  }
   T.v[t] = S.v[t];
   T.rho[t] = S.rho[t];
   T.u[t] = S.u[t];
@@ -88,14 +110,15 @@ __global__ __launch_bounds__(BLOCK) void k_tape(St<double> S, KParams K,
 
 // objective: jpart[block] += dt sum_{obj} f over the block's particles,
 // f = p^2 (kind 0) or u^2 (kind 1)
-__global__ __launch_bounds__(BLOCK) void k_objective(St<double> S, KParams K, int kind,
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_objective(St<R> S, KParams K, int kind,
                                                      const uint8_t* __restrict__ objmask,
                                                      double* __restrict__ jpart) {
   __shared__ double sh[BLOCK];
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   double v = 0.0;
   if (i < K.n && objmask[S.id[i]]) {
-    const double f = kind == 1 ? S.u[i] : S.p[i];
+    const double f = kind == 1 ? S.u[i] : (double)S.p[i];
     v = K.dt * f * f;
   }
   sh[threadIdx.x] = v;
@@ -111,8 +134,9 @@ __global__ __launch_bounds__(BLOCK) void k_objective(St<double> S, KParams K, in
 //    the step -> rate adjoints (sorted) and the direct part of the state
 //    adjoints before the step (id order); the objective, scaled by the
 //    incoming adjoint seed = dL/dJ, seeds lp.
-__global__ __launch_bounds__(BLOCK) void k_adj_corrector(Tape T, KParams K, int its,
-                                                         AdjState L, AdjScratch W,
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_adj_corrector(Tape<R> T, KParams K, int its,
+                                                         AdjState L, AdjScratch<R> W,
                                                          const uint8_t* __restrict__ objmask,
                                                          int kind, double seed) {
   const int k = blockIdx.x * BLOCK + threadIdx.x;
@@ -131,7 +155,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_corrector(Tape T, KParams K, int 
     brho = L.rho[id];
     ldrho += c * L.rho[id];
     const double ub = (its != 1) ? T.u_min[k] : T.u[k];  // its == 1: u is the state u
-    const double uu = ub + c * T.du[k];
+    const double uu = ub + c * (double)T.du[k];
     if (!(uu < 0.0)) {
       bu = L.u[id];
       ldu += c * L.u[id];
@@ -155,78 +179,82 @@ __global__ __launch_bounds__(BLOCK) void k_adj_corrector(Tape T, KParams K, int 
   L.bvy[id] = bvy;
   L.brho[id] = brho;
   L.bu[id] = bu;
-  W.lax[k] = lax;
-  W.lay[k] = lay;
-  W.ldrho[k] = is_rate(K, id) ? ldrho : 0.0;
-  W.ldu[k] = ldu;
-  W.lavx[k] = lavx;
-  W.lavy[k] = lavy;
-  W.lp[k] = (objmask[id] && kind == 0) ? seed * 2.0 * dt * T.p[k] : 0.0;
-  W.lus[k] = (objmask[id] && kind == 1) ? seed * 2.0 * dt * T.u[k] : 0.0;
+  W.lax[k] = (R)lax;
+  W.lay[k] = (R)lay;
+  W.ldrho[k] = is_rate(K, id) ? (R)ldrho : (R)0;
+  W.ldu[k] = (R)ldu;
+  W.lavx[k] = (R)lavx;
+  W.lavy[k] = (R)lavy;
+  W.lp[k] = (objmask[id] && kind == 0) ? (R)(seed * 2.0 * dt * (double)T.p[k]) : (R)0;
+  W.lus[k] = (objmask[id] && kind == 1) ? (R)(seed * 2.0 * dt * T.u[k]) : (R)0;
 }
 
-__device__ __forceinline__ void wendland_q(const KParams& K, double r2, double& q, double& F,
-                                           double& Wv) {
-  q = sqrt(r2) * K.invh;
-  const double t = q - 2.0;
-  F = K.C1 * (t * t * t);
-  const double s = 1.0 - 0.5 * q;
-  Wv = K.wnorm * (s * s) * (s * s) * (1.0 + 2.0 * q);
+template <typename R>
+__device__ __forceinline__ void wendland_q(const KParams& K, R r2, R& q, R& F, R& Wv) {
+  q = fsqrt(r2) * (R)K.invh;
+  const R t = q - (R)2;
+  F = (R)K.C1 * (t * t * t);
+  const R s = (R)1 - (R)0.5 * q;
+  Wv = (R)K.wnorm * (s * s) * (s * s) * ((R)1 + (R)2 * q);
 }
 
-__device__ __forceinline__ Rows tape_rows(const KParams& K, const Tape& T, int k) {
+template <typename R>
+__device__ __forceinline__ Rows tape_rows(const KParams& K, const Tape<R>& T, int k) {
   return rows_of(K, T.skey[k], T.cstart);
 }
 
 // 2. force-pass adjoint, gathered: particle k (as i) against every
 //    neighbour l (as j); accumulates k's own input adjoints.
-template <bool AV, bool AH, bool XS>
-__global__ __launch_bounds__(BLOCK) void k_adj_force(Tape T, KParams K, AdjScratch W,
+template <typename R, bool AV, bool AH, bool XS>
+__global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjScratch<R> W,
                                                      double* __restrict__ alpha_part,
                                                      double* __restrict__ q0_part) {
   __shared__ double sh[BLOCK];
-  double lq0 = 0.0;
+  double lq0 = 0.0, lal = 0.0;
   const int k = blockIdx.x * BLOCK + threadIdx.x;
-  double lal = 0.0;
   if (k < K.n) {
-    const double m = K.mass, h = K.h;
-    const double2 xk = T.xy[k], vk = T.v[k];
-    const double Qk = 1.0 / T.rho[k], Ak = T.p[k] * Qk;
-    const double lak_x = W.lax[k], lak_y = W.lay[k], lduk = W.ldu[k];
-    const double lavk_x = W.lavx[k], lavk_y = W.lavy[k];
-    const double psik = T.psi[k], uk = T.u[k];
-    double lx = 0, ly = 0, lvx = 0, lvy = 0, lQ = 0, lA = 0, lpsi = 0, lu = 0;
+    using R2 = typename V2<R>::type;
+    const R m = (R)K.mass, h = (R)K.h;
+    const auto pk = tape_pos(T, k);
+    const R2 vk = T.v[k];
+    const R Qk = (R)1 / dens(K, T.rho[k]), Ak = T.p[k] * Qk;
+    const R lak_x = W.lax[k], lak_y = W.lay[k], lduk = W.ldu[k];
+    const R lavk_x = W.lavx[k], lavk_y = W.lavy[k];
+    const R psik = T.psi[k], uk = (R)T.u[k];
+    const R cut2 = cutoff2<R>(K);
+    const R c0 = (R)K.c0, al = (R)K.av_alpha, be = (R)K.av_beta;
+    R lx = 0, ly = 0, lvx = 0, lvy = 0, lQ = 0, lA = 0, lpsi = 0, lu = 0;
     const Rows rw = tape_rows(K, T, k);
     for (int r = 0; r < 3; ++r)
       for (int l = rw.t0[r]; l < rw.t1[r]; ++l) {
         if (l == k) continue;
-        const double2 xl = T.xy[l];
-        const double rx = xk.x - xl.x, ry = xk.y - xl.y;
-        const double r2 = rx * rx + ry * ry;
-        if (!(r2 < K.cut2)) continue;
-        double q, F, Wv;
+        R rx, ry;
+        delta(pk, tape_pos(T, l), K, rx, ry);
+        const R r2 = rx * rx + ry * ry;
+        if (!(r2 < cut2)) continue;
+        R q, F, Wv;
         wendland_q(K, r2, q, F, Wv);
-        const double2 vl = T.v[l];
-        const double dvx = vk.x - vl.x, dvy = vk.y - vl.y;
-        const double vr = dvx * rx + dvy * ry;
-        const double Ql = 1.0 / T.rho[l], Al = T.p[l] * Ql;
-        const double sQ = Qk + Ql;
-        const double M = 2.0 * Qk * Ql / sQ;
-        double Kc = (K.pa_sph == 0) ? (Ak * Ql + Al * Qk) : (Ak * Qk + Al * Ql);
-        const bool av_on = AV && vr < 0.0;
-        const double einv = 1.0 / (r2 + K.av_e2);
-        const double phi = h * vr * einv;
-        if (av_on) Kc += (K.av_beta * phi - K.av_alpha * K.c0) * phi * M;
-        const double G = m * F * Kc;
-        const double Dax = lak_x - W.lax[l], Day = lak_y - W.lay[l];
-        const double sig = lduk + W.ldu[l];
-        const double lG = -(Dax * rx + Day * ry) + 0.5 * sig * vr;
-        double lrx = -G * Dax, lry = -G * Day;
-        double lvr = 0.5 * sig * G;
-        double ldvx = 0, ldvy = 0;
-        double lF = lG * m * Kc;
-        const double lK = lG * m * F;
-        double lM = 0, lr2 = 0, lW = 0;
+        const R2 vl = T.v[l];
+        const R dvx = vk.x - vl.x, dvy = vk.y - vl.y;
+        const R vr = dvx * rx + dvy * ry;
+        const R Ql = (R)1 / dens(K, T.rho[l]), Al = T.p[l] * Ql;
+        const R sQ = Qk + Ql;
+        const R M = (R)2 * Qk * Ql / sQ;
+        R Kc = (K.pa_sph == 0) ? (Ak * Ql + Al * Qk) : (Ak * Qk + Al * Ql);
+        const bool av_on = AV && vr < (R)0;
+        const R einv = (R)1 / (r2 + (R)K.av_e2);
+        const R phi = h * vr * einv;
+        if (av_on) Kc += (be * phi - al * c0) * phi * M;
+        const R G = m * F * Kc;
+        const R Dax = lak_x - W.lax[l], Day = lak_y - W.lay[l];
+        const R sig = lduk + W.ldu[l];
+        const R lG = -(Dax * rx + Day * ry) + (R)0.5 * sig * vr;
+        R lrx = -G * Dax, lry = -G * Day;
+        R lvr = (R)0.5 * sig * G;
+        R ldvx = 0, ldvy = 0;
+        R lF = lG * m * Kc;
+        const R lK = lG * m * F;
+        R lM = 0, lr2 = 0, lW = 0;
         if (K.pa_sph == 0) {
           lA += lK * Ql;
           lQ += lK * Al;
@@ -235,43 +263,44 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape T, KParams K, AdjScrat
           lQ += lK * Ak;
         }
         if (av_on) {
-          const double dPi = (2.0 * K.av_beta * phi - K.av_alpha * K.c0) * M;
-          lM += lK * (K.av_beta * phi - K.av_alpha * K.c0) * phi;
-          lal += lK * (-K.c0 * phi * M);
-          const double lphi = lK * dPi;
+          const R dPi = ((R)2 * be * phi - al * c0) * M;
+          lM += lK * (be * phi - al * c0) * phi;
+          lal += (double)(lK * (-c0 * phi * M));
+          const R lphi = lK * dPi;
           lvr += lphi * h * einv;
           lr2 += lphi * (-phi * einv);
         }
         if (AH) {
-          const double eh = 1.0 / (r2 + K.ah_e2);
-          const double du_ = uk - T.u[l];
-          const double dl = lduk - W.ldu[l];
-          const double ps = psik + T.psi[l];
-          const double H = m * ps * F * r2 * M * eh;
-          const double lH = dl * du_;
+          const R eh = (R)1 / (r2 + (R)K.ah_e2);
+          const R du_ = uk - (R)T.u[l];
+          const R dl = lduk - W.ldu[l];
+          const R ps = psik + T.psi[l];
+          const R H = m * ps * F * r2 * M * eh;
+          const R lH = dl * du_;
           lu += dl * H;
           lpsi += lH * m * F * r2 * M * eh;
           lF += lH * m * ps * r2 * M * eh;
-          lr2 += lH * m * ps * F * M * K.ah_e2 * eh * eh;
+          lr2 += lH * m * ps * F * M * (R)K.ah_e2 * eh * eh;
           lM += lH * m * ps * F * r2 * eh;
         }
         if (XS) {
-          const double Dvx = lavk_x - W.lavx[l], Dvy = lavk_y - W.lavy[l];
-          const double w = K.xsph_eps * m * M * Wv;
-          const double lw = -(Dvx * dvx + Dvy * dvy);
+          const R Dvx = lavk_x - W.lavx[l], Dvy = lavk_y - W.lavy[l];
+          const R w = (R)K.xsph_eps * m * M * Wv;
+          const R lw = -(Dvx * dvx + Dvy * dvy);
           ldvx += -w * Dvx;
           ldvy += -w * Dvy;
-          lM += lw * K.xsph_eps * m * Wv;
-          lW += lw * K.xsph_eps * m * M;
+          lM += lw * (R)K.xsph_eps * m * Wv;
+          lW += lw * (R)K.xsph_eps * m * M;
         }
-        lQ += lM * 2.0 * Ql * Ql / (sQ * sQ);
-        const double s = 1.0 - 0.5 * q;
-        const double lq = lF * 3.0 * K.C1 * (q - 2.0) * (q - 2.0) + lW * K.wnorm * s * s * s * (-5.0 * q);
-        lr2 += lq / (2.0 * h * h * q);
+        lQ += lM * (R)2 * Ql * Ql / (sQ * sQ);
+        const R s = (R)1 - (R)0.5 * q;
+        const R lq = lF * (R)3 * (R)K.C1 * (q - (R)2) * (q - (R)2) +
+                     lW * (R)K.wnorm * s * s * s * ((R)-5 * q);
+        lr2 += lq / ((R)2 * h * h * q);
         ldvx += lvr * rx;
         ldvy += lvr * ry;
-        lrx += lvr * dvx + 2.0 * lr2 * rx;
-        lry += lvr * dvy + 2.0 * lr2 * ry;
+        lrx += lvr * dvx + (R)2 * lr2 * rx;
+        lry += lvr * dvy + (R)2 * lr2 * ry;
         lx += lrx;
         ly += lry;
         lvx += ldvx;
@@ -279,12 +308,13 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape T, KParams K, AdjScrat
       }
     // heat source du_k += q0 E_k Q_k, E = exp(-hs_k |x_k - c|^2)
     if ((K.flags & 64) && T.id[k] < K.np) {
+      const double2 xk = T.xy[k];
       const double E = heat_shape(K, xk.x, xk.y);
-      lq0 = lduk * E * Qk;
-      lQ += lduk * K.hs_q0 * E;
-      const double lE = lduk * K.hs_q0 * Qk * E * (-2.0 * K.hs_k);
-      lx += lE * (xk.x - K.hs_x);
-      ly += lE * (xk.y - K.hs_y);
+      lq0 = (double)lduk * E * (double)Qk;
+      lQ += lduk * (R)(K.hs_q0 * E);
+      const double lE = (double)lduk * K.hs_q0 * (double)Qk * E * (-2.0 * K.hs_k);
+      lx += (R)(lE * (xk.x - K.hs_x));
+      ly += (R)(lE * (xk.y - K.hs_y));
     }
     W.lx[k] = lx;
     W.ly[k] = ly;
@@ -314,54 +344,60 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape T, KParams K, AdjScrat
 }
 
 // 3. EOS and art_heat coefficient adjoint (per particle)
-__global__ __launch_bounds__(BLOCK) void k_adj_eos(Tape T, KParams K, AdjScratch W) {
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_adj_eos(Tape<R> T, KParams K, AdjScratch<R> W) {
   const int k = blockIdx.x * BLOCK + threadIdx.x;
   if (k >= K.n) return;
-  const double rho = T.rho[k];
-  const double Q = 1.0 / rho;
-  double lp = W.lp[k] + W.lA[k] * Q;
-  double lrhoe = -Q * Q * (W.lQ[k] + W.lA[k] * T.p[k]);
-  const double dv = T.divv[k];  // S / rho
-  const double dpsi = (dv < 0.0) ? -2.0 * K.g2h2 : 0.0;
-  const double ldv = W.lpsi[k] * dpsi;
+  const R rho = dens(K, T.rho[k]);
+  const R Q = (R)1 / rho;
+  R lp = W.lp[k] + W.lA[k] * Q;
+  R lrhoe = -Q * Q * (W.lQ[k] + W.lA[k] * T.p[k]);
+  const R dv = T.divv[k];  // S / rho
+  const R dpsi = (dv < (R)0) ? (R)(-2.0 * K.g2h2) : (R)0;
+  const R ldv = W.lpsi[k] * dpsi;
   W.lS[k] = ldv / rho;
   lrhoe += -ldv * dv / rho;
-  double lrhoh = 0.0;
+  R lrhoh = 0;
   if (is_rate(K, T.id[k]) && !(T.flag[k] & 2u)) {
-    const double t = rho / K.rho0;
-    const double t2 = t * t;
-    lrhoh = lrhoe + lp * 7.0 * K.B / K.rho0 * (t2 * t2 * t2);
+    // dp/drho = 7 B / rho0 (rho / rho0)^6
+    const R t = rho / (R)K.rho0;
+    const R t2 = t * t;
+    lrhoh = lrhoe + lp * (R)(7.0 * K.B / K.rho0) * (t2 * t2 * t2);
   }
   W.lrhoh[k] = lrhoh;
 }
 
 // 4. density adjoint, gathered; adds to the position / velocity adjoints
-__global__ __launch_bounds__(BLOCK) void k_adj_density(Tape T, KParams K, AdjScratch W) {
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_adj_density(Tape<R> T, KParams K, AdjScratch<R> W) {
   const int k = blockIdx.x * BLOCK + threadIdx.x;
   if (k >= K.n) return;
-  const double m = K.mass, h = K.h;
-  const double2 xk = T.xy[k], vk = T.v[k];
-  const double ltk = W.ldrho[k] - W.lS[k];
-  double lx = 0, ly = 0, lvx = 0, lvy = 0;
+  using R2 = typename V2<R>::type;
+  const R m = (R)K.mass, h = (R)K.h;
+  const auto pk = tape_pos(T, k);
+  const R2 vk = T.v[k];
+  const R ltk = W.ldrho[k] - W.lS[k];
+  const R cut2 = cutoff2<R>(K);
+  R lx = 0, ly = 0, lvx = 0, lvy = 0;
   const Rows rw = tape_rows(K, T, k);
   for (int r = 0; r < 3; ++r)
     for (int l = rw.t0[r]; l < rw.t1[r]; ++l) {
       if (l == k) continue;
-      const double2 xl = T.xy[l];
-      const double rx = xk.x - xl.x, ry = xk.y - xl.y;
-      const double r2 = rx * rx + ry * ry;
-      if (!(r2 < K.cut2)) continue;
-      const double lt = ltk + W.ldrho[l] - W.lS[l];
-      double q, F, Wv;
+      R rx, ry;
+      delta(pk, tape_pos(T, l), K, rx, ry);
+      const R r2 = rx * rx + ry * ry;
+      if (!(r2 < cut2)) continue;
+      const R lt = ltk + W.ldrho[l] - W.lS[l];
+      R q, F, Wv;
       wendland_q(K, r2, q, F, Wv);
-      const double2 vl = T.v[l];
-      const double dvx = vk.x - vl.x, dvy = vk.y - vl.y;
-      const double vr = dvx * rx + dvy * ry;
-      const double lvr = lt * m * F;
-      const double lF = lt * m * vr;
-      const double lr2 = lF * 3.0 * K.C1 * (q - 2.0) * (q - 2.0) / (2.0 * h * h * q);
-      lx += lvr * dvx + 2.0 * lr2 * rx;
-      ly += lvr * dvy + 2.0 * lr2 * ry;
+      const R2 vl = T.v[l];
+      const R dvx = vk.x - vl.x, dvy = vk.y - vl.y;
+      const R vr = dvx * rx + dvy * ry;
+      const R lvr = lt * m * F;
+      const R lF = lt * m * vr;
+      const R lr2 = lF * (R)3 * (R)K.C1 * (q - (R)2) * (q - (R)2) / ((R)2 * h * h * q);
+      lx += lvr * dvx + (R)2 * lr2 * rx;
+      ly += lvr * dvy + (R)2 * lr2 * ry;
       lvx += lvr * rx;
       lvy += lvr * ry;
     }
@@ -374,30 +410,33 @@ __global__ __launch_bounds__(BLOCK) void k_adj_density(Tape T, KParams K, AdjScr
 // 5. predictor adjoint (time_integration.f:136-154): folds the rate-input
 //    adjoints into the state adjoints before the step and into the carried
 //    rate adjoints of the previous step
-__global__ __launch_bounds__(BLOCK) void k_adj_predictor(Tape T, KParams K, int its, AdjState L,
-                                                         AdjScratch W) {
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_adj_predictor(Tape<R> T, KParams K, int its,
+                                                         AdjState L, AdjScratch<R> W) {
   const int k = blockIdx.x * BLOCK + threadIdx.x;
   if (k >= K.n) return;
   const int id = T.id[k];
   const double h2 = K.dt / 2.0;
-  L.bx[id] += W.lx[k];
-  L.by[id] += W.ly[k];
-  L.bvx[id] += W.lvx[k];
-  L.bvy[id] += W.lvy[k];
-  L.brho[id] += W.lrhoh[k];
+  const double wlx = W.lx[k], wly = W.ly[k], wlvx = W.lvx[k], wlvy = W.lvy[k];
+  const double wlrho = W.lrhoh[k], wlu = W.lu[k];
+  L.bx[id] += wlx;
+  L.by[id] += wly;
+  L.bvx[id] += wlvx;
+  L.bvy[id] += wlvy;
+  L.brho[id] += wlrho;
   double cax = 0, cay = 0, cdrho = 0, cdu = 0;
   if (its != 1 && is_rate(K, id)) {
-    cdrho = h2 * W.lrhoh[k];
+    cdrho = h2 * wlrho;
     if (!(T.flag[k] & 1u)) {
-      L.bu[id] += W.lu[k];
-      cdu = h2 * W.lu[k];
+      L.bu[id] += wlu;
+      cdu = h2 * wlu;
     }
     if (id < K.np) {
-      cax = h2 * W.lvx[k];
-      cay = h2 * W.lvy[k];
+      cax = h2 * wlvx;
+      cay = h2 * wlvy;
     }
   } else {
-    L.bu[id] += W.lu[k];
+    L.bu[id] += wlu;
   }
   L.cax[id] = cax;
   L.cay[id] = cay;

@@ -12,8 +12,9 @@ included); the FP64 plain form runs the same steps for the FP32/FP64 speedup
 and for the accuracy comparison.  Inputs are resident in HBM before the timed
 region starts; the timed region is K whole steps.
 
-The adjoint leg reports the adjoint / forward device-time ratio of the FP64
-rollout on the 100k dam break (400 steps, checkpoint every 20).
+The adjoint leg reports the adjoint / forward device-time ratio of the
+rollout on the 100k dam break (400 steps, checkpoint every 20), FP32 form
+(forward and backward) first, FP64 beside it.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp64]
                   [--no-adjoint]
@@ -114,37 +115,45 @@ def run_form(precision, args, case_fn):
 
 
 def adjoint_ratio(steps=400, ckpt=20):
-    """Adjoint / forward device time of the FP64 rollout on the 100k dam
-    break (400 steps, checkpoints every 20 steps)."""
+    """Adjoint / forward device time of the rollout on the 100k dam break
+    (400 steps, checkpoints every 20 steps): the FP32 chosen form (forward
+    and backward) as the headline, the FP64 plain form beside it."""
     import armando_amd
     from armando_amd import cases
 
-    case, obj = cases.dam_break_adjoint_100k()
-    sim = armando_amd.Simulation(case)
-    sim.adjoint_rollout(5, 5, obj)  # warm up the kernels
-    sim.close()
-    sim = armando_amd.Simulation(case)
-    J, da, g, (t_fwd, t_bwd) = sim.adjoint_rollout(steps, ckpt, obj)
-    sim.close()
-    plain = armando_amd.Simulation(case)
-    plain.time_integration(1, 5)
-    plain.close()
-    plain = armando_amd.Simulation(case)
-    t0 = time.perf_counter()
-    plain.time_integration(1, steps)
-    t_plain = 1e3 * (time.perf_counter() - t0)
-    plain.close()
+    out = {}
+    for name, prec in (("fp32", cases.FP32), ("fp64", cases.FP64)):
+        case, obj = cases.dam_break_adjoint_100k(precision=prec)
+        sim = armando_amd.Simulation(case)
+        sim.adjoint_rollout(5, 5, obj)  # warm up the kernels
+        sim.close()
+        sim = armando_amd.Simulation(case)
+        J, da, g, (t_fwd, t_bwd) = sim.adjoint_rollout(steps, ckpt, obj)
+        sim.close()
+        plain = armando_amd.Simulation(case)
+        plain.time_integration(1, 5)
+        plain.close()
+        plain = armando_amd.Simulation(case)
+        t0 = time.perf_counter()
+        plain.time_integration(1, steps)
+        t_plain = 1e3 * (time.perf_counter() - t0)
+        plain.close()
+        out[name] = {
+            "ratio": t_bwd / t_fwd,
+            "forward_ms": t_fwd,
+            "backward_ms": t_bwd,
+            "plain_forward_ms": t_plain,
+            "ratio_vs_plain_forward": t_bwd / t_plain,
+            "J": J,
+            "dJ_dalpha": da,
+            "dJ_dv0_norm": float(np.linalg.norm(g)),
+        }
     return {
-        "workload": f"dam break {case['np']} fluid + {case['nv']} wall particles, FP64, "
+        "workload": f"dam break {case['np']} fluid + {case['nv']} wall particles, "
                     f"{steps} steps, checkpoint every {ckpt}",
-        "ratio": t_bwd / t_fwd,
-        "forward_ms": t_fwd,
-        "backward_ms": t_bwd,
-        "plain_forward_ms": t_plain,
-        "ratio_vs_plain_forward": t_bwd / t_plain,
-        "J": J,
-        "dJ_dalpha": da,
-        "dJ_dv0_norm": float(np.linalg.norm(g)),
+        "ratio": out["fp32"]["ratio"],
+        "dtype": "f32",
+        **out,
     }
 
 

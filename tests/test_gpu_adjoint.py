@@ -20,8 +20,8 @@ def banner(msg):
     print(f"[gpu-adj] {msg}", flush=True)
 
 
-def adjoint_case(scale=0.02, seed=777):
-    c = cases.dam_break_small(scale=scale)
+def adjoint_case(scale=0.02, seed=777, precision=cases.FP64):
+    c = cases.dam_break_small(scale=scale, precision=precision)
     rng = np.random.default_rng(seed)
     nf = c["np"]
     c["vx"] = c["vx"].copy()
@@ -95,15 +95,43 @@ def test_adjoint_matches_central_differences():
         assert abs(fd - ad) <= 1e-5 * abs(fd), (k, fd, ad)
 
 
-def test_adjoint_independent_of_checkpoint_interval():
-    c, obj = adjoint_case()
+@pytest.mark.parametrize("prec", [cases.FP64, cases.FP32])
+def test_adjoint_independent_of_checkpoint_interval(prec):
+    c, obj = adjoint_case(precision=prec)
     steps = 12
-    banner("checkpoint interval")
+    banner(f"checkpoint interval prec={prec}")
     ref = gpu_rollout(c, obj, steps, 1)
     for ck in (5, 12):
         out = gpu_rollout(c, obj, steps, ck)
         assert out[0] == ref[0] and out[1] == ref[1]
         assert np.array_equal(out[2], ref[2])
+
+
+def test_fp32_adjoint_against_fp64_differences():
+    """The FP32 chosen form's adjoint (FP32 pair arithmetic, FP64 adjoint
+    accumulators) against FP64 central differences."""
+    c, obj = adjoint_case()
+    c32, _ = adjoint_case(precision=cases.FP32)
+    steps = 16
+    banner("fp32 adjoint vs fp64 FD")
+    J, da, g, _ = gpu_rollout(c32, obj, steps, 4)
+    a0 = c["av_alpha"]
+    e = 1e-4 * a0
+    fd = (gpu_rollout(c, obj, steps, 4, alpha=a0 + e)[0]
+          - gpu_rollout(c, obj, steps, 4, alpha=a0 - e)[0]) / (2 * e)
+    assert abs(fd - da) <= 1e-3 * abs(fd), (fd, da)
+    nf = c["np"]
+    z = cases.normal01(1001, 2 * nf)
+    d = np.zeros((c["n"], 2))
+    d[:nf, 0], d[:nf, 1] = z[:nf], z[nf:]
+    ev = 1e-6
+    cp, cm = dict(c), dict(c)
+    cp["vx"], cp["vy"] = c["vx"] + ev * d[:, 0], c["vy"] + ev * d[:, 1]
+    cm["vx"], cm["vy"] = c["vx"] - ev * d[:, 0], c["vy"] - ev * d[:, 1]
+    fdv = (gpu_rollout(cp, obj, steps, 4)[0] - gpu_rollout(cm, obj, steps, 4)[0]) / (2 * ev)
+    ad = float(np.sum(g * d))
+    banner(f"  alpha fd {fd:.6e} fp32 adj {da:.6e}; v0 fd {fdv:.6e} fp32 adj {ad:.6e}")
+    assert abs(fdv - ad) <= 1e-3 * abs(fdv)
 
 
 def test_adjoint_leaves_the_final_state():
@@ -150,14 +178,20 @@ def test_adjoint_100k_400_steps_against_central_differences_on_16_directions():
     banner(f"100k adjoint n={c['n']} steps={steps} ckpt={ck}")
     J, da, g, (tf, tb) = gpu_rollout(c, obj, steps, ck)
     assert J > 0 and tb <= 3.0 * tf, (J, tf, tb)
+    # the FP32 chosen form, forward and backward, against the same FP64 differences
+    c32, _ = cases.dam_break_adjoint_100k(precision=cases.FP32)
+    J32, da32, g32, (tf32, tb32) = gpu_rollout(c32, obj, steps, ck)
+    banner(f"  fp32: J {J32:.6e} (fp64 {J:.6e}), fwd {tf32:.1f} ms bwd {tb32:.1f} ms")
+    assert abs(J32 - J) <= 1e-4 * J and tb32 <= 3.0 * tf32
     a0 = c["av_alpha"]
     e = 1e-4 * a0
     fd = (gpu_objective(c, obj, steps, ck, alpha=a0 + e)
           - gpu_objective(c, obj, steps, ck, alpha=a0 - e)) / (2 * e)
     assert abs(fd - da) <= 1e-3 * abs(fd), (fd, da)
+    assert abs(fd - da32) <= 1e-3 * abs(fd), (fd, da32)
     nf = c["np"]
     e = 1e-4
-    worst = 0.0
+    worst = worst32 = 0.0
     for seed in range(1001, 1017):
         z = cases.normal01(seed, 2 * nf)
         d = np.zeros((c["n"], 2))
@@ -170,4 +204,8 @@ def test_adjoint_100k_400_steps_against_central_differences_on_16_directions():
         rel = abs(fdv - ad) / abs(fdv)
         worst = max(worst, rel)
         assert rel <= 1e-3, (seed, fdv, ad)
-    banner(f"100k adjoint: alpha fd {fd:.6e} adj {da:.6e}; worst direction rel {worst:.2e}")
+        rel32 = abs(fdv - float(np.sum(g32 * d))) / abs(fdv)
+        worst32 = max(worst32, rel32)
+        assert rel32 <= 1e-3, (seed, fdv, float(np.sum(g32 * d)))
+    banner(f"100k adjoint: alpha fd {fd:.6e} adj {da:.6e} fp32 {da32:.6e}; worst direction "
+           f"rel {worst:.2e} (fp32 {worst32:.2e})")
