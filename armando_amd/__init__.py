@@ -279,10 +279,10 @@ class Simulation:
         self._check(self.L.arm_set_objective(self.ctx, int(kind)))
 
     def param_grads(self):
-        """(dJ/d av_alpha, dJ/d hs_q0) of the last rollout backward."""
-        out = np.zeros(2)
-        self._check(self.L.arm_rollout_param_grads(self.ctx, _ptr(out), 2))
-        return float(out[0]), float(out[1])
+        """(dJ/d av_alpha, dJ/d hs_q0, dJ/d visc_eta) of the last rollout backward."""
+        out = np.zeros(3)
+        self._check(self.L.arm_rollout_param_grads(self.ctx, _ptr(out), 3))
+        return float(out[0]), float(out[1]), float(out[2])
 
     def set_gather_path(self, tiled=True):
         self._check(self.L.arm_set_gather_path(self.ctx, 1 if tiled else 0))

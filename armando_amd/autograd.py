@@ -37,11 +37,12 @@ def extension():
 
 
 class RolloutLoss:
-    """J(v0, alpha, q0) = sum over nsteps of dt * sum_{obj} f, f = p^2
+    """J(v0, alpha, q0, eta) = sum over nsteps of dt * sum_{obj} f, f = p^2
     (objective OBJ_PRESSURE) or u^2 (OBJ_ENERGY), as a torch function whose
     backward is the hand-written adjoint (checkpoint every ckpt_every steps).
     v0: (n, 2) initial velocities; alpha: artificial viscosity; q0: heat
-    source amplitude (used when the case has the heat source)."""
+    source amplitude (used when the case has the heat source); eta: dynamic
+    viscosity (used when the case has the physical viscosity)."""
 
     def __init__(self, case, obj_ids, nsteps, ckpt_every, objective=0):
         ext = extension()
@@ -52,13 +53,16 @@ class RolloutLoss:
         obj = torch.from_numpy(np.asarray(obj_ids, np.int32))
         self.n = int(case["n"])
         self.q0 = float(params["hs_q0"])
+        self.eta = float(params["visc_eta"])
         self.handle = ext.make_rollout(params, x, rho, u, int(case["np"]), int(case["nv"]), obj,
                                        int(nsteps), int(ckpt_every), int(objective))
 
-    def __call__(self, v0, alpha, q0=None):
+    def __call__(self, v0, alpha, q0=None, eta=None):
         if q0 is None:
             q0 = torch.tensor(self.q0, dtype=torch.float64)
-        return extension().rollout_loss(self.handle, v0, alpha, q0)
+        if eta is None:
+            eta = torch.tensor(self.eta, dtype=torch.float64)
+        return extension().rollout_loss(self.handle, v0, alpha, q0, eta)
 
     def close(self):
         if getattr(self, "handle", 0):

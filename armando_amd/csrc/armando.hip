@@ -568,11 +568,13 @@ int copy_checkpoint(arm_ctx* c, const St<R>& S, const Checkpoint<R>& k, bool sav
   return ARM_OK;
 }
 
-template <typename R> int alloc_tape(Scratch& sc, Tape<R>& t, int n, int ncell) {
+template <typename R> int alloc_tape(Scratch& sc, Tape<R>& t, int n, int ncell, bool visc) {
   int rc;
   t.cxy = nullptr;
   t.off = nullptr;
+  t.tau = nullptr;
   if (sizeof(R) == 4 && ((rc = sc.get(&t.cxy, n)) || (rc = sc.get(&t.off, n)))) return rc;
+  if (visc && (rc = sc.get(&t.tau, 4 * (size_t)n))) return rc;
   if ((rc = sc.get(&t.id, n)) || (rc = sc.get(&t.xy, n)) || (rc = sc.get(&t.v, n)) ||
       (rc = sc.get(&t.rho, n)) || (rc = sc.get(&t.u, n)) || (rc = sc.get(&t.psi, n)) ||
       (rc = sc.get(&t.divv, n)) || (rc = sc.get(&t.p, n)) || (rc = sc.get(&t.drho, n)) ||
@@ -583,27 +585,40 @@ template <typename R> int alloc_tape(Scratch& sc, Tape<R>& t, int n, int ncell) 
   return ARM_OK;
 }
 
-template <typename R, bool AV, bool AH, bool XS>
+template <typename R, bool AV, bool AH, bool XS, bool VI>
 void launch_adj_force(arm_ctx* c, const Tape<R>& T, const AdjScratch<R>& W, double* apart,
-                      double* qpart) {
-  hipLaunchKernelGGL((k_adj_force<R, AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
-                     c->stream, T, c->K, W, apart, qpart);
+                      double* qpart, double* epart) {
+  hipLaunchKernelGGL((k_adj_force<R, AV, AH, XS, VI>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
+                     c->stream, T, c->K, W, apart, qpart, epart);
 }
 
-template <typename R>
-void adj_force(arm_ctx* c, const Tape<R>& T, const AdjScratch<R>& W, double* apart,
-               double* qpart) {
+template <typename R, bool VI>
+void adj_force_v(arm_ctx* c, const Tape<R>& T, const AdjScratch<R>& W, double* apart,
+                 double* qpart, double* epart) {
   const int f = c->prm.flags;
   const int code = ((f & ARM_ART_VISC) ? 4 : 0) | ((f & ARM_ART_HEAT) ? 2 : 0) | ((f & ARM_AV_VEL) ? 1 : 0);
   switch (code) {
-    case 0: launch_adj_force<R, false, false, false>(c, T, W, apart, qpart); break;
-    case 1: launch_adj_force<R, false, false, true>(c, T, W, apart, qpart); break;
-    case 2: launch_adj_force<R, false, true, false>(c, T, W, apart, qpart); break;
-    case 3: launch_adj_force<R, false, true, true>(c, T, W, apart, qpart); break;
-    case 4: launch_adj_force<R, true, false, false>(c, T, W, apart, qpart); break;
-    case 5: launch_adj_force<R, true, false, true>(c, T, W, apart, qpart); break;
-    case 6: launch_adj_force<R, true, true, false>(c, T, W, apart, qpart); break;
-    default: launch_adj_force<R, true, true, true>(c, T, W, apart, qpart); break;
+    case 0: launch_adj_force<R, false, false, false, VI>(c, T, W, apart, qpart, epart); break;
+    case 1: launch_adj_force<R, false, false, true, VI>(c, T, W, apart, qpart, epart); break;
+    case 2: launch_adj_force<R, false, true, false, VI>(c, T, W, apart, qpart, epart); break;
+    case 3: launch_adj_force<R, false, true, true, VI>(c, T, W, apart, qpart, epart); break;
+    case 4: launch_adj_force<R, true, false, false, VI>(c, T, W, apart, qpart, epart); break;
+    case 5: launch_adj_force<R, true, false, true, VI>(c, T, W, apart, qpart, epart); break;
+    case 6: launch_adj_force<R, true, true, false, VI>(c, T, W, apart, qpart, epart); break;
+    default: launch_adj_force<R, true, true, true, VI>(c, T, W, apart, qpart, epart); break;
+  }
+}
+
+// force-pass adjoint, then (physical viscosity) the shear-tensor adjoint
+template <typename R>
+void adj_force(arm_ctx* c, const Tape<R>& T, const AdjScratch<R>& W, double* apart,
+               double* qpart, double* epart) {
+  if (c->prm.flags & ARM_VISC) {
+    adj_force_v<R, true>(c, T, W, apart, qpart, epart);
+    hipLaunchKernelGGL(k_adj_tensor<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, T, c->K,
+                       W);
+  } else {
+    adj_force_v<R, false>(c, T, W, apart, qpart, epart);
   }
 }
 
@@ -621,7 +636,7 @@ struct AdjRun {
   Scratch sc;
   int nsteps = 0, ckpt = 0, nck = 0;
   uint8_t* obj = nullptr;
-  double *jpart = nullptr, *apart = nullptr, *qpart = nullptr;
+  double *jpart = nullptr, *apart = nullptr, *qpart = nullptr, *epart = nullptr;
   int kind = ARM_OBJ_PRESSURE;
   AdjState L{};
   AdjArrays<double> a64;
@@ -648,9 +663,7 @@ int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, in
   const int n = c->n, ncell = c->K.ncell;
   const int nb = grid_of(n);
   drop_adjoint(c);
-  if ((c->prm.flags & ARM_VISC) || c->prm.eos != ARM_EOS_TAIT)
-    return fail(c, ARM_ERR_UNSUPPORTED,
-                "the adjoint covers the Tait EOS without physical viscosity");
+  const bool visc = (c->prm.flags & ARM_VISC) != 0;
   AdjRun* R = new AdjRun(c);
   c->adj = R;
   Scratch& sc = R->sc;
@@ -660,7 +673,7 @@ int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, in
   R->nck = (nsteps + ckpt - 1) / ckpt;
   R->kind = c->objkind;
   if ((rc = sc.get(&R->obj, n)) || (rc = sc.get(&R->jpart, nb)) || (rc = sc.get(&R->apart, nb)) ||
-      (rc = sc.get(&R->qpart, nb)))
+      (rc = sc.get(&R->qpart, nb)) || (rc = sc.get(&R->epart, nb)))
     return rc;
   {
     std::vector<uint8_t> h(n, 0);
@@ -679,7 +692,7 @@ int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, in
     if ((rc = alloc_checkpoint(sc, k, n))) return rc;
   Ar.tape.resize(ckpt);
   for (auto& t : Ar.tape)
-    if ((rc = alloc_tape(sc, t, n, ncell))) return rc;
+    if ((rc = alloc_tape(sc, t, n, ncell, visc))) return rc;
   double** lf[] = {&R->L.x,   &R->L.y,   &R->L.vx,  &R->L.vy,  &R->L.rho,   &R->L.u,
                    &R->L.bx,  &R->L.by,  &R->L.bvx, &R->L.bvy, &R->L.brho,  &R->L.bu,
                    &R->L.cax, &R->L.cay, &R->L.cdrho, &R->L.cdu};
@@ -690,6 +703,9 @@ int rollout_forward(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, in
                &W.lpsi, &W.lS, &W.lrhoh, &W.lu,  &W.lvx,  &W.lvy,  &W.lx,  &W.ly,  &W.lus};
   for (Rf** q : wf)
     if ((rc = sc.get(q, n))) return rc;
+  W.lTxx = W.lTyy = W.lTxy = nullptr;
+  if (visc && ((rc = sc.get(&W.lTxx, n)) || (rc = sc.get(&W.lTyy, n)) || (rc = sc.get(&W.lTxy, n))))
+    return rc;
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
@@ -737,6 +753,7 @@ int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0,
   for (double** q : lf) HIPCHK(c, hipMemsetAsync(*q, 0, sizeof(double) * n, c->stream));
   HIPCHK(c, hipMemsetAsync(R->apart, 0, sizeof(double) * nb, c->stream));
   HIPCHK(c, hipMemsetAsync(R->qpart, 0, sizeof(double) * nb, c->stream));
+  HIPCHK(c, hipMemsetAsync(R->epart, 0, sizeof(double) * nb, c->stream));
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
@@ -754,7 +771,7 @@ int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0,
       const Tape<Rf>& T = Ar.tape[its - s0];
       hipLaunchKernelGGL(k_adj_corrector<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its,
                          L, W, R->obj, R->kind, seed);
-      adj_force<Rf>(c, T, W, R->apart, R->qpart);
+      adj_force<Rf>(c, T, W, R->apart, R->qpart, R->epart);
       hipLaunchKernelGGL(k_adj_eos<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
       hipLaunchKernelGGL(k_adj_density<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
       hipLaunchKernelGGL(k_adj_predictor<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its,
@@ -772,9 +789,10 @@ int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0,
   HIPCHK(c, hipEventRecord(e1, c->stream));
   // leave the context at the final forward state
   if ((rc = copy_checkpoint(c, stA<Rf>(c), Ar.ck[nck], false))) return rc;
-  std::vector<double> ap(nb), qp(nb), gx(n), gy(n);
+  std::vector<double> ap(nb), qp(nb), ep(nb), gx(n), gy(n);
   HIPCHK(c, hipMemcpyAsync(ap.data(), R->apart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(qp.data(), R->qpart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ep.data(), R->epart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(gx.data(), L.vx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(gy.data(), L.vy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -782,13 +800,15 @@ int rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0,
   (void)hipEventElapsedTime(&f, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  double As = 0, Qs = 0;
+  double As = 0, Qs = 0, Es = 0;
   for (int k = 0; k < nb; ++k) {
     As += ap[k];
     Qs += qp[k];
+    Es += ep[k];
   }
   c->grads[ARM_GRAD_ALPHA] = As;
   c->grads[ARM_GRAD_Q0] = Qs;
+  c->grads[ARM_GRAD_VISC] = Es;
   if (dJ_dalpha) *dJ_dalpha = As;
   if (dJ_dv0)
     for (int i = 0; i < n; ++i) {

@@ -1,12 +1,13 @@
 // autograd.cpp -- torch::autograd::Function over the C ABI adjoint rollout.
 //
-// J(v0, alpha, q0) = sum over steps of dt * sum_{i in obj} f_i (f = p^2 or
-// u^2) for nsteps of time_integration started from a fixed initial state
-// with initial velocities v0 (n x 2), artificial-viscosity alpha and heat
-// source amplitude q0.  forward() runs arm_rollout_forward (checkpoints kept
-// in the device context); backward() runs arm_rollout_backward with the
-// incoming gradient as the adjoint seed and returns dJ/dv0, dJ/dalpha and
-// dJ/dq0.  Host code only; the device work is in libarmando.so.
+// J(v0, alpha, q0, eta) = sum over steps of dt * sum_{i in obj} f_i
+// (f = p^2 or u^2) for nsteps of time_integration started from a fixed
+// initial state with initial velocities v0 (n x 2), artificial-viscosity
+// alpha, heat source amplitude q0 and dynamic viscosity eta.  forward() runs
+// arm_rollout_forward (checkpoints kept in the device context); backward()
+// runs arm_rollout_backward with the incoming gradient as the adjoint seed
+// and returns dJ/dv0, dJ/dalpha, dJ/dq0 and dJ/deta.  Host code only; the
+// device work is in libarmando.so.
 #include <torch/extension.h>
 
 #include <cstdint>
@@ -44,7 +45,8 @@ std::vector<double> to_vec(const torch::Tensor& t) {
 class RolloutFn : public torch::autograd::Function<RolloutFn> {
  public:
   static torch::Tensor forward(torch::autograd::AutogradContext* actx, torch::Tensor v0,
-                               torch::Tensor alpha, torch::Tensor q0, int64_t handle) {
+                               torch::Tensor alpha, torch::Tensor q0, torch::Tensor eta,
+                               int64_t handle) {
     Rollout* r = as_rollout(handle);
     const int n = r->np + r->nv;
     TORCH_CHECK(v0.dim() == 2 && v0.size(0) == n && v0.size(1) == 2, "v0 must be (n, 2)");
@@ -52,6 +54,7 @@ class RolloutFn : public torch::autograd::Function<RolloutFn> {
     arm_params P = r->prm;
     P.av_alpha = alpha.detach().to(torch::kCPU, torch::kFloat64).item<double>();
     P.hs_q0 = q0.detach().to(torch::kCPU, torch::kFloat64).item<double>();
+    P.visc_eta = eta.detach().to(torch::kCPU, torch::kFloat64).item<double>();
     check(r, arm_update_params(r->ctx, &P), "arm_update_params");
     check(r, arm_upload(r->ctx, r->np, r->nv, r->x.data(), v.data_ptr<double>(), r->rho.data(),
                         r->u.data()),
@@ -74,11 +77,12 @@ class RolloutFn : public torch::autograd::Function<RolloutFn> {
     auto dv = torch::zeros({n, 2}, torch::dtype(torch::kFloat64));
     check(r, arm_rollout_backward(r->ctx, seed, nullptr, dv.data_ptr<double>(), nullptr),
           "arm_rollout_backward");
-    double g[ARM_NGRAD] = {0.0, 0.0};
+    double g[ARM_NGRAD] = {0.0, 0.0, 0.0};
     check(r, arm_rollout_param_grads(r->ctx, g, ARM_NGRAD), "arm_rollout_param_grads");
     auto opts = torch::dtype(torch::kFloat64).device(grads[0].device());
     return {dv.to(torch::Device(dev)), torch::tensor(g[ARM_GRAD_ALPHA], opts),
-            torch::tensor(g[ARM_GRAD_Q0], opts), torch::Tensor()};
+            torch::tensor(g[ARM_GRAD_Q0], opts), torch::tensor(g[ARM_GRAD_VISC], opts),
+            torch::Tensor()};
   }
 };
 
@@ -150,8 +154,9 @@ void free_rollout(int64_t h) {
   delete r;
 }
 
-torch::Tensor rollout_loss(int64_t h, torch::Tensor v0, torch::Tensor alpha, torch::Tensor q0) {
-  return RolloutFn::apply(v0, alpha, q0, h);
+torch::Tensor rollout_loss(int64_t h, torch::Tensor v0, torch::Tensor alpha, torch::Tensor q0,
+                           torch::Tensor eta) {
+  return RolloutFn::apply(v0, alpha, q0, eta, h);
 }
 
 }  // namespace
@@ -160,5 +165,6 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.doc() = "torch.autograd over the MI355X WCSPH adjoint rollout";
   m.def("make_rollout", &make_rollout, "create a rollout context (returns a handle)");
   m.def("free_rollout", &free_rollout, "destroy a rollout context");
-  m.def("rollout_loss", &rollout_loss, "J(v0, alpha, q0) with a hand-written adjoint backward");
+  m.def("rollout_loss", &rollout_loss,
+        "J(v0, alpha, q0, eta) with a hand-written adjoint backward");
 }

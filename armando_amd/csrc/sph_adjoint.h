@@ -6,7 +6,7 @@
 //
 // Differentiates the step exactly as the forward kernels compute it:
 //   density   drho_i = sum_j m F vr,  S_i = -sum_j m F vr (divv_i = S_i/rho_i)
-//   EOS       p = Tait(rho), clamp p < pmin -> p = 0, rho = rho0
+//   EOS       p = Tait(rho) or c0^2 rho + p0, clamp p < pmin -> p = 0, rho = rho0
 //   psi_i = g1 h c0 + g2 h^2 (|divv_i| - divv_i),  Q = 1/rho,  A = p/rho
 //   force     G = m F (A_i Q_j + A_j Q_i + Pi),  a_i = -sum G r + g,
 //             du_i = sum G vr/2 + H (u_i - u_j),  av_i = -sum eps m M W dv
@@ -20,7 +20,8 @@
 // Pair terms are gathered per particle: a particle accumulates only the
 // derivatives of its own inputs, each pair being visited from both ends, so
 // no atomics are needed.  The objective is J = sum_steps dt sum_obj f,
-// f = p^2 or u^2; the heat source (flag 64) adds dJ/d hs_q0.
+// f = p^2 or u^2; the heat source (flag 64) adds dJ/d hs_q0 and the
+// physical viscosity (flag 32: shear tensor, viscous force, tds/dt) dJ/d eta.
 #pragma once
 
 #include "sph_kernels.h"
@@ -51,6 +52,7 @@ template <typename R> struct Tape {
   uint8_t* flag;
   uint32_t* skey;
   int* cstart;
+  R* tau;  // physical viscosity: (Txx, Tyy, Txy, tds) per particle, else null
 };
 
 __device__ __forceinline__ PosD tape_pos(const Tape<double>& T, int t) {
@@ -75,6 +77,7 @@ template <typename R> struct AdjScratch {
   R *lax, *lay, *ldrho, *ldu, *lavx, *lavy, *lp;
   R *lQ, *lA, *lpsi, *lS, *lrhoh, *lu, *lvx, *lvy, *lx, *ly;
   R* lus;  // objective seed of the energy (ARM_OBJ_ENERGY)
+  R *lTxx, *lTyy, *lTxy;  // physical viscosity: adjoint of the shear tensor
 };
 
 template <typename R>
@@ -106,6 +109,8 @@ __global__ __launch_bounds__(BLOCK) void k_tape(St<R> S, KParams K,
   T.v_min[t] = S.v_min[t];
   T.flag[t] = S.flag[t];
   T.skey[t] = skey[t];
+  if (T.tau)
+    for (int c = 0; c < 4; ++c) T.tau[4 * (size_t)t + c] = S.tau[4 * (size_t)t + c];
 }
 
 // objective: jpart[block] += dt sum_{obj} f over the block's particles,
@@ -205,12 +210,13 @@ __device__ __forceinline__ Rows tape_rows(const KParams& K, const Tape<R>& T, in
 
 // 2. force-pass adjoint, gathered: particle k (as i) against every
 //    neighbour l (as j); accumulates k's own input adjoints.
-template <typename R, bool AV, bool AH, bool XS>
+template <typename R, bool AV, bool AH, bool XS, bool VI>
 __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjScratch<R> W,
                                                      double* __restrict__ alpha_part,
-                                                     double* __restrict__ q0_part) {
+                                                     double* __restrict__ q0_part,
+                                                     double* __restrict__ eta_part) {
   __shared__ double sh[BLOCK];
-  double lq0 = 0.0, lal = 0.0;
+  double lq0 = 0.0, lal = 0.0, leta = 0.0;
   const int k = blockIdx.x * BLOCK + threadIdx.x;
   if (k < K.n) {
     using R2 = typename V2<R>::type;
@@ -224,6 +230,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjSc
     const R cut2 = cutoff2<R>(K);
     const R c0 = (R)K.c0, al = (R)K.av_alpha, be = (R)K.av_beta;
     R lx = 0, ly = 0, lvx = 0, lvy = 0, lQ = 0, lA = 0, lpsi = 0, lu = 0;
+    // physical viscosity: W = c T, c = eta (pa_sph 0) or eta Q^2 (pa_sph 1)
+    R Tk[3] = {0, 0, 0}, ck = 0, lWk[3] = {0, 0, 0};
+    if (VI) {
+      const R* tk = T.tau + 4 * (size_t)k;
+      Tk[0] = tk[0];
+      Tk[1] = tk[1];
+      Tk[2] = tk[2];
+      ck = visc_c(K, Qk);
+    }
     const Rows rw = tape_rows(K, T, k);
     for (int r = 0; r < 3; ++r)
       for (int l = rw.t0[r]; l < rw.t1[r]; ++l) {
@@ -292,6 +307,21 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjSc
           lM += lw * (R)K.xsph_eps * m * Wv;
           lW += lw * (R)K.xsph_eps * m * M;
         }
+        if (VI) {
+          // a_k += m F S.r, S = (Q_k Q_l if pa_sph 0) (W_k + W_l); a_l gets -that
+          const R* tl = T.tau + 4 * (size_t)l;
+          const R cl = visc_c(K, Ql);
+          R Sr[3] = {ck * Tk[0] + cl * tl[0], ck * Tk[1] + cl * tl[1], ck * Tk[2] + cl * tl[2]};
+          const R qq = K.pa_sph == 0 ? Qk * Ql : (R)1;
+          const R Sx = qq * Sr[0], Sy = qq * Sr[1], Sxy = qq * Sr[2];
+          const R mF = m * F;
+          lF += m * (Dax * (Sx * rx + Sxy * ry) + Day * (Sxy * rx + Sy * ry));
+          lrx += mF * (Dax * Sx + Day * Sxy);
+          lry += mF * (Dax * Sxy + Day * Sy);
+          const R lS[3] = {mF * Dax * rx, mF * Day * ry, mF * (Dax * ry + Day * rx)};
+          if (K.pa_sph == 0) lQ += Ql * (lS[0] * Sr[0] + lS[1] * Sr[1] + lS[2] * Sr[2]);
+          for (int c = 0; c < 3; ++c) lWk[c] += qq * lS[c];
+        }
         lQ += lM * (R)2 * Ql * Ql / (sQ * sQ);
         const R s = (R)1 - (R)0.5 * q;
         const R lq = lF * (R)3 * (R)K.C1 * (q - (R)2) * (q - (R)2) +
@@ -306,6 +336,19 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjSc
         lvx += ldvx;
         lvy += ldvy;
       }
+    if (VI) {
+      // W_k = c_k T_k and tds_k = eta/2 Q_k (Txx^2 + Tyy^2 + 2 Txy^2), added to du_k
+      const R eta = (R)K.eta;
+      const R tw = Tk[0] * lWk[0] + Tk[1] * lWk[1] + Tk[2] * lWk[2];
+      const R tt = Tk[0] * Tk[0] + Tk[1] * Tk[1] + (R)2 * Tk[2] * Tk[2];
+      const R g = lduk * eta * Qk;
+      W.lTxx[k] = ck * lWk[0] + g * Tk[0];
+      W.lTyy[k] = ck * lWk[1] + g * Tk[1];
+      W.lTxy[k] = ck * lWk[2] + g * (R)2 * Tk[2];
+      if (K.pa_sph == 1) lQ += (R)2 * eta * Qk * tw;
+      lQ += lduk * (R)0.5 * eta * tt;
+      leta = (double)(K.pa_sph == 0 ? tw : Qk * Qk * tw) + (double)(lduk * (R)0.5 * Qk * tt);
+    }
     // heat source du_k += q0 E_k Q_k, E = exp(-hs_k |x_k - c|^2)
     if ((K.flags & 64) && T.id[k] < K.np) {
       const double2 xk = T.xy[k];
@@ -341,6 +384,72 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjSc
     __syncthreads();
   }
   if (threadIdx.x == 0) q0_part[blockIdx.x] += sh[0];
+  if (VI) {
+    __syncthreads();
+    sh[threadIdx.x] = leta;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) eta_part[blockIdx.x] += sh[0];
+  }
+}
+
+// 2b. shear-tensor adjoint (physical viscosity), gathered.  The pair term
+//     H(dv, dW), dv = v_l - v_k, dW = F r, r = x_k - x_l, enters
+//     T_k with m Q_l and T_l with m Q_k (H is the same from both ends), so
+//     particle k collects dv, r and Q_k derivatives of lambda_H =
+//     m Q_l lT_k + m Q_k lT_l.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_adj_tensor(Tape<R> T, KParams K, AdjScratch<R> W) {
+  const int k = blockIdx.x * BLOCK + threadIdx.x;
+  if (k >= K.n) return;
+  using R2 = typename V2<R>::type;
+  const R m = (R)K.mass, h = (R)K.h;
+  const auto pk = tape_pos(T, k);
+  const R2 vk = T.v[k];
+  const R Qk = (R)1 / dens(K, T.rho[k]);
+  const R Lk[3] = {W.lTxx[k], W.lTyy[k], W.lTxy[k]};
+  const R cut2 = cutoff2<R>(K);
+  const R two3 = (R)(2.0 / 3.0);
+  R lx = 0, ly = 0, lvx = 0, lvy = 0, lQ = 0;
+  const Rows rw = tape_rows(K, T, k);
+  for (int r = 0; r < 3; ++r)
+    for (int l = rw.t0[r]; l < rw.t1[r]; ++l) {
+      if (l == k) continue;
+      R rx, ry;
+      delta(pk, tape_pos(T, l), K, rx, ry);
+      const R r2 = rx * rx + ry * ry;
+      if (!(r2 < cut2)) continue;
+      R q, F, Wv;
+      wendland_q(K, r2, q, F, Wv);
+      const R2 vl = T.v[l];
+      const R dvx = vl.x - vk.x, dvy = vl.y - vk.y;
+      const R dWx = F * rx, dWy = F * ry;
+      const R a = dvx * dWx, b = dvy * dWy;
+      const R H[3] = {two3 * ((R)2 * a - b), two3 * ((R)2 * b - a), dvx * dWy + dvy * dWx};
+      const R Ql = (R)1 / dens(K, T.rho[l]);
+      const R Ll[3] = {W.lTxx[l], W.lTyy[l], W.lTxy[l]};
+      R lH[3];
+      for (int c = 0; c < 3; ++c) lH[c] = m * Ql * Lk[c] + m * Qk * Ll[c];
+      lQ += m * (H[0] * Ll[0] + H[1] * Ll[1] + H[2] * Ll[2]);
+      const R la = two3 * ((R)2 * lH[0] - lH[1]);
+      const R lb = two3 * ((R)2 * lH[1] - lH[0]);
+      const R ldvx = la * dWx + lH[2] * dWy, ldvy = lb * dWy + lH[2] * dWx;
+      const R ldWx = la * dvx + lH[2] * dvy, ldWy = lb * dvy + lH[2] * dvx;
+      lvx -= ldvx;
+      lvy -= ldvy;
+      const R lF = ldWx * rx + ldWy * ry;
+      const R lr2 = lF * (R)3 * (R)K.C1 * (q - (R)2) * (q - (R)2) / ((R)2 * h * h * q);
+      lx += F * ldWx + (R)2 * lr2 * rx;
+      ly += F * ldWy + (R)2 * lr2 * ry;
+    }
+  W.lx[k] += lx;
+  W.ly[k] += ly;
+  W.lvx[k] += lvx;
+  W.lvy[k] += lvy;
+  W.lQ[k] += lQ;
 }
 
 // 3. EOS and art_heat coefficient adjoint (per particle)
@@ -359,10 +468,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_eos(Tape<R> T, KParams K, AdjScra
   lrhoe += -ldv * dv / rho;
   R lrhoh = 0;
   if (is_rate(K, T.id[k]) && !(T.flag[k] & 2u)) {
-    // dp/drho = 7 B / rho0 (rho / rho0)^6
-    const R t = rho / (R)K.rho0;
-    const R t2 = t * t;
-    lrhoh = lrhoe + lp * (R)(7.0 * K.B / K.rho0) * (t2 * t2 * t2);
+    if (K.eos == 1) {
+      // linear: dp/drho = c0^2
+      lrhoh = lrhoe + lp * (R)K.lin_c2;
+    } else {
+      // Tait: dp/drho = 7 B / rho0 (rho / rho0)^6
+      const R t = rho / (R)K.rho0;
+      const R t2 = t * t;
+      lrhoh = lrhoe + lp * (R)(7.0 * K.B / K.rho0) * (t2 * t2 * t2);
+    }
   }
   W.lrhoh[k] = lrhoh;
 }

@@ -71,9 +71,9 @@ template <typename R> struct St {
   double* u_min;
   R2* v_min;
   uint8_t* flag;  // bit 0: predictor clamped u at 0; bit 1: EOS clamp taken
-  // physical viscosity, per particle: 4 values (Wxx, Wyy, Wxy, tds) with
-  // W = eta T (pa_sph 0) or eta T / rho^2 (pa_sph 1), T the shear tensor
-  // and tds = eta/(2 rho) T:T
+  // physical viscosity, per particle: 4 values (Txx, Tyy, Txy, tds), T the
+  // shear tensor and tds = eta/(2 rho) T:T; the force uses W = eta T
+  // (pa_sph 0) or eta T / rho^2 (pa_sph 1)
   R* tau;
 };
 

@@ -334,9 +334,15 @@ template <typename R> struct Tau3 {
   R xx, yy, xy;
 };
 
-template <typename R> __device__ __forceinline__ Tau3<R> tau_of(const St<R>& S, int t) {
+// Viscous weight of a particle: W = c T with c = eta (pa_sph 0) or
+// eta / rho^2 = eta Q^2 (pa_sph 1).
+template <typename R> __device__ __forceinline__ R visc_c(const KParams& K, R Q) {
+  return K.pa_sph == 0 ? (R)K.eta : (R)K.eta * Q * Q;
+}
+
+template <typename R> __device__ __forceinline__ Tau3<R> tau_of(const St<R>& S, int t, R c) {
   const R* w = S.tau + 4 * (size_t)t;
-  return {w[0], w[1], w[2]};
+  return {c * w[0], c * w[1], c * w[2]};
 }
 
 // Q = 1/rho and A = p/rho of a particle from its stored density.
@@ -354,9 +360,10 @@ __device__ __forceinline__ void force_begin(const St<R>& S, const KParams& K, in
   if constexpr (sizeof(R) == 4) o.u = S.uf[i]; else o.u = (R)S.u[i];
   if (VI) {
     const R* w = S.tau + 4 * (size_t)i;
-    o.wxx = w[0];
-    o.wyy = w[1];
-    o.wxy = w[2];
+    const R c = visc_c(K, o.Q);
+    o.wxx = c * w[0];
+    o.wyy = c * w[1];
+    o.wxy = c * w[2];
     o.tds = w[3];
   }
 }
@@ -512,7 +519,7 @@ __device__ __forceinline__ void force_direct(St<R>& S, const KParams& K, int i,
       R Qj, Aj;
       q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
       force_pair<R, PRESS, AV, AH, XS, VI>(K, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t],
-                                           u_of(S, t), VI ? tau_of(S, t) : Tau3<R>{});
+                                           u_of(S, t), VI ? tau_of(S, t, visc_c(K, Qj)) : Tau3<R>{});
     }
   force_end<R, PRESS, XS, VI>(S, K, i, o);
 }
@@ -579,11 +586,10 @@ __device__ __forceinline__ void tens_end(St<R>& S, const KParams& K, int i, R Qi
                                          const TensOwn<R>& o) {
   const R eta = (R)K.eta;
   const R tt = o.xx * o.xx + o.yy * o.yy + (R)2 * (o.xy * o.xy);
-  const R f = K.pa_sph == 0 ? eta : eta * Qi * Qi;
   R* w = S.tau + 4 * (size_t)i;
-  w[0] = f * o.xx;
-  w[1] = f * o.yy;
-  w[2] = f * o.xy;
+  w[0] = o.xx;
+  w[1] = o.yy;
+  w[2] = o.xy;
   w[3] = (R)0.5 * eta * Qi * tt;
 }
 

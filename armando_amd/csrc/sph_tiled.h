@@ -109,16 +109,16 @@ template <int CAP> struct DensTile<double, CAP> {
   __device__ __forceinline__ double2 vel(int l) const { return v[l]; }
 };
 
-// Neighbour W of the physical viscosity, staged only when VI.
+// Neighbour W = c T of the physical viscosity, staged only when VI.
 template <typename R, int CAP, bool VI> struct TauTile {
-  __device__ __forceinline__ void put(int, const St<R>&, int) {}
+  __device__ __forceinline__ void put(int, const St<R>&, int, R) {}
   __device__ __forceinline__ Tau3<R> get(int) const { return Tau3<R>{}; }
 };
 template <int CAP> struct TauTile<float, CAP, true> {
   float4 w[CAP];
-  __device__ __forceinline__ void put(int l, const St<float>& S, int t) {
+  __device__ __forceinline__ void put(int l, const St<float>& S, int t, float c) {
     const float4 q = reinterpret_cast<const float4*>(S.tau)[t];
-    w[l] = q;
+    w[l] = make_float4(c * q.x, c * q.y, c * q.z, q.w);
   }
   __device__ __forceinline__ Tau3<float> get(int l) const {
     const float4 q = w[l];
@@ -128,10 +128,10 @@ template <int CAP> struct TauTile<float, CAP, true> {
 template <int CAP> struct TauTile<double, CAP, true> {
   double2 w01[CAP];
   double w2[CAP];
-  __device__ __forceinline__ void put(int l, const St<double>& S, int t) {
+  __device__ __forceinline__ void put(int l, const St<double>& S, int t, double c) {
     const double* q = S.tau + 4 * (size_t)t;
-    w01[l] = make_double2(q[0], q[1]);
-    w2[l] = q[2];
+    w01[l] = make_double2(c * q[0], c * q[1]);
+    w2[l] = c * q[2];
   }
   __device__ __forceinline__ Tau3<double> get(int l) const {
     const double2 q = w01[l];
@@ -153,7 +153,7 @@ template <int CAP, bool VI> struct ForceTile<float, CAP, VI> {
     q_and_a(K, S.rho[t], S.p[t], Q, A);
     b[l] = make_float4(Q, A, S.psi[t], S.uf[t]);
     cx[l] = (float)(S.cxy[t] & 0xffffu);
-    tw.put(l, S, t);
+    tw.put(l, S, t, visc_c(K, Q));
   }
   __device__ __forceinline__ LaneF lane(int l) const {
     const float4 q = a[l];
@@ -183,7 +183,7 @@ template <int CAP, bool VI> struct ForceTile<double, CAP, VI> {
     q_and_a(K, S.rho[t], S.p[t], Q, A);
     rp[l] = make_double2(Q, A);
     pu[l] = make_double2(S.psi[t], S.u[t]);
-    tw.put(l, S, t);
+    tw.put(l, S, t, visc_c(K, Q));
   }
   __device__ __forceinline__ PosD lane(int l) const {
     const double2 q = x[l];

@@ -158,8 +158,7 @@ int arm_time_integration(arm_ctx* ctx, int its0, int nsteps);
    (2, n) in the original order, and the forward / backward device times in
    ms (dJ/d hs_q0: arm_rollout_param_grads).  The context is left at the
    state after nsteps.  Either form (the FP32 form's reverse pair arithmetic
-   is FP32, its state adjoint FP64); continuity density, Tait EOS, no
-   physical viscosity. */
+   is FP32, its state adjoint FP64); continuity density. */
 int arm_adjoint_rollout(arm_ctx* ctx, int nsteps, int ckpt_every, const int32_t* obj_ids,
                         int nobj, double* J, double* dJ_dalpha, double* dJ_dv0, double* ms_fwd,
                         double* ms_bwd);
@@ -181,11 +180,13 @@ int arm_update_params(arm_ctx* ctx, const arm_params* prm);
 #define ARM_OBJ_ENERGY 1
 int arm_set_objective(arm_ctx* ctx, int kind);
 /* Parameter gradients of the last arm_rollout_backward (scaled by its seed):
-   out[ARM_GRAD_ALPHA] = dJ/d av_alpha, out[ARM_GRAD_Q0] = dJ/d hs_q0.  n is
-   the length of out.  No counterpart in the reference. */
+   out[ARM_GRAD_ALPHA] = dJ/d av_alpha, out[ARM_GRAD_Q0] = dJ/d hs_q0,
+   out[ARM_GRAD_VISC] = dJ/d visc_eta (with ARM_VISC).  n is the length of
+   out.  No counterpart in the reference. */
 #define ARM_GRAD_ALPHA 0
 #define ARM_GRAD_Q0 1
-#define ARM_NGRAD 2
+#define ARM_GRAD_VISC 2
+#define ARM_NGRAD 3
 int arm_rollout_param_grads(arm_ctx* ctx, double* out, int n);
 
 /* FP64 sums over real particles: mass, momentum x, momentum y, kinetic

@@ -95,7 +95,7 @@ def test_heat_source_adjoint_against_central_differences():
     c, obj = heated_dam()
     steps, ck = 20, 6
     banner(f"dJ/dq0, energy objective, n={c['n']} steps={steps}")
-    J, (da, dq), g = rollout(c, obj, steps, ck, armando_amd.OBJ_ENERGY)
+    J, (da, dq, _), g = rollout(c, obj, steps, ck, armando_amd.OBJ_ENERGY)
     q0 = c["hs_q0"]
     e = 1e-4 * q0
     Jp = rollout(c, obj, steps, ck, armando_amd.OBJ_ENERGY, q0=q0 + e, backward=False)[0]
@@ -118,7 +118,7 @@ def test_heat_source_adjoint_against_central_differences():
         ad = float(np.sum(g * d))
         assert abs(fdv - ad) <= 1e-5 * abs(fdv), (seed, fdv, ad)
     # the pressure objective does not see the internal energy
-    _, (da_p, dq_p), _ = rollout(c, obj, steps, ck, armando_amd.OBJ_PRESSURE)
+    _, (da_p, dq_p, _), _ = rollout(c, obj, steps, ck, armando_amd.OBJ_PRESSURE)
     assert dq_p == 0.0 and da_p != 0.0
 
 

@@ -126,10 +126,120 @@ def test_fp32_form_close_to_fp64_with_viscosity():
     b.close()
 
 
-def test_adjoint_rejects_viscosity():
-    c = cases.shear_cavity()
-    sim = armando_amd.Simulation(c)
-    with pytest.raises(armando_amd.ArmandoError) as e:
-        sim.rollout_forward(4, 2, np.array([0], np.int32))
-    assert e.value.code == -5
+def viscous_dam(precision=cases.FP64, eta=0.05):
+    c = cases.dam_break_small(scale=0.02, precision=precision)
+    c["flags"] = c["flags"] | cases.VISC
+    c["visc_eta"] = eta
+    nf = c["np"]
+    z = cases.normal01(777, 2 * nf)
+    c["vx"] = c["vx"].copy()
+    c["vy"] = c["vy"].copy()
+    c["vx"][:nf] = 0.01 * z[:nf]
+    c["vy"][:nf] = 0.01 * z[nf:]
+    obj = np.nonzero((c["x"] < 0) & (c["y"] < 0.2 * 1.2 * 0.02))[0].astype(np.int32)
+    return c, obj
+
+
+def visc_rollout(c, obj, steps, ckpt, eta=None, backward=True):
+    c2 = dict(c)
+    if eta is not None:
+        c2["visc_eta"] = eta
+    sim = armando_amd.Simulation(c2)
+    J = sim.rollout_forward(steps, ckpt, obj)[0]
+    out = (J,)
+    if backward:
+        _, g, _ = sim.rollout_backward(1.0)
+        out = (J, sim.param_grads(), g)
     sim.close()
+    return out
+
+
+@pytest.mark.parametrize("pa", [0, 1])
+def test_viscous_adjoint_against_central_differences(pa):
+    """dJ/d eta, dJ/d alpha and dJ/dv0 . d through the shear tensor, the
+    viscous force and tds/dt, against central differences of the GPU forward."""
+    c, obj = viscous_dam()
+    c["pa_sph"] = pa
+    steps, ck = 16, 5
+    banner(f"viscous adjoint pa_sph={pa} n={c['n']} steps={steps}")
+    J, (da, _, de), g = visc_rollout(c, obj, steps, ck)
+    eta = c["visc_eta"]
+    e = 1e-4 * eta
+    fde = (visc_rollout(c, obj, steps, ck, eta=eta + e, backward=False)[0]
+           - visc_rollout(c, obj, steps, ck, eta=eta - e, backward=False)[0]) / (2 * e)
+    banner(f"  d/deta fd {fde:.8e} adjoint {de:.8e}")
+    assert abs(fde - de) <= 1e-5 * abs(fde)
+    a0 = c["av_alpha"]
+    ea = 1e-4 * a0
+    ca, cb = dict(c), dict(c)
+    ca["av_alpha"], cb["av_alpha"] = a0 + ea, a0 - ea
+    fda = (visc_rollout(ca, obj, steps, ck, backward=False)[0]
+           - visc_rollout(cb, obj, steps, ck, backward=False)[0]) / (2 * ea)
+    assert abs(fda - da) <= 1e-5 * abs(fda), (fda, da)
+    nf = c["np"]
+    z = cases.normal01(1003, 2 * nf)
+    d = np.zeros((c["n"], 2))
+    d[:nf, 0], d[:nf, 1] = z[:nf], z[nf:]
+    ev = 1e-6
+    cp, cm = dict(c), dict(c)
+    cp["vx"], cp["vy"] = c["vx"] + ev * d[:, 0], c["vy"] + ev * d[:, 1]
+    cm["vx"], cm["vy"] = c["vx"] - ev * d[:, 0], c["vy"] - ev * d[:, 1]
+    fdv = (visc_rollout(cp, obj, steps, ck, backward=False)[0]
+           - visc_rollout(cm, obj, steps, ck, backward=False)[0]) / (2 * ev)
+    ad = float(np.sum(g * d))
+    assert abs(fdv - ad) <= 1e-5 * abs(fdv), (fdv, ad)
+
+
+def test_viscous_adjoint_fp32_against_fp64_differences():
+    c, obj = viscous_dam()
+    c32, _ = viscous_dam(precision=cases.FP32)
+    steps, ck = 16, 5
+    banner("viscous adjoint fp32")
+    _, (da, _, de), _ = visc_rollout(c32, obj, steps, ck)
+    eta = c["visc_eta"]
+    e = 1e-4 * eta
+    fde = (visc_rollout(c, obj, steps, ck, eta=eta + e, backward=False)[0]
+           - visc_rollout(c, obj, steps, ck, eta=eta - e, backward=False)[0]) / (2 * e)
+    banner(f"  d/deta fd {fde:.8e} fp32 adjoint {de:.8e}")
+    assert abs(fde - de) <= 1e-3 * abs(fde)
+
+
+def test_viscous_adjoint_at_zero_viscosity():
+    """With the tensor pass on and eta = 0 the forward equals the inviscid one
+    and dJ/d eta is still defined: against the second-order one-sided
+    difference (-3 J(0) + 4 J(e) - J(2e)) / 2e (eta must not be negative)."""
+    c, obj = viscous_dam(eta=0.0)
+    steps, ck = 12, 4
+    J, (_, _, de), _ = visc_rollout(c, obj, steps, ck)
+    c0 = dict(c)
+    c0["flags"] = c["flags"] & ~cases.VISC
+    sim = armando_amd.Simulation(c0)
+    J0 = sim.rollout_forward(steps, ck, obj)[0]
+    sim.close()
+    assert J == J0
+    e = 1e-4
+    fde = (-3.0 * J + 4.0 * visc_rollout(c, obj, steps, ck, eta=e, backward=False)[0]
+           - visc_rollout(c, obj, steps, ck, eta=2 * e, backward=False)[0]) / (2 * e)
+    banner(f"  eta=0: d/deta fd {fde:.8e} adjoint {de:.8e}")
+    assert abs(fde - de) <= 1e-4 * abs(fde)
+
+
+def test_torch_autograd_viscosity_input():
+    import torch
+
+    from armando_amd.autograd import RolloutLoss
+
+    c, obj = viscous_dam()
+    steps = 8
+    ref = visc_rollout(c, obj, steps, 4)
+    loss = RolloutLoss(c, obj, steps, 4)
+    v0 = torch.tensor(np.stack([c["vx"], c["vy"]], 1), dtype=torch.float64, requires_grad=True)
+    alpha = torch.tensor(c["av_alpha"], dtype=torch.float64, requires_grad=True)
+    eta = torch.tensor(c["visc_eta"], dtype=torch.float64, requires_grad=True)
+    J = loss(v0, alpha, None, eta)
+    assert J.item() == ref[0]
+    J.backward()
+    assert eta.grad.item() == pytest.approx(ref[1][2], rel=1e-12)
+    assert alpha.grad.item() == pytest.approx(ref[1][0], rel=1e-12)
+    assert np.allclose(v0.grad.numpy(), ref[2], rtol=1e-12, atol=0)
+    loss.close()
