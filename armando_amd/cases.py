@@ -240,7 +240,116 @@ def with_heat_source(case, q0, x, y, sigma):
     return c
 
 
+# ------------------------------------------------------------------ 3D
+def grid_for3(x, y, z, cell, margin_cells=4, top_extra=0.0):
+    """3D hash grid covering the particles with a margin (cell = 2h)."""
+    o, n = [], []
+    for a, extra in ((x, 0.0), (y, 0.0), (z, top_extra)):
+        lo = float(np.min(a)) - margin_cells * cell
+        o.append(lo)
+        n.append(int(math.ceil((float(np.max(a)) + extra - lo) / cell)) + margin_cells + 1)
+    return o[0], o[1], o[2], n[0], n[1], n[2]
+
+
+def box_lattice3(nx=10, ny=9, nz=8, dx=0.01, jitter=0.1, seed=7, vel=0.0, precision=FP64,
+                 h_fac=1.3):
+    """3D lattice block of real particles only (no walls), jittered, with
+    seeded velocities, for routine-level parity.  Draws per particle (i fastest,
+    then j, then k): x, y, z jitter, vx, vy, vz, rho, u."""
+    K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    n = I.size
+    r = uniform01(seed, 8 * n).reshape(n, 8)
+    x = (I + 0.5) * dx + (2 * r[:, 0] - 1) * jitter * dx
+    y = (J + 0.5) * dx + (2 * r[:, 1] - 1) * jitter * dx
+    z = (K + 0.5) * dx + (2 * r[:, 2] - 1) * jitter * dx
+    h = h_fac * dx
+    cell = 2.0 * h
+    ox, oy, oz, nxg, nyg, nzg = grid_for3(x, y, z, cell)
+    return dict(
+        name="box3", dim=3, n=n, np=n, nv=0, x=x, y=y, z=z,
+        vx=vel * (2 * r[:, 3] - 1), vy=vel * (2 * r[:, 4] - 1), vz=vel * (2 * r[:, 5] - 1),
+        rho=1000.0 * (1.0 + 0.01 * (2 * r[:, 6] - 1)), u=1.0 + 0.5 * r[:, 7],
+        precision=precision, flags=ART_VISC | ART_HEAT | AV_VEL, pa_sph=0, density_method=0,
+        kernel=3, h=h, mass=1000.0 * dx ** 3, rho0=1000.0, c0=10.0, pmin=-1e30,
+        av_alpha=1.0, av_beta=1.0, av_eta=0.1, ah_g1=0.1, ah_g2=1.0, ah_eta=0.1,
+        xsph_eps=0.3, gx=0.0, gy=0.0, gz=0.0, dt=1e-5, ox=ox, oy=oy, oz=oz, cell=cell,
+        nx=nxg, ny=nyg, nz=nzg,
+    )
+
+
+def dam_break3d(col=(0.6, 0.6, 1.2), tank=(3.2, 1.0, 1.6), dx=7.1e-3, jitter=0.01, seed=4242,
+                h_fac=1.3, layers=3, precision=FP64, alpha=0.01, beta=0.0, eps=0.5, g=9.81,
+                cfl=0.2, heat=True):
+    """3D dam break with dynamic (density-integrating) wall particles.
+
+    Fluid: the lattice (i+1/2, j+1/2, k+1/2) dx, i < col_x/dx, j < col_y/dx,
+    k < col_z/dx, in order k outer, j, i inner, each coordinate moved by
+    U(-jitter, jitter) dx (splitmix64(seed), x, y, z per particle).  Walls:
+    every lattice point (i+1/2, j+1/2, k+1/2) dx with -layers <= i <
+    nxw + layers, -layers <= j < nyw + layers, -layers <= k < nzw that lies
+    outside the tank [0, nxw) x [0, nyw) x [0, nzw) (a box open at the top,
+    `layers` thick).  Hydrostatic start in z, gravity -g along z,
+    c0 = 10 sqrt(g col_z), dt = cfl h / c0, Wendland C2 3D."""
+    ni, nj, nk = (int(round(c / dx)) for c in col)
+    K, J, I = np.meshgrid(np.arange(nk), np.arange(nj), np.arange(ni), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    nf = I.size
+    x = (I + 0.5) * dx
+    y = (J + 0.5) * dx
+    z = (K + 0.5) * dx
+    if jitter:
+        u = uniform01(seed, 3 * nf).reshape(nf, 3)
+        x = x + (2.0 * u[:, 0] - 1.0) * jitter * dx
+        y = y + (2.0 * u[:, 1] - 1.0) * jitter * dx
+        z = z + (2.0 * u[:, 2] - 1.0) * jitter * dx
+    nxw, nyw, nzw = (int(round(t / dx)) for t in tank)
+    L = layers
+    K, J, I = np.meshgrid(np.arange(-L, nzw), np.arange(-L, nyw + L), np.arange(-L, nxw + L),
+                          indexing="ij")
+    outside = (I < 0) | (I >= nxw) | (J < 0) | (J >= nyw) | (K < 0)
+    wx = (I[outside] + 0.5) * dx
+    wy = (J[outside] + 0.5) * dx
+    wz = (K[outside] + 0.5) * dx
+    rho0 = 1000.0
+    col_h = col[2]
+    c0 = 10.0 * math.sqrt(g * col_h)
+    B = rho0 * c0 * c0 / 7.0
+    h = h_fac * dx
+
+    def hydro(zz):
+        return rho0 * (1.0 + rho0 * g * (col_h - zz) / B) ** (1.0 / 7.0)
+
+    under = (wx <= col[0]) & (wy <= col[1]) & (wz <= col_h)
+    rho_w = np.where(under, hydro(np.minimum(wz, col_h)), rho0)
+    X = np.concatenate([x, wx])
+    Y = np.concatenate([y, wy])
+    Z = np.concatenate([z, wz])
+    n = X.size
+    cell = 2.0 * h
+    ox, oy, oz, nxg, nyg, nzg = grid_for3(X, Y, Z, cell, top_extra=0.25 * tank[2])
+    flags = DBC | GRAVITY | AV_VEL | (ART_VISC if alpha or beta else 0) | (ART_HEAT if heat else 0)
+    return dict(
+        name="dam_break3d", dim=3, n=n, np=nf, nv=n - nf,
+        x=X, y=Y, z=Z, vx=np.zeros(n), vy=np.zeros(n), vz=np.zeros(n),
+        rho=np.concatenate([hydro(z), rho_w]), u=np.zeros(n),
+        precision=precision, flags=flags, pa_sph=0, density_method=0,
+        kernel=3, h=h, mass=rho0 * dx ** 3, rho0=rho0, c0=c0, pmin=0.0,
+        av_alpha=alpha, av_beta=beta, av_eta=0.1,
+        ah_g1=0.1, ah_g2=1.0, ah_eta=0.1,
+        xsph_eps=eps, gx=0.0, gy=0.0, gz=-g, dt=cfl * h / c0,
+        ox=ox, oy=oy, oz=oz, cell=cell, nx=nxg, ny=nyg, nz=nzg,
+    )
+
+
+def dam_break3d_small(scale=0.1, dx=7.1e-3, precision=FP64, seed=4242, jitter=0.01):
+    """The 3D dam break with column and tank dimensions times `scale`."""
+    return dam_break3d(col=(0.6 * scale, 0.6 * scale, 1.2 * scale),
+                       tank=(3.2 * scale, 1.0 * scale, 1.6 * scale), dx=dx,
+                       precision=precision, seed=seed, jitter=jitter)
+
+
 PARAM_KEYS = ("precision", "flags", "pa_sph", "density_method", "h", "mass", "rho0", "c0",
               "pmin", "av_alpha", "av_beta", "av_eta", "ah_g1", "ah_g2", "ah_eta", "xsph_eps",
               "gx", "gy", "dt", "ox", "oy", "cell", "nx", "ny", "visc_eta", "eos", "eos_p0",
-              "hs_q0", "hs_x", "hs_y", "hs_sigma")
+              "hs_q0", "hs_x", "hs_y", "hs_sigma", "dim", "oz", "gz", "nz")

@@ -56,7 +56,8 @@ extern "C" {
 enum {
   ORC_K_WENDLAND2D = 0, /* alpha = 7/(4 pi h^2), support 2h */
   ORC_K_CUBIC2D = 1,    /* kernel.f skf=0, dim=2 factor, REAL(4) constants */
-  ORC_K_CUBIC3D = 2     /* kernel.f skf=0, dim=3 factor (for the as-shipped dim=3 build) */
+  ORC_K_CUBIC3D = 2,    /* kernel.f skf=0, dim=3 factor (for the as-shipped dim=3 build) */
+  ORC_K_WENDLAND3D = 3  /* alpha = 21/(16 pi h^3), support 2h */
 };
 
 enum {
@@ -97,6 +98,9 @@ typedef struct {
   int eos;         /* ORC_EOS_* ; c = c0 for both, clamp p < pmin -> 0, rho0 */
   double eos_p0;   /* ORC_EOS_LINEAR pressure offset */
   double hs_q0, hs_x, hs_y, hs_sigma; /* ORC_HEAT_SOURCE */
+  int dim;         /* 2 (default when 0) or 3: the orc3_* routines */
+  double oz, gz;   /* 3D: grid origin in z, body force in z */
+  int nz;          /* 3D: grid cells in z (key = ix + iy nx + iz nx ny) */
 } orc_params;
 
 /* Particle state, SoA, FP64. */
@@ -227,6 +231,45 @@ void orc_kat_interaction_f32(int n, int maxn, const int32_t* list, const float* 
                              const float* y, const float* vx, const float* vy,
                              const float* rho, const float* p, float smooth, float mass,
                              float sound, float* drho, float* ax, float* ay);
+
+/* ---- 3D (dim = 3), oracle/armando_oracle3d.c --------------------------- */
+/* Particle state in 3D, SoA, FP64. */
+typedef struct {
+  int n, np;
+  double *x, *y, *z, *vx, *vy, *vz, *rho, *u, *p, *c;
+  double *drho, *du, *ax, *ay, *az, *avx, *avy, *avz, *divv;
+  double *rho_min, *u_min, *vx_min, *vy_min, *vz_min;
+} orc3_state;
+
+orc3_state* orc3_state_new(int n, int np);
+void orc3_state_free(orc3_state* s);
+/* 3D cell keys ix + iy nx + iz nx ny; returns particles outside the grid. */
+int orc3_keys(const orc_params* P, int n, const double* x, const double* y, const double* z,
+              int32_t* key);
+/* Unique pairs i<j with r^2 < (2h)^2 (FP64), sorted by (i, j). */
+int orc3_pairs_f64(const orc_params* P, int n, const double* x, const double* y,
+                   const double* z, int cap, int* niac, int* pair_i, int* pair_j);
+/* FP32 form: integer cell and FP32 in-cell offset, and the pairs of the
+   criterion dx = (off_i - off_j) + (float)(c_i - c_j) * (float)cell. */
+void orc3_split_f32(const orc_params* P, int n, const double* x, const double* y,
+                    const double* z, int32_t* cx, int32_t* cy, int32_t* cz, float* ox,
+                    float* oy, float* oz);
+int orc3_pairs_f32(const orc_params* P, int n, const int32_t* cx, const int32_t* cy,
+                   const int32_t* cz, const float* ox, const float* oy, const float* oz,
+                   int cap, int* niac, int* pair_i, int* pair_j);
+/* The per-routine outputs of one step in gather form (every particle counted
+   as rate particle): sum_density rho_sum, con_density drho, int_force a/du
+   (P->pa_sph), art_visc a/du, art_heat du (+ divv), av_vel av.  Arrays of n. */
+int orc3_routines(const orc_params* P, int n, const double* x, const double* y,
+                  const double* z, const double* vx, const double* vy, const double* vz,
+                  const double* rho, const double* p, const double* u, const double* c,
+                  double* rho_sum, double* drho, double* if_a, double* if_du, double* av_a,
+                  double* av_du, double* ah_du, double* divv, double* xs);
+/* single_step (gather twin, (cell, id) order) and time_integration in 3D. */
+int orc3_single_step_gather(const orc_params* P, orc3_state* s);
+void orc3_predictor(const orc_params* P, orc3_state* s);
+void orc3_corrector(const orc_params* P, orc3_state* s, int its);
+int orc3_run(const orc_params* P, orc3_state* s, int its0, int nsteps);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liborc.so")
 REF_PATH = os.path.join(HERE, "_ref", "libarmref.so")
 
-K_WENDLAND2D, K_CUBIC2D, K_CUBIC3D = 0, 1, 2
+K_WENDLAND2D, K_CUBIC2D, K_CUBIC3D, K_WENDLAND3D = 0, 1, 2, 3
 ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
 EOS_TAIT, EOS_LINEAR = 0, 1
 
@@ -32,6 +32,7 @@ class Params(C.Structure):
         ("visc_eta", C.c_double), ("eos", C.c_int), ("eos_p0", C.c_double),
         ("hs_q0", C.c_double), ("hs_x", C.c_double), ("hs_y", C.c_double),
         ("hs_sigma", C.c_double),
+        ("dim", C.c_int), ("oz", C.c_double), ("gz", C.c_double), ("nz", C.c_int),
     ]
 
 
@@ -41,6 +42,16 @@ class State(C.Structure):
             "x", "y", "vx", "vy", "rho", "u", "p", "c", "drho", "du", "ax", "ay",
             "avx", "avy", "divv", "rho_min", "u_min", "vx_min", "vy_min")
     ]
+
+
+STATE3_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "rho", "u", "p", "c", "drho", "du", "ax",
+                 "ay", "az", "avx", "avy", "avz", "divv", "rho_min", "u_min", "vx_min",
+                 "vy_min", "vz_min")
+
+
+class State3(C.Structure):
+    _fields_ = [("n", C.c_int), ("np", C.c_int)] + [
+        (name, C.POINTER(C.c_double)) for name in STATE3_FIELDS]
 
 
 STATE_FIELDS = ("x", "y", "vx", "vy", "rho", "u", "p", "c", "drho", "du", "ax", "ay",
@@ -97,6 +108,19 @@ def lib():
                                     _ip]),
         "orc_clist_f32": (None, [C.c_int, _fp, _fp, C.c_float, C.c_float, C.c_float, C.c_int,
                                  C.c_int, C.c_float, C.c_int, _ip, _ip]),
+        "orc3_state_new": (C.POINTER(State3), [C.c_int, C.c_int]),
+        "orc3_state_free": (None, [C.POINTER(State3)]),
+        "orc3_keys": (C.c_int, [_PP, C.c_int, _dp, _dp, _dp, _ip]),
+        "orc3_pairs_f64": (C.c_int, [_PP, C.c_int, _dp, _dp, _dp, C.c_int, C.POINTER(C.c_int),
+                                     _ip, _ip]),
+        "orc3_split_f32": (None, [_PP, C.c_int, _dp, _dp, _dp, _ip, _ip, _ip, _fp, _fp, _fp]),
+        "orc3_pairs_f32": (C.c_int, [_PP, C.c_int, _ip, _ip, _ip, _fp, _fp, _fp, C.c_int,
+                                     C.POINTER(C.c_int), _ip, _ip]),
+        "orc3_routines": (C.c_int, [_PP, C.c_int] + [_dp] * 10 + [_dp] * 9),
+        "orc3_single_step_gather": (C.c_int, [_PP, C.POINTER(State3)]),
+        "orc3_predictor": (None, [_PP, C.POINTER(State3)]),
+        "orc3_corrector": (None, [_PP, C.POINTER(State3), C.c_int]),
+        "orc3_run": (C.c_int, [_PP, C.POINTER(State3), C.c_int, C.c_int]),
         "orc_kat_interaction_f32": (None, [C.c_int, C.c_int, _ip, _fp, _fp, _fp, _fp, _fp, _fp,
                                            C.c_float, C.c_float, C.c_float, _fp, _fp, _fp]),
     }
@@ -260,3 +284,101 @@ def adjoint(P, st, nsteps, objmask):
     if rc:
         raise RuntimeError(f"orc_adjoint rc={rc}")
     return J.value, da.value, gx, gy
+
+
+# ------------------------------------------------------------------ 3D
+def _d(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def keys3(P, x, y, z):
+    k = np.zeros(len(x), np.int32)
+    bad = lib().orc3_keys(C.byref(P), len(x), _d(x), _d(y), _d(z), k)
+    return k, bad
+
+
+def pairs3_f64(P, x, y, z, cap=None):
+    n = len(x)
+    cap = cap or max(64, n * 64)
+    pi = np.zeros(cap, np.int32)
+    pj = np.zeros(cap, np.int32)
+    niac = C.c_int(0)
+    rc = lib().orc3_pairs_f64(C.byref(P), n, _d(x), _d(y), _d(z), cap, C.byref(niac), pi, pj)
+    if rc:
+        raise RuntimeError(f"orc3_pairs_f64 rc={rc}")
+    return pi[:niac.value].copy(), pj[:niac.value].copy()
+
+
+def split3_f32(P, x, y, z):
+    n = len(x)
+    cells = [np.zeros(n, np.int32) for _ in range(3)]
+    offs = [np.zeros(n, np.float32) for _ in range(3)]
+    lib().orc3_split_f32(C.byref(P), n, _d(x), _d(y), _d(z), *cells, *offs)
+    return cells, offs
+
+
+def pairs3_f32(P, cells, offs, cap=None):
+    n = len(cells[0])
+    cap = cap or max(64, n * 64)
+    pi = np.zeros(cap, np.int32)
+    pj = np.zeros(cap, np.int32)
+    niac = C.c_int(0)
+    rc = lib().orc3_pairs_f32(C.byref(P), n, *[np.ascontiguousarray(c, np.int32) for c in cells],
+                              *[np.ascontiguousarray(o, np.float32) for o in offs], cap,
+                              C.byref(niac), pi, pj)
+    if rc:
+        raise RuntimeError(f"orc3_pairs_f32 rc={rc}")
+    return pi[:niac.value].copy(), pj[:niac.value].copy()
+
+
+def routines3(P, x, y, z, vx, vy, vz, rho, p, u, c):
+    """Per-routine outputs of one 3D step in gather form: a dict of
+    rho_sum, drho, if_a (n, 3), if_du, av_a (n, 3), av_du, ah_du, divv,
+    xs (n, 3)."""
+    n = len(x)
+    out = {k: np.zeros(n) for k in ("rho_sum", "drho", "if_du", "av_du", "ah_du", "divv")}
+    for k in ("if_a", "av_a", "xs"):
+        out[k] = np.zeros((n, 3))
+    rc = lib().orc3_routines(C.byref(P), n, *[_d(a) for a in (x, y, z, vx, vy, vz, rho, p, u, c)],
+                             out["rho_sum"], out["drho"], out["if_a"], out["if_du"], out["av_a"],
+                             out["av_du"], out["ah_du"], out["divv"], out["xs"])
+    if rc:
+        raise RuntimeError(f"orc3_routines rc={rc}")
+    return out
+
+
+class OracleState3:
+    """Owns an orc3_state; fields are exposed as numpy views."""
+
+    def __init__(self, n, np_fluid):
+        self.ptr = lib().orc3_state_new(n, np_fluid)
+        self.n = n
+        self.np = np_fluid
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().orc3_state_free(self.ptr)
+            self.ptr = None
+
+    def field(self, name):
+        p = getattr(self.ptr.contents, name)
+        return np.ctypeslib.as_array(p, shape=(self.n,))
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            self.field(k)[:] = v
+
+    def get(self, *names):
+        return {k: self.field(k).copy() for k in names}
+
+
+def run3(P, st, its0, nsteps):
+    rc = lib().orc3_run(C.byref(P), st.ptr, its0, nsteps)
+    if rc:
+        raise RuntimeError(f"orc3_run rc={rc}")
+
+
+def single_step3(P, st):
+    rc = lib().orc3_single_step_gather(C.byref(P), st.ptr)
+    if rc:
+        raise RuntimeError(f"orc3_single_step_gather rc={rc}")

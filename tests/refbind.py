@@ -57,17 +57,19 @@ def _i(v):
     return C.byref(C.c_int(v))
 
 
-def vec3(a, b):
+def vec3(a, b, c=None):
     X = np.zeros((len(a), 3))
     X[:, 0] = a
     X[:, 1] = b
+    if c is not None:
+        X[:, 2] = c
     return X
 
 
-def link_list(x, y, h):
+def link_list(x, y, h, z=None):
     def f():
         n = len(x)
-        X = vec3(x, y)
+        X = vec3(x, y, z)
         H = np.full(n, h)
         cap = n * 64 + 64
         pi = np.zeros(cap, np.int32)
@@ -78,10 +80,10 @@ def link_list(x, y, h):
     return run_big_stack(f)
 
 
-def direct_find(x, y, h):
+def direct_find(x, y, h, z=None):
     def f():
         n = len(x)
-        X = vec3(x, y)
+        X = vec3(x, y, z)
         H = np.full(n, h)
         cap = n * 64 + 64
         pi = np.zeros(cap, np.int32)
@@ -92,13 +94,15 @@ def direct_find(x, y, h):
     return run_big_stack(f)
 
 
-def pair_routines(x, y, vx, vy, rho, p, u, c, h, mass, pi, pj):
+def pair_routines(x, y, vx, vy, rho, p, u, c, h, mass, pi, pj, z=None, vz=None):
     """Runs sum_density, con_density, int_force (pa_sph 0 and 1),
-    art_visc, art_heat and av_vel with every particle counted as real."""
+    art_visc, art_heat and av_vel with every particle counted as real.
+    With z and vz the third components are real (the build's dim = 3) and
+    the z parts of the vector outputs are returned too (*_az, xs_z)."""
     def f():
         n = len(x)
-        X = vec3(x, y)
-        V = vec3(vx, vy)
+        X = vec3(x, y, z)
+        V = vec3(vx, vy, vz)
         H = np.full(n, h)
         M = np.full(n, mass)
         P1 = np.ascontiguousarray(pi + 1, np.int32)
@@ -128,11 +132,15 @@ def pair_routines(x, y, vx, vy, rho, p, u, c, h, mass, pi, pj):
                              _p(dv), _p(du), _p(tds), _i(niac), _p(P1), _p(P2), C.byref(dt),
                              _i(0), _i(pa), C.byref(visc))
             out[f"if{pa}_ax"], out[f"if{pa}_ay"], out[f"if{pa}_du"] = dv[:, 0].copy(), dv[:, 1].copy(), du
+            if z is not None:
+                out[f"if{pa}_az"] = dv[:, 2].copy()
         dv = np.zeros((n, 3))
         du = np.zeros(n)
         lib().art_visc_(_i(n), _p(X), _p(V), _p(H), _p(M), _p(R), _p(Cc), _p(dv), _p(du),
                         _i(niac), _p(P1), _p(P2), _i(0))
         out["av_ax"], out["av_ay"], out["av_du"] = dv[:, 0].copy(), dv[:, 1].copy(), du
+        if z is not None:
+            out["av_az"] = dv[:, 2].copy()
         du = np.zeros(n)
         lib().art_heat_(_i(n), _p(X), _p(V), _p(H), _p(M), _p(R), _p(U), _p(Cc), _p(du),
                         _i(niac), _p(P1), _p(P2), _i(0))
@@ -141,6 +149,8 @@ def pair_routines(x, y, vx, vy, rho, p, u, c, h, mass, pi, pj):
         lib().av_vel_(_i(n), _p(X), _p(V), _p(H), _p(M), _p(R), _p(av), _i(niac), _p(P1),
                       _p(P2), _i(0))
         out["xs_x"], out["xs_y"] = av[:, 0].copy(), av[:, 1].copy()
+        if z is not None:
+            out["xs_z"] = av[:, 2].copy()
         return out
     return run_big_stack(f)
 
