@@ -34,7 +34,7 @@ EXPORTS = ("arm_create", "arm_destroy", "arm_last_error", "arm_domain_violations
            "arm_conservation", "arm_pass_times", "arm_reset_pass_times", "arm_set_timing",
            "arm_set_gather_path", "arm_adjoint_rollout", "arm_rollout_forward",
            "arm_rollout_backward", "arm_update_params", "arm_set_objective",
-           "arm_rollout_param_grads")
+           "arm_rollout_param_grads", "arm_conservation3")
 
 
 class ArmParams(C.Structure):
@@ -50,13 +50,14 @@ class ArmParams(C.Structure):
         ("visc_eta", C.c_double), ("eos", C.c_int), ("eos_p0", C.c_double),
         ("hs_q0", C.c_double), ("hs_x", C.c_double), ("hs_y", C.c_double),
         ("hs_sigma", C.c_double),
+        ("dim", C.c_int), ("oz", C.c_double), ("gz", C.c_double), ("nz", C.c_int),
     ]
 
 
 # fields a case may leave out, with their defaults (no physical viscosity,
 # Tait EOS)
 PARAM_DEFAULTS = {"visc_eta": 0.0, "eos": 0, "eos_p0": 0.0, "hs_q0": 0.0, "hs_x": 0.0,
-                  "hs_y": 0.0, "hs_sigma": 0.0}
+                  "hs_y": 0.0, "hs_sigma": 0.0, "dim": 2, "oz": 0.0, "gz": 0.0, "nz": 1}
 OBJ_PRESSURE, OBJ_ENERGY = 0, 1
 
 
@@ -101,6 +102,7 @@ def load_library(path=LIB_PATH):
         "arm_corrector": (C.c_int, [vp, C.c_int]),
         "arm_time_integration": (C.c_int, [vp, C.c_int, C.c_int]),
         "arm_conservation": (C.c_int, [vp, _dp]),
+        "arm_conservation3": (C.c_int, [vp, _dp]),
         "arm_pass_times": (C.c_int, [vp, C.c_int, dpn, C.POINTER(C.c_int)]),
         "arm_reset_pass_times": (C.c_int, [vp]),
         "arm_set_timing": (C.c_int, [vp, C.c_int]),
@@ -141,15 +143,19 @@ class Simulation:
         self.n = int(case["n"])
         self.np = int(case["np"])
         self.params = params_from(case)
+        self.dim = 3 if self.params.dim == 3 else 2
         h = C.c_void_p()
         rc = self.L.arm_create(C.byref(self.params), int(capacity or self.n), C.byref(h))
         self.ctx = h
         self._check(rc)
-        X = np.ascontiguousarray(np.stack([case["x"], case["y"]], 1), np.float64)
-        V = np.ascontiguousarray(np.stack([case["vx"], case["vy"]], 1), np.float64)
+        X = np.ascontiguousarray(np.stack([case[k] for k in self._axes()], 1), np.float64)
+        V = np.ascontiguousarray(np.stack([case["v" + k] for k in self._axes()], 1), np.float64)
         self._check(self.L.arm_upload(self.ctx, self.np, self.n - self.np, _ptr(X), _ptr(V),
                                       _ptr(np.ascontiguousarray(case["rho"], np.float64)),
                                       _ptr(np.ascontiguousarray(case["u"], np.float64))))
+
+    def _axes(self):
+        return ("x", "y", "z")[:self.dim]
 
     def _check(self, rc):
         if rc != ARM_OK:
@@ -199,38 +205,45 @@ class Simulation:
 
     # ---- data ---------------------------------------------------------------
     def state(self):
-        n = self.n
-        X = np.zeros((n, 2))
-        V = np.zeros((n, 2))
+        n, d = self.n, self.dim
+        X = np.zeros((n, d))
+        V = np.zeros((n, d))
         rho = np.zeros(n)
         p = np.zeros(n)
         u = np.zeros(n)
         self._check(self.L.arm_download(self.ctx, _ptr(X), _ptr(V), _ptr(rho), _ptr(p), _ptr(u)))
-        return dict(x=X[:, 0].copy(), y=X[:, 1].copy(), vx=V[:, 0].copy(), vy=V[:, 1].copy(),
-                    rho=rho, p=p, u=u)
+        out = dict(rho=rho, p=p, u=u)
+        for a, k in enumerate(self._axes()):
+            out[k] = X[:, a].copy()
+            out["v" + k] = V[:, a].copy()
+        return out
 
     def rates(self):
-        n = self.n
+        n, d = self.n, self.dim
         dr = np.zeros(n)
-        A = np.zeros((n, 2))
+        A = np.zeros((n, d))
         du = np.zeros(n)
-        W = np.zeros((n, 2))
+        W = np.zeros((n, d))
         dv = np.zeros(n)
         self._check(self.L.arm_download_rates(self.ctx, _ptr(dr), _ptr(A), _ptr(du), _ptr(W),
                                               _ptr(dv)))
-        return dict(drho=dr, ax=A[:, 0].copy(), ay=A[:, 1].copy(), du=du, avx=W[:, 0].copy(),
-                    avy=W[:, 1].copy(), divv=dv)
+        out = dict(drho=dr, du=du, divv=dv)
+        for a, k in enumerate(self._axes()):
+            out["a" + k] = A[:, a].copy()
+            out["av" + k] = W[:, a].copy()
+        return out
 
     def pairs(self, cap=None):
         niac = C.c_int(0)
-        cap = cap or self.n * 16 + 64
+        cap = cap or self.n * (16 if self.dim == 2 else 64) + 64
         pi = np.zeros(cap, np.int32)
         pj = np.zeros(cap, np.int32)
         self._check(self.L.arm_get_pairs(self.ctx, cap, C.byref(niac), pi, pj))
         return pi[:niac.value].copy(), pj[:niac.value].copy()
 
     def cells(self):
-        ncell = int(self.case["nx"]) * int(self.case["ny"])
+        ncell = int(self.case["nx"]) * int(self.case["ny"]) * \
+            (int(self.case["nz"]) if self.dim == 3 else 1)
         k = np.zeros(self.n, np.int32)
         s = np.zeros(self.n, np.int32)
         cs = np.zeros(ncell + 1, np.int32)
@@ -238,8 +251,15 @@ class Simulation:
         return k, s, cs
 
     def conservation(self):
+        """mass, momentum x, momentum y, kinetic energy, internal energy."""
         out = np.zeros(5)
         self._check(self.L.arm_conservation(self.ctx, out))
+        return out
+
+    def conservation3(self):
+        """mass, momentum x, y, z, kinetic energy, internal energy."""
+        out = np.zeros(6)
+        self._check(self.L.arm_conservation3(self.ctx, out))
         return out
 
     def adjoint_rollout(self, nsteps, ckpt_every, obj_ids):

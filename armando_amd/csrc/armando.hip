@@ -21,6 +21,7 @@
 #include "sph_kernels.h"
 #include "sph_tiled.h"
 #include "sph_adjoint.h"
+#include "sph3d.h"
 
 using namespace arm;
 
@@ -53,6 +54,9 @@ struct arm_ctx {
   // state double buffer (the active set is A)
   St<double> A64{}, B64{};
   St<float> A32{}, B32{};
+  // three dimensions (prm.dim == 3): the 3D state double buffer
+  St3<double> A3_64{}, B3_64{};
+  St3<float> A3_32{}, B3_32{};
   // sort workspace
   uint32_t *key_by_id = nullptr, *slot_by_id = nullptr;
   uint32_t *sk[2] = {nullptr, nullptr}, *sv[2] = {nullptr, nullptr};
@@ -1008,6 +1012,387 @@ template <typename R> int conservation_t(arm_ctx* c, double out[5]) {
   return ARM_OK;
 }
 
+// ---- three dimensions ---------------------------------------------------------
+// The 3D step on the same context machinery (sort workspace, cell ranges,
+// timing); only the state layout and the gathers differ.
+template <typename R> int alloc_state3(arm_ctx* c, St3<R>& S, int cap) {
+  int rc = ARM_OK;
+#define AL(f, cnt) \
+  if ((rc = dalloc(c, &S.f, (cnt))) != ARM_OK) return rc;
+  AL(id, cap);
+  AL(xyz, cap);
+  if (sizeof(R) == 4) {
+    AL(pf, cap);
+    AL(uf, cap);
+  }
+  AL(v, cap);
+  AL(rho, cap);
+  AL(u, cap);
+  AL(p, cap);
+  AL(psi, cap);
+  AL(drho, cap);
+  AL(du, cap);
+  AL(a, cap);
+  AL(av, cap);
+  AL(divv, cap);
+  AL(rho_min, cap);
+  AL(u_min, cap);
+  AL(v_min, cap);
+  AL(flag, cap);
+#undef AL
+  return ARM_OK;
+}
+
+bool is3d(const arm_ctx* c) { return c->prm.dim == 3; }
+
+template <typename R> St3<R>& st3A(arm_ctx* c);
+template <> St3<double>& st3A<double>(arm_ctx* c) { return c->A3_64; }
+template <> St3<float>& st3A<float>(arm_ctx* c) { return c->A3_32; }
+template <typename R> St3<R>& st3B(arm_ctx* c);
+template <> St3<double>& st3B<double>(arm_ctx* c) { return c->B3_64; }
+template <> St3<float>& st3B<float>(arm_ctx* c) { return c->B3_32; }
+
+template <typename R> int link_list3_t(arm_ctx* c) {
+  const int n = c->n;
+  St3<R>& A = st3A<R>(c);
+  St3<R>& B = st3B<R>(c);
+  {
+    PassTimer tm(c, ARM_PASS_KEYS);
+    zero_counts(c);
+    hipLaunchKernelGGL(k3_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                       key_out(c), c->d_err);
+  }
+  const uint32_t* sval = nullptr;
+  bool ranges = false;
+  {
+    PassTimer tm(c, ARM_PASS_SORT);
+    int rc = run_sort(c, A.id, &sval, &ranges);
+    if (rc) return rc;
+  }
+  {
+    PassTimer tm(c, ARM_PASS_REORDER);
+    hipLaunchKernelGGL(k3_reorder_all<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
+                       c->K, c->skey, sval, c->cstart, ranges ? 0 : 1);
+  }
+  HIPCHK(c, hipGetLastError());
+  std::swap(A, B);
+  c->sorted = true;
+  return check_domain(c);
+}
+
+template <typename R> int density3_t(arm_ctx* c, int method, bool eos) {
+  St3<R>& A = st3A<R>(c);
+  PassTimer tm(c, ARM_PASS_DENSITY);
+  const dim3 g(grid_of(c->n)), b(BLOCK);
+  if (method == 1) {
+    if (eos)
+      hipLaunchKernelGGL((k3_density<R, true, true>), g, b, 0, c->stream, A, c->K, c->skey, c->cstart);
+    else
+      hipLaunchKernelGGL((k3_density<R, true, false>), g, b, 0, c->stream, A, c->K, c->skey, c->cstart);
+  } else {
+    if (eos)
+      hipLaunchKernelGGL((k3_density<R, false, true>), g, b, 0, c->stream, A, c->K, c->skey, c->cstart);
+    else
+      hipLaunchKernelGGL((k3_density<R, false, false>), g, b, 0, c->stream, A, c->K, c->skey, c->cstart);
+  }
+  HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+template <typename R> int eos3_t(arm_ctx* c) {
+  PassTimer tm(c, ARM_PASS_EOS);
+  hipLaunchKernelGGL(k3_eos<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, st3A<R>(c), c->K);
+  HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+template <typename R, bool P, bool AV, bool AH, bool XS> void launch_force3(arm_ctx* c) {
+  hipLaunchKernelGGL((k3_force<R, P, AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                     st3A<R>(c), c->K, c->skey, c->cstart);
+}
+
+template <typename R> int force3_t(arm_ctx* c, bool press, int flags, bool xs) {
+  const bool av = (flags & ARM_ART_VISC) != 0, ah = (flags & ARM_ART_HEAT) != 0;
+  if (press && (flags & ARM_VISC))
+    return fail(c, ARM_ERR_UNSUPPORTED, "physical viscosity is implemented in 2D only");
+  PassTimer tm(c, ARM_PASS_FORCE);
+  const int code = (press ? 8 : 0) | (av ? 4 : 0) | (ah ? 2 : 0) | (xs ? 1 : 0);
+  switch (code) {
+    case 8: launch_force3<R, true, false, false, false>(c); break;
+    case 9: launch_force3<R, true, false, false, true>(c); break;
+    case 10: launch_force3<R, true, false, true, false>(c); break;
+    case 11: launch_force3<R, true, false, true, true>(c); break;
+    case 12: launch_force3<R, true, true, false, false>(c); break;
+    case 13: launch_force3<R, true, true, false, true>(c); break;
+    case 14: launch_force3<R, true, true, true, false>(c); break;
+    case 15: launch_force3<R, true, true, true, true>(c); break;
+    case 1: launch_force3<R, false, false, false, true>(c); break;
+    default: return fail(c, ARM_ERR_ARG, "force pass with nothing to compute");
+  }
+  HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+template <typename R> int zero_av3(arm_ctx* c) {
+  HIPCHK(c, hipMemsetAsync(st3A<R>(c).av, 0, sizeof(typename V4<R>::type) * (size_t)c->n,
+                           c->stream));
+  return ARM_OK;
+}
+
+template <typename R> int single_step3_t(arm_ctx* c) {
+  int rc;
+  if ((rc = density3_t<R>(c, c->prm.density_method, true))) return rc;
+  const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
+  if ((rc = force3_t<R>(c, true, c->prm.flags, xs))) return rc;
+  if (!xs && (rc = zero_av3<R>(c))) return rc;
+  c->have_rates = true;
+  return ARM_OK;
+}
+
+template <typename R> int predictor3_t(arm_ctx* c) {
+  PassTimer tm(c, ARM_PASS_INTEGRATE);
+  hipLaunchKernelGGL(k3_predictor<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                     st3A<R>(c), c->K);
+  HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+template <typename R> int corrector3_t(arm_ctx* c, int its, bool keys) {
+  PassTimer tm(c, ARM_PASS_INTEGRATE);
+  hipLaunchKernelGGL(k3_corrector<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                     st3A<R>(c), c->K, its, keys ? 1 : 0, keys ? key_out(c) : KeyOut{},
+                     c->d_err);
+  HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+// The fused step of time_integration.f:131-235 in 3D, as fused_step in 2D.
+template <typename R> int fused_step3(arm_ctx* c, int its) {
+  const int n = c->n;
+  const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
+  int rc;
+  const uint32_t* sval = nullptr;
+  bool ranges = false;
+  {
+    PassTimer tm(c, ARM_PASS_SORT);
+    if ((rc = run_sort(c, st3A<R>(c).id, &sval, &ranges))) return rc;
+  }
+  {
+    St3<R>& A = st3A<R>(c);
+    St3<R>& B = st3B<R>(c);
+    PassTimer tm(c, ARM_PASS_REORDER);
+    hipLaunchKernelGGL(k3_reorder_pred<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
+                       c->K, c->skey, sval, c->cstart, its != 1 ? 1 : 0, ranges ? 0 : 1);
+    std::swap(A, B);
+  }
+  if (!xs && (rc = zero_av3<R>(c))) return rc;
+  c->sorted = true;
+  if ((rc = density3_t<R>(c, c->prm.density_method, true))) return rc;
+  if ((rc = force3_t<R>(c, true, c->prm.flags, xs))) return rc;
+  return corrector3_t<R>(c, its, true);
+}
+
+template <typename R> int time_integration3_t(arm_ctx* c, int its0, int nsteps) {
+  int rc;
+  {
+    PassTimer tm(c, ARM_PASS_KEYS);
+    zero_counts(c);
+    hipLaunchKernelGGL(k3_keys<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, st3A<R>(c),
+                       c->K, key_out(c), c->d_err);
+    HIPCHK(c, hipGetLastError());
+  }
+  if (!(c->prm.flags & ARM_AV_VEL) && (rc = zero_av3<R>(c))) return rc;
+  for (int its = its0; its < its0 + nsteps; ++its)
+    if ((rc = fused_step3<R>(c, its))) return rc;
+  c->have_rates = true;
+  HIPCHK(c, hipGetLastError());
+  return check_domain(c);
+}
+
+template <typename R> int pairs3_t(arm_ctx* c, int cap, int* niac, int* pi, int* pj) {
+  St3<R>& A = st3A<R>(c);
+  const int n = c->n;
+  hipLaunchKernelGGL(k3_pair_count<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                     c->skey, c->cstart, c->d_cnt, nullptr, nullptr, nullptr);
+  HIPCHK(c, hipGetLastError());
+  std::vector<int> cnt(n);
+  HIPCHK(c, hipMemcpyAsync(cnt.data(), c->d_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  long long total = 0;
+  std::vector<int> offs(n);
+  for (int i = 0; i < n; ++i) {
+    offs[i] = (int)total;
+    total += cnt[i];
+  }
+  if (total > 0x7fffffffLL) return fail(c, ARM_ERR_CAPACITY, "%lld pairs exceed int range", total);
+  if (niac) *niac = (int)total;
+  if (!pi) return ARM_OK;
+  if (total > cap) return fail(c, ARM_ERR_CAPACITY, "%lld pairs exceed cap %d", total, cap);
+  int *d_off = nullptr, *d_i = nullptr, *d_j = nullptr;
+  HIPCHK(c, hipMalloc(&d_off, sizeof(int) * std::max(n, 1)));
+  HIPCHK(c, hipMalloc(&d_i, sizeof(int) * std::max<long long>(total, 1)));
+  HIPCHK(c, hipMalloc(&d_j, sizeof(int) * std::max<long long>(total, 1)));
+  (void)hipMemcpyAsync(d_off, offs.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream);
+  hipLaunchKernelGGL(k3_pair_count<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                     c->skey, c->cstart, nullptr, d_off, d_i, d_j);
+  std::vector<int> vi(total), vj(total);
+  (void)hipMemcpyAsync(vi.data(), d_i, sizeof(int) * total, hipMemcpyDeviceToHost, c->stream);
+  (void)hipMemcpyAsync(vj.data(), d_j, sizeof(int) * total, hipMemcpyDeviceToHost, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_off);
+  (void)hipFree(d_i);
+  (void)hipFree(d_j);
+  if (e != hipSuccess) return fail(c, ARM_ERR_HIP, "pair emission failed: %s", hipGetErrorString(e));
+  std::vector<unsigned long long> kk(total);
+  for (long long k = 0; k < total; ++k)
+    kk[k] = ((unsigned long long)(unsigned)vi[k] << 32) | (unsigned)vj[k];
+  std::sort(kk.begin(), kk.end());
+  for (long long k = 0; k < total; ++k) {
+    pi[k] = (int)(kk[k] >> 32);
+    pj[k] = (int)(kk[k] & 0xffffffffu);
+  }
+  return ARM_OK;
+}
+
+template <typename R> int upload3_t(arm_ctx* c, const double* x, const double* v,
+                                    const double* rho, const double* u) {
+  using R4 = typename V4<R>::type;
+  const int n = c->n;
+  St3<R>& A = st3A<R>(c);
+  std::vector<int32_t> id(n);
+  std::vector<double4> xyz(n);
+  std::vector<R4> vv(n);
+  std::vector<R> rr(n), pp(n, (R)0);
+  std::vector<double> uu(n);
+  std::vector<float> uf(n);
+  for (int i = 0; i < n; ++i) {
+    id[i] = i;
+    xyz[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], 0.0);
+    vv[i].x = (R)(v ? v[3 * i] : 0.0);
+    vv[i].y = (R)(v ? v[3 * i + 1] : 0.0);
+    vv[i].z = (R)(v ? v[3 * i + 2] : 0.0);
+    vv[i].w = (R)0;
+    const double r = rho ? rho[i] : c->prm.rho0;
+    rr[i] = Layout<R>::f32 ? (R)(r - c->prm.rho0) : (R)r;
+    uu[i] = u ? u[i] : 0.0;
+    uf[i] = (float)uu[i];
+  }
+  auto cp = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+  };
+  HIPCHK(c, cp(A.id, id.data(), sizeof(int32_t) * n));
+  HIPCHK(c, cp(A.xyz, xyz.data(), sizeof(double4) * n));
+  HIPCHK(c, cp(A.v, vv.data(), sizeof(R4) * n));
+  HIPCHK(c, cp(A.rho, rr.data(), sizeof(R) * n));
+  HIPCHK(c, cp(A.p, pp.data(), sizeof(R) * n));
+  HIPCHK(c, cp(A.u, uu.data(), sizeof(double) * n));
+  if (Layout<R>::f32) HIPCHK(c, cp(A.uf, uf.data(), sizeof(float) * n));
+  HIPCHK(c, hipMemsetAsync(A.drho, 0, sizeof(R) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.du, 0, sizeof(R) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.a, 0, sizeof(R4) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.av, 0, sizeof(R4) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.divv, 0, sizeof(R) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.psi, 0, sizeof(R) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.rho_min, 0, sizeof(R) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.u_min, 0, sizeof(double) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.v_min, 0, sizeof(R4) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.flag, 0, n, c->stream));
+  if (Layout<R>::f32) {
+    hipLaunchKernelGGL(k3_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
+                       KeyOut{}, c->d_err);
+    HIPCHK(c, hipGetLastError());
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_domain(c);
+}
+
+template <typename R> int download3_t(arm_ctx* c, double* x, double* v, double* rho, double* p,
+                                      double* u) {
+  using R4 = typename V4<R>::type;
+  const int n = c->n;
+  St3<R>& A = st3A<R>(c);
+  std::vector<int32_t> id;
+  std::vector<double4> xyz;
+  std::vector<R4> vv;
+  std::vector<R> rr, pp;
+  std::vector<double> uu;
+  int rc;
+  if ((rc = d2h(c, id, A.id, n))) return rc;
+  if (x && (rc = d2h(c, xyz, A.xyz, n))) return rc;
+  if (v && (rc = d2h(c, vv, A.v, n))) return rc;
+  if (rho && (rc = d2h(c, rr, A.rho, n))) return rc;
+  if (p && (rc = d2h(c, pp, A.p, n))) return rc;
+  if (u && (rc = d2h(c, uu, A.u, n))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < n; ++s) {
+    const int i = id[s];
+    if (x) {
+      x[3 * i] = xyz[s].x;
+      x[3 * i + 1] = xyz[s].y;
+      x[3 * i + 2] = xyz[s].z;
+    }
+    if (v) {
+      v[3 * i] = (double)vv[s].x;
+      v[3 * i + 1] = (double)vv[s].y;
+      v[3 * i + 2] = (double)vv[s].z;
+    }
+    if (rho) rho[i] = Layout<R>::f32 ? c->prm.rho0 + (double)rr[s] : (double)rr[s];
+    if (p) p[i] = (double)pp[s];
+    if (u) u[i] = uu[s];
+  }
+  return ARM_OK;
+}
+
+template <typename R> int download_rates3_t(arm_ctx* c, double* drho, double* dvdt, double* dudt,
+                                            double* av, double* divv) {
+  using R4 = typename V4<R>::type;
+  const int n = c->n;
+  St3<R>& A = st3A<R>(c);
+  std::vector<int32_t> id;
+  std::vector<R> dr, du, dv;
+  std::vector<R4> aa, ww;
+  int rc;
+  if ((rc = d2h(c, id, A.id, n))) return rc;
+  if (drho && (rc = d2h(c, dr, A.drho, n))) return rc;
+  if (dudt && (rc = d2h(c, du, A.du, n))) return rc;
+  if (divv && (rc = d2h(c, dv, A.divv, n))) return rc;
+  if (dvdt && (rc = d2h(c, aa, A.a, n))) return rc;
+  if (av && (rc = d2h(c, ww, A.av, n))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < n; ++s) {
+    const int i = id[s];
+    if (drho) drho[i] = (double)dr[s];
+    if (dudt) dudt[i] = (double)du[s];
+    if (divv) divv[i] = (double)dv[s];
+    if (dvdt) {
+      dvdt[3 * i] = (double)aa[s].x;
+      dvdt[3 * i + 1] = (double)aa[s].y;
+      dvdt[3 * i + 2] = (double)aa[s].z;
+    }
+    if (av) {
+      av[3 * i] = (double)ww[s].x;
+      av[3 * i + 1] = (double)ww[s].y;
+      av[3 * i + 2] = (double)ww[s].z;
+    }
+  }
+  return ARM_OK;
+}
+
+template <typename R> int conservation3_t(arm_ctx* c, double out[6]) {
+  const int nb = grid_of(c->n);
+  hipLaunchKernelGGL(k3_conserve<R>, dim3(nb), dim3(BLOCK), 0, c->stream, st3A<R>(c), c->K,
+                     c->d_part);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> part((size_t)nb * 6);
+  HIPCHK(c, hipMemcpyAsync(part.data(), c->d_part, sizeof(double) * part.size(),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 6; ++k) out[k] = 0.0;
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 6; ++k) out[k] += part[(size_t)b * 6 + k];
+  return ARM_OK;
+}
+
 #define DISPATCH(c, call) (is32(c) ? call<float> : call<double>)
 
 void set_kparams(KParams& K, const arm_params& P) {
@@ -1056,6 +1441,15 @@ void set_kparams(KParams& K, const arm_params& P) {
   K.hs_x = P.hs_x;
   K.hs_y = P.hs_y;
   K.hs_k = P.hs_sigma > 0 ? 1.0 / (2.0 * P.hs_sigma * P.hs_sigma) : 0.0;
+  K.dim = P.dim == 3 ? 3 : 2;
+  K.nz = K.dim == 3 ? P.nz : 1;
+  K.oz = P.oz;
+  K.gz = P.gz;
+  if (K.dim == 3) {
+    K.ncell = P.nx * P.ny * P.nz;
+    K.wnorm = 21.0 / (16.0 * M_PI * P.h * P.h * P.h);
+    K.C1 = 0.625 * K.wnorm / (P.h * P.h);
+  }
 }
 
 int need_sorted(arm_ctx* c) {
@@ -1085,10 +1479,18 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     rc = fail(c, ARM_ERR_ARG, "h, mass, rho0 and dt must be positive");
   else if (!(P.cell >= 2.0 * P.h))
     rc = fail(c, ARM_ERR_ARG, "cell (%g) must be at least the kernel support 2h (%g)", P.cell, 2 * P.h);
+  else if (P.dim != 0 && P.dim != 2 && P.dim != 3)
+    rc = fail(c, ARM_ERR_ARG, "dim must be 2 or 3");
   else if (P.nx < 1 || P.ny < 1 || (long long)P.nx * P.ny >= (1LL << 30))
     rc = fail(c, ARM_ERR_ARG, "bad grid %d x %d", P.nx, P.ny);
-  else if (P.precision == ARM_FP32 && (P.nx > 65535 || P.ny > 65535))
+  else if (P.dim == 3 && (P.nz < 1 || (long long)P.nx * P.ny * P.nz >= (1LL << 30)))
+    rc = fail(c, ARM_ERR_ARG, "bad grid %d x %d x %d", P.nx, P.ny, P.nz);
+  else if (P.dim != 3 && P.precision == ARM_FP32 && (P.nx > 65535 || P.ny > 65535))
     rc = fail(c, ARM_ERR_ARG, "FP32 form packs cell coordinates in 16 bits");
+  else if (P.dim == 3 && P.precision == ARM_FP32 && (P.nx > 2048 || P.ny > 2048 || P.nz > 1024))
+    rc = fail(c, ARM_ERR_ARG, "3D FP32 form packs cell coordinates in 11 + 11 + 10 bits");
+  else if (P.dim == 3 && (P.flags & (ARM_VISC | ARM_HEAT_SOURCE)))
+    rc = fail(c, ARM_ERR_UNSUPPORTED, "ARM_VISC and ARM_HEAT_SOURCE are implemented in 2D only");
   else if (P.pa_sph != 0 && P.pa_sph != 1)
     rc = fail(c, ARM_ERR_UNSUPPORTED, "pa_sph must be 0 or 1");
   else if (P.density_method != 0 && P.density_method != 1)
@@ -1115,8 +1517,13 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
 
   hipError_t e = hipStreamCreate(&c->stream);
   if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
-  if (!rc) rc = is32(c) ? alloc_state(c, c->A32, capacity) : alloc_state(c, c->A64, capacity);
-  if (!rc) rc = is32(c) ? alloc_state(c, c->B32, capacity) : alloc_state(c, c->B64, capacity);
+  if (is3d(c)) {
+    if (!rc) rc = is32(c) ? alloc_state3(c, c->A3_32, capacity) : alloc_state3(c, c->A3_64, capacity);
+    if (!rc) rc = is32(c) ? alloc_state3(c, c->B3_32, capacity) : alloc_state3(c, c->B3_64, capacity);
+  } else {
+    if (!rc) rc = is32(c) ? alloc_state(c, c->A32, capacity) : alloc_state(c, c->A64, capacity);
+    if (!rc) rc = is32(c) ? alloc_state(c, c->B32, capacity) : alloc_state(c, c->B64, capacity);
+  }
   if (!rc) rc = dalloc(c, &c->key_by_id, capacity);
   if (!rc) rc = dalloc(c, &c->slot_by_id, capacity);
   for (int k = 0; k < 2 && !rc; ++k) {
@@ -1138,7 +1545,7 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
   }
   if (!rc) rc = dalloc(c, &c->cstart, (size_t)ncell + 1);
   if (!rc) rc = dalloc(c, &c->d_err, 1);
-  if (!rc) rc = dalloc(c, &c->d_part, (size_t)grid_of(capacity) * 5);
+  if (!rc) rc = dalloc(c, &c->d_part, (size_t)grid_of(capacity) * 6);
   if (!rc) rc = dalloc(c, &c->d_cnt, capacity);
   if (!rc) rc = dalloc(c, &c->nbm, capacity);
   if (!rc) {
@@ -1184,7 +1591,8 @@ int arm_upload(arm_ctx* c, int np, int nv, const double* x, const double* v, con
   c->masks_valid = false;
   c->have_rates = false;
   c->domain_bad = 0;
-  int rc = DISPATCH(c, upload_t)(c, x, v, rho, nullptr, u);
+  int rc = is3d(c) ? DISPATCH(c, upload3_t)(c, x, v, rho, u)
+                    : DISPATCH(c, upload_t)(c, x, v, rho, nullptr, u);
   c->loaded = (rc == ARM_OK);
   return rc;
 }
@@ -1192,6 +1600,7 @@ int arm_upload(arm_ctx* c, int np, int nv, const double* x, const double* v, con
 int arm_download(arm_ctx* c, double* x, double* v, double* rho, double* p, double* u) {
   if (!c) return ARM_ERR_ARG;
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (is3d(c)) return DISPATCH(c, download3_t)(c, x, v, rho, p, u);
   return DISPATCH(c, download_t)(c, x, v, rho, p, u);
 }
 
@@ -1199,15 +1608,18 @@ int arm_download_rates(arm_ctx* c, double* drho, double* dvdt, double* dudt, dou
                        double* divv) {
   if (!c) return ARM_ERR_ARG;
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (is3d(c)) return DISPATCH(c, download_rates3_t)(c, drho, dvdt, dudt, av, divv);
   return DISPATCH(c, download_rates_t)(c, drho, dvdt, dudt, av, divv);
 }
 
 int arm_link_list(arm_ctx* c, int* niac) {
   if (!c) return ARM_ERR_ARG;
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
-  int rc = DISPATCH(c, link_list_t)(c);
+  int rc = is3d(c) ? DISPATCH(c, link_list3_t)(c) : DISPATCH(c, link_list_t)(c);
   if (rc) return rc;
-  if (niac) rc = DISPATCH(c, pairs_t)(c, 0, niac, nullptr, nullptr);
+  if (niac)
+    rc = is3d(c) ? DISPATCH(c, pairs3_t)(c, 0, niac, nullptr, nullptr)
+                 : DISPATCH(c, pairs_t)(c, 0, niac, nullptr, nullptr);
   return rc;
 }
 
@@ -1215,6 +1627,7 @@ int arm_get_pairs(arm_ctx* c, int cap, int* niac, int* pair_i, int* pair_j) {
   if (!c || !pair_i || !pair_j || cap < 0) return ARM_ERR_ARG;
   int rc = need_sorted(c);
   if (rc) return rc;
+  if (is3d(c)) return DISPATCH(c, pairs3_t)(c, cap, niac, pair_i, pair_j);
   return DISPATCH(c, pairs_t)(c, cap, niac, pair_i, pair_j);
 }
 
@@ -1222,7 +1635,8 @@ int arm_get_cells(arm_ctx* c, int32_t* key_sorted, int32_t* id_sorted, int32_t* 
   if (!c) return ARM_ERR_ARG;
   int rc = need_sorted(c);
   if (rc) return rc;
-  const int32_t* dids = is32(c) ? c->A32.id : c->A64.id;
+  const int32_t* dids = is3d(c) ? (is32(c) ? c->A3_32.id : c->A3_64.id)
+                                : (is32(c) ? c->A32.id : c->A64.id);
   if (key_sorted)
     HIPCHK(c, hipMemcpyAsync(key_sorted, c->skey, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
   if (id_sorted)
@@ -1239,7 +1653,8 @@ int arm_density(arm_ctx* c, int method) {
   if (method != 0 && method != 1) return fail(c, ARM_ERR_UNSUPPORTED, "density method %d", method);
   int rc = need_sorted(c);
   if (rc) return rc;
-  rc = DISPATCH(c, density_t)(c, method, false);
+  rc = is3d(c) ? DISPATCH(c, density3_t)(c, method, false)
+               : DISPATCH(c, density_t)(c, method, false);
   if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
   return rc;
 }
@@ -1248,7 +1663,7 @@ int arm_eos(arm_ctx* c) {
   if (!c) return ARM_ERR_ARG;
   int rc = need_sorted(c);
   if (rc) return rc;
-  rc = DISPATCH(c, eos_t)(c);
+  rc = is3d(c) ? DISPATCH(c, eos3_t)(c) : DISPATCH(c, eos_t)(c);
   if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
   return rc;
 }
@@ -1257,7 +1672,8 @@ int arm_internal_force(arm_ctx* c, int flags) {
   if (!c) return ARM_ERR_ARG;
   int rc = need_sorted(c);
   if (rc) return rc;
-  rc = DISPATCH(c, force_t)(c, true, flags, false);
+  rc = is3d(c) ? DISPATCH(c, force3_t)(c, true, flags, false)
+               : DISPATCH(c, force_t)(c, true, flags, false);
   if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
   return rc;
 }
@@ -1266,7 +1682,8 @@ int arm_av_vel(arm_ctx* c) {
   if (!c) return ARM_ERR_ARG;
   int rc = need_sorted(c);
   if (rc) return rc;
-  rc = DISPATCH(c, force_t)(c, false, 0, true);
+  rc = is3d(c) ? DISPATCH(c, force3_t)(c, false, 0, true)
+               : DISPATCH(c, force_t)(c, false, 0, true);
   if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
   return rc;
 }
@@ -1275,7 +1692,7 @@ int arm_single_step(arm_ctx* c) {
   if (!c) return ARM_ERR_ARG;
   int rc = need_sorted(c);
   if (rc) return rc;
-  rc = DISPATCH(c, single_step_t)(c);
+  rc = is3d(c) ? DISPATCH(c, single_step3_t)(c) : DISPATCH(c, single_step_t)(c);
   if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
   return rc;
 }
@@ -1283,7 +1700,7 @@ int arm_single_step(arm_ctx* c) {
 int arm_predictor(arm_ctx* c) {
   if (!c) return ARM_ERR_ARG;
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
-  int rc = DISPATCH(c, predictor_t)(c);
+  int rc = is3d(c) ? DISPATCH(c, predictor3_t)(c) : DISPATCH(c, predictor_t)(c);
   if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
   return rc;
 }
@@ -1292,11 +1709,17 @@ int arm_corrector(arm_ctx* c, int its) {
   if (!c) return ARM_ERR_ARG;
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
   if (!c->have_rates) return fail(c, ARM_ERR_STATE, "no rates: run arm_single_step first");
-  int rc = DISPATCH(c, corrector_t)(c, its, false);
+  int rc = is3d(c) ? DISPATCH(c, corrector3_t)(c, its, false)
+                    : DISPATCH(c, corrector_t)(c, its, false);
   if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
   // positions moved: the cell order and masks must be rebuilt before the next gather
   c->masks_valid = false;
-  if (!rc && is32(c)) {
+  if (!rc && is32(c) && is3d(c)) {
+    hipLaunchKernelGGL(k3_keys<float>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, c->A3_32,
+                       c->K, KeyOut{}, c->d_err);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else if (!rc && is32(c)) {
     // refresh cell + offset of the FP32 form from the FP64 accumulators
     St<float>& A = c->A32;
     hipLaunchKernelGGL(k_keys<float>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
@@ -1312,7 +1735,8 @@ int arm_time_integration(arm_ctx* c, int its0, int nsteps) {
   if (!c || its0 < 1 || nsteps < 0) return ARM_ERR_ARG;
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
   if (nsteps == 0) return ARM_OK;
-  int rc = DISPATCH(c, time_integration_t)(c, its0, nsteps);
+  int rc = is3d(c) ? DISPATCH(c, time_integration3_t)(c, its0, nsteps)
+                    : DISPATCH(c, time_integration_t)(c, its0, nsteps);
   // the last corrector moved the particles after the last sort
   c->sorted = false;
   return rc;
@@ -1321,7 +1745,34 @@ int arm_time_integration(arm_ctx* c, int its0, int nsteps) {
 int arm_conservation(arm_ctx* c, double out[5]) {
   if (!c || !out) return ARM_ERR_ARG;
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (is3d(c)) {
+    double o6[6];
+    int rc = DISPATCH(c, conservation3_t)(c, o6);
+    if (rc) return rc;
+    out[0] = o6[0];
+    out[1] = o6[1];
+    out[2] = o6[2];
+    out[3] = o6[4];
+    out[4] = o6[5];
+    return ARM_OK;
+  }
   return DISPATCH(c, conservation_t)(c, out);
+}
+
+int arm_conservation3(arm_ctx* c, double out[6]) {
+  if (!c || !out) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (is3d(c)) return DISPATCH(c, conservation3_t)(c, out);
+  double o5[5];
+  int rc = DISPATCH(c, conservation_t)(c, o5);
+  if (rc) return rc;
+  out[0] = o5[0];
+  out[1] = o5[1];
+  out[2] = o5[2];
+  out[3] = 0.0;
+  out[4] = o5[3];
+  out[5] = o5[4];
+  return ARM_OK;
 }
 
 int arm_pass_times(arm_ctx* c, int n, double* ms, int* counts) {
@@ -1350,6 +1801,7 @@ int arm_rollout_forward(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* o
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
   if (c->prm.density_method != 0)
     return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint differentiates continuity density only");
+  if (is3d(c)) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint is implemented in 2D only");
   const int ck = std::min(ckpt_every, nsteps);
   const int rc = is32(c) ? rollout_forward<float>(c, nsteps, ck, obj_ids, nobj, J, ms)
                          : rollout_forward<double>(c, nsteps, ck, obj_ids, nobj, J, ms);
@@ -1378,7 +1830,8 @@ int arm_update_params(arm_ctx* c, const arm_params* prm) {
   const arm_params& O = c->prm;
   if (P.precision != O.precision || P.nx != O.nx || P.ny != O.ny || P.ox != O.ox ||
       P.oy != O.oy || P.cell != O.cell || P.h != O.h || P.flags != O.flags ||
-      P.density_method != O.density_method || P.pa_sph != O.pa_sph || P.eos != O.eos)
+      P.density_method != O.density_method || P.pa_sph != O.pa_sph || P.eos != O.eos ||
+      P.dim != O.dim || (P.dim == 3 && (P.nz != O.nz || P.oz != O.oz)))
     return fail(c, ARM_ERR_ARG, "only the physical scalars can change after arm_create");
   if ((P.flags & ARM_VISC) && !(P.visc_eta >= 0))
     return fail(c, ARM_ERR_ARG, "visc_eta must be non-negative");

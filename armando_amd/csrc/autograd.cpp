@@ -124,6 +124,7 @@ int64_t make_rollout(py::dict p, torch::Tensor x, torch::Tensor rho, torch::Tens
   P.hs_x = p.contains("hs_x") ? d("hs_x") : 0.0;
   P.hs_y = p.contains("hs_y") ? d("hs_y") : 0.0;
   P.hs_sigma = p.contains("hs_sigma") ? d("hs_sigma") : 0.0;
+  P.dim = 2;  // the adjoint rollout is two-dimensional
   r->np = (int)np;
   r->nv = (int)nv;
   r->x = to_vec(x);

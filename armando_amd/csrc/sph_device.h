@@ -37,6 +37,10 @@ struct KParams {
   double lin_p0r;  // lin_p0 + lin_c2 rho0 (FP32 form: p = lin_c2 rho' + lin_p0r)
   double eta;      // dynamic viscosity (flag 32)
   double hs_q0, hs_x, hs_y, hs_k;  // heat source (flag 64): q0 exp(-k |x - c|^2) / rho
+  // three dimensions: key = ix + iy nx + iz nx ny over nz layers from oz;
+  // wnorm and C1 then hold the 3D Wendland constants
+  int dim, nz;
+  double oz, gz;
 };
 
 // Gaussian heat source factor exp(-k |x - c|^2) at a position.

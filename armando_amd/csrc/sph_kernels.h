@@ -257,8 +257,8 @@ __device__ __forceinline__ void dens_begin(const KParams& K, DensOwn<R>& o) {
 // linear) with the p < pmin clamp, then the art_heat coefficient from the
 // raw divergence sum (divv = sum / rho).  Shared by k_eos and the density
 // pass with the EOS folded in, so both give the same bits.
-template <typename R>
-__device__ __forceinline__ void eos_point(St<R>& S, const KParams& K, int i, bool rate, R rho,
+template <typename R, typename SS>
+__device__ __forceinline__ void eos_point(SS& S, const KParams& K, int i, bool rate, R rho,
                                           R raw_divv) {
   if (rate) {
     R p = K.eos == 1 ? linear_p(K, rho) : tait(K, rho);
@@ -281,8 +281,8 @@ __device__ __forceinline__ void eos_point(St<R>& S, const KParams& K, int i, boo
 // negative of the continuity sum; the EOS divides it by rho.  With EOS the
 // equation of state follows at once (single_step.f:119-186 comes right after
 // the density call).
-template <typename R, bool SUM, bool EOS = false>
-__device__ __forceinline__ void dens_end(St<R>& S, const KParams& K, int i, const DensOwn<R>& o) {
+template <typename R, bool SUM, bool EOS = false, typename SS = St<R>>
+__device__ __forceinline__ void dens_end(SS& S, const KParams& K, int i, const DensOwn<R>& o) {
   const bool rate = is_rate(K, S.id[i]);
   R rho = (R)0;
   if (!SUM) {

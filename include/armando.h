@@ -88,6 +88,16 @@ typedef struct arm_params {
   int eos;         /* ARM_EOS_*; both clamp p < pmin to p = 0, rho = rho0 */
   double eos_p0;   /* ARM_EOS_LINEAR pressure offset */
   double hs_q0, hs_x, hs_y, hs_sigma; /* ARM_HEAT_SOURCE amplitude (W/m^3), centre, width */
+  /* dimension: 2 (or 0) or 3.  With 3 the host arrays are x(3, n), v(3, n),
+     dvdt(3, n), av(3, n); the hash key is ix + iy*nx + iz*nx*ny with
+     iz = (int)((z - oz)/cell) < nz and the 27 neighbour cells of
+     CERNsource/link_list.f:49-99 (dim = 3) / Code/armando.cu:3142-3188;
+     the kernel is Wendland C2 3D, alpha = 21/(16 pi h^3); gz is the body
+     force along z.  ARM_VISC and ARM_HEAT_SOURCE are 2D only.  FP32 form:
+     nx, ny <= 2048 and nz <= 1024 (packed cell). */
+  int dim;
+  double oz, gz;
+  int nz;
 } arm_params;
 
 typedef struct arm_ctx arm_ctx;
@@ -100,14 +110,14 @@ const char* arm_last_error(const arm_ctx* ctx);
 /* Number of particles outside the grid seen by the last failing call. */
 int arm_domain_violations(const arm_ctx* ctx);
 
-/* Upload the state; x, v are x(2, np+nv) (Fortran layout); rho, u have
+/* Upload the state; x, v are x(dim, np+nv) (Fortran layout); rho, u have
    np+nv entries.  Replaces [CERNsource/sph.f:104-108] allocation + input. */
 int arm_upload(arm_ctx* ctx, int np, int nv, const double* x, const double* v,
                const double* rho, const double* u);
 /* Download in the original particle order.  Any pointer may be NULL. */
 int arm_download(arm_ctx* ctx, double* x, double* v, double* rho, double* p, double* u);
 /* Rates of the last single_step, original order; any pointer may be NULL.
-   dvdt and av are (2, n); divv is the velocity divergence of art_heat. */
+   dvdt and av are (dim, n); divv is the velocity divergence of art_heat. */
 int arm_download_rates(arm_ctx* ctx, double* drho, double* dvdt, double* dudt, double* av,
                        double* divv);
 
@@ -192,6 +202,9 @@ int arm_rollout_param_grads(arm_ctx* ctx, double* out, int n);
 /* FP64 sums over real particles: mass, momentum x, momentum y, kinetic
    energy, internal energy. */
 int arm_conservation(arm_ctx* ctx, double out[5]);
+/* The same in either dimension with the z momentum: mass, momentum x, y, z,
+   kinetic energy, internal energy (momentum z = 0 in 2D). */
+int arm_conservation3(arm_ctx* ctx, double out[6]);
 
 /* Per-pass device time accumulated since the last reset, in ms, measured
    with HIP events on the context stream; counts[] = launches per pass.
