@@ -285,36 +285,73 @@ __device__ __forceinline__ void dens3_pair(const KParams& K, R dx, R dy, R dz, R
   }
 }
 
+// Neighbour masks recorded by the density pass for the force pass: for row k
+// of particle i, bit b of word q (q < M3_WORDS) says whether the candidate at
+// sorted position t0_k + 32 q + b is a neighbour, t0_k the first position of
+// the row's three cells.  Structure of arrays: word k * M3_WORDS + q of
+// sorted particle i at w[(k * M3_WORDS + q) * stride + i]; word M3_OK is 1
+// when every row had at most 32 M3_WORDS candidates (complete masks).
+constexpr int M3_WORDS = 3;
+constexpr int M3_OK = 9 * M3_WORDS;
+constexpr int M3_TOTAL = M3_OK + 1;
+
+struct Mask3 {
+  uint32_t* w;
+  int stride;
+  __device__ __forceinline__ void put(int q, int i, uint32_t m) const {
+    w[(size_t)q * stride + i] = m;
+  }
+  __device__ __forceinline__ uint32_t get(int q, int i) const {
+    return w[(size_t)q * stride + i];
+  }
+};
+
 template <typename R, bool SUM, bool EOS>
 __device__ __forceinline__ void density3_direct(St3<R>& S, const KParams& K, int i,
                                                 const uint32_t* __restrict__ skey,
-                                                const int* __restrict__ cstart) {
+                                                const int* __restrict__ cstart, Mask3 nbm) {
   const auto pi = load_pos3(S, i);
   const auto vi = S.v[i];
   const Rows9 rw = rows3_of(K, skey[i], cstart);
   const R cut2 = cutoff2<R>(K);
   DensOwn<R> o;
   dens_begin<R, SUM>(K, o);
-  for (int k = 0; k < 9; ++k)
-    for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
+  bool ok = true;
+  for (int k = 0; k < 9; ++k) {
+    const int t0 = rw.t0[k], t1 = rw.t1[k];
+    if (t1 - t0 > 32 * M3_WORDS) ok = false;
+    uint32_t m0 = 0, m1 = 0, m2 = 0;
+    for (int t = t0; t < t1; ++t) {
       if (t == i) continue;
       const auto pj = load_pos3(S, t);
       R dx, dy, dz;
       const R r2 = delta3(pi, pj, K, dx, dy, dz);
       if (!(r2 < cut2)) continue;
+      const int b = t - t0;
+      const uint32_t bit = 1u << (b & 31);
+      m0 |= b < 32 ? bit : 0u;
+      m1 |= (b >= 32 && b < 64) ? bit : 0u;
+      m2 |= (b >= 64 && b < 96) ? bit : 0u;
       const auto vj = S.v[t];
       dens3_pair<R, SUM>(K, dx, dy, dz, r2, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, o.sum, o.rs);
     }
+    if (nbm.w) {
+      nbm.put(k * M3_WORDS + 0, i, m0);
+      nbm.put(k * M3_WORDS + 1, i, m1);
+      nbm.put(k * M3_WORDS + 2, i, m2);
+    }
+  }
+  if (nbm.w) nbm.put(M3_OK, i, ok ? 1u : 0u);
   dens_end<R, SUM, EOS>(S, K, i, o);
 }
 
 template <typename R, bool SUM, bool EOS>
 __global__ __launch_bounds__(BLOCK) void k3_density(St3<R> S, KParams K,
                                                     const uint32_t* __restrict__ skey,
-                                                    const int* __restrict__ cstart) {
+                                                    const int* __restrict__ cstart, Mask3 nbm) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
-  density3_direct<R, SUM, EOS>(S, K, i, skey, cstart);
+  density3_direct<R, SUM, EOS>(S, K, i, skey, cstart, nbm);
 }
 
 template <typename R>
@@ -421,31 +458,66 @@ template <typename R> __device__ __forceinline__ R u3_of(const St3<R>& S, int t)
   if constexpr (sizeof(R) == 4) return S.uf[t]; else return (R)S.u[t];
 }
 
+// One neighbour t of particle i in the direct path.
+template <typename R, bool PRESS, bool AV, bool AH, bool XS, typename P3>
+__device__ __forceinline__ void force3_visit(const St3<R>& S, const KParams& K, const P3& pi,
+                                             ForceOwn3<R>& o, int t) {
+  const auto pj = load_pos3(S, t);
+  R dx, dy, dz;
+  const R r2 = delta3(pi, pj, K, dx, dy, dz);
+  const auto vj = S.v[t];
+  R Qj, Aj;
+  q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
+  force3_pair<R, PRESS, AV, AH, XS>(K, o, dx, dy, dz, r2, vj.x, vj.y, vj.z, Qj, Aj, S.psi[t],
+                                    u3_of(S, t));
+}
+
 template <typename R, bool PRESS, bool AV, bool AH, bool XS>
-__global__ __launch_bounds__(BLOCK) void k3_force(St3<R> S, KParams K,
-                                                  const uint32_t* __restrict__ skey,
-                                                  const int* __restrict__ cstart) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= K.n) return;
+__device__ __forceinline__ void force3_direct(St3<R>& S, const KParams& K, int i,
+                                              const uint32_t* __restrict__ skey,
+                                              const int* __restrict__ cstart, Mask3 nbm) {
   const auto pi = load_pos3(S, i);
   ForceOwn3<R> o;
   force3_begin(S, K, i, o);
   const Rows9 rw = rows3_of(K, skey[i], cstart);
-  const R cut2 = cutoff2<R>(K);
-  for (int k = 0; k < 9; ++k)
-    for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
-      if (t == i) continue;
-      const auto pj = load_pos3(S, t);
-      R dx, dy, dz;
-      const R r2 = delta3(pi, pj, K, dx, dy, dz);
-      if (!(r2 < cut2)) continue;
-      const auto vj = S.v[t];
-      R Qj, Aj;
-      q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
-      force3_pair<R, PRESS, AV, AH, XS>(K, o, dx, dy, dz, r2, vj.x, vj.y, vj.z, Qj, Aj, S.psi[t],
-                                        u3_of(S, t));
-    }
+  if (nbm.w && nbm.get(M3_OK, i)) {
+    uint32_t mw[M3_OK];
+#pragma unroll
+    for (int q = 0; q < M3_OK; ++q) mw[q] = nbm.get(q, i);  // all loads in flight at once
+#pragma unroll 1
+    for (int k = 0; k < 9; ++k)
+#pragma unroll 1
+      for (int q = 0; q < M3_WORDS; ++q) {
+        uint32_t m = mw[k * M3_WORDS + q];
+        const int tq = rw.t0[k] + 32 * q;
+        while (m) {
+          const int b = __builtin_ctz(m);
+          m &= m - 1;
+          force3_visit<R, PRESS, AV, AH, XS>(S, K, pi, o, tq + b);
+        }
+      }
+  } else {
+    const R cut2 = cutoff2<R>(K);
+    for (int k = 0; k < 9; ++k)
+      for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
+        if (t == i) continue;
+        const auto pj = load_pos3(S, t);
+        R dx, dy, dz;
+        const R r2 = delta3(pi, pj, K, dx, dy, dz);
+        if (!(r2 < cut2)) continue;
+        force3_visit<R, PRESS, AV, AH, XS>(S, K, pi, o, t);
+      }
+  }
   force3_end<R, PRESS, XS>(S, K, i, o);
+}
+
+template <typename R, bool PRESS, bool AV, bool AH, bool XS>
+__global__ __launch_bounds__(BLOCK) void k3_force(St3<R> S, KParams K,
+                                                  const uint32_t* __restrict__ skey,
+                                                  const int* __restrict__ cstart, Mask3 nbm) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= K.n) return;
+  force3_direct<R, PRESS, AV, AH, XS>(S, K, i, skey, cstart, nbm);
 }
 
 // ------------------------------------------------------------------------

@@ -224,9 +224,10 @@ int arm_pass_times(arm_ctx* ctx, int n, double* ms, int* counts);
 int arm_reset_pass_times(arm_ctx* ctx);
 /* Enable (1) or disable (0) per-pass event timing. */
 int arm_set_timing(arm_ctx* ctx, int on);
-/* Select the gather implementation: 1 (default) LDS-tiled cell segments,
-   0 direct per-particle gathers from global memory.  Both visit neighbours
-   in the same order with the same arithmetic and give identical results. */
+/* Select the gather implementation: 1 LDS-tiled cell segments, 0 direct
+   per-particle gathers from global memory.  Both visit neighbours in the
+   same order with the same arithmetic and give identical results.  The
+   default is 1 in 2D and 0 in 3D (faster there). */
 int arm_set_gather_path(arm_ctx* ctx, int tiled);
 
 #ifdef __cplusplus
