@@ -22,7 +22,6 @@
 #include "sph_tiled.h"
 #include "sph_adjoint.h"
 #include "sph3d.h"
-#include "sph3d_tiled.h"
 
 using namespace arm;
 
@@ -59,10 +58,8 @@ struct arm_ctx {
   St3<double> A3_64{}, B3_64{};
   St3<float> A3_32{}, B3_32{};
   uint32_t* nbm3 = nullptr;  // 3D neighbour masks (M3_TOTAL words per particle)
-  int* seg_list = nullptr;   // 3D tiled gathers: segments holding particles
-  int* seg_count = nullptr;
-  bool segs_valid = false;   // seg_list matches the current cell ranges
-  int ncu = 0;               // compute units of the device
+  void* pack3_vq = nullptr;  // 3D force pass: packed neighbour data (Pack3)
+  void* pack3_ap = nullptr;
   // sort workspace
   uint32_t *key_by_id = nullptr, *slot_by_id = nullptr;
   uint32_t *sk[2] = {nullptr, nullptr}, *sv[2] = {nullptr, nullptr};
@@ -1084,54 +1081,15 @@ template <typename R> int link_list3_t(arm_ctx* c) {
   std::swap(A, B);
   c->sorted = true;
   c->masks_valid = false;
-  c->segs_valid = false;
   return check_domain(c);
 }
 
-// Segments of the 3D tiled gathers (CW cells of one x-row each).
-template <typename R> int seg3_count(const arm_ctx* c, int* nseg) {
-  *nseg = (c->K.nx + Tile3Cfg<R>::CW - 1) / Tile3Cfg<R>::CW;
-  return c->K.ny * c->K.nz * *nseg;
-}
-
-// Lists the segments that hold particles (once per sort).
-template <typename R> void list_segments3(arm_ctx* c) {
-  if (c->segs_valid) return;
-  int nseg;
-  const int nb = seg3_count<R>(c, &nseg);
-  hipLaunchKernelGGL(k_cs_zero, dim3(1), dim3(CS_THREADS), 0, c->stream, c->seg_count, 1);
-  hipLaunchKernelGGL(k3_active_segments, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->K,
-                     c->cstart, nseg, nb, Tile3Cfg<R>::CW, c->seg_list, c->seg_count);
-  c->segs_valid = true;
-}
-
-// Persistent grid: the resident workgroups of the kernel, a multiple of the
-// XCD count.
-template <typename KernelT> int persistent_grid(const arm_ctx* c, KernelT kernel, int blk) {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, blk, 0) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  const int g = std::max(c->ncu, 1) * per_cu;
-  return (g + NXCD - 1) / NXCD * NXCD;
-}
-
+// 3D density gather (direct: per-particle walks of the nine rows; an
+// LDS-tiled variant measured twice as slow, the 27-cell halo filling the
+// LDS for few lanes).  Records the neighbour masks for the force pass.
 template <typename R, bool SUM, bool EOS> void launch_density3(arm_ctx* c) {
-  St3<R>& A = st3A<R>(c);
-  const Mask3 m{c->nbm3, c->cap};
-  if (c->tiled) {
-    using T = Tile3Cfg<R>;
-    int nseg;
-    seg3_count<R>(c, &nseg);
-    list_segments3<R>(c);
-    auto kern = k3_density_tiled<R, SUM, EOS, T::BLK, T::CW, T::CAP>;
-    static const int grid = persistent_grid(c, kern, T::BLK);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(T::BLK), 0, c->stream, A, c->K, c->skey, c->cstart,
-                       nseg, c->seg_list, c->seg_count, m);
-  } else {
-    hipLaunchKernelGGL((k3_density<R, SUM, EOS>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
-                       A, c->K, c->skey, c->cstart, m);
-  }
+  hipLaunchKernelGGL((k3_density<R, SUM, EOS>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                     st3A<R>(c), c->K, c->skey, c->cstart, Mask3{c->nbm3, c->cap});
   c->masks_valid = true;
 }
 
@@ -1152,21 +1110,18 @@ template <typename R> int eos3_t(arm_ctx* c) {
   return ARM_OK;
 }
 
+template <typename R> Pack3<R> pack3(arm_ctx* c) {
+  using R4 = typename V4<R>::type;
+  return Pack3<R>{static_cast<R4*>(c->pack3_vq), static_cast<R4*>(c->pack3_ap)};
+}
+
 template <typename R, bool P, bool AV, bool AH, bool XS> void launch_force3(arm_ctx* c) {
   const Mask3 m{c->masks_valid ? c->nbm3 : nullptr, c->cap};
-  if (c->tiled) {
-    using T = Tile3Cfg<R>;
-    int nseg;
-    seg3_count<R>(c, &nseg);
-    list_segments3<R>(c);
-    auto kern = k3_force_tiled<R, P, AV, AH, XS, T::BLK, T::CW, T::CAP>;
-    static const int grid = persistent_grid(c, kern, T::BLK);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(T::BLK), 0, c->stream, st3A<R>(c), c->K, c->skey,
-                       c->cstart, nseg, c->seg_list, c->seg_count, m);
-  } else {
-    hipLaunchKernelGGL((k3_force<R, P, AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
-                       c->stream, st3A<R>(c), c->K, c->skey, c->cstart, m);
-  }
+  const Pack3<R> pk = pack3<R>(c);
+  hipLaunchKernelGGL(k3_force_pack<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                     st3A<R>(c), c->K, pk);
+  hipLaunchKernelGGL((k3_force<R, P, AV, AH, XS>), dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                     st3A<R>(c), pk, c->K, c->skey, c->cstart, m);
 }
 
 template <typename R> int force3_t(arm_ctx* c, bool press, int flags, bool xs) {
@@ -1243,7 +1198,6 @@ template <typename R> int fused_step3(arm_ctx* c, int its) {
                        c->K, c->skey, sval, c->cstart, its != 1 ? 1 : 0, ranges ? 0 : 1);
     std::swap(A, B);
     c->masks_valid = false;
-    c->segs_valid = false;
   }
   if (!xs && (rc = zero_av3<R>(c))) return rc;
   c->sorted = true;
@@ -1547,8 +1501,6 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     rc = fail(c, ARM_ERR_ARG, "bad grid %d x %d x %d", P.nx, P.ny, P.nz);
   else if (P.dim != 3 && P.precision == ARM_FP32 && (P.nx > 65535 || P.ny > 65535))
     rc = fail(c, ARM_ERR_ARG, "FP32 form packs cell coordinates in 16 bits");
-  else if (P.dim == 3 && P.precision == ARM_FP32 && (P.nx > 2048 || P.ny > 2048 || P.nz > 1024))
-    rc = fail(c, ARM_ERR_ARG, "3D FP32 form packs cell coordinates in 11 + 11 + 10 bits");
   else if (P.dim == 3 && (P.flags & (ARM_VISC | ARM_HEAT_SOURCE)))
     rc = fail(c, ARM_ERR_UNSUPPORTED, "ARM_VISC and ARM_HEAT_SOURCE are implemented in 2D only");
   else if (P.pa_sph != 0 && P.pa_sph != 1)
@@ -1566,9 +1518,6 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     return rc;
   }
   set_kparams(c->K, P);
-  // 3D: the direct gathers with neighbour masks measure faster than the
-  // LDS-tiled segments (27 cells of ~18 particles fill the LDS for few lanes)
-  c->tiled = !is3d(c);
 
   c->cap = capacity;
   const int ncell = c->K.ncell;
@@ -1613,15 +1562,12 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
   if (!rc) rc = dalloc(c, &c->nbm, capacity);
   if (!rc && is3d(c)) rc = dalloc(c, &c->nbm3, (size_t)M3_TOTAL * capacity);
   if (!rc && is3d(c)) {
-    int nseg;
-    const int nb = is32(c) ? seg3_count<float>(c, &nseg) : seg3_count<double>(c, &nseg);
-    rc = dalloc(c, &c->seg_list, (size_t)nb);
-    if (!rc) rc = dalloc(c, &c->seg_count, 1);
-    int dev = 0;
-    if (!rc && (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
-                    hipSuccess))
-      rc = fail(c, ARM_ERR_HIP, "cannot query the compute-unit count");
+    const size_t r4 = is32(c) ? sizeof(float4) : sizeof(double4);
+    double4* q = nullptr;
+    rc = dalloc(c, &q, (r4 * capacity + sizeof(double4) - 1) / sizeof(double4));
+    c->pack3_vq = q;
+    if (!rc) rc = dalloc(c, &q, (r4 * capacity + sizeof(double4) - 1) / sizeof(double4));
+    c->pack3_ap = q;
   }
   if (!rc) {
     e = hipMemset(c->d_err, 0, sizeof(int));
@@ -1664,7 +1610,6 @@ int arm_upload(arm_ctx* c, int np, int nv, const double* x, const double* v, con
   c->K.np = np;
   c->sorted = false;
   c->masks_valid = false;
-  c->segs_valid = false;
   c->have_rates = false;
   c->domain_bad = 0;
   int rc = is3d(c) ? DISPATCH(c, upload3_t)(c, x, v, rho, u)

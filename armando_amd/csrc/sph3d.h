@@ -10,10 +10,12 @@
 //     range [cell_start[c-1], cell_start[c+2]) of the sorted arrays;
 //   * Wendland C2 in 3D: KParams.wnorm = 21/(16 pi h^3), C1 = 5 wnorm/(8 h^2)
 //     (grad W = C1 (q-2)^3 (x_i - x_j) as in 2D);
-//   * FP32 form: an integer cell (11 + 11 + 10 bits) and an FP32 in-cell
-//     offset per axis, packed in one float4 so a neighbour's position is one
-//     16-byte load; the difference of two positions per axis is
-//     (off_i - off_j) + (float)(c_i - c_j) * (float)cell.
+//   * FP32 form: an integer cell and an FP32 in-cell offset per axis; the
+//     difference of two positions per axis is
+//     (off_i - off_j) + (float)(c_i - c_j) * (float)cell.  A particle keeps
+//     (off_x, off_y, off_z, (float)cx) in one float4 so that a neighbour's
+//     position is one 16-byte load; the y and z cell differences are those
+//     of the neighbour row being walked.
 // Every lane writes only its own particle: no atomics, bitwise repeatable.
 #pragma once
 
@@ -51,19 +53,11 @@ template <typename R> struct St3 {
   uint8_t* flag;
 };
 
-constexpr int C3_BITS_X = 11, C3_BITS_Y = 11, C3_BITS_Z = 10;
-
-__device__ __forceinline__ float pack_cell3(int ix, int iy, int iz) {
-  return __uint_as_float((uint32_t)ix | ((uint32_t)iy << C3_BITS_X) |
-                         ((uint32_t)iz << (C3_BITS_X + C3_BITS_Y)));
-}
-
 struct Pos3D {
   double x, y, z;
 };
 struct Pos3F {
-  int cx, cy, cz;
-  float ox, oy, oz;
+  float cxf, ox, oy, oz;  // cell column as float, in-cell offsets
 };
 
 __device__ __forceinline__ Pos3D load_pos3(const St3<double>& S, int t) {
@@ -72,15 +66,23 @@ __device__ __forceinline__ Pos3D load_pos3(const St3<double>& S, int t) {
 }
 __device__ __forceinline__ Pos3F load_pos3(const St3<float>& S, int t) {
   const float4 q = S.pf[t];
-  const uint32_t c = __float_as_uint(q.w);
-  return {int(c & ((1u << C3_BITS_X) - 1u)), int((c >> C3_BITS_X) & ((1u << C3_BITS_Y) - 1u)),
-          int(c >> (C3_BITS_X + C3_BITS_Y)), q.x, q.y, q.z};
+  return {q.w, q.x, q.y, q.z};
 }
 
+// The cell-difference term (float)(c_i - c_j) * cell of one axis (FP32
+// form); in the row walk the y and z differences are the constants of the
+// row, 1 - k % 3 and 1 - k / 3 for row k.  Unused in the FP64 form.
+template <typename R> __device__ __forceinline__ R row_u3(const KParams& K, int d) {
+  return (R)((float)d * K.cellf);
+}
+template <> __device__ __forceinline__ double row_u3<double>(const KParams&, int) { return 0.0; }
+
 // x_i - x_j and r^2 = (dx^2 + dy^2) + dz^2, rounded step by step (no
-// contraction in this module), as the oracle restates them.
-__device__ __forceinline__ double delta3(const Pos3D& a, const Pos3D& b, const KParams&,
-                                         double& dx, double& dy, double& dz) {
+// contraction in this module), as the oracle restates them; uy, uz are the
+// row's cell terms.  The x cell columns differ by at most one, exactly
+// representable, so (cxf_i - cxf_j) is the integer difference.
+__device__ __forceinline__ double delta3(const Pos3D& a, const Pos3D& b, const KParams&, double,
+                                         double, double& dx, double& dy, double& dz) {
   dx = a.x - b.x;
   dy = a.y - b.y;
   dz = a.z - b.z;
@@ -89,10 +91,10 @@ __device__ __forceinline__ double delta3(const Pos3D& a, const Pos3D& b, const K
   return r2 + dz * dz;
 }
 __device__ __forceinline__ float delta3(const Pos3F& a, const Pos3F& b, const KParams& K,
-                                        float& dx, float& dy, float& dz) {
-  dx = (a.ox - b.ox) + (float)(a.cx - b.cx) * K.cellf;
-  dy = (a.oy - b.oy) + (float)(a.cy - b.cy) * K.cellf;
-  dz = (a.oz - b.oz) + (float)(a.cz - b.cz) * K.cellf;
+                                        float uy, float uz, float& dx, float& dy, float& dz) {
+  dx = (a.ox - b.ox) + (a.cxf - b.cxf) * K.cellf;
+  dy = (a.oy - b.oy) + uy;
+  dz = (a.oz - b.oz) + uz;
   float r2 = dx * dx;
   r2 = r2 + dy * dy;
   return r2 + dz * dz;
@@ -125,8 +127,7 @@ __device__ __forceinline__ float4 split3(const KParams& K, double4 q) {
   const double bx = K.ox + (double)ix * K.cell;
   const double by = K.oy + (double)iy * K.cell;
   const double bz = K.oz + (double)iz * K.cell;
-  return make_float4((float)(q.x - bx), (float)(q.y - by), (float)(q.z - bz),
-                     pack_cell3(ix, iy, iz));
+  return make_float4((float)(q.x - bx), (float)(q.y - by), (float)(q.z - bz), (float)ix);
 }
 
 template <typename R>
@@ -319,13 +320,14 @@ __device__ __forceinline__ void density3_direct(St3<R>& S, const KParams& K, int
   bool ok = true;
   for (int k = 0; k < 9; ++k) {
     const int t0 = rw.t0[k], t1 = rw.t1[k];
+    const R uy = row_u3<R>(K, 1 - k % 3), uz = row_u3<R>(K, 1 - k / 3);
     if (t1 - t0 > 32 * M3_WORDS) ok = false;
     uint32_t m0 = 0, m1 = 0, m2 = 0;
     for (int t = t0; t < t1; ++t) {
       if (t == i) continue;
       const auto pj = load_pos3(S, t);
       R dx, dy, dz;
-      const R r2 = delta3(pi, pj, K, dx, dy, dz);
+      const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
       if (!(r2 < cut2)) continue;
       const int b = t - t0;
       const uint32_t bit = 1u << (b & 31);
@@ -458,23 +460,53 @@ template <typename R> __device__ __forceinline__ R u3_of(const St3<R>& S, int t)
   if constexpr (sizeof(R) == 4) return S.uf[t]; else return (R)S.u[t];
 }
 
+// Neighbour data of the force pass packed per particle, so that a
+// neighbour is three 16-byte (FP32) loads: the position record of the state
+// plus vq = (v, Q = 1/rho) and ap = (A = p/rho, psi, u, 0), written by
+// k3_force_pack right before the gather with q_and_a's arithmetic.
+template <typename R> struct Pack3 {
+  using R4 = typename V4<R>::type;
+  R4* vq;
+  R4* ap;
+};
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k3_force_pack(St3<R> S, KParams K, Pack3<R> P) {
+  using R4 = typename V4<R>::type;
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= K.n) return;
+  R Q, A;
+  q_and_a(K, S.rho[i], S.p[i], Q, A);
+  const R4 v = S.v[i];
+  R4 vq, ap;
+  vq.x = v.x;
+  vq.y = v.y;
+  vq.z = v.z;
+  vq.w = Q;
+  ap.x = A;
+  ap.y = S.psi[i];
+  ap.z = u3_of(S, i);
+  ap.w = (R)0;
+  P.vq[i] = vq;
+  P.ap[i] = ap;
+}
+
 // One neighbour t of particle i in the direct path.
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, typename P3>
-__device__ __forceinline__ void force3_visit(const St3<R>& S, const KParams& K, const P3& pi,
-                                             ForceOwn3<R>& o, int t) {
+__device__ __forceinline__ void force3_visit(const St3<R>& S, const Pack3<R>& P, const KParams& K,
+                                             const P3& pi, R uy, R uz, ForceOwn3<R>& o, int t) {
   const auto pj = load_pos3(S, t);
   R dx, dy, dz;
-  const R r2 = delta3(pi, pj, K, dx, dy, dz);
-  const auto vj = S.v[t];
-  R Qj, Aj;
-  q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
-  force3_pair<R, PRESS, AV, AH, XS>(K, o, dx, dy, dz, r2, vj.x, vj.y, vj.z, Qj, Aj, S.psi[t],
-                                    u3_of(S, t));
+  const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
+  const auto vq = P.vq[t];
+  const auto ap = P.ap[t];
+  force3_pair<R, PRESS, AV, AH, XS>(K, o, dx, dy, dz, r2, vq.x, vq.y, vq.z, vq.w, ap.x, ap.y,
+                                    ap.z);
 }
 
 template <typename R, bool PRESS, bool AV, bool AH, bool XS>
-__device__ __forceinline__ void force3_direct(St3<R>& S, const KParams& K, int i,
-                                              const uint32_t* __restrict__ skey,
+__device__ __forceinline__ void force3_direct(St3<R>& S, const Pack3<R>& P, const KParams& K,
+                                              int i, const uint32_t* __restrict__ skey,
                                               const int* __restrict__ cstart, Mask3 nbm) {
   const auto pi = load_pos3(S, i);
   ForceOwn3<R> o;
@@ -485,7 +517,8 @@ __device__ __forceinline__ void force3_direct(St3<R>& S, const KParams& K, int i
 #pragma unroll
     for (int q = 0; q < M3_OK; ++q) mw[q] = nbm.get(q, i);  // all loads in flight at once
 #pragma unroll 1
-    for (int k = 0; k < 9; ++k)
+    for (int k = 0; k < 9; ++k) {
+      const R uy = row_u3<R>(K, 1 - k % 3), uz = row_u3<R>(K, 1 - k / 3);
 #pragma unroll 1
       for (int q = 0; q < M3_WORDS; ++q) {
         uint32_t m = mw[k * M3_WORDS + q];
@@ -493,31 +526,34 @@ __device__ __forceinline__ void force3_direct(St3<R>& S, const KParams& K, int i
         while (m) {
           const int b = __builtin_ctz(m);
           m &= m - 1;
-          force3_visit<R, PRESS, AV, AH, XS>(S, K, pi, o, tq + b);
+          force3_visit<R, PRESS, AV, AH, XS>(S, P, K, pi, uy, uz, o, tq + b);
         }
       }
+    }
   } else {
     const R cut2 = cutoff2<R>(K);
-    for (int k = 0; k < 9; ++k)
+    for (int k = 0; k < 9; ++k) {
+      const R uy = row_u3<R>(K, 1 - k % 3), uz = row_u3<R>(K, 1 - k / 3);
       for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
         if (t == i) continue;
         const auto pj = load_pos3(S, t);
         R dx, dy, dz;
-        const R r2 = delta3(pi, pj, K, dx, dy, dz);
+        const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
         if (!(r2 < cut2)) continue;
-        force3_visit<R, PRESS, AV, AH, XS>(S, K, pi, o, t);
+        force3_visit<R, PRESS, AV, AH, XS>(S, P, K, pi, uy, uz, o, t);
       }
+    }
   }
   force3_end<R, PRESS, XS>(S, K, i, o);
 }
 
 template <typename R, bool PRESS, bool AV, bool AH, bool XS>
-__global__ __launch_bounds__(BLOCK) void k3_force(St3<R> S, KParams K,
+__global__ __launch_bounds__(BLOCK) void k3_force(St3<R> S, Pack3<R> P, KParams K,
                                                   const uint32_t* __restrict__ skey,
                                                   const int* __restrict__ cstart, Mask3 nbm) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
-  force3_direct<R, PRESS, AV, AH, XS>(S, K, i, skey, cstart, nbm);
+  force3_direct<R, PRESS, AV, AH, XS>(S, P, K, i, skey, cstart, nbm);
 }
 
 // ------------------------------------------------------------------------
@@ -630,13 +666,14 @@ __global__ __launch_bounds__(BLOCK) void k3_pair_count(St3<R> S, KParams K,
   const R cut2 = cutoff2<R>(K);
   int c = 0;
   const int o = offs ? offs[i] : 0;
-  for (int k = 0; k < 9; ++k)
+  for (int k = 0; k < 9; ++k) {
+    const R uy = row_u3<R>(K, 1 - k % 3), uz = row_u3<R>(K, 1 - k / 3);
     for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
       const int idj = S.id[t];
       if (idj <= idi) continue;
       const auto pj = load_pos3(S, t);
       R dx, dy, dz;
-      const R r2 = delta3(pi, pj, K, dx, dy, dz);
+      const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
       if (!(r2 < cut2)) continue;
       if (offs) {
         out_i[o + c] = idi;
@@ -644,6 +681,7 @@ __global__ __launch_bounds__(BLOCK) void k3_pair_count(St3<R> S, KParams K,
       }
       ++c;
     }
+  }
   if (cnt) cnt[i] = c;
 }
 

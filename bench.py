@@ -16,8 +16,12 @@ The adjoint leg reports the adjoint / forward device-time ratio of the
 rollout on the 100k dam break (400 steps, checkpoint every 20), FP32 form
 (forward and backward) first, FP64 beside it.
 
+The 3D leg runs the 3D dam break (0.6 x 0.6 x 1.2 m column in a
+3.2 x 1.0 x 1.6 m tank, dx = 7.1e-3 m, three wall layers: 2,229,862
+particles, Wendland C2 3D, 27-cell hash) in both forms.
+
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp64]
-                  [--no-adjoint]
+                  [--no-adjoint] [--no-3d]
 """
 import argparse
 import json
@@ -44,6 +48,10 @@ BYTES = {
     # density gather: reads skey, pos, v, id; writes drho, divv
     ("density", 1): 4 + 4 + 8 + 8 + 4 + (4 + 4),
     ("density", 0): 4 + 16 + 16 + 4 + (8 + 8),
+    # 3D force gather: reads skey, cell + offset (float4), v (float4), rho', p, psi, u,
+    # id; writes a (float4), du, av (float4)
+    ("force3", 1): 4 + 16 + 16 + 4 + 4 + 4 + 4 + 4 + (16 + 4 + 16),
+    ("force3", 0): 4 + 32 + 32 + 8 + 8 + 8 + 8 + 4 + (32 + 8 + 32),
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -63,6 +71,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 form")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D dam break leg")
+    ap.add_argument("--steps-3d", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work for the baseline sample")
     ap.add_argument("--scale", type=float, default=1.0,
@@ -154,6 +164,63 @@ def adjoint_ratio(steps=400, ckpt=20):
         "ratio": out["fp32"]["ratio"],
         "dtype": "f32",
         **out,
+    }
+
+
+def dam3d_leg(steps, warmup=5):
+    """The 3D dam break in the FP32 chosen and the FP64 plain form: whole
+    steps of time_integration, per-pass device times, FP32-vs-FP64 accuracy."""
+    import armando_amd
+    from armando_amd import cases
+
+    res = {}
+    for name, prec in (("fp32", cases.FP32), ("fp64", cases.FP64)):
+        case = cases.dam_break3d(precision=prec)
+        sim = armando_amd.Simulation(case)
+        niac = sim.link_list(count=True)
+        sim.time_integration(1, warmup)
+        t0 = time.perf_counter()
+        sim.time_integration(1 + warmup, steps)
+        wall = time.perf_counter() - t0
+        sim.set_timing(True)
+        sim.reset_pass_times()
+        sim.time_integration(1 + warmup + steps, 20)
+        passes = {k: ms / max(cnt, 1) for k, (ms, cnt) in sim.pass_times().items()}
+        sim.set_timing(False)
+        res[name] = dict(case=case, niac=niac, wall=wall, passes=passes, state=sim.state())
+        sim.close()
+    r32, r64 = res["fp32"], res["fp64"]
+    case = r32["case"]
+    n = case["n"]
+    t_force = r32["passes"]["force"] * 1e-3
+    achieved = n * BYTES[("force3", 1)] / t_force / 1e9
+    acc = {}
+    for k in ("rho", "p", "vx", "vy", "vz", "x", "y", "z"):
+        d = r32["state"][k] - r64["state"][k]
+        nrm = np.linalg.norm(r64["state"][k])
+        acc[k] = float(np.linalg.norm(d) / nrm) if nrm > 0 else float(np.linalg.norm(d))
+    v32 = n * steps / r32["wall"]
+    v64 = n * steps / r64["wall"]
+    return {
+        "workload": (f"3D dam break, {case['np']} fluid + {case['nv']} wall particles, "
+                     f"dx 7.1e-3 m, grid {case['nx']}x{case['ny']}x{case['nz']}"),
+        "value": v32,
+        "unit": "particle-steps/s",
+        "dtype": "f32",
+        "ms_per_step": 1e3 * r32["wall"] / steps,
+        "steps": steps,
+        "pairs_i_lt_j": r32["niac"],
+        "pair_interactions_per_s": 2 * r32["niac"] / t_force,
+        "passes_ms": r32["passes"],
+        "fp64": {"value": v64, "ms_per_step": 1e3 * r64["wall"] / steps,
+                 "passes_ms": r64["passes"]},
+        "speedup_fp32_over_fp64": v32 / v64,
+        "roofline_force": {"kernel": "k3_force<float,...>", "bound": "hbm",
+                           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS,
+                           "bytes_per_particle": BYTES[("force3", 1)],
+                           "avg_launch_ms": t_force * 1e3},
+        "accuracy_fp32_vs_fp64_rel_l2": {"after_steps": warmup + steps + 20, **acc},
     }
 
 
@@ -269,6 +336,8 @@ def main():
             "after_steps": args.warmup + args.steps + r32["timed_steps"], **acc}
     if not args.no_adjoint:
         out["adjoint"] = adjoint_ratio()
+    if not args.no_3d:
+        out["dam_break_3d"] = dam3d_leg(args.steps_3d)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out))

@@ -93,8 +93,7 @@ typedef struct arm_params {
      iz = (int)((z - oz)/cell) < nz and the 27 neighbour cells of
      CERNsource/link_list.f:49-99 (dim = 3) / Code/armando.cu:3142-3188;
      the kernel is Wendland C2 3D, alpha = 21/(16 pi h^3); gz is the body
-     force along z.  ARM_VISC and ARM_HEAT_SOURCE are 2D only.  FP32 form:
-     nx, ny <= 2048 and nz <= 1024 (packed cell). */
+     force along z.  ARM_VISC and ARM_HEAT_SOURCE are 2D only. */
   int dim;
   double oz, gz;
   int nz;
@@ -224,10 +223,10 @@ int arm_pass_times(arm_ctx* ctx, int n, double* ms, int* counts);
 int arm_reset_pass_times(arm_ctx* ctx);
 /* Enable (1) or disable (0) per-pass event timing. */
 int arm_set_timing(arm_ctx* ctx, int on);
-/* Select the gather implementation: 1 LDS-tiled cell segments, 0 direct
-   per-particle gathers from global memory.  Both visit neighbours in the
-   same order with the same arithmetic and give identical results.  The
-   default is 1 in 2D and 0 in 3D (faster there). */
+/* Select the gather implementation: 1 (default) LDS-tiled cell segments,
+   0 direct per-particle gathers from global memory.  Both visit neighbours
+   in the same order with the same arithmetic and give identical results.
+   In 3D the direct gathers are the only ones (the switch has no effect). */
 int arm_set_gather_path(arm_ctx* ctx, int tiled);
 
 #ifdef __cplusplus

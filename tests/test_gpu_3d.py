@@ -203,50 +203,24 @@ def test_3d_rejects_2d_only_options():
     assert e.value.code == -5
 
 
-@pytest.mark.parametrize("prec", [cases.FP64, cases.FP32])
-def test_tiled3_gather_equals_direct(prec):
-    """LDS-tiled and direct 3D gathers visit the same neighbours in the same
-    order with the same arithmetic: identical bits after several steps."""
-    case = dam(prec)
-    banner(f"3D tiled vs direct prec={prec}")
-    a = armando_amd.Simulation(case)
-    b = armando_amd.Simulation(case)
-    a.set_gather_path(True)
-    b.set_gather_path(False)
-    a.time_integration(1, 8)
-    b.time_integration(1, 8)
-    sa, sb = a.state(), b.state()
-    for k in ("x", "y", "z", "vx", "vy", "vz", "rho", "u", "p"):
-        assert np.array_equal(sa[k], sb[k]), k
-    a.close()
-    b.close()
-
-
-@pytest.mark.parametrize("prec", [cases.FP64, cases.FP32])
-def test_dense3_rows_overflow_masks_and_tiles(prec):
-    """h = 2 dx: rows of more than 96 candidates (masks incomplete, the force
-    pass tests again) and, for the thicker block, segments larger than the
-    LDS tile (the direct path takes over); all paths agree bit for bit, and
-    the FP64 step matches the oracle."""
+def test_dense3_rows_overflow_masks():
+    """h = 2 dx: rows of more than 96 candidates, so the density pass marks
+    the masks incomplete and the force pass tests the candidates again; the
+    FP64 step still matches the oracle and the FP32 form the FP64 form."""
     for shape in ((40, 4, 4), (16, 14, 12)):
-        case = cases.box_lattice3(nx=shape[0], ny=shape[1], nz=shape[2], jitter=0.3, vel=0.4,
-                                  seed=29, h_fac=2.0, precision=prec)
-        banner(f"3D dense rows {shape} prec={prec} n={case['n']}")
-        a = armando_amd.Simulation(case)
-        b = armando_amd.Simulation(case)
-        a.set_gather_path(True)
-        b.set_gather_path(False)
-        for s in (a, b):
-            s.link_list()
-            s.single_step()
-        ga, gb = a.rates(), b.rates()
+        got = {}
+        for prec in (cases.FP64, cases.FP32):
+            case = cases.box_lattice3(nx=shape[0], ny=shape[1], nz=shape[2], jitter=0.3,
+                                      vel=0.4, seed=29, h_fac=2.0, precision=prec)
+            banner(f"3D dense rows {shape} prec={prec} n={case['n']}")
+            sim = armando_amd.Simulation(case)
+            sim.link_list()
+            sim.single_step()
+            got[prec] = sim.rates()
+            sim.close()
+        P = orc.params_from(case)
+        st = oracle_state(case)
+        orc.single_step3(P, st)
         for k in ("drho", "ax", "ay", "az", "du", "avx", "avy", "avz"):
-            assert np.array_equal(ga[k], gb[k]), k
-        if prec == cases.FP64:
-            P = orc.params_from(case)
-            st = oracle_state(case)
-            orc.single_step3(P, st)
-            for k in ("drho", "ax", "ay", "az", "du", "avx", "avy", "avz"):
-                assert relmax(ga[k], st.field(k)) < 1e-10, k
-        a.close()
-        b.close()
+            assert relmax(got[cases.FP64][k], st.field(k)) < 1e-10, k
+            assert relmax(got[cases.FP32][k], st.field(k)) < 1e-3, k
