@@ -17,6 +17,7 @@ import numpy as np
 
 # flags (include/armando.h)
 ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
+CONDUCTION, PERIODIC_X = 128, 256
 EOS_TAIT, EOS_LINEAR = 0, 1
 FP64, FP32 = 0, 1
 
@@ -240,6 +241,66 @@ def with_heat_source(case, q0, x, y, sigma):
     return c
 
 
+def heated_channel(length=0.4, width=0.05, dx=1e-4, layers=3, umax=0.5, rho0=1.0e4,
+                   cond_k=15.0, cp=145.0, mu=1.8e-3, T0=300.0, sigma=5e-3, seed=2024,
+                   h_fac=1.3, precision=FP32, cfl=0.2):
+    """2D heated channel: a fluid layer of `width` between two no-slip wall
+    blocks of `layers` particle rows, periodic in x with period `length`.
+
+    Fluid on the lattice ((i+1/2) dx, (j+1/2) dx), i < length/dx,
+    j < width/dx, no jitter; walls at y = -(l+1/2) dx and width + (l+1/2) dx.
+    Poiseuille flow u(y) = umax (1 - ((y - width/2)/(width/2))^2) held by the
+    body force gx = 2 (mu/rho0) umax / (width/2)^2 against the physical
+    viscosity mu (so the start is the steady profile); Tait EOS with
+    c0 = 10 umax; Cleary-Monaghan conduction (k, cp) plus artificial heat;
+    u = cp T0 initially; a Gaussian heat source of width sigma at the centre
+    of the channel with amplitude Q0 = 1e8 + 9e8 U, U = uniform01(seed)[0]
+    (W/m^3); dt = cfl h / c0.  The periodic grid has nx = floor(length/2h)
+    cells of size length/nx >= 2h."""
+    ni = int(round(length / dx))
+    nj = int(round(width / dx))
+    L = ni * dx
+    I, J = np.meshgrid(np.arange(ni), np.arange(nj))
+    x = (I.ravel() + 0.5) * dx
+    y = (J.ravel() + 0.5) * dx
+    H = 0.5 * nj * dx
+    vx = umax * (1.0 - ((y - H) / H) ** 2)
+    wx, wy = [], []
+    for lyr in range(layers):
+        xs = (np.arange(ni) + 0.5) * dx
+        wx += [xs, xs]
+        wy += [np.full(ni, -(lyr + 0.5) * dx), np.full(ni, nj * dx + (lyr + 0.5) * dx)]
+    X = np.concatenate([x] + wx)
+    Y = np.concatenate([y] + wy)
+    n, nf = X.size, x.size
+    h = h_fac * dx
+    nxg = int(math.floor(L / (2.0 * h)))
+    cell = L / nxg
+    oy = float(np.min(Y)) - 4 * cell
+    nyg = int(math.ceil((float(np.max(Y)) - oy) / cell)) + 5
+    c0 = 10.0 * umax
+    gx = 2.0 * (mu / rho0) * umax / (H * H)
+    q0 = 1e8 + 9e8 * float(uniform01(seed, 1)[0])
+    flags = (DBC | GRAVITY | ART_HEAT | VISC | HEAT_SOURCE | CONDUCTION | PERIODIC_X)
+    return dict(
+        name="heated_channel", n=n, np=nf, nv=n - nf,
+        x=X, y=Y, vx=np.concatenate([vx, np.zeros(n - nf)]), vy=np.zeros(n),
+        rho=np.full(n, rho0), u=np.full(n, cp * T0),
+        precision=precision, flags=flags, pa_sph=1, density_method=0,
+        kernel=0, h=h, mass=rho0 * dx * dx, rho0=rho0, c0=c0, pmin=-1e30,
+        av_alpha=0.0, av_beta=0.0, av_eta=0.1, ah_g1=0.1, ah_g2=1.0, ah_eta=0.1,
+        xsph_eps=0.0, gx=gx, gy=0.0, dt=cfl * h / c0,
+        ox=0.0, oy=oy, cell=cell, nx=nxg, ny=nyg,
+        visc_eta=mu, hs_q0=q0, hs_x=0.5 * L, hs_y=H, hs_sigma=sigma,
+        cond_k=cond_k, cp=cp,
+    )
+
+
+def heated_channel_small(length=0.02, width=0.005, dx=1e-4, precision=FP64, **kw):
+    """The heated channel at a reduced size (same dx and physics)."""
+    return heated_channel(length=length, width=width, dx=dx, precision=precision, **kw)
+
+
 # ------------------------------------------------------------------ 3D
 def grid_for3(x, y, z, cell, margin_cells=4, top_extra=0.0):
     """3D hash grid covering the particles with a margin (cell = 2h)."""
@@ -352,4 +413,4 @@ def dam_break3d_small(scale=0.1, dx=7.1e-3, precision=FP64, seed=4242, jitter=0.
 PARAM_KEYS = ("precision", "flags", "pa_sph", "density_method", "h", "mass", "rho0", "c0",
               "pmin", "av_alpha", "av_beta", "av_eta", "ah_g1", "ah_g2", "ah_eta", "xsph_eps",
               "gx", "gy", "dt", "ox", "oy", "cell", "nx", "ny", "visc_eta", "eos", "eos_p0",
-              "hs_q0", "hs_x", "hs_y", "hs_sigma", "dim", "oz", "gz", "nz")
+              "hs_q0", "hs_x", "hs_y", "hs_sigma", "dim", "oz", "gz", "nz", "cond_k", "cp")

@@ -543,7 +543,9 @@ static int nrate_of(const orc_params* P, const orc_state* s) {
 
 int orc_single_step(const orc_params* P, orc_state* s, int niac, const int* pi,
                     const int* pj) {
-  /* single_step.f:92-234 */
+  /* single_step.f:92-234 (a pair list carries no periodic shift, and the
+     reference has no conduction: both live in the gather twin only) */
+  if (P->flags & (ORC_CONDUCTION | ORC_PERIODIC_X)) return -2;
   const int n = s->n, np = s->np, nr = nrate_of(P, s);
   double* avdv_x = (double*)calloc((size_t)n, sizeof(double));
   double* avdv_y = (double*)calloc((size_t)n, sizeof(double));
@@ -613,6 +615,48 @@ static void free_cells(cells_t* C) {
   free(C->start);
 }
 
+/* The cells cx-1, cx, cx+1 of cell row ry, in that order, as ranges of the
+   sorted order with their x shift in periods: one range, or two when the
+   grid is periodic in x and cx is the first or last column (the wrapped
+   cell nx-1 comes first for cx = 0, the wrapped cell 0 last for
+   cx = nx-1). */
+typedef struct {
+  int t0, t1, sh;
+} span_t;
+
+static int row_spans(const orc_params* P, const int32_t* start, int cx, int ry, span_t sp[2]) {
+  const int nx = P->nx, base = ry * nx;
+  if ((P->flags & ORC_PERIODIC_X) && cx == 0) {
+    sp[0].t0 = start[base + nx - 1];
+    sp[0].t1 = start[base + nx];
+    sp[0].sh = 1;
+    sp[1].t0 = start[base];
+    sp[1].t1 = start[base + 2];
+    sp[1].sh = 0;
+    return 2;
+  }
+  if ((P->flags & ORC_PERIODIC_X) && cx == nx - 1) {
+    sp[0].t0 = start[base + nx - 2];
+    sp[0].t1 = start[base + nx];
+    sp[0].sh = 0;
+    sp[1].t0 = start[base];
+    sp[1].t1 = start[base + 1];
+    sp[1].sh = -1;
+    return 2;
+  }
+  const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < nx ? cx + 1 : nx - 1;
+  sp[0].t0 = start[base + x0];
+  sp[0].t1 = start[base + x1 + 1];
+  sp[0].sh = 0;
+  return 1;
+}
+
+/* x_i - x_j across sh periods: (x_i - x_j) + sh (nx cell). */
+static double dx_of(const orc_params* P, double xi, double xj, int sh) {
+  const double d = xi - xj;
+  return sh ? d + (double)sh * ((double)P->nx * P->cell) : d;
+}
+
 int orc_single_step_gather(const orc_params* P, orc_state* s) {
   const int n = s->n, np = s->np, nr = nrate_of(P, s);
   const double m = P->mass, h = P->h;
@@ -635,15 +679,16 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
   for (int i = 0; i < n; ++i) {
     const int key = C.key[i];
     const int cx = key % nx, cy = key / nx;
-    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < nx ? cx + 1 : nx - 1;
     double dr = 0.0, rs = m * w0;
     for (int ry = cy - 1; ry <= cy + 1; ++ry) {
       if (ry < 0 || ry >= ny) continue;
-      const int t0 = C.start[ry * nx + x0], t1 = C.start[ry * nx + x1 + 1];
-      for (int t = t0; t < t1; ++t) {
+      span_t sp[2];
+      const int ns = row_spans(P, C.start, cx, ry, sp);
+      for (int q = 0; q < ns; ++q)
+      for (int t = sp[q].t0; t < sp[q].t1; ++t) {
         const int j = C.sorted[t];
         if (j == i) continue;
-        const double dx = x[i] - x[j], dy = y[i] - y[j];
+        const double dx = dx_of(P, x[i], x[j], sp[q].sh), dy = y[i] - y[j];
         double r2 = dx * dx;
         r2 = r2 + dy * dy;
         if (!(r2 < cut2)) continue;
@@ -681,15 +726,16 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
     for (int i = 0; i < n; ++i) {
       const int key = C.key[i];
       const int cx = key % nx, cy = key / nx;
-      const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < nx ? cx + 1 : nx - 1;
       double txx = 0.0, tyy = 0.0, txy = 0.0;
       for (int ry = cy - 1; ry <= cy + 1; ++ry) {
         if (ry < 0 || ry >= ny) continue;
-        const int t0 = C.start[ry * nx + x0], t1 = C.start[ry * nx + x1 + 1];
-        for (int t = t0; t < t1; ++t) {
+        span_t sp[2];
+        const int ns = row_spans(P, C.start, cx, ry, sp);
+        for (int q = 0; q < ns; ++q)
+        for (int t = sp[q].t0; t < sp[q].t1; ++t) {
           const int j = C.sorted[t];
           if (j == i) continue;
-          const double dx = x[i] - x[j], dy = y[i] - y[j];
+          const double dx = dx_of(P, x[i], x[j], sp[q].sh), dy = y[i] - y[j];
           double r2 = dx * dx;
           r2 = r2 + dy * dy;
           if (!(r2 < cut2)) continue;
@@ -713,6 +759,7 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
   const int f_av = (P->flags & ORC_ART_VISC) != 0;
   const int f_ah = (P->flags & ORC_ART_HEAT) != 0;
   const int f_xs = (P->flags & ORC_AV_VEL) != 0;
+  const int f_cd = (P->flags & ORC_CONDUCTION) != 0;
   const double* rho = s->rho;
   const double* p = s->p;
   const double* c = s->c;
@@ -723,17 +770,18 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
   for (int i = 0; i < n; ++i) {
     const int key = C.key[i];
     const int cx = key % nx, cy = key / nx;
-    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < nx ? cx + 1 : nx - 1;
     double fx = 0.0, fy = 0.0, due = 0.0, avfx = 0.0, avfy = 0.0, avdu = 0.0, ahdu = 0.0;
-    double xsx = 0.0, xsy = 0.0;
+    double cdu = 0.0, xsx = 0.0, xsy = 0.0;
     const double pr_i = p[i] / (rho[i] * rho[i]);
     for (int ry = cy - 1; ry <= cy + 1; ++ry) {
       if (ry < 0 || ry >= ny) continue;
-      const int t0 = C.start[ry * nx + x0], t1 = C.start[ry * nx + x1 + 1];
-      for (int t = t0; t < t1; ++t) {
+      span_t sp[2];
+      const int ns = row_spans(P, C.start, cx, ry, sp);
+      for (int q = 0; q < ns; ++q)
+      for (int t = sp[q].t0; t < sp[q].t1; ++t) {
         const int j = C.sorted[t];
         if (j == i) continue;
-        const double dx = x[i] - x[j], dy = y[i] - y[j];
+        const double dx = dx_of(P, x[i], x[j], sp[q].sh), dy = y[i] - y[j];
         double r2 = dx * dx;
         r2 = r2 + dy * dy;
         if (!(r2 < cut2)) continue;
@@ -788,6 +836,14 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
           const double ps = (2.0 * pij) / (mrho * (r2 + ah_e2)) * rdw;
           ahdu = ahdu + m * ps * (u[i] - u[j]);
         }
+        if (f_cd) {
+          /* Cleary & Monaghan (1999), uniform k: 4 k k/(k + k) = 2 k */
+          double rdw = dx * dwx;
+          rdw = rdw + dy * dwy;
+          const double kk = 4.0 * P->cond_k * P->cond_k / (P->cond_k + P->cond_k);
+          const double dT = u[i] / P->cp - u[j] / P->cp;
+          cdu = cdu + m * kk * dT / (rho[i] * rho[j]) * rdw / (r2 + ah_e2);
+        }
         if (f_xs) {
           xsx = xsx - P->xsph_eps * m / mrho * dvx * w;
           xsy = xsy - P->xsph_eps * m / mrho * dvy * w;
@@ -803,7 +859,7 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
       const double* ti = tt + 3 * i;
       tds = 0.5 * eta / rho[i] * (ti[0] * ti[0] + ti[1] * ti[1] + 2.0 * ti[2] * ti[2]);
     }
-    s->du[i] = (tds + 0.5 * due) + avdu + ahdu;
+    s->du[i] = (tds + 0.5 * due) + avdu + ahdu + cdu;
     if ((P->flags & ORC_HEAT_SOURCE) && i < np) s->du[i] += heat_source(P, x[i], y[i], rho[i]);
     s->avx[i] = xsx;
     s->avy[i] = xsy;
@@ -835,6 +891,16 @@ void orc_predictor(const orc_params* P, orc_state* s) {
   }
 }
 
+/* Periodic in x: a position that left [ox, ox + nx cell) re-enters on the
+   other side. */
+static double wrap_x(const orc_params* P, double x) {
+  if (!(P->flags & ORC_PERIODIC_X)) return x;
+  const double L = (double)P->nx * P->cell;
+  if (x >= P->ox + L) return x - L;
+  if (x < P->ox) return x + L;
+  return x;
+}
+
 void orc_corrector(const orc_params* P, orc_state* s, int its) {
   /* time_integration.f:186-217 */
   const double dt = P->dt;
@@ -850,6 +916,7 @@ void orc_corrector(const orc_params* P, orc_state* s, int its) {
         s->x[i] = s->x[i] + dt * s->vx[i];
         s->vy[i] = s->vy_min[i] + dt * s->ay[i] + s->avy[i];
         s->y[i] = s->y[i] + dt * s->vy[i];
+        s->x[i] = wrap_x(P, s->x[i]);
       }
     } else {
       s->u[i] = s->u[i] + (dt / 2.0) * s->du[i];
@@ -860,6 +927,7 @@ void orc_corrector(const orc_params* P, orc_state* s, int its) {
         s->x[i] = s->x[i] + dt * s->vx[i];
         s->vy[i] = s->vy[i] + (dt / 2.0) * s->ay[i];
         s->y[i] = s->y[i] + dt * s->vy[i];
+        s->x[i] = wrap_x(P, s->x[i]);
       }
     }
   }
@@ -946,7 +1014,7 @@ static int cmp_int(const void* a, const void* b) {
 }
 
 /* generic lexicographic half list over the fixed grid using a predicate */
-typedef int (*pair_pred)(const void* ctx, int i, int j);
+typedef int (*pair_pred)(const void* ctx, int i, int j, int sh);
 
 static int half_pairs(const orc_params* P, int n, const int32_t* cx, const int32_t* cy,
                       pair_pred pred, const void* ctx, int cap, int* niac, int* pair_i,
@@ -966,11 +1034,13 @@ static int half_pairs(const orc_params* P, int n, const int32_t* cx, const int32
     int nb = 0;
     for (int ry = cy[i] - 1; ry <= cy[i] + 1; ++ry) {
       if (ry < 0 || ry >= ny) continue;
-      const int x0 = cx[i] > 0 ? cx[i] - 1 : 0, x1 = cx[i] + 1 < nx ? cx[i] + 1 : nx - 1;
-      for (int t = start[ry * nx + x0]; t < start[ry * nx + x1 + 1]; ++t) {
-        const int j = sorted[t];
-        if (j > i && pred(ctx, i, j) && nb < 4096) buf[nb++] = j;
-      }
+      span_t sp[2];
+      const int ns = row_spans(P, start, cx[i], ry, sp);
+      for (int q = 0; q < ns; ++q)
+        for (int t = sp[q].t0; t < sp[q].t1; ++t) {
+          const int j = sorted[t];
+          if (j > i && pred(ctx, i, j, sp[q].sh) && nb < 4096) buf[nb++] = j;
+        }
     }
     qsort(buf, (size_t)nb, sizeof(int), cmp_int);
     for (int k = 0; k < nb; ++k) {
@@ -995,12 +1065,13 @@ typedef struct {
   const int32_t *cx, *cy;
   const float *ox, *oy;
   float cellf, cut2f;
+  int nx;
 } f32_ctx;
 
-static int pred_f32(const void* vctx, int i, int j) {
+static int pred_f32(const void* vctx, int i, int j, int sh) {
   const f32_ctx* c = (const f32_ctx*)vctx;
   const float tx = c->ox[i] - c->ox[j];
-  const float ux = (float)(c->cx[i] - c->cx[j]) * c->cellf;
+  const float ux = (float)(c->cx[i] - c->cx[j] + sh * c->nx) * c->cellf;
   const float dx = tx + ux;
   const float ty = c->oy[i] - c->oy[j];
   const float uy = (float)(c->cy[i] - c->cy[j]) * c->cellf;
@@ -1021,17 +1092,19 @@ int orc_pairs_f32(const orc_params* P, int n, const int32_t* cx, const int32_t* 
   c.oy = offy;
   c.cellf = (float)P->cell;
   c.cut2f = (float)((2.0 * P->h) * (2.0 * P->h));
+  c.nx = P->nx;
   return half_pairs(P, n, cx, cy, pred_f32, &c, cap, niac, pair_i, pair_j);
 }
 
 typedef struct {
   const double *x, *y;
   double cut2;
+  const orc_params* P;
 } f64_ctx;
 
-static int pred_f64(const void* vctx, int i, int j) {
+static int pred_f64(const void* vctx, int i, int j, int sh) {
   const f64_ctx* c = (const f64_ctx*)vctx;
-  const double dx = c->x[i] - c->x[j], dy = c->y[i] - c->y[j];
+  const double dx = dx_of(c->P, c->x[i], c->x[j], sh), dy = c->y[i] - c->y[j];
   double r2 = dx * dx;
   r2 = r2 + dy * dy;
   return r2 < c->cut2;
@@ -1050,6 +1123,7 @@ int orc_pairs_f64(const orc_params* P, int n, const double* x, const double* y, 
   c.x = x;
   c.y = y;
   c.cut2 = (2.0 * P->h) * (2.0 * P->h);
+  c.P = P;
   int rc = half_pairs(P, n, cx, cy, pred_f64, &c, cap, niac, pair_i, pair_j);
   free(cx);
   free(cy);

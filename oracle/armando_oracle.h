@@ -68,8 +68,13 @@ enum {
   ORC_GRAVITY = 16,  /* body force on fluid (armando2Dcpu.cu:937-958) */
   ORC_VISC = 32,     /* physical viscosity: shear tensor, viscous force and
                         tds/dt of internal_force.f:73-171, uniform eta */
-  ORC_HEAT_SOURCE = 64 /* du += hs_q0 exp(-|x - c|^2/(2 hs_sigma^2)) / rho on real
+  ORC_HEAT_SOURCE = 64, /* du += hs_q0 exp(-|x - c|^2/(2 hs_sigma^2)) / rho on real
                           particles (the exdudt slot of single_step.f:217-222) */
+  ORC_CONDUCTION = 128, /* Cleary-Monaghan heat conduction, uniform conductivity k:
+                           du_i += sum m 4 k k/(k + k) (T_i - T_j)/(rho_i rho_j)
+                                   (x_ij . grad_i W)/(r^2 + (h ah_eta)^2), T = u/cp */
+  ORC_PERIODIC_X = 256  /* periodic in x with period nx cell from ox: the cells
+                           cx - 1 and cx + 1 wrap, x_i - x_j gains -+ nx cell */
 };
 
 enum {
@@ -101,6 +106,7 @@ typedef struct {
   int dim;         /* 2 (default when 0) or 3: the orc3_* routines */
   double oz, gz;   /* 3D: grid origin in z, body force in z */
   int nz;          /* 3D: grid cells in z (key = ix + iy nx + iz nx ny) */
+  double cond_k, cp; /* ORC_CONDUCTION: conductivity (W/m/K), heat capacity (J/kg/K) */
 } orc_params;
 
 /* Particle state, SoA, FP64. */

@@ -14,6 +14,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libarmref.so")
 
 K_WENDLAND2D, K_CUBIC2D, K_CUBIC3D, K_WENDLAND3D = 0, 1, 2, 3
 ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
+CONDUCTION, PERIODIC_X = 128, 256
 EOS_TAIT, EOS_LINEAR = 0, 1
 
 
@@ -33,6 +34,7 @@ class Params(C.Structure):
         ("hs_q0", C.c_double), ("hs_x", C.c_double), ("hs_y", C.c_double),
         ("hs_sigma", C.c_double),
         ("dim", C.c_int), ("oz", C.c_double), ("gz", C.c_double), ("nz", C.c_int),
+        ("cond_k", C.c_double), ("cp", C.c_double),
     ]
 
 
