@@ -1455,6 +1455,10 @@ void set_kparams(KParams& K, const arm_params& P) {
   K.hs_x = P.hs_x;
   K.hs_y = P.hs_y;
   K.hs_k = P.hs_sigma > 0 ? 1.0 / (2.0 * P.hs_sigma * P.hs_sigma) : 0.0;
+  K.periodic = (P.flags & ARM_PERIODIC_X) ? 1 : 0;
+  K.cond = (P.flags & ARM_CONDUCTION)
+               ? 4.0 * P.cond_k * P.cond_k / (P.cond_k + P.cond_k) / P.cp
+               : 0.0;
   K.dim = P.dim == 3 ? 3 : 2;
   K.nz = K.dim == 3 ? P.nz : 1;
   K.oz = P.oz;
@@ -1501,8 +1505,13 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     rc = fail(c, ARM_ERR_ARG, "bad grid %d x %d x %d", P.nx, P.ny, P.nz);
   else if (P.dim != 3 && P.precision == ARM_FP32 && (P.nx > 65535 || P.ny > 65535))
     rc = fail(c, ARM_ERR_ARG, "FP32 form packs cell coordinates in 16 bits");
-  else if (P.dim == 3 && (P.flags & (ARM_VISC | ARM_HEAT_SOURCE)))
-    rc = fail(c, ARM_ERR_UNSUPPORTED, "ARM_VISC and ARM_HEAT_SOURCE are implemented in 2D only");
+  else if (P.dim == 3 && (P.flags & (ARM_VISC | ARM_HEAT_SOURCE | ARM_CONDUCTION | ARM_PERIODIC_X)))
+    rc = fail(c, ARM_ERR_UNSUPPORTED,
+              "ARM_VISC, ARM_HEAT_SOURCE, ARM_CONDUCTION and ARM_PERIODIC_X are 2D only");
+  else if ((P.flags & ARM_PERIODIC_X) && P.nx < 3)
+    rc = fail(c, ARM_ERR_ARG, "a periodic grid needs nx >= 3 cells");
+  else if ((P.flags & ARM_CONDUCTION) && !(P.cond_k > 0 && P.cp > 0))
+    rc = fail(c, ARM_ERR_ARG, "conduction needs cond_k > 0 and cp > 0");
   else if (P.pa_sph != 0 && P.pa_sph != 1)
     rc = fail(c, ARM_ERR_UNSUPPORTED, "pa_sph must be 0 or 1");
   else if (P.density_method != 0 && P.density_method != 1)
@@ -1823,6 +1832,8 @@ int arm_rollout_forward(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* o
   if (c->prm.density_method != 0)
     return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint differentiates continuity density only");
   if (is3d(c)) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint is implemented in 2D only");
+  if (c->prm.flags & (ARM_CONDUCTION | ARM_PERIODIC_X))
+    return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint has no conduction or periodic boundary");
   const int ck = std::min(ckpt_every, nsteps);
   const int rc = is32(c) ? rollout_forward<float>(c, nsteps, ck, obj_ids, nobj, J, ms)
                          : rollout_forward<double>(c, nsteps, ck, obj_ids, nobj, J, ms);
@@ -1858,6 +1869,8 @@ int arm_update_params(arm_ctx* c, const arm_params* prm) {
     return fail(c, ARM_ERR_ARG, "visc_eta must be non-negative");
   if ((P.flags & ARM_HEAT_SOURCE) && !(P.hs_sigma > 0))
     return fail(c, ARM_ERR_ARG, "hs_sigma must be positive");
+  if ((P.flags & ARM_CONDUCTION) && !(P.cond_k > 0 && P.cp > 0))
+    return fail(c, ARM_ERR_ARG, "conduction needs cond_k > 0 and cp > 0");
   if (!(P.mass > 0) || !(P.rho0 > 0) || !(P.dt > 0))
     return fail(c, ARM_ERR_ARG, "mass, rho0 and dt must be positive");
   c->prm = P;

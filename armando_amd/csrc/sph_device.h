@@ -41,6 +41,8 @@ struct KParams {
   // wnorm and C1 then hold the 3D Wendland constants
   int dim, nz;
   double oz, gz;
+  int periodic;  // flag 256: periodic in x with period nx * cell
+  double cond;   // flag 128: 4 k k/(k + k) / cp, the conduction factor on u_i - u_j
 };
 
 // Gaussian heat source factor exp(-k |x - c|^2) at a position.
@@ -99,17 +101,21 @@ __device__ __forceinline__ PosF load_pos(const St<float>& S, int t) {
   return {int(c & 0xffffu), int(c >> 16), o.x, o.y};
 }
 
-// x_i - x_j.  The module is compiled without floating-point contraction,
-// so these are the exact operation sequences the oracle restates.
-__device__ __forceinline__ void delta(const PosD& a, const PosD& b, const KParams&, double& dx,
-                                      double& dy) {
+// x_i - x_j, with the neighbour sh cells (sh = 0 or -+nx across the seam of
+// a periodic grid) to the right.  The module is compiled without
+// floating-point contraction, so these are the exact operation sequences
+// the oracle restates: (x_i - x_j) + sh cell in FP64, and
+// (off_i - off_j) + (float)(c_i - c_j + sh) cell in the FP32 form.
+__device__ __forceinline__ void delta(const PosD& a, const PosD& b, const KParams& K, double& dx,
+                                      double& dy, int sh = 0) {
   dx = a.x - b.x;
+  if (sh) dx = dx + (double)sh * K.cell;
   dy = a.y - b.y;
 }
 __device__ __forceinline__ void delta(const PosF& a, const PosF& b, const KParams& K, float& dx,
-                                      float& dy) {
+                                      float& dy, int sh = 0) {
   const float tx = a.ox - b.ox;
-  const float ux = (float)(a.cx - b.cx) * K.cellf;
+  const float ux = (float)(a.cx - b.cx + sh) * K.cellf;
   dx = tx + ux;
   const float ty = a.oy - b.oy;
   const float uy = (float)(a.cy - b.cy) * K.cellf;

@@ -195,29 +195,58 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
 // ------------------------------------------------------------------------
 // neighbour walk helpers
 // ------------------------------------------------------------------------
+// Candidate ranges of the three rows.  On a grid periodic in x the first
+// and last columns see the other end: the wrapped cell is a range of its
+// own, visited where its image lies (first for cx = 0, last for cx = nx-1),
+// with the shift wsh = +nx or -nx cells; wsh = 0 otherwise.
 struct Rows {
   int t0[3], t1[3];
+  int w0[3], w1[3];
+  int wsh;
 };
 
 __device__ __forceinline__ Rows rows_of(const KParams& K, uint32_t key,
                                         const int* __restrict__ cstart) {
   const int cy = (int)key / K.nx;
   const int cx = (int)key - cy * K.nx;
-  const int x0 = cx > 0 ? cx - 1 : 0;
-  const int x1 = cx + 1 < K.nx ? cx + 1 : K.nx - 1;
   Rows r;
+  r.wsh = 0;
+  int x0 = cx > 0 ? cx - 1 : 0;
+  int x1 = cx + 1 < K.nx ? cx + 1 : K.nx - 1;
+  int wc = 0;
+  if (K.periodic && cx == 0) {
+    r.wsh = K.nx;
+    wc = K.nx - 1;
+  } else if (K.periodic && cx == K.nx - 1) {
+    r.wsh = -K.nx;
+    wc = 0;
+  }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int ry = cy - 1 + k;
+    r.w0[k] = 0;
+    r.w1[k] = 0;
     if (ry < 0 || ry >= K.ny) {
       r.t0[k] = 0;
       r.t1[k] = 0;
     } else {
       r.t0[k] = cstart[ry * K.nx + x0];
       r.t1[k] = cstart[ry * K.nx + x1 + 1];
+      if (r.wsh) {
+        r.w0[k] = cstart[ry * K.nx + wc];
+        r.w1[k] = cstart[ry * K.nx + wc + 1];
+      }
     }
   }
   return r;
+}
+
+// Part s (0, 1, 2) of row k in visiting order: [a, b) with shift sh.
+__device__ __forceinline__ void row_part(const Rows& r, int k, int s, int& a, int& b, int& sh) {
+  const bool wrapped = (s == 0 && r.wsh > 0) || (s == 2 && r.wsh < 0);
+  a = s == 1 ? r.t0[k] : wrapped ? r.w0[k] : 0;
+  b = s == 1 ? r.t1[k] : wrapped ? r.w1[k] : 0;
+  sh = wrapped ? r.wsh : 0;
 }
 
 // ------------------------------------------------------------------------
@@ -413,9 +442,15 @@ __device__ __forceinline__ void force_pair(const KParams& K, ForceOwn<R>& o, R d
     o.fx = o.fx - G * dx;
     o.fy = o.fy - G * dy;
     R dut = (R)0.5 * G * vr;
+    R inv_eh = (R)0;
     if (AH) {
-      const R inv_eh = (AV && K.same_e2) ? inv_ea : frcp(r2 + (R)K.ah_e2);
+      inv_eh = (AV && K.same_e2) ? inv_ea : frcp(r2 + (R)K.ah_e2);
       dut = dut + m * (o.psi + psij) * F * r2 * inv_mrho * inv_eh * (o.u - uj);
+    }
+    if (PRESS && (K.flags & 128)) {
+      // Cleary-Monaghan conduction: m cond Q_i Q_j (u_i - u_j) F r^2/(r^2 + (h eta)^2)
+      if (!AH) inv_eh = frcp(r2 + (R)K.ah_e2);
+      dut = dut + m * (R)K.cond * o.Q * Qj * F * r2 * inv_eh * (o.u - uj);
     }
     o.du.add(dut);
   } else if (AH) {
@@ -479,19 +514,23 @@ __device__ __forceinline__ void density_direct(St<R>& S, const KParams& K, int i
   DensOwn<R> o;
   dens_begin<R, SUM>(K, o);
   uint32_t rm[3] = {0u, 0u, 0u};
-  bool ok = true;
+  bool ok = rw.wsh == 0;  // masks cover the unwrapped rows only
   for (int k = 0; k < 3; ++k) {
     if (rw.t1[k] - rw.t0[k] > 32) ok = false;
-    for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
-      if (t == i) continue;
-      const auto pj = load_pos(S, t);
-      R dx, dy;
-      delta(pi, pj, K, dx, dy);
-      const R r2 = dx * dx + dy * dy;
-      if (!(r2 < cut2)) continue;
-      if (t - rw.t0[k] < 32) rm[k] |= 1u << (t - rw.t0[k]);
-      const auto vj = S.v[t];
-      dens_pair<R, SUM>(K, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
+    for (int s = 0; s < 3; ++s) {
+      int a, b, sh;
+      row_part(rw, k, s, a, b, sh);
+      for (int t = a; t < b; ++t) {
+        if (t == i) continue;
+        const auto pj = load_pos(S, t);
+        R dx, dy;
+        delta(pi, pj, K, dx, dy, sh);
+        const R r2 = dx * dx + dy * dy;
+        if (!(r2 < cut2)) continue;
+        if (sh == 0 && t - rw.t0[k] < 32) rm[k] |= 1u << (t - rw.t0[k]);
+        const auto vj = S.v[t];
+        dens_pair<R, SUM>(K, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
+      }
     }
   }
   if (nbm) nbm[i] = make_uint4(rm[0], rm[1], rm[2], ok ? 1u : 0u);
@@ -508,18 +547,23 @@ __device__ __forceinline__ void force_direct(St<R>& S, const KParams& K, int i,
   const Rows rw = rows_of(K, skey[i], cstart);
   const R cut2 = cutoff2<R>(K);
   for (int k = 0; k < 3; ++k)
-    for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
-      if (t == i) continue;
-      const auto pj = load_pos(S, t);
-      R dx, dy;
-      delta(pi, pj, K, dx, dy);
-      const R r2 = dx * dx + dy * dy;
-      if (!(r2 < cut2)) continue;
-      const auto vj = S.v[t];
-      R Qj, Aj;
-      q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
-      force_pair<R, PRESS, AV, AH, XS, VI>(K, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t],
-                                           u_of(S, t), VI ? tau_of(S, t, visc_c(K, Qj)) : Tau3<R>{});
+    for (int s = 0; s < 3; ++s) {
+      int a, b, sh;
+      row_part(rw, k, s, a, b, sh);
+      for (int t = a; t < b; ++t) {
+        if (t == i) continue;
+        const auto pj = load_pos(S, t);
+        R dx, dy;
+        delta(pi, pj, K, dx, dy, sh);
+        const R r2 = dx * dx + dy * dy;
+        if (!(r2 < cut2)) continue;
+        const auto vj = S.v[t];
+        R Qj, Aj;
+        q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
+        force_pair<R, PRESS, AV, AH, XS, VI>(K, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t],
+                                             u_of(S, t),
+                                             VI ? tau_of(S, t, visc_c(K, Qj)) : Tau3<R>{});
+      }
     }
   force_end<R, PRESS, XS, VI>(S, K, i, o);
 }
@@ -603,15 +647,19 @@ __device__ __forceinline__ void tensor_direct(St<R>& S, const KParams& K, int i,
   const R cut2 = cutoff2<R>(K);
   TensOwn<R> o;
   for (int k = 0; k < 3; ++k)
-    for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
-      if (t == i) continue;
-      const auto pj = load_pos(S, t);
-      R dx, dy;
-      delta(pi, pj, K, dx, dy);
-      const R r2 = dx * dx + dy * dy;
-      if (!(r2 < cut2)) continue;
-      const auto vj = S.v[t];
-      tens_pair(K, o, dx, dy, r2, vj.x - vi.x, vj.y - vi.y, (R)1 / dens(K, S.rho[t]));
+    for (int s = 0; s < 3; ++s) {
+      int a, b, sh;
+      row_part(rw, k, s, a, b, sh);
+      for (int t = a; t < b; ++t) {
+        if (t == i) continue;
+        const auto pj = load_pos(S, t);
+        R dx, dy;
+        delta(pi, pj, K, dx, dy, sh);
+        const R r2 = dx * dx + dy * dy;
+        if (!(r2 < cut2)) continue;
+        const auto vj = S.v[t];
+        tens_pair(K, o, dx, dy, r2, vj.x - vi.x, vj.y - vi.y, (R)1 / dens(K, S.rho[t]));
+      }
     }
   tens_end(S, K, i, (R)1 / dens(K, S.rho[i]), o);
 }
@@ -701,6 +749,13 @@ __global__ __launch_bounds__(BLOCK) void k_corrector(St<R> S, KParams K, int its
         S.v[i] = v;
         q.x = q.x + dt * (double)v.x;
         q.y = q.y + dt * (double)v.y;
+        if (K.periodic) {  // re-enter on the other side of the period
+          const double L = (double)K.nx * K.cell;
+          if (q.x >= K.ox + L)
+            q.x = q.x - L;
+          else if (q.x < K.ox)
+            q.x = q.x + L;
+        }
         S.xy[i] = q;
       }
     }
@@ -734,19 +789,23 @@ __global__ __launch_bounds__(BLOCK) void k_pair_count(St<R> S, KParams K,
   int c = 0;
   int o = offs ? offs[i] : 0;
   for (int k = 0; k < 3; ++k)
-    for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
-      const int idj = S.id[t];
-      if (idj <= idi) continue;
-      const auto pj = load_pos(S, t);
-      R dx, dy;
-      delta(pi, pj, K, dx, dy);
-      const R r2 = dx * dx + dy * dy;
-      if (!(r2 < cut2)) continue;
-      if (offs) {
-        out_i[o + c] = idi;
-        out_j[o + c] = idj;
+    for (int s = 0; s < 3; ++s) {
+      int a, b, sh;
+      row_part(rw, k, s, a, b, sh);
+      for (int t = a; t < b; ++t) {
+        const int idj = S.id[t];
+        if (idj <= idi) continue;
+        const auto pj = load_pos(S, t);
+        R dx, dy;
+        delta(pi, pj, K, dx, dy, sh);
+        const R r2 = dx * dx + dy * dy;
+        if (!(r2 < cut2)) continue;
+        if (offs) {
+          out_i[o + c] = idi;
+          out_j[o + c] = idj;
+        }
+        ++c;
       }
-      ++c;
     }
   if (cnt) cnt[i] = c;
 }

@@ -261,6 +261,16 @@ __device__ __forceinline__ Segment<CW> segment_of(const KParams& K, const int* _
   return g;
 }
 
+// A particle of the first or last cell column of a grid periodic in x: its
+// neighbours wrap to the other end, outside the segment's tile, so the
+// direct path (which walks the wrapped cells) takes it.
+template <int CW>
+__device__ __forceinline__ bool seam(const KParams& K, const Segment<CW>& g, uint32_t key) {
+  if (!K.periodic) return false;
+  const int cx = (int)key - g.cy * K.nx;
+  return cx == 0 || cx == K.nx - 1;
+}
+
 // Neighbour mask of candidates l0 .. l0+len-1 (len <= 32) of row k: eight
 // independent tile reads and tests per batch, no per-candidate branch; the
 // bits past len and the lane's own bit are cleared afterwards.
@@ -363,7 +373,7 @@ __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
   for (int i0 = g.O0; i0 < g.O1; i0 += BLK) {
     const int i = i0 + threadIdx.x;
     if (i >= g.O1) continue;
-    if (!tiled) {
+    if (!tiled || seam(K, g, skey[i])) {
       density_direct<R, SUM, EOS>(S, K, i, skey, cstart, nbm);
       continue;
     }
@@ -408,7 +418,7 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
   for (int i0 = g.O0; i0 < g.O1; i0 += BLK) {
     const int i = i0 + threadIdx.x;
     if (i >= g.O1) continue;
-    if (!tiled) {
+    if (!tiled || seam(K, g, skey[i])) {
       tensor_direct(S, K, i, skey, cstart);
       continue;
     }
@@ -452,7 +462,7 @@ __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
   for (int i0 = g.O0; i0 < g.O1; i0 += BLK) {
     const int i = i0 + threadIdx.x;
     if (i >= g.O1) continue;
-    if (!tiled) {
+    if (!tiled || seam(K, g, skey[i])) {
       force_direct<R, PRESS, AV, AH, XS, VI>(S, K, i, skey, cstart);
       continue;
     }

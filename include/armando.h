@@ -60,6 +60,15 @@ extern "C" {
                               (the external heating slot exdudt of
                               CERNsource/single_step.f:217-222) */
 
+#define ARM_CONDUCTION 128 /* Cleary-Monaghan heat conduction with a uniform conductivity:
+                              du_i += sum m 4 k k/(k + k) (T_i - T_j)/(rho_i rho_j)
+                                      (x_ij . grad_i W)/(r^2 + (h ah_eta)^2), T = u/cp
+                              (no counterpart in the reference) */
+#define ARM_PERIODIC_X 256 /* periodic in x with period nx*cell from ox: the neighbour
+                              cells wrap and positions re-enter on the other side
+                              (nx >= 3; the reference's wrap is commented out,
+                              ArmandoList/armando2Dlist.cu:856-870) */
+
 /* equations of state */
 #define ARM_EOS_TAIT 0   /* p = rho0 c0^2/7 ((rho/rho0)^7 - 1), c = c0
                             [FORTRANsource/eos.f90:255-283] */
@@ -93,10 +102,12 @@ typedef struct arm_params {
      iz = (int)((z - oz)/cell) < nz and the 27 neighbour cells of
      CERNsource/link_list.f:49-99 (dim = 3) / Code/armando.cu:3142-3188;
      the kernel is Wendland C2 3D, alpha = 21/(16 pi h^3); gz is the body
-     force along z.  ARM_VISC and ARM_HEAT_SOURCE are 2D only. */
+     force along z.  ARM_VISC, ARM_HEAT_SOURCE, ARM_CONDUCTION and
+     ARM_PERIODIC_X are 2D only. */
   int dim;
   double oz, gz;
   int nz;
+  double cond_k, cp; /* ARM_CONDUCTION: conductivity (W/m/K), heat capacity (J/kg/K) */
 } arm_params;
 
 typedef struct arm_ctx arm_ctx;
