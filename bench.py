@@ -20,8 +20,13 @@ The 3D leg runs the 3D dam break (0.6 x 0.6 x 1.2 m column in a
 3.2 x 1.0 x 1.6 m tank, dx = 7.1e-3 m, three wall layers: 2,229,862
 particles, Wendland C2 3D, 27-cell hash) in both forms.
 
+The channel leg runs the 2D heated channel (0.4 x 0.05 m, periodic in x,
+dx = 1e-4 m: 2,000,000 fluid + 24,000 wall particles; Poiseuille flow at
+0.5 m/s peak, physical viscosity, Cleary-Monaghan conduction, artificial
+heat, Gaussian heat source) for 2000 steps in both forms.
+
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp64]
-                  [--no-adjoint] [--no-3d]
+                  [--no-adjoint] [--no-3d] [--no-channel]
 """
 import argparse
 import json
@@ -73,6 +78,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D dam break leg")
     ap.add_argument("--steps-3d", type=int, default=100)
+    ap.add_argument("--no-channel", action="store_true", help="skip the heated channel leg")
+    ap.add_argument("--steps-channel", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work for the baseline sample")
     ap.add_argument("--scale", type=float, default=1.0,
@@ -224,6 +231,58 @@ def dam3d_leg(steps, warmup=5):
     }
 
 
+def channel_leg(steps, warmup=5):
+    """The heated channel in the FP32 chosen and the FP64 plain form: whole
+    steps of time_integration, per-pass device times, FP32-vs-FP64 accuracy
+    and the temperature rise under the source."""
+    import armando_amd
+    from armando_amd import cases
+
+    res = {}
+    for name, prec in (("fp32", cases.FP32), ("fp64", cases.FP64)):
+        case = cases.heated_channel(precision=prec)
+        sim = armando_amd.Simulation(case)
+        niac = sim.link_list(count=True)
+        sim.time_integration(1, warmup)
+        t0 = time.perf_counter()
+        sim.time_integration(1 + warmup, steps)
+        wall = time.perf_counter() - t0
+        sim.set_timing(True)
+        sim.reset_pass_times()
+        sim.time_integration(1 + warmup + steps, 20)
+        passes = {k: ms / max(cnt, 1) for k, (ms, cnt) in sim.pass_times().items()}
+        sim.set_timing(False)
+        res[name] = dict(case=case, niac=niac, wall=wall, passes=passes, state=sim.state())
+        sim.close()
+    r32, r64 = res["fp32"], res["fp64"]
+    case = r32["case"]
+    n, nf = case["n"], case["np"]
+    s32, s64 = r32["state"], r64["state"]
+    acc = {}
+    for k in ("rho", "u", "x", "y"):
+        acc[k] = float(np.linalg.norm(s32[k] - s64[k]) / np.linalg.norm(s64[k]))
+    v32 = np.concatenate([s32["vx"], s32["vy"]])
+    v64 = np.concatenate([s64["vx"], s64["vy"]])
+    acc["v"] = float(np.linalg.norm(v32 - v64) / np.linalg.norm(v64))
+    dT = (s32["u"][:nf] - case["u"][:nf]) / case["cp"]
+    return {
+        "workload": (f"2D heated channel, {nf} fluid + {case['nv']} wall particles, periodic x, "
+                     f"Q0 {case['hs_q0']:.4g} W/m^3"),
+        "value": n * steps / r32["wall"],
+        "unit": "particle-steps/s",
+        "dtype": "f32",
+        "ms_per_step": 1e3 * r32["wall"] / steps,
+        "steps": steps,
+        "pairs_i_lt_j": r32["niac"],
+        "passes_ms": r32["passes"],
+        "fp64": {"value": n * steps / r64["wall"], "ms_per_step": 1e3 * r64["wall"] / steps,
+                 "passes_ms": r64["passes"]},
+        "speedup_fp32_over_fp64": r64["wall"] / r32["wall"],
+        "max_temperature_rise_K": float(np.max(dT)),
+        "accuracy_fp32_vs_fp64_rel_l2": {"after_steps": warmup + steps + 20, **acc},
+    }
+
+
 def cpu_baseline(case, seconds):
     """The oracle's faithful serial restatement of the reference loop
     (half pair list, scatter; one core) on the same seeded input."""
@@ -338,6 +397,8 @@ def main():
         out["adjoint"] = adjoint_ratio()
     if not args.no_3d:
         out["dam_break_3d"] = dam3d_leg(args.steps_3d)
+    if not args.no_channel:
+        out["heated_channel"] = channel_leg(args.steps_channel)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out))
