@@ -266,11 +266,11 @@ class Simulation:
         return out
 
     def adjoint_rollout(self, nsteps, ckpt_every, obj_ids):
-        """J, dJ/dalpha, dJ/dv0 (n, 2) and (forward, backward) device ms."""
+        """J, dJ/dalpha, dJ/dv0 (n, dim) and (forward, backward) device ms."""
         ids = np.ascontiguousarray(obj_ids, np.int32)
         J = C.c_double(0)
         da = C.c_double(0)
-        g = np.zeros((self.n, 2))
+        g = np.zeros((self.n, self.dim))
         tf = C.c_double(0)
         tb = C.c_double(0)
         self._check(self.L.arm_adjoint_rollout(self.ctx, nsteps, ckpt_every, ids, len(ids),
@@ -289,9 +289,9 @@ class Simulation:
         return J.value, tf.value
 
     def rollout_backward(self, seed=1.0):
-        """seed * (dJ/dalpha, dJ/dv0 (n, 2)) and the backward device ms."""
+        """seed * (dJ/dalpha, dJ/dv0 (n, dim)) and the backward device ms."""
         da = C.c_double(0)
-        g = np.zeros((self.n, 2))
+        g = np.zeros((self.n, self.dim))
         tb = C.c_double(0)
         self._check(self.L.arm_rollout_backward(self.ctx, seed, C.byref(da), _ptr(g),
                                                 C.byref(tb)))

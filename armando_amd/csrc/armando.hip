@@ -22,6 +22,7 @@
 #include "sph_tiled.h"
 #include "sph_adjoint.h"
 #include "sph3d.h"
+#include "sph3d_adjoint.h"
 
 using namespace arm;
 
@@ -631,6 +632,22 @@ void adj_force(arm_ctx* c, const Tape<R>& T, const AdjScratch<R>& W, double* apa
 
 }  // namespace
 
+// the part of the 3D state a step reads at its start
+template <typename R> struct Checkpoint3 {
+  using R4 = typename V4<R>::type;
+  int32_t* id;
+  double4* xyz;
+  R4 *v, *a;
+  R *rho, *p, *du, *drho;
+  double* u;
+};
+
+template <typename R> struct AdjArrays3 {
+  std::vector<Checkpoint3<R>> ck;
+  std::vector<Tape3<R>> tape;
+  AdjScratch3<R> W{};
+};
+
 // Checkpoints, tapes and adjoint arrays of one rollout, kept between
 // arm_rollout_forward and arm_rollout_backward.
 template <typename R> struct AdjArrays {
@@ -653,6 +670,15 @@ struct AdjRun {
       return a64;
     else
       return a32;
+  }
+  AdjState3 L3{};
+  AdjArrays3<double> b64;
+  AdjArrays3<float> b32;
+  template <typename R> AdjArrays3<R>& arrays3() {
+    if constexpr (sizeof(R) == 8)
+      return b64;
+    else
+      return b32;
   }
   explicit AdjRun(arm_ctx* c) : sc(c) {}
 };
@@ -1180,7 +1206,7 @@ template <typename R> int corrector3_t(arm_ctx* c, int its, bool keys) {
 }
 
 // The fused step of time_integration.f:131-235 in 3D, as fused_step in 2D.
-template <typename R> int fused_step3(arm_ctx* c, int its) {
+template <typename R> int fused_step3(arm_ctx* c, int its, const StepHooks& hk = StepHooks{}) {
   const int n = c->n;
   const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
   int rc;
@@ -1202,7 +1228,15 @@ template <typename R> int fused_step3(arm_ctx* c, int its) {
   if (!xs && (rc = zero_av3<R>(c))) return rc;
   c->sorted = true;
   if ((rc = density3_t<R>(c, c->prm.density_method, true))) return rc;
+  if (hk.jpart)
+    hipLaunchKernelGGL(k3_objective<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, st3A<R>(c),
+                       c->K, hk.kind, hk.objmask, hk.jpart);
   if ((rc = force3_t<R>(c, true, c->prm.flags, xs))) return rc;
+  if (hk.tape) {
+    const int nb = std::max(grid_of(n), grid_of(c->K.ncell + 1));
+    hipLaunchKernelGGL(k3_tape<R>, dim3(nb), dim3(BLOCK), 0, c->stream, st3A<R>(c), c->K, c->skey,
+                       c->cstart, *static_cast<const Tape3<R>*>(hk.tape));
+  }
   return corrector3_t<R>(c, its, true);
 }
 
@@ -1405,6 +1439,229 @@ template <typename R> int conservation3_t(arm_ctx* c, double out[6]) {
   for (int b = 0; b < nb; ++b)
     for (int k = 0; k < 6; ++k) out[k] += part[(size_t)b * 6 + k];
   return ARM_OK;
+}
+
+// ---- 3D adjoint rollout ------------------------------------------------------
+template <typename R> int alloc_checkpoint3(Scratch& sc, Checkpoint3<R>& k, int n) {
+  int rc;
+  if ((rc = sc.get(&k.id, n)) || (rc = sc.get(&k.xyz, n)) || (rc = sc.get(&k.v, n)) ||
+      (rc = sc.get(&k.a, n)) || (rc = sc.get(&k.rho, n)) || (rc = sc.get(&k.u, n)) ||
+      (rc = sc.get(&k.p, n)) || (rc = sc.get(&k.du, n)) || (rc = sc.get(&k.drho, n)))
+    return rc;
+  return ARM_OK;
+}
+
+template <typename R>
+int copy_checkpoint3(arm_ctx* c, const St3<R>& S, const Checkpoint3<R>& k, bool save) {
+  using R4 = typename V4<R>::type;
+  const size_t n = (size_t)c->n;
+  auto cp = [&](void* st, void* ck, size_t bytes) {
+    return save ? hipMemcpyAsync(ck, st, bytes, hipMemcpyDeviceToDevice, c->stream)
+                : hipMemcpyAsync(st, ck, bytes, hipMemcpyDeviceToDevice, c->stream);
+  };
+  HIPCHK(c, cp(S.id, k.id, n * sizeof(int32_t)));
+  HIPCHK(c, cp(S.xyz, k.xyz, n * sizeof(double4)));
+  HIPCHK(c, cp(S.v, k.v, n * sizeof(R4)));
+  HIPCHK(c, cp(S.a, k.a, n * sizeof(R4)));
+  HIPCHK(c, cp(S.rho, k.rho, n * sizeof(R)));
+  HIPCHK(c, cp(S.u, k.u, n * sizeof(double)));
+  HIPCHK(c, cp(S.p, k.p, n * sizeof(R)));
+  HIPCHK(c, cp(S.du, k.du, n * sizeof(R)));
+  HIPCHK(c, cp(S.drho, k.drho, n * sizeof(R)));
+  return ARM_OK;
+}
+
+template <typename R> int alloc_tape3(Scratch& sc, Tape3<R>& t, int n, int ncell) {
+  int rc;
+  t.pf = nullptr;
+  if (sizeof(R) == 4 && (rc = sc.get(&t.pf, n))) return rc;
+  if ((rc = sc.get(&t.id, n)) || (rc = sc.get(&t.xyz, n)) || (rc = sc.get(&t.v, n)) ||
+      (rc = sc.get(&t.rho, n)) || (rc = sc.get(&t.u, n)) || (rc = sc.get(&t.psi, n)) ||
+      (rc = sc.get(&t.divv, n)) || (rc = sc.get(&t.p, n)) || (rc = sc.get(&t.drho, n)) ||
+      (rc = sc.get(&t.du, n)) || (rc = sc.get(&t.a, n)) || (rc = sc.get(&t.av, n)) ||
+      (rc = sc.get(&t.rho_min, n)) || (rc = sc.get(&t.u_min, n)) || (rc = sc.get(&t.v_min, n)) ||
+      (rc = sc.get(&t.flag, n)) || (rc = sc.get(&t.skey, n)) || (rc = sc.get(&t.cstart, ncell + 1)))
+    return rc;
+  return ARM_OK;
+}
+
+template <typename R>
+void adj_force3(arm_ctx* c, const Tape3<R>& T, const AdjScratch3<R>& W, double* apart) {
+  const int f = c->prm.flags;
+  const int code = ((f & ARM_ART_VISC) ? 4 : 0) | ((f & ARM_ART_HEAT) ? 2 : 0) | ((f & ARM_AV_VEL) ? 1 : 0);
+  const dim3 g(grid_of(c->n)), b(BLOCK);
+  switch (code) {
+    case 0: hipLaunchKernelGGL((k3_adj_force<R, false, false, false>), g, b, 0, c->stream, T, c->K, W, apart); break;
+    case 1: hipLaunchKernelGGL((k3_adj_force<R, false, false, true>), g, b, 0, c->stream, T, c->K, W, apart); break;
+    case 2: hipLaunchKernelGGL((k3_adj_force<R, false, true, false>), g, b, 0, c->stream, T, c->K, W, apart); break;
+    case 3: hipLaunchKernelGGL((k3_adj_force<R, false, true, true>), g, b, 0, c->stream, T, c->K, W, apart); break;
+    case 4: hipLaunchKernelGGL((k3_adj_force<R, true, false, false>), g, b, 0, c->stream, T, c->K, W, apart); break;
+    case 5: hipLaunchKernelGGL((k3_adj_force<R, true, false, true>), g, b, 0, c->stream, T, c->K, W, apart); break;
+    case 6: hipLaunchKernelGGL((k3_adj_force<R, true, true, false>), g, b, 0, c->stream, T, c->K, W, apart); break;
+    default: hipLaunchKernelGGL((k3_adj_force<R, true, true, true>), g, b, 0, c->stream, T, c->K, W, apart); break;
+  }
+}
+
+// The 3D counterpart of rollout_forward: checkpoints every ckpt steps and
+// the objective, on the 3D state.
+template <typename Rf>
+int rollout_forward3(arm_ctx* c, int nsteps, int ckpt, const int32_t* obj_ids, int nobj,
+                     double* J, double* ms) {
+  const int n = c->n, ncell = c->K.ncell;
+  const int nb = grid_of(n);
+  drop_adjoint(c);
+  AdjRun* R = new AdjRun(c);
+  c->adj = R;
+  Scratch& sc = R->sc;
+  int rc;
+  R->nsteps = nsteps;
+  R->ckpt = ckpt;
+  R->nck = (nsteps + ckpt - 1) / ckpt;
+  R->kind = c->objkind;
+  if ((rc = sc.get(&R->obj, n)) || (rc = sc.get(&R->jpart, nb)) || (rc = sc.get(&R->apart, nb)))
+    return rc;
+  {
+    std::vector<uint8_t> h(n, 0);
+    for (int k = 0; k < nobj; ++k) {
+      if (obj_ids[k] < 0 || obj_ids[k] >= n)
+        return fail(c, ARM_ERR_ARG, "objective id %d out of range", obj_ids[k]);
+      h[obj_ids[k]] = 1;
+    }
+    HIPCHK(c, hipMemcpyAsync(R->obj, h.data(), n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  HIPCHK(c, hipMemsetAsync(R->jpart, 0, sizeof(double) * nb, c->stream));
+  AdjArrays3<Rf>& Ar = R->arrays3<Rf>();
+  Ar.ck.resize(R->nck + 1);
+  for (auto& k : Ar.ck)
+    if ((rc = alloc_checkpoint3(sc, k, n))) return rc;
+  Ar.tape.resize(ckpt);
+  for (auto& t : Ar.tape)
+    if ((rc = alloc_tape3(sc, t, n, ncell))) return rc;
+  AdjState3& L = R->L3;
+  double** lf[] = {&L.x,   &L.y,   &L.z,    &L.vx,  &L.vy,  &L.vz,  &L.rho,   &L.u,
+                   &L.bx,  &L.by,  &L.bz,   &L.bvx, &L.bvy, &L.bvz, &L.brho,  &L.bu,
+                   &L.cax, &L.cay, &L.caz,  &L.cdrho, &L.cdu};
+  for (double** q : lf)
+    if ((rc = sc.get(q, n))) return rc;
+  AdjScratch3<Rf>& W = Ar.W;
+  Rf** wf[] = {&W.lax, &W.lay, &W.laz, &W.ldrho, &W.ldu, &W.lavx, &W.lavy, &W.lavz,
+               &W.lp,  &W.lQ,  &W.lA,  &W.lpsi,  &W.lS,  &W.lrhoh, &W.lu, &W.lvx,
+               &W.lvy, &W.lvz, &W.lx,  &W.ly,    &W.lz,  &W.lus};
+  for (Rf** q : wf)
+    if ((rc = sc.get(q, n))) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, hipEventRecord(e0, c->stream));
+  zero_counts(c);
+  hipLaunchKernelGGL(k3_keys<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, st3A<Rf>(c), c->K,
+                     key_out(c), c->d_err);
+  if (!(c->prm.flags & ARM_AV_VEL) && (rc = zero_av3<Rf>(c))) return rc;
+  StepHooks fwd;
+  fwd.objmask = R->obj;
+  fwd.jpart = R->jpart;
+  fwd.kind = R->kind;
+  for (int its = 1; its <= nsteps; ++its) {
+    if ((its - 1) % ckpt == 0)
+      if ((rc = copy_checkpoint3(c, st3A<Rf>(c), Ar.ck[(its - 1) / ckpt], true))) return rc;
+    if ((rc = fused_step3<Rf>(c, its, fwd))) return rc;
+  }
+  if ((rc = copy_checkpoint3(c, st3A<Rf>(c), Ar.ck[R->nck], true))) return rc;
+  HIPCHK(c, hipEventRecord(e1, c->stream));
+  std::vector<double> jp(nb);
+  HIPCHK(c, hipMemcpyAsync(jp.data(), R->jpart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float f = 0.f;
+  (void)hipEventElapsedTime(&f, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  double Js = 0;
+  for (int k = 0; k < nb; ++k) Js += jp[k];
+  if (J) *J = Js;
+  if (ms) *ms = f;
+  c->sorted = false;
+  c->masks_valid = false;
+  c->have_rates = true;
+  return check_domain(c);
+}
+
+template <typename Rf>
+int rollout_backward3(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0, double* ms) {
+  AdjRun* R = c->adj;
+  const int n = c->n, nb = grid_of(n);
+  const int nsteps = R->nsteps, ckpt = R->ckpt, nck = R->nck;
+  AdjState3& L = R->L3;
+  AdjArrays3<Rf>& Ar = R->arrays3<Rf>();
+  AdjScratch3<Rf>& W = Ar.W;
+  int rc;
+  double** lf[] = {&L.x,  &L.y,   &L.z,   &L.vx,  &L.vy,    &L.vz,  &L.rho,
+                   &L.u,  &L.cax, &L.cay, &L.caz, &L.cdrho, &L.cdu};
+  for (double** q : lf) HIPCHK(c, hipMemsetAsync(*q, 0, sizeof(double) * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(R->apart, 0, sizeof(double) * nb, c->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, hipEventRecord(e0, c->stream));
+  for (int seg = nck - 1; seg >= 0; --seg) {
+    const int s0 = 1 + seg * ckpt, s1 = std::min(s0 + ckpt, nsteps + 1);
+    if ((rc = copy_checkpoint3(c, st3A<Rf>(c), Ar.ck[seg], false))) return rc;
+    zero_counts(c);
+    hipLaunchKernelGGL(k3_keys<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, st3A<Rf>(c), c->K,
+                       key_out(c), c->d_err);
+    for (int its = s0; its < s1; ++its) {
+      StepHooks rec;
+      rec.tape = &Ar.tape[its - s0];
+      if ((rc = fused_step3<Rf>(c, its, rec))) return rc;
+    }
+    for (int its = s1 - 1; its >= s0; --its) {
+      const Tape3<Rf>& T = Ar.tape[its - s0];
+      hipLaunchKernelGGL(k3_adj_corrector<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its,
+                         L, W, R->obj, R->kind, seed);
+      adj_force3<Rf>(c, T, W, R->apart);
+      hipLaunchKernelGGL(k3_adj_eos<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
+      hipLaunchKernelGGL(k3_adj_density<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, W);
+      hipLaunchKernelGGL(k3_adj_predictor<Rf>, dim3(nb), dim3(BLOCK), 0, c->stream, T, c->K, its,
+                         L, W);
+      HIPCHK(c, hipGetLastError());
+      std::swap(L.x, L.bx);
+      std::swap(L.y, L.by);
+      std::swap(L.z, L.bz);
+      std::swap(L.vx, L.bvx);
+      std::swap(L.vy, L.bvy);
+      std::swap(L.vz, L.bvz);
+      std::swap(L.rho, L.brho);
+      std::swap(L.u, L.bu);
+    }
+  }
+  HIPCHK(c, hipEventRecord(e1, c->stream));
+  if ((rc = copy_checkpoint3(c, st3A<Rf>(c), Ar.ck[nck], false))) return rc;
+  std::vector<double> ap(nb), gx(n), gy(n), gz(n);
+  HIPCHK(c, hipMemcpyAsync(ap.data(), R->apart, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gx.data(), L.vx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gy.data(), L.vy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(gz.data(), L.vz, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float f = 0.f;
+  (void)hipEventElapsedTime(&f, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  double As = 0;
+  for (int k = 0; k < nb; ++k) As += ap[k];
+  c->grads[ARM_GRAD_ALPHA] = As;
+  c->grads[ARM_GRAD_Q0] = 0.0;
+  c->grads[ARM_GRAD_VISC] = 0.0;
+  if (dJ_dalpha) *dJ_dalpha = As;
+  if (dJ_dv0)
+    for (int i = 0; i < n; ++i) {
+      dJ_dv0[3 * i] = gx[i];
+      dJ_dv0[3 * i + 1] = gy[i];
+      dJ_dv0[3 * i + 2] = gz[i];
+    }
+  if (ms) *ms = f;
+  c->sorted = false;
+  c->masks_valid = false;
+  return check_domain(c);
 }
 
 #define DISPATCH(c, call) (is32(c) ? call<float> : call<double>)
@@ -1831,12 +2088,16 @@ int arm_rollout_forward(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* o
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
   if (c->prm.density_method != 0)
     return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint differentiates continuity density only");
-  if (is3d(c)) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint is implemented in 2D only");
   if (c->prm.flags & (ARM_CONDUCTION | ARM_PERIODIC_X))
     return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint has no conduction or periodic boundary");
   const int ck = std::min(ckpt_every, nsteps);
-  const int rc = is32(c) ? rollout_forward<float>(c, nsteps, ck, obj_ids, nobj, J, ms)
-                         : rollout_forward<double>(c, nsteps, ck, obj_ids, nobj, J, ms);
+  int rc;
+  if (is3d(c))
+    rc = is32(c) ? rollout_forward3<float>(c, nsteps, ck, obj_ids, nobj, J, ms)
+                 : rollout_forward3<double>(c, nsteps, ck, obj_ids, nobj, J, ms);
+  else
+    rc = is32(c) ? rollout_forward<float>(c, nsteps, ck, obj_ids, nobj, J, ms)
+                 : rollout_forward<double>(c, nsteps, ck, obj_ids, nobj, J, ms);
   if (rc) drop_adjoint(c);
   return rc;
 }
@@ -1844,6 +2105,9 @@ int arm_rollout_forward(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* o
 int arm_rollout_backward(arm_ctx* c, double seed, double* dJ_dalpha, double* dJ_dv0, double* ms) {
   if (!c) return ARM_ERR_ARG;
   if (!c->adj) return fail(c, ARM_ERR_STATE, "no arm_rollout_forward to differentiate");
+  if (is3d(c))
+    return is32(c) ? rollout_backward3<float>(c, seed, dJ_dalpha, dJ_dv0, ms)
+                   : rollout_backward3<double>(c, seed, dJ_dalpha, dJ_dv0, ms);
   return is32(c) ? rollout_backward<float>(c, seed, dJ_dalpha, dJ_dv0, ms)
                  : rollout_backward<double>(c, seed, dJ_dalpha, dJ_dv0, ms);
 }

@@ -175,10 +175,12 @@ int arm_time_integration(arm_ctx* ctx, int its0, int nsteps);
    hand-written adjoint kernels of the corrector, force, EOS, density and
    predictor run in reverse.  The neighbour list of each step is that of the
    forward step.  Outputs (any may be NULL): J, dJ/d av_alpha, dJ/dv0 as
-   (2, n) in the original order, and the forward / backward device times in
-   ms (dJ/d hs_q0: arm_rollout_param_grads).  The context is left at the
+   (dim, n) in the original order, and the forward / backward device times
+   in ms (dJ/d hs_q0: arm_rollout_param_grads).  The context is left at the
    state after nsteps.  Either form (the FP32 form's reverse pair arithmetic
-   is FP32, its state adjoint FP64); continuity density. */
+   is FP32, its state adjoint FP64); continuity density; 2D and 3D (3D:
+   without the physical viscosity and the heat source, which are 2D only);
+   no conduction or periodic boundary. */
 int arm_adjoint_rollout(arm_ctx* ctx, int nsteps, int ckpt_every, const int32_t* obj_ids,
                         int nobj, double* J, double* dJ_dalpha, double* dJ_dv0, double* ms_fwd,
                         double* ms_bwd);
