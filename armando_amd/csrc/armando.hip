@@ -1235,7 +1235,8 @@ template <typename R> int fused_step3(arm_ctx* c, int its, const StepHooks& hk =
   if (hk.tape) {
     const int nb = std::max(grid_of(n), grid_of(c->K.ncell + 1));
     hipLaunchKernelGGL(k3_tape<R>, dim3(nb), dim3(BLOCK), 0, c->stream, st3A<R>(c), c->K, c->skey,
-                       c->cstart, *static_cast<const Tape3<R>*>(hk.tape));
+                       c->cstart, Mask3{c->masks_valid ? c->nbm3 : nullptr, c->cap},
+                       *static_cast<const Tape3<R>*>(hk.tape));
   }
   return corrector3_t<R>(c, its, true);
 }
@@ -1480,7 +1481,8 @@ template <typename R> int alloc_tape3(Scratch& sc, Tape3<R>& t, int n, int ncell
       (rc = sc.get(&t.divv, n)) || (rc = sc.get(&t.p, n)) || (rc = sc.get(&t.drho, n)) ||
       (rc = sc.get(&t.du, n)) || (rc = sc.get(&t.a, n)) || (rc = sc.get(&t.av, n)) ||
       (rc = sc.get(&t.rho_min, n)) || (rc = sc.get(&t.u_min, n)) || (rc = sc.get(&t.v_min, n)) ||
-      (rc = sc.get(&t.flag, n)) || (rc = sc.get(&t.skey, n)) || (rc = sc.get(&t.cstart, ncell + 1)))
+      (rc = sc.get(&t.flag, n)) || (rc = sc.get(&t.skey, n)) || (rc = sc.get(&t.cstart, ncell + 1)) ||
+      (rc = sc.get(&t.nbm, (size_t)M3_TOTAL * n)))
     return rc;
   return ARM_OK;
 }

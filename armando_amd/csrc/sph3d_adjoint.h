@@ -39,6 +39,7 @@ template <typename R> struct Tape3 {
   uint8_t* flag;
   uint32_t* skey;
   int* cstart;
+  uint32_t* nbm;  // the forward's neighbour masks (Mask3 layout, stride n)
 };
 
 __device__ __forceinline__ Pos3D tape_pos3(const Tape3<double>& T, int t) {
@@ -68,7 +69,8 @@ template <typename R> struct AdjScratch3 {
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k3_tape(St3<R> S, KParams K,
                                                  const uint32_t* __restrict__ skey,
-                                                 const int* __restrict__ cstart, Tape3<R> T) {
+                                                 const int* __restrict__ cstart, Mask3 nbm,
+                                                 Tape3<R> T) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   const int stride = gridDim.x * BLOCK;
   for (int c = t; c <= K.ncell; c += stride) T.cstart[c] = cstart[c];
@@ -91,6 +93,45 @@ __global__ __launch_bounds__(BLOCK) void k3_tape(St3<R> S, KParams K,
   T.v_min[t] = S.v_min[t];
   T.flag[t] = S.flag[t];
   T.skey[t] = skey[t];
+  for (int q = 0; q < M3_TOTAL; ++q)
+    T.nbm[(size_t)q * K.n + t] = nbm.w ? nbm.get(q, t) : 0u;
+}
+
+// Calls visit(l, uy, uz) for every neighbour l of particle k of the taped
+// step, in the forward's order: from the recorded masks when they are
+// complete, else by testing the candidates again.
+template <typename R, typename Visit>
+__device__ __forceinline__ void for_tape_neighbours3(const Tape3<R>& T, const KParams& K, int k,
+                                                     Visit&& visit) {
+  const Rows9 rw = rows3_of(K, T.skey[k], T.cstart);
+  const size_t st = (size_t)K.n;
+  if (T.nbm[M3_OK * st + k]) {
+    for (int row = 0; row < 9; ++row) {
+      const R uy = row_u3<R>(K, 1 - row % 3), uz = row_u3<R>(K, 1 - row / 3);
+      for (int q = 0; q < M3_WORDS; ++q) {
+        uint32_t m = T.nbm[(size_t)(row * M3_WORDS + q) * st + k];
+        const int tq = rw.t0[row] + 32 * q;
+        while (m) {
+          const int b = __builtin_ctz(m);
+          m &= m - 1;
+          visit(tq + b, uy, uz);
+        }
+      }
+    }
+    return;
+  }
+  const auto pk = tape_pos3(T, k);
+  const R cut2 = cutoff2<R>(K);
+  for (int row = 0; row < 9; ++row) {
+    const R uy = row_u3<R>(K, 1 - row % 3), uz = row_u3<R>(K, 1 - row / 3);
+    for (int l = rw.t0[row]; l < rw.t1[row]; ++l) {
+      if (l == k) continue;
+      R d0, d1, d2;
+      const R r2 = delta3(pk, tape_pos3(T, l), K, uy, uz, d0, d1, d2);
+      if (!(r2 < cut2)) continue;
+      visit(l, uy, uz);
+    }
+  }
 }
 
 // objective: jpart[block] += dt sum_{obj} f, f = p^2 (kind 0) or u^2 (kind 1)
@@ -202,18 +243,12 @@ __global__ __launch_bounds__(BLOCK) void k3_adj_force(Tape3<R> T, KParams K, Adj
     const R lavk[3] = {W.lavx[k], W.lavy[k], W.lavz[k]};
     const R lduk = W.ldu[k];
     const R psik = T.psi[k], uk = (R)T.u[k];
-    const R cut2 = cutoff2<R>(K);
     const R c0 = (R)K.c0, al = (R)K.av_alpha, be = (R)K.av_beta;
     R lr_acc[3] = {0, 0, 0}, lv_acc[3] = {0, 0, 0};
     R lQ = 0, lA = 0, lpsi = 0, lu = 0;
-    const Rows9 rw = rows3_of(K, T.skey[k], T.cstart);
-    for (int row = 0; row < 9; ++row) {
-      const R uy = row_u3<R>(K, 1 - row % 3), uz = row_u3<R>(K, 1 - row / 3);
-      for (int l = rw.t0[row]; l < rw.t1[row]; ++l) {
-        if (l == k) continue;
+    for_tape_neighbours3<R>(T, K, k, [&](int l, R uy, R uz) {
         R r[3];
         const R r2 = delta3(pk, tape_pos3(T, l), K, uy, uz, r[0], r[1], r[2]);
-        if (!(r2 < cut2)) continue;
         R q, F, Wv;
         wendland3_q(K, r2, q, F, Wv);
         const auto vl = T.v[l];
@@ -284,8 +319,7 @@ __global__ __launch_bounds__(BLOCK) void k3_adj_force(Tape3<R> T, KParams K, Adj
           lr_acc[c] += lr[c];
           lv_acc[c] += ldv[c];
         }
-      }
-    }
+    });
     W.lx[k] = lr_acc[0];
     W.ly[k] = lr_acc[1];
     W.lz[k] = lr_acc[2];
@@ -344,16 +378,10 @@ __global__ __launch_bounds__(BLOCK) void k3_adj_density(Tape3<R> T, KParams K,
   const auto pk = tape_pos3(T, k);
   const auto vk = T.v[k];
   const R ltk = W.ldrho[k] - W.lS[k];
-  const R cut2 = cutoff2<R>(K);
   R lr[3] = {0, 0, 0}, lv[3] = {0, 0, 0};
-  const Rows9 rw = rows3_of(K, T.skey[k], T.cstart);
-  for (int row = 0; row < 9; ++row) {
-    const R uy = row_u3<R>(K, 1 - row % 3), uz = row_u3<R>(K, 1 - row / 3);
-    for (int l = rw.t0[row]; l < rw.t1[row]; ++l) {
-      if (l == k) continue;
+  for_tape_neighbours3<R>(T, K, k, [&](int l, R uy, R uz) {
       R r[3];
       const R r2 = delta3(pk, tape_pos3(T, l), K, uy, uz, r[0], r[1], r[2]);
-      if (!(r2 < cut2)) continue;
       const R lt = ltk + W.ldrho[l] - W.lS[l];
       R q, F, Wv;
       wendland3_q(K, r2, q, F, Wv);
@@ -367,8 +395,7 @@ __global__ __launch_bounds__(BLOCK) void k3_adj_density(Tape3<R> T, KParams K,
         lr[c] += lvr * dv[c] + (R)2 * lr2 * r[c];
         lv[c] += lvr * r[c];
       }
-    }
-  }
+  });
   W.lx[k] += lr[0];
   W.ly[k] += lr[1];
   W.lz[k] += lr[2];

@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D dam break leg")
     ap.add_argument("--steps-3d", type=int, default=100)
+    ap.add_argument("--steps-3d-adjoint", type=int, default=500,
+                    help="steps of the 3D FP32 adjoint rollout (0: skip)")
     ap.add_argument("--no-channel", action="store_true", help="skip the heated channel leg")
     ap.add_argument("--steps-channel", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -174,9 +176,42 @@ def adjoint_ratio(steps=400, ckpt=20):
     }
 
 
-def dam3d_leg(steps, warmup=5):
+def dam3d_adjoint(steps, ckpt=20, seed=4243, vstd=0.01):
+    """The adjoint rollout of the 3D dam break, FP32 chosen form: initial
+    velocities N(0, vstd) per component on the fluid (seeded), objective the
+    time-integrated p^2 of the left-wall particles below 0.48 m, checkpoints
+    every `ckpt` steps.  Returns the adjoint/forward device-time ratio."""
+    import armando_amd
+    from armando_amd import cases
+
+    c = cases.dam_break3d(precision=cases.FP32)
+    rng = np.random.default_rng(seed)
+    nf = c["np"]
+    for k in ("vx", "vy", "vz"):
+        c[k] = c[k].copy()
+        c[k][:nf] = rng.normal(0.0, vstd, nf)
+    obj = np.nonzero((c["x"] < 0) & (c["z"] < 0.48))[0].astype(np.int32)
+    sim = armando_amd.Simulation(c)
+    sim.adjoint_rollout(3, 3, obj)  # warm up the kernels
+    sim.close()
+    sim = armando_amd.Simulation(c)
+    J, da, g, (t_fwd, t_bwd) = sim.adjoint_rollout(steps, ckpt, obj)
+    sim.close()
+    # memory of the rollout: one tape per step of a segment (the state after
+    # the force pass, 165 bytes per particle, and the neighbour masks, 112),
+    # one checkpoint per segment boundary (92 bytes per particle)
+    tape_b, ck_b = 165 + 112, 92
+    mem = c["n"] * (ckpt * tape_b + (steps // ckpt + 2) * ck_b)
+    return {"steps": steps, "checkpoint_every": ckpt, "ratio": t_bwd / t_fwd,
+            "forward_ms": t_fwd, "backward_ms": t_bwd, "J": J, "dJ_dalpha": da,
+            "dJ_dv0_norm": float(np.linalg.norm(g)), "objective_particles": int(len(obj)),
+            "rollout_bytes": mem}
+
+
+def dam3d_leg(steps, warmup=5, adjoint_steps=500):
     """The 3D dam break in the FP32 chosen and the FP64 plain form: whole
-    steps of time_integration, per-pass device times, FP32-vs-FP64 accuracy."""
+    steps of time_integration, per-pass device times, FP32-vs-FP64 accuracy;
+    then the FP32 adjoint rollout."""
     import armando_amd
     from armando_amd import cases
 
@@ -228,6 +263,7 @@ def dam3d_leg(steps, warmup=5):
                            "bytes_per_particle": BYTES[("force3", 1)],
                            "avg_launch_ms": t_force * 1e3},
         "accuracy_fp32_vs_fp64_rel_l2": {"after_steps": warmup + steps + 20, **acc},
+        "adjoint": dam3d_adjoint(adjoint_steps) if adjoint_steps else None,
     }
 
 
@@ -396,7 +432,7 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_ratio()
     if not args.no_3d:
-        out["dam_break_3d"] = dam3d_leg(args.steps_3d)
+        out["dam_break_3d"] = dam3d_leg(args.steps_3d, adjoint_steps=args.steps_3d_adjoint)
     if not args.no_channel:
         out["heated_channel"] = channel_leg(args.steps_channel)
     if not args.no_cpu:
