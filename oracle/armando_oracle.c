@@ -27,9 +27,9 @@ static const double TWO_THIRDS_F = 0.6666666865348816;
 
 /* Faithful per-component form of kernel.f (skf = 0) and the Wendland C2 of
    armando2Dcpu.cu:116-172.  dwdx = grad_i W for dx = x_i - x_j. */
-static double kern(const orc_params* P, double r, double dx, double dy, double* dwx,
-                   double* dwy) {
-  const double h = P->h;
+/* W and grad W of smoothing length h (kernel.f:31-56 takes h as mh). */
+static double kern_h(const orc_params* P, double h, double r, double dx, double dy, double* dwx,
+                     double* dwy) {
   const double q = r / h;
   if (P->kernel == ORC_K_WENDLAND2D) {
     /* W = a (1 - q/2)^4 (1 + 2q), dW/dr = (a/h)(5/8) q (q-2)^3, a = 7/(4 pi h^2).
@@ -68,6 +68,11 @@ static double kern(const orc_params* P, double r, double dx, double dy, double* 
   *dwx = 0.0;
   *dwy = 0.0;
   return 0.0;
+}
+
+static double kern(const orc_params* P, double r, double dx, double dy, double* dwx,
+                   double* dwy) {
+  return kern_h(P, P->h, r, dx, dy, dwx, dwy);
 }
 
 double orc_kernel(const orc_params* P, double r, double* F) {
@@ -522,7 +527,7 @@ orc_state* orc_state_new(int n, int np) {
   double** f[] = {&s->x,   &s->y,   &s->vx,  &s->vy,   &s->rho,     &s->u,
                   &s->p,   &s->c,   &s->drho, &s->du,  &s->ax,      &s->ay,
                   &s->avx, &s->avy, &s->divv, &s->rho_min, &s->u_min, &s->vx_min,
-                  &s->vy_min};
+                  &s->vy_min, &s->h};
   for (size_t k = 0; k < sizeof(f) / sizeof(f[0]); ++k)
     *f[k] = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
   return s;
@@ -532,7 +537,7 @@ void orc_state_free(orc_state* s) {
   if (!s) return;
   double* f[] = {s->x,   s->y,   s->vx,  s->vy,   s->rho,     s->u,      s->p,
                  s->c,   s->drho, s->du, s->ax,   s->ay,      s->avx,    s->avy,
-                 s->divv, s->rho_min, s->u_min, s->vx_min, s->vy_min};
+                 s->divv, s->rho_min, s->u_min, s->vx_min, s->vy_min, s->h};
   for (size_t k = 0; k < sizeof(f) / sizeof(f[0]); ++k) free(f[k]);
   free(s);
 }
@@ -545,7 +550,7 @@ int orc_single_step(const orc_params* P, orc_state* s, int niac, const int* pi,
                     const int* pj) {
   /* single_step.f:92-234 (a pair list carries no periodic shift, and the
      reference has no conduction: both live in the gather twin only) */
-  if (P->flags & (ORC_CONDUCTION | ORC_PERIODIC_X)) return -2;
+  if ((P->flags & (ORC_CONDUCTION | ORC_PERIODIC_X)) || P->sle) return -2;
   const int n = s->n, np = s->np, nr = nrate_of(P, s);
   double* avdv_x = (double*)calloc((size_t)n, sizeof(double));
   double* avdv_y = (double*)calloc((size_t)n, sizeof(double));
@@ -657,10 +662,20 @@ static double dx_of(const orc_params* P, double xi, double xj, int sh) {
   return sh ? d + (double)sh * ((double)P->nx * P->cell) : d;
 }
 
+/* Smoothing length of pair (i, j): the uniform h, or with P->sle the mean
+   mh = (h_i + h_j)/2 of every reference pair routine (density.f:54,
+   internal_force.f:187, art_visc.f:64, art_heat.f:78, av_vel.f:46). */
+static inline double pair_h(const orc_params* P, const orc_state* s, int i, int j) {
+  return P->sle ? 0.5 * (s->h[i] + s->h[j]) : P->h;
+}
+
+static inline double own_h(const orc_params* P, const orc_state* s, int i) {
+  return P->sle ? s->h[i] : P->h;
+}
+
 int orc_single_step_gather(const orc_params* P, orc_state* s) {
   const int n = s->n, np = s->np, nr = nrate_of(P, s);
-  const double m = P->mass, h = P->h;
-  const double cut2 = (2.0 * P->h) * (2.0 * P->h);
+  const double m = P->mass;
   cells_t C;
   int bad = build_cells(P, s, &C);
   if (bad) {
@@ -673,12 +688,12 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
   const double* vx = s->vx;
   const double* vy = s->vy;
   /* pass A: continuity (or summation) */
-  double w0dx, w0dy;
-  const double w0 = kern(P, 0.0, 0.0, 0.0, &w0dx, &w0dy);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     const int key = C.key[i];
     const int cx = key % nx, cy = key / nx;
+    double w0dx, w0dy;
+    const double w0 = kern_h(P, own_h(P, s, i), 0.0, 0.0, 0.0, &w0dx, &w0dy);
     double dr = 0.0, rs = m * w0;
     for (int ry = cy - 1; ry <= cy + 1; ++ry) {
       if (ry < 0 || ry >= ny) continue;
@@ -691,9 +706,10 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
         const double dx = dx_of(P, x[i], x[j], sp[q].sh), dy = y[i] - y[j];
         double r2 = dx * dx;
         r2 = r2 + dy * dy;
-        if (!(r2 < cut2)) continue;
+        const double hij = pair_h(P, s, i, j);
+        if (!(r2 < (2.0 * hij) * (2.0 * hij))) continue;
         double dwx, dwy;
-        const double w = kern(P, sqrt(r2), dx, dy, &dwx, &dwy);
+        const double w = kern_h(P, hij, sqrt(r2), dx, dy, &dwx, &dwy);
         double vcc = (vx[i] - vx[j]) * dwx;
         vcc = vcc + (vy[i] - vy[j]) * dwy;
         dr = dr + m * vcc;
@@ -715,7 +731,8 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
   for (int i = 0; i < n; ++i) {
     const double dv = s->divv[i] / s->rho[i];
     s->divv[i] = dv;
-    psi[i] = P->ah_g1 * h * s->c[i] + P->ah_g2 * h * h * (fabs(dv) - dv);
+    const double hi = own_h(P, s, i);
+    psi[i] = P->ah_g1 * hi * s->c[i] + P->ah_g2 * hi * hi * (fabs(dv) - dv);
   }
   /* shear tensor (internal_force.f:73-142) with the post-EOS densities */
   const int visc = (P->flags & ORC_VISC) != 0;
@@ -738,9 +755,10 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
           const double dx = dx_of(P, x[i], x[j], sp[q].sh), dy = y[i] - y[j];
           double r2 = dx * dx;
           r2 = r2 + dy * dy;
-          if (!(r2 < cut2)) continue;
+          const double hij = pair_h(P, s, i, j);
+          if (!(r2 < (2.0 * hij) * (2.0 * hij))) continue;
           double dwx, dwy;
-          kern(P, sqrt(r2), dx, dy, &dwx, &dwy);
+          kern_h(P, hij, sqrt(r2), dx, dy, &dwx, &dwy);
           const double dvx = vx[j] - vx[i], dvy = vy[j] - vy[i];
           const double hxx = 2.0 / 3.0 * (2.0 * dvx * dwx - dvy * dwy);
           const double hyy = 2.0 / 3.0 * (2.0 * dvy * dwy - dvx * dwx);
@@ -764,14 +782,17 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
   const double* p = s->p;
   const double* c = s->c;
   const double* u = s->u;
-  const double av_e2 = (h * P->av_eta) * (h * P->av_eta);
-  const double ah_e2 = (h * P->ah_eta) * (h * P->ah_eta);
+  /* h update (hsml.f): dh/dt = (h/dim) sum m/rho_j (v_j - v_i).grad W, or
+     h = 2 (m/rho)^(1/dim); dim 2 for this step (3 reproduces the reference
+     as shipped, options.inc dim = 3) */
+  const double hdim = P->dim == 3 ? 3.0 : 2.0;
+  double* hnew = P->sle ? (double*)malloc(sizeof(double) * (size_t)n) : NULL;
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     const int key = C.key[i];
     const int cx = key % nx, cy = key / nx;
     double fx = 0.0, fy = 0.0, due = 0.0, avfx = 0.0, avfy = 0.0, avdu = 0.0, ahdu = 0.0;
-    double cdu = 0.0, xsx = 0.0, xsy = 0.0;
+    double cdu = 0.0, xsx = 0.0, xsy = 0.0, dvh = 0.0;
     const double pr_i = p[i] / (rho[i] * rho[i]);
     for (int ry = cy - 1; ry <= cy + 1; ++ry) {
       if (ry < 0 || ry >= ny) continue;
@@ -784,10 +805,18 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
         const double dx = dx_of(P, x[i], x[j], sp[q].sh), dy = y[i] - y[j];
         double r2 = dx * dx;
         r2 = r2 + dy * dy;
-        if (!(r2 < cut2)) continue;
+        const double h = pair_h(P, s, i, j);
+        if (!(r2 < (2.0 * h) * (2.0 * h))) continue;
+        const double av_e2 = (h * P->av_eta) * (h * P->av_eta);
+        const double ah_e2 = (h * P->ah_eta) * (h * P->ah_eta);
         double dwx, dwy;
-        const double w = kern(P, sqrt(r2), dx, dy, &dwx, &dwy);
+        const double w = kern_h(P, h, sqrt(r2), dx, dy, &dwx, &dwy);
         const double dvx = vx[i] - vx[j], dvy = vy[i] - vy[j];
+        if (P->sle == 2) {
+          double vdh = (vx[j] - vx[i]) * dwx;
+          vdh = vdh + (vy[j] - vy[i]) * dwy;
+          dvh = dvh + m / rho[j] * vdh;
+        }
         double vdw = dvx * dwx;
         vdw = vdw + dvy * dwy;
         double a;
@@ -863,6 +892,21 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
     if ((P->flags & ORC_HEAT_SOURCE) && i < np) s->du[i] += heat_source(P, x[i], y[i], rho[i]);
     s->avx[i] = xsx;
     s->avy[i] = xsy;
+    if (hnew) {
+      double hn = s->h[i];
+      if (i < np && P->sle == 1) {
+        hn = 2.0 * pow(m / rho[i], (double)(1.0f / (float)hdim));
+      } else if (i < np && P->sle == 2) {
+        const double dh = (s->h[i] / hdim) * dvh;
+        hn = s->h[i] + P->dt * dh;
+        if (hn <= 0.0) hn = hn - P->dt * dh;
+      }
+      hnew[i] = hn;
+    }
+  }
+  if (hnew) {
+    memcpy(s->h, hnew, sizeof(double) * (size_t)n);
+    free(hnew);
   }
   free(tt);
   free(psi);

@@ -107,6 +107,10 @@ typedef struct {
   double oz, gz;   /* 3D: grid origin in z, body force in z */
   int nz;          /* 3D: grid cells in z (key = ix + iy nx + iz nx ny) */
   double cond_k, cp; /* ORC_CONDUCTION: conductivity (W/m/K), heat capacity (J/kg/K) */
+  int sle;           /* smoothing-length evolution (hsml.f): 0 fixed h (s->h unused),
+                        1 h = 2 (m/rho)^(1/dim), 2 dh/dt = (h/dim) div v; with 1 or 2
+                        every pair uses mh = (h_i + h_j)/2 and real particles update
+                        s->h after the forces (gather twin only) */
 } orc_params;
 
 /* Particle state, SoA, FP64. */
@@ -117,6 +121,8 @@ typedef struct {
   double *drho, *du, *ax, *ay, *avx, *avy, *divv;
   /* leapfrog save arrays */
   double *rho_min, *u_min, *vx_min, *vy_min;
+  /* per-particle smoothing length (P->sle != 0) */
+  double* h;
 } orc_state;
 
 /* ---- kernel ---------------------------------------------------------- */

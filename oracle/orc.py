@@ -34,7 +34,7 @@ class Params(C.Structure):
         ("hs_q0", C.c_double), ("hs_x", C.c_double), ("hs_y", C.c_double),
         ("hs_sigma", C.c_double),
         ("dim", C.c_int), ("oz", C.c_double), ("gz", C.c_double), ("nz", C.c_int),
-        ("cond_k", C.c_double), ("cp", C.c_double),
+        ("cond_k", C.c_double), ("cp", C.c_double), ("sle", C.c_int),
     ]
 
 
@@ -42,7 +42,7 @@ class State(C.Structure):
     _fields_ = [("n", C.c_int), ("np", C.c_int)] + [
         (name, C.POINTER(C.c_double)) for name in (
             "x", "y", "vx", "vy", "rho", "u", "p", "c", "drho", "du", "ax", "ay",
-            "avx", "avy", "divv", "rho_min", "u_min", "vx_min", "vy_min")
+            "avx", "avy", "divv", "rho_min", "u_min", "vx_min", "vy_min", "h")
     ]
 
 
@@ -57,7 +57,7 @@ class State3(C.Structure):
 
 
 STATE_FIELDS = ("x", "y", "vx", "vy", "rho", "u", "p", "c", "drho", "du", "ax", "ay",
-                "avx", "avy", "divv", "rho_min", "u_min", "vx_min", "vy_min")
+                "avx", "avy", "divv", "rho_min", "u_min", "vx_min", "vy_min", "h")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")

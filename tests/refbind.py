@@ -66,11 +66,16 @@ def vec3(a, b, c=None):
     return X
 
 
+def hvec(h, n):
+    """Per-particle smoothing lengths from a scalar or an array."""
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(h, np.float64), (n,)))
+
+
 def link_list(x, y, h, z=None):
     def f():
         n = len(x)
         X = vec3(x, y, z)
-        H = np.full(n, h)
+        H = hvec(h, n)
         cap = n * 64 + 64
         pi = np.zeros(cap, np.int32)
         pj = np.zeros(cap, np.int32)
@@ -103,7 +108,7 @@ def pair_routines(x, y, vx, vy, rho, p, u, c, h, mass, pi, pj, z=None, vz=None):
         n = len(x)
         X = vec3(x, y, z)
         V = vec3(vx, vy, vz)
-        H = np.full(n, h)
+        H = hvec(h, n)
         M = np.full(n, mass)
         P1 = np.ascontiguousarray(pi + 1, np.int32)
         P2 = np.ascontiguousarray(pj + 1, np.int32)
@@ -194,4 +199,24 @@ def eos_poly(rho, u, rho0, a1, a2, a3, b0, b1, t1, t2, pmin):
             lib().eos_poly_(C.byref(r), C.byref(e), C.byref(out), *[C.byref(a) for a in args])
             p[k] = out.value
         return p
+    return run_big_stack(f)
+
+
+def h_upgrade(x, y, vx, vy, h, mass, rho, pi, pj, dt, sle):
+    """hsml.f h_upgrade over all particles (nt = n), cubic kernel (skf 0);
+    returns the new smoothing lengths.  The build has dim = 3: the update
+    is dh/dt = (h/3) div v, or h = 2 (m/rho)^(1/3)."""
+    def f():
+        n = len(x)
+        X = vec3(x, y)
+        V = vec3(vx, vy)
+        H = np.array(hvec(h, n))
+        M = np.full(n, mass)
+        R = np.ascontiguousarray(rho, np.float64)
+        P1 = np.ascontiguousarray(pi + 1, np.int32)
+        P2 = np.ascontiguousarray(pj + 1, np.int32)
+        d = C.c_double(dt)
+        lib().h_upgrade_(_i(n), _p(X), _p(V), _p(H), _p(M), _p(R), _i(len(pi)), _p(P1), _p(P2),
+                         C.byref(d), _i(sle), _i(0))
+        return H
     return run_big_stack(f)
