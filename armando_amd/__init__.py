@@ -35,7 +35,7 @@ EXPORTS = ("arm_create", "arm_destroy", "arm_last_error", "arm_domain_violations
            "arm_conservation", "arm_pass_times", "arm_reset_pass_times", "arm_set_timing",
            "arm_set_gather_path", "arm_adjoint_rollout", "arm_rollout_forward",
            "arm_rollout_backward", "arm_update_params", "arm_set_objective",
-           "arm_rollout_param_grads", "arm_conservation3")
+           "arm_rollout_param_grads", "arm_conservation3", "arm_upload_h", "arm_download_h")
 
 
 class ArmParams(C.Structure):
@@ -52,7 +52,7 @@ class ArmParams(C.Structure):
         ("hs_q0", C.c_double), ("hs_x", C.c_double), ("hs_y", C.c_double),
         ("hs_sigma", C.c_double),
         ("dim", C.c_int), ("oz", C.c_double), ("gz", C.c_double), ("nz", C.c_int),
-        ("cond_k", C.c_double), ("cp", C.c_double),
+        ("cond_k", C.c_double), ("cp", C.c_double), ("sle", C.c_int),
     ]
 
 
@@ -60,7 +60,7 @@ class ArmParams(C.Structure):
 # Tait EOS)
 PARAM_DEFAULTS = {"visc_eta": 0.0, "eos": 0, "eos_p0": 0.0, "hs_q0": 0.0, "hs_x": 0.0,
                   "hs_y": 0.0, "hs_sigma": 0.0, "dim": 2, "oz": 0.0, "gz": 0.0, "nz": 1,
-                  "cond_k": 0.0, "cp": 0.0}
+                  "cond_k": 0.0, "cp": 0.0, "sle": 0}
 OBJ_PRESSURE, OBJ_ENERGY = 0, 1
 
 
@@ -106,6 +106,8 @@ def load_library(path=LIB_PATH):
         "arm_time_integration": (C.c_int, [vp, C.c_int, C.c_int]),
         "arm_conservation": (C.c_int, [vp, _dp]),
         "arm_conservation3": (C.c_int, [vp, _dp]),
+        "arm_upload_h": (C.c_int, [vp, _dp]),
+        "arm_download_h": (C.c_int, [vp, _dp]),
         "arm_pass_times": (C.c_int, [vp, C.c_int, dpn, C.POINTER(C.c_int)]),
         "arm_reset_pass_times": (C.c_int, [vp]),
         "arm_set_timing": (C.c_int, [vp, C.c_int]),
@@ -257,6 +259,16 @@ class Simulation:
         """mass, momentum x, momentum y, kinetic energy, internal energy."""
         out = np.zeros(5)
         self._check(self.L.arm_conservation(self.ctx, out))
+        return out
+
+    def set_h(self, h):
+        """Per-particle smoothing lengths (sle != 0), original order."""
+        self._check(self.L.arm_upload_h(self.ctx, np.ascontiguousarray(h, np.float64)))
+
+    def h(self):
+        """Per-particle smoothing lengths (sle != 0), original order."""
+        out = np.zeros(self.n)
+        self._check(self.L.arm_download_h(self.ctx, out))
         return out
 
     def conservation3(self):

@@ -151,6 +151,8 @@ template <typename R> int alloc_state(arm_ctx* c, St<R>& S, int cap) {
   AL(v_min, cap);
   AL(flag, cap);
   AL(tau, 4 * (size_t)cap);
+  AL(hs, cap);
+  AL(hn, cap);
 #undef AL
   return ARM_OK;
 }
@@ -272,14 +274,18 @@ template <typename R> St<R>& stB(arm_ctx* c);
 template <> St<double>& stB<double>(arm_ctx* c) { return c->B64; }
 template <> St<float>& stB<float>(arm_ctx* c) { return c->B32; }
 
+// d_err[0]: particles outside the hash grid; d_err[1]: particles whose
+// evolving smoothing length outgrew the cell (2h > cell)
 int check_domain(arm_ctx* c) {
-  int bad = 0;
-  HIPCHK(c, hipMemcpyAsync(&bad, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  int bad[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(bad, c->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (bad) {
-    c->domain_bad = bad;
-    HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
-    return fail(c, ARM_ERR_DOMAIN, "%d particle(s) outside the hash grid", bad);
+  if (bad[0] || bad[1]) {
+    c->domain_bad = bad[0] + bad[1];
+    HIPCHK(c, hipMemsetAsync(c->d_err, 0, 2 * sizeof(int), c->stream));
+    if (bad[0]) return fail(c, ARM_ERR_DOMAIN, "%d particle(s) outside the hash grid", bad[0]);
+    return fail(c, ARM_ERR_DOMAIN, "%d particle(s) with 2h larger than the cell %g", bad[1],
+                c->K.cell);
   }
   return ARM_OK;
 }
@@ -321,7 +327,7 @@ template <typename R> int nseg_of(const arm_ctx* c) {
 template <typename R, bool SUM, bool EOS> void launch_density(arm_ctx* c) {
   St<R>& A = stA<R>(c);
   using T = TileCfg<R>;
-  if (c->tiled) {
+  if (c->tiled && !c->K.sle) {
     const int nseg = nseg_of<R>(c);
     hipLaunchKernelGGL((k_density_tiled<R, SUM, EOS, T::BLK, T::CW, T::CAP>),
                        dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
@@ -356,14 +362,14 @@ template <typename R, bool P, bool AV, bool AH, bool XS, bool VI = false>
 void launch_force(arm_ctx* c) {
   St<R>& A = stA<R>(c);
   using T = TileCfg<R>;
-  if (c->tiled) {
+  if (c->tiled && !c->K.sle) {
     const int nseg = nseg_of<R>(c);
     hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, T::BLK, T::CW, T::CAP>),
                        dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
                        c->cstart, nseg, c->masks_valid ? c->nbm : nullptr);
   } else {
     hipLaunchKernelGGL((k_force<R, P, AV, AH, XS, VI>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
-                       c->stream, A, c->K, c->skey, c->cstart);
+                       c->stream, A, c->K, c->skey, c->cstart, c->d_err);
   }
 }
 
@@ -371,7 +377,7 @@ void launch_force(arm_ctx* c) {
 template <typename R> void launch_tensor(arm_ctx* c) {
   St<R>& A = stA<R>(c);
   using T = TileCfg<R>;
-  if (c->tiled) {
+  if (c->tiled && !c->K.sle) {
     const int nseg = nseg_of<R>(c);
     hipLaunchKernelGGL((k_tensor_tiled<R, T::BLK, T::CW, T::CAP>), dim3(c->K.ny * nseg),
                        dim3(T::BLK), 0, c->stream, A, c->K, c->skey, c->cstart, nseg,
@@ -419,11 +425,18 @@ template <typename R> int zero_av(arm_ctx* c) {
   return ARM_OK;
 }
 
+// The smoothing lengths written by the force pass become the current ones
+// (hsml.f: h_upgrade closes single_step).
+template <typename R> void take_new_h(arm_ctx* c) {
+  if (c->K.sle) std::swap(stA<R>(c).hs, stA<R>(c).hn);
+}
+
 template <typename R> int single_step_t(arm_ctx* c) {
   int rc;
   if ((rc = density_t<R>(c, c->prm.density_method, true))) return rc;
   const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
   if ((rc = force_t<R>(c, true, c->prm.flags, xs))) return rc;
+  take_new_h<R>(c);
   if (!xs && (rc = zero_av<R>(c))) return rc;
   c->have_rates = true;
   return ARM_OK;
@@ -496,6 +509,7 @@ template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
     hipLaunchKernelGGL(k_objective<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, stA<R>(c),
                        c->K, hk.kind, hk.objmask, hk.jpart);
   if ((rc = force_t<R>(c, true, c->prm.flags, xs))) return rc;
+  take_new_h<R>(c);
   if (hk.tape) {
     const int nb = std::max(grid_of(n), grid_of(c->K.ncell + 1));
     hipLaunchKernelGGL(k_tape<R>, dim3(nb), dim3(BLOCK), 0, c->stream, stA<R>(c), c->K, c->skey,
@@ -942,6 +956,11 @@ template <typename R> int upload_t(arm_ctx* c, const double* x, const double* v,
   HIPCHK(c, hipMemsetAsync(A.u_min, 0, sizeof(double) * n, c->stream));
   HIPCHK(c, hipMemsetAsync(A.v_min, 0, sizeof(R2) * n, c->stream));
   HIPCHK(c, hipMemsetAsync(A.flag, 0, n, c->stream));
+  {
+    std::vector<double> hh(n, c->prm.h);
+    HIPCHK(c, cp(A.hs, hh.data(), sizeof(double) * n));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   // FP32 form: cell + offset from the FP64 positions (the keys kernel writes them)
   if (Layout<R>::f32) {
     hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
@@ -1714,6 +1733,9 @@ void set_kparams(KParams& K, const arm_params& P) {
   K.hs_x = P.hs_x;
   K.hs_y = P.hs_y;
   K.hs_k = P.hs_sigma > 0 ? 1.0 / (2.0 * P.hs_sigma * P.hs_sigma) : 0.0;
+  K.sle = P.sle;
+  K.av_eta = P.av_eta;
+  K.ah_eta = P.ah_eta;
   K.periodic = (P.flags & ARM_PERIODIC_X) ? 1 : 0;
   K.cond = (P.flags & ARM_CONDUCTION)
                ? 4.0 * P.cond_k * P.cond_k / (P.cond_k + P.cond_k) / P.cp
@@ -1767,6 +1789,10 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
   else if (P.dim == 3 && (P.flags & (ARM_VISC | ARM_HEAT_SOURCE | ARM_CONDUCTION | ARM_PERIODIC_X)))
     rc = fail(c, ARM_ERR_UNSUPPORTED,
               "ARM_VISC, ARM_HEAT_SOURCE, ARM_CONDUCTION and ARM_PERIODIC_X are 2D only");
+  else if (P.sle < 0 || P.sle > 2)
+    rc = fail(c, ARM_ERR_ARG, "sle must be 0, 1 or 2");
+  else if (P.dim == 3 && P.sle)
+    rc = fail(c, ARM_ERR_UNSUPPORTED, "the smoothing-length evolution is 2D only");
   else if ((P.flags & ARM_PERIODIC_X) && P.nx < 3)
     rc = fail(c, ARM_ERR_ARG, "a periodic grid needs nx >= 3 cells");
   else if ((P.flags & ARM_CONDUCTION) && !(P.cond_k > 0 && P.cp > 0))
@@ -1824,7 +1850,7 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     }
   }
   if (!rc) rc = dalloc(c, &c->cstart, (size_t)ncell + 1);
-  if (!rc) rc = dalloc(c, &c->d_err, 1);
+  if (!rc) rc = dalloc(c, &c->d_err, 2);
   if (!rc) rc = dalloc(c, &c->d_part, (size_t)grid_of(capacity) * 6);
   if (!rc) rc = dalloc(c, &c->d_cnt, capacity);
   if (!rc) rc = dalloc(c, &c->nbm, capacity);
@@ -1838,7 +1864,7 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     c->pack3_ap = q;
   }
   if (!rc) {
-    e = hipMemset(c->d_err, 0, sizeof(int));
+    e = hipMemset(c->d_err, 0, 2 * sizeof(int));
     if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipMemset: %s", hipGetErrorString(e));
   }
   if (!rc) {
@@ -1899,6 +1925,44 @@ int arm_download_rates(arm_ctx* c, double* drho, double* dvdt, double* dudt, dou
   if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
   if (is3d(c)) return DISPATCH(c, download_rates3_t)(c, drho, dvdt, dudt, av, divv);
   return DISPATCH(c, download_rates_t)(c, drho, dvdt, dudt, av, divv);
+}
+
+int arm_upload_h(arm_ctx* c, const double* h) {
+  if (!c || !h) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (!c->prm.sle || is3d(c)) return fail(c, ARM_ERR_STATE, "per-particle h needs sle = 1 or 2 (2D)");
+  const int n = c->n;
+  std::vector<int32_t> id(n);
+  const int32_t* did = is32(c) ? c->A32.id : c->A64.id;
+  double* dh = is32(c) ? c->A32.hs : c->A64.hs;
+  HIPCHK(c, hipMemcpyAsync(id.data(), did, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<double> hh(n);
+  for (int s = 0; s < n; ++s) {
+    const double v = h[id[s]];
+    if (!(v > 0) || !(2.0 * v <= c->prm.cell))
+      return fail(c, ARM_ERR_ARG, "h[%d] = %g outside (0, cell/2]", id[s], v);
+    hh[s] = v;
+  }
+  HIPCHK(c, hipMemcpyAsync(dh, hh.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ARM_OK;
+}
+
+int arm_download_h(arm_ctx* c, double* h) {
+  if (!c || !h) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (is3d(c)) return fail(c, ARM_ERR_UNSUPPORTED, "the smoothing-length evolution is 2D only");
+  const int n = c->n;
+  std::vector<int32_t> id(n);
+  std::vector<double> hh(n);
+  const int32_t* did = is32(c) ? c->A32.id : c->A64.id;
+  const double* dh = is32(c) ? c->A32.hs : c->A64.hs;
+  HIPCHK(c, hipMemcpyAsync(id.data(), did, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hh.data(), dh, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < n; ++s) h[id[s]] = hh[s];
+  return ARM_OK;
 }
 
 int arm_link_list(arm_ctx* c, int* niac) {
@@ -2092,6 +2156,7 @@ int arm_rollout_forward(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* o
     return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint differentiates continuity density only");
   if (c->prm.flags & (ARM_CONDUCTION | ARM_PERIODIC_X))
     return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint has no conduction or periodic boundary");
+  if (c->prm.sle) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint keeps h fixed (sle = 0)");
   const int ck = std::min(ckpt_every, nsteps);
   int rc;
   if (is3d(c))
@@ -2129,7 +2194,7 @@ int arm_update_params(arm_ctx* c, const arm_params* prm) {
   if (P.precision != O.precision || P.nx != O.nx || P.ny != O.ny || P.ox != O.ox ||
       P.oy != O.oy || P.cell != O.cell || P.h != O.h || P.flags != O.flags ||
       P.density_method != O.density_method || P.pa_sph != O.pa_sph || P.eos != O.eos ||
-      P.dim != O.dim || (P.dim == 3 && (P.nz != O.nz || P.oz != O.oz)))
+      P.dim != O.dim || (P.dim == 3 && (P.nz != O.nz || P.oz != O.oz)) || P.sle != O.sle)
     return fail(c, ARM_ERR_ARG, "only the physical scalars can change after arm_create");
   if ((P.flags & ARM_VISC) && !(P.visc_eta >= 0))
     return fail(c, ARM_ERR_ARG, "visc_eta must be non-negative");

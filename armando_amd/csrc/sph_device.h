@@ -43,7 +43,30 @@ struct KParams {
   double oz, gz;
   int periodic;  // flag 256: periodic in x with period nx * cell
   double cond;   // flag 128: 4 k k/(k + k) / cp, the conduction factor on u_i - u_j
+  // smoothing-length evolution (hsml.f): 0 fixed h; 1 h = 2 sqrt(m/rho);
+  // 2 dh/dt = (h/2) div v.  With 1 or 2 the state carries h per particle
+  // and every pair uses mh = (h_i + h_j)/2 (with_h below).
+  int sle;
+  double av_eta, ah_eta;
 };
+
+// K with the smoothing length mh: h, 1/h, the Wendland 2D constants, the
+// cut-off (2 mh)^2, the artificial viscosity and heat regularisations
+// (mh eta)^2 and the art_heat coefficient constants of mh.
+__device__ __forceinline__ KParams with_h(const KParams& K, double mh) {
+  KParams P = K;
+  P.h = mh;
+  P.invh = 1.0 / mh;
+  P.wnorm = 7.0 / (4.0 * M_PI * mh * mh);
+  P.C1 = 0.625 * P.wnorm / (mh * mh);
+  P.cut2 = (2.0 * mh) * (2.0 * mh);
+  P.cut2f = (float)P.cut2;
+  P.av_e2 = (mh * K.av_eta) * (mh * K.av_eta);
+  P.ah_e2 = (mh * K.ah_eta) * (mh * K.ah_eta);
+  P.psi0 = K.ah_g1 * mh * K.c0;
+  P.g2h2 = K.ah_g2 * mh * mh;
+  return P;
+}
 
 // Gaussian heat source factor exp(-k |x - c|^2) at a position.
 __device__ __forceinline__ double heat_shape(const KParams& K, double x, double y) {
@@ -81,6 +104,10 @@ template <typename R> struct St {
   // shear tensor and tds = eta/(2 rho) T:T; the force uses W = eta T
   // (pa_sph 0) or eta T / rho^2 (pa_sph 1)
   R* tau;
+  // per-particle smoothing length (KParams.sle != 0, FP64 in both forms) and
+  // the updated one written by the force pass
+  double* hs;
+  double* hn;
 };
 
 __device__ __forceinline__ bool is_rate(const KParams& K, int id) {

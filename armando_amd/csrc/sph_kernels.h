@@ -127,6 +127,7 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_all(St<R> A, St<R> Bs, KParam
   Bs.u_min[t] = A.u_min[s];
   Bs.v_min[t] = A.v_min[s];
   Bs.flag[t] = A.flag[s];
+  if (K.sle) Bs.hs[t] = A.hs[s];
 }
 
 // Fused reorder + leapfrog predictor (time_integration.f:136-154): the
@@ -147,6 +148,7 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
   Bs.id[t] = id;
   const double2 q = A.xy[s];
   Bs.xy[t] = q;
+  if (K.sle) Bs.hs[t] = A.hs[s];
   if constexpr (sizeof(R) == 4) {
     // cell + offset recomputed from the FP64 accumulator (as k_keys does)
     int ix, iy;
@@ -568,6 +570,42 @@ __device__ __forceinline__ void force_direct(St<R>& S, const KParams& K, int i,
   force_end<R, PRESS, XS, VI>(S, K, i, o);
 }
 
+// ---- per-particle smoothing length (K.sle != 0): the direct gathers with
+// the pair constants of mh = (h_i + h_j)/2 (with_h), the neighbour test
+// r^2 < (2 mh)^2, the self term and the art_heat coefficient of h_i.  No
+// masks are recorded (the force pass tests the candidates again).
+template <typename R, bool SUM, bool EOS>
+__device__ __forceinline__ void density_direct_h(St<R>& S, const KParams& K, int i,
+                                                 const uint32_t* __restrict__ skey,
+                                                 const int* __restrict__ cstart,
+                                                 uint4* __restrict__ nbm) {
+  const auto pi = load_pos(S, i);
+  const auto vi = S.v[i];
+  const double hi = S.hs[i];
+  const KParams Ki = with_h(K, hi);
+  const Rows rw = rows_of(K, skey[i], cstart);
+  DensOwn<R> o;
+  dens_begin<R, SUM>(Ki, o);
+  for (int k = 0; k < 3; ++k)
+    for (int s = 0; s < 3; ++s) {
+      int a, b, sh;
+      row_part(rw, k, s, a, b, sh);
+      for (int t = a; t < b; ++t) {
+        if (t == i) continue;
+        const auto pj = load_pos(S, t);
+        R dx, dy;
+        delta(pi, pj, K, dx, dy, sh);
+        const R r2 = dx * dx + dy * dy;
+        const KParams Kp = with_h(K, 0.5 * (hi + S.hs[t]));
+        if (!(r2 < cutoff2<R>(Kp))) continue;
+        const auto vj = S.v[t];
+        dens_pair<R, SUM>(Kp, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
+      }
+    }
+  if (nbm) nbm[i] = make_uint4(0u, 0u, 0u, 0u);
+  dens_end<R, SUM, EOS>(S, Ki, i, o);
+}
+
 template <typename R, bool SUM, bool EOS>
 __global__ __launch_bounds__(BLOCK) void k_density(St<R> S, KParams K,
                                                    const uint32_t* __restrict__ skey,
@@ -575,7 +613,10 @@ __global__ __launch_bounds__(BLOCK) void k_density(St<R> S, KParams K,
                                                    uint4* __restrict__ nbm) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
-  density_direct<R, SUM, EOS>(S, K, i, skey, cstart, nbm);
+  if (K.sle)
+    density_direct_h<R, SUM, EOS>(S, K, i, skey, cstart, nbm);
+  else
+    density_direct<R, SUM, EOS>(S, K, i, skey, cstart, nbm);
 }
 
 // Tait EOS (eos.f90:255-283) or the linear EOS on rate particles, then the
@@ -584,16 +625,87 @@ template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_eos(St<R> S, KParams K) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
-  eos_point(S, K, i, is_rate(K, S.id[i]), S.rho[i], S.divv[i]);
+  if (K.sle)
+    eos_point(S, with_h(K, S.hs[i]), i, is_rate(K, S.id[i]), S.rho[i], S.divv[i]);
+  else
+    eos_point(S, K, i, is_rate(K, S.id[i]), S.rho[i], S.divv[i]);
+}
+
+// The new smoothing length of particle i from the force pass
+// (hsml.f h_upgrade on real particles): sle 1 h = 2 sqrt(m/rho), sle 2
+// h += dt (h/2) dvh with dvh = sum m/rho_j (v_j - v_i).grad W; a step that
+// would make h non-positive is undone.  Counts in err[1] the particles whose
+// 2h outgrew the cell.
+template <typename R>
+__device__ __forceinline__ void h_update(St<R>& S, const KParams& K, int i, R dvh,
+                                         int* __restrict__ err) {
+  const double h = S.hs[i];
+  double hn = h;
+  if (S.id[i] < K.np) {
+    if (K.sle == 1) {
+      hn = 2.0 * sqrt(K.mass / (double)dens(K, S.rho[i]));
+    } else {
+      const double dh = (h / 2.0) * (double)dvh;
+      hn = h + K.dt * dh;
+      if (hn <= 0.0) hn = hn - K.dt * dh;
+    }
+  }
+  if (2.0 * hn > K.cell) atomicAdd(err + 1, 1);
+  S.hn[i] = hn;
+}
+
+template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI>
+__device__ __forceinline__ void force_direct_h(St<R>& S, const KParams& K, int i,
+                                               const uint32_t* __restrict__ skey,
+                                               const int* __restrict__ cstart,
+                                               int* __restrict__ err) {
+  const auto pi = load_pos(S, i);
+  const double hi = S.hs[i];
+  ForceOwn<R> o;
+  force_begin<R, VI>(S, K, i, o);
+  const Rows rw = rows_of(K, skey[i], cstart);
+  R dvh = 0;
+  for (int k = 0; k < 3; ++k)
+    for (int s = 0; s < 3; ++s) {
+      int a, b, sh;
+      row_part(rw, k, s, a, b, sh);
+      for (int t = a; t < b; ++t) {
+        if (t == i) continue;
+        const auto pj = load_pos(S, t);
+        R dx, dy;
+        delta(pi, pj, K, dx, dy, sh);
+        const R r2 = dx * dx + dy * dy;
+        const KParams Kp = with_h(K, 0.5 * (hi + S.hs[t]));
+        if (!(r2 < cutoff2<R>(Kp))) continue;
+        const auto vj = S.v[t];
+        R Qj, Aj;
+        q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
+        force_pair<R, PRESS, AV, AH, XS, VI>(Kp, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t],
+                                             u_of(S, t),
+                                             VI ? tau_of(S, t, visc_c(K, Qj)) : Tau3<R>{});
+        if (PRESS && K.sle == 2) {
+          const R tq = fsqrt(r2) * (R)Kp.invh - (R)2;
+          const R F = (R)Kp.C1 * (tq * tq * tq);
+          const R vr = (o.v.x - vj.x) * dx + (o.v.y - vj.y) * dy;
+          dvh = dvh - (R)K.mass * Qj * F * vr;
+        }
+      }
+    }
+  force_end<R, PRESS, XS, VI>(S, K, i, o);
+  if (PRESS) h_update(S, K, i, dvh, err);
 }
 
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI>
 __global__ __launch_bounds__(BLOCK) void k_force(St<R> S, KParams K,
                                                  const uint32_t* __restrict__ skey,
-                                                 const int* __restrict__ cstart) {
+                                                 const int* __restrict__ cstart,
+                                                 int* __restrict__ err) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
-  force_direct<R, PRESS, AV, AH, XS, VI>(S, K, i, skey, cstart);
+  if (K.sle)
+    force_direct_h<R, PRESS, AV, AH, XS, VI>(S, K, i, skey, cstart, err);
+  else
+    force_direct<R, PRESS, AV, AH, XS, VI>(S, K, i, skey, cstart);
 }
 
 // ------------------------------------------------------------------------
@@ -665,12 +777,43 @@ __device__ __forceinline__ void tensor_direct(St<R>& S, const KParams& K, int i,
 }
 
 template <typename R>
+__device__ __forceinline__ void tensor_direct_h(St<R>& S, const KParams& K, int i,
+                                                const uint32_t* __restrict__ skey,
+                                                const int* __restrict__ cstart) {
+  const auto pi = load_pos(S, i);
+  const auto vi = S.v[i];
+  const double hi = S.hs[i];
+  const Rows rw = rows_of(K, skey[i], cstart);
+  TensOwn<R> o;
+  for (int k = 0; k < 3; ++k)
+    for (int s = 0; s < 3; ++s) {
+      int a, b, sh;
+      row_part(rw, k, s, a, b, sh);
+      for (int t = a; t < b; ++t) {
+        if (t == i) continue;
+        const auto pj = load_pos(S, t);
+        R dx, dy;
+        delta(pi, pj, K, dx, dy, sh);
+        const R r2 = dx * dx + dy * dy;
+        const KParams Kp = with_h(K, 0.5 * (hi + S.hs[t]));
+        if (!(r2 < cutoff2<R>(Kp))) continue;
+        const auto vj = S.v[t];
+        tens_pair(Kp, o, dx, dy, r2, vj.x - vi.x, vj.y - vi.y, (R)1 / dens(K, S.rho[t]));
+      }
+    }
+  tens_end(S, K, i, (R)1 / dens(K, S.rho[i]), o);
+}
+
+template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_tensor(St<R> S, KParams K,
                                                   const uint32_t* __restrict__ skey,
                                                   const int* __restrict__ cstart) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
-  tensor_direct(S, K, i, skey, cstart);
+  if (K.sle)
+    tensor_direct_h(S, K, i, skey, cstart);
+  else
+    tensor_direct(S, K, i, skey, cstart);
 }
 
 // ------------------------------------------------------------------------
@@ -799,7 +942,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair_count(St<R> S, KParams K,
         R dx, dy;
         delta(pi, pj, K, dx, dy, sh);
         const R r2 = dx * dx + dy * dy;
-        if (!(r2 < cut2)) continue;
+        const R cut = K.sle ? cutoff2<R>(with_h(K, 0.5 * (S.hs[i] + S.hs[t]))) : cut2;
+        if (!(r2 < cut)) continue;
         if (offs) {
           out_i[o + c] = idi;
           out_j[o + c] = idj;

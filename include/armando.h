@@ -108,6 +108,11 @@ typedef struct arm_params {
   double oz, gz;
   int nz;
   double cond_k, cp; /* ARM_CONDUCTION: conductivity (W/m/K), heat capacity (J/kg/K) */
+  int sle; /* smoothing-length evolution [CERNsource/hsml.f:1-76, called at the end of
+              single_step.f:225]: 0 fixed h; 1 h = 2 sqrt(m/rho); 2 dh/dt = (h/dim) div v.
+              With 1 or 2 (2D only) h is per particle (initially prm.h, or
+              arm_upload_h), every pair uses mh = (h_i + h_j)/2 and the direct
+              gathers run; 2h must stay within the cell (ARM_ERR_DOMAIN otherwise). */
 } arm_params;
 
 typedef struct arm_ctx arm_ctx;
@@ -130,6 +135,12 @@ int arm_download(arm_ctx* ctx, double* x, double* v, double* rho, double* p, dou
    dvdt and av are (dim, n); divv is the velocity divergence of art_heat. */
 int arm_download_rates(arm_ctx* ctx, double* drho, double* dvdt, double* dudt, double* av,
                        double* divv);
+
+/* Per-particle smoothing lengths h(n) in the original order (sle != 0), the
+   reference's h array [CERNsource/time_integration.f:62]: set after
+   arm_upload, read back after steps. */
+int arm_upload_h(arm_ctx* ctx, const double* h);
+int arm_download_h(arm_ctx* ctx, double* h);
 
 /* link_list(nt, x, h, niac, pair_i, pair_j, skf) [CERNsource/link_list.f:1]:
    hashes, sorts by (cell, particle) and builds cell ranges on the device.
