@@ -329,9 +329,14 @@ template <typename R, bool SUM, bool EOS> void launch_density(arm_ctx* c) {
   using T = TileCfg<R>;
   if (c->tiled && !c->K.sle) {
     const int nseg = nseg_of<R>(c);
-    hipLaunchKernelGGL((k_density_tiled<R, SUM, EOS, T::BLK, T::CW, T::CAP>),
-                       dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
-                       c->cstart, nseg, c->nbm);
+    if (c->K.periodic)
+      hipLaunchKernelGGL((k_density_tiled<R, SUM, EOS, true, T::BLK, T::CW, T::CAP>),
+                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
+                         c->cstart, nseg, c->nbm);
+    else
+      hipLaunchKernelGGL((k_density_tiled<R, SUM, EOS, false, T::BLK, T::CW, T::CAP>),
+                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
+                         c->cstart, nseg, c->nbm);
   } else {
     hipLaunchKernelGGL((k_density<R, SUM, EOS>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
                        c->stream, A, c->K, c->skey, c->cstart, c->nbm);
@@ -364,9 +369,15 @@ void launch_force(arm_ctx* c) {
   using T = TileCfg<R>;
   if (c->tiled && !c->K.sle) {
     const int nseg = nseg_of<R>(c);
-    hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, T::BLK, T::CW, T::CAP>),
-                       dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
-                       c->cstart, nseg, c->masks_valid ? c->nbm : nullptr);
+    const uint4* nbm = c->masks_valid ? c->nbm : nullptr;
+    if (c->K.periodic)
+      hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, true, T::BLK, T::CW, T::CAP>),
+                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
+                         c->cstart, nseg, nbm);
+    else
+      hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, false, T::BLK, T::CW, T::CAP>),
+                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
+                         c->cstart, nseg, nbm);
   } else {
     hipLaunchKernelGGL((k_force<R, P, AV, AH, XS, VI>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
                        c->stream, A, c->K, c->skey, c->cstart, c->d_err);
@@ -379,9 +390,15 @@ template <typename R> void launch_tensor(arm_ctx* c) {
   using T = TileCfg<R>;
   if (c->tiled && !c->K.sle) {
     const int nseg = nseg_of<R>(c);
-    hipLaunchKernelGGL((k_tensor_tiled<R, T::BLK, T::CW, T::CAP>), dim3(c->K.ny * nseg),
-                       dim3(T::BLK), 0, c->stream, A, c->K, c->skey, c->cstart, nseg,
-                       c->masks_valid ? c->nbm : nullptr);
+    const uint4* nbm = c->masks_valid ? c->nbm : nullptr;
+    if (c->K.periodic)
+      hipLaunchKernelGGL((k_tensor_tiled<R, true, T::BLK, T::CW, T::CAP>),
+                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
+                         c->cstart, nseg, nbm);
+    else
+      hipLaunchKernelGGL((k_tensor_tiled<R, false, T::BLK, T::CW, T::CAP>),
+                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
+                         c->cstart, nseg, nbm);
   } else {
     hipLaunchKernelGGL(k_tensor<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
                        c->skey, c->cstart);

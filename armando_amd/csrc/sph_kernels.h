@@ -197,58 +197,74 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
 // ------------------------------------------------------------------------
 // neighbour walk helpers
 // ------------------------------------------------------------------------
-// Candidate ranges of the three rows.  On a grid periodic in x the first
-// and last columns see the other end: the wrapped cell is a range of its
-// own, visited where its image lies (first for cx = 0, last for cx = nx-1),
-// with the shift wsh = +nx or -nx cells; wsh = 0 otherwise.
+// Candidate ranges of the three rows cy-1 .. cy+1 over the x-adjacent cells.
 struct Rows {
   int t0[3], t1[3];
-  int w0[3], w1[3];
-  int wsh;
 };
 
 __device__ __forceinline__ Rows rows_of(const KParams& K, uint32_t key,
                                         const int* __restrict__ cstart) {
   const int cy = (int)key / K.nx;
   const int cx = (int)key - cy * K.nx;
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx + 1 < K.nx ? cx + 1 : K.nx - 1;
   Rows r;
-  r.wsh = 0;
-  int x0 = cx > 0 ? cx - 1 : 0;
-  int x1 = cx + 1 < K.nx ? cx + 1 : K.nx - 1;
-  int wc = 0;
-  if (K.periodic && cx == 0) {
-    r.wsh = K.nx;
-    wc = K.nx - 1;
-  } else if (K.periodic && cx == K.nx - 1) {
-    r.wsh = -K.nx;
-    wc = 0;
-  }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int ry = cy - 1 + k;
-    r.w0[k] = 0;
-    r.w1[k] = 0;
     if (ry < 0 || ry >= K.ny) {
       r.t0[k] = 0;
       r.t1[k] = 0;
     } else {
       r.t0[k] = cstart[ry * K.nx + x0];
       r.t1[k] = cstart[ry * K.nx + x1 + 1];
-      if (r.wsh) {
-        r.w0[k] = cstart[ry * K.nx + wc];
-        r.w1[k] = cstart[ry * K.nx + wc + 1];
-      }
     }
   }
   return r;
 }
 
-// Part s (0, 1, 2) of row k in visiting order: [a, b) with shift sh.
-__device__ __forceinline__ void row_part(const Rows& r, int k, int s, int& a, int& b, int& sh) {
-  const bool wrapped = (s == 0 && r.wsh > 0) || (s == 2 && r.wsh < 0);
-  a = s == 1 ? r.t0[k] : wrapped ? r.w0[k] : 0;
-  b = s == 1 ? r.t1[k] : wrapped ? r.w1[k] : 0;
-  sh = wrapped ? r.wsh : 0;
+// On a grid periodic in x the first and last columns see the other end:
+// the wrapped column's ranges of the same three rows and the shift of its
+// image, sh = +nx cells for cx = 0 and -nx for cx = nx-1 (0 elsewhere).
+__device__ __forceinline__ Rows wrap_rows(const KParams& K, uint32_t key,
+                                          const int* __restrict__ cstart, int& sh) {
+  const int cy = (int)key / K.nx;
+  const int cx = (int)key - cy * K.nx;
+  sh = cx == 0 ? K.nx : cx == K.nx - 1 ? -K.nx : 0;
+  const int wc = cx == 0 ? K.nx - 1 : 0;
+  Rows r;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int ry = cy - 1 + k;
+    if (sh == 0 || ry < 0 || ry >= K.ny) {
+      r.t0[k] = 0;
+      r.t1[k] = 0;
+    } else {
+      r.t0[k] = cstart[ry * K.nx + wc];
+      r.t1[k] = cstart[ry * K.nx + wc + 1];
+    }
+  }
+  return r;
+}
+
+// Every candidate of the cell of `key`: f(t, sh, k, t0) for candidate t of
+// row k (t0 = the row's first candidate), first the unwrapped rows (sh = 0),
+// then, with periodic x, the wrapped column.  The two phases keep the
+// unwrapped walk free of the wrap's row bounds; WRAP = false compiles the
+// wrap out (callers that never see a periodic grid).
+template <bool WRAP = true, typename F>
+__device__ __forceinline__ void for_candidates(const KParams& K, uint32_t key,
+                                               const int* __restrict__ cstart, F&& f) {
+  const Rows rw = rows_of(K, key, cstart);
+  for (int k = 0; k < 3; ++k)
+    for (int t = rw.t0[k]; t < rw.t1[k]; ++t) f(t, 0, k, rw.t0[k]);
+  if (WRAP && K.periodic) {
+    int sh;
+    const Rows w = wrap_rows(K, key, cstart, sh);
+    if (sh)
+      for (int k = 0; k < 3; ++k)
+        for (int t = w.t0[k]; t < w.t1[k]; ++t) f(t, sh, k, w.t0[k]);
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -504,69 +520,56 @@ template <typename R> __device__ __forceinline__ R u_of(const St<R>& S, int t) {
 // Also records the neighbour masks of the three rows (bit b = candidate
 // t0 + b) for the force pass; w = 0 marks them incomplete (a row longer than
 // 32 candidates).
-template <typename R, bool SUM, bool EOS = false>
+template <typename R, bool SUM, bool EOS = false, bool WRAP = true>
 __device__ __forceinline__ void density_direct(St<R>& S, const KParams& K, int i,
                                                const uint32_t* __restrict__ skey,
                                                const int* __restrict__ cstart,
                                                uint4* __restrict__ nbm) {
   const auto pi = load_pos(S, i);
   const auto vi = S.v[i];
-  const Rows rw = rows_of(K, skey[i], cstart);
   const R cut2 = cutoff2<R>(K);
   DensOwn<R> o;
   dens_begin<R, SUM>(K, o);
   uint32_t rm[3] = {0u, 0u, 0u};
-  bool ok = rw.wsh == 0;  // masks cover the unwrapped rows only
-  for (int k = 0; k < 3; ++k) {
-    if (rw.t1[k] - rw.t0[k] > 32) ok = false;
-    for (int s = 0; s < 3; ++s) {
-      int a, b, sh;
-      row_part(rw, k, s, a, b, sh);
-      for (int t = a; t < b; ++t) {
-        if (t == i) continue;
-        const auto pj = load_pos(S, t);
-        R dx, dy;
-        delta(pi, pj, K, dx, dy, sh);
-        const R r2 = dx * dx + dy * dy;
-        if (!(r2 < cut2)) continue;
-        if (sh == 0 && t - rw.t0[k] < 32) rm[k] |= 1u << (t - rw.t0[k]);
-        const auto vj = S.v[t];
-        dens_pair<R, SUM>(K, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
-      }
-    }
-  }
+  bool ok = true;  // masks cover the unwrapped rows of up to 32 candidates
+  for_candidates<WRAP>(K, skey[i], cstart, [&](int t, int sh, int k, int t0) {
+    if (sh != 0 || t - t0 >= 32) ok = false;
+    if (t == i) return;
+    const auto pj = load_pos(S, t);
+    R dx, dy;
+    delta(pi, pj, K, dx, dy, sh);
+    const R r2 = dx * dx + dy * dy;
+    if (!(r2 < cut2)) return;
+    if (sh == 0 && t - t0 < 32) rm[k] |= 1u << (t - t0);
+    const auto vj = S.v[t];
+    dens_pair<R, SUM>(K, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
+  });
   if (nbm) nbm[i] = make_uint4(rm[0], rm[1], rm[2], ok ? 1u : 0u);
   dens_end<R, SUM, EOS>(S, K, i, o);
 }
 
-template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI = false>
+template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI = false, bool WRAP = true>
 __device__ __forceinline__ void force_direct(St<R>& S, const KParams& K, int i,
                                              const uint32_t* __restrict__ skey,
                                              const int* __restrict__ cstart) {
   const auto pi = load_pos(S, i);
   ForceOwn<R> o;
   force_begin<R, VI>(S, K, i, o);
-  const Rows rw = rows_of(K, skey[i], cstart);
   const R cut2 = cutoff2<R>(K);
-  for (int k = 0; k < 3; ++k)
-    for (int s = 0; s < 3; ++s) {
-      int a, b, sh;
-      row_part(rw, k, s, a, b, sh);
-      for (int t = a; t < b; ++t) {
-        if (t == i) continue;
-        const auto pj = load_pos(S, t);
-        R dx, dy;
-        delta(pi, pj, K, dx, dy, sh);
-        const R r2 = dx * dx + dy * dy;
-        if (!(r2 < cut2)) continue;
-        const auto vj = S.v[t];
-        R Qj, Aj;
-        q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
-        force_pair<R, PRESS, AV, AH, XS, VI>(K, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t],
-                                             u_of(S, t),
-                                             VI ? tau_of(S, t, visc_c(K, Qj)) : Tau3<R>{});
-      }
-    }
+  for_candidates<WRAP>(K, skey[i], cstart, [&](int t, int sh, int, int) {
+    if (t == i) return;
+    const auto pj = load_pos(S, t);
+    R dx, dy;
+    delta(pi, pj, K, dx, dy, sh);
+    const R r2 = dx * dx + dy * dy;
+    if (!(r2 < cut2)) return;
+    const auto vj = S.v[t];
+    R Qj, Aj;
+    q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
+    force_pair<R, PRESS, AV, AH, XS, VI>(K, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t],
+                                         u_of(S, t),
+                                         VI ? tau_of(S, t, visc_c(K, Qj)) : Tau3<R>{});
+  });
   force_end<R, PRESS, XS, VI>(S, K, i, o);
 }
 
@@ -583,25 +586,19 @@ __device__ __forceinline__ void density_direct_h(St<R>& S, const KParams& K, int
   const auto vi = S.v[i];
   const double hi = S.hs[i];
   const KParams Ki = with_h(K, hi);
-  const Rows rw = rows_of(K, skey[i], cstart);
   DensOwn<R> o;
   dens_begin<R, SUM>(Ki, o);
-  for (int k = 0; k < 3; ++k)
-    for (int s = 0; s < 3; ++s) {
-      int a, b, sh;
-      row_part(rw, k, s, a, b, sh);
-      for (int t = a; t < b; ++t) {
-        if (t == i) continue;
-        const auto pj = load_pos(S, t);
-        R dx, dy;
-        delta(pi, pj, K, dx, dy, sh);
-        const R r2 = dx * dx + dy * dy;
-        const KParams Kp = with_h(K, 0.5 * (hi + S.hs[t]));
-        if (!(r2 < cutoff2<R>(Kp))) continue;
-        const auto vj = S.v[t];
-        dens_pair<R, SUM>(Kp, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
-      }
-    }
+  for_candidates(K, skey[i], cstart, [&](int t, int sh, int, int) {
+    if (t == i) return;
+    const auto pj = load_pos(S, t);
+    R dx, dy;
+    delta(pi, pj, K, dx, dy, sh);
+    const R r2 = dx * dx + dy * dy;
+    const KParams Kp = with_h(K, 0.5 * (hi + S.hs[t]));
+    if (!(r2 < cutoff2<R>(Kp))) return;
+    const auto vj = S.v[t];
+    dens_pair<R, SUM>(Kp, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
+  });
   if (nbm) nbm[i] = make_uint4(0u, 0u, 0u, 0u);
   dens_end<R, SUM, EOS>(S, Ki, i, o);
 }
@@ -663,34 +660,28 @@ __device__ __forceinline__ void force_direct_h(St<R>& S, const KParams& K, int i
   const double hi = S.hs[i];
   ForceOwn<R> o;
   force_begin<R, VI>(S, K, i, o);
-  const Rows rw = rows_of(K, skey[i], cstart);
   R dvh = 0;
-  for (int k = 0; k < 3; ++k)
-    for (int s = 0; s < 3; ++s) {
-      int a, b, sh;
-      row_part(rw, k, s, a, b, sh);
-      for (int t = a; t < b; ++t) {
-        if (t == i) continue;
-        const auto pj = load_pos(S, t);
-        R dx, dy;
-        delta(pi, pj, K, dx, dy, sh);
-        const R r2 = dx * dx + dy * dy;
-        const KParams Kp = with_h(K, 0.5 * (hi + S.hs[t]));
-        if (!(r2 < cutoff2<R>(Kp))) continue;
-        const auto vj = S.v[t];
-        R Qj, Aj;
-        q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
-        force_pair<R, PRESS, AV, AH, XS, VI>(Kp, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t],
-                                             u_of(S, t),
-                                             VI ? tau_of(S, t, visc_c(K, Qj)) : Tau3<R>{});
-        if (PRESS && K.sle == 2) {
-          const R tq = fsqrt(r2) * (R)Kp.invh - (R)2;
-          const R F = (R)Kp.C1 * (tq * tq * tq);
-          const R vr = (o.v.x - vj.x) * dx + (o.v.y - vj.y) * dy;
-          dvh = dvh - (R)K.mass * Qj * F * vr;
-        }
-      }
+  for_candidates(K, skey[i], cstart, [&](int t, int sh, int, int) {
+    if (t == i) return;
+    const auto pj = load_pos(S, t);
+    R dx, dy;
+    delta(pi, pj, K, dx, dy, sh);
+    const R r2 = dx * dx + dy * dy;
+    const KParams Kp = with_h(K, 0.5 * (hi + S.hs[t]));
+    if (!(r2 < cutoff2<R>(Kp))) return;
+    const auto vj = S.v[t];
+    R Qj, Aj;
+    q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
+    force_pair<R, PRESS, AV, AH, XS, VI>(Kp, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t],
+                                         u_of(S, t),
+                                         VI ? tau_of(S, t, visc_c(K, Qj)) : Tau3<R>{});
+    if (PRESS && K.sle == 2) {
+      const R tq = fsqrt(r2) * (R)Kp.invh - (R)2;
+      const R F = (R)Kp.C1 * (tq * tq * tq);
+      const R vr = (o.v.x - vj.x) * dx + (o.v.y - vj.y) * dy;
+      dvh = dvh - (R)K.mass * Qj * F * vr;
     }
+  });
   force_end<R, PRESS, XS, VI>(S, K, i, o);
   if (PRESS) h_update(S, K, i, dvh, err);
 }
@@ -749,30 +740,24 @@ __device__ __forceinline__ void tens_end(St<R>& S, const KParams& K, int i, R Qi
   w[3] = (R)0.5 * eta * Qi * tt;
 }
 
-template <typename R>
+template <typename R, bool WRAP = true>
 __device__ __forceinline__ void tensor_direct(St<R>& S, const KParams& K, int i,
                                               const uint32_t* __restrict__ skey,
                                               const int* __restrict__ cstart) {
   const auto pi = load_pos(S, i);
   const auto vi = S.v[i];
-  const Rows rw = rows_of(K, skey[i], cstart);
   const R cut2 = cutoff2<R>(K);
   TensOwn<R> o;
-  for (int k = 0; k < 3; ++k)
-    for (int s = 0; s < 3; ++s) {
-      int a, b, sh;
-      row_part(rw, k, s, a, b, sh);
-      for (int t = a; t < b; ++t) {
-        if (t == i) continue;
-        const auto pj = load_pos(S, t);
-        R dx, dy;
-        delta(pi, pj, K, dx, dy, sh);
-        const R r2 = dx * dx + dy * dy;
-        if (!(r2 < cut2)) continue;
-        const auto vj = S.v[t];
-        tens_pair(K, o, dx, dy, r2, vj.x - vi.x, vj.y - vi.y, (R)1 / dens(K, S.rho[t]));
-      }
-    }
+  for_candidates<WRAP>(K, skey[i], cstart, [&](int t, int sh, int, int) {
+    if (t == i) return;
+    const auto pj = load_pos(S, t);
+    R dx, dy;
+    delta(pi, pj, K, dx, dy, sh);
+    const R r2 = dx * dx + dy * dy;
+    if (!(r2 < cut2)) return;
+    const auto vj = S.v[t];
+    tens_pair(K, o, dx, dy, r2, vj.x - vi.x, vj.y - vi.y, (R)1 / dens(K, S.rho[t]));
+  });
   tens_end(S, K, i, (R)1 / dens(K, S.rho[i]), o);
 }
 
@@ -783,24 +768,18 @@ __device__ __forceinline__ void tensor_direct_h(St<R>& S, const KParams& K, int 
   const auto pi = load_pos(S, i);
   const auto vi = S.v[i];
   const double hi = S.hs[i];
-  const Rows rw = rows_of(K, skey[i], cstart);
   TensOwn<R> o;
-  for (int k = 0; k < 3; ++k)
-    for (int s = 0; s < 3; ++s) {
-      int a, b, sh;
-      row_part(rw, k, s, a, b, sh);
-      for (int t = a; t < b; ++t) {
-        if (t == i) continue;
-        const auto pj = load_pos(S, t);
-        R dx, dy;
-        delta(pi, pj, K, dx, dy, sh);
-        const R r2 = dx * dx + dy * dy;
-        const KParams Kp = with_h(K, 0.5 * (hi + S.hs[t]));
-        if (!(r2 < cutoff2<R>(Kp))) continue;
-        const auto vj = S.v[t];
-        tens_pair(Kp, o, dx, dy, r2, vj.x - vi.x, vj.y - vi.y, (R)1 / dens(K, S.rho[t]));
-      }
-    }
+  for_candidates(K, skey[i], cstart, [&](int t, int sh, int, int) {
+    if (t == i) return;
+    const auto pj = load_pos(S, t);
+    R dx, dy;
+    delta(pi, pj, K, dx, dy, sh);
+    const R r2 = dx * dx + dy * dy;
+    const KParams Kp = with_h(K, 0.5 * (hi + S.hs[t]));
+    if (!(r2 < cutoff2<R>(Kp))) return;
+    const auto vj = S.v[t];
+    tens_pair(Kp, o, dx, dy, r2, vj.x - vi.x, vj.y - vi.y, (R)1 / dens(K, S.rho[t]));
+  });
   tens_end(S, K, i, (R)1 / dens(K, S.rho[i]), o);
 }
 
@@ -927,30 +906,24 @@ __global__ __launch_bounds__(BLOCK) void k_pair_count(St<R> S, KParams K,
   if (i >= K.n) return;
   const auto pi = load_pos(S, i);
   const int idi = S.id[i];
-  const Rows rw = rows_of(K, skey[i], cstart);
   const R cut2 = cutoff2<R>(K);
   int c = 0;
   int o = offs ? offs[i] : 0;
-  for (int k = 0; k < 3; ++k)
-    for (int s = 0; s < 3; ++s) {
-      int a, b, sh;
-      row_part(rw, k, s, a, b, sh);
-      for (int t = a; t < b; ++t) {
-        const int idj = S.id[t];
-        if (idj <= idi) continue;
-        const auto pj = load_pos(S, t);
-        R dx, dy;
-        delta(pi, pj, K, dx, dy, sh);
-        const R r2 = dx * dx + dy * dy;
-        const R cut = K.sle ? cutoff2<R>(with_h(K, 0.5 * (S.hs[i] + S.hs[t]))) : cut2;
-        if (!(r2 < cut)) continue;
-        if (offs) {
-          out_i[o + c] = idi;
-          out_j[o + c] = idj;
-        }
-        ++c;
-      }
+  for_candidates(K, skey[i], cstart, [&](int t, int sh, int, int) {
+    const int idj = S.id[t];
+    if (idj <= idi) return;
+    const auto pj = load_pos(S, t);
+    R dx, dy;
+    delta(pi, pj, K, dx, dy, sh);
+    const R r2 = dx * dx + dy * dy;
+    const R cut = K.sle ? cutoff2<R>(with_h(K, 0.5 * (S.hs[i] + S.hs[t]))) : cut2;
+    if (!(r2 < cut)) return;
+    if (offs) {
+      out_i[o + c] = idi;
+      out_j[o + c] = idj;
     }
+    ++c;
+  });
   if (cnt) cnt[i] = c;
 }
 

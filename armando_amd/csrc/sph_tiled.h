@@ -263,10 +263,11 @@ __device__ __forceinline__ Segment<CW> segment_of(const KParams& K, const int* _
 
 // A particle of the first or last cell column of a grid periodic in x: its
 // neighbours wrap to the other end, outside the segment's tile, so the
-// direct path (which walks the wrapped cells) takes it.
-template <int CW>
+// direct path (which walks the wrapped cells) takes it.  PER = false: the
+// kernel was launched for a grid without periodic x.
+template <bool PER, int CW>
 __device__ __forceinline__ bool seam(const KParams& K, const Segment<CW>& g, uint32_t key) {
-  if (!K.periodic) return false;
+  if (!PER) return false;
   const int cx = (int)key - g.cy * K.nx;
   return cx == 0 || cx == K.nx - 1;
 }
@@ -354,7 +355,7 @@ __device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& 
   }
 }
 
-template <typename R, bool SUM, bool EOS, int BLK, int CW, int CAP>
+template <typename R, bool SUM, bool EOS, bool PER, int BLK, int CW, int CAP>
 __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
                                                        const uint32_t* __restrict__ skey,
                                                        const int* __restrict__ cstart, int nseg,
@@ -373,8 +374,8 @@ __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
   for (int i0 = g.O0; i0 < g.O1; i0 += BLK) {
     const int i = i0 + threadIdx.x;
     if (i >= g.O1) continue;
-    if (!tiled || seam(K, g, skey[i])) {
-      density_direct<R, SUM, EOS>(S, K, i, skey, cstart, nbm);
+    if (!tiled || seam<PER>(K, g, skey[i])) {
+      density_direct<R, SUM, EOS, PER>(S, K, i, skey, cstart, nbm);
       continue;
     }
     const int self = g.lo[1] + (i - g.rs[1]);
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
   }
 }
 
-template <typename R, int BLK, int CW, int CAP>
+template <typename R, bool PER, int BLK, int CW, int CAP>
 __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
                                                       const uint32_t* __restrict__ skey,
                                                       const int* __restrict__ cstart, int nseg,
@@ -418,8 +419,8 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
   for (int i0 = g.O0; i0 < g.O1; i0 += BLK) {
     const int i = i0 + threadIdx.x;
     if (i >= g.O1) continue;
-    if (!tiled || seam(K, g, skey[i])) {
-      tensor_direct(S, K, i, skey, cstart);
+    if (!tiled || seam<PER>(K, g, skey[i])) {
+      tensor_direct<R, PER>(S, K, i, skey, cstart);
       continue;
     }
     const int self = g.lo[1] + (i - g.rs[1]);
@@ -443,7 +444,8 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
   }
 }
 
-template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI, int BLK, int CW, int CAP>
+template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI, bool PER, int BLK, int CW,
+          int CAP>
 __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
                                                      const uint32_t* __restrict__ skey,
                                                      const int* __restrict__ cstart, int nseg,
@@ -462,8 +464,8 @@ __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
   for (int i0 = g.O0; i0 < g.O1; i0 += BLK) {
     const int i = i0 + threadIdx.x;
     if (i >= g.O1) continue;
-    if (!tiled || seam(K, g, skey[i])) {
-      force_direct<R, PRESS, AV, AH, XS, VI>(S, K, i, skey, cstart);
+    if (!tiled || seam<PER>(K, g, skey[i])) {
+      force_direct<R, PRESS, AV, AH, XS, VI, PER>(S, K, i, skey, cstart);
       continue;
     }
     const int self = g.lo[1] + (i - g.rs[1]);
