@@ -17,7 +17,7 @@ import numpy as np
 
 # flags (include/armando.h)
 ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
-CONDUCTION, PERIODIC_X = 128, 256
+CONDUCTION, PERIODIC_X, GHOSTS = 128, 256, 512
 EOS_TAIT, EOS_LINEAR = 0, 1
 FP64, FP32 = 0, 1
 
@@ -140,6 +140,59 @@ def dam_break_small(scale=0.05, precision=FP64, seed=12345, jitter=0.01, dx=1e-3
     """Same physics at a reduced size: column/tank dimensions times `scale`."""
     return dam_break(col_w=0.6 * scale, col_h=1.2 * scale, tank_w=3.2 * scale,
                      tank_h=1.6 * scale, dx=dx, precision=precision, seed=seed, jitter=jitter)
+
+
+def ghost_tank(col_w=0.03, col_h=0.06, tank_w=0.16, dx=1e-3, jitter=0.01, seed=12345,
+               h_fac=1.3, spacing=0.5, precision=FP64, alpha=0.01, g=9.81, cfl=0.2):
+    """Dam break against virtual walls with mirror ghosts (ghost.f).
+
+    The fluid column is the dam break's (hydrostatic start, same jitter).
+    The tank is one row of virtual particles at `spacing` dx along the
+    bottom (y = 0) and the sides (x = 0, x = tank_w, up to 2 col_h), each
+    carrying its inward unit normal in v, as the reference's virtual ring
+    does (CERNsource/test_cases.f:185-189); rho = rho0, p = 0, u = 0.  No
+    dynamic boundary: the walls act through the ghosts and their own pairs.
+    """
+    ni = int(round(col_w / dx))
+    nk = int(round(col_h / dx))
+    nf = ni * nk
+    I, K = np.meshgrid(np.arange(ni), np.arange(nk))
+    x = (I.ravel() + 0.5) * dx
+    y = (K.ravel() + 0.5) * dx
+    if jitter:
+        u = uniform01(seed, 2 * nf).reshape(nf, 2)
+        x = x + (2.0 * u[:, 0] - 1.0) * jitter * dx
+        y = y + (2.0 * u[:, 1] - 1.0) * jitter * dx
+    ds = spacing * dx
+    nb_ = int(round(tank_w / ds))
+    ns_ = int(round(2.0 * col_h / ds))
+    bx = np.arange(nb_ + 1) * ds
+    sy = np.arange(1, ns_ + 1) * ds
+    wx = np.concatenate([bx, np.zeros(ns_), np.full(ns_, tank_w)])
+    wy = np.concatenate([np.zeros(nb_ + 1), sy, sy])
+    nxn = np.concatenate([np.zeros(nb_ + 1), np.ones(ns_), -np.ones(ns_)])
+    nyn = np.concatenate([np.ones(nb_ + 1), np.zeros(ns_), np.zeros(ns_)])
+    rho0 = 1000.0
+    c0 = 10.0 * math.sqrt(g * col_h)
+    B = rho0 * c0 * c0 / 7.0
+    h = h_fac * dx
+    rho_f = rho0 * (1.0 + rho0 * g * (col_h - y) / B) ** (1.0 / 7.0)
+    X = np.concatenate([x, wx])
+    Y = np.concatenate([y, wy])
+    n = X.size
+    cell = 2.0 * h
+    ox, oy, nxg, nyg = grid_for(X, Y, cell)
+    return dict(
+        name="ghost_tank", n=n, np=nf, nv=n - nf,
+        x=X, y=Y, vx=np.concatenate([np.zeros(nf), nxn]),
+        vy=np.concatenate([np.zeros(nf), nyn]),
+        rho=np.concatenate([rho_f, np.full(n - nf, rho0)]), u=np.zeros(n),
+        precision=precision, flags=GHOSTS | GRAVITY | AV_VEL | ART_VISC | ART_HEAT,
+        pa_sph=0, density_method=0, kernel=0, h=h, mass=rho0 * dx * dx, rho0=rho0, c0=c0,
+        pmin=-1e30, av_alpha=alpha, av_beta=0.0, av_eta=0.1, ah_g1=0.1, ah_g2=1.0, ah_eta=0.1,
+        xsph_eps=0.5, gx=0.0, gy=-g, dt=cfl * h / c0,
+        ox=ox, oy=oy, cell=cell, nx=nxg, ny=nyg,
+    )
 
 
 def box_lattice(nx=20, ny=15, dx=0.01, jitter=0.05, seed=7, vel=0.0, precision=FP64, h_fac=1.3):

@@ -520,16 +520,28 @@ void orc_eos(const orc_params* P, int n, double* rho, double* p, double* c) {
 /* state, single step, integration                                      */
 /* ------------------------------------------------------------------ */
 
-orc_state* orc_state_new(int n, int np) {
+static orc_state* state_alloc(int cap, int n, int np) {
   orc_state* s = (orc_state*)calloc(1, sizeof(orc_state));
   s->n = n;
   s->np = np;
+  s->nb = n;
+  s->cap = cap;
   double** f[] = {&s->x,   &s->y,   &s->vx,  &s->vy,   &s->rho,     &s->u,
                   &s->p,   &s->c,   &s->drho, &s->du,  &s->ax,      &s->ay,
                   &s->avx, &s->avy, &s->divv, &s->rho_min, &s->u_min, &s->vx_min,
                   &s->vy_min, &s->h};
   for (size_t k = 0; k < sizeof(f) / sizeof(f[0]); ++k)
-    *f[k] = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    *f[k] = (double*)calloc((size_t)(cap > 0 ? cap : 1), sizeof(double));
+  return s;
+}
+
+orc_state* orc_state_new(int n, int np) { return state_alloc(n, n, np); }
+
+orc_state* orc_state_new_ghosts(int np, int nv) {
+  orc_state* s = state_alloc(np + nv + np, np + nv, np);
+  s->xp = (double*)calloc((size_t)(np > 0 ? np : 1), sizeof(double));
+  s->yp = (double*)calloc((size_t)(np > 0 ? np : 1), sizeof(double));
+  s->gpar = (int*)calloc((size_t)(np > 0 ? np : 1), sizeof(int));
   return s;
 }
 
@@ -539,11 +551,82 @@ void orc_state_free(orc_state* s) {
                  s->c,   s->drho, s->du, s->ax,   s->ay,      s->avx,    s->avy,
                  s->divv, s->rho_min, s->u_min, s->vx_min, s->vy_min, s->h};
   for (size_t k = 0; k < sizeof(f) / sizeof(f[0]); ++k) free(f[k]);
+  free(s->xp);
+  free(s->yp);
+  free(s->gpar);
   free(s);
 }
 
+int orc_ghost_particles(const orc_params* P, orc_state* s) {
+  /* ghost.f:1-86 (the loops over the pair list become a loop over the
+     virtual particles with the pair test of the last search) */
+  const int np = s->np, nb = s->nb;
+  const double* xp = s->have_prev ? s->xp : s->x;
+  const double* yp = s->have_prev ? s->yp : s->y;
+  const double cut = (2.0 * P->h) * (2.0 * P->h);
+  double *x = s->x, *y = s->y, *vx = s->vx, *vy = s->vy;
+  int ng = 0;
+  for (int i = 0; i < np; ++i) {
+    int best = -1;
+    double bd = 0.0;
+    for (int j = np; j < nb; ++j) {
+      const double dx = xp[i] - x[j], dy = yp[i] - y[j];
+      double r2 = dx * dx;
+      r2 = r2 + dy * dy;
+      if (!(r2 < cut)) continue;
+      /* ghost.f:42-49: squared distance at the current positions */
+      const double ex = x[j] - x[i], ey = y[j] - y[i];
+      double dd = ex * ex;
+      dd = dd + ey * ey;
+      if (best < 0 || dd < bd) {
+        best = j;
+        bd = dd;
+      }
+    }
+    if (best < 0) continue;
+    const int g = nb + ng, j = best;
+    if (g >= s->cap) return -1;
+    /* ghost.f:60-79 */
+    double d1 = 0.0, d2 = 0.0;
+    d1 = d1 + (x[i] - x[j]) * vx[j];
+    d1 = d1 + (y[i] - y[j]) * vy[j];
+    d2 = d2 + vx[i] * vx[j];
+    d2 = d2 + vy[i] * vy[j];
+    x[g] = x[i] - 2.0 * vx[j] * d1;
+    y[g] = y[i] - 2.0 * vy[j] * d1;
+    vx[g] = vx[i] - 2.0 * vx[j] * d2;
+    vy[g] = vy[i] - 2.0 * vy[j] * d2;
+    s->rho[g] = s->rho[i];
+    s->p[g] = s->p[i];
+    s->u[g] = s->u[i];
+    s->c[g] = s->c[i];
+    s->h[g] = s->h[i];
+    s->gpar[ng] = i;
+    ++ng;
+  }
+  s->n = nb + ng;
+  for (int i = 0; i < np; ++i) {
+    s->xp[i] = x[i];
+    s->yp[i] = y[i];
+  }
+  s->have_prev = 1;
+  return ng;
+}
+
+/* single_step.f:188-196: after the EOS the ghosts take rho, p and u of
+   their parents. */
+static void ghost_copy(orc_state* s) {
+  if (!s->gpar) return;
+  for (int k = s->nb; k < s->n; ++k) {
+    const int i = s->gpar[k - s->nb];
+    s->rho[k] = s->rho[i];
+    s->p[k] = s->p[i];
+    s->u[k] = s->u[i];
+  }
+}
+
 static int nrate_of(const orc_params* P, const orc_state* s) {
-  return (P->flags & ORC_DBC) ? s->n : s->np;
+  return (P->flags & ORC_DBC) ? s->nb : s->np;
 }
 
 int orc_single_step(const orc_params* P, orc_state* s, int niac, const int* pi,
@@ -568,6 +651,7 @@ int orc_single_step(const orc_params* P, orc_state* s, int niac, const int* pi,
     orc_sum_density(P, nr, niac, pi, pj, s->x, s->y, s->rho);
   }
   orc_eos(P, nr, s->rho, s->p, s->c);
+  ghost_copy(s);
   orc_int_force(P, n, niac, pi, pj, s->x, s->y, s->vx, s->vy, s->rho, s->p, s->ax, s->ay,
                 s->du);
   if (P->flags & ORC_ART_VISC)
@@ -727,6 +811,7 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
     s->divv[i] = -dr; /* divided by rho after the EOS */
   }
   orc_eos(P, nr, s->rho, s->p, s->c);
+  ghost_copy(s);
   double* psi = (double*)malloc(sizeof(double) * (size_t)n);
   for (int i = 0; i < n; ++i) {
     const double dv = s->divv[i] / s->rho[i];
@@ -988,6 +1073,10 @@ int orc_run(const orc_params* P, orc_state* s, int its0, int nsteps, int gather)
   int rc = 0;
   for (int its = its0; its < its0 + nsteps && rc == 0; ++its) {
     if (its != 1) orc_predictor(P, s);
+    if ((P->flags & ORC_GHOSTS) && s->gpar && orc_ghost_particles(P, s) < 0) {
+      rc = -4;
+      break;
+    }
     if (gather) {
       rc = orc_single_step_gather(P, s);
       if (rc) rc = -3;

@@ -73,8 +73,12 @@ enum {
   ORC_CONDUCTION = 128, /* Cleary-Monaghan heat conduction, uniform conductivity k:
                            du_i += sum m 4 k k/(k + k) (T_i - T_j)/(rho_i rho_j)
                                    (x_ij . grad_i W)/(r^2 + (h ah_eta)^2), T = u/cp */
-  ORC_PERIODIC_X = 256  /* periodic in x with period nx cell from ox: the cells
+  ORC_PERIODIC_X = 256, /* periodic in x with period nx cell from ox: the cells
                            cx - 1 and cx + 1 wrap, x_i - x_j gains -+ nx cell */
+  ORC_GHOSTS = 512      /* mirror ghosts of ghost.f: the virtual particles (np .. nb-1)
+                           carry the wall normal in v; every step each real particle
+                           with a virtual neighbour gets a ghost mirrored across the
+                           plane of the nearest one (orc_ghost_particles) */
 };
 
 enum {
@@ -123,6 +127,13 @@ typedef struct {
   double *rho_min, *u_min, *vx_min, *vy_min;
   /* per-particle smoothing length (P->sle != 0) */
   double* h;
+  /* ORC_GHOSTS (orc_state_new_ghosts): particles nb .. n-1 are the ghosts of
+     the current step, ghost nb + g mirrors real particle gpar[g]; xp, yp hold
+     the positions of the last neighbour search (have_prev = 0: none yet);
+     cap = allocated particles.  Without ghosts nb = cap = n. */
+  int nb, cap, have_prev;
+  double *xp, *yp;
+  int* gpar;
 } orc_state;
 
 /* ---- kernel ---------------------------------------------------------- */
@@ -182,7 +193,18 @@ void orc_eos(const orc_params* P, int n, double* rho, double* p, double* c);
 
 /* ---- state and step ---------------------------------------------------- */
 orc_state* orc_state_new(int n, int np);
+/* A state of np real and nv virtual particles with room for np ghosts. */
+orc_state* orc_state_new_ghosts(int np, int nv);
 void orc_state_free(orc_state* s);
+/* ghost.f:1-86 with a candidate list restated: a real particle i's
+   candidates are the virtual particles j with |xp_i - x_j|^2 < (2h)^2 (the
+   pairs of the last neighbour search, time_integration.f:157-161 calls it
+   before the new link_list); the nearest at the current positions wins
+   (ties: the smaller index).  Ghosts are appended in the order of their
+   parents: x_g = x_i - 2 n (x_i - x_j).n, v_g = v_i - 2 n v_i.n with n = v_j,
+   and rho, p, u, c, h of the parent.  Then xp, yp = x, y (the search of this
+   step).  Returns the ghost count, or -1 if the capacity is exceeded. */
+int orc_ghost_particles(const orc_params* P, orc_state* s);
 /* single_step.f restated with the faithful half-list routines. */
 int orc_single_step(const orc_params* P, orc_state* s, int niac, const int* pi, const int* pj);
 /* Same rates, gather form over the fixed grid (OpenMP).  Returns the
@@ -192,7 +214,8 @@ int orc_single_step_gather(const orc_params* P, orc_state* s);
 void orc_predictor(const orc_params* P, orc_state* s);
 void orc_corrector(const orc_params* P, orc_state* s, int its);
 /* nsteps of time_integration starting at step number its0 (1-based).
-   gather=0: faithful serial half-list path; gather=1: gather path.
+   gather=0: faithful serial half-list path; gather=1: gather path.  With
+   ORC_GHOSTS the ghosts are rebuilt after the predictor of every step.
    Returns 0, or a negative error. */
 int orc_run(const orc_params* P, orc_state* s, int its0, int nsteps, int gather);
 /* FP64 conservation sums: mass, momentum x, momentum y, kinetic energy,

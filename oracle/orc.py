@@ -14,7 +14,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libarmref.so")
 
 K_WENDLAND2D, K_CUBIC2D, K_CUBIC3D, K_WENDLAND3D = 0, 1, 2, 3
 ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
-CONDUCTION, PERIODIC_X = 128, 256
+CONDUCTION, PERIODIC_X, GHOSTS = 128, 256, 512
 EOS_TAIT, EOS_LINEAR = 0, 1
 
 
@@ -43,7 +43,9 @@ class State(C.Structure):
         (name, C.POINTER(C.c_double)) for name in (
             "x", "y", "vx", "vy", "rho", "u", "p", "c", "drho", "du", "ax", "ay",
             "avx", "avy", "divv", "rho_min", "u_min", "vx_min", "vy_min", "h")
-    ]
+    ] + [("nb", C.c_int), ("cap", C.c_int), ("have_prev", C.c_int),
+         ("xp", C.POINTER(C.c_double)), ("yp", C.POINTER(C.c_double)),
+         ("gpar", C.POINTER(C.c_int))]
 
 
 STATE3_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "rho", "u", "p", "c", "drho", "du", "ax",
@@ -92,6 +94,8 @@ def lib():
         "orc_eos_tait": (None, [_PP, C.c_int, _dp, _dp, _dp]),
         "orc_eos": (None, [_PP, C.c_int, _dp, _dp, _dp]),
         "orc_state_new": (C.POINTER(State), [C.c_int, C.c_int]),
+        "orc_state_new_ghosts": (C.POINTER(State), [C.c_int, C.c_int]),
+        "orc_ghost_particles": (C.c_int, [_PP, C.POINTER(State)]),
         "orc_state_free": (None, [C.POINTER(State)]),
         "orc_single_step": (C.c_int, [_PP, C.POINTER(State), C.c_int, _ip, _ip]),
         "orc_single_step_gather": (C.c_int, [_PP, C.POINTER(State)]),
@@ -254,6 +258,46 @@ class OracleState:
 
     def get(self, *names):
         return {k: self.field(k).copy() for k in names}
+
+
+class GhostState(OracleState):
+    """An orc_state of np real and nv virtual particles with room for np
+    ghosts; fields are views of the live particles (n = nb + ghosts)."""
+
+    def __init__(self, np_fluid, nv):
+        self.ptr = lib().orc_state_new_ghosts(np_fluid, nv)
+        self.np = np_fluid
+        self.nb = np_fluid + nv
+
+    @property
+    def n(self):
+        return self.ptr.contents.n
+
+    @n.setter
+    def n(self, value):
+        pass
+
+    @property
+    def ng(self):
+        return self.ptr.contents.n - self.nb
+
+    def set_prev(self, x, y):
+        """Positions of the last neighbour search (real particles)."""
+        c = self.ptr.contents
+        np.ctypeslib.as_array(c.xp, shape=(self.np,))[:] = x[:self.np]
+        np.ctypeslib.as_array(c.yp, shape=(self.np,))[:] = y[:self.np]
+        c.have_prev = 1
+
+    def parents(self):
+        return np.ctypeslib.as_array(self.ptr.contents.gpar, shape=(max(self.ng, 1),))[
+            :self.ng].copy()
+
+
+def ghost_particles(P, st):
+    ng = lib().orc_ghost_particles(C.byref(P), st.ptr)
+    if ng < 0:
+        raise RuntimeError("orc_ghost_particles: capacity exceeded")
+    return ng
 
 
 def run(P, st, its0, nsteps, gather=True):

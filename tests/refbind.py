@@ -220,3 +220,35 @@ def h_upgrade(x, y, vx, vy, h, mass, rho, pi, pj, dt, sle):
                          C.byref(d), _i(sle), _i(0))
         return H
     return run_big_stack(f)
+
+
+def ghost_particles(x, y, vx, vy, h, mass, rho, p, np_, nv, pi, pj):
+    """ghost.f ghost_particles over the given pair list (0-based, i < j);
+    returns (ng, index_ghost (0-based ghost index per real particle, -1 for
+    none), x, y, vx, vy, h, mass, rho, p of the n + ng particles)."""
+    def f():
+        n = np_ + nv
+        cap = n + np_
+        X = np.zeros((cap, 3))
+        X[:n, 0], X[:n, 1] = x, y
+        V = np.zeros((cap, 3))
+        V[:n, 0], V[:n, 1] = vx, vy
+        H = np.zeros(cap)
+        H[:n] = hvec(h, n)
+        M = np.zeros(cap)
+        M[:n] = mass
+        R = np.zeros(cap)
+        R[:n] = rho
+        Pp = np.zeros(cap)
+        Pp[:n] = p
+        P1 = np.ascontiguousarray(pi + 1, np.int32)
+        P2 = np.ascontiguousarray(pj + 1, np.int32)
+        ng = C.c_int(0)
+        idx = np.zeros(max(np_, 1), np.int32)
+        lib().ghost_particles_(_i(np_), _i(nv), C.byref(ng), _p(X), _p(V), _p(H), _p(M), _p(R),
+                               _p(Pp), _i(len(pi)), _p(P1), _p(P2), _p(idx))
+        k = n + ng.value
+        return dict(ng=ng.value, index=idx[:np_].astype(np.int64) - 1, x=X[:k, 0].copy(),
+                    y=X[:k, 1].copy(), vx=V[:k, 0].copy(), vy=V[:k, 1].copy(), h=H[:k].copy(),
+                    mass=M[:k].copy(), rho=R[:k].copy(), p=Pp[:k].copy())
+    return run_big_stack(f)
