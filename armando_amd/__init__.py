@@ -23,7 +23,7 @@ ARM_OK = 0
 ERRORS = {-1: "ARM_ERR_ARG", -2: "ARM_ERR_HIP", -3: "ARM_ERR_CAPACITY", -4: "ARM_ERR_DOMAIN",
           -5: "ARM_ERR_UNSUPPORTED", -6: "ARM_ERR_STATE"}
 ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
-CONDUCTION, PERIODIC_X = 128, 256
+CONDUCTION, PERIODIC_X, GHOSTS = 128, 256, 512
 EOS_TAIT, EOS_LINEAR = 0, 1
 FP64, FP32 = 0, 1
 PASSES = ("keys", "sort", "reorder", "density", "eos", "force", "integrate")
@@ -35,7 +35,8 @@ EXPORTS = ("arm_create", "arm_destroy", "arm_last_error", "arm_domain_violations
            "arm_conservation", "arm_pass_times", "arm_reset_pass_times", "arm_set_timing",
            "arm_set_gather_path", "arm_adjoint_rollout", "arm_rollout_forward",
            "arm_rollout_backward", "arm_update_params", "arm_set_objective",
-           "arm_rollout_param_grads", "arm_conservation3", "arm_upload_h", "arm_download_h")
+           "arm_rollout_param_grads", "arm_conservation3", "arm_upload_h", "arm_download_h",
+           "arm_ghost_particles", "arm_ghost_count", "arm_download_ghosts")
 
 
 class ArmParams(C.Structure):
@@ -150,7 +151,10 @@ class Simulation:
         self.params = params_from(case)
         self.dim = 3 if self.params.dim == 3 else 2
         h = C.c_void_p()
-        rc = self.L.arm_create(C.byref(self.params), int(capacity or self.n), C.byref(h))
+        if capacity is None:
+            # mirror ghosts: room for the live set and the next one
+            capacity = self.n + (2 * self.np if self.params.flags & GHOSTS else 0)
+        rc = self.L.arm_create(C.byref(self.params), int(capacity), C.byref(h))
         self.ctx = h
         self._check(rc)
         X = np.ascontiguousarray(np.stack([case[k] for k in self._axes()], 1), np.float64)
@@ -222,6 +226,29 @@ class Simulation:
             out[k] = X[:, a].copy()
             out["v" + k] = V[:, a].copy()
         return out
+
+    # ---- mirror ghosts (ghost.f) -------------------------------------------
+    def ghost_particles(self):
+        ng = C.c_int(0)
+        self._check(self.L.arm_ghost_particles(self.ctx, C.byref(ng)))
+        return ng.value
+
+    def ghost_count(self):
+        ng = C.c_int(0)
+        self._check(self.L.arm_ghost_count(self.ctx, C.byref(ng)))
+        return ng.value
+
+    def ghosts(self):
+        """The ghosts of the current state in number order."""
+        ng = self.ghost_count()
+        X = np.zeros((max(ng, 1), 2))
+        V = np.zeros((max(ng, 1), 2))
+        rho, p, u = np.zeros(max(ng, 1)), np.zeros(max(ng, 1)), np.zeros(max(ng, 1))
+        par = np.zeros(max(ng, 1), np.int32)
+        self._check(self.L.arm_download_ghosts(self.ctx, _ptr(X), _ptr(V), _ptr(rho), _ptr(p),
+                                               _ptr(u), _ptr(par)))
+        return dict(x=X[:ng, 0].copy(), y=X[:ng, 1].copy(), vx=V[:ng, 0].copy(),
+                    vy=V[:ng, 1].copy(), rho=rho[:ng], p=p[:ng], u=u[:ng], parent=par[:ng])
 
     def rates(self):
         n, d = self.n, self.dim

@@ -23,6 +23,7 @@
 #include "sph_adjoint.h"
 #include "sph3d.h"
 #include "sph3d_adjoint.h"
+#include "sph_ghost.h"
 
 using namespace arm;
 
@@ -43,6 +44,16 @@ struct arm_ctx {
   KParams K{};
   int cap = 0;
   int n = 0, np = 0;
+  // mirror ghosts (ARM_GHOSTS): particles nb = np + nv onward are ghosts; ng
+  // of them live in the sorted state; after arm_ghost_particles the slots
+  // from gfirst hold ng_new new ones and those below are stale until the
+  // next arm_link_list (ghost_pending)
+  bool ghosts = false;
+  int nb = 0, ng = 0, ng_new = 0, gfirst = 0;
+  bool ghost_pending = false;
+  bool have_ranges = false;  // cstart describes the current slot order
+  Ghosts G{};
+  int* d_ng = nullptr;
   bool loaded = false, sorted = false, have_rates = false;
   bool tiled = true;  // LDS-tiled gathers (false: direct global-memory gathers)
   struct AdjRun* adj = nullptr;  // checkpoints of the last arm_rollout_forward
@@ -217,17 +228,23 @@ KeyOut key_out(arm_ctx* c) {
   return KeyOut{c->key_by_id, c->slot_by_id, nullptr, nullptr};
 }
 
+// Cells of the counting sort: the grid, plus with ghosts the extra cell
+// ncell that takes the ghosts of the previous step after every real cell.
+int sort_cells(const arm_ctx* c) { return c->K.ncell + (c->ghosts ? 1 : 0); }
+
 // Clears the counters of the counting sort before a run of keyed steps.
 void zero_counts(arm_ctx* c) {
   if (c->counting)
-    hipLaunchKernelGGL(k_cs_zero, dim3(std::min(1024, (c->K.ncell + CS_THREADS - 1) / CS_THREADS)),
-                       dim3(CS_THREADS), 0, c->stream, c->cnt, c->K.ncell);
+    hipLaunchKernelGGL(k_cs_zero, dim3(std::min(1024, (sort_cells(c) + CS_THREADS - 1) / CS_THREADS)),
+                       dim3(CS_THREADS), 0, c->stream, c->cnt, sort_cells(c));
 }
 
-int run_sort(arm_ctx* c, const int32_t* id_of, const uint32_t** sval_out, bool* ranges_done) {
-  const int n = c->n;
+// Sorts the first n slots (c->n when n < 0).
+int run_sort(arm_ctx* c, const int32_t* id_of, const uint32_t** sval_out, bool* ranges_done,
+             int n_sort = -1) {
+  const int n = n_sort < 0 ? c->n : n_sort;
   if (c->counting) {
-    const int ncell = c->K.ncell;
+    const int ncell = sort_cells(c);
     const int nb = (n + CS_THREADS - 1) / CS_THREADS;
     hipLaunchKernelGGL(k_cs_tile_sum, dim3(c->cs_tiles), dim3(CS_THREADS), 0, c->stream, c->cnt,
                        ncell, c->tsum);
@@ -236,7 +253,7 @@ int run_sort(arm_ctx* c, const int32_t* id_of, const uint32_t** sval_out, bool* 
     hipLaunchKernelGGL(k_cs_scatter, dim3(nb), dim3(CS_THREADS), 0, c->stream, c->key_by_id,
                        id_of, n, c->cstart, c->cnt, c->ent, c->sk[1]);
     hipLaunchKernelGGL(k_cs_rank, dim3(nb), dim3(CS_THREADS), 0, c->stream, c->ent, c->sk[1], n,
-                       c->cstart, c->sk[0], c->sv[0]);
+                       c->cstart, c->sk[0], c->sv[0], c->ghosts ? c->K.ncell : -1);
     HIPCHK(c, hipGetLastError());
     c->skey = c->sk[0];
     *sval_out = c->sv[0];
@@ -291,15 +308,18 @@ int check_domain(arm_ctx* c) {
 }
 
 // keys of the current state -> sort -> permute (all fields) + cell ranges
+// With ghosts: only the set made by arm_ghost_particles since the last sort
+// (slots from gfirst) enters the sorted state; every older ghost drops out.
 template <typename R> int link_list_t(arm_ctx* c) {
   const int n = c->n;
   St<R>& A = stA<R>(c);
   St<R>& B = stB<R>(c);
+  const int gfirst = c->ghost_pending ? c->gfirst : n;
   {
     PassTimer tm(c, ARM_PASS_KEYS);
     zero_counts(c);
     hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
-                       key_out(c), c->d_err);
+                       key_out(c), c->d_err, gfirst);
   }
   const uint32_t* sval = nullptr;
   bool ranges = false;
@@ -308,10 +328,14 @@ template <typename R> int link_list_t(arm_ctx* c) {
     int rc = run_sort(c, A.id, &sval, &ranges);
     if (rc) return rc;
   }
+  c->ng = c->ghost_pending ? c->ng_new : 0;
+  c->ghost_pending = false;
+  c->n = c->K.n = c->nb + c->ng;
+  c->have_ranges = true;
   {
     PassTimer tm(c, ARM_PASS_REORDER);
-    hipLaunchKernelGGL(k_reorder_all<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
-                       c->K, c->skey, sval, c->cstart, ranges ? 0 : 1);
+    hipLaunchKernelGGL(k_reorder_all<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, B,
+                       c->K, c->skey, sval, c->cstart, ranges ? 0 : 1, c->G);
   }
   HIPCHK(c, hipGetLastError());
   std::swap(A, B);
@@ -344,6 +368,14 @@ template <typename R, bool SUM, bool EOS> void launch_density(arm_ctx* c) {
   c->masks_valid = true;
 }
 
+// single_step.f:188-196: after the EOS the ghosts take rho, p, u of their
+// parents.
+template <typename R> void ghost_copy(arm_ctx* c) {
+  if (c->ghosts && c->ng > 0)
+    hipLaunchKernelGGL(k_ghost_copy<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                       stA<R>(c), c->K, c->G);
+}
+
 // density pass; with eos the equation of state is applied in the same kernel
 template <typename R> int density_t(arm_ctx* c, int method, bool eos = false) {
   PassTimer tm(c, ARM_PASS_DENSITY);
@@ -351,6 +383,7 @@ template <typename R> int density_t(arm_ctx* c, int method, bool eos = false) {
     eos ? launch_density<R, true, true>(c) : launch_density<R, true, false>(c);
   else
     eos ? launch_density<R, false, true>(c) : launch_density<R, false, false>(c);
+  if (eos) ghost_copy<R>(c);
   HIPCHK(c, hipGetLastError());
   return ARM_OK;
 }
@@ -359,6 +392,7 @@ template <typename R> int eos_t(arm_ctx* c) {
   St<R>& A = stA<R>(c);
   PassTimer tm(c, ARM_PASS_EOS);
   hipLaunchKernelGGL(k_eos<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K);
+  ghost_copy<R>(c);
   HIPCHK(c, hipGetLastError());
   return ARM_OK;
 }
@@ -484,9 +518,50 @@ template <typename R> int begin_steps(arm_ctx* c) {
   St<R>& A = stA<R>(c);
   PassTimer tm(c, ARM_PASS_KEYS);
   zero_counts(c);
+  // every ghost present is replaced by the first step's own
   hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
-                     key_out(c), c->d_err);
+                     key_out(c), c->d_err, c->n);
   HIPCHK(c, hipGetLastError());
+  return ARM_OK;
+}
+
+// ghost.f on the sorted state: select the wall of every real particle,
+// number the ghosts in parent order (a scan of the flags by id) and write
+// them after the n_old live slots.  keys: emit their keys for the counting
+// sort (the fused step); the per-routine link_list hashes them itself.
+// Reads the ghost count back (the next launches are sized by it).
+template <typename R> int make_ghosts(arm_ctx* c, bool keys) {
+  St<R>& A = stA<R>(c);
+  const int n_old = c->n, np = c->np;
+  c->ng_new = 0;
+  if (np == 0) return ARM_OK;
+  PassTimer tm(c, ARM_PASS_KEYS);
+  hipLaunchKernelGGL(k_ghost_select<R>, dim3(grid_of(n_old)), dim3(BLOCK), 0, c->stream, A,
+                     c->K, c->G, c->cstart, n_old);
+  const int tiles = (np + CS_TILE - 1) / CS_TILE;
+  hipLaunchKernelGGL(k_cs_tile_sum, dim3(tiles), dim3(CS_THREADS), 0, c->stream, c->G.gflag, np,
+                     c->tsum);
+  hipLaunchKernelGGL(k_cs_tile_scan, dim3(tiles), dim3(CS_THREADS), 0, c->stream, c->G.gflag,
+                     np, 0, c->tsum, c->G.grank);
+  hipLaunchKernelGGL(k_ghost_make<R>, dim3(grid_of(n_old)), dim3(BLOCK), 0, c->stream, A, c->K,
+                     c->G, n_old, c->cap, keys ? 1 : 0, keys ? key_out(c) : KeyOut{}, c->d_err,
+                     c->d_ng);
+  HIPCHK(c, hipGetLastError());
+  int hv[3] = {0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(hv, c->d_ng, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hv + 1, c->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if ((long long)n_old + hv[0] > c->cap) {
+    c->loaded = false;
+    return fail(c, ARM_ERR_CAPACITY, "%d ghosts on %d live particles exceed capacity %d "
+                "(upload again)", hv[0], n_old, c->cap);
+  }
+  if (hv[1] || hv[2]) {
+    c->loaded = false;
+    int rc = check_domain(c);
+    return rc ? rc : fail(c, ARM_ERR_DOMAIN, "ghost outside the hash grid");
+  }
+  c->ng_new = hv[0];
   return ARM_OK;
 }
 
@@ -501,21 +576,33 @@ struct StepHooks {
 };
 
 template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
-  const int n = c->n;
   const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
   int rc;
   const uint32_t* sval = nullptr;
   bool ranges = false;
+  // with ghosts: the new set goes after the live slots; the old set carries
+  // the extra key (corrector) and drops out of the sort
+  int n_sort = c->n;
+  if (c->ghosts) {
+    if ((rc = make_ghosts<R>(c, true))) return rc;
+    n_sort = c->n + c->ng_new;
+  }
   {
     PassTimer tm(c, ARM_PASS_SORT);
-    if ((rc = run_sort(c, stA<R>(c).id, &sval, &ranges))) return rc;
+    if ((rc = run_sort(c, stA<R>(c).id, &sval, &ranges, n_sort))) return rc;
   }
+  if (c->ghosts) {
+    c->ng = c->ng_new;
+    c->n = c->K.n = c->nb + c->ng;
+  }
+  c->have_ranges = true;
+  const int n = c->n;
   {
     St<R>& A = stA<R>(c);
     St<R>& B = stB<R>(c);
     PassTimer tm(c, ARM_PASS_REORDER);
     hipLaunchKernelGGL(k_reorder_pred<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
-                       c->K, c->skey, sval, c->cstart, its != 1 ? 1 : 0, ranges ? 0 : 1);
+                       c->K, c->skey, sval, c->cstart, its != 1 ? 1 : 0, ranges ? 0 : 1, c->G);
     std::swap(A, B);
     c->masks_valid = false;
   }
@@ -538,6 +625,8 @@ template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
 
 template <typename R> int time_integration_t(arm_ctx* c, int its0, int nsteps) {
   int rc;
+  // ghosts are selected on a sorted state (cell ranges of the last sort)
+  if (c->ghosts && (!c->have_ranges || c->ghost_pending) && (rc = link_list_t<R>(c))) return rc;
   if ((rc = begin_steps<R>(c))) return rc;
   if (!(c->prm.flags & ARM_AV_VEL) && (rc = zero_av<R>(c))) return rc;
   const StepHooks none;
@@ -981,7 +1070,7 @@ template <typename R> int upload_t(arm_ctx* c, const double* x, const double* v,
   // FP32 form: cell + offset from the FP64 positions (the keys kernel writes them)
   if (Layout<R>::f32) {
     hipLaunchKernelGGL(k_keys<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, c->K,
-                       KeyOut{}, c->d_err);
+                       KeyOut{}, c->d_err, 0);
     HIPCHK(c, hipGetLastError());
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1014,6 +1103,7 @@ template <typename R> int download_t(arm_ctx* c, double* x, double* v, double* r
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int s = 0; s < n; ++s) {
     const int i = id[s];
+    if (i >= c->nb) continue;  // ghosts: arm_download_ghosts
     if (x) {
       x[2 * i] = xy[s].x;
       x[2 * i + 1] = xy[s].y;
@@ -1025,6 +1115,44 @@ template <typename R> int download_t(arm_ctx* c, double* x, double* v, double* r
     if (rho) rho[i] = Layout<R>::f32 ? c->prm.rho0 + (double)rr[s] : (double)rr[s];
     if (p) p[i] = (double)pp[s];
     if (u) u[i] = uu[s];
+  }
+  return ARM_OK;
+}
+
+template <typename R> int download_ghosts_t(arm_ctx* c, double* x, double* v, double* rho,
+                                            double* p, double* u, int32_t* parent) {
+  using R2 = typename V2<R>::type;
+  const int n = c->n;
+  St<R>& A = stA<R>(c);
+  std::vector<int32_t> id, pid;
+  std::vector<double2> xy;
+  std::vector<R2> vv;
+  std::vector<R> rr, pp;
+  std::vector<double> uu;
+  int rc;
+  if ((rc = d2h(c, id, A.id, n))) return rc;
+  if ((rc = d2h(c, pid, c->G.gpid, n))) return rc;
+  if (x && (rc = d2h(c, xy, A.xy, n))) return rc;
+  if (v && (rc = d2h(c, vv, A.v, n))) return rc;
+  if (rho && (rc = d2h(c, rr, A.rho, n))) return rc;
+  if (p && (rc = d2h(c, pp, A.p, n))) return rc;
+  if (u && (rc = d2h(c, uu, A.u, n))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < n; ++s) {
+    const int g = id[s] - c->nb;
+    if (g < 0) continue;
+    if (x) {
+      x[2 * g] = xy[s].x;
+      x[2 * g + 1] = xy[s].y;
+    }
+    if (v) {
+      v[2 * g] = (double)vv[s].x;
+      v[2 * g + 1] = (double)vv[s].y;
+    }
+    if (rho) rho[g] = Layout<R>::f32 ? c->prm.rho0 + (double)rr[s] : (double)rr[s];
+    if (p) p[g] = (double)pp[s];
+    if (u) u[g] = uu[s];
+    if (parent) parent[g] = pid[s];
   }
   return ARM_OK;
 }
@@ -1047,6 +1175,7 @@ template <typename R> int download_rates_t(arm_ctx* c, double* drho, double* dvd
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int s = 0; s < n; ++s) {
     const int i = id[s];
+    if (i >= c->nb) continue;  // ghosts carry no rates of their own
     if (drho) drho[i] = (double)dr[s];
     if (dudt) dudt[i] = (double)du[s];
     if (divv) divv[i] = (double)dv[s];
@@ -1446,6 +1575,7 @@ template <typename R> int download_rates3_t(arm_ctx* c, double* drho, double* dv
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int s = 0; s < n; ++s) {
     const int i = id[s];
+    if (i >= c->nb) continue;  // ghosts carry no rates of their own
     if (drho) drho[i] = (double)dr[s];
     if (dudt) dudt[i] = (double)du[s];
     if (divv) divv[i] = (double)dv[s];
@@ -1824,6 +1954,10 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     rc = fail(c, ARM_ERR_ARG, "visc_eta must be non-negative");
   else if ((P.flags & ARM_HEAT_SOURCE) && !(P.hs_sigma > 0))
     rc = fail(c, ARM_ERR_ARG, "hs_sigma must be positive");
+  else if ((P.flags & ARM_GHOSTS) &&
+           (P.dim == 3 || (P.flags & (ARM_DBC | ARM_PERIODIC_X)) || P.sle))
+    rc = fail(c, ARM_ERR_UNSUPPORTED,
+              "ARM_GHOSTS is 2D, without ARM_DBC, ARM_PERIODIC_X or sle");
   if (rc) {
     *out = c;  // so that arm_last_error can report
     return rc;
@@ -1855,18 +1989,37 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
   }
   if (!rc) rc = dalloc(c, &c->hist, (size_t)(1 << c->sort_bits) * c->ntiles);
   if (!rc) rc = dalloc(c, &c->tot, (size_t)(1 << c->sort_bits));
-  c->cs_tiles = (ncell + CS_TILE - 1) / CS_TILE;
+  c->ghosts = (P.flags & ARM_GHOSTS) != 0;
+  // the counting sort runs over ncell cells, plus one with ghosts; its tile
+  // sums also serve the ghost numbering (a scan over the real ids)
+  c->cs_tiles = (ncell + 1 + CS_TILE - 1) / CS_TILE;
   c->counting = c->cs_tiles <= CS_MAX_TILES;
+  if (!rc && c->ghosts && !c->counting)
+    rc = fail(c, ARM_ERR_UNSUPPORTED, "ARM_GHOSTS needs a grid of at most %d cells",
+              CS_TILE * CS_MAX_TILES - 1);
   if (!rc && c->counting) {
-    rc = dalloc(c, &c->cnt, (size_t)ncell);
-    if (!rc) rc = dalloc(c, &c->tsum, (size_t)c->cs_tiles);
+    const int tiles = std::max(c->cs_tiles, (capacity + CS_TILE - 1) / CS_TILE);
+    rc = dalloc(c, &c->cnt, (size_t)ncell + 1);
+    if (!rc) rc = dalloc(c, &c->tsum, (size_t)tiles);
     if (!rc) rc = dalloc(c, &c->ent, (size_t)capacity);
     if (!rc) {
-      hipError_t ez = hipMemset(c->cnt, 0, sizeof(int) * (size_t)ncell);
+      hipError_t ez = hipMemset(c->cnt, 0, sizeof(int) * ((size_t)ncell + 1));
       if (ez != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipMemset: %s", hipGetErrorString(ez));
     }
   }
-  if (!rc) rc = dalloc(c, &c->cstart, (size_t)ncell + 1);
+  if (!rc) rc = dalloc(c, &c->cstart, (size_t)ncell + 2);
+  if (!rc && c->ghosts) {
+    Ghosts& G = c->G;
+    rc = dalloc(c, &G.xprev, capacity);
+    if (!rc) rc = dalloc(c, &G.gsel, capacity);
+    if (!rc) rc = dalloc(c, &G.gflag, capacity);
+    if (!rc) rc = dalloc(c, &G.grank, (size_t)capacity + 1);
+    if (!rc) rc = dalloc(c, &G.gpar, capacity);
+    if (!rc) rc = dalloc(c, &G.gwall, capacity);
+    if (!rc) rc = dalloc(c, &G.gpid, capacity);
+    if (!rc) rc = dalloc(c, &G.sid, capacity);
+    if (!rc) rc = dalloc(c, &c->d_ng, 1);
+  }
   if (!rc) rc = dalloc(c, &c->d_err, 2);
   if (!rc) rc = dalloc(c, &c->d_part, (size_t)grid_of(capacity) * 6);
   if (!rc) rc = dalloc(c, &c->d_cnt, capacity);
@@ -1917,14 +2070,26 @@ int arm_upload(arm_ctx* c, int np, int nv, const double* x, const double* v, con
   if (np + nv < 1) return fail(c, ARM_ERR_ARG, "no particles");
   c->n = np + nv;
   c->np = np;
+  c->nb = c->n;
+  c->ng = c->ng_new = 0;
+  c->ghost_pending = false;
+  c->have_ranges = false;
   c->K.n = c->n;
   c->K.np = np;
+  c->K.nb = c->n;
   c->sorted = false;
   c->masks_valid = false;
   c->have_rates = false;
   c->domain_bad = 0;
   int rc = is3d(c) ? DISPATCH(c, upload3_t)(c, x, v, rho, u)
                     : DISPATCH(c, upload_t)(c, x, v, rho, nullptr, u);
+  if (!rc && c->ghosts) {  // no search yet: the previous positions are these
+    const double2* xy = is32(c) ? c->A32.xy : c->A64.xy;
+    hipError_t e = hipMemcpyAsync(c->G.xprev, xy, sizeof(double2) * (size_t)c->n,
+                                  hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "xprev copy: %s", hipGetErrorString(e));
+  }
   c->loaded = (rc == ARM_OK);
   return rc;
 }
@@ -2093,7 +2258,7 @@ int arm_corrector(arm_ctx* c, int its) {
     // refresh cell + offset of the FP32 form from the FP64 accumulators
     St<float>& A = c->A32;
     hipLaunchKernelGGL(k_keys<float>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
-                       KeyOut{}, c->d_err);
+                       KeyOut{}, c->d_err, 0);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
@@ -2110,6 +2275,39 @@ int arm_time_integration(arm_ctx* c, int its0, int nsteps) {
   // the last corrector moved the particles after the last sort
   c->sorted = false;
   return rc;
+}
+
+int arm_ghost_particles(arm_ctx* c, int* ng) {
+  if (!c) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (!c->ghosts) return fail(c, ARM_ERR_UNSUPPORTED, "the context was created without ARM_GHOSTS");
+  if (!c->have_ranges) return fail(c, ARM_ERR_STATE, "arm_link_list has not run on this state");
+  if (c->ghost_pending)
+    return fail(c, ARM_ERR_STATE, "arm_link_list must follow arm_ghost_particles");
+  int rc = DISPATCH(c, make_ghosts)(c, false);
+  if (rc) return rc;
+  c->gfirst = c->n;
+  c->n = c->K.n = c->n + c->ng_new;
+  c->ghost_pending = true;
+  c->sorted = false;
+  c->masks_valid = false;
+  if (ng) *ng = c->ng_new;
+  return ARM_OK;
+}
+
+int arm_ghost_count(arm_ctx* c, int* ng) {
+  if (!c || !ng) return ARM_ERR_ARG;
+  *ng = c->ghost_pending ? c->ng_new : c->ng;
+  return ARM_OK;
+}
+
+int arm_download_ghosts(arm_ctx* c, double* x, double* v, double* rho, double* p, double* u,
+                        int32_t* parent) {
+  if (!c) return ARM_ERR_ARG;
+  if (!c->loaded) return fail(c, ARM_ERR_STATE, "no state uploaded");
+  if (!c->ghosts) return fail(c, ARM_ERR_UNSUPPORTED, "the context was created without ARM_GHOSTS");
+  if (c->ghost_pending) return fail(c, ARM_ERR_STATE, "arm_link_list must run first");
+  return DISPATCH(c, download_ghosts_t)(c, x, v, rho, p, u, parent);
 }
 
 int arm_conservation(arm_ctx* c, double out[5]) {
@@ -2173,6 +2371,7 @@ int arm_rollout_forward(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* o
     return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint differentiates continuity density only");
   if (c->prm.flags & (ARM_CONDUCTION | ARM_PERIODIC_X))
     return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint has no conduction or periodic boundary");
+  if (c->ghosts) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint has no mirror ghosts");
   if (c->prm.sle) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint keeps h fixed (sle = 0)");
   const int ck = std::min(ckpt_every, nsteps);
   int rc;

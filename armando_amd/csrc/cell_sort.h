@@ -173,15 +173,22 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_scatter(const uint32_t* __res
   entc[pos] = c;
 }
 
+// skip: a cell whose order does not matter (the extra cell of stale ghosts,
+// or -1): its entries keep their scatter positions.
 __global__ __launch_bounds__(CS_THREADS) void k_cs_rank(const uint64_t* __restrict__ ent,
                                                         const uint32_t* __restrict__ entc, int n,
                                                         const int* __restrict__ cstart,
                                                         uint32_t* __restrict__ skey,
-                                                        uint32_t* __restrict__ sval) {
+                                                        uint32_t* __restrict__ sval, int skip) {
   const int p = blockIdx.x * CS_THREADS + threadIdx.x;
   if (p >= n) return;
   const uint32_t c = entc[p];
   const uint64_t e = ent[p];
+  if ((int)c == skip) {
+    skey[p] = c;
+    sval[p] = (uint32_t)e;
+    return;
+  }
   const int a = cstart[c], b = cstart[c + 1];
   const uint32_t id = (uint32_t)(e >> 32);
   int rank = 0;

@@ -48,6 +48,9 @@ struct KParams {
   // and every pair uses mh = (h_i + h_j)/2 (with_h below).
   int sle;
   double av_eta, ah_eta;
+  // mirror ghosts (flag 512, ghost.f): ids >= nb (= np + nv) are the ghosts
+  // of the current step; without ghosts nb = n
+  int nb;
 };
 
 // K with the smoothing length mh: h, 1/h, the Wendland 2D constants, the

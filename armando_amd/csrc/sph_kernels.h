@@ -35,14 +35,20 @@ __device__ __forceinline__ int cell_of(const KParams& K, double x, double y, int
 // its slot with the per-cell counts for the counting sort; a stable sort of
 // the keys listed in id order yields (cell, id) order).  FP32 form: also
 // refreshes the integer cell and FP32 offset from the FP64 accumulator.
+// Ghosts (ids >= nb) in slots below gfirst are those of an earlier step:
+// their key is the extra cell ncell, which the sort puts after every real
+// cell, so they drop out of the sorted state.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_keys(St<R> S, KParams K, KeyOut ko,
-                                                int* __restrict__ err) {
+                                                int* __restrict__ err, int gfirst) {
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   const bool valid = s < K.n;
   uint32_t key = 0;
   int id = 0;
-  if (valid) {
+  if (valid && S.id[s] >= K.nb && s < gfirst) {
+    id = S.id[s];
+    key = (uint32_t)K.ncell;
+  } else if (valid) {
     const double2 q = S.xy[s];
     int ix, iy;
     bool ok;
@@ -94,18 +100,94 @@ __device__ __forceinline__ void fill_cell_start(const uint32_t* __restrict__ ske
   }
 }
 
+// ------------------------------------------------------------------------
+// mirror ghosts (flag 512; CERNsource/ghost.f:1-86)
+// ------------------------------------------------------------------------
+struct Ghosts {
+  double2* xprev;  // by slot: position at the last sort (the neighbour search
+                   // whose pair list ghost_particles walks)
+  int* gsel;       // by slot: slot of the chosen virtual particle, or -1
+  int* gflag;      // by real id: 1 if the particle gets a ghost
+  int* grank;      // by real id: its ghost's number (exclusive scan of gflag)
+  int* gpar;       // by staging slot: the parent's slot (order before the sort)
+  int* gwall;      // by staging slot: the virtual particle's slot (same order)
+  int* gpid;       // by slot of the sorted order: a ghost's parent id
+  int* sid;        // by real id: slot in the sorted order
+};
+
+// The fields of ghost slot t (sorted order) from staging slot s: position
+// as built, velocity = the parent's (after the predictor when predict) with
+// its component along the wall normal n reversed, v - 2 n (v.n)
+// (ghost.f:60-79); rho, u, p of the parent.  The EOS pass copies rho, p, u
+// again after the equation of state (single_step.f:188-196).
+template <typename R>
+__device__ __forceinline__ void ghost_slot(const St<R>& A, St<R>& Bs, const KParams& K,
+                                           const Ghosts& G, int s, int t, int id,
+                                           int predict) {
+  using R2 = typename V2<R>::type;
+  const int ps = G.gpar[s], ws = G.gwall[s];
+  const double2 q = A.xy[s];
+  Bs.id[t] = id;
+  Bs.xy[t] = q;
+  if constexpr (sizeof(R) == 4) {
+    int ix, iy;
+    bool ok;
+    cell_of(K, q.x, q.y, ix, iy, ok);
+    const double bx = K.ox + (double)ix * K.cell;
+    const double by = K.oy + (double)iy * K.cell;
+    Bs.cxy[t] = (uint32_t)ix | ((uint32_t)iy << 16);
+    Bs.off[t] = make_float2((float)(q.x - bx), (float)(q.y - by));
+  }
+  R2 v = A.v[ps];
+  R rho = A.rho[ps];
+  double u = A.u[ps];
+  if (predict) {  // the parent's predictor, as k_reorder_pred applies it
+    const double hdt = K.dt / 2.0;
+    u = u + hdt * (double)A.du[ps];
+    if (u < 0.0) u = 0.0;
+    if (K.density_method == 0) rho = rho + (R)hdt * A.drho[ps];
+    const R2 a = A.a[ps];
+    v.x = v.x + (R)hdt * a.x;
+    v.y = v.y + (R)hdt * a.y;
+  }
+  const R2 nrm = A.v[ws];
+  R d2 = (R)0;
+  d2 = d2 + v.x * nrm.x;
+  d2 = d2 + v.y * nrm.y;
+  R2 vg;
+  vg.x = v.x - (R)2 * nrm.x * d2;
+  vg.y = v.y - (R)2 * nrm.y * d2;
+  Bs.v[t] = vg;
+  Bs.rho[t] = rho;
+  Bs.u[t] = u;
+  if constexpr (sizeof(R) == 4) Bs.uf[t] = (float)u;
+  Bs.p[t] = A.p[ps];
+  Bs.flag[t] = 0;
+  G.gpid[t] = A.id[ps];
+}
+
 // Permute every field from state A to state B (per-routine link_list);
 // fill = 0 when the sort already wrote cell_start.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_reorder_all(St<R> A, St<R> Bs, KParams K,
                                                        const uint32_t* __restrict__ skey,
                                                        const uint32_t* __restrict__ sval,
-                                                       int* __restrict__ cstart, int fill) {
+                                                       int* __restrict__ cstart, int fill,
+                                                       Ghosts G) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   const bool valid = t < K.n;
   if (fill) fill_cell_start(skey, t, valid, K.n, K.ncell, cstart);
   if (!valid) return;
   const int s = (int)sval[t];
+  if (G.xprev) {
+    const int id = A.id[s];
+    if (id >= K.nb) {
+      ghost_slot(A, Bs, K, G, s, t, id, 0);
+      return;
+    }
+    G.xprev[t] = A.xy[s];
+    if (id < K.np) G.sid[id] = t;
+  }
   Bs.id[t] = A.id[s];
   Bs.xy[t] = A.xy[s];
   if constexpr (sizeof(R) == 4) {
@@ -137,7 +219,7 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
                                                         const uint32_t* __restrict__ skey,
                                                         const uint32_t* __restrict__ sval,
                                                         int* __restrict__ cstart, int predict,
-                                                        int fill) {
+                                                        int fill, Ghosts G) {
   using R2 = typename V2<R>::type;
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   const bool valid = t < K.n;
@@ -145,6 +227,14 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
   if (!valid) return;
   const int s = (int)sval[t];
   const int id = A.id[s];
+  if (G.xprev) {
+    if (id >= K.nb) {
+      ghost_slot(A, Bs, K, G, s, t, id, predict);
+      return;
+    }
+    G.xprev[t] = A.xy[s];
+    if (id < K.np) G.sid[id] = t;
+  }
   Bs.id[t] = id;
   const double2 q = A.xy[s];
   Bs.xy[t] = q;
@@ -881,7 +971,9 @@ __global__ __launch_bounds__(BLOCK) void k_corrector(St<R> S, KParams K, int its
         S.xy[i] = q;
       }
     }
-    if (keys) {
+    if (keys && id >= K.nb) {
+      key = (uint32_t)K.ncell;  // this step's ghost: out of the next sort
+    } else if (keys) {
       int ix, iy;
       bool ok;
       key = (uint32_t)cell_of(K, q.x, q.y, ix, iy, ok);

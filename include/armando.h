@@ -68,6 +68,12 @@ extern "C" {
                               cells wrap and positions re-enter on the other side
                               (nx >= 3; the reference's wrap is commented out,
                               ArmandoList/armando2Dlist.cu:856-870) */
+#define ARM_GHOSTS 512     /* mirror ghosts [CERNsource/ghost.f:1-86]: the virtual
+                              particles carry the unit wall normal in v; every step
+                              each real particle with a virtual neighbour gets a ghost
+                              mirrored across the nearest one's plane (2D, without
+                              ARM_DBC, ARM_PERIODIC_X or sle; the capacity must hold
+                              np + nv plus twice the ghost count) */
 
 /* equations of state */
 #define ARM_EOS_TAIT 0   /* p = rho0 c0^2/7 ((rho/rho0)^7 - 1), c = c0
@@ -174,8 +180,27 @@ int arm_single_step(arm_ctx* ctx);
 int arm_predictor(arm_ctx* ctx);
 int arm_corrector(arm_ctx* ctx, int its);
 /* time_integration [CERNsource/time_integration.f:131-235] for steps
-   its0 .. its0+nsteps-1 (refresh_step = 1), fused on the device. */
+   its0 .. its0+nsteps-1 (refresh_step = 1), fused on the device.  With
+   ARM_GHOSTS every step makes its ghosts after the predictor, as
+   time_integration.f:157-161 does. */
 int arm_time_integration(arm_ctx* ctx, int its0, int nsteps);
+
+/* ghost_particles(np, nv, ng, x, v, h, mass, rho, p, niac, pair_i, pair_j,
+   index_ghost) [CERNsource/ghost.f:1]: for each real particle, the virtual
+   particles within 2h at the last arm_link_list are its candidates and the
+   nearest one now (ties: smaller id) defines the mirror; the ghost gets
+   x - 2 n ((x - x_w).n), v - 2 n (v.n) and the parent's rho, p, u.  Ghosts
+   are numbered in parent order (ids np+nv+g) and replace the previous set
+   at the next arm_link_list (which must follow; an arm_link_list without a
+   new set drops the ghosts); after the EOS they take the parent's rho, p,
+   u again [single_step.f:188-196].  ng (may be NULL) receives their count. */
+int arm_ghost_particles(arm_ctx* ctx, int* ng);
+/* Number of ghosts of the current state. */
+int arm_ghost_count(arm_ctx* ctx, int* ng);
+/* The ghosts in number order (any pointer may be NULL): x, v as (2, ng),
+   rho, p, u, and the parent's id. */
+int arm_download_ghosts(arm_ctx* ctx, double* x, double* v, double* rho, double* p, double* u,
+                        int32_t* parent);
 
 /* Discrete adjoint of time_integration (no counterpart in the reference).
    From the current state as step 1, runs nsteps forward, storing the state
