@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -53,7 +54,8 @@ struct arm_ctx {
   bool ghost_pending = false;
   bool have_ranges = false;  // cstart describes the current slot order
   Ghosts G{};
-  int* d_ng = nullptr;
+  int* d_ng = nullptr;   // = d_err + 2: read back with the error counts
+  int* h_pin = nullptr;  // pinned host copy of d_err[0..2]
   bool loaded = false, sorted = false, have_rates = false;
   bool tiled = true;  // LDS-tiled gathers (false: direct global-memory gathers)
   struct AdjRun* adj = nullptr;  // checkpoints of the last arm_rollout_forward
@@ -537,7 +539,7 @@ template <typename R> int make_ghosts(arm_ctx* c, bool keys) {
   if (np == 0) return ARM_OK;
   PassTimer tm(c, ARM_PASS_KEYS);
   hipLaunchKernelGGL(k_ghost_select<R>, dim3(grid_of(n_old)), dim3(BLOCK), 0, c->stream, A,
-                     c->K, c->G, c->cstart, n_old);
+                     c->K, c->G, n_old);
   const int tiles = (np + CS_TILE - 1) / CS_TILE;
   hipLaunchKernelGGL(k_cs_tile_sum, dim3(tiles), dim3(CS_THREADS), 0, c->stream, c->G.gflag, np,
                      c->tsum);
@@ -547,10 +549,11 @@ template <typename R> int make_ghosts(arm_ctx* c, bool keys) {
                      c->G, n_old, c->cap, keys ? 1 : 0, keys ? key_out(c) : KeyOut{}, c->d_err,
                      c->d_ng);
   HIPCHK(c, hipGetLastError());
-  int hv[3] = {0, 0, 0};
-  HIPCHK(c, hipMemcpyAsync(hv, c->d_ng, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(hv + 1, c->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  // one copy: the error counts and the ghost count (d_ng = d_err + 2)
+  HIPCHK(c, hipMemcpyAsync(c->h_pin, c->d_err, 3 * sizeof(int), hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int hv[3] = {c->h_pin[2], c->h_pin[0], c->h_pin[1]};
   if ((long long)n_old + hv[0] > c->cap) {
     c->loaded = false;
     return fail(c, ARM_ERR_CAPACITY, "%d ghosts on %d live particles exceed capacity %d "
@@ -2018,9 +2021,20 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     if (!rc) rc = dalloc(c, &G.gwall, capacity);
     if (!rc) rc = dalloc(c, &G.gpid, capacity);
     if (!rc) rc = dalloc(c, &G.sid, capacity);
-    if (!rc) rc = dalloc(c, &c->d_ng, 1);
+    if (!rc) rc = dalloc(c, &G.vnear, (size_t)ncell);
+    if (!rc) rc = dalloc(c, &G.vstart, (size_t)ncell + 1);
+    if (!rc) rc = dalloc(c, &G.vxy, capacity);
+    if (!rc) rc = dalloc(c, &G.vnrm, capacity);
+    if (!rc) rc = dalloc(c, &G.vid, capacity);
+    if (!rc) {
+      void* hp = nullptr;
+      e = hipHostMalloc(&hp, 4 * sizeof(int), hipHostMallocDefault);
+      if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
+      c->h_pin = static_cast<int*>(hp);
+    }
   }
-  if (!rc) rc = dalloc(c, &c->d_err, 2);
+  if (!rc) rc = dalloc(c, &c->d_err, 3);
+  if (!rc) c->d_ng = c->d_err + 2;
   if (!rc) rc = dalloc(c, &c->d_part, (size_t)grid_of(capacity) * 6);
   if (!rc) rc = dalloc(c, &c->d_cnt, capacity);
   if (!rc) rc = dalloc(c, &c->nbm, capacity);
@@ -2034,7 +2048,7 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     c->pack3_ap = q;
   }
   if (!rc) {
-    e = hipMemset(c->d_err, 0, 2 * sizeof(int));
+    e = hipMemset(c->d_err, 0, 3 * sizeof(int));
     if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipMemset: %s", hipGetErrorString(e));
   }
   if (!rc) {
@@ -2055,6 +2069,7 @@ void arm_destroy(arm_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -2087,8 +2102,60 @@ int arm_upload(arm_ctx* c, int np, int nv, const double* x, const double* v, con
     const double2* xy = is32(c) ? c->A32.xy : c->A64.xy;
     hipError_t e = hipMemcpyAsync(c->G.xprev, xy, sizeof(double2) * (size_t)c->n,
                                   hipMemcpyDeviceToDevice, c->stream);
+    // the virtual particles never move: their (cell, id) list, once
+    {
+      const int ncell = c->K.ncell;
+      std::vector<std::pair<long long, int>> kv;
+      kv.reserve(nv);
+      for (int j = np; j < np + nv; ++j) {
+        const double tx = (x[2 * j] - c->K.ox) / c->K.cell;
+        const double ty = (x[2 * j + 1] - c->K.oy) / c->K.cell;
+        long long key = ncell;  // outside the grid: never a candidate
+        if (tx >= 0.0 && ty >= 0.0 && tx < (double)c->K.nx && ty < (double)c->K.ny)
+          key = (long long)(int)tx + (long long)(int)ty * c->K.nx;
+        kv.emplace_back(key, j);
+      }
+      std::sort(kv.begin(), kv.end());
+      std::vector<int> vstart((size_t)ncell + 1, 0), vid(nv);
+      std::vector<double2> vxy(nv), vnrm(nv);
+      size_t q = 0;
+      for (int cc = 0; cc <= ncell; ++cc) {
+        while (q < kv.size() && kv[q].first < cc) ++q;
+        vstart[cc] = (int)q;
+      }
+      for (int k = 0; k < nv; ++k) {
+        const int j = kv[k].second;
+        vid[k] = j;
+        vxy[k] = make_double2(x[2 * j], x[2 * j + 1]);
+        vnrm[k] = make_double2(v ? v[2 * j] : 0.0, v ? v[2 * j + 1] : 0.0);
+      }
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(c->G.vstart, vstart.data(), sizeof(int) * vstart.size(),
+                           hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess && nv > 0)
+        e = hipMemcpyAsync(c->G.vid, vid.data(), sizeof(int) * nv, hipMemcpyHostToDevice,
+                           c->stream);
+      if (e == hipSuccess && nv > 0)
+        e = hipMemcpyAsync(c->G.vxy, vxy.data(), sizeof(double2) * nv, hipMemcpyHostToDevice,
+                           c->stream);
+      if (e == hipSuccess && nv > 0)
+        e = hipMemcpyAsync(c->G.vnrm, vnrm.data(), sizeof(double2) * nv,
+                           hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    // the cells near the (fixed) virtual particles
+    if (e == hipSuccess) e = hipMemsetAsync(c->G.vnear, 0, (size_t)c->K.ncell, c->stream);
+    if (e == hipSuccess) {
+      if (is32(c))
+        hipLaunchKernelGGL(k_ghost_mark<float>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                           c->A32, c->K, c->G, c->n);
+      else
+        hipLaunchKernelGGL(k_ghost_mark<double>, dim3(grid_of(c->n)), dim3(BLOCK), 0,
+                           c->stream, c->A64, c->K, c->G, c->n);
+      e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "xprev copy: %s", hipGetErrorString(e));
+    if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "ghost set-up: %s", hipGetErrorString(e));
   }
   c->loaded = (rc == ARM_OK);
   return rc;

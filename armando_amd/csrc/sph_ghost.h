@@ -3,10 +3,13 @@
 // The virtual particles (ids np .. nb-1) carry the unit wall normal in v and
 // never move.  Every step, before the sort (time_integration.f:157-161 calls
 // ghost_particles after the predictor and before link_list):
+//   k_ghost_mark    (once per upload) the cells near a virtual particle
 //   k_ghost_select  each real particle takes, among the virtual particles
 //                   within 2h of its position at the last sort (the pair
 //                   list ghost_particles walks), the nearest one at its
-//                   current position; ties go to the smaller id
+//                   current position; ties go to the smaller id.  The
+//                   virtual particles come from a fixed (cell, id) list built
+//                   at upload.
 //   (scan)          the flags by id give each ghost its number, so ghosts
 //                   are numbered in the order of their parents, as ghost.f
 //                   appends them
@@ -23,9 +26,31 @@
 
 namespace arm {
 
+// The cells whose 3x3 block holds a virtual particle (the walls never move,
+// so this is set once per upload): only particles there can have
+// candidates.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_ghost_mark(St<R> A, KParams K, Ghosts G, int n) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= n) return;
+  const int id = A.id[s];
+  if (id < K.np || id >= K.nb) return;
+  const double2 q = A.xy[s];
+  int ix, iy;
+  bool ok;
+  cell_of(K, q.x, q.y, ix, iy, ok);
+  if (!ok) return;
+  for (int dy = -1; dy <= 1; ++dy)
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int cx = ix + dx, cy = iy + dy;
+      if (cx >= 0 && cx < K.nx && cy >= 0 && cy < K.ny) G.vnear[cy * K.nx + cx] = 1;
+    }
+}
+
+// gsel[s]: the chosen virtual particle's index in the fixed (cell, id)
+// list, or -1.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_ghost_select(St<R> A, KParams K, Ghosts G,
-                                                        const int* __restrict__ cstart,
                                                         int n_old) {
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   if (s >= n_old) return;
@@ -39,13 +64,12 @@ __global__ __launch_bounds__(BLOCK) void k_ghost_select(St<R> A, KParams K, Ghos
   const double cut = (2.0 * K.h) * (2.0 * K.h);
   int best = -1, bid = 0;
   double bd = 0.0;
-  if (ok) {
-    const Rows rw = rows_of(K, key, cstart);
+  if (ok && G.vnear[key]) {
+    const Rows rw = rows_of(K, key, G.vstart);
     for (int k = 0; k < 3; ++k)
       for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
-        const int idt = A.id[t];
-        if (idt < K.np || idt >= K.nb) continue;
-        const double2 w = A.xy[t];
+        const int idt = G.vid[t];
+        const double2 w = G.vxy[t];
         const double dx = xp.x - w.x, dy = xp.y - w.y;
         double r2 = dx * dx;
         r2 = r2 + dy * dy;
@@ -87,8 +111,8 @@ __global__ __launch_bounds__(BLOCK) void k_ghost_make(St<R> A, KParams K, Ghosts
       if (valid) {
         const int t = G.gsel[s];
         const double2 q = A.xy[s];
-        const double2 w = A.xy[t];
-        const double nx = (double)A.v[t].x, ny = (double)A.v[t].y;
+        const double2 w = G.vxy[t];
+        const double nx = G.vnrm[t].x, ny = G.vnrm[t].y;
         double d1 = 0.0;
         d1 = d1 + (q.x - w.x) * nx;
         d1 = d1 + (q.y - w.y) * ny;

@@ -110,9 +110,16 @@ struct Ghosts {
   int* gflag;      // by real id: 1 if the particle gets a ghost
   int* grank;      // by real id: its ghost's number (exclusive scan of gflag)
   int* gpar;       // by staging slot: the parent's slot (order before the sort)
-  int* gwall;      // by staging slot: the virtual particle's slot (same order)
+  int* gwall;      // by staging slot: the virtual particle (index in vxy)
   int* gpid;       // by slot of the sorted order: a ghost's parent id
   int* sid;        // by real id: slot in the sorted order
+  uint8_t* vnear;  // by cell: 1 if a virtual particle lies in its 3x3 block
+  // the virtual particles (fixed) in (cell, id) order: positions, wall
+  // normals, ids and the cell ranges vstart[ncell + 1]
+  int* vstart;
+  double2* vxy;
+  double2* vnrm;
+  int* vid;
 };
 
 // The fields of ghost slot t (sorted order) from staging slot s: position
@@ -150,7 +157,9 @@ __device__ __forceinline__ void ghost_slot(const St<R>& A, St<R>& Bs, const KPar
     v.x = v.x + (R)hdt * a.x;
     v.y = v.y + (R)hdt * a.y;
   }
-  const R2 nrm = A.v[ws];
+  R2 nrm;
+  nrm.x = (R)G.vnrm[ws].x;
+  nrm.y = (R)G.vnrm[ws].y;
   R d2 = (R)0;
   d2 = d2 + v.x * nrm.x;
   d2 = d2 + v.y * nrm.y;

@@ -25,8 +25,12 @@ dx = 1e-4 m: 2,000,000 fluid + 24,000 wall particles; Poiseuille flow at
 0.5 m/s peak, physical viscosity, Cleary-Monaghan conduction, artificial
 heat, Gaussian heat source) for 2000 steps in both forms.
 
+The ghost leg runs the 720k dam-break column against one row of virtual
+particles carrying the wall normals, with mirror ghosts rebuilt every step
+(ghost.f), for 500 steps in both forms.
+
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp64]
-                  [--no-adjoint] [--no-3d] [--no-channel]
+                  [--no-adjoint] [--no-3d] [--no-channel] [--no-ghosts]
 """
 import argparse
 import json
@@ -82,6 +86,8 @@ def parse():
                     help="steps of the 3D FP32 adjoint rollout (0: skip)")
     ap.add_argument("--no-channel", action="store_true", help="skip the heated channel leg")
     ap.add_argument("--steps-channel", type=int, default=2000)
+    ap.add_argument("--no-ghosts", action="store_true", help="skip the mirror-ghost leg")
+    ap.add_argument("--steps-ghosts", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU work for the baseline sample")
     ap.add_argument("--scale", type=float, default=1.0,
@@ -319,6 +325,48 @@ def channel_leg(steps, warmup=5):
     }
 
 
+def ghost_leg(steps, warmup=5):
+    """The 720k dam-break column against virtual walls with mirror ghosts,
+    both forms: whole steps of time_integration (the ghost count is read
+    back every step), per-pass device times and the ghost count."""
+    import armando_amd
+    from armando_amd import cases
+
+    res = {}
+    for name, prec in (("fp32", cases.FP32), ("fp64", cases.FP64)):
+        case = cases.ghost_tank(col_w=0.6, col_h=1.2, tank_w=3.2, precision=prec)
+        sim = armando_amd.Simulation(case)
+        sim.time_integration(1, warmup)
+        t0 = time.perf_counter()
+        sim.time_integration(1 + warmup, steps)
+        wall = time.perf_counter() - t0
+        ng = sim.ghost_count()
+        sim.set_timing(True)
+        sim.reset_pass_times()
+        sim.time_integration(1 + warmup + steps, 20)
+        passes = {k: ms / max(cnt, 1) for k, (ms, cnt) in sim.pass_times().items()}
+        sim.set_timing(False)
+        res[name] = dict(case=case, wall=wall, passes=passes, ng=ng)
+        sim.close()
+    r32, r64 = res["fp32"], res["fp64"]
+    case = r32["case"]
+    n = case["n"]
+    return {
+        "workload": (f"2D dam break, {case['np']} fluid + {case['nv']} virtual wall particles "
+                     f"with mirror ghosts (ghost.f)"),
+        "value": n * steps / r32["wall"],
+        "unit": "particle-steps/s",
+        "dtype": "f32",
+        "ms_per_step": 1e3 * r32["wall"] / steps,
+        "steps": steps,
+        "ghosts_at_end": r32["ng"],
+        "passes_ms": r32["passes"],
+        "fp64": {"value": n * steps / r64["wall"], "ms_per_step": 1e3 * r64["wall"] / steps,
+                 "ghosts_at_end": r64["ng"], "passes_ms": r64["passes"]},
+        "speedup_fp32_over_fp64": r64["wall"] / r32["wall"],
+    }
+
+
 def cpu_baseline(case, seconds):
     """The oracle's faithful serial restatement of the reference loop
     (half pair list, scatter; one core) on the same seeded input."""
@@ -334,10 +382,25 @@ def cpu_baseline(case, seconds):
     t0 = time.perf_counter()
     orc.run(P, st, 2, k, gather=False)
     dt = time.perf_counter() - t0
-    return dict(value=case["n"] * k / dt, unit="particle-steps/s", cores=1, kind="port",
-                sample=f"steps 2..{k + 1} of the same 720k dam break, oracle half-list "
-                       f"restatement of CERNsource time_integration.f, 1 thread, "
-                       f"{dt:.1f} s")
+    out = dict(value=case["n"] * k / dt, unit="particle-steps/s", cores=1, kind="port",
+               sample=f"steps 2..{k + 1} of the same 720k dam break, oracle half-list "
+                      f"restatement of CERNsource time_integration.f, 1 thread, "
+                      f"{dt:.1f} s")
+    # the oracle's gather form on every host core the job may use (OpenMP)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    orc.lib().orc_set_threads(threads)
+    t0 = time.perf_counter()
+    orc.run(P, st, k + 2, 1, gather=True)
+    one = time.perf_counter() - t0
+    kk = max(1, int(0.75 * seconds / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    orc.run(P, st, k + 3, kk, gather=True)
+    dt2 = time.perf_counter() - t0
+    out["all_cores"] = dict(value=case["n"] * kk / dt2, unit="particle-steps/s", cores=threads,
+                            kind="port",
+                            sample=f"{kk} further steps, oracle gather form (cell order), "
+                                   f"OpenMP {threads} threads, {dt2:.1f} s")
+    return out
 
 
 def main():
@@ -435,6 +498,8 @@ def main():
         out["dam_break_3d"] = dam3d_leg(args.steps_3d, adjoint_steps=args.steps_3d_adjoint)
     if not args.no_channel:
         out["heated_channel"] = channel_leg(args.steps_channel)
+    if not args.no_ghosts:
+        out["ghost_walls"] = ghost_leg(args.steps_ghosts)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out))
