@@ -70,6 +70,12 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE x 1.0): 115.1 MB read + 15.1 MB written.  Measured on the
 # committed kernels; profiles/dam720k_pmc.md holds the runs.
 FORCE_TRAFFIC_BYTES = 130.2e6
+# Wavefront VALU instructions per launch of the same kernel (SQ_INSTS_VALU,
+# same runs): with 4 cycles per wave64 instruction on each of the 1024 SIMDs
+# at 2.4 GHz they give the fraction of the launch the VALUs issue, the bound
+# of this gather.
+FORCE_VALU_INSTS = 3.94e7
+SIMDS, CLOCK_HZ = 1024, 2.4e9
 
 
 def parse():
@@ -463,7 +469,7 @@ def main():
         "pair_interactions_per_s": 2 * r32["niac"] / t_force,
         "passes_ms": {k: pass_ms(r32, k) for k in r32["passes"]},
         "roofline": {
-            "kernel": "k_force<float,...> (internal_force + art_visc + art_heat + av_vel)",
+            "kernel": "k_force_tiled<float,...> (internal_force + art_visc + art_heat + av_vel)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -472,6 +478,8 @@ def main():
             "traffic": FORCE_TRAFFIC_BYTES if args.scale == 1.0 else None,
             "bytes_per_particle": BYTES[("force", 1)],
             "avg_launch_ms": t_force * 1e3,
+            "valu_issue_frac": (FORCE_VALU_INSTS * 4 / SIMDS / (t_force * CLOCK_HZ)
+                                if args.scale == 1.0 else None),
         },
     }
     if r64 is not None:
