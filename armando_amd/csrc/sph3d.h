@@ -275,14 +275,12 @@ __device__ __forceinline__ void dens3_pair(const KParams& K, R dx, R dy, R dz, R
   const R q = fsqrt(r2) * (R)K.invh;
   const R tq = q - (R)2;
   const R F = (R)K.C1 * (tq * tq * tq);
-  R vcc = dvx * (F * dx);
-  vcc = vcc + dvy * (F * dy);
-  vcc = vcc + dvz * (F * dz);
+  const R vcc = fmr(dvz, F * dz, fmr(dvy, F * dy, dvx * (F * dx)));
   sum.add((R)K.mass * vcc);
   if (SUM) {
-    const R s = (R)1 - (R)0.5 * q;
+    const R s = fmr((R)-0.5, q, (R)1);
     const R s2 = s * s;
-    rs.add((R)K.mass * ((R)K.wnorm * (s2 * s2) * ((R)1 + (R)2 * q)));
+    rs.add((R)K.mass * ((R)K.wnorm * (s2 * s2) * fmr((R)2, q, (R)1)));
   }
 }
 
@@ -391,29 +389,28 @@ __device__ __forceinline__ void force3_pair(const KParams& K, ForceOwn3<R>& o, R
   const R tq = q - (R)2;
   const R F = (R)K.C1 * (tq * tq * tq);
   const R dvx = o.v.x - vjx, dvy = o.v.y - vjy, dvz = o.v.z - vjz;
-  R vr = dvx * dx;
-  vr = vr + dvy * dy;
-  vr = vr + dvz * dz;
+  const R vr = fmr(dvz, dz, fmr(dvy, dy, dvx * dx));
   const R inv_mrho = (R)2 * o.Q * Qj * frcp(o.Q + Qj);
   R coef = 0;
-  if (PRESS) coef = (K.pa_sph == 0) ? (o.A * Qj + Aj * o.Q) : (o.A * o.Q + Aj * Qj);
+  if (PRESS) coef = (K.pa_sph == 0) ? fmr(o.A, Qj, Aj * o.Q) : fmr(o.A, o.Q, Aj * Qj);
   R inv_ea = 0;
   if (AV) {
     inv_ea = frcp(r2 + (R)K.av_e2);
     if (vr < (R)0) {
       const R phi = (R)K.h * vr * inv_ea;
-      coef = coef + ((R)K.av_beta * phi - (R)K.av_alpha * (R)K.c0) * phi * inv_mrho;
+      const R pc = fmr((R)K.av_beta, phi, -((R)K.av_alpha * (R)K.c0));
+      coef = fmr(pc * phi, inv_mrho, coef);
     }
   }
   if (PRESS || AV) {
     const R G = m * F * coef;
-    o.fx = o.fx - G * dx;
-    o.fy = o.fy - G * dy;
-    o.fz = o.fz - G * dz;
+    o.fx = fmr(-G, dx, o.fx);
+    o.fy = fmr(-G, dy, o.fy);
+    o.fz = fmr(-G, dz, o.fz);
     R dut = (R)0.5 * G * vr;
     if (AH) {
       const R inv_eh = (AV && K.same_e2) ? inv_ea : frcp(r2 + (R)K.ah_e2);
-      dut = dut + m * (o.psi + psij) * F * r2 * inv_mrho * inv_eh * (o.u - uj);
+      dut = fmr(m * (o.psi + psij) * F * r2 * inv_mrho * inv_eh, o.u - uj, dut);
     }
     o.du.add(dut);
   } else if (AH) {
@@ -421,13 +418,13 @@ __device__ __forceinline__ void force3_pair(const KParams& K, ForceOwn3<R>& o, R
     o.du.add(m * (o.psi + psij) * F * r2 * inv_mrho * inv_eh * (o.u - uj));
   }
   if (XS) {
-    const R s = (R)1 - (R)0.5 * q;
+    const R s = fmr((R)-0.5, q, (R)1);
     const R s2 = s * s;
-    const R w = (R)K.wnorm * (s2 * s2) * ((R)1 + (R)2 * q);
+    const R w = (R)K.wnorm * (s2 * s2) * fmr((R)2, q, (R)1);
     const R wf = (R)K.eps_m * inv_mrho * w;
-    o.xsx = o.xsx - wf * dvx;
-    o.xsy = o.xsy - wf * dvy;
-    o.xsz = o.xsz - wf * dvz;
+    o.xsx = fmr(-wf, dvx, o.xsx);
+    o.xsy = fmr(-wf, dvy, o.xsy);
+    o.xsz = fmr(-wf, dvz, o.xsz);
   }
 }
 

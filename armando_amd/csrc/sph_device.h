@@ -161,6 +161,14 @@ __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(
 __device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// a*b + c in the pair bodies: one fused rounding in the FP32 form; in the
+// FP64 form a rounded product and a rounded sum, the oracle's arithmetic
+// (the callers order the operands so that this is the same sum as the
+// reference's expression).  The neighbour criterion (dx, dy, r^2) never
+// uses it, so the FP32 pair sets stay those of the unfused restatement.
+__device__ __forceinline__ float fmr(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmr(double a, double b, double c) { return a * b + c; }
+
 // Density in the pair arithmetic: absolute (FP64 form) or rho0 + perturbation.
 __device__ __forceinline__ double dens(const KParams&, double r) { return r; }
 __device__ __forceinline__ float dens(const KParams& K, float r) { return (float)K.rho0 + r; }

@@ -380,13 +380,12 @@ __device__ __forceinline__ void dens_pair(const KParams& K, R dx, R dy, R r2, R 
   const R tq = q - (R)2;
   const R F = (R)K.C1 * (tq * tq * tq);
   const R dwx = F * dx, dwy = F * dy;
-  R vcc = dvx * dwx;
-  vcc = vcc + dvy * dwy;
+  const R vcc = fmr(dvy, dwy, dvx * dwx);
   sum.add((R)K.mass * vcc);
   if (SUM) {
-    const R s = (R)1 - (R)0.5 * q;
+    const R s = fmr((R)-0.5, q, (R)1);
     const R s2 = s * s;
-    rs.add((R)K.mass * ((R)K.wnorm * (s2 * s2) * ((R)1 + (R)2 * q)));
+    rs.add((R)K.mass * ((R)K.wnorm * (s2 * s2) * fmr((R)2, q, (R)1)));
   }
 }
 
@@ -528,8 +527,8 @@ __device__ __forceinline__ void visc_pair(const KParams& K, ForceOwn<R>& o, R mF
     syy = qq * syy;
     sxy = qq * sxy;
   }
-  o.fx = o.fx + mF * (sxx * dx + sxy * dy);
-  o.fy = o.fy + mF * (sxy * dx + syy * dy);
+  o.fx = fmr(mF, fmr(sxy, dy, sxx * dx), o.fx);
+  o.fy = fmr(mF, fmr(syy, dy, sxy * dx), o.fy);
 }
 
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI = false>
@@ -541,33 +540,34 @@ __device__ __forceinline__ void force_pair(const KParams& K, ForceOwn<R>& o, R d
   const R tq = q - (R)2;
   const R F = (R)K.C1 * (tq * tq * tq);
   const R dvx = o.v.x - vjx, dvy = o.v.y - vjy;
-  const R vr = dvx * dx + dvy * dy;
+  const R vr = fmr(dvy, dy, dvx * dx);
   const R inv_mrho = (R)2 * o.Q * Qj * frcp(o.Q + Qj);
   R coef = 0;
-  if (PRESS) coef = (K.pa_sph == 0) ? (o.A * Qj + Aj * o.Q) : (o.A * o.Q + Aj * Qj);
+  if (PRESS) coef = (K.pa_sph == 0) ? fmr(o.A, Qj, Aj * o.Q) : fmr(o.A, o.Q, Aj * Qj);
   R inv_ea = 0;
   if (AV) {
     inv_ea = frcp(r2 + (R)K.av_e2);
     if (vr < (R)0) {
       const R phi = (R)K.h * vr * inv_ea;
-      coef = coef + ((R)K.av_beta * phi - (R)K.av_alpha * (R)K.c0) * phi * inv_mrho;
+      const R pc = fmr((R)K.av_beta, phi, -((R)K.av_alpha * (R)K.c0));
+      coef = fmr(pc * phi, inv_mrho, coef);
     }
   }
   if (VI) visc_pair(K, o, m * F, dx, dy, Qj, wj);
   if (PRESS || AV) {
     const R G = m * F * coef;
-    o.fx = o.fx - G * dx;
-    o.fy = o.fy - G * dy;
+    o.fx = fmr(-G, dx, o.fx);
+    o.fy = fmr(-G, dy, o.fy);
     R dut = (R)0.5 * G * vr;
     R inv_eh = (R)0;
     if (AH) {
       inv_eh = (AV && K.same_e2) ? inv_ea : frcp(r2 + (R)K.ah_e2);
-      dut = dut + m * (o.psi + psij) * F * r2 * inv_mrho * inv_eh * (o.u - uj);
+      dut = fmr(m * (o.psi + psij) * F * r2 * inv_mrho * inv_eh, o.u - uj, dut);
     }
     if (PRESS && (K.flags & 128)) {
       // Cleary-Monaghan conduction: m cond Q_i Q_j (u_i - u_j) F r^2/(r^2 + (h eta)^2)
       if (!AH) inv_eh = frcp(r2 + (R)K.ah_e2);
-      dut = dut + m * (R)K.cond * o.Q * Qj * F * r2 * inv_eh * (o.u - uj);
+      dut = fmr(m * (R)K.cond * o.Q * Qj * F * r2 * inv_eh, o.u - uj, dut);
     }
     o.du.add(dut);
   } else if (AH) {
@@ -575,12 +575,12 @@ __device__ __forceinline__ void force_pair(const KParams& K, ForceOwn<R>& o, R d
     o.du.add(m * (o.psi + psij) * F * r2 * inv_mrho * inv_eh * (o.u - uj));
   }
   if (XS) {
-    const R s = (R)1 - (R)0.5 * q;
+    const R s = fmr((R)-0.5, q, (R)1);
     const R s2 = s * s;
-    const R w = (R)K.wnorm * (s2 * s2) * ((R)1 + (R)2 * q);
+    const R w = (R)K.wnorm * (s2 * s2) * fmr((R)2, q, (R)1);
     const R wf = (R)K.eps_m * inv_mrho * w;
-    o.xsx = o.xsx - wf * dvx;
-    o.xsy = o.xsy - wf * dvy;
+    o.xsx = fmr(-wf, dvx, o.xsx);
+    o.xsy = fmr(-wf, dvy, o.xsy);
   }
 }
 
@@ -819,12 +819,12 @@ __device__ __forceinline__ void tens_pair(const KParams& K, TensOwn<R>& o, R dx,
   const R dwx = F * dx, dwy = F * dy;
   const R mq = (R)K.mass * Qj;
   const R a = dvx * dwx, b = dvy * dwy;
-  const R hxx = (R)(2.0 / 3.0) * ((R)2 * a - b);
-  const R hyy = (R)(2.0 / 3.0) * ((R)2 * b - a);
-  const R hxy = dvx * dwy + dvy * dwx;
-  o.xx = o.xx + mq * hxx;
-  o.yy = o.yy + mq * hyy;
-  o.xy = o.xy + mq * hxy;
+  const R hxx = (R)(2.0 / 3.0) * fmr((R)2, a, -b);
+  const R hyy = (R)(2.0 / 3.0) * fmr((R)2, b, -a);
+  const R hxy = fmr(dvy, dwx, dvx * dwy);
+  o.xx = fmr(mq, hxx, o.xx);
+  o.yy = fmr(mq, hyy, o.yy);
+  o.xy = fmr(mq, hxy, o.xy);
 }
 
 template <typename R>
