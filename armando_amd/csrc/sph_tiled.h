@@ -332,7 +332,10 @@ __device__ __forceinline__ void for_neighbours(const KParams& K, const Segment<C
   }
 }
 
-// Same walk driven by masks recorded by the density pass.
+// Same walk driven by masks recorded by the density pass: one loop over the
+// three rows' bits in row order (the order of the per-row walk), so that a
+// wavefront runs for its longest lane's neighbour count rather than for the
+// sum of the per-row maxima.
 template <typename R, int CW, typename Visit>
 __device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& g,
                                            const int (*cs)[CW + 3], uint32_t key, uint4 nm,
@@ -340,18 +343,20 @@ __device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& 
   const int nx = K.nx;
   const int cx = (int)key - g.cy * nx;
   const int a0 = (cx > 0 ? cx - 1 : 0) - g.slo;
-  const uint32_t rm[3] = {nm.x, nm.y, nm.z};
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    if (g.rl[k] == 0) continue;
-    const R uy = (R)row_uy(K, k);
-    const int l0 = g.lo[k] - g.rs[k] + cs[k][a0];
-    uint32_t m = rm[k];
-    while (m) {
-      const int b = __builtin_ctz(m);
-      m &= m - 1;
-      visit(l0 + b, uy);
-    }
+  const int l0 = g.lo[0] - g.rs[0] + cs[0][a0];
+  const int l1 = g.lo[1] - g.rs[1] + cs[1][a0];
+  const int l2 = g.lo[2] - g.rs[2] + cs[2][a0];
+  uint32_t m0 = nm.x, m1 = nm.y, m2 = nm.z;
+  while (m0 | m1 | m2) {
+    const bool s0 = m0 != 0u, s1 = !s0 && m1 != 0u;
+    const uint32_t m = s0 ? m0 : (s1 ? m1 : m2);
+    const int l = (s0 ? l0 : (s1 ? l1 : l2)) + (int)__builtin_ctz(m);
+    const R uy = s0 ? (R)row_uy(K, 0) : (s1 ? (R)row_uy(K, 1) : (R)row_uy(K, 2));
+    const uint32_t rest = m & (m - 1u);
+    m0 = s0 ? rest : m0;
+    m1 = s1 ? rest : m1;
+    m2 = (s0 || s1) ? m2 : rest;
+    visit(l, uy);
   }
 }
 
