@@ -510,20 +510,41 @@ __device__ __forceinline__ void force3_direct(St3<R>& S, const Pack3<R>& P, cons
   force3_begin(S, K, i, o);
   const Rows9 rw = rows3_of(K, skey[i], cstart);
   if (nbm.w && nbm.get(M3_OK, i)) {
-    uint32_t mw[M3_OK];
+    // One loop over every set bit of the 27 words in (row, word, bit) order,
+    // the order of the per-word walk: a wavefront runs for its longest
+    // lane's neighbour count instead of the sum of the 27 per-word maxima.
+    // Each lane keeps its words and row starts in its own LDS column (they
+    // are indexed by the lane's position in the walk) and a bitmap of its
+    // non-empty words.
+    __shared__ uint32_t lw[M3_OK][BLOCK];
+    __shared__ int lt[9][BLOCK];
+    const int lane = threadIdx.x;
+    uint32_t nz = 0u;
 #pragma unroll
-    for (int q = 0; q < M3_OK; ++q) mw[q] = nbm.get(q, i);  // all loads in flight at once
-#pragma unroll 1
-    for (int k = 0; k < 9; ++k) {
-      const R uy = row_u3<R>(K, 1 - k % 3), uz = row_u3<R>(K, 1 - k / 3);
-#pragma unroll 1
-      for (int q = 0; q < M3_WORDS; ++q) {
-        uint32_t m = mw[k * M3_WORDS + q];
-        const int tq = rw.t0[k] + 32 * q;
-        while (m) {
-          const int b = __builtin_ctz(m);
-          m &= m - 1;
-          force3_visit<R, PRESS, AV, AH, XS>(S, P, K, pi, uy, uz, o, tq + b);
+    for (int q = 0; q < M3_OK; ++q) {
+      const uint32_t m = nbm.get(q, i);
+      lw[q][lane] = m;
+      nz |= (m != 0u ? 1u : 0u) << q;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) lt[k][lane] = rw.t0[k];
+    if (nz) {
+      int w = __builtin_ctz(nz);
+      nz &= nz - 1u;
+      uint32_t m = lw[w][lane];
+      while (true) {
+        const int b = __builtin_ctz(m);
+        m &= m - 1u;
+        const int k = (w * 11) >> 5;  // w / 3 for w < 27
+        const int q = w - 3 * k;
+        const int ky = (k * 11) >> 5;  // k / 3 for k < 9
+        const R uy = row_u3<R>(K, 1 - (k - 3 * ky)), uz = row_u3<R>(K, 1 - ky);
+        force3_visit<R, PRESS, AV, AH, XS>(S, P, K, pi, uy, uz, o, lt[k][lane] + 32 * q + b);
+        if (m == 0u) {
+          if (nz == 0u) break;
+          w = __builtin_ctz(nz);
+          nz &= nz - 1u;
+          m = lw[w][lane];
         }
       }
     }
