@@ -332,21 +332,13 @@ __device__ __forceinline__ void for_neighbours(const KParams& K, const Segment<C
   }
 }
 
-// Same walk driven by masks recorded by the density pass: one loop over the
-// three rows' bits in row order (the order of the per-row walk), so that a
-// wavefront runs for its longest lane's neighbour count rather than for the
-// sum of the per-row maxima.
-template <typename R, int CW, typename Visit>
-__device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& g,
-                                           const int (*cs)[CW + 3], uint32_t key, uint4 nm,
-                                           Visit&& visit) {
-  const int nx = K.nx;
-  const int cx = (int)key - g.cy * nx;
-  const int a0 = (cx > 0 ? cx - 1 : 0) - g.slo;
-  const int l0 = g.lo[0] - g.rs[0] + cs[0][a0];
-  const int l1 = g.lo[1] - g.rs[1] + cs[1][a0];
-  const int l2 = g.lo[2] - g.rs[2] + cs[2][a0];
-  uint32_t m0 = nm.x, m1 = nm.y, m2 = nm.z;
+// One loop over the bits of three row masks in row order (bit b of row k:
+// tile entry l_k + b), so that a wavefront runs for its longest lane's
+// neighbour count rather than for the sum of the per-row maxima; the order
+// of the visits, and so every sum, is that of a row-by-row walk.
+template <typename R, typename Visit>
+__device__ __forceinline__ void walk_masks(const KParams& K, uint32_t m0, uint32_t m1,
+                                           uint32_t m2, int l0, int l1, int l2, Visit&& visit) {
   while (m0 | m1 | m2) {
     const bool s0 = m0 != 0u, s1 = !s0 && m1 != 0u;
     const uint32_t m = s0 ? m0 : (s1 ? m1 : m2);
@@ -358,6 +350,18 @@ __device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& 
     m2 = (s0 || s1) ? m2 : rest;
     visit(l, uy);
   }
+}
+
+// The same walk driven by masks recorded by the density pass.
+template <typename R, int CW, typename Visit>
+__device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& g,
+                                           const int (*cs)[CW + 3], uint32_t key, uint4 nm,
+                                           Visit&& visit) {
+  const int nx = K.nx;
+  const int cx = (int)key - g.cy * nx;
+  const int a0 = (cx > 0 ? cx - 1 : 0) - g.slo;
+  walk_masks<R>(K, nm.x, nm.y, nm.z, g.lo[0] - g.rs[0] + cs[0][a0],
+                g.lo[1] - g.rs[1] + cs[1][a0], g.lo[2] - g.rs[2] + cs[2][a0], visit);
 }
 
 template <typename R, bool SUM, bool EOS, bool PER, int BLK, int CW, int CAP>
