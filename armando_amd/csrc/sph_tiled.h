@@ -68,26 +68,26 @@ __device__ __forceinline__ void tile_delta(const KParams& K, const LaneF& p, flo
 
 template <typename R, int CAP> struct DensTile;
 template <int CAP> struct DensTile<float, CAP> {
-  float4 a[CAP + TPAD];  // offx, offy, vx, vy
-  float cx[CAP + TPAD];  // cell column as float
+  // the neighbour test reads one 16-byte record per candidate
+  float4 a[CAP + TPAD];  // offx, offy, cell column as float, vx
+  float vy[CAP + TPAD];
   __device__ __forceinline__ void put(int l, const St<float>& S, const KParams&, int t) {
     const float2 o = S.off[t];
     const float2 v = S.v[t];
-    a[l] = make_float4(o.x, o.y, v.x, v.y);
-    cx[l] = (float)(S.cxy[t] & 0xffffu);
+    a[l] = make_float4(o.x, o.y, (float)(S.cxy[t] & 0xffffu), v.x);
+    vy[l] = v.y;
   }
   __device__ __forceinline__ LaneF lane(int l) const {
     const float4 q = a[l];
-    return {cx[l], q.x, q.y};
+    return {q.z, q.x, q.y};
   }
   __device__ __forceinline__ void delta(const KParams& K, const LaneF& p, int l, float uy,
                                         float& dx, float& dy) const {
     const float4 q = a[l];
-    tile_delta(K, p, lds_at(cx, off4(l)), q.x, q.y, uy, dx, dy);
+    tile_delta(K, p, q.z, q.x, q.y, uy, dx, dy);
   }
   __device__ __forceinline__ float2 vel(int l) const {
-    const float4 q = a[l];
-    return make_float2(q.z, q.w);
+    return make_float2(a[l].w, lds_at(vy, off4(l)));
   }
 };
 template <int CAP> struct DensTile<double, CAP> {
