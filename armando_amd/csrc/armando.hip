@@ -472,10 +472,20 @@ template <typename R> int force_t(arm_ctx* c, bool press, int flags, bool xs) {
   return ARM_OK;
 }
 
-template <typename R> int zero_av(arm_ctx* c) {
-  St<R>& A = stA<R>(c);
-  HIPCHK(c, hipMemsetAsync(A.av, 0, sizeof(typename V2<R>::type) * (size_t)c->n, c->stream));
+// Clears a device array inside the step with a fill kernel (not a memset
+// node, so that a caller may capture the step in a graph).
+int zero_bytes(arm_ctx* c, void* p, size_t bytes) {
+  const size_t nw = bytes / 4;
+  const int nb = (int)std::min<size_t>((nw + BLOCK - 1) / BLOCK, 4096);
+  if (nb > 0)
+    hipLaunchKernelGGL(k_zero_words, dim3(nb), dim3(BLOCK), 0, c->stream,
+                       static_cast<uint32_t*>(p), nw);
+  HIPCHK(c, hipGetLastError());
   return ARM_OK;
+}
+
+template <typename R> int zero_av(arm_ctx* c) {
+  return zero_bytes(c, stA<R>(c).av, sizeof(typename V2<R>::type) * (size_t)c->n);
 }
 
 // The smoothing lengths written by the force pass become the current ones
@@ -1341,9 +1351,7 @@ template <typename R> int force3_t(arm_ctx* c, bool press, int flags, bool xs) {
 }
 
 template <typename R> int zero_av3(arm_ctx* c) {
-  HIPCHK(c, hipMemsetAsync(st3A<R>(c).av, 0, sizeof(typename V4<R>::type) * (size_t)c->n,
-                           c->stream));
-  return ARM_OK;
+  return zero_bytes(c, st3A<R>(c).av, sizeof(typename V4<R>::type) * (size_t)c->n);
 }
 
 template <typename R> int single_step3_t(arm_ctx* c) {

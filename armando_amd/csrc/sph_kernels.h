@@ -15,6 +15,13 @@ namespace arm {
 
 constexpr int BLOCK = 256;
 
+// Zeroes nw 32-bit words.
+__global__ __launch_bounds__(BLOCK) void k_zero_words(uint32_t* __restrict__ p, size_t nw) {
+  for (size_t k = blockIdx.x * (size_t)BLOCK + threadIdx.x; k < nw;
+       k += (size_t)gridDim.x * BLOCK)
+    p[k] = 0u;
+}
+
 // ------------------------------------------------------------------------
 // cell keys (updateHashHost, armando2Dcpu.cu:2080-2101) in FP64
 // ------------------------------------------------------------------------
