@@ -630,7 +630,8 @@ template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
   if (hk.tape) {
     const int nb = std::max(grid_of(n), grid_of(c->K.ncell + 1));
     hipLaunchKernelGGL(k_tape<R>, dim3(nb), dim3(BLOCK), 0, c->stream, stA<R>(c), c->K, c->skey,
-                       c->cstart, *static_cast<const Tape<R>*>(hk.tape));
+                       c->cstart, c->masks_valid ? c->nbm : nullptr,
+                       *static_cast<const Tape<R>*>(hk.tape));
   }
   if ((rc = corrector_t<R>(c, its, true))) return rc;
   return ARM_OK;
@@ -721,7 +722,8 @@ template <typename R> int alloc_tape(Scratch& sc, Tape<R>& t, int n, int ncell, 
       (rc = sc.get(&t.divv, n)) || (rc = sc.get(&t.p, n)) || (rc = sc.get(&t.drho, n)) ||
       (rc = sc.get(&t.du, n)) || (rc = sc.get(&t.a, n)) || (rc = sc.get(&t.av, n)) ||
       (rc = sc.get(&t.rho_min, n)) || (rc = sc.get(&t.u_min, n)) || (rc = sc.get(&t.v_min, n)) ||
-      (rc = sc.get(&t.flag, n)) || (rc = sc.get(&t.skey, n)) || (rc = sc.get(&t.cstart, ncell + 1)))
+      (rc = sc.get(&t.flag, n)) || (rc = sc.get(&t.skey, n)) || (rc = sc.get(&t.cstart, ncell + 1)) ||
+      (rc = sc.get(&t.nbm, n)))
     return rc;
   return ARM_OK;
 }

@@ -53,6 +53,7 @@ template <typename R> struct Tape {
   uint32_t* skey;
   int* cstart;
   R* tau;  // physical viscosity: (Txx, Tyy, Txy, tds) per particle, else null
+  uint4* nbm;  // the density pass's neighbour masks (three rows + complete flag)
 };
 
 __device__ __forceinline__ PosD tape_pos(const Tape<double>& T, int t) {
@@ -83,7 +84,8 @@ template <typename R> struct AdjScratch {
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_tape(St<R> S, KParams K,
                                                 const uint32_t* __restrict__ skey,
-                                                const int* __restrict__ cstart, Tape<R> T) {
+                                                const int* __restrict__ cstart,
+                                                const uint4* __restrict__ nbm, Tape<R> T) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   const int stride = gridDim.x * BLOCK;
   for (int c = t; c <= K.ncell; c += stride) T.cstart[c] = cstart[c];
@@ -109,6 +111,7 @@ __global__ __launch_bounds__(BLOCK) void k_tape(St<R> S, KParams K,
   T.v_min[t] = S.v_min[t];
   T.flag[t] = S.flag[t];
   T.skey[t] = skey[t];
+  T.nbm[t] = nbm ? nbm[t] : make_uint4(0u, 0u, 0u, 0u);
   if (T.tau)
     for (int c = 0; c < 4; ++c) T.tau[4 * (size_t)t + c] = S.tau[4 * (size_t)t + c];
 }
@@ -208,6 +211,43 @@ __device__ __forceinline__ Rows tape_rows(const KParams& K, const Tape<R>& T, in
   return rows_of(K, T.skey[k], T.cstart);
 }
 
+// The neighbours l of particle k in the forward gathers' order (rows bottom
+// to top, ascending position), visit(l, rx, ry, r2): from the forward's
+// masks when they are complete (bit b of row r = candidate t0_r + b, walked
+// in one loop over the three rows), else by testing every candidate again.
+template <typename R, typename LP, typename Visit>
+__device__ __forceinline__ void for_tape_neighbours(const KParams& K, const Tape<R>& T, int k,
+                                                    const LP& pk, Visit&& visit) {
+  const Rows rw = tape_rows(K, T, k);
+  const uint4 nm = T.nbm[k];
+  if (nm.w) {
+    uint32_t m0 = nm.x, m1 = nm.y, m2 = nm.z;
+    while (m0 | m1 | m2) {
+      const bool s0 = m0 != 0u, s1 = !s0 && m1 != 0u;
+      const uint32_t m = s0 ? m0 : (s1 ? m1 : m2);
+      const int l = (s0 ? rw.t0[0] : (s1 ? rw.t0[1] : rw.t0[2])) + (int)__builtin_ctz(m);
+      const uint32_t rest = m & (m - 1u);
+      m0 = s0 ? rest : m0;
+      m1 = s1 ? rest : m1;
+      m2 = (s0 || s1) ? m2 : rest;
+      R rx, ry;
+      delta(pk, tape_pos(T, l), K, rx, ry);
+      visit(l, rx, ry, rx * rx + ry * ry);
+    }
+    return;
+  }
+  const R cut2 = cutoff2<R>(K);
+  for (int r = 0; r < 3; ++r)
+    for (int l = rw.t0[r]; l < rw.t1[r]; ++l) {
+      if (l == k) continue;
+      R rx, ry;
+      delta(pk, tape_pos(T, l), K, rx, ry);
+      const R r2 = rx * rx + ry * ry;
+      if (!(r2 < cut2)) continue;
+      visit(l, rx, ry, r2);
+    }
+}
+
 // 2. force-pass adjoint, gathered: particle k (as i) against every
 //    neighbour l (as j); accumulates k's own input adjoints.
 template <typename R, bool AV, bool AH, bool XS, bool VI>
@@ -227,7 +267,6 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjSc
     const R lak_x = W.lax[k], lak_y = W.lay[k], lduk = W.ldu[k];
     const R lavk_x = W.lavx[k], lavk_y = W.lavy[k];
     const R psik = T.psi[k], uk = (R)T.u[k];
-    const R cut2 = cutoff2<R>(K);
     const R c0 = (R)K.c0, al = (R)K.av_alpha, be = (R)K.av_beta;
     R lx = 0, ly = 0, lvx = 0, lvy = 0, lQ = 0, lA = 0, lpsi = 0, lu = 0;
     // physical viscosity: W = c T, c = eta (pa_sph 0) or eta Q^2 (pa_sph 1)
@@ -239,103 +278,96 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjSc
       Tk[2] = tk[2];
       ck = visc_c(K, Qk);
     }
-    const Rows rw = tape_rows(K, T, k);
-    for (int r = 0; r < 3; ++r)
-      for (int l = rw.t0[r]; l < rw.t1[r]; ++l) {
-        if (l == k) continue;
-        R rx, ry;
-        delta(pk, tape_pos(T, l), K, rx, ry);
-        const R r2 = rx * rx + ry * ry;
-        if (!(r2 < cut2)) continue;
-        R q, F, Wv;
-        wendland_q(K, r2, q, F, Wv);
-        const R2 vl = T.v[l];
-        const R dvx = vk.x - vl.x, dvy = vk.y - vl.y;
-        const R vr = dvx * rx + dvy * ry;
-        const R Ql = (R)1 / dens(K, T.rho[l]), Al = T.p[l] * Ql;
-        const R sQ = Qk + Ql;
-        const R M = (R)2 * Qk * Ql / sQ;
-        R Kc = (K.pa_sph == 0) ? (Ak * Ql + Al * Qk) : (Ak * Qk + Al * Ql);
-        const bool av_on = AV && vr < (R)0;
-        const R einv = (R)1 / (r2 + (R)K.av_e2);
-        const R phi = h * vr * einv;
-        if (av_on) Kc += (be * phi - al * c0) * phi * M;
-        const R G = m * F * Kc;
-        const R Dax = lak_x - W.lax[l], Day = lak_y - W.lay[l];
-        const R sig = lduk + W.ldu[l];
-        const R lG = -(Dax * rx + Day * ry) + (R)0.5 * sig * vr;
-        R lrx = -G * Dax, lry = -G * Day;
-        R lvr = (R)0.5 * sig * G;
-        R ldvx = 0, ldvy = 0;
-        R lF = lG * m * Kc;
-        const R lK = lG * m * F;
-        R lM = 0, lr2 = 0, lW = 0;
-        if (K.pa_sph == 0) {
-          lA += lK * Ql;
-          lQ += lK * Al;
-        } else {
-          lA += lK * Qk;
-          lQ += lK * Ak;
-        }
-        if (av_on) {
-          const R dPi = ((R)2 * be * phi - al * c0) * M;
-          lM += lK * (be * phi - al * c0) * phi;
-          lal += (double)(lK * (-c0 * phi * M));
-          const R lphi = lK * dPi;
-          lvr += lphi * h * einv;
-          lr2 += lphi * (-phi * einv);
-        }
-        if (AH) {
-          const R eh = (R)1 / (r2 + (R)K.ah_e2);
-          const R du_ = uk - (R)T.u[l];
-          const R dl = lduk - W.ldu[l];
-          const R ps = psik + T.psi[l];
-          const R H = m * ps * F * r2 * M * eh;
-          const R lH = dl * du_;
-          lu += dl * H;
-          lpsi += lH * m * F * r2 * M * eh;
-          lF += lH * m * ps * r2 * M * eh;
-          lr2 += lH * m * ps * F * M * (R)K.ah_e2 * eh * eh;
-          lM += lH * m * ps * F * r2 * eh;
-        }
-        if (XS) {
-          const R Dvx = lavk_x - W.lavx[l], Dvy = lavk_y - W.lavy[l];
-          const R w = (R)K.xsph_eps * m * M * Wv;
-          const R lw = -(Dvx * dvx + Dvy * dvy);
-          ldvx += -w * Dvx;
-          ldvy += -w * Dvy;
-          lM += lw * (R)K.xsph_eps * m * Wv;
-          lW += lw * (R)K.xsph_eps * m * M;
-        }
-        if (VI) {
-          // a_k += m F S.r, S = (Q_k Q_l if pa_sph 0) (W_k + W_l); a_l gets -that
-          const R* tl = T.tau + 4 * (size_t)l;
-          const R cl = visc_c(K, Ql);
-          R Sr[3] = {ck * Tk[0] + cl * tl[0], ck * Tk[1] + cl * tl[1], ck * Tk[2] + cl * tl[2]};
-          const R qq = K.pa_sph == 0 ? Qk * Ql : (R)1;
-          const R Sx = qq * Sr[0], Sy = qq * Sr[1], Sxy = qq * Sr[2];
-          const R mF = m * F;
-          lF += m * (Dax * (Sx * rx + Sxy * ry) + Day * (Sxy * rx + Sy * ry));
-          lrx += mF * (Dax * Sx + Day * Sxy);
-          lry += mF * (Dax * Sxy + Day * Sy);
-          const R lS[3] = {mF * Dax * rx, mF * Day * ry, mF * (Dax * ry + Day * rx)};
-          if (K.pa_sph == 0) lQ += Ql * (lS[0] * Sr[0] + lS[1] * Sr[1] + lS[2] * Sr[2]);
-          for (int c = 0; c < 3; ++c) lWk[c] += qq * lS[c];
-        }
-        lQ += lM * (R)2 * Ql * Ql / (sQ * sQ);
-        const R s = (R)1 - (R)0.5 * q;
-        const R lq = lF * (R)3 * (R)K.C1 * (q - (R)2) * (q - (R)2) +
-                     lW * (R)K.wnorm * s * s * s * ((R)-5 * q);
-        lr2 += lq / ((R)2 * h * h * q);
-        ldvx += lvr * rx;
-        ldvy += lvr * ry;
-        lrx += lvr * dvx + (R)2 * lr2 * rx;
-        lry += lvr * dvy + (R)2 * lr2 * ry;
-        lx += lrx;
-        ly += lry;
-        lvx += ldvx;
-        lvy += ldvy;
+    for_tape_neighbours(K, T, k, pk, [&](int l, R rx, R ry, R r2) {
+      R q, F, Wv;
+      wendland_q(K, r2, q, F, Wv);
+      const R2 vl = T.v[l];
+      const R dvx = vk.x - vl.x, dvy = vk.y - vl.y;
+      const R vr = dvx * rx + dvy * ry;
+      const R Ql = (R)1 / dens(K, T.rho[l]), Al = T.p[l] * Ql;
+      const R sQ = Qk + Ql;
+      const R M = (R)2 * Qk * Ql / sQ;
+      R Kc = (K.pa_sph == 0) ? (Ak * Ql + Al * Qk) : (Ak * Qk + Al * Ql);
+      const bool av_on = AV && vr < (R)0;
+      const R einv = (R)1 / (r2 + (R)K.av_e2);
+      const R phi = h * vr * einv;
+      if (av_on) Kc += (be * phi - al * c0) * phi * M;
+      const R G = m * F * Kc;
+      const R Dax = lak_x - W.lax[l], Day = lak_y - W.lay[l];
+      const R sig = lduk + W.ldu[l];
+      const R lG = -(Dax * rx + Day * ry) + (R)0.5 * sig * vr;
+      R lrx = -G * Dax, lry = -G * Day;
+      R lvr = (R)0.5 * sig * G;
+      R ldvx = 0, ldvy = 0;
+      R lF = lG * m * Kc;
+      const R lK = lG * m * F;
+      R lM = 0, lr2 = 0, lW = 0;
+      if (K.pa_sph == 0) {
+        lA += lK * Ql;
+        lQ += lK * Al;
+      } else {
+        lA += lK * Qk;
+        lQ += lK * Ak;
       }
+      if (av_on) {
+        const R dPi = ((R)2 * be * phi - al * c0) * M;
+        lM += lK * (be * phi - al * c0) * phi;
+        lal += (double)(lK * (-c0 * phi * M));
+        const R lphi = lK * dPi;
+        lvr += lphi * h * einv;
+        lr2 += lphi * (-phi * einv);
+      }
+      if (AH) {
+        const R eh = (R)1 / (r2 + (R)K.ah_e2);
+        const R du_ = uk - (R)T.u[l];
+        const R dl = lduk - W.ldu[l];
+        const R ps = psik + T.psi[l];
+        const R H = m * ps * F * r2 * M * eh;
+        const R lH = dl * du_;
+        lu += dl * H;
+        lpsi += lH * m * F * r2 * M * eh;
+        lF += lH * m * ps * r2 * M * eh;
+        lr2 += lH * m * ps * F * M * (R)K.ah_e2 * eh * eh;
+        lM += lH * m * ps * F * r2 * eh;
+      }
+      if (XS) {
+        const R Dvx = lavk_x - W.lavx[l], Dvy = lavk_y - W.lavy[l];
+        const R w = (R)K.xsph_eps * m * M * Wv;
+        const R lw = -(Dvx * dvx + Dvy * dvy);
+        ldvx += -w * Dvx;
+        ldvy += -w * Dvy;
+        lM += lw * (R)K.xsph_eps * m * Wv;
+        lW += lw * (R)K.xsph_eps * m * M;
+      }
+      if (VI) {
+        // a_k += m F S.r, S = (Q_k Q_l if pa_sph 0) (W_k + W_l); a_l gets -that
+        const R* tl = T.tau + 4 * (size_t)l;
+        const R cl = visc_c(K, Ql);
+        R Sr[3] = {ck * Tk[0] + cl * tl[0], ck * Tk[1] + cl * tl[1], ck * Tk[2] + cl * tl[2]};
+        const R qq = K.pa_sph == 0 ? Qk * Ql : (R)1;
+        const R Sx = qq * Sr[0], Sy = qq * Sr[1], Sxy = qq * Sr[2];
+        const R mF = m * F;
+        lF += m * (Dax * (Sx * rx + Sxy * ry) + Day * (Sxy * rx + Sy * ry));
+        lrx += mF * (Dax * Sx + Day * Sxy);
+        lry += mF * (Dax * Sxy + Day * Sy);
+        const R lS[3] = {mF * Dax * rx, mF * Day * ry, mF * (Dax * ry + Day * rx)};
+        if (K.pa_sph == 0) lQ += Ql * (lS[0] * Sr[0] + lS[1] * Sr[1] + lS[2] * Sr[2]);
+        for (int c = 0; c < 3; ++c) lWk[c] += qq * lS[c];
+      }
+      lQ += lM * (R)2 * Ql * Ql / (sQ * sQ);
+      const R s = (R)1 - (R)0.5 * q;
+      const R lq = lF * (R)3 * (R)K.C1 * (q - (R)2) * (q - (R)2) +
+                   lW * (R)K.wnorm * s * s * s * ((R)-5 * q);
+      lr2 += lq / ((R)2 * h * h * q);
+      ldvx += lvr * rx;
+      ldvy += lvr * ry;
+      lrx += lvr * dvx + (R)2 * lr2 * rx;
+      lry += lvr * dvy + (R)2 * lr2 * ry;
+      lx += lrx;
+      ly += lry;
+      lvx += ldvx;
+      lvy += ldvy;
+    });
     if (VI) {
       // W_k = c_k T_k and tds_k = eta/2 Q_k (Txx^2 + Tyy^2 + 2 Txy^2), added to du_k
       const R eta = (R)K.eta;
@@ -411,40 +443,32 @@ __global__ __launch_bounds__(BLOCK) void k_adj_tensor(Tape<R> T, KParams K, AdjS
   const R2 vk = T.v[k];
   const R Qk = (R)1 / dens(K, T.rho[k]);
   const R Lk[3] = {W.lTxx[k], W.lTyy[k], W.lTxy[k]};
-  const R cut2 = cutoff2<R>(K);
   const R two3 = (R)(2.0 / 3.0);
   R lx = 0, ly = 0, lvx = 0, lvy = 0, lQ = 0;
-  const Rows rw = tape_rows(K, T, k);
-  for (int r = 0; r < 3; ++r)
-    for (int l = rw.t0[r]; l < rw.t1[r]; ++l) {
-      if (l == k) continue;
-      R rx, ry;
-      delta(pk, tape_pos(T, l), K, rx, ry);
-      const R r2 = rx * rx + ry * ry;
-      if (!(r2 < cut2)) continue;
-      R q, F, Wv;
-      wendland_q(K, r2, q, F, Wv);
-      const R2 vl = T.v[l];
-      const R dvx = vl.x - vk.x, dvy = vl.y - vk.y;
-      const R dWx = F * rx, dWy = F * ry;
-      const R a = dvx * dWx, b = dvy * dWy;
-      const R H[3] = {two3 * ((R)2 * a - b), two3 * ((R)2 * b - a), dvx * dWy + dvy * dWx};
-      const R Ql = (R)1 / dens(K, T.rho[l]);
-      const R Ll[3] = {W.lTxx[l], W.lTyy[l], W.lTxy[l]};
-      R lH[3];
-      for (int c = 0; c < 3; ++c) lH[c] = m * Ql * Lk[c] + m * Qk * Ll[c];
-      lQ += m * (H[0] * Ll[0] + H[1] * Ll[1] + H[2] * Ll[2]);
-      const R la = two3 * ((R)2 * lH[0] - lH[1]);
-      const R lb = two3 * ((R)2 * lH[1] - lH[0]);
-      const R ldvx = la * dWx + lH[2] * dWy, ldvy = lb * dWy + lH[2] * dWx;
-      const R ldWx = la * dvx + lH[2] * dvy, ldWy = lb * dvy + lH[2] * dvx;
-      lvx -= ldvx;
-      lvy -= ldvy;
-      const R lF = ldWx * rx + ldWy * ry;
-      const R lr2 = lF * (R)3 * (R)K.C1 * (q - (R)2) * (q - (R)2) / ((R)2 * h * h * q);
-      lx += F * ldWx + (R)2 * lr2 * rx;
-      ly += F * ldWy + (R)2 * lr2 * ry;
-    }
+  for_tape_neighbours(K, T, k, pk, [&](int l, R rx, R ry, R r2) {
+    R q, F, Wv;
+    wendland_q(K, r2, q, F, Wv);
+    const R2 vl = T.v[l];
+    const R dvx = vl.x - vk.x, dvy = vl.y - vk.y;
+    const R dWx = F * rx, dWy = F * ry;
+    const R a = dvx * dWx, b = dvy * dWy;
+    const R H[3] = {two3 * ((R)2 * a - b), two3 * ((R)2 * b - a), dvx * dWy + dvy * dWx};
+    const R Ql = (R)1 / dens(K, T.rho[l]);
+    const R Ll[3] = {W.lTxx[l], W.lTyy[l], W.lTxy[l]};
+    R lH[3];
+    for (int c = 0; c < 3; ++c) lH[c] = m * Ql * Lk[c] + m * Qk * Ll[c];
+    lQ += m * (H[0] * Ll[0] + H[1] * Ll[1] + H[2] * Ll[2]);
+    const R la = two3 * ((R)2 * lH[0] - lH[1]);
+    const R lb = two3 * ((R)2 * lH[1] - lH[0]);
+    const R ldvx = la * dWx + lH[2] * dWy, ldvy = lb * dWy + lH[2] * dWx;
+    const R ldWx = la * dvx + lH[2] * dvy, ldWy = lb * dvy + lH[2] * dvx;
+    lvx -= ldvx;
+    lvy -= ldvy;
+    const R lF = ldWx * rx + ldWy * ry;
+    const R lr2 = lF * (R)3 * (R)K.C1 * (q - (R)2) * (q - (R)2) / ((R)2 * h * h * q);
+    lx += F * ldWx + (R)2 * lr2 * rx;
+    ly += F * ldWy + (R)2 * lr2 * ry;
+  });
   W.lx[k] += lx;
   W.ly[k] += ly;
   W.lvx[k] += lvx;
@@ -491,30 +515,22 @@ __global__ __launch_bounds__(BLOCK) void k_adj_density(Tape<R> T, KParams K, Adj
   const auto pk = tape_pos(T, k);
   const R2 vk = T.v[k];
   const R ltk = W.ldrho[k] - W.lS[k];
-  const R cut2 = cutoff2<R>(K);
   R lx = 0, ly = 0, lvx = 0, lvy = 0;
-  const Rows rw = tape_rows(K, T, k);
-  for (int r = 0; r < 3; ++r)
-    for (int l = rw.t0[r]; l < rw.t1[r]; ++l) {
-      if (l == k) continue;
-      R rx, ry;
-      delta(pk, tape_pos(T, l), K, rx, ry);
-      const R r2 = rx * rx + ry * ry;
-      if (!(r2 < cut2)) continue;
-      const R lt = ltk + W.ldrho[l] - W.lS[l];
-      R q, F, Wv;
-      wendland_q(K, r2, q, F, Wv);
-      const R2 vl = T.v[l];
-      const R dvx = vk.x - vl.x, dvy = vk.y - vl.y;
-      const R vr = dvx * rx + dvy * ry;
-      const R lvr = lt * m * F;
-      const R lF = lt * m * vr;
-      const R lr2 = lF * (R)3 * (R)K.C1 * (q - (R)2) * (q - (R)2) / ((R)2 * h * h * q);
-      lx += lvr * dvx + (R)2 * lr2 * rx;
-      ly += lvr * dvy + (R)2 * lr2 * ry;
-      lvx += lvr * rx;
-      lvy += lvr * ry;
-    }
+  for_tape_neighbours(K, T, k, pk, [&](int l, R rx, R ry, R r2) {
+    const R lt = ltk + W.ldrho[l] - W.lS[l];
+    R q, F, Wv;
+    wendland_q(K, r2, q, F, Wv);
+    const R2 vl = T.v[l];
+    const R dvx = vk.x - vl.x, dvy = vk.y - vl.y;
+    const R vr = dvx * rx + dvy * ry;
+    const R lvr = lt * m * F;
+    const R lF = lt * m * vr;
+    const R lr2 = lF * (R)3 * (R)K.C1 * (q - (R)2) * (q - (R)2) / ((R)2 * h * h * q);
+    lx += lvr * dvx + (R)2 * lr2 * rx;
+    ly += lvr * dvy + (R)2 * lr2 * ry;
+    lvx += lvr * rx;
+    lvy += lvr * ry;
+  });
   W.lx[k] += lx;
   W.ly[k] += ly;
   W.lvx[k] += lvx;
