@@ -209,3 +209,33 @@ def test_adjoint_100k_400_steps_against_central_differences_on_16_directions():
         assert rel32 <= 1e-3, (seed, fdv, float(np.sum(g32 * d)))
     banner(f"100k adjoint: alpha fd {fd:.6e} adj {da:.6e} fp32 {da32:.6e}; worst direction "
            f"rel {worst:.2e} (fp32 {worst32:.2e})")
+
+
+@pytest.mark.parametrize("h_fac", [1.6, 2.6])
+def test_adjoint_with_incomplete_masks_matches_oracle(h_fac):
+    """Rows of more than 32 candidates leave the forward's neighbour masks
+    incomplete: the reverse gathers then test the candidates again (h = 1.6
+    dx mixes both walks, h = 2.6 dx takes the second everywhere)."""
+    s = 0.02
+    c = cases.dam_break(col_w=0.6 * s, col_h=1.2 * s, tank_w=3.2 * s, tank_h=1.6 * s, dx=1e-3,
+                        h_fac=h_fac)
+    rng = np.random.default_rng(5)
+    nf = c["np"]
+    c["vx"] = c["vx"].copy()
+    c["vy"] = c["vy"].copy()
+    c["vx"][:nf] = rng.normal(0.0, 0.01, nf)
+    c["vy"][:nf] = rng.normal(0.0, 0.01, nf)
+    obj = np.nonzero((c["x"] < 0) & (c["y"] < 0.2 * 1.2 * s))[0].astype(np.int32)
+    steps = 12
+    banner(f"gpu vs oracle adjoint, h = {h_fac} dx, n={c['n']}")
+    J, da, g, _ = gpu_rollout(c, obj, steps, 5)
+    P = orc.params_from(c)
+    st = orc.OracleState(c["n"], c["np"])
+    st.set(**{k: c[k] for k in ("x", "y", "vx", "vy", "rho", "u")})
+    mask = np.zeros(c["n"], np.int32)
+    mask[obj] = 1
+    Jo, dao, gxo, gyo = orc.adjoint(P, st, steps, mask)
+    assert abs(J - Jo) <= 1e-10 * abs(Jo)
+    assert abs(da - dao) <= 1e-8 * abs(dao)
+    go = np.stack([gxo, gyo], 1)[:nf]
+    assert np.linalg.norm(g[:nf] - go) <= 1e-8 * np.linalg.norm(go)
