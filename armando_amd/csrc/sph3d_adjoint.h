@@ -106,16 +106,22 @@ __device__ __forceinline__ void for_tape_neighbours3(const Tape3<R>& T, const KP
   const Rows9 rw = rows3_of(K, T.skey[k], T.cstart);
   const size_t st = (size_t)K.n;
   if (T.nbm[M3_OK * st + k]) {
+    // per row, one loop over the bits of its three words (in word order)
     for (int row = 0; row < 9; ++row) {
       const R uy = row_u3<R>(K, 1 - row % 3), uz = row_u3<R>(K, 1 - row / 3);
-      for (int q = 0; q < M3_WORDS; ++q) {
-        uint32_t m = T.nbm[(size_t)(row * M3_WORDS + q) * st + k];
-        const int tq = rw.t0[row] + 32 * q;
-        while (m) {
-          const int b = __builtin_ctz(m);
-          m &= m - 1;
-          visit(tq + b, uy, uz);
-        }
+      uint32_t m0 = T.nbm[(size_t)(row * M3_WORDS + 0) * st + k];
+      uint32_t m1 = T.nbm[(size_t)(row * M3_WORDS + 1) * st + k];
+      uint32_t m2 = T.nbm[(size_t)(row * M3_WORDS + 2) * st + k];
+      const int t0 = rw.t0[row];
+      while (m0 | m1 | m2) {
+        const bool s0 = m0 != 0u, s1 = !s0 && m1 != 0u;
+        const uint32_t m = s0 ? m0 : (s1 ? m1 : m2);
+        const int l = t0 + (s0 ? 0 : (s1 ? 32 : 64)) + (int)__builtin_ctz(m);
+        const uint32_t rest = m & (m - 1u);
+        m0 = s0 ? rest : m0;
+        m1 = s1 ? rest : m1;
+        m2 = (s0 || s1) ? m2 : rest;
+        visit(l, uy, uz);
       }
     }
     return;
