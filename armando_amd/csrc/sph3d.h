@@ -316,29 +316,93 @@ __device__ __forceinline__ void density3_direct(St3<R>& S, const KParams& K, int
   DensOwn<R> o;
   dens_begin<R, SUM>(K, o);
   bool ok = true;
-  for (int k = 0; k < 9; ++k) {
-    const int t0 = rw.t0[k], t1 = rw.t1[k];
-    const R uy = row_u3<R>(K, 1 - k % 3), uz = row_u3<R>(K, 1 - k / 3);
-    if (t1 - t0 > 32 * M3_WORDS) ok = false;
-    uint32_t m0 = 0, m1 = 0, m2 = 0;
-    for (int t = t0; t < t1; ++t) {
-      if (t == i) continue;
-      const auto pj = load_pos3(S, t);
-      R dx, dy, dz;
-      const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
-      if (!(r2 < cut2)) continue;
-      const int b = t - t0;
-      const uint32_t bit = 1u << (b & 31);
-      m0 |= b < 32 ? bit : 0u;
-      m1 |= (b >= 32 && b < 64) ? bit : 0u;
-      m2 |= (b >= 64 && b < 96) ? bit : 0u;
-      const auto vj = S.v[t];
-      dens3_pair<R, SUM>(K, dx, dy, dz, r2, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, o.sum, o.rs);
+  // FP32 form, per row: the neighbour test of the first 96 candidates,
+  // eight at a time with no branch, into three mask words; then the pair
+  // sums over the set bits (one loop over the three words); then any
+  // candidates past 96 test and sum one by one.  The pairs are summed in
+  // ascending position, as the single test-and-sum walk of the FP64 form
+  // (below) does.
+  if constexpr (sizeof(R) == 4) {
+    constexpr int NB = 8;
+    for (int k = 0; k < 9; ++k) {
+      const int t0 = rw.t0[k], t1 = rw.t1[k];
+      const R uy = row_u3<R>(K, 1 - k % 3), uz = row_u3<R>(K, 1 - k / 3);
+      if (t1 - t0 > 32 * M3_WORDS) ok = false;
+      const int tA = t1 < t0 + 32 * M3_WORDS ? t1 : t0 + 32 * M3_WORDS;
+      uint32_t m0 = 0, m1 = 0, m2 = 0;
+      for (int c0 = t0; c0 < tA; c0 += NB) {
+  #pragma unroll
+        for (int q = 0; q < NB; ++q) {
+          const int t = c0 + q;
+          const bool in = t < tA;
+          const auto pj = load_pos3(S, in ? t : t0);
+          R dx, dy, dz;
+          const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
+          const int b = t - t0;
+          const uint32_t bit = (in && t != i && r2 < cut2) ? (1u << (b & 31)) : 0u;
+          m0 |= b < 32 ? bit : 0u;
+          m1 |= (b >= 32 && b < 64) ? bit : 0u;
+          m2 |= b >= 64 ? bit : 0u;
+        }
+      }
+      {
+        uint32_t w0 = m0, w1 = m1, w2 = m2;
+        while (w0 | w1 | w2) {
+          const bool s0 = w0 != 0u, s1 = !s0 && w1 != 0u;
+          const uint32_t m = s0 ? w0 : (s1 ? w1 : w2);
+          const int t = t0 + (s0 ? 0 : (s1 ? 32 : 64)) + (int)__builtin_ctz(m);
+          const uint32_t rest = m & (m - 1u);
+          w0 = s0 ? rest : w0;
+          w1 = s1 ? rest : w1;
+          w2 = (s0 || s1) ? w2 : rest;
+          const auto pj = load_pos3(S, t);
+          R dx, dy, dz;
+          const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
+          const auto vj = S.v[t];
+          dens3_pair<R, SUM>(K, dx, dy, dz, r2, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, o.sum,
+                             o.rs);
+        }
+      }
+      for (int t = tA; t < t1; ++t) {
+        if (t == i) continue;
+        const auto pj = load_pos3(S, t);
+        R dx, dy, dz;
+        const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
+        if (!(r2 < cut2)) continue;
+        const auto vj = S.v[t];
+        dens3_pair<R, SUM>(K, dx, dy, dz, r2, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, o.sum, o.rs);
+      }
+      if (nbm.w) {
+        nbm.put(k * M3_WORDS + 0, i, m0);
+        nbm.put(k * M3_WORDS + 1, i, m1);
+        nbm.put(k * M3_WORDS + 2, i, m2);
+      }
     }
-    if (nbm.w) {
-      nbm.put(k * M3_WORDS + 0, i, m0);
-      nbm.put(k * M3_WORDS + 1, i, m1);
-      nbm.put(k * M3_WORDS + 2, i, m2);
+  } else {
+    for (int k = 0; k < 9; ++k) {
+      const int t0 = rw.t0[k], t1 = rw.t1[k];
+      const R uy = row_u3<R>(K, 1 - k % 3), uz = row_u3<R>(K, 1 - k / 3);
+      if (t1 - t0 > 32 * M3_WORDS) ok = false;
+      uint32_t m0 = 0, m1 = 0, m2 = 0;
+      for (int t = t0; t < t1; ++t) {
+        if (t == i) continue;
+        const auto pj = load_pos3(S, t);
+        R dx, dy, dz;
+        const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
+        if (!(r2 < cut2)) continue;
+        const int b = t - t0;
+        const uint32_t bit = 1u << (b & 31);
+        m0 |= b < 32 ? bit : 0u;
+        m1 |= (b >= 32 && b < 64) ? bit : 0u;
+        m2 |= (b >= 64 && b < 96) ? bit : 0u;
+        const auto vj = S.v[t];
+        dens3_pair<R, SUM>(K, dx, dy, dz, r2, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, o.sum, o.rs);
+      }
+      if (nbm.w) {
+        nbm.put(k * M3_WORDS + 0, i, m0);
+        nbm.put(k * M3_WORDS + 1, i, m1);
+        nbm.put(k * M3_WORDS + 2, i, m2);
+      }
     }
   }
   if (nbm.w) nbm.put(M3_OK, i, ok ? 1u : 0u);
