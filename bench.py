@@ -74,7 +74,7 @@ FORCE_TRAFFIC_BYTES = 130.2e6
 # same runs): with 4 cycles per wave64 instruction on each of the 1024 SIMDs
 # at 2.4 GHz they give the fraction of the launch the VALUs issue, the bound
 # of this gather.
-FORCE_VALU_INSTS = 3.54e7
+FORCE_VALU_INSTS = 2.92e7
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 
 
