@@ -98,6 +98,10 @@ def parse():
                     help="approximate CPU work for the baseline sample")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="dam-break size factor (1.0 = the 720k workload)")
+    ap.add_argument("--oracle-steps", type=int, default=0,
+                    help="also step both forms and the oracle (OpenMP, host cores) this many "
+                         "steps from the initial state and report the errors against the "
+                         "oracle (1000 steps: about 45 s of CPU work on 16 threads)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB three times (a kernel of "
                          "known traffic for scaling the memory counters of a profiler)")
@@ -373,6 +377,43 @@ def ghost_leg(steps, warmup=5):
     }
 
 
+def oracle_leg(steps, case_fn):
+    """Both forms and the oracle's gather form (time_integration.f:131-235,
+    on the host's cores) stepped `steps` steps from the same seeded state;
+    rel-L2, max-abs and max-relative errors of every field against the
+    oracle."""
+    import armando_amd
+    from armando_amd import cases
+    from oracle import orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    orc.lib().orc_set_threads(threads)
+    case = case_fn(cases.FP64)
+    P = orc.params_from(case)
+    st = orc.OracleState(case["n"], case["np"])
+    st.set(**{k: case[k] for k in ("x", "y", "vx", "vy", "rho", "u")})
+    t0 = time.perf_counter()
+    orc.run(P, st, 1, steps, gather=True)
+    t_orc = time.perf_counter() - t0
+    names = ("rho", "p", "vx", "vy", "u", "x", "y")
+    ref = {k: st.field(k).copy() for k in names}
+    out = {"after_steps": steps, "oracle_threads": threads, "oracle_seconds": t_orc}
+    for tag, prec in (("fp64", cases.FP64), ("fp32", cases.FP32)):
+        sim = armando_amd.Simulation(case_fn(prec))
+        sim.time_integration(1, steps)
+        s = sim.state()
+        sim.close()
+        acc = {}
+        for k in names:
+            d = s[k] - ref[k]
+            scale = float(np.max(np.abs(ref[k])))
+            acc[k] = {"rel_l2": float(np.linalg.norm(d) / max(np.linalg.norm(ref[k]), 1e-300)),
+                      "max_abs": float(np.max(np.abs(d))),
+                      "max_rel": float(np.max(np.abs(d))) / max(scale, 1e-300)}
+        out[tag] = acc
+    return out
+
+
 def cpu_baseline(case, seconds):
     """The oracle's faithful serial restatement of the reference loop
     (half pair list, scatter; one core) on the same seeded input."""
@@ -508,6 +549,8 @@ def main():
         out["heated_channel"] = channel_leg(args.steps_channel)
     if not args.no_ghosts:
         out["ghost_walls"] = ghost_leg(args.steps_ghosts)
+    if args.oracle_steps > 0:
+        out["accuracy_vs_oracle"] = oracle_leg(args.oracle_steps, case_fn)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out))
