@@ -36,7 +36,8 @@ EXPORTS = ("arm_create", "arm_destroy", "arm_last_error", "arm_domain_violations
            "arm_set_gather_path", "arm_adjoint_rollout", "arm_rollout_forward",
            "arm_rollout_backward", "arm_update_params", "arm_set_objective",
            "arm_rollout_param_grads", "arm_conservation3", "arm_upload_h", "arm_download_h",
-           "arm_ghost_particles", "arm_ghost_count", "arm_download_ghosts")
+           "arm_ghost_particles", "arm_ghost_count", "arm_download_ghosts",
+           "arm_set_deposition")
 
 
 class ArmParams(C.Structure):
@@ -120,6 +121,7 @@ def load_library(path=LIB_PATH):
         "arm_update_params": (C.c_int, [vp, C.POINTER(ArmParams)]),
         "arm_set_objective": (C.c_int, [vp, C.c_int]),
         "arm_rollout_param_grads": (C.c_int, [vp, dpn, C.c_int]),
+        "arm_set_deposition": (C.c_int, [vp, dpn, dpn, C.c_void_p, dpn, C.c_double]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -335,6 +337,25 @@ class Simulation:
         self._check(self.L.arm_rollout_backward(self.ctx, seed, C.byref(da), _ptr(g),
                                                 C.byref(tb)))
         return da.value, g, tb.value
+
+    def set_deposition(self, binning=None, t_end=float("inf")):
+        """The energy-deposition heat source of fluka_heat.f (Cartesian
+        binning): `binning` a dict with origin (3), cell (3), n (3) and data
+        (n0 n1 n2 values in W/m^3, x fastest), as `armando_amd.io.read_fluka_*`
+        return; it acts in the steps with its dt <= t_end.  None switches the
+        source off."""
+        if binning is None:
+            self._check(self.L.arm_set_deposition(self.ctx, None, None, None, None, 0.0))
+            return
+        o = np.ascontiguousarray(binning["origin"], np.float64)
+        d = np.ascontiguousarray(binning["cell"], np.float64)
+        nb = np.ascontiguousarray(binning["n"], np.int32)
+        data = np.ascontiguousarray(np.asarray(binning["data"], np.float64).reshape(-1))
+        if o.size != 3 or d.size != 3 or nb.size != 3 or data.size != int(np.prod(nb.astype(np.int64))):
+            raise ValueError("binning needs origin, cell and n of 3 entries and n0*n1*n2 values")
+        self._check(self.L.arm_set_deposition(self.ctx, _ptr(o), _ptr(d),
+                                               nb.ctypes.data_as(C.c_void_p), _ptr(data),
+                                               float(t_end)))
 
     def set_objective(self, kind):
         """OBJ_PRESSURE (J = sum dt sum p^2) or OBJ_ENERGY (u^2) for the next rollout."""

@@ -63,6 +63,12 @@ struct arm_ctx {
   double grads[ARM_NGRAD] = {0};   // parameter gradients of the last backward
   bool masks_valid = false;  // nbm holds the neighbour masks of the current order
   uint4* nbm = nullptr;
+  // deposition heat source (arm_set_deposition): the map on the device, the
+  // deposition time and the number of the step the next force pass belongs
+  // to (its of single_step.f:219)
+  double* d_dep = nullptr;
+  double dep_tend = 0.0;
+  int its_next = 1;
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   // state double buffer (the active set is A)
@@ -441,9 +447,19 @@ template <typename R> void launch_tensor(arm_ctx* c) {
   }
 }
 
+// The deposition heat source acts in the steps with its dt <= t_end
+// (single_step.f:219).
+void dep_window(arm_ctx* c) {
+  if (c->d_dep && c->its_next * c->prm.dt <= c->dep_tend)
+    c->K.flags |= KF_DEPOSITION;
+  else
+    c->K.flags &= ~KF_DEPOSITION;
+}
+
 // dispatch over the compile-time flag combination
 template <typename R> int force_t(arm_ctx* c, bool press, int flags, bool xs) {
   const bool av = (flags & ARM_ART_VISC) != 0, ah = (flags & ARM_ART_HEAT) != 0;
+  dep_window(c);
   const bool vi = press && (flags & ARM_VISC) != 0;
   PassTimer tm(c, ARM_PASS_FORCE);
   if (vi) launch_tensor<R>(c);
@@ -515,6 +531,7 @@ template <typename R> int predictor_t(arm_ctx* c) {
 
 template <typename R> int corrector_t(arm_ctx* c, int its, bool keys) {
   St<R>& A = stA<R>(c);
+  c->its_next = its + 1;
   PassTimer tm(c, ARM_PASS_INTEGRATE);
   hipLaunchKernelGGL(k_corrector<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
                      its, keys ? 1 : 0, keys ? key_out(c) : KeyOut{}, c->d_err);
@@ -591,6 +608,7 @@ struct StepHooks {
 template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
   const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
   int rc;
+  c->its_next = its;
   const uint32_t* sval = nullptr;
   bool ranges = false;
   // with ghosts: the new set goes after the live slots; the old set carries
@@ -2077,6 +2095,7 @@ void arm_destroy(arm_ctx* c) {
   drop_adjoint(c);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->d_dep) (void)hipFree(c->d_dep);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
@@ -2168,6 +2187,7 @@ int arm_upload(arm_ctx* c, int np, int nv, const double* x, const double* v, con
     if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "ghost set-up: %s", hipGetErrorString(e));
   }
   c->loaded = (rc == ARM_OK);
+  c->its_next = 1;
   return rc;
 }
 
@@ -2449,6 +2469,7 @@ int arm_rollout_forward(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* o
   if (c->prm.flags & (ARM_CONDUCTION | ARM_PERIODIC_X))
     return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint has no conduction or periodic boundary");
   if (c->ghosts) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint has no mirror ghosts");
+  if (c->d_dep) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint has no deposition heat source");
   if (c->prm.sle) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint keeps h fixed (sle = 0)");
   const int ck = std::min(ckpt_every, nsteps);
   int rc;
@@ -2500,6 +2521,54 @@ int arm_update_params(arm_ctx* c, const arm_params* prm) {
   c->prm = P;
   set_kparams(c->K, P);
   drop_adjoint(c);
+  return ARM_OK;
+}
+
+int arm_set_deposition(arm_ctx* c, const double* origin, const double* cell, const int* nbin,
+                       const double* data, double t_end) {
+  if (!c) return ARM_ERR_ARG;
+  if (!data) {  // switch the source off
+    if (c->d_dep) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      HIPCHK(c, hipFree(c->d_dep));
+    }
+    c->d_dep = nullptr;
+    c->K.dep = nullptr;
+    return ARM_OK;
+  }
+  if (is3d(c)) return fail(c, ARM_ERR_UNSUPPORTED, "the deposition heat source is 2D only");
+  if (!origin || !cell || !nbin) return fail(c, ARM_ERR_ARG, "null origin, cell or nbin");
+  size_t count = 1;
+  for (int k = 0; k < 3; ++k) {
+    if (nbin[k] < 1 || !(cell[k] > 0) || !std::isfinite(origin[k]))
+      return fail(c, ARM_ERR_ARG, "bin %d: n = %d, width %g, origin %g", k, nbin[k], cell[k],
+                  origin[k]);
+    count *= (size_t)nbin[k];
+  }
+  if (count > ((size_t)1 << 31)) return fail(c, ARM_ERR_ARG, "map of %zu bins", count);
+  // the 2D step reads the slice z = 0 (fluka_heat.f:488-493)
+  if (!(origin[2] <= 0.0 && 0.0 <= origin[2] + nbin[2] * cell[2]))
+    return fail(c, ARM_ERR_ARG, "z = 0 lies outside the binning's z range [%g, %g]", origin[2],
+                origin[2] + nbin[2] * cell[2]);
+  if (std::isnan(t_end)) return fail(c, ARM_ERR_ARG, "t_end is NaN");
+  if (c->d_dep) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(c->d_dep));
+    c->d_dep = nullptr;
+    c->K.dep = nullptr;
+  }
+  void* p = nullptr;
+  HIPCHK(c, hipMalloc(&p, count * sizeof(double)));
+  c->d_dep = static_cast<double*>(p);
+  HIPCHK(c, hipMemcpyAsync(p, data, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->K.dep = c->d_dep;
+  for (int k = 0; k < 3; ++k) {
+    c->K.dep_n[k] = nbin[k];
+    c->K.dep_o[k] = origin[k];
+    c->K.dep_d[k] = cell[k];
+  }
+  c->dep_tend = t_end;
   return ARM_OK;
 }
 

@@ -51,7 +51,17 @@ struct KParams {
   // mirror ghosts (flag 512, ghost.f): ids >= nb (= np + nv) are the ghosts
   // of the current step; without ghosts nb = n
   int nb;
+  // binned energy deposition (fluka_heat.f, Cartesian binning): map of
+  // dep_n[0] dep_n[1] dep_n[2] values (W/m^3, x fastest) from dep_o in bins
+  // of dep_d; applied in the steps that carry KF_DEPOSITION in flags
+  const double* dep;
+  int dep_n[3];
+  double dep_o[3], dep_d[3];
 };
+
+// Set in KParams::flags by the host for the steps whose time its dt lies
+// within the deposition time (single_step.f:219); not a caller flag.
+constexpr int KF_DEPOSITION = 1 << 24;
 
 // K with the smoothing length mh: h, 1/h, the Wendland 2D constants, the
 // cut-off (2 mh)^2, the artificial viscosity and heat regularisations
@@ -69,6 +79,54 @@ __device__ __forceinline__ KParams with_h(const KParams& K, double mh) {
   P.psi0 = K.ah_g1 * mh * K.c0;
   P.g2h2 = K.ah_g2 * mh * mh;
   return P;
+}
+
+// One axis of the deposition interpolation (fluka_heat.f:256-286): the bins
+// on either side of the position, clamped to the map, and the local
+// coordinate t in [-1, 1) between their centres.
+__device__ __forceinline__ void dep_axis(double s, int n, int& lo, int& hi, double& t) {
+  const int ir = (int)s;  // truncation, as Fortran int()
+  double xr = s - ir;
+  if (xr >= 0.5) {
+    lo = ir;
+    hi = ir + 1;
+    xr = 2.0 * xr - 2.0;
+  } else {
+    lo = ir - 1;
+    hi = ir;
+    xr = 2.0 * xr;
+  }
+  lo = lo < 0 ? 0 : lo;
+  hi = hi >= n ? n - 1 : hi;
+  t = xr;
+}
+
+// Trilinear value of the deposition map at (x, y, 0) between bin centres
+// (fluka_heat.f:485-573, dim = 2, tfluka = 0), 0 outside the binning.  The
+// z slice was checked to lie inside the map when it was set.
+__device__ __forceinline__ double deposition_at(const KParams& K, double x, double y) {
+  const int* n = K.dep_n;
+  const double* o = K.dep_o;
+  const double* d = K.dep_d;
+  if (!(x >= o[0] && y >= o[1] && x <= o[0] + n[0] * d[0] && y <= o[1] + n[1] * d[1]))
+    return 0.0;
+  int i0, i1, j0, j1, k0, k1;
+  double a, b, c;
+  dep_axis((x - o[0]) / d[0], n[0], i0, i1, a);
+  dep_axis((y - o[1]) / d[1], n[1], j0, j1, b);
+  dep_axis((-o[2]) / d[2], n[2], k0, k1, c);
+  const size_t sy = (size_t)n[0], sz = (size_t)n[0] * (size_t)n[1];
+  const double* m = K.dep;
+  auto at = [&](int i, int j, int k) { return m[(size_t)i + (size_t)j * sy + (size_t)k * sz]; };
+  double v = 0.125 * (1.0 - a) * (1.0 - b) * (1.0 - c) * at(i0, j0, k0);
+  v = v + 0.125 * (1.0 + a) * (1.0 - b) * (1.0 - c) * at(i1, j0, k0);
+  v = v + 0.125 * (1.0 + a) * (1.0 + b) * (1.0 - c) * at(i1, j1, k0);
+  v = v + 0.125 * (1.0 - a) * (1.0 + b) * (1.0 - c) * at(i0, j1, k0);
+  v = v + 0.125 * (1.0 - a) * (1.0 - b) * (1.0 + c) * at(i0, j0, k1);
+  v = v + 0.125 * (1.0 + a) * (1.0 - b) * (1.0 + c) * at(i1, j0, k1);
+  v = v + 0.125 * (1.0 + a) * (1.0 + b) * (1.0 + c) * at(i1, j1, k1);
+  v = v + 0.125 * (1.0 - a) * (1.0 + b) * (1.0 + c) * at(i0, j1, k1);
+  return v;
 }
 
 // Gaussian heat source factor exp(-k |x - c|^2) at a position.

@@ -608,6 +608,10 @@ __device__ __forceinline__ void force_end(St<R>& S, const KParams& K, int i, con
       const double2 q = S.xy[i];
       du = du + (R)(K.hs_q0 * heat_shape(K, q.x, q.y)) * o.Q;
     }
+    if ((K.flags & KF_DEPOSITION) && id < K.np) {
+      const double2 q = S.xy[i];
+      du = du + (R)deposition_at(K, q.x, q.y) * o.Q;
+    }
     S.du[i] = du;
   }
   if (XS) {

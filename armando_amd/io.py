@@ -18,6 +18,10 @@ Host-side utilities around the device state (`Simulation.state()` /
   pressure and energy files.
 * VTK legacy polydata, one file per step.
 
+* FLUKA USRBIN energy-deposition maps (CERNsource/fluka_heat.f:1-99 binary,
+  :100-199 text `fluka_binning.FORM`), Cartesian binning: the map the
+  deposition heat source interpolates (`Simulation.set_deposition`).
+
 The reference formats print 9 significant digits; `precise=True` writes 17
 (round-trip exact doubles) in the same column layout.
 """
@@ -243,3 +247,87 @@ def case_from_state(template, state, np_real):
     c["np"] = int(np_real)
     c["nv"] = c["n"] - c["np"]
     return c
+
+
+# ------------------------------------------------------------------ FLUKA
+class Binning(dict):
+    """A Cartesian deposition map: origin (3), cell (3), n (3), type, data
+    (n[0] n[1] n[2] values in W/m^3, x fastest)."""
+
+
+def _scale_binning(typ, minb, dbin, nbin, vals, translation, size_scale, energy_scale,
+                   n_primaries, dep_time):
+    # fluka_heat.f:76-95 / :176-195: sizes scaled by sf, then translated;
+    # values (GeV/cm^3 per primary in FLUKA's units) times
+    # np ef / (dt sf^3).  minb, dbin and vals are REAL(4) in the reference.
+    if typ % 10 != 0:
+        raise ValueError(f"binning type {typ}: only Cartesian binning (type 0) is supported")
+    sf = float(size_scale)
+    origin = [sf * float(np.float32(m)) + float(t) for m, t in zip(minb, translation)]
+    cell = [sf * float(np.float32(d)) for d in dbin]
+    factor = (float(n_primaries) * float(energy_scale)) / (float(dep_time) * sf ** 3)
+    data = factor * np.asarray(vals, dtype=np.float32).astype(np.float64)
+    return Binning(origin=origin, cell=cell, n=[int(k) for k in nbin], type=int(typ % 10),
+                   data=data)
+
+
+def read_fluka_text(path, translation=(0.0, 0.0, 0.0), size_scale=1.0, energy_scale=1.0,
+                    n_primaries=1.0, dep_time=1.0):
+    """read_binning_txt (fluka_heat.f:100-199): five header lines, the
+    binning type, three `min max n width` lines, one skipped line, then the
+    n0 n1 n2 values, free format."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    typ = int(lines[5].replace(",", " ").split()[0])
+    minb, nbin, dbin = [], [], []
+    for k in range(3):
+        t = lines[6 + k].replace(",", " ").split()
+        minb.append(np.float32(t[0]))
+        nbin.append(int(t[2]))
+        dbin.append(np.float32(t[3]))
+    count = nbin[0] * nbin[1] * nbin[2]
+    vals = []
+    for line in lines[10:]:
+        vals.extend(line.replace(",", " ").split())
+        if len(vals) >= count:
+            break
+    if len(vals) < count:
+        raise ValueError(f"{path}: {len(vals)} values, {count} expected")
+    vals = np.array([np.float32(v) for v in vals[:count]], dtype=np.float32)
+    return _scale_binning(typ, minb, dbin, nbin, vals, translation, size_scale, energy_scale,
+                          n_primaries, dep_time)
+
+
+def _records(path):
+    """Records of a Fortran sequential unformatted file (4-byte markers)."""
+    raw = open(path, "rb").read()
+    out, pos = [], 0
+    while pos + 4 <= len(raw):
+        n = int(np.frombuffer(raw, "<i4", 1, pos)[0])
+        body = raw[pos + 4:pos + 4 + n]
+        tail = int(np.frombuffer(raw, "<i4", 1, pos + 4 + n)[0])
+        if tail != n:
+            raise ValueError(f"{path}: bad record marker at byte {pos}")
+        out.append(body)
+        pos += 8 + n
+    return out
+
+
+def read_fluka_binary(path, translation=(0.0, 0.0, 0.0), size_scale=1.0, energy_scale=1.0,
+                      n_primaries=1.0, dep_time=1.0):
+    """read_binning_bin (fluka_heat.f:1-99): the USRBIN unformatted file,
+    records (title, header, weight, ncase), (mbin, name, type, dist, then
+    min max n width for each axis, lntzer, b1, b2, tc), (values)."""
+    rec = _records(path)
+    h = rec[1]
+    typ = int(np.frombuffer(h, "<i4", 1, 14)[0])
+    minb, nbin, dbin = [], [], []
+    for k in range(3):
+        o = 22 + 16 * k
+        minb.append(np.frombuffer(h, "<f4", 1, o)[0])
+        nbin.append(int(np.frombuffer(h, "<i4", 1, o + 8)[0]))
+        dbin.append(np.frombuffer(h, "<f4", 1, o + 12)[0])
+    count = nbin[0] * nbin[1] * nbin[2]
+    vals = np.frombuffer(rec[2], "<f4", count)
+    return _scale_binning(typ, minb, dbin, nbin, vals, translation, size_scale, energy_scale,
+                          n_primaries, dep_time)

@@ -366,7 +366,8 @@ int orc_adjoint(const orc_params* P, const orc_state* s0, int nsteps, const int*
                 double* J_out, double* dJ_dalpha, double* dJ_dvx0, double* dJ_dvy0) {
   const int n = s0->n, np = s0->np;
   if (P->density_method != 0) return -5; /* continuity density only */
-  if ((P->flags & (ORC_VISC | ORC_HEAT_SOURCE | ORC_CONDUCTION | ORC_PERIODIC_X)) || P->sle || P->eos != ORC_EOS_TAIT)
+  if ((P->flags & (ORC_VISC | ORC_HEAT_SOURCE | ORC_CONDUCTION | ORC_PERIODIC_X)) || P->sle || P->eos != ORC_EOS_TAIT ||
+      P->dep)
     return -5; /* Tait, no viscosity, no heat source */
   const double dt = P->dt, h2 = dt / 2.0;
   snap_t* sn = (snap_t*)calloc((size_t)nsteps + 1, sizeof(snap_t));

@@ -486,6 +486,73 @@ static inline double heat_source(const orc_params* P, double x, double y, double
   return P->hs_q0 * exp(-k * (dx * dx + dy * dy)) / rho;
 }
 
+/* One axis of fluka_heat.f:256-286: the two bins around the position and
+   the local coordinate t in [-1, 1) between their centres (bin ir: lower or
+   upper half), clamped to the map. */
+static inline void dep_axis(double s, int n, int* lo, int* hi, double* t) {
+  const int ir = (int)s; /* Fortran int(): truncation */
+  double xr = s - ir;
+  if (xr >= 0.5) {
+    *lo = ir;
+    *hi = ir + 1;
+    xr = 2.0 * xr - 2.0;
+  } else {
+    *lo = ir - 1;
+    *hi = ir;
+    xr = 2.0 * xr;
+  }
+  if (*lo < 0) *lo = 0;
+  if (*hi >= n) *hi = n - 1;
+  *t = xr;
+}
+
+double orc_deposition_at(const orc_params* P, double x, double y, int* inside) {
+  /* fluka_heat.f:488-573 */
+  const int* n = P->dep_n;
+  const double* o = P->dep_o;
+  const double* d = P->dep_d;
+  *inside = x >= o[0] && y >= o[1] && x <= o[0] + n[0] * d[0] && y <= o[1] + n[1] * d[1];
+  if (!*inside) return 0.0;
+  int i0, i1, j0, j1, k0, k1;
+  double a, b, c;
+  dep_axis((x - o[0]) / d[0], n[0], &i0, &i1, &a);
+  dep_axis((y - o[1]) / d[1], n[1], &j0, &j1, &b);
+  dep_axis((-o[2]) / d[2], n[2], &k0, &k1, &c);
+  const size_t sx = 1, sy = (size_t)n[0], sz = (size_t)n[0] * (size_t)n[1];
+#define DEP(i, j, k) P->dep[(size_t)(i) * sx + (size_t)(j) * sy + (size_t)(k) * sz]
+  const double f1 = 0.125 * (1.0 - a) * (1.0 - b) * (1.0 - c);
+  const double f2 = 0.125 * (1.0 + a) * (1.0 - b) * (1.0 - c);
+  const double f3 = 0.125 * (1.0 + a) * (1.0 + b) * (1.0 - c);
+  const double f4 = 0.125 * (1.0 - a) * (1.0 + b) * (1.0 - c);
+  const double f5 = 0.125 * (1.0 - a) * (1.0 - b) * (1.0 + c);
+  const double f6 = 0.125 * (1.0 + a) * (1.0 - b) * (1.0 + c);
+  const double f7 = 0.125 * (1.0 + a) * (1.0 + b) * (1.0 + c);
+  const double f8 = 0.125 * (1.0 - a) * (1.0 + b) * (1.0 + c);
+  const double v = f1 * DEP(i0, j0, k0) + f2 * DEP(i1, j0, k0) + f3 * DEP(i1, j1, k0) +
+                   f4 * DEP(i0, j1, k0) + f5 * DEP(i0, j0, k1) + f6 * DEP(i1, j0, k1) +
+                   f7 * DEP(i1, j1, k1) + f8 * DEP(i0, j1, k1);
+#undef DEP
+  return v;
+}
+
+void orc_fluka_heat(const orc_params* P, int np, const double* x, const double* y,
+                    const double* rho, double* dedt) {
+  /* fluka_heat.f:480-483: dedt reset, then the binned value / rho inside */
+  for (int i = 0; i < np; ++i) {
+    int in;
+    const double v = orc_deposition_at(P, x[i], y[i], &in);
+    dedt[i] = in ? v / rho[i] : 0.0;
+  }
+}
+
+static inline int dep_active(const orc_params* P) { return P->dep != NULL && P->dep_on; }
+
+static inline double deposition(const orc_params* P, double x, double y, double rho) {
+  int in;
+  const double v = orc_deposition_at(P, x, y, &in);
+  return in ? v / rho : 0.0;
+}
+
 void orc_eos_tait(const orc_params* P, int n, double* rho, double* p, double* c) {
   /* eos.f90:255-283 */
   for (int i = 0; i < n; ++i) {
@@ -671,6 +738,7 @@ int orc_single_step(const orc_params* P, orc_state* s, int niac, const int* pi,
     s->du[i] = s->du[i] + avdu[i] + ahdu[i];
     if ((P->flags & ORC_HEAT_SOURCE) && i < np)
       s->du[i] = s->du[i] + heat_source(P, s->x[i], s->y[i], s->rho[i]);
+    if (dep_active(P) && i < np) s->du[i] = s->du[i] + deposition(P, s->x[i], s->y[i], s->rho[i]);
   }
   free(avdv_x);
   free(avdv_y);
@@ -975,6 +1043,7 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
     }
     s->du[i] = (tds + 0.5 * due) + avdu + ahdu + cdu;
     if ((P->flags & ORC_HEAT_SOURCE) && i < np) s->du[i] += heat_source(P, x[i], y[i], rho[i]);
+    if (dep_active(P) && i < np) s->du[i] += deposition(P, x[i], y[i], rho[i]);
     s->avx[i] = xsx;
     s->avy[i] = xsy;
     if (hnew) {
@@ -1071,7 +1140,10 @@ int orc_run(const orc_params* P, orc_state* s, int its0, int nsteps, int gather)
     pj = (int*)malloc(sizeof(int) * (size_t)cap);
   }
   int rc = 0;
+  orc_params Q = *P;
   for (int its = its0; its < its0 + nsteps && rc == 0; ++its) {
+    Q.dep_on = its * P->dt <= P->dep_tend; /* single_step.f:219 */
+    P = &Q;
     if (its != 1) orc_predictor(P, s);
     if ((P->flags & ORC_GHOSTS) && s->gpar && orc_ghost_particles(P, s) < 0) {
       rc = -4;

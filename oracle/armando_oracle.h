@@ -115,6 +115,18 @@ typedef struct {
                         1 h = 2 (m/rho)^(1/dim), 2 dh/dt = (h/dim) div v; with 1 or 2
                         every pair uses mh = (h_i + h_j)/2 and real particles update
                         s->h after the forces (gather twin only) */
+  /* binned energy deposition (fluka_heat.f:201-704, Cartesian binning,
+     tfluka = 0, the 2D branch): with dep != NULL and dep_on, real particles
+     gain du += (trilinear map value) / rho after every other term (the
+     exdudt slot of single_step.f:217-226).  dep holds dep_n[0] dep_n[1]
+     dep_n[2] bin values (W/m^3), x fastest; dep_o and dep_d are the origin
+     and bin size.  orc_run sets dep_on for step its to its dt <= dep_tend
+     (single_step.f:219); a direct single-step call uses dep_on as given. */
+  const double* dep;
+  int dep_n[3];
+  double dep_o[3], dep_d[3];
+  double dep_tend;
+  int dep_on;
 } orc_params;
 
 /* Particle state, SoA, FP64. */
@@ -135,6 +147,14 @@ typedef struct {
   double *xp, *yp;
   int* gpar;
 } orc_state;
+
+/* fluka_heat.f:485-573 (dim = 2, tfluka = 0): dedt[i] of real particles
+   0 .. np-1, 0 outside the binning; the z slice of the 2D branch is z = 0. */
+void orc_fluka_heat(const orc_params* P, int np, const double* x, const double* y,
+                    const double* rho, double* dedt);
+/* The interpolated map value (before the division by rho) at (x, y), and
+   whether (x, y) lies inside the binning. */
+double orc_deposition_at(const orc_params* P, double x, double y, int* inside);
 
 /* ---- kernel ---------------------------------------------------------- */
 /* w and the factor F with grad_i W = F * (x_i - x_j) (kernel.f returns

@@ -247,7 +247,7 @@ static int nrate3(const orc_params* P, const orc3_state* s) {
 }
 
 int orc3_single_step_gather(const orc_params* P, orc3_state* s) {
-  if ((P->flags & (ORC_VISC | ORC_HEAT_SOURCE | ORC_CONDUCTION | ORC_PERIODIC_X)) || P->sle)
+  if ((P->flags & (ORC_VISC | ORC_HEAT_SOURCE | ORC_CONDUCTION | ORC_PERIODIC_X)) || P->sle || P->dep)
     return -2; /* 2D only */
   const int n = s->n, np = s->np, nr = nrate3(P, s);
   const double m = P->mass, h = P->h, cut2 = (2.0 * h) * (2.0 * h);

@@ -252,3 +252,41 @@ def ghost_particles(x, y, vx, vy, h, mass, rho, p, np_, nv, pi, pj):
                     y=X[:k, 1].copy(), vx=V[:k, 0].copy(), vy=V[:k, 1].copy(), h=H[:k].copy(),
                     mass=M[:k].copy(), rho=R[:k].copy(), p=Pp[:k].copy())
     return run_big_stack(f)
+
+
+def fluka_heat(x, y, rho, origin, cell, nbin, data):
+    """fluka_heat.f fluka_heat (as shipped, dim = 3, Cartesian binning
+    tfluka = 0) at z = 0 for every particle; dedt starts at zero (its 3D
+    branch leaves particles outside the binning untouched)."""
+    def f():
+        n = len(x)
+        X = vec3(x, y)
+        R = np.ascontiguousarray(rho, np.float64)
+        out = np.zeros(n)
+        O = np.ascontiguousarray(origin, np.float64)
+        D = np.ascontiguousarray(cell, np.float64)
+        N = np.ascontiguousarray(nbin, np.int32)
+        M = np.ascontiguousarray(np.asarray(data, np.float64).reshape(-1))
+        lib().fluka_heat_(_i(n), _p(X), _p(R), _p(out), _p(O), _p(D), _p(N), _i(0), _p(M))
+        return out
+    return run_big_stack(f)
+
+
+def read_binning(kind, dirpath, translation, sf, ef, npart, dt, count):
+    """read_binning_txt / read_binning_bin of fluka_heat.f on the file
+    fluka_binning.FORM / fluka_binning in dirpath: (origin, cell, n, type,
+    data[:count])."""
+    def f():
+        tr = np.ascontiguousarray(translation, np.float64)
+        O = np.zeros(3)
+        D = np.zeros(3)
+        N = np.zeros(3, np.int32)
+        T = C.c_int(-1)
+        data = np.zeros(count)
+        d = (dirpath.rstrip("/") + "/").encode()
+        fn = lib().read_binning_txt_ if kind == "txt" else lib().read_binning_bin_
+        dbl = lambda v: C.byref(C.c_double(v))  # noqa: E731
+        fn(_p(tr), dbl(sf), dbl(ef), dbl(npart), dbl(dt), _p(O), _p(D), _p(N), C.byref(T),
+           _p(data), C.c_char_p(d), C.c_size_t(len(d)))
+        return O, D, N, T.value, data
+    return run_big_stack(f)

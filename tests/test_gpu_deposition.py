@@ -1,0 +1,152 @@
+"""The energy-deposition heat source (CERNsource/fluka_heat.f:201-704,
+Cartesian binning, applied while its dt <= dtfluka as single_step.f:219
+does) on the GPU, through the C ABI, against the oracle.
+
+Bar: FP64 fields within 1e-10 of the oracle over steps that cross the end
+of the deposition time; the FP32 form within 1e-5 rel-L2 of FP64 on u; the
+routine-by-routine loop equal to the fused loop bit for bit; the errors of
+arm_set_deposition.
+"""
+import numpy as np
+import pytest
+
+import armando_amd
+from armando_amd import cases
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+FIELDS_IN = ("x", "y", "vx", "vy", "rho", "u")
+
+
+def banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def relmax(a, b):
+    scale = max(float(np.max(np.abs(b))), 1e-300)
+    return float(np.max(np.abs(a - b))) / scale
+
+
+def binning(seed=21):
+    """A map over the lower part of the 0.03 x 0.06 m column, z range
+    [-0.01, 0.01]."""
+    n = (5, 6, 2)
+    rng = np.random.default_rng(seed)
+    return dict(origin=(0.005, 0.004, -0.01), cell=(0.004, 0.006, 0.01), n=n,
+                data=rng.uniform(1e8, 1e9, int(np.prod(n))))
+
+
+def oracle_run(case, b, t_end, nsteps):
+    P = orc.params_from(case)
+    orc.set_deposition(P, b["origin"], b["cell"], b["n"], b["data"], t_end)
+    st = orc.OracleState(case["n"], case["np"])
+    st.set(**{k: case[k] for k in FIELDS_IN})
+    orc.run(P, st, 1, nsteps, gather=True)
+    return st
+
+
+@pytest.mark.parametrize("t_steps", [6.5, 1e9])
+def test_deposition_steps_match_oracle(t_steps):
+    case = cases.dam_break_small(scale=0.05)
+    b = binning()
+    t_end = t_steps * case["dt"]
+    banner(f"deposition fp64, 12 steps, t_end = {t_steps} dt")
+    sim = armando_amd.Simulation(case)
+    sim.set_deposition(b, t_end)
+    sim.time_integration(1, 12)
+    s = sim.state()
+    sim.close()
+    st = oracle_run(case, b, t_end, 12)
+    for k in ("x", "y", "rho", "u", "p"):
+        e = relmax(s[k], st.field(k))
+        banner(f"  {k} relmax {e:.2e}")
+        assert e < 1e-10, k
+    for k in ("vx", "vy"):
+        assert np.max(np.abs(s[k] - st.field(k))) < 1e-10 * max(1.0, np.max(np.abs(st.field(k))))
+    if t_steps < 12:
+        # steps 7-12 lie past the deposition time: the source stopped heating
+        longer = oracle_run(case, b, 1e30, 12).field("u")
+        assert np.max(longer - s["u"]) > 1.0
+    # the source heated the particles in the map and not the others
+    plain = armando_amd.Simulation(case)
+    plain.time_integration(1, 12)
+    u0 = plain.state()["u"]
+    plain.close()
+    heated = s["u"] - u0 > 1.0
+    assert heated.sum() > 50 and (~heated).sum() > 50
+
+
+def test_fp32_form_close_to_fp64():
+    b = binning()
+    out = {}
+    for prec in (cases.FP64, cases.FP32):
+        case = cases.dam_break_small(scale=0.05, precision=prec)
+        sim = armando_amd.Simulation(case)
+        sim.set_deposition(b, 8.5 * case["dt"])
+        sim.time_integration(1, 16)
+        out[prec] = sim.state()
+        sim.close()
+    for k in ("u", "rho", "vx", "vy"):
+        a, r = out[cases.FP32][k], out[cases.FP64][k]
+        rel = np.linalg.norm(a - r) / np.linalg.norm(r)
+        banner(f"  fp32 vs fp64 {k} rel-L2 {rel:.2e}")
+        assert rel < 1e-5, k
+
+
+def test_routine_by_routine_equals_fused():
+    """The step number of the routine-by-routine calls follows the last
+    corrector, so the deposition window ends at the same step."""
+    case = cases.dam_break_small(scale=0.05)
+    b = binning()
+    t_end = 4.5 * case["dt"]
+    a = armando_amd.Simulation(case)
+    a.set_deposition(b, t_end)
+    a.time_integration(1, 8)
+    sa = a.state()
+    c = armando_amd.Simulation(case)
+    c.set_deposition(b, t_end)
+    for its in range(1, 9):
+        if its != 1:
+            c.predictor()
+        c.link_list()
+        c.single_step()
+        c.corrector(its)
+    sc = c.state()
+    for k in ("x", "y", "vx", "vy", "rho", "u", "p"):
+        assert np.array_equal(sa[k], sc[k]), k
+    a.close()
+    c.close()
+
+
+def test_switching_off_and_errors():
+    case = cases.dam_break_small(scale=0.05)
+    b = binning()
+    sim = armando_amd.Simulation(case)
+    bad = dict(b, origin=(0.005, 0.004, 0.001))  # z = 0 below the binning
+    with pytest.raises(armando_amd.ArmandoError) as e:
+        sim.set_deposition(bad)
+    assert e.value.code == -1
+    with pytest.raises(armando_amd.ArmandoError) as e:
+        sim.set_deposition(dict(b, cell=(0.004, 0.0, 0.01)))
+    assert e.value.code == -1
+    sim.set_deposition(b)
+    with pytest.raises(armando_amd.ArmandoError) as e:
+        sim.rollout_forward(4, 2, np.arange(10, dtype=np.int32))
+    assert e.value.code == -5
+    sim.set_deposition(None)
+    sim.time_integration(1, 5)
+    s = sim.state()
+    sim.close()
+    plain = armando_amd.Simulation(case)
+    plain.time_integration(1, 5)
+    p = plain.state()
+    plain.close()
+    for k in ("x", "y", "vx", "vy", "rho", "u"):
+        assert np.array_equal(s[k], p[k]), k
+    c3 = cases.dam_break3d_small(scale=0.1)
+    sim3 = armando_amd.Simulation(c3)
+    with pytest.raises(armando_amd.ArmandoError) as e:
+        sim3.set_deposition(b)
+    assert e.value.code == -5
+    sim3.close()
