@@ -484,6 +484,9 @@ template <typename R> int force_t(arm_ctx* c, bool press, int flags, bool xs) {
     case 1: launch_force<R, false, false, false, true>(c); break;
     default: return fail(c, ARM_ERR_ARG, "force pass with nothing to compute");
   }
+  if (press && (c->K.flags & KF_DEPOSITION))
+    hipLaunchKernelGGL(k_deposition<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                       stA<R>(c), c->K);
   HIPCHK(c, hipGetLastError());
   return ARM_OK;
 }
@@ -1352,6 +1355,7 @@ template <typename R> int force3_t(arm_ctx* c, bool press, int flags, bool xs) {
   const bool av = (flags & ARM_ART_VISC) != 0, ah = (flags & ARM_ART_HEAT) != 0;
   if (press && (flags & ARM_VISC))
     return fail(c, ARM_ERR_UNSUPPORTED, "physical viscosity is implemented in 2D only");
+  dep_window(c);
   PassTimer tm(c, ARM_PASS_FORCE);
   const int code = (press ? 8 : 0) | (av ? 4 : 0) | (ah ? 2 : 0) | (xs ? 1 : 0);
   switch (code) {
@@ -1366,6 +1370,9 @@ template <typename R> int force3_t(arm_ctx* c, bool press, int flags, bool xs) {
     case 1: launch_force3<R, false, false, false, true>(c); break;
     default: return fail(c, ARM_ERR_ARG, "force pass with nothing to compute");
   }
+  if (press && (c->K.flags & KF_DEPOSITION))
+    hipLaunchKernelGGL(k3_deposition<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                       st3A<R>(c), c->K);
   HIPCHK(c, hipGetLastError());
   return ARM_OK;
 }
@@ -1393,6 +1400,7 @@ template <typename R> int predictor3_t(arm_ctx* c) {
 }
 
 template <typename R> int corrector3_t(arm_ctx* c, int its, bool keys) {
+  c->its_next = its + 1;
   PassTimer tm(c, ARM_PASS_INTEGRATE);
   hipLaunchKernelGGL(k3_corrector<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
                      st3A<R>(c), c->K, its, keys ? 1 : 0, keys ? key_out(c) : KeyOut{},
@@ -1406,6 +1414,7 @@ template <typename R> int fused_step3(arm_ctx* c, int its, const StepHooks& hk =
   const int n = c->n;
   const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
   int rc;
+  c->its_next = its;
   const uint32_t* sval = nullptr;
   bool ranges = false;
   {
@@ -2536,7 +2545,6 @@ int arm_set_deposition(arm_ctx* c, const double* origin, const double* cell, con
     c->K.dep = nullptr;
     return ARM_OK;
   }
-  if (is3d(c)) return fail(c, ARM_ERR_UNSUPPORTED, "the deposition heat source is 2D only");
   if (!origin || !cell || !nbin) return fail(c, ARM_ERR_ARG, "null origin, cell or nbin");
   size_t count = 1;
   for (int k = 0; k < 3; ++k) {
@@ -2547,7 +2555,7 @@ int arm_set_deposition(arm_ctx* c, const double* origin, const double* cell, con
   }
   if (count > ((size_t)1 << 31)) return fail(c, ARM_ERR_ARG, "map of %zu bins", count);
   // the 2D step reads the slice z = 0 (fluka_heat.f:488-493)
-  if (!(origin[2] <= 0.0 && 0.0 <= origin[2] + nbin[2] * cell[2]))
+  if (!is3d(c) && !(origin[2] <= 0.0 && 0.0 <= origin[2] + nbin[2] * cell[2]))
     return fail(c, ARM_ERR_ARG, "z = 0 lies outside the binning's z range [%g, %g]", origin[2],
                 origin[2] + nbin[2] * cell[2]);
   if (std::isnan(t_end)) return fail(c, ARM_ERR_ARG, "t_end is NaN");

@@ -492,6 +492,18 @@ __device__ __forceinline__ void force3_pair(const KParams& K, ForceOwn3<R>& o, R
   }
 }
 
+// The deposition heat source after the 3D force pass (fluka_heat.f:235-337),
+// as k_deposition in 2D.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k3_deposition(St3<R> S, KParams K) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= K.n || S.id[i] >= K.np) return;
+  const double4 q = S.xyz[i];
+  R Q, A;
+  q_and_a(K, S.rho[i], S.p[i], Q, A);
+  S.du[i] = S.du[i] + (R)deposition_at(K, q.x, q.y, q.z) * Q;
+}
+
 template <typename R, bool PRESS, bool XS>
 __device__ __forceinline__ void force3_end(St3<R>& S, const KParams& K, int i,
                                            const ForceOwn3<R>& o) {

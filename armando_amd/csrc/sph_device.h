@@ -101,20 +101,22 @@ __device__ __forceinline__ void dep_axis(double s, int n, int& lo, int& hi, doub
   t = xr;
 }
 
-// Trilinear value of the deposition map at (x, y, 0) between bin centres
-// (fluka_heat.f:485-573, dim = 2, tfluka = 0), 0 outside the binning.  The
-// z slice was checked to lie inside the map when it was set.
-__device__ __forceinline__ double deposition_at(const KParams& K, double x, double y) {
+// Trilinear value of the deposition map at (x, y, z) between bin centres
+// (fluka_heat.f:235-337, dim = 3, tfluka = 0), 0 outside the binning.  The
+// 2D branch (:485-573) is the same at z = 0: (0 - oz)/dz is its -oz/dz, and
+// z = 0 was checked to lie within the map when it was set.
+__device__ __forceinline__ double deposition_at(const KParams& K, double x, double y, double z) {
   const int* n = K.dep_n;
   const double* o = K.dep_o;
   const double* d = K.dep_d;
-  if (!(x >= o[0] && y >= o[1] && x <= o[0] + n[0] * d[0] && y <= o[1] + n[1] * d[1]))
+  if (!(x >= o[0] && y >= o[1] && z >= o[2] && x <= o[0] + n[0] * d[0] &&
+        y <= o[1] + n[1] * d[1] && z <= o[2] + n[2] * d[2]))
     return 0.0;
   int i0, i1, j0, j1, k0, k1;
   double a, b, c;
   dep_axis((x - o[0]) / d[0], n[0], i0, i1, a);
   dep_axis((y - o[1]) / d[1], n[1], j0, j1, b);
-  dep_axis((-o[2]) / d[2], n[2], k0, k1, c);
+  dep_axis((z - o[2]) / d[2], n[2], k0, k1, c);
   const size_t sy = (size_t)n[0], sz = (size_t)n[0] * (size_t)n[1];
   const double* m = K.dep;
   auto at = [&](int i, int j, int k) { return m[(size_t)i + (size_t)j * sy + (size_t)k * sz]; };

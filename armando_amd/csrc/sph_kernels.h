@@ -608,10 +608,6 @@ __device__ __forceinline__ void force_end(St<R>& S, const KParams& K, int i, con
       const double2 q = S.xy[i];
       du = du + (R)(K.hs_q0 * heat_shape(K, q.x, q.y)) * o.Q;
     }
-    if ((K.flags & KF_DEPOSITION) && id < K.np) {
-      const double2 q = S.xy[i];
-      du = du + (R)deposition_at(K, q.x, q.y) * o.Q;
-    }
     S.du[i] = du;
   }
   if (XS) {
@@ -620,6 +616,20 @@ __device__ __forceinline__ void force_end(St<R>& S, const KParams& K, int i, con
     w.y = o.xsy;
     S.av[i] = w;
   }
+}
+
+// The deposition heat source after the force pass, in the steps where it
+// acts: du += v(x, y, 0) Q with Q = 1/rho as the force pass computes it
+// (fluka_heat.f:485-573; exdudt comes last in single_step.f:226).  A kernel
+// of its own so that the force gather carries none of its registers.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_deposition(St<R> S, KParams K) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= K.n || S.id[i] >= K.np) return;
+  const double2 q = S.xy[i];
+  R Q, A;
+  q_and_a(K, S.rho[i], S.p[i], Q, A);
+  S.du[i] = S.du[i] + (R)deposition_at(K, q.x, q.y, 0.0) * Q;
 }
 
 template <typename R> __device__ __forceinline__ R u_of(const St<R>& S, int t) {

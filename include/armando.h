@@ -232,19 +232,19 @@ int arm_rollout_backward(arm_ctx* ctx, double seed, double* dJ_dalpha, double* d
    not change.  Drops the checkpoints of a previous rollout. */
 int arm_update_params(arm_ctx* ctx, const arm_params* prm);
 /* Energy deposition heat source [CERNsource/fluka_heat.f:201-704, called at
-   single_step.f:219-222], Cartesian binning (tfluka = 0), 2D: in every step
+   single_step.f:219-222], Cartesian binning (tfluka = 0): in every step
    its with its*dt <= t_end each real particle gains
-   du += v(x, y) / rho, v the trilinear interpolation between bin centres
-   of the map at (x, y, z = 0), clamped to the edge bins, 0 outside the
-   binning.  data holds nbin[0]*nbin[1]*nbin[2] values in W/m^3 with x
+   du += v(x) / rho, v the trilinear interpolation between bin centres
+   of the map at the particle (2D: at (x, y, z = 0)), clamped to the edge
+   bins, 0 outside the binning.  data holds nbin[0]*nbin[1]*nbin[2] values in W/m^3 with x
    fastest (the datafluka array after read_binning_*: FLUKA values times
    np ef/(dt sf^3)); origin and cell are the binning's corner and bin size
-   (m); z = 0 must lie within the z range.  data = NULL switches the source
+   (m); in 2D z = 0 must lie within the z range.  data = NULL switches the source
    off.  The step number is the its of arm_time_integration, or for the
    routine-by-routine calls the one after the last arm_corrector (1 after
    arm_upload).  Only the Cartesian binning is implemented (the readers of
-   armando_amd.io reject the R-phi-z binning, tfluka 1 and 7); 3D contexts
-   return ARM_ERR_UNSUPPORTED, and so do the rollouts while a map is set. */
+   armando_amd.io reject the R-phi-z binning, tfluka 1 and 7); the rollouts
+   return ARM_ERR_UNSUPPORTED while a map is set. */
 int arm_set_deposition(arm_ctx* ctx, const double* origin, const double* cell, const int* nbin,
                        const double* data, double t_end);
 /* Objective of the next rollout: J = sum over steps of dt * sum_{obj} f_i

@@ -506,18 +506,20 @@ static inline void dep_axis(double s, int n, int* lo, int* hi, double* t) {
   *t = xr;
 }
 
-double orc_deposition_at(const orc_params* P, double x, double y, int* inside) {
-  /* fluka_heat.f:488-573 */
+double orc_deposition_at(const orc_params* P, double x, double y, double z, int* inside) {
+  /* fluka_heat.f:237-337; with z = 0 the 2D branch :488-573 ((0 - oz)/dz is
+     its -oz/dz, and the z bounds hold for a map that spans z = 0) */
   const int* n = P->dep_n;
   const double* o = P->dep_o;
   const double* d = P->dep_d;
-  *inside = x >= o[0] && y >= o[1] && x <= o[0] + n[0] * d[0] && y <= o[1] + n[1] * d[1];
+  *inside = x >= o[0] && y >= o[1] && z >= o[2] && x <= o[0] + n[0] * d[0] &&
+            y <= o[1] + n[1] * d[1] && z <= o[2] + n[2] * d[2];
   if (!*inside) return 0.0;
   int i0, i1, j0, j1, k0, k1;
   double a, b, c;
   dep_axis((x - o[0]) / d[0], n[0], &i0, &i1, &a);
   dep_axis((y - o[1]) / d[1], n[1], &j0, &j1, &b);
-  dep_axis((-o[2]) / d[2], n[2], &k0, &k1, &c);
+  dep_axis((z - o[2]) / d[2], n[2], &k0, &k1, &c);
   const size_t sx = 1, sy = (size_t)n[0], sz = (size_t)n[0] * (size_t)n[1];
 #define DEP(i, j, k) P->dep[(size_t)(i) * sx + (size_t)(j) * sy + (size_t)(k) * sz]
   const double f1 = 0.125 * (1.0 - a) * (1.0 - b) * (1.0 - c);
@@ -540,7 +542,18 @@ void orc_fluka_heat(const orc_params* P, int np, const double* x, const double* 
   /* fluka_heat.f:480-483: dedt reset, then the binned value / rho inside */
   for (int i = 0; i < np; ++i) {
     int in;
-    const double v = orc_deposition_at(P, x[i], y[i], &in);
+    const double v = orc_deposition_at(P, x[i], y[i], 0.0, &in);
+    dedt[i] = in ? v / rho[i] : 0.0;
+  }
+}
+
+void orc3_fluka_heat(const orc_params* P, int np, const double* x, const double* y,
+                     const double* z, const double* rho, double* dedt) {
+  /* fluka_heat.f:233-337, with exdudt zeroed at the start of the step
+     (single_step.f:92-96) */
+  for (int i = 0; i < np; ++i) {
+    int in;
+    const double v = orc_deposition_at(P, x[i], y[i], z[i], &in);
     dedt[i] = in ? v / rho[i] : 0.0;
   }
 }
@@ -549,7 +562,7 @@ static inline int dep_active(const orc_params* P) { return P->dep != NULL && P->
 
 static inline double deposition(const orc_params* P, double x, double y, double rho) {
   int in;
-  const double v = orc_deposition_at(P, x, y, &in);
+  const double v = orc_deposition_at(P, x, y, 0.0, &in);
   return in ? v / rho : 0.0;
 }
 

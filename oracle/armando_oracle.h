@@ -116,7 +116,8 @@ typedef struct {
                         every pair uses mh = (h_i + h_j)/2 and real particles update
                         s->h after the forces (gather twin only) */
   /* binned energy deposition (fluka_heat.f:201-704, Cartesian binning,
-     tfluka = 0, the 2D branch): with dep != NULL and dep_on, real particles
+     tfluka = 0; the 2D routines read the slice z = 0): with dep != NULL and
+     dep_on, real particles
      gain du += (trilinear map value) / rho after every other term (the
      exdudt slot of single_step.f:217-226).  dep holds dep_n[0] dep_n[1]
      dep_n[2] bin values (W/m^3), x fastest; dep_o and dep_d are the origin
@@ -152,9 +153,10 @@ typedef struct {
    0 .. np-1, 0 outside the binning; the z slice of the 2D branch is z = 0. */
 void orc_fluka_heat(const orc_params* P, int np, const double* x, const double* y,
                     const double* rho, double* dedt);
-/* The interpolated map value (before the division by rho) at (x, y), and
-   whether (x, y) lies inside the binning. */
-double orc_deposition_at(const orc_params* P, double x, double y, int* inside);
+/* The interpolated map value (before the division by rho) at (x, y, z)
+   (fluka_heat.f:235-337, the dim = 3 branch; the 2D branch is the same at
+   z = 0), and whether the point lies inside the binning. */
+double orc_deposition_at(const orc_params* P, double x, double y, double z, int* inside);
 
 /* ---- kernel ---------------------------------------------------------- */
 /* w and the factor F with grad_i W = F * (x_i - x_j) (kernel.f returns
@@ -325,6 +327,10 @@ int orc3_single_step_gather(const orc_params* P, orc3_state* s);
 void orc3_predictor(const orc_params* P, orc3_state* s);
 void orc3_corrector(const orc_params* P, orc3_state* s, int its);
 int orc3_run(const orc_params* P, orc3_state* s, int its0, int nsteps);
+/* fluka_heat.f:235-337 (dim = 3, tfluka = 0): dedt of particles 0 .. np-1,
+   0 outside the binning. */
+void orc3_fluka_heat(const orc_params* P, int np, const double* x, const double* y,
+                     const double* z, const double* rho, double* dedt);
 
 #ifdef __cplusplus
 }

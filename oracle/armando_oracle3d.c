@@ -247,7 +247,7 @@ static int nrate3(const orc_params* P, const orc3_state* s) {
 }
 
 int orc3_single_step_gather(const orc_params* P, orc3_state* s) {
-  if ((P->flags & (ORC_VISC | ORC_HEAT_SOURCE | ORC_CONDUCTION | ORC_PERIODIC_X)) || P->sle || P->dep)
+  if ((P->flags & (ORC_VISC | ORC_HEAT_SOURCE | ORC_CONDUCTION | ORC_PERIODIC_X)) || P->sle)
     return -2; /* 2D only */
   const int n = s->n, np = s->np, nr = nrate3(P, s);
   const double m = P->mass, h = P->h, cut2 = (2.0 * h) * (2.0 * h);
@@ -361,6 +361,11 @@ int orc3_single_step_gather(const orc_params* P, orc3_state* s) {
     s->ay[i] = f[1] + (g ? P->gy : 0.0) + avf[1];
     s->az[i] = f[2] + (g ? P->gz : 0.0) + avf[2];
     s->du[i] = 0.5 * due + avdu + ahdu;
+    if (P->dep && P->dep_on && i < np) {
+      int in;
+      const double v = orc_deposition_at(P, s->x[i], s->y[i], s->z[i], &in);
+      if (in) s->du[i] = s->du[i] + v / s->rho[i];
+    }
     s->avx[i] = xs[0];
     s->avy[i] = xs[1];
     s->avz[i] = xs[2];
@@ -426,7 +431,10 @@ void orc3_corrector(const orc_params* P, orc3_state* s, int its) {
 }
 
 int orc3_run(const orc_params* P, orc3_state* s, int its0, int nsteps) {
+  orc_params Q = *P;
   for (int its = its0; its < its0 + nsteps; ++its) {
+    Q.dep_on = its * P->dt <= P->dep_tend; /* single_step.f:219 */
+    P = &Q;
     if (its != 1) orc3_predictor(P, s);
     const int rc = orc3_single_step_gather(P, s);
     if (rc == -2) return -2;

@@ -96,6 +96,7 @@ def lib():
         "orc_eos_tait": (None, [_PP, C.c_int, _dp, _dp, _dp]),
         "orc_eos": (None, [_PP, C.c_int, _dp, _dp, _dp]),
         "orc_fluka_heat": (None, [_PP, C.c_int, _dp, _dp, _dp, _dp]),
+        "orc3_fluka_heat": (None, [_PP, C.c_int, _dp, _dp, _dp, _dp, _dp]),
         "orc_state_new": (C.POINTER(State), [C.c_int, C.c_int]),
         "orc_state_new_ghosts": (C.POINTER(State), [C.c_int, C.c_int]),
         "orc_ghost_particles": (C.c_int, [_PP, C.POINTER(State)]),
@@ -177,6 +178,14 @@ def fluka_heat(P, x, y, rho, np_real=None):
     n = len(x) if np_real is None else np_real
     out = np.zeros(n)
     lib().orc_fluka_heat(C.byref(P), n, x, y, rho, out)
+    return out
+
+
+def fluka_heat3(P, x, y, z, rho):
+    """dedt of fluka_heat.f (3D, Cartesian binning) for every particle."""
+    x, y, z, rho = _d(x), _d(y), _d(z), _d(rho)
+    out = np.zeros(len(x))
+    lib().orc3_fluka_heat(C.byref(P), len(x), x, y, z, rho, out)
     return out
 
 

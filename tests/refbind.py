@@ -254,13 +254,14 @@ def ghost_particles(x, y, vx, vy, h, mass, rho, p, np_, nv, pi, pj):
     return run_big_stack(f)
 
 
-def fluka_heat(x, y, rho, origin, cell, nbin, data):
+def fluka_heat(x, y, rho, origin, cell, nbin, data, z=None):
     """fluka_heat.f fluka_heat (as shipped, dim = 3, Cartesian binning
-    tfluka = 0) at z = 0 for every particle; dedt starts at zero (its 3D
-    branch leaves particles outside the binning untouched)."""
+    tfluka = 0) for every particle, at z = 0 unless z is given; dedt starts
+    at zero, as single_step.f:92-96 leaves it (the 3D branch does not touch
+    particles outside the binning)."""
     def f():
         n = len(x)
-        X = vec3(x, y)
+        X = vec3(x, y, z)
         R = np.ascontiguousarray(rho, np.float64)
         out = np.zeros(n)
         O = np.ascontiguousarray(origin, np.float64)

@@ -54,6 +54,25 @@ def test_oracle_equals_reference_fluka_heat():
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.skipif(not refbind.available(), reason="oracle/_ref not built")
+def test_oracle_3d_equals_reference_fluka_heat():
+    """The dim = 3 branch with real z, points above, inside and below the
+    binning's z range."""
+    rng = np.random.default_rng(17)
+    x, y = positions(rng, 4000)
+    z = rng.uniform(ORIGIN[2] - 0.02, ORIGIN[2] + NBIN[2] * CELL[2] + 0.02, len(x))
+    z[:12] = [ORIGIN[2], ORIGIN[2] + NBIN[2] * CELL[2]] * 6
+    rho = rng.uniform(900.0, 1100.0, len(x))
+    data = rng.uniform(-1e6, 5e8, int(np.prod(NBIN)))
+    want = refbind.fluka_heat(x, y, rho, ORIGIN, CELL, NBIN, data, z=z)
+    got = orc.fluka_heat3(oracle_params(data), x, y, z, rho)
+    assert np.count_nonzero(want) > 1000
+    np.testing.assert_array_equal(got, want)
+    # at z = 0 the 3D routine is the 2D one
+    np.testing.assert_array_equal(orc.fluka_heat3(oracle_params(data), x, y, 0 * z, rho),
+                                  orc.fluka_heat(oracle_params(data), x, y, rho))
+
+
 def test_constant_map_and_outside():
     rng = np.random.default_rng(12)
     x, y = positions(rng, 2000)

@@ -144,9 +144,35 @@ def test_switching_off_and_errors():
     plain.close()
     for k in ("x", "y", "vx", "vy", "rho", "u"):
         assert np.array_equal(s[k], p[k]), k
-    c3 = cases.dam_break3d_small(scale=0.1)
-    sim3 = armando_amd.Simulation(c3)
-    with pytest.raises(armando_amd.ArmandoError) as e:
-        sim3.set_deposition(b)
-    assert e.value.code == -5
-    sim3.close()
+
+
+def test_deposition_3d_matches_oracle():
+    """The dim = 3 branch (fluka_heat.f:235-337) with real z, in a 3D dam
+    break whose column the map covers in part; the deposition time ends
+    after step 5 of 10."""
+    case = cases.dam_break3d_small(scale=0.1)
+    n = (4, 3, 4)
+    rng = np.random.default_rng(22)
+    b = dict(origin=(0.01, 0.0, 0.02), cell=(0.01, 0.015, 0.02), n=n,
+             data=rng.uniform(1e8, 1e9, int(np.prod(n))))
+    t_end = 5.5 * case["dt"]
+    banner(f"3D deposition fp64 n={case['n']}, 10 steps")
+    sim = armando_amd.Simulation(case)
+    sim.set_deposition(b, t_end)
+    sim.time_integration(1, 10)
+    s = sim.state()
+    sim.close()
+    P = orc.params_from(case)
+    orc.set_deposition(P, b["origin"], b["cell"], b["n"], b["data"], t_end)
+    st = orc.OracleState3(case["n"], case["np"])
+    st.set(**{k: case[k] for k in ("x", "y", "z", "vx", "vy", "vz", "rho", "u")})
+    orc.run3(P, st, 1, 10)
+    for k in ("x", "y", "z", "rho", "u"):
+        e = relmax(s[k], st.field(k))
+        banner(f"  {k} relmax {e:.2e}")
+        assert e < 1e-10, k
+    for k in ("vx", "vy", "vz"):
+        ref = st.field(k)
+        assert np.max(np.abs(s[k] - ref)) < 1e-10 * max(1.0, np.max(np.abs(ref))), k
+    heated = s["u"] > 1.0
+    assert heated.sum() > 20 and (~heated[:case["np"]]).sum() > 20
