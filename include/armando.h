@@ -167,7 +167,8 @@ int arm_density(arm_ctx* ctx, int method);
 int arm_eos(arm_ctx* ctx);
 /* int_force [internal_force.f:1] plus, by flags, art_visc [art_visc.f:1]
    and art_heat [art_heat.f:1], fused in one gather; adds gravity when
-   ARM_GRAVITY is set in the context flags.  With ARM_VISC in flags a shear
+   ARM_GRAVITY is set in the context flags, and the external heating of
+   single_step.f:217-222 (ARM_HEAT_SOURCE, arm_set_deposition) to du.  With ARM_VISC in flags a shear
    tensor pass runs first and the viscous force and tds/dt are added
    [internal_force.f:73-171, 196-430]. */
 int arm_internal_force(arm_ctx* ctx, int flags);
