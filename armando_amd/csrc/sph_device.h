@@ -59,6 +59,18 @@ struct KParams {
   double dep_o[3], dep_d[3];
 };
 
+// The dispatcher deals the workgroups of a launch to the eight XCDs in turn
+// (workgroup b runs on XCD b mod 8), and each XCD has its own L2.  xcd_block
+// renumbers them so that XCD x covers one contiguous share of the grid:
+// neighbouring particles, which read the same neighbour rows, then share an
+// L2.  A bijection of 0 .. gridDim.x-1 for any grid size.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int q = nb / NXCD, r = nb - q * NXCD, x = b - (b / NXCD) * NXCD;
+  return x * q + (x < r ? x : r) + b / NXCD;
+}
+
 // Set in KParams::flags by the host for the steps whose time its dt lies
 // within the deposition time (single_step.f:219); not a caller flag.
 constexpr int KF_DEPOSITION = 1 << 24;
