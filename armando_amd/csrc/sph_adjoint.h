@@ -257,7 +257,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjSc
                                                      double* __restrict__ eta_part) {
   __shared__ double sh[BLOCK];
   double lq0 = 0.0, lal = 0.0, leta = 0.0;
-  const int k = blockIdx.x * BLOCK + threadIdx.x;
+  // one L2 per share of the tape; the partials stay indexed by that share
+  const int kb = xcd_block();
+  const int k = kb * BLOCK + threadIdx.x;
   if (k < K.n) {
     using R2 = typename V2<R>::type;
     const R m = (R)K.mass, h = (R)K.h;
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjSc
     if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) alpha_part[blockIdx.x] += sh[0];
+  if (threadIdx.x == 0) alpha_part[kb] += sh[0];
   __syncthreads();
   sh[threadIdx.x] = lq0;
   __syncthreads();
@@ -415,7 +417,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjSc
     if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) q0_part[blockIdx.x] += sh[0];
+  if (threadIdx.x == 0) q0_part[kb] += sh[0];
   if (VI) {
     __syncthreads();
     sh[threadIdx.x] = leta;
@@ -424,7 +426,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjSc
       if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) eta_part[blockIdx.x] += sh[0];
+    if (threadIdx.x == 0) eta_part[kb] += sh[0];
   }
 }
 
@@ -435,7 +437,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_force(Tape<R> T, KParams K, AdjSc
 //     m Q_l lT_k + m Q_k lT_l.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_adj_tensor(Tape<R> T, KParams K, AdjScratch<R> W) {
-  const int k = blockIdx.x * BLOCK + threadIdx.x;
+  const int k = xcd_block() * BLOCK + threadIdx.x;  // one L2 per share of the tape
   if (k >= K.n) return;
   using R2 = typename V2<R>::type;
   const R m = (R)K.mass, h = (R)K.h;
@@ -508,7 +510,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_eos(Tape<R> T, KParams K, AdjScra
 // 4. density adjoint, gathered; adds to the position / velocity adjoints
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_adj_density(Tape<R> T, KParams K, AdjScratch<R> W) {
-  const int k = blockIdx.x * BLOCK + threadIdx.x;
+  const int k = xcd_block() * BLOCK + threadIdx.x;  // one L2 per share of the tape
   if (k >= K.n) return;
   using R2 = typename V2<R>::type;
   const R m = (R)K.mass, h = (R)K.h;
