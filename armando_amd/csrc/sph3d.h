@@ -331,18 +331,33 @@ __device__ __forceinline__ void density3_direct(St3<R>& S, const KParams& K, int
       const int tA = t1 < t0 + 32 * M3_WORDS ? t1 : t0 + 32 * M3_WORDS;
       uint32_t m0 = 0, m1 = 0, m2 = 0;
       for (int c0 = t0; c0 < tA; c0 += NB) {
+        // a batch starts a multiple of NB past t0, so its bits lie in one
+        // mask word: test into a byte, then place the byte once
+        uint32_t bb = 0u;
   #pragma unroll
         for (int q = 0; q < NB; ++q) {
           const int t = c0 + q;
-          const bool in = t < tA;
-          const auto pj = load_pos3(S, in ? t : t0);
+          const auto pj = load_pos3(S, t < tA ? t : t0);
           R dx, dy, dz;
           const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
-          const int b = t - t0;
-          const uint32_t bit = (in && t != i && r2 < cut2) ? (1u << (b & 31)) : 0u;
-          m0 |= b < 32 ? bit : 0u;
-          m1 |= (b >= 32 && b < 64) ? bit : 0u;
-          m2 |= b >= 64 ? bit : 0u;
+          bb |= (r2 < cut2 ? 1u : 0u) << q;
+        }
+        const int len = tA - c0;
+        if (len < NB) bb &= (1u << len) - 1u;
+        const int b0 = c0 - t0;
+        const uint32_t mb = bb << (b0 & 31);
+        m0 |= b0 < 32 ? mb : 0u;
+        m1 |= (b0 >= 32 && b0 < 64) ? mb : 0u;
+        m2 |= b0 >= 64 ? mb : 0u;
+      }
+      {
+        // the lane's own position passed the test at r = 0: clear it
+        const int sb = i - t0;
+        if (sb >= 0 && sb < tA - t0) {
+          const uint32_t keep = ~(1u << (sb & 31));
+          m0 &= sb < 32 ? keep : ~0u;
+          m1 &= (sb >= 32 && sb < 64) ? keep : ~0u;
+          m2 &= sb >= 64 ? keep : ~0u;
         }
       }
       {
