@@ -569,7 +569,7 @@ template <typename R> int make_ghosts(arm_ctx* c, bool keys) {
   if (np == 0) return ARM_OK;
   PassTimer tm(c, ARM_PASS_KEYS);
   hipLaunchKernelGGL(k_ghost_select<R>, dim3(grid_of(n_old)), dim3(BLOCK), 0, c->stream, A,
-                     c->K, c->G, n_old);
+                     c->K, c->G, c->skey, n_old);
   const int tiles = (np + CS_TILE - 1) / CS_TILE;
   hipLaunchKernelGGL(k_cs_tile_sum, dim3(tiles), dim3(CS_THREADS), 0, c->stream, c->G.gflag, np,
                      c->tsum);

@@ -48,23 +48,24 @@ __global__ __launch_bounds__(BLOCK) void k_ghost_mark(St<R> A, KParams K, Ghosts
 }
 
 // gsel[s]: the chosen virtual particle's index in the fixed (cell, id)
-// list, or -1.
+// list, or -1.  The cell of xprev[s] is the key slot s was sorted by
+// (skey[s]: the slots keep the order of the last sort until the next one),
+// so a particle away from the walls reads four bytes and leaves.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_ghost_select(St<R> A, KParams K, Ghosts G,
+                                                        const uint32_t* __restrict__ skey,
                                                         int n_old) {
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   if (s >= n_old) return;
   const int id = A.id[s];
   if (id >= K.np) return;
-  const double2 xp = G.xprev[s];
-  const double2 q = A.xy[s];
-  int ix, iy;
-  bool ok;
-  const uint32_t key = (uint32_t)cell_of(K, xp.x, xp.y, ix, iy, ok);
+  const uint32_t key = skey[s];
   const double cut = (2.0 * K.h) * (2.0 * K.h);
   int best = -1, bid = 0;
   double bd = 0.0;
-  if (ok && G.vnear[key]) {
+  if (G.vnear[key]) {
+    const double2 xp = G.xprev[s];
+    const double2 q = A.xy[s];
     const Rows rw = rows_of(K, key, G.vstart);
     for (int k = 0; k < 3; ++k)
       for (int t = rw.t0[k]; t < rw.t1[k]; ++t) {
