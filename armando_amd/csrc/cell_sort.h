@@ -113,13 +113,42 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* __restrict__ wsu
   return before + x - v;
 }
 
+// The CS_PER_LANE consecutive counters of a lane (zero past n): two 16-byte
+// loads where the run lies inside the array (p comes from hipMalloc and c0
+// is a multiple of CS_PER_LANE, so the loads are aligned).
+static_assert(CS_PER_LANE == 8, "load8/store8 move eight counters");
+__device__ __forceinline__ void load8(const int* __restrict__ p, int c0, int n, int* v) {
+  if (c0 + 8 <= n) {
+    const int4 a = *reinterpret_cast<const int4*>(p + c0);
+    const int4 b = *reinterpret_cast<const int4*>(p + c0 + 4);
+    v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w;
+    v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = (c0 + k < n) ? p[c0 + k] : 0;
+  }
+}
+
+__device__ __forceinline__ void store8(int* __restrict__ p, int c0, int n, const int* v) {
+  if (c0 + 8 <= n) {
+    *reinterpret_cast<int4*>(p + c0) = make_int4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<int4*>(p + c0 + 4) = make_int4(v[4], v[5], v[6], v[7]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (c0 + k < n) p[c0 + k] = v[k];
+  }
+}
+
 __global__ __launch_bounds__(CS_THREADS) void k_cs_tile_sum(const int* __restrict__ count,
                                                             int ncell, int* __restrict__ tsum) {
   __shared__ int wsum[CS_THREADS / 64];
   const int c0 = blockIdx.x * CS_TILE + threadIdx.x * CS_PER_LANE;
+  int v[CS_PER_LANE];
+  load8(count, c0, ncell, v);
   int s = 0;
 #pragma unroll
-  for (int k = 0; k < CS_PER_LANE; ++k) s += (c0 + k < ncell) ? count[c0 + k] : 0;
+  for (int k = 0; k < CS_PER_LANE; ++k) s += v[k];
   int total;
   block_exclusive_scan<CS_THREADS>(s, wsum, &total);
   if (threadIdx.x == 0) tsum[blockIdx.x] = total;
@@ -132,12 +161,10 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_tile_scan(const int* __restri
   __shared__ int wsum[CS_THREADS / 64];
   const int c0 = blockIdx.x * CS_TILE + threadIdx.x * CS_PER_LANE;
   int v[CS_PER_LANE];
+  load8(count, c0, ncell, v);
   int s = 0;
 #pragma unroll
-  for (int k = 0; k < CS_PER_LANE; ++k) {
-    v[k] = (c0 + k < ncell) ? count[c0 + k] : 0;
-    s += v[k];
-  }
+  for (int k = 0; k < CS_PER_LANE; ++k) s += v[k];
   int part = 0;
   for (int j = threadIdx.x; j < (int)blockIdx.x; j += CS_THREADS) part += tsum[j];
   int base;
@@ -145,11 +172,13 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_tile_scan(const int* __restri
   __syncthreads();  // wsum is reused
   int total;
   int run = base + block_exclusive_scan<CS_THREADS>(s, wsum, &total);
+  int o[CS_PER_LANE];
 #pragma unroll
   for (int k = 0; k < CS_PER_LANE; ++k) {
-    if (c0 + k < ncell) cstart[c0 + k] = run;
+    o[k] = run;
     run += v[k];
   }
+  store8(cstart, c0, ncell, o);
   if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
 }
 
