@@ -355,6 +355,16 @@ def conservation(P, st):
     return out
 
 
+def predictor(P, st):
+    """The half step of time_integration.f:136-154 on the stored rates."""
+    lib().orc_predictor(C.byref(P), st.ptr)
+
+
+def corrector(P, st, its):
+    """The update of time_integration.f:186-217 for step its."""
+    lib().orc_corrector(C.byref(P), st.ptr, its)
+
+
 def adjoint(P, st, nsteps, objmask):
     """J and dJ/d(alpha, vx0, vy0) by the oracle's discrete adjoint."""
     n = st.n
