@@ -6,8 +6,11 @@
 // alpha, heat source amplitude q0 and dynamic viscosity eta.  forward() runs
 // arm_rollout_forward (checkpoints kept in the device context); backward()
 // runs arm_rollout_backward with the incoming gradient as the adjoint seed
-// and returns dJ/dv0, dJ/dalpha, dJ/dq0 and dJ/deta.  Host code only; the
-// device work is in libarmando.so.
+// and returns dJ/dv0, dJ/dalpha, dJ/dq0 and dJ/deta.  The context holds the
+// checkpoints of one forward at a time: every forward gets a generation
+// number, and a backward whose forward is no longer the live one first runs
+// that forward again from its saved inputs.  Host code only; the device work
+// is in libarmando.so.
 #include <torch/extension.h>
 
 #include <cstdint>
@@ -25,6 +28,8 @@ struct Rollout {
   std::vector<double> x, rho, u;  // x is (2, n): x and y of a particle adjacent
   std::vector<int32_t> obj;
   int nsteps = 0, ckpt = 0;
+  int64_t next_gen = 0;  // generation of the next forward
+  int64_t live_gen = -1;  // generation whose checkpoints the context holds
 };
 
 void check(const Rollout* r, int rc, const char* what) {
@@ -42,6 +47,28 @@ std::vector<double> to_vec(const torch::Tensor& t) {
   return std::vector<double>(c.data_ptr<double>(), c.data_ptr<double>() + c.numel());
 }
 
+double scalar(const torch::Tensor& t) {
+  return t.detach().to(torch::kCPU, torch::kFloat64).item<double>();
+}
+
+// arm_rollout_forward from the fixed initial state with initial velocities v
+// ((n, 2), CPU, FP64) and the three scalars; returns J.
+double run_forward(Rollout* r, const torch::Tensor& v, double alpha, double q0, double eta) {
+  arm_params P = r->prm;
+  P.av_alpha = alpha;
+  P.hs_q0 = q0;
+  P.visc_eta = eta;
+  check(r, arm_update_params(r->ctx, &P), "arm_update_params");
+  check(r, arm_upload(r->ctx, r->np, r->nv, r->x.data(), v.data_ptr<double>(), r->rho.data(),
+                      r->u.data()),
+        "arm_upload");
+  double J = 0.0;
+  check(r, arm_rollout_forward(r->ctx, r->nsteps, r->ckpt, r->obj.data(), (int)r->obj.size(),
+                               &J, nullptr),
+        "arm_rollout_forward");
+  return J;
+}
+
 class RolloutFn : public torch::autograd::Function<RolloutFn> {
  public:
   static torch::Tensor forward(torch::autograd::AutogradContext* actx, torch::Tensor v0,
@@ -50,20 +77,18 @@ class RolloutFn : public torch::autograd::Function<RolloutFn> {
     Rollout* r = as_rollout(handle);
     const int n = r->np + r->nv;
     TORCH_CHECK(v0.dim() == 2 && v0.size(0) == n && v0.size(1) == 2, "v0 must be (n, 2)");
-    auto v = v0.detach().to(torch::kCPU, torch::kFloat64).contiguous();
-    arm_params P = r->prm;
-    P.av_alpha = alpha.detach().to(torch::kCPU, torch::kFloat64).item<double>();
-    P.hs_q0 = q0.detach().to(torch::kCPU, torch::kFloat64).item<double>();
-    P.visc_eta = eta.detach().to(torch::kCPU, torch::kFloat64).item<double>();
-    check(r, arm_update_params(r->ctx, &P), "arm_update_params");
-    check(r, arm_upload(r->ctx, r->np, r->nv, r->x.data(), v.data_ptr<double>(), r->rho.data(),
-                        r->u.data()),
-          "arm_upload");
-    double J = 0.0;
-    check(r, arm_rollout_forward(r->ctx, r->nsteps, r->ckpt, r->obj.data(), (int)r->obj.size(),
-                                 &J, nullptr),
-          "arm_rollout_forward");
+    auto v = v0.detach().to(torch::kCPU, torch::kFloat64).contiguous().clone();
+    const double a = scalar(alpha), q = scalar(q0), e = scalar(eta);
+    r->live_gen = -1;  // a failed forward leaves no usable checkpoints
+    const double J = run_forward(r, v, a, q, e);
+    const int64_t gen = r->next_gen++;
+    r->live_gen = gen;
     actx->saved_data["handle"] = handle;
+    actx->saved_data["gen"] = gen;
+    actx->saved_data["v"] = v;
+    actx->saved_data["alpha"] = a;
+    actx->saved_data["q0"] = q;
+    actx->saved_data["eta"] = e;
     actx->saved_data["v_device"] = v0.device().str();
     return torch::tensor(J, torch::dtype(torch::kFloat64).device(alpha.device()));
   }
@@ -71,6 +96,14 @@ class RolloutFn : public torch::autograd::Function<RolloutFn> {
   static torch::autograd::tensor_list backward(torch::autograd::AutogradContext* actx,
                                                torch::autograd::tensor_list grads) {
     Rollout* r = as_rollout(actx->saved_data["handle"].toInt());
+    const int64_t gen = actx->saved_data["gen"].toInt();
+    if (r->live_gen != gen) {
+      // a later forward replaced this one's checkpoints: run it again
+      r->live_gen = -1;
+      run_forward(r, actx->saved_data["v"].toTensor(), actx->saved_data["alpha"].toDouble(),
+                  actx->saved_data["q0"].toDouble(), actx->saved_data["eta"].toDouble());
+      r->live_gen = gen;
+    }
     const std::string dev = actx->saved_data["v_device"].toStringRef();
     const double seed = grads[0].detach().to(torch::kCPU, torch::kFloat64).item<double>();
     const int n = r->np + r->nv;

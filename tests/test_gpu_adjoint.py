@@ -167,6 +167,43 @@ def test_torch_autograd_function_backward_is_the_adjoint():
     loss.close()
 
 
+def test_torch_backward_of_an_earlier_forward():
+    """Two forwards on one RolloutLoss, then the backward of the first: its
+    gradients are those of the first forward's inputs, not of the second's
+    (the context keeps one forward's checkpoints; the first is run again)."""
+    import torch
+
+    from armando_amd.autograd import RolloutLoss
+
+    c, obj = adjoint_case()
+    steps = 10
+    banner("torch autograd: backward of an earlier forward")
+    a1, a2 = c["av_alpha"], 3.0 * c["av_alpha"]
+    ref1 = gpu_rollout(c, obj, steps, 4, alpha=a1)
+    ref2 = gpu_rollout(c, obj, steps, 4, alpha=a2)
+    assert ref1[1] != ref2[1]
+    loss = RolloutLoss(c, obj, steps, 4)
+    v0 = np.stack([c["vx"], c["vy"]], 1)
+    va = torch.tensor(v0, dtype=torch.float64, requires_grad=True)
+    vb = torch.tensor(1.5 * v0, dtype=torch.float64, requires_grad=True)
+    al1 = torch.tensor(a1, dtype=torch.float64, requires_grad=True)
+    al2 = torch.tensor(a2, dtype=torch.float64, requires_grad=True)
+    J1 = loss(va, al1)
+    J2 = loss(vb, al2)
+    assert J1.item() == ref1[0]
+    J1.backward()
+    assert al1.grad.item() == pytest.approx(ref1[1], rel=1e-12)
+    assert np.allclose(va.grad.numpy(), ref1[2], rtol=1e-12, atol=0)
+    J2.backward()
+    c2 = dict(c)
+    c2["vx"], c2["vy"] = 1.5 * c["vx"], 1.5 * c["vy"]
+    ref2b = gpu_rollout(c2, obj, steps, 4, alpha=a2)
+    assert J2.item() == ref2b[0]
+    assert al2.grad.item() == pytest.approx(ref2b[1], rel=1e-12)
+    assert np.allclose(vb.grad.numpy(), ref2b[2], rtol=1e-12, atol=0)
+    loss.close()
+
+
 def test_adjoint_100k_400_steps_against_central_differences_on_16_directions():
     """The 100k adjoint workload at full size: 400 steps, checkpoint every 20.
     dJ/dalpha and dJ/dv0 . d for 16 directions d ~ N(0, 1) per fluid particle
