@@ -37,7 +37,9 @@ EXPORTS = ("arm_create", "arm_destroy", "arm_last_error", "arm_domain_violations
            "arm_rollout_backward", "arm_update_params", "arm_set_objective",
            "arm_rollout_param_grads", "arm_conservation3", "arm_upload_h", "arm_download_h",
            "arm_ghost_particles", "arm_ghost_count", "arm_download_ghosts",
-           "arm_set_deposition")
+           "arm_set_deposition", "arm_link_list_static")
+ARM_STATIC_MAX_PARTICLES = 1 << 20  # arm_link_list_static limits (include/armando.h)
+ARM_STATIC_MAX_CELLS = 1 << 20
 
 
 class ArmParams(C.Structure):
@@ -122,6 +124,8 @@ def load_library(path=LIB_PATH):
         "arm_set_objective": (C.c_int, [vp, C.c_int]),
         "arm_rollout_param_grads": (C.c_int, [vp, dpn, C.c_int]),
         "arm_set_deposition": (C.c_int, [vp, dpn, dpn, C.c_void_p, dpn, C.c_double]),
+        # device pointers: x, key_sorted, id_sorted, cell_start, outside, stream
+        "arm_link_list_static": (C.c_int, [C.POINTER(ArmParams), C.c_int] + [vp] * 6),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -2608,3 +2609,130 @@ int arm_set_timing(arm_ctx* c, int on) {
 }
 
 }  // extern "C"
+
+// =============================================================================
+// arm_link_list_static: the cell keys, the counting sort and the cell ranges on
+// caller device buffers, with the sort's workspace in a static device array
+// (no allocation on any call).
+// =============================================================================
+namespace {
+
+constexpr size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t ST_N = ARM_STATIC_MAX_PARTICLES;
+constexpr size_t ST_C = ARM_STATIC_MAX_CELLS;
+constexpr size_t ST_TILES = (ST_C + CS_TILE - 1) / CS_TILE;
+static_assert(ST_TILES <= (size_t)CS_MAX_TILES, "static grid beyond the counting sort");
+// regions: per-cell counters, tile sums, key by particle, (id, slot) entries
+// grouped by cell, their cells, the outside count when the caller passes none
+constexpr size_t ST_COUNT = 0;
+constexpr size_t ST_TSUM = ST_COUNT + al256(4 * ST_C);
+constexpr size_t ST_KEY = ST_TSUM + al256(4 * ST_TILES);
+constexpr size_t ST_ENT = ST_KEY + al256(4 * ST_N);
+constexpr size_t ST_ENTC = ST_ENT + al256(8 * ST_N);
+constexpr size_t ST_OUT = ST_ENTC + al256(4 * ST_N);
+constexpr size_t ST_BYTES = ST_OUT + 256;
+
+}  // namespace
+
+// One copy per GPU, held until the process ends (about 20 MB).  Host code
+// never uses this name as an address: static_scratch() gives each device's.
+static __device__ __align__(256) uint8_t scratch[ST_BYTES];
+
+namespace {
+
+constexpr int ST_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_dev[ST_MAX_DEVICES] = {};
+
+// Address of the current device's copy of scratch, looked up on the first
+// call on that device.
+int static_scratch(uint8_t** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return ARM_ERR_HIP;
+  if (dev < 0 || dev >= ST_MAX_DEVICES) return ARM_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return ARM_ERR_HIP;
+    g_scratch_dev[dev] = static_cast<uint8_t*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return ARM_OK;
+}
+
+// Keys of the particles of x(2, n) in id order (slot t = particle t), with
+// the per-cell counts of the counting sort; outside counts the particles
+// outside the grid (their key is that of cell 0, as in k_keys).
+__global__ __launch_bounds__(CS_THREADS) void k_keys_xy(const double* __restrict__ x, int n,
+                                                        KParams K, KeyOut ko,
+                                                        int* __restrict__ outside) {
+  const int t = blockIdx.x * CS_THREADS + threadIdx.x;
+  const bool valid = t < n;
+  uint32_t key = 0;
+  if (valid) {
+    int ix, iy;
+    bool ok;
+    key = (uint32_t)cell_of(K, x[2 * (size_t)t], x[2 * (size_t)t + 1], ix, iy, ok);
+    if (!ok) atomicAdd(outside, 1);
+  }
+  emit_key(ko, valid, t, t, key);
+}
+
+}  // namespace
+
+extern "C" int arm_link_list_static(const arm_params* prm, int n, const double* x,
+                                    int32_t* key_sorted, int32_t* id_sorted,
+                                    int32_t* cell_start, int32_t* outside, void* stream) {
+  if (!prm || n < 0 || !cell_start || (n > 0 && (!x || !key_sorted || !id_sorted)))
+    return ARM_ERR_ARG;
+  if (prm->dim == 3) return ARM_ERR_UNSUPPORTED;
+  if ((prm->dim != 0 && prm->dim != 2) || !(prm->cell > 0) || prm->nx < 1 || prm->ny < 1)
+    return ARM_ERR_ARG;
+  const long long ncell_ll = (long long)prm->nx * prm->ny;
+  if (n > ARM_STATIC_MAX_PARTICLES || ncell_ll > ARM_STATIC_MAX_CELLS) return ARM_ERR_CAPACITY;
+  // k_cs_tile_scan stores cell_start eight entries at a time in 16-byte words
+  if ((reinterpret_cast<uintptr_t>(cell_start) & 15u) != 0u) return ARM_ERR_ARG;
+  uint8_t* base = nullptr;
+  const int rc = static_scratch(&base);
+  if (rc) return rc;
+  const int ncell = (int)ncell_ll;
+  KParams K{};
+  K.n = n;
+  K.nx = prm->nx;
+  K.ny = prm->ny;
+  K.ncell = ncell;
+  K.ox = prm->ox;
+  K.oy = prm->oy;
+  K.cell = prm->cell;
+  int* count = reinterpret_cast<int*>(base + ST_COUNT);
+  int* tsum = reinterpret_cast<int*>(base + ST_TSUM);
+  uint32_t* key = reinterpret_cast<uint32_t*>(base + ST_KEY);
+  uint64_t* ent = reinterpret_cast<uint64_t*>(base + ST_ENT);
+  uint32_t* entc = reinterpret_cast<uint32_t*>(base + ST_ENTC);
+  int* out = outside ? outside : reinterpret_cast<int*>(base + ST_OUT);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int tiles = (ncell + CS_TILE - 1) / CS_TILE;
+  const int nb = (n + CS_THREADS - 1) / CS_THREADS;
+  // fill kernels, not memsets: a caller may capture this call in a graph
+  hipLaunchKernelGGL(k_cs_zero, dim3(std::min(1024, (ncell + CS_THREADS - 1) / CS_THREADS)),
+                     dim3(CS_THREADS), 0, s, count, ncell);
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(BLOCK), 0, s, reinterpret_cast<uint32_t*>(out),
+                     (size_t)1);
+  if (n > 0)
+    hipLaunchKernelGGL(k_keys_xy, dim3(nb), dim3(CS_THREADS), 0, s, x, n, K,
+                       KeyOut{nullptr, nullptr, key, count}, out);
+  hipLaunchKernelGGL(k_cs_tile_sum, dim3(tiles), dim3(CS_THREADS), 0, s, count, ncell, tsum);
+  hipLaunchKernelGGL(k_cs_tile_scan, dim3(tiles), dim3(CS_THREADS), 0, s, count, ncell, n, tsum,
+                     reinterpret_cast<int*>(cell_start));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cs_scatter, dim3(nb), dim3(CS_THREADS), 0, s, key,
+                       static_cast<const int32_t*>(nullptr), n,
+                       reinterpret_cast<const int*>(cell_start), count, ent, entc);
+    hipLaunchKernelGGL(k_cs_rank, dim3(nb), dim3(CS_THREADS), 0, s, ent, entc, n,
+                       reinterpret_cast<const int*>(cell_start),
+                       reinterpret_cast<uint32_t*>(key_sorted),
+                       reinterpret_cast<uint32_t*>(id_sorted), -1);
+  }
+  return hipGetLastError() == hipSuccess ? ARM_OK : ARM_ERR_HIP;
+}

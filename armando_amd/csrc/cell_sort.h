@@ -182,7 +182,8 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_tile_scan(const int* __restri
   if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
 }
 
-// entry = id << 32 | slot; entc = its cell.  Particles in slot order.
+// entry = id << 32 | slot; entc = its cell.  Particles in slot order; with
+// id_of null the particle in slot t has id t.
 __global__ __launch_bounds__(CS_THREADS) void k_cs_scatter(const uint32_t* __restrict__ key_by_slot,
                                                            const int32_t* __restrict__ id_of,
                                                            int n, const int* __restrict__ cstart,
@@ -198,7 +199,8 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_scatter(const uint32_t* __res
   old = __shfl(old, r.hl, 64);
   if (!valid) return;
   const int pos = cstart[c] + old - 1 - r.rank;
-  ent[pos] = ((uint64_t)(uint32_t)id_of[t] << 32) | (uint64_t)(uint32_t)t;
+  const uint32_t id = id_of ? (uint32_t)id_of[t] : (uint32_t)t;
+  ent[pos] = ((uint64_t)id << 32) | (uint64_t)(uint32_t)t;
   entc[pos] = c;
 }
 

@@ -289,6 +289,34 @@ int arm_pass_times(arm_ctx* ctx, int n, double* ms, int* counts);
 int arm_reset_pass_times(arm_ctx* ctx);
 /* Enable (1) or disable (0) per-pass event timing. */
 int arm_set_timing(arm_ctx* ctx, int on);
+/* No counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  The cell keys, the (cell, id) counting sort and the
+   cell ranges of arm_link_list + arm_get_cells, without a context, on device
+   buffers the caller owns:
+     x           device, x(2, n) FP64 (Fortran layout), particle t = id t;
+     key_sorted  device, n entries: sorted cell keys;
+     id_sorted   device, n entries: particle ids in (cell, id) order;
+     cell_start  device, nx*ny + 1 entries, 16-byte aligned: cell c holds
+                 sorted positions cell_start[c] .. cell_start[c+1]-1;
+     outside     device, 1 entry, or NULL: receives the number of particles
+                 outside the grid (arm_link_list's ARM_ERR_DOMAIN; such a
+                 particle is put in cell 0 and the result is not valid);
+     stream      hipStream_t to enqueue on (NULL: the null stream).
+   Of prm only dim (2 or 0), ox, oy, cell, nx, ny are read.  The call
+   enqueues the work and returns without waiting; it calls no allocator.
+   The sort's workspace is a static device array of about 20 MB, one per
+   GPU, kept until the process ends and shared by every call on that GPU:
+   calls that overlap on one GPU race.  The first call on a device looks
+   up that device's copy (make it outside a graph capture).  Limits:
+   n <= ARM_STATIC_MAX_PARTICLES and nx*ny <= ARM_STATIC_MAX_CELLS
+   (ARM_ERR_CAPACITY beyond them, before any work); 3D: ARM_ERR_UNSUPPORTED. */
+#define ARM_STATIC_MAX_PARTICLES (1 << 20)
+#define ARM_STATIC_MAX_CELLS (1 << 20)
+int arm_link_list_static(const arm_params* prm, int n, const double* x, int32_t* key_sorted,
+                         int32_t* id_sorted, int32_t* cell_start, int32_t* outside,
+                         void* stream);
+
 /* Select the gather implementation: 1 (default) LDS-tiled cell segments,
    0 direct per-particle gathers from global memory.  Both visit neighbours
    in the same order with the same arithmetic and give identical results.
