@@ -131,6 +131,35 @@ def dam_break(col_w=0.6, col_h=1.2, tank_w=3.2, tank_h=1.6, dx=1e-3, jitter=0.01
     )
 
 
+def c_path_dam_state(state, np_real, precision=FP64, alpha=0.01, eps=0.5, cfl=0.2):
+    """A case from a state of the reference's C dam break (ArmandoList
+    initDamBreak, armando2Dcpu.cu:1790-1908), as armando_amd.io.read_c and
+    io.walls_last give it (fluid first): its material constants (rho0 = 1000,
+    c0 = 50, the mass and h of the state's info columns) and its fixed grid
+    (origin (-1, -1), cells of 2h over [-1, 6] x [-1, 4], nX = (int)(7 / 2h)
+    + 1), stepped by this boundary's step with dynamic walls, gravity,
+    Monaghan viscosity, art_heat and XSPH.  Density starts at rho0 and the
+    energy at 0, as initDamBreak sets them (the state files carry no field)."""
+    rho0, c0 = 1000.0, 50.0
+    h = float(state["h"][0])
+    mass = float(state["mass"][0])
+    cell = 2.0 * h
+    n = len(state["x"])
+    flags = DBC | GRAVITY | AV_VEL | ART_VISC | ART_HEAT
+    return dict(
+        name="c_path_dam_state", n=n, np=int(np_real), nv=n - int(np_real),
+        x=np.ascontiguousarray(state["x"], np.float64),
+        y=np.ascontiguousarray(state["y"], np.float64),
+        vx=np.ascontiguousarray(state["vx"], np.float64),
+        vy=np.ascontiguousarray(state["vy"], np.float64),
+        rho=np.full(n, rho0), u=np.zeros(n),
+        precision=precision, flags=flags, pa_sph=0, density_method=0, kernel=0, h=h,
+        mass=mass, rho0=rho0, c0=c0, pmin=0.0, av_alpha=alpha, av_beta=0.0, av_eta=0.1,
+        ah_g1=0.1, ah_g2=1.0, ah_eta=0.1, xsph_eps=eps, gx=0.0, gy=-9.81, dt=cfl * h / c0,
+        ox=-1.0, oy=-1.0, cell=cell, nx=int(7.0 / cell) + 1, ny=int(5.0 / cell) + 1,
+    )
+
+
 def dam_break_720k(precision=FP64):
     """The 720,000-fluid-particle dam break (seed 12345)."""
     return dam_break(precision=precision)
