@@ -95,6 +95,8 @@ struct arm_ctx {
   uint64_t* ent = nullptr;  // (id, slot) entries grouped by cell (cell in sk[1])
   uint32_t* skey = nullptr;  // sorted keys of the current order
   int* cstart = nullptr;
+  // d_err[0]: particles outside the grid, [1]: h outgrew the cell, [2]: ghost
+  // count, [3]: the halt word of a fused run (KParams::halt)
   int* d_err = nullptr;
   double* d_part = nullptr;
   int* d_cnt = nullptr;
@@ -255,14 +257,15 @@ int run_sort(arm_ctx* c, const int32_t* id_of, const uint32_t** sval_out, bool* 
   if (c->counting) {
     const int ncell = sort_cells(c);
     const int nb = (n + CS_THREADS - 1) / CS_THREADS;
+    const Halt hl{c->K.halt, c->K.its};
     hipLaunchKernelGGL(k_cs_tile_sum, dim3(c->cs_tiles), dim3(CS_THREADS), 0, c->stream, c->cnt,
-                       ncell, c->tsum);
+                       ncell, c->tsum, hl);
     hipLaunchKernelGGL(k_cs_tile_scan, dim3(c->cs_tiles), dim3(CS_THREADS), 0, c->stream, c->cnt,
-                       ncell, n, c->tsum, c->cstart);
+                       ncell, n, c->tsum, c->cstart, hl);
     hipLaunchKernelGGL(k_cs_scatter, dim3(nb), dim3(CS_THREADS), 0, c->stream, c->key_by_id,
-                       id_of, n, c->cstart, c->cnt, c->ent, c->sk[1]);
+                       id_of, n, c->cstart, c->cnt, c->ent, c->sk[1], hl);
     hipLaunchKernelGGL(k_cs_rank, dim3(nb), dim3(CS_THREADS), 0, c->stream, c->ent, c->sk[1], n,
-                       c->cstart, c->sk[0], c->sv[0], c->ghosts ? c->K.ncell : -1);
+                       c->cstart, c->sk[0], c->sv[0], c->ghosts ? c->K.ncell : -1, hl);
     HIPCHK(c, hipGetLastError());
     c->skey = c->sk[0];
     *sval_out = c->sv[0];
@@ -300,21 +303,67 @@ template <typename R> St<R>& stB(arm_ctx* c);
 template <> St<double>& stB<double>(arm_ctx* c) { return c->B64; }
 template <> St<float>& stB<float>(arm_ctx* c) { return c->B32; }
 
-// d_err[0]: particles outside the hash grid; d_err[1]: particles whose
-// evolving smoothing length outgrew the cell (2h > cell)
-int check_domain(arm_ctx* c) {
-  int bad[2] = {0, 0};
-  HIPCHK(c, hipMemcpyAsync(bad, c->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+// Swaps the active state buffer with the other one.
+void swap_state(arm_ctx* c) {
+  if (c->prm.dim == 3) {
+    if (c->prm.precision == ARM_FP32)
+      std::swap(c->A3_32, c->B3_32);
+    else
+      std::swap(c->A3_64, c->B3_64);
+  } else if (c->prm.precision == ARM_FP32) {
+    std::swap(c->A32, c->B32);
+  } else {
+    std::swap(c->A64, c->B64);
+  }
+}
+
+// Reads and clears the error words: d_err[0] particles outside the hash
+// grid, d_err[1] particles whose evolving smoothing length outgrew the cell
+// (2h > cell), d_err[3] the halt word.  its_end: one past the last step of a
+// fused run whose kernels halted at the first failure (-1 otherwise).  Each
+// halted step still swapped the state buffers on the host without writing
+// them, so an odd number of halted steps is undone here: the state is then
+// the one the failing step left.
+int check_domain(arm_ctx* c, int its_end = -1) {
+  int bad[4] = {0, 0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(bad, c->d_err, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (bad[0] || bad[1]) {
+  const int stop = bad[3];
+  if (stop && its_end >= 0 && (its_end - stop) % 2 != 0) swap_state(c);
+  if (bad[0] || bad[1] || stop) {
     c->domain_bad = bad[0] + bad[1];
     HIPCHK(c, hipMemsetAsync(c->d_err, 0, 2 * sizeof(int), c->stream));
-    if (bad[0]) return fail(c, ARM_ERR_DOMAIN, "%d particle(s) outside the hash grid", bad[0]);
-    return fail(c, ARM_ERR_DOMAIN, "%d particle(s) with 2h larger than the cell %g", bad[1],
-                c->K.cell);
+    HIPCHK(c, hipMemsetAsync(c->d_err + 3, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    char where[96] = "";
+    if (stop && its_end >= 0)
+      snprintf(where, sizeof(where), " after step %d (steps %d.. not run; the state is that of step %d)",
+               stop - 1, stop, stop - 1);
+    if (bad[0])
+      return fail(c, ARM_ERR_DOMAIN, "%d particle(s) outside the hash grid%s", bad[0], where);
+    return fail(c, ARM_ERR_DOMAIN, "%d particle(s) with 2h larger than the cell %g%s", bad[1],
+                c->K.cell, where);
   }
   return ARM_OK;
 }
+
+// A fused run of steps that stops at its first domain error: while it lives
+// the step kernels read the halt word (cleared at its start by a fill
+// kernel) and the step number c->K.its.
+struct FusedRun {
+  arm_ctx* c;
+  FusedRun(arm_ctx* c_, int its0) : c(c_) {
+    c->K.halt = c->d_err + 3;
+    c->K.its = its0 - 1;  // the keys hashed before the first step
+    const int nb = 1;
+    hipLaunchKernelGGL(k_zero_words, dim3(nb), dim3(BLOCK), 0, c->stream,
+                       reinterpret_cast<uint32_t*>(c->K.halt), (size_t)1);
+  }
+  ~FusedRun() {
+    c->K.halt = nullptr;
+    c->K.its = 0;
+  }
+};
 
 // keys of the current state -> sort -> permute (all fields) + cell ranges
 // With ghosts: only the set made by arm_ghost_particles since the last sort
@@ -573,9 +622,9 @@ template <typename R> int make_ghosts(arm_ctx* c, bool keys) {
                      c->K, c->G, c->skey, n_old);
   const int tiles = (np + CS_TILE - 1) / CS_TILE;
   hipLaunchKernelGGL(k_cs_tile_sum, dim3(tiles), dim3(CS_THREADS), 0, c->stream, c->G.gflag, np,
-                     c->tsum);
+                     c->tsum, Halt{});
   hipLaunchKernelGGL(k_cs_tile_scan, dim3(tiles), dim3(CS_THREADS), 0, c->stream, c->G.gflag,
-                     np, 0, c->tsum, c->G.grank);
+                     np, 0, c->tsum, c->G.grank, Halt{});
   hipLaunchKernelGGL(k_ghost_make<R>, dim3(grid_of(n_old)), dim3(BLOCK), 0, c->stream, A, c->K,
                      c->G, n_old, c->cap, keys ? 1 : 0, keys ? key_out(c) : KeyOut{}, c->d_err,
                      c->d_ng);
@@ -613,6 +662,7 @@ template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
   const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
   int rc;
   c->its_next = its;
+  c->K.its = its;
   const uint32_t* sval = nullptr;
   bool ranges = false;
   // with ghosts: the new set goes after the live slots; the old set carries
@@ -663,6 +713,7 @@ template <typename R> int time_integration_t(arm_ctx* c, int its0, int nsteps) {
   int rc;
   // ghosts are selected on a sorted state (cell ranges of the last sort)
   if (c->ghosts && (!c->have_ranges || c->ghost_pending) && (rc = link_list_t<R>(c))) return rc;
+  FusedRun run(c, its0);
   if ((rc = begin_steps<R>(c))) return rc;
   if (!(c->prm.flags & ARM_AV_VEL) && (rc = zero_av<R>(c))) return rc;
   const StepHooks none;
@@ -670,7 +721,7 @@ template <typename R> int time_integration_t(arm_ctx* c, int its0, int nsteps) {
     if ((rc = fused_step<R>(c, its, none))) return rc;
   c->have_rates = true;
   HIPCHK(c, hipGetLastError());
-  return check_domain(c);
+  return check_domain(c, its0 + nsteps);
 }
 
 // ---- adjoint rollout (FP64) ---------------------------------------------------
@@ -1416,6 +1467,7 @@ template <typename R> int fused_step3(arm_ctx* c, int its, const StepHooks& hk =
   const bool xs = (c->prm.flags & ARM_AV_VEL) != 0;
   int rc;
   c->its_next = its;
+  c->K.its = its;
   const uint32_t* sval = nullptr;
   bool ranges = false;
   {
@@ -1449,6 +1501,7 @@ template <typename R> int fused_step3(arm_ctx* c, int its, const StepHooks& hk =
 
 template <typename R> int time_integration3_t(arm_ctx* c, int its0, int nsteps) {
   int rc;
+  FusedRun run(c, its0);
   {
     PassTimer tm(c, ARM_PASS_KEYS);
     zero_counts(c);
@@ -1461,7 +1514,7 @@ template <typename R> int time_integration3_t(arm_ctx* c, int its0, int nsteps) 
     if ((rc = fused_step3<R>(c, its))) return rc;
   c->have_rates = true;
   HIPCHK(c, hipGetLastError());
-  return check_domain(c);
+  return check_domain(c, its0 + nsteps);
 }
 
 template <typename R> int pairs3_t(arm_ctx* c, int cap, int* niac, int* pi, int* pj) {
@@ -2071,7 +2124,7 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
       c->h_pin = static_cast<int*>(hp);
     }
   }
-  if (!rc) rc = dalloc(c, &c->d_err, 3);
+  if (!rc) rc = dalloc(c, &c->d_err, 4);
   if (!rc) c->d_ng = c->d_err + 2;
   if (!rc) rc = dalloc(c, &c->d_part, (size_t)grid_of(capacity) * 6);
   if (!rc) rc = dalloc(c, &c->d_cnt, capacity);
@@ -2086,7 +2139,7 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     c->pack3_ap = q;
   }
   if (!rc) {
-    e = hipMemset(c->d_err, 0, 3 * sizeof(int));
+    e = hipMemset(c->d_err, 0, 4 * sizeof(int));
     if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipMemset: %s", hipGetErrorString(e));
   }
   if (!rc) {
@@ -2722,17 +2775,18 @@ extern "C" int arm_link_list_static(const arm_params* prm, int n, const double* 
   if (n > 0)
     hipLaunchKernelGGL(k_keys_xy, dim3(nb), dim3(CS_THREADS), 0, s, x, n, K,
                        KeyOut{nullptr, nullptr, key, count}, out);
-  hipLaunchKernelGGL(k_cs_tile_sum, dim3(tiles), dim3(CS_THREADS), 0, s, count, ncell, tsum);
+  hipLaunchKernelGGL(k_cs_tile_sum, dim3(tiles), dim3(CS_THREADS), 0, s, count, ncell, tsum,
+                     Halt{});
   hipLaunchKernelGGL(k_cs_tile_scan, dim3(tiles), dim3(CS_THREADS), 0, s, count, ncell, n, tsum,
-                     reinterpret_cast<int*>(cell_start));
+                     reinterpret_cast<int*>(cell_start), Halt{});
   if (n > 0) {
     hipLaunchKernelGGL(k_cs_scatter, dim3(nb), dim3(CS_THREADS), 0, s, key,
                        static_cast<const int32_t*>(nullptr), n,
-                       reinterpret_cast<const int*>(cell_start), count, ent, entc);
+                       reinterpret_cast<const int*>(cell_start), count, ent, entc, Halt{});
     hipLaunchKernelGGL(k_cs_rank, dim3(nb), dim3(CS_THREADS), 0, s, ent, entc, n,
                        reinterpret_cast<const int*>(cell_start),
                        reinterpret_cast<uint32_t*>(key_sorted),
-                       reinterpret_cast<uint32_t*>(id_sorted), -1);
+                       reinterpret_cast<uint32_t*>(id_sorted), -1, Halt{});
   }
   return hipGetLastError() == hipSuccess ? ARM_OK : ARM_ERR_HIP;
 }

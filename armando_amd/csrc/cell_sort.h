@@ -77,6 +77,18 @@ __device__ __forceinline__ void emit_key(const KeyOut& ko, bool valid, int t, in
   }
 }
 
+// The fused steps' halt word (see KParams::halt): the sort of step its does
+// nothing once an earlier launch recorded a failure at or before its.
+struct Halt {
+  const int* word = nullptr;
+  int its = 0;
+  __device__ __forceinline__ bool on() const {
+    if (!word) return false;
+    const int h = *word;
+    return h != 0 && its >= h;
+  }
+};
+
 // Clears the per-cell counters (a fill kernel, so that it may be captured).
 __global__ __launch_bounds__(CS_THREADS) void k_cs_zero(int* __restrict__ count, int ncell) {
   for (int c = blockIdx.x * CS_THREADS + threadIdx.x; c < ncell; c += gridDim.x * CS_THREADS)
@@ -141,7 +153,9 @@ __device__ __forceinline__ void store8(int* __restrict__ p, int c0, int n, const
 }
 
 __global__ __launch_bounds__(CS_THREADS) void k_cs_tile_sum(const int* __restrict__ count,
-                                                            int ncell, int* __restrict__ tsum) {
+                                                            int ncell, int* __restrict__ tsum,
+                                                            Halt hl) {
+  if (hl.on()) return;
   __shared__ int wsum[CS_THREADS / 64];
   const int c0 = blockIdx.x * CS_TILE + threadIdx.x * CS_PER_LANE;
   int v[CS_PER_LANE];
@@ -157,7 +171,8 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_tile_sum(const int* __restric
 __global__ __launch_bounds__(CS_THREADS) void k_cs_tile_scan(const int* __restrict__ count,
                                                              int ncell, int n,
                                                              const int* __restrict__ tsum,
-                                                             int* __restrict__ cstart) {
+                                                             int* __restrict__ cstart, Halt hl) {
+  if (hl.on()) return;
   __shared__ int wsum[CS_THREADS / 64];
   const int c0 = blockIdx.x * CS_TILE + threadIdx.x * CS_PER_LANE;
   int v[CS_PER_LANE];
@@ -189,7 +204,8 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_scatter(const uint32_t* __res
                                                            int n, const int* __restrict__ cstart,
                                                            int* __restrict__ count,
                                                            uint64_t* __restrict__ ent,
-                                                           uint32_t* __restrict__ entc) {
+                                                           uint32_t* __restrict__ entc, Halt hl) {
+  if (hl.on()) return;
   const int t = blockIdx.x * CS_THREADS + threadIdx.x;
   const bool valid = t < n;
   const uint32_t c = valid ? key_by_slot[t] : 0xffffffffu;
@@ -210,7 +226,9 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_rank(const uint64_t* __restri
                                                         const uint32_t* __restrict__ entc, int n,
                                                         const int* __restrict__ cstart,
                                                         uint32_t* __restrict__ skey,
-                                                        uint32_t* __restrict__ sval, int skip) {
+                                                        uint32_t* __restrict__ sval, int skip,
+                                                        Halt hl) {
+  if (hl.on()) return;
   const int p = blockIdx.x * CS_THREADS + threadIdx.x;
   if (p >= n) return;
   const uint32_t c = entc[p];

@@ -142,7 +142,10 @@ __global__ __launch_bounds__(BLOCK) void k3_keys(St3<R> S, KParams K, KeyOut ko,
     int ix, iy, iz;
     bool ok;
     key = cell_of3(K, q.x, q.y, q.z, ix, iy, iz, ok);
-    if (!ok) atomicAdd(err, 1);
+    if (!ok) {
+      atomicAdd(err, 1);
+      raise_halt(K);
+    }
     id = S.id[s];
     if constexpr (sizeof(R) == 4) S.pf[s] = split3(K, q);
   }
@@ -189,6 +192,7 @@ __global__ __launch_bounds__(BLOCK) void k3_reorder_pred(St3<R> A, St3<R> Bs, KP
                                                          const uint32_t* __restrict__ sval,
                                                          int* __restrict__ cstart, int predict,
                                                          int fill) {
+  if (halted(K)) return;
   using R4 = typename V4<R>::type;
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   const bool valid = t < K.n;
@@ -428,6 +432,7 @@ template <typename R, bool SUM, bool EOS>
 __global__ __launch_bounds__(BLOCK) void k3_density(St3<R> S, KParams K,
                                                     const uint32_t* __restrict__ skey,
                                                     const int* __restrict__ cstart, Mask3 nbm) {
+  if (halted(K)) return;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
   density3_direct<R, SUM, EOS>(S, K, i, skey, cstart, nbm);
@@ -511,6 +516,7 @@ __device__ __forceinline__ void force3_pair(const KParams& K, ForceOwn3<R>& o, R
 // as k_deposition in 2D.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k3_deposition(St3<R> S, KParams K) {
+  if (halted(K)) return;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n || S.id[i] >= K.np) return;
   const double4 q = S.xyz[i];
@@ -560,6 +566,7 @@ template <typename R> struct Pack3 {
 
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k3_force_pack(St3<R> S, KParams K, Pack3<R> P) {
+  if (halted(K)) return;
   using R4 = typename V4<R>::type;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
@@ -660,6 +667,7 @@ template <typename R, bool PRESS, bool AV, bool AH, bool XS>
 __global__ __launch_bounds__(BLOCK) void k3_force(St3<R> S, Pack3<R> P, KParams K,
                                                   const uint32_t* __restrict__ skey,
                                                   const int* __restrict__ cstart, Mask3 nbm) {
+  if (halted(K)) return;
   // contiguous shares per XCD: the packed records of the neighbour rows
   // stay in one L2 (3.6x less traffic beyond L2, 915 -> 880 us in FP32);
   // the density pass, bound by instruction issue, keeps the plain order,
@@ -709,6 +717,7 @@ __global__ __launch_bounds__(BLOCK) void k3_predictor(St3<R> S, KParams K) {
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k3_corrector(St3<R> S, KParams K, int its, int keys,
                                                       KeyOut ko, int* __restrict__ err) {
+  if (halted(K)) return;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   const bool valid = i < K.n;
   uint32_t key = 0;
@@ -754,7 +763,10 @@ __global__ __launch_bounds__(BLOCK) void k3_corrector(St3<R> S, KParams K, int i
       int ix, iy, iz;
       bool ok;
       key = cell_of3(K, q.x, q.y, q.z, ix, iy, iz, ok);
-      if (!ok) atomicAdd(err, 1);
+      if (!ok) {
+      atomicAdd(err, 1);
+      raise_halt(K);
+    }
     }
   }
   if (keys) emit_key(ko, valid, i, id, key);

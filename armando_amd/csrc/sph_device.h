@@ -57,7 +57,26 @@ struct KParams {
   const double* dep;
   int dep_n[3];
   double dep_o[3], dep_d[3];
+  // stop at the first domain error of a fused run of steps: its is the step
+  // the launch belongs to; *halt (null outside fused runs) is 0, or the
+  // first step that must not run (set by the launch that found a particle
+  // outside the grid, or an h that outgrew the cell)
+  int its;
+  int* halt;
 };
+
+// True in every launch of a step at or after the recorded failure.  The
+// word is written only by earlier launches, so all lanes agree.
+__device__ __forceinline__ bool halted(const KParams& K) {
+  if (!K.halt) return false;
+  const int h = *K.halt;
+  return h != 0 && K.its >= h;
+}
+
+// Records that steps after this launch's step must not run (first failure wins).
+__device__ __forceinline__ void raise_halt(const KParams& K) {
+  if (K.halt) atomicCAS(K.halt, 0, K.its + 1);
+}
 
 // The dispatcher deals the workgroups of a launch to the eight XCDs in turn
 // (workgroup b runs on XCD b mod 8), and each XCD has its own L2.  xcd_block

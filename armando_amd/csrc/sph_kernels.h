@@ -60,7 +60,10 @@ __global__ __launch_bounds__(BLOCK) void k_keys(St<R> S, KParams K, KeyOut ko,
     int ix, iy;
     bool ok;
     key = (uint32_t)cell_of(K, q.x, q.y, ix, iy, ok);
-    if (!ok) atomicAdd(err, 1);
+    if (!ok) {
+      atomicAdd(err, 1);
+      raise_halt(K);
+    }
     id = S.id[s];
     if constexpr (sizeof(R) == 4) {
       const double bx = K.ox + (double)ix * K.cell;
@@ -236,6 +239,7 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
                                                         const uint32_t* __restrict__ sval,
                                                         int* __restrict__ cstart, int predict,
                                                         int fill, Ghosts G) {
+  if (halted(K)) return;
   using R2 = typename V2<R>::type;
   const int t = blockIdx.x * BLOCK + threadIdx.x;
   const bool valid = t < K.n;
@@ -624,6 +628,7 @@ __device__ __forceinline__ void force_end(St<R>& S, const KParams& K, int i, con
 // of its own so that the force gather carries none of its registers.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_deposition(St<R> S, KParams K) {
+  if (halted(K)) return;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n || S.id[i] >= K.np) return;
   const double2 q = S.xy[i];
@@ -728,6 +733,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(St<R> S, KParams K,
                                                    const uint32_t* __restrict__ skey,
                                                    const int* __restrict__ cstart,
                                                    uint4* __restrict__ nbm) {
+  if (halted(K)) return;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
   if (K.sle)
@@ -767,7 +773,10 @@ __device__ __forceinline__ void h_update(St<R>& S, const KParams& K, int i, R dv
       if (hn <= 0.0) hn = hn - K.dt * dh;
     }
   }
-  if (2.0 * hn > K.cell) atomicAdd(err + 1, 1);
+  if (2.0 * hn > K.cell) {
+    atomicAdd(err + 1, 1);
+    raise_halt(K);
+  }
   S.hn[i] = hn;
 }
 
@@ -811,6 +820,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(St<R> S, KParams K,
                                                  const uint32_t* __restrict__ skey,
                                                  const int* __restrict__ cstart,
                                                  int* __restrict__ err) {
+  if (halted(K)) return;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
   if (K.sle)
@@ -907,6 +917,7 @@ template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_tensor(St<R> S, KParams K,
                                                   const uint32_t* __restrict__ skey,
                                                   const int* __restrict__ cstart) {
+  if (halted(K)) return;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= K.n) return;
   if (K.sle)
@@ -956,6 +967,7 @@ __global__ __launch_bounds__(BLOCK) void k_predictor(St<R> S, KParams K) {
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_corrector(St<R> S, KParams K, int its, int keys,
                                                      KeyOut ko, int* __restrict__ err) {
+  if (halted(K)) return;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   const bool valid = i < K.n;
   uint32_t key = 0;
@@ -1007,7 +1019,10 @@ __global__ __launch_bounds__(BLOCK) void k_corrector(St<R> S, KParams K, int its
       int ix, iy;
       bool ok;
       key = (uint32_t)cell_of(K, q.x, q.y, ix, iy, ok);
-      if (!ok) atomicAdd(err, 1);
+      if (!ok) {
+        atomicAdd(err, 1);
+        raise_halt(K);
+      }
     }
   }
   if (keys) emit_key(ko, valid, i, id, key);

@@ -369,6 +369,7 @@ __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
                                                        const uint32_t* __restrict__ skey,
                                                        const int* __restrict__ cstart, int nseg,
                                                        uint4* __restrict__ nbm) {
+  if (halted(K)) return;
   __shared__ DensTile<R, CAP> T;
   __shared__ int cs[3][CW + 3];
   const Segment<CW> g = segment_of<CW>(K, cstart, nseg, cs);
@@ -414,6 +415,7 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
                                                       const uint32_t* __restrict__ skey,
                                                       const int* __restrict__ cstart, int nseg,
                                                       const uint4* __restrict__ nbm) {
+  if (halted(K)) return;
   __shared__ TensTile<R, CAP> T;
   __shared__ int cs[3][CW + 3];
   const Segment<CW> g = segment_of<CW>(K, cstart, nseg, cs);
@@ -459,6 +461,7 @@ __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
                                                      const uint32_t* __restrict__ skey,
                                                      const int* __restrict__ cstart, int nseg,
                                                      const uint4* __restrict__ nbm) {
+  if (halted(K)) return;
   __shared__ ForceTile<R, CAP, VI> T;
   __shared__ int cs[3][CW + 3];
   const Segment<CW> g = segment_of<CW>(K, cstart, nseg, cs);

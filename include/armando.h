@@ -156,7 +156,10 @@ int arm_link_list(arm_ctx* ctx, int* niac);
    parity checks; ARM_ERR_CAPACITY if more than cap. */
 int arm_get_pairs(arm_ctx* ctx, int cap, int* niac, int* pair_i, int* pair_j);
 /* Sorted cell keys, sorted particle ids and cell_start[nx*ny + 1]
-   (cell c holds sorted positions cell_start[c] .. cell_start[c+1]-1). */
+   (cell c holds sorted positions cell_start[c] .. cell_start[c+1]-1).
+   key_sorted and id_sorted receive the live particle count: np + nv, plus
+   with ARM_GHOSTS the ghosts of the current state (arm_ghost_count), whose
+   ids are np + nv + g. */
 int arm_get_cells(arm_ctx* ctx, int32_t* key_sorted, int32_t* id_sorted, int32_t* cell_start);
 
 /* sum_density / con_density [CERNsource/density.f:1,166] (method 1 / 0). */
@@ -183,7 +186,13 @@ int arm_corrector(arm_ctx* ctx, int its);
 /* time_integration [CERNsource/time_integration.f:131-235] for steps
    its0 .. its0+nsteps-1 (refresh_step = 1), fused on the device.  With
    ARM_GHOSTS every step makes its ghosts after the predictor, as
-   time_integration.f:157-161 does. */
+   time_integration.f:157-161 does.  Like link_list.f:63-67's stop, the run
+   ends at the first step whose corrector puts a particle outside the hash
+   grid (or whose force pass grows 2h beyond the cell, sle != 0): the later
+   steps do not run, the call returns ARM_ERR_DOMAIN, arm_last_error names
+   the step, and the state (arm_download) is the one that step left, with
+   the offending particles outside the grid.  Stepping on needs a state
+   inside the grid again (arm_upload). */
 int arm_time_integration(arm_ctx* ctx, int its0, int nsteps);
 
 /* ghost_particles(np, nv, ng, x, v, h, mass, rho, p, niac, pair_i, pair_j,
