@@ -282,8 +282,9 @@ class Simulation:
     def cells(self):
         ncell = int(self.case["nx"]) * int(self.case["ny"]) * \
             (int(self.case["nz"]) if self.dim == 3 else 1)
-        k = np.zeros(self.n, np.int32)
-        s = np.zeros(self.n, np.int32)
+        live = self.n + (self.ghost_count() if self.params.flags & GHOSTS else 0)
+        k = np.zeros(live, np.int32)
+        s = np.zeros(live, np.int32)
         cs = np.zeros(ncell + 1, np.int32)
         self._check(self.L.arm_get_cells(self.ctx, k, s, cs))
         return k, s, cs

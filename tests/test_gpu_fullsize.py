@@ -83,3 +83,55 @@ def test_fp32_form_close_to_oracle_at_full_size(oracle_fields):
         e = rel_l2(s[k], ref[k])
         banner(f"  {k} rel-L2 {e:.2e}")
         assert e < b, k
+
+
+def pair_codes(i, j, n):
+    return i.astype(np.int64) * n + j.astype(np.int64)
+
+
+@pytest.mark.parametrize("after", [0, STEPS])
+def test_cells_and_pairs_bit_exact_at_full_size(after):
+    """Cell keys, the (cell, id) permutation, cell_start and the unique pair
+    sets of the 739,218-particle dam break against the oracle's keys, stable
+    sort and pair search on the same positions: the seeded start, and the
+    positions each form reached after 200 steps (downloaded, so the oracle
+    sees the FP64 accumulators the GPU hashes).  The FP64 form's pairs follow
+    r^2 < (2h)^2 in FP64 (link_list.f:153-159), the FP32 form's the same
+    test on cell + FP32 offset positions; the two sets differ by a handful of
+    pairs at r = 2h, counted here."""
+    sets = {}
+    for prec in (cases.FP64, cases.FP32):
+        case = cases.dam_break_720k(precision=prec)
+        sim = armando_amd.Simulation(case)
+        if after:
+            sim.time_integration(1, after)
+        s = sim.state()
+        niac = sim.link_list(count=True)
+        k, ids, cs = sim.cells()
+        gi, gj = sim.pairs()
+        sim.close()
+        x, y = s["x"], s["y"]
+        P = orc.params_from(case)
+        key, bad = orc.keys(P, x, y)
+        assert bad == 0
+        sid, start = orc.sort_cells(key, int(case["nx"]) * int(case["ny"]))
+        assert np.array_equal(ids, sid)
+        assert np.array_equal(k, key[sid])
+        assert np.array_equal(cs, start)
+        if prec == cases.FP64:
+            oi, oj = orc.pairs_f64(P, x, y)
+        else:
+            cx, cy, ox, oy = orc.split_f32(P, x, y)
+            oi, oj = orc.pairs_f32(P, cx, cy, ox, oy)
+        assert niac == len(oi)
+        assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+        sets[prec] = (pair_codes(oi, oj, case["n"]), x, y)
+        banner(f"  after {after} steps, form {prec}: keys, permutation, cell_start and "
+               f"{niac} pairs bit-exact")
+    if after == 0:  # the same positions: the two criteria side by side
+        a, b = sets[cases.FP64][0], sets[cases.FP32][0]
+        only64 = np.setdiff1d(a, b, assume_unique=True).size
+        only32 = np.setdiff1d(b, a, assume_unique=True).size
+        banner(f"  FP32 vs FP64 criterion at the seeded start: {only64} pairs only in FP64, "
+               f"{only32} only in FP32, of {a.size}")
+        assert only64 + only32 < 1e-5 * a.size

@@ -161,3 +161,30 @@ def test_ghost_rejections():
     with pytest.raises(armando_amd.ArmandoError):
         sim.ghost_particles()  # no search yet
     sim.close()
+
+
+def test_cells_with_ghosts_hold_the_live_set():
+    """arm_get_cells after ghost_particles -> link_list: keys and ids of the
+    np + nv + ng live particles (ghost ids np + nv + g) in (cell, id) order,
+    and the cell ranges, against the oracle's keys and stable sort of the
+    same positions (the ghosts' as downloaded)."""
+    c = tank()
+    banner(f"cells with ghosts n={c['n']}")
+    sim = armando_amd.Simulation(c)
+    sim.link_list()
+    ng = sim.ghost_particles()
+    sim.link_list()
+    assert ng > 0 and sim.ghost_count() == ng
+    k, ids, cs = sim.cells()
+    assert len(k) == c["n"] + ng and len(ids) == c["n"] + ng
+    s, g = sim.state(), sim.ghosts()
+    sim.close()
+    x = np.concatenate([s["x"], g["x"]])
+    y = np.concatenate([s["y"], g["y"]])
+    P = orc.params_from(c)
+    key, bad = orc.keys(P, x, y)
+    assert bad == 0
+    sid, start = orc.sort_cells(key, int(c["nx"]) * int(c["ny"]))
+    assert np.array_equal(ids, sid)
+    assert np.array_equal(k, key[sid])
+    assert np.array_equal(cs, start)
