@@ -455,6 +455,16 @@ template <typename R> int eos_t(arm_ctx* c) {
   return ARM_OK;
 }
 
+// The FP32 tiled force gather of the dam-break flag set (pressure, artificial
+// viscosity and heat, XSPH) with pa_sph and same_e2 fixed at compile time.
+template <int PC> void launch_force_hot(arm_ctx* c, int nseg, const uint4* nbm) {
+  using T = TileCfg<float>;
+  hipLaunchKernelGGL(
+      (k_force_tiled<float, true, true, true, true, false, false, T::BLK, T::CW, T::CAP, PC>),
+      dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, stA<float>(c), c->K, c->skey, c->cstart,
+      nseg, nbm);
+}
+
 template <typename R, bool P, bool AV, bool AH, bool XS, bool VI = false>
 void launch_force(arm_ctx* c) {
   St<R>& A = stA<R>(c);
@@ -462,14 +472,23 @@ void launch_force(arm_ctx* c) {
   if (c->tiled && !c->K.sle) {
     const int nseg = nseg_of<R>(c);
     const uint4* nbm = c->masks_valid ? c->nbm : nullptr;
-    if (c->K.periodic)
+    if (sizeof(R) == 4 && P && AV && AH && XS && !VI && !c->K.periodic &&
+        !(c->K.flags & ARM_CONDUCTION)) {
+      switch ((c->K.pa_sph == 0 ? 1 : 0) | (c->K.same_e2 ? 2 : 0)) {
+        case 0: launch_force_hot<0>(c, nseg, nbm); break;
+        case 1: launch_force_hot<1>(c, nseg, nbm); break;
+        case 2: launch_force_hot<2>(c, nseg, nbm); break;
+        default: launch_force_hot<3>(c, nseg, nbm); break;
+      }
+    } else if (c->K.periodic) {
       hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, true, T::BLK, T::CW, T::CAP>),
                          dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
                          c->cstart, nseg, nbm);
-    else
+    } else {
       hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, false, T::BLK, T::CW, T::CAP>),
                          dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
                          c->cstart, nseg, nbm);
+    }
   } else {
     hipLaunchKernelGGL((k_force<R, P, AV, AH, XS, VI>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
                        c->stream, A, c->K, c->skey, c->cstart, c->d_err);
@@ -1963,6 +1982,10 @@ void set_kparams(KParams& K, const arm_params& P) {
   K.g2h2 = P.ah_g2 * P.h * P.h;
   K.eps_m = P.xsph_eps * P.mass;
   K.same_e2 = (K.av_e2 == K.ah_e2) ? 1 : 0;
+  K.f_cm = -8.0 * K.C1 * P.mass;
+  K.f_ava = -0.5 * P.av_alpha * P.c0;
+  K.f_avb = 0.5 * P.av_beta;
+  K.f_xs = 0.5 * K.wnorm * K.eps_m;
   K.cellf = (float)P.cell;
   K.cut2f = (float)K.cut2;
   K.eos = P.eos;
@@ -1981,6 +2004,7 @@ void set_kparams(KParams& K, const arm_params& P) {
   K.cond = (P.flags & ARM_CONDUCTION)
                ? 4.0 * P.cond_k * P.cond_k / (P.cond_k + P.cond_k) / P.cp
                : 0.0;
+  K.f_cond2 = 2.0 * K.cond;
   K.dim = P.dim == 3 ? 3 : 2;
   K.nz = K.dim == 3 ? P.nz : 1;
   K.oz = P.oz;
@@ -1989,6 +2013,8 @@ void set_kparams(KParams& K, const arm_params& P) {
     K.ncell = P.nx * P.ny * P.nz;
     K.wnorm = 21.0 / (16.0 * M_PI * P.h * P.h * P.h);
     K.C1 = 0.625 * K.wnorm / (P.h * P.h);
+    K.f_cm = -8.0 * K.C1 * P.mass;
+    K.f_xs = 0.5 * K.wnorm * K.eps_m;
   }
 }
 

@@ -30,6 +30,9 @@ struct KParams {
   double g2h2;   // g2 h h
   double eps_m;  // XSPH eps * mass
   int same_e2;   // artificial viscosity and heat share (h eta)^2
+  // FP32 pair body constants: m F = f_cm s^3 with s = 1 - q/2 (f_cm = -8 C1 m);
+  // (beta phi - alpha c0)/2 = f_avb phi + f_ava; XSPH wnorm eps m / 2; 2 cond
+  double f_cm, f_ava, f_avb, f_xs, f_cond2;
   float cellf, cut2f;
   int eos;         // 0 Tait, 1 linear
   double lin_c2;   // linear EOS: p = lin_c2 rho + lin_p0
@@ -109,6 +112,8 @@ __device__ __forceinline__ KParams with_h(const KParams& K, double mh) {
   P.ah_e2 = (mh * K.ah_eta) * (mh * K.ah_eta);
   P.psi0 = K.ah_g1 * mh * K.c0;
   P.g2h2 = K.ah_g2 * mh * mh;
+  P.f_cm = -8.0 * P.C1 * K.mass;
+  P.f_xs = 0.5 * P.wnorm * K.eps_m;
   return P;
 }
 

@@ -480,6 +480,7 @@ template <typename R> struct ForceOwn {
   using R2 = typename V2<R>::type;
   R2 v;
   R Q, A, psi, u;
+  R q4;  // 4 Q (FP32 pair body: 2 / rho_bar = q4 Q_j / (Q + Q_j))
   R wxx = 0, wyy = 0, wxy = 0, tds = 0;  // physical viscosity (tau of i)
   R fx = 0, fy = 0, xsx = 0, xsy = 0;
   Acc<R> du;
@@ -512,6 +513,7 @@ template <typename R, bool VI = false>
 __device__ __forceinline__ void force_begin(const St<R>& S, const KParams& K, int i, ForceOwn<R>& o) {
   o.v = S.v[i];
   q_and_a(K, S.rho[i], S.p[i], o.Q, o.A);
+  o.q4 = (R)4 * o.Q;
   o.psi = S.psi[i];
   if constexpr (sizeof(R) == 4) o.u = S.uf[i]; else o.u = (R)S.u[i];
   if (VI) {
@@ -542,10 +544,78 @@ __device__ __forceinline__ void visc_pair(const KParams& K, ForceOwn<R>& o, R mF
   o.fy = fmr(mF, fmr(syy, dy, sxy * dx), o.fy);
 }
 
-template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI = false>
+// The FP32 form's pair terms: the same quantities with fewer operations.
+//  - m F = f_cm s^3 with s = 1 - q/2, since (q - 2)^3 = -8 s^3; W = wnorm s^4 (1 + 2q).
+//  - imr2 = 2/rho_bar = 4 Q Q_j/(Q + Q_j); the factors 1/2 this doubles go into
+//    the constants f_ava, f_avb (artificial viscosity) and f_xs (XSPH).
+//  - The energy rate is accumulated doubled (G vr plus twice the heat terms)
+//    and halved once in force_end.
+//  - The viscosity term uses phi = min(vr, 0) h/(r^2 + (h eta)^2): zero for
+//    receding pairs, which then add an exact zero (the reference's vr < 0 test).
+//  - r^2 of the arithmetic is one fused multiply-add (the neighbour test, done
+//    before, uses the unfused form).
+// PC < 0 reads pa_sph and same_e2 from K; PC >= 0 fixes them at compile time
+// (bit 0: pa_sph == 0, bit 1: same_e2) with the same expressions, so both give
+// identical bits, and leaves out conduction (its callers run without it).
+template <bool PRESS, bool AV, bool AH, bool XS, bool VI, int PC>
+__device__ __forceinline__ void force_pair_f32(const KParams& K, ForceOwn<float>& o, float dx,
+                                               float dy, float vjx, float vjy, float Qj,
+                                               float Aj, float psij, float uj,
+                                               const Tau3<float>& wj) {
+  const bool pa0 = PC < 0 ? K.pa_sph == 0 : (PC & 1) != 0;
+  const bool se = PC < 0 ? K.same_e2 != 0 : (PC & 2) != 0;
+  const float r2 = __builtin_fmaf(dx, dx, dy * dy);
+  const float q = fsqrt(r2) * (float)K.invh;
+  const float s = __builtin_fmaf(-0.5f, q, 1.0f);
+  const float s2 = s * s;
+  const float mF = (float)K.f_cm * (s2 * s);
+  const float dvx = o.v.x - vjx, dvy = o.v.y - vjy;
+  const float vr = __builtin_fmaf(dvy, dy, dvx * dx);
+  const float imr2 = o.q4 * Qj * frcp(o.Q + Qj);
+  float coef = 0.f;
+  if (PRESS) coef = pa0 ? __builtin_fmaf(o.A, Qj, Aj * o.Q) : __builtin_fmaf(o.A, o.Q, Aj * Qj);
+  float inv_ea = 0.f;
+  if (AV) {
+    inv_ea = frcp(r2 + (float)K.av_e2);
+    const float phi = __builtin_fminf(vr, 0.f) * ((float)K.h * inv_ea);
+    const float pc = __builtin_fmaf((float)K.f_avb, phi, (float)K.f_ava);
+    coef = __builtin_fmaf(pc * phi, imr2, coef);
+  }
+  if (VI) visc_pair(K, o, mF, dx, dy, Qj, wj);
+  if (PRESS || AV) {
+    const float G = mF * coef;
+    o.fx = __builtin_fmaf(-G, dx, o.fx);
+    o.fy = __builtin_fmaf(-G, dy, o.fy);
+    float d = G * vr;
+    float inv_eh = 0.f;
+    if (AH) {
+      inv_eh = (AV && se) ? inv_ea : frcp(r2 + (float)K.ah_e2);
+      d = __builtin_fmaf((o.psi + psij) * mF * (r2 * inv_eh) * imr2, o.u - uj, d);
+    }
+    if (PRESS && PC < 0 && (K.flags & 128)) {  // Cleary-Monaghan conduction
+      if (!AH) inv_eh = frcp(r2 + (float)K.ah_e2);
+      d = __builtin_fmaf((float)K.f_cond2 * o.Q * Qj * mF * (r2 * inv_eh), o.u - uj, d);
+    }
+    o.du.add(d);
+  } else if (AH) {
+    const float inv_eh = frcp(r2 + (float)K.ah_e2);
+    o.du.add((o.psi + psij) * mF * (r2 * inv_eh) * imr2 * (o.u - uj));
+  }
+  if (XS) {
+    const float wf = (float)K.f_xs * ((s2 * s2) * __builtin_fmaf(2.f, q, 1.f)) * imr2;
+    o.xsx = __builtin_fmaf(-wf, dvx, o.xsx);
+    o.xsy = __builtin_fmaf(-wf, dvy, o.xsy);
+  }
+}
+
+template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI = false, int PC = -1>
 __device__ __forceinline__ void force_pair(const KParams& K, ForceOwn<R>& o, R dx, R dy, R r2,
                                            R vjx, R vjy, R Qj, R Aj, R psij, R uj,
                                            const Tau3<R>& wj = Tau3<R>{}) {
+  if constexpr (sizeof(R) == 4) {
+    force_pair_f32<PRESS, AV, AH, XS, VI, PC>(K, o, dx, dy, vjx, vjy, Qj, Aj, psij, uj, wj);
+    return;
+  }
   const R m = (R)K.mass;
   const R q = fsqrt(r2) * (R)K.invh;
   const R tq = q - (R)2;
@@ -606,8 +676,10 @@ __device__ __forceinline__ void force_end(St<R>& S, const KParams& K, int i, con
     acc.x = o.fx + ex;
     acc.y = o.fy + ey;
     S.a[i] = acc;
-    // du = tds/dt + the pair sums (internal_force.f:433-435) [+ heat source]
-    R du = VI ? o.du.get() + o.tds : o.du.get();
+    // du = tds/dt + the pair sums (internal_force.f:433-435) [+ heat source];
+    // the FP32 pair body accumulates the sums doubled
+    const R sums = sizeof(R) == 4 ? (R)0.5 * o.du.get() : o.du.get();
+    R du = VI ? sums + o.tds : sums;
     if ((K.flags & 64) && id < K.np) {
       const double2 q = S.xy[i];
       du = du + (R)(K.hs_q0 * heat_shape(K, q.x, q.y)) * o.Q;

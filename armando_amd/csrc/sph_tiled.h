@@ -141,42 +141,50 @@ template <int CAP> struct TauTile<double, CAP, true> {
 
 template <typename R, int CAP, bool VI = false> struct ForceTile;
 template <int CAP, bool VI> struct ForceTile<float, CAP, VI> {
-  float4 a[CAP + TPAD];  // offx, offy, vx, vy
-  float4 b[CAP + TPAD];  // Q = 1/rho, A = p/rho, psi, u
-  float cx[CAP + TPAD];  // cell column as float
+  float4 a[CAP + TPAD];   // offx, offy, vx, vy
+  float4 b[CAP + TPAD];   // Q = 1/rho, A = p/rho, psi, u
+  float2 cu[CAP + TPAD];  // cell column as float, the cell-row term of the record's row
   TauTile<float, CAP + TPAD, VI> tw;
-  __device__ __forceinline__ void put(int l, const St<float>& S, const KParams& K, int t) {
+  __device__ __forceinline__ void put(int l, const St<float>& S, const KParams& K, int t, int k) {
     const float2 o = S.off[t];
     const float2 v = S.v[t];
     a[l] = make_float4(o.x, o.y, v.x, v.y);
     float Q, A;
     q_and_a(K, S.rho[t], S.p[t], Q, A);
     b[l] = make_float4(Q, A, S.psi[t], S.uf[t]);
-    cx[l] = (float)(S.cxy[t] & 0xffffu);
+    cu[l] = make_float2((float)(S.cxy[t] & 0xffffu), row_uy(K, k));
     tw.put(l, S, t, visc_c(K, Q));
   }
   __device__ __forceinline__ LaneF lane(int l) const {
     const float4 q = a[l];
-    return {cx[l], q.x, q.y};
+    return {cu[l].x, q.x, q.y};
   }
   __device__ __forceinline__ void delta(const KParams& K, const LaneF& p, int l, float uy,
                                         float& dx, float& dy) const {
     const float4 q = a[l];
-    tile_delta(K, p, lds_at(cx, off4(l)), q.x, q.y, uy, dx, dy);
+    tile_delta(K, p, cu[l].x, q.x, q.y, uy, dx, dy);
   }
-  template <bool PRESS, bool AV, bool AH, bool XS>
+  // the same with the row term of the record's own row (lanes are particles
+  // of the segment's row)
+  __device__ __forceinline__ void delta_rec(const KParams& K, const LaneF& p, int l, float& dx,
+                                            float& dy) const {
+    const float4 q = a[l];
+    const float2 c = cu[l];
+    tile_delta(K, p, c.x, q.x, q.y, c.y, dx, dy);
+  }
+  template <bool PRESS, bool AV, bool AH, bool XS, int PC = -1>
   __device__ __forceinline__ void pair(const KParams& K, ForceOwn<float>& o, int l, float dx,
                                        float dy, float r2) const {
     const float4 q = a[l];
     const float4 w = b[l];
-    force_pair<float, PRESS, AV, AH, XS, VI>(K, o, dx, dy, r2, q.z, q.w, w.x, w.y, w.z, w.w,
-                                             tw.get(l));
+    force_pair<float, PRESS, AV, AH, XS, VI, PC>(K, o, dx, dy, r2, q.z, q.w, w.x, w.y, w.z,
+                                                 w.w, tw.get(l));
   }
 };
 template <int CAP, bool VI> struct ForceTile<double, CAP, VI> {
   double2 x[CAP + TPAD], v[CAP + TPAD], rp[CAP + TPAD], pu[CAP + TPAD];  // rp = (Q, A), pu = (psi, u)
   TauTile<double, CAP + TPAD, VI> tw;
-  __device__ __forceinline__ void put(int l, const St<double>& S, const KParams& K, int t) {
+  __device__ __forceinline__ void put(int l, const St<double>& S, const KParams& K, int t, int) {
     x[l] = S.xy[t];
     v[l] = S.v[t];
     double Q, A;
@@ -195,7 +203,11 @@ template <int CAP, bool VI> struct ForceTile<double, CAP, VI> {
     dx = p.x - q.x;
     dy = p.y - q.y;
   }
-  template <bool PRESS, bool AV, bool AH, bool XS>
+  __device__ __forceinline__ void delta_rec(const KParams& K, const PosD& p, int l, double& dx,
+                                            double& dy) const {
+    delta(K, p, l, 0.0, dx, dy);
+  }
+  template <bool PRESS, bool AV, bool AH, bool XS, int PC = -1>
   __device__ __forceinline__ void pair(const KParams& K, ForceOwn<double>& o, int l, double dx,
                                        double dy, double r2) const {
     const double2 vv = v[l];
@@ -352,6 +364,32 @@ __device__ __forceinline__ void walk_masks(const KParams& K, uint32_t m0, uint32
   }
 }
 
+// The same order with less work per neighbour: a lane walks one row mask at
+// a time and takes the next non-empty row only when the current one is
+// exhausted, so the per-neighbour step is a bit scan; the row change runs
+// only in the iterations where some lane of the wavefront needs it.  The
+// visitor gets the tile entry alone (the record carries its row term).
+template <typename Visit>
+__device__ __forceinline__ void walk_rows(uint32_t m0, uint32_t m1, uint32_t m2, int l0, int l1,
+                                          int l2, Visit&& visit) {
+  uint32_t m = m0;
+  int base = l0;
+  while (true) {
+    if (m == 0u) {
+      const bool t1 = m1 != 0u;
+      m = t1 ? m1 : m2;
+      base = t1 ? l1 : l2;
+      m1 = t1 ? m2 : 0u;
+      l1 = l2;
+      m2 = 0u;
+      if (m == 0u) break;
+    }
+    const int b = (int)__builtin_ctz(m);
+    m &= m - 1u;
+    visit(base + b);
+  }
+}
+
 // The same walk driven by masks recorded by the density pass.
 template <typename R, int CW, typename Visit>
 __device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& g,
@@ -455,8 +493,10 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
   }
 }
 
+// PC: pa_sph and same_e2 fixed at compile time for the FP32 pair body
+// (force_pair_f32), -1 to read them from K.
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI, bool PER, int BLK, int CW,
-          int CAP>
+          int CAP, int PC = -1>
 __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
                                                      const uint32_t* __restrict__ skey,
                                                      const int* __restrict__ cstart, int nseg,
@@ -470,7 +510,8 @@ __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
   if (tiled) {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      for (int q = threadIdx.x; q < g.rl[k]; q += BLK) T.put(g.lo[k] + q, S, K, g.rs[k] + q);
+      for (int q = threadIdx.x; q < g.rl[k]; q += BLK)
+        T.put(g.lo[k] + q, S, K, g.rs[k] + q, k);
   }
   __syncthreads();
   for (int i0 = g.O0; i0 < g.O1; i0 += BLK) {
@@ -488,14 +529,26 @@ __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
       R dx, dy;
       T.delta(K, pi, l, uy, dx, dy);
       const R r2 = dx * dx + dy * dy;
-      T.template pair<PRESS, AV, AH, XS>(K, o, l, dx, dy, r2);
+      T.template pair<PRESS, AV, AH, XS, PC>(K, o, l, dx, dy, r2);
     };
     const uint32_t key = skey[i];
     const uint4 nm = nbm ? nbm[i] : make_uint4(0u, 0u, 0u, 0u);
-    if (nm.w)
+    if (nm.w && sizeof(R) == 4) {
+      // FP32: the row-refill walk, the row term from the tile record
+      const int nx = K.nx;
+      const int a0 = ((int)key - g.cy * nx > 0 ? (int)key - g.cy * nx - 1 : 0) - g.slo;
+      walk_rows(nm.x, nm.y, nm.z, g.lo[0] - g.rs[0] + cs[0][a0],
+                g.lo[1] - g.rs[1] + cs[1][a0], g.lo[2] - g.rs[2] + cs[2][a0], [&](int l) {
+                  R dx, dy;
+                  T.delta_rec(K, pi, l, dx, dy);
+                  const R r2 = dx * dx + dy * dy;
+                  T.template pair<PRESS, AV, AH, XS, PC>(K, o, l, dx, dy, r2);
+                });
+    } else if (nm.w) {
       for_masked<R, CW>(K, g, cs, key, nm, visit);
-    else
+    } else {
       for_neighbours<R, CW>(K, g, cs, T, pi, i, key, visit);
+    }
     force_end<R, PRESS, XS, VI>(S, K, i, o);
   }
 }

@@ -29,12 +29,23 @@ The ghost leg runs the 720k dam-break column against one row of virtual
 particles carrying the wall normals, with mirror ghosts rebuilt every step
 (ghost.f), for 500 steps in both forms.
 
+The reference-state leg steps the reference's own 85,773-particle dam-break
+state (ArmandoList/new_{pos,vel,info}.txt, committed as
+tests/golden/damstate85773.npz) for 2000 steps in both forms, beside the
+particle-steps/s of THINS Tab. 1 case 3 (85,773 particles, Tesla C1060).
+
+The oracle leg steps both forms and the oracle (OpenMP on the host cores)
+1000 steps from the same seeded 720k state and reports the errors of every
+field and the conservation sums against the oracle.
+
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp64]
                   [--no-adjoint] [--no-3d] [--no-channel] [--no-ghosts]
+                  [--no-refstate] [--oracle-steps K]
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -45,37 +56,53 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s, 720k 2D dam break, FP32 vs FP64; adjoint/forward time ratio"
 
-# Algorithmic bytes per particle per launch: every particle-indexed array the
-# kernel reads once plus every one it writes once (neighbour re-reads,
+# Algorithmic bytes per particle per launch: every particle-indexed input the
+# pass needs read once plus every output written once (neighbour re-reads,
 # cell-range arrays and uniform constants are not counted).
 BYTES = {
-    # force gather: reads skey, cell, offset, v, rho', p, psi, u, id;
-    # writes a, du, av
-    ("force", 1): 4 + 4 + 8 + 8 + 4 + 4 + 4 + 4 + 4 + (8 + 4 + 8),
-    # FP64: reads skey, xy, v, rho, p, psi, u, id; writes a, du, av
-    ("force", 0): 4 + 16 + 16 + 8 + 8 + 8 + 8 + 4 + (16 + 8 + 16),
-    # density gather: reads skey, pos, v, id; writes drho, divv
-    ("density", 1): 4 + 4 + 8 + 8 + 4 + (4 + 4),
-    ("density", 0): 4 + 16 + 16 + 4 + (8 + 8),
-    # 3D force gather: reads skey, cell + offset (float4), v (float4), rho', p, psi, u,
+    # force pass (internal_force + art_visc + art_heat fused): reads cell 4 +
+    # offset 8 + v 8 + rho 4 + p 4 + u 4 + divv 4 + type 1, writes a 8 + du 4
+    ("force", 1): 37 + 12,
+    # FP64: reads x 16 + v 16 + rho 8 + p 8 + u 8 + divv 8 + type 1, writes a 16 + du 8
+    ("force", 0): 65 + 24,
+    # density + divv pass (FP32 / FP64)
+    ("density", 1): 29,
+    ("density", 0): 49,
+    # 3D force gather: reads key, cell + offset (float4), v (float4), rho', p, psi, u,
     # id; writes a (float4), du, av (float4)
     ("force3", 1): 4 + 16 + 16 + 4 + 4 + 4 + 4 + 4 + (16 + 4 + 16),
     ("force3", 0): 4 + 32 + 32 + 8 + 8 + 8 + 8 + 4 + (32 + 8 + 32),
 }
+# What the FP32 force gather actually stages and writes per particle: key,
+# cell, offset, v, rho', p, psi, u, id read; a, du, av written (the XSPH
+# correction and the art_heat coefficient travel with it).
+STAGED_BYTES = {1: 4 + 4 + 8 + 8 + 4 + 4 + 4 + 4 + 4 + (8 + 4 + 8),
+                0: 4 + 16 + 16 + 8 + 8 + 8 + 8 + 4 + (16 + 8 + 16)}
 HBM_PEAK_GBS = 8000.0
-
-# HBM bytes per launch of the FP32 force gather on the 720k workload, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE scaled by what the same counters
-# read for a plain 256 MiB copy (read = FETCH_SIZE x 2.0, written =
-# WRITE_SIZE x 1.0): 115.1 MB read + 15.1 MB written.  Measured on the
-# committed kernels; profiles/dam720k_pmc.md holds the runs.
-FORCE_TRAFFIC_BYTES = 130.2e6
-# Wavefront VALU instructions per launch of the same kernel (SQ_INSTS_VALU,
-# same runs): with 4 cycles per wave64 instruction on each of the 1024 SIMDs
-# at 2.4 GHz they give the fraction of the launch the VALUs issue, the bound
-# of this gather.
-FORCE_VALU_INSTS = 2.92e7
 SIMDS, CLOCK_HZ = 1024, 2.4e9
+PROFILE_PMC = os.path.join(ROOT, "profiles", "dam720k_pmc.md")
+
+
+def profile_figures(kernel_prefix, path=PROFILE_PMC):
+    """Per-launch figures of a kernel from the committed counter profile: the
+    ```bench-figures``` block of JSON lines {"kernel", "traffic_bytes",
+    "valu_insts", "commit", ...} that the profile records.  None when the
+    file or the kernel is missing."""
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"```bench-figures\n(.*?)```", text, re.S)
+    if not m:
+        return None
+    for line in m.group(1).splitlines():
+        line = line.strip()
+        if not line:
+            continue
+        rec = json.loads(line)
+        if rec.get("kernel", "").startswith(kernel_prefix):
+            return rec
+    return None
 
 
 def parse():
@@ -98,10 +125,14 @@ def parse():
                     help="approximate CPU work for the baseline sample")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="dam-break size factor (1.0 = the 720k workload)")
-    ap.add_argument("--oracle-steps", type=int, default=0,
-                    help="also step both forms and the oracle (OpenMP, host cores) this many "
-                         "steps from the initial state and report the errors against the "
-                         "oracle (1000 steps: about 45 s of CPU work on 16 threads)")
+    ap.add_argument("--oracle-steps", type=int, default=1000,
+                    help="step both forms and the oracle (OpenMP, host cores) this many steps "
+                         "from the initial state and report the errors and conservation sums "
+                         "against the oracle (1000 steps: about 45 s of CPU work on 16 "
+                         "threads; 0 skips)")
+    ap.add_argument("--no-refstate", action="store_true",
+                    help="skip the leg on the reference's 85,773-particle state")
+    ap.add_argument("--steps-refstate", type=int, default=2000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB three times (a kernel of "
                          "known traffic for scaling the memory counters of a profiler)")
@@ -143,10 +174,8 @@ def run_form(precision, args, case_fn):
     passes = sim.pass_times()
     sim.set_timing(False)
     state = sim.state()
-    cons = sim.conservation()
     sim.close()
-    return dict(case=case, niac=niac, wall=wall, passes=passes, timed_steps=nt, state=state,
-                conservation=cons)
+    return dict(case=case, niac=niac, wall=wall, passes=passes, timed_steps=nt, state=state)
 
 
 def adjoint_ratio(steps=400, ckpt=20):
@@ -397,12 +426,32 @@ def oracle_leg(steps, case_fn):
     t_orc = time.perf_counter() - t0
     names = ("rho", "p", "vx", "vy", "u", "x", "y")
     ref = {k: st.field(k).copy() for k in names}
-    out = {"after_steps": steps, "oracle_threads": threads, "oracle_seconds": t_orc}
+    st0 = orc.OracleState(case["n"], case["np"])
+    st0.set(**{k: case[k] for k in ("x", "y", "vx", "vy", "rho", "u")})
+    cons_keys = ("mass", "momentum_x", "momentum_y", "kinetic", "internal")
+    out = {"after_steps": steps, "oracle_threads": threads, "oracle_seconds": t_orc,
+           "conservation": {"sums": list(cons_keys),
+                            "oracle": {"start": orc.conservation(P, st0).tolist(),
+                                       "end": orc.conservation(P, st).tolist()}}}
     for tag, prec in (("fp64", cases.FP64), ("fp32", cases.FP32)):
         sim = armando_amd.Simulation(case_fn(prec))
+        c0 = sim.conservation()
         sim.time_integration(1, steps)
         s = sim.state()
+        c1 = sim.conservation()
         sim.close()
+        ref_end = orc.conservation(P, st)
+        out["conservation"][tag] = {
+            "start": c0.tolist(), "end": c1.tolist(),
+            # against the oracle's sums after the same steps: the mass exactly,
+            # momenta on the flow's scale sqrt(2 M KE), energies relative
+            "vs_oracle": {
+                "mass_rel": float(abs(c1[0] - ref_end[0]) / ref_end[0]),
+                "momentum_rel": float(max(abs(c1[1] - ref_end[1]), abs(c1[2] - ref_end[2]))
+                                      / np.sqrt(2.0 * ref_end[0] * max(ref_end[3], 1e-300))),
+                "kinetic_rel": float(abs(c1[3] - ref_end[3]) / max(abs(ref_end[3]), 1e-300)),
+                "internal_rel": float(abs(c1[4] - ref_end[4]) / max(abs(ref_end[4]), 1e-300)),
+            }}
         acc = {}
         for k in names:
             d = s[k] - ref[k]
@@ -412,6 +461,49 @@ def oracle_leg(steps, case_fn):
                       "max_rel": float(np.max(np.abs(d))) / max(scale, 1e-300)}
         out[tag] = acc
     return out
+
+
+def refstate_leg(steps, warmup=5):
+    """The reference's 85,773-particle dam-break state (outputData of
+    armando2Dcpu.cu, committed as tests/golden/damstate85773.npz) stepped by
+    both forms; THINS Tab. 1 case 3 (85,773 particles) ran 40,000 steps in
+    8 m 34 s on a Tesla C1060 and 784 m on one 2.67 GHz Xeon core."""
+    import armando_amd
+    from armando_amd import cases
+    from armando_amd import io as aio
+
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "damstate85773.npz"))
+    pos, vel = fx["pos"].astype(np.float64), fx["vel"].astype(np.float64)
+    mass, h = (float(v) for v in fx["mass_h"].astype(np.float64))
+    n = len(pos)
+    st = {"x": pos[:, 0], "y": pos[:, 1], "vx": vel[:, 0], "vy": vel[:, 1],
+          "mat": fx["mat"].astype(np.int64), "mass": np.full(n, mass), "h": np.full(n, h)}
+    st, npf = aio.walls_last(st)
+    res = {}
+    for name, prec in (("fp32", cases.FP32), ("fp64", cases.FP64)):
+        case = cases.c_path_dam_state(st, npf, precision=prec)
+        sim = armando_amd.Simulation(case)
+        sim.time_integration(1, warmup)
+        t0 = time.perf_counter()
+        sim.time_integration(1 + warmup, steps)
+        res[name] = time.perf_counter() - t0
+        sim.close()
+    thins_gpu = 85773 * 40000 / (8 * 60 + 34)
+    thins_cpu = 85773 * 40000 / (784 * 60)
+    v32 = n * steps / res["fp32"]
+    return {
+        "workload": (f"the reference's dam-break state, {npf} fluid + {n - npf} wall particles "
+                     f"(ArmandoList/new_*.txt), grid {case['nx']}x{case['ny']}"),
+        "value": v32, "unit": "particle-steps/s", "dtype": "f32", "steps": steps,
+        "ms_per_step": 1e3 * res["fp32"] / steps,
+        "fp64": {"value": n * steps / res["fp64"], "ms_per_step": 1e3 * res["fp64"] / steps},
+        "thins_tab1_case3": {
+            "gpu_c1060_particle_steps_per_s": thins_gpu,
+            "cpu_1core_particle_steps_per_s": thins_cpu,
+            "note": "the reference step is TVD-RK3 (three force evaluations); one leapfrog "
+                    "step here is one",
+            "ratio_fp32_vs_c1060": v32 / thins_gpu},
+    }
 
 
 def cpu_baseline(case, seconds):
@@ -484,6 +576,7 @@ def main():
     t_force = pass_ms(r32, "force") * 1e-3
     bytes_force = n * BYTES[("force", 1)]
     achieved = bytes_force / t_force / 1e9
+    prof = profile_figures("arm::k_force_tiled<float") if args.scale == 1.0 else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -516,11 +609,15 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": FORCE_TRAFFIC_BYTES if args.scale == 1.0 else None,
+            "traffic": prof["traffic_bytes"] if prof else None,
             "bytes_per_particle": BYTES[("force", 1)],
             "avg_launch_ms": t_force * 1e3,
-            "valu_issue_frac": (FORCE_VALU_INSTS * 4 / SIMDS / (t_force * CLOCK_HZ)
-                                if args.scale == 1.0 else None),
+            "staged_bytes_per_particle": STAGED_BYTES[1],
+            "staged_frac": n * STAGED_BYTES[1] / t_force / 1e9 / HBM_PEAK_GBS,
+            "valu_issue_frac": (prof["valu_insts"] * 4 / SIMDS / (t_force * CLOCK_HZ)
+                                if prof else None),
+            "counters_source": (f"profiles/dam720k_pmc.md, kernel {prof['kernel']}, "
+                                f"commit {prof['commit']}" if prof else None),
         },
     }
     if r64 is not None:
@@ -549,6 +646,8 @@ def main():
         out["heated_channel"] = channel_leg(args.steps_channel)
     if not args.no_ghosts:
         out["ghost_walls"] = ghost_leg(args.steps_ghosts)
+    if not args.no_refstate:
+        out["reference_state"] = refstate_leg(args.steps_refstate)
     if args.oracle_steps > 0:
         out["accuracy_vs_oracle"] = oracle_leg(args.oracle_steps, case_fn)
     if not args.no_cpu:
