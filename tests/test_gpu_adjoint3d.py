@@ -120,3 +120,59 @@ def test_adjoint3_leaves_the_final_state():
         assert np.array_equal(sa[k], sb[k]), k
     a.close()
     b.close()
+
+
+def full_size_case3(precision, seed=4243, vstd=0.01):
+    """The 3D dam break of the bench at full size (2,229,862 particles) with
+    initial velocities N(0, vstd) per component on the fluid (seeded) and the
+    objective on the left-wall particles below 0.48 m."""
+    c = cases.dam_break3d(precision=precision)
+    rng = np.random.default_rng(seed)
+    nf = c["np"]
+    for k in ("vx", "vy", "vz"):
+        c[k] = c[k].copy()
+        c[k][:nf] = rng.normal(0.0, vstd, nf)
+    obj = np.nonzero((c["x"] < 0) & (c["z"] < 0.48))[0].astype(np.int32)
+    return c, obj
+
+
+def objective3(c, obj, nsteps, ckpt):
+    sim = armando_amd.Simulation(c)
+    J = sim.rollout_forward(nsteps, ckpt, obj)[0]
+    sim.close()
+    return J
+
+
+def test_fp32_adjoint3_full_size_against_fp64_differences():
+    """The full-size 3D rollout (500 steps, checkpoint every 20) in the FP32
+    chosen form: dJ/dalpha and dJ/dv0 . d on four seeded directions against
+    central differences of the FP64 forward with eps 1e-4 (relative for
+    alpha, m/s for v0); bound 1e-3."""
+    steps, ck = 500, 20
+    c32, obj = full_size_case3(cases.FP32)
+    c64, _ = full_size_case3(cases.FP64)
+    banner(f"3D full-size adjoint n={c32['n']} steps={steps} ckpt={ck}")
+    J32, da32, g32, (tf, tb) = rollout(c32, obj, steps, ck)
+    banner(f"  fp32 J {J32:.6e}, forward {tf:.0f} ms, backward {tb:.0f} ms")
+    assert J32 > 0 and tb <= 3.0 * tf
+    a0 = c64["av_alpha"]
+    e = 1e-4 * a0
+    ca, cb = dict(c64), dict(c64)
+    ca["av_alpha"], cb["av_alpha"] = a0 + e, a0 - e
+    fda = (objective3(ca, obj, steps, ck) - objective3(cb, obj, steps, ck)) / (2 * e)
+    banner(f"  d/dalpha fd {fda:.8e} fp32 adjoint {da32:.8e}")
+    assert abs(fda - da32) <= 1e-3 * abs(fda), (fda, da32)
+    nf = c64["np"]
+    ev = 1e-4
+    for seed in (2001, 2002, 2003, 2004):
+        z = cases.normal01(seed, 3 * nf)
+        d = np.zeros((c64["n"], 3))
+        d[:nf, 0], d[:nf, 1], d[:nf, 2] = z[:nf], z[nf:2 * nf], z[2 * nf:]
+        cp, cm = dict(c64), dict(c64)
+        for a, k in enumerate(("vx", "vy", "vz")):
+            cp[k] = c64[k] + ev * d[:, a]
+            cm[k] = c64[k] - ev * d[:, a]
+        fdv = (objective3(cp, obj, steps, ck) - objective3(cm, obj, steps, ck)) / (2 * ev)
+        ad = float(np.sum(g32 * d))
+        banner(f"  direction {seed}: fd {fdv:.8e} fp32 adjoint {ad:.8e}")
+        assert abs(fdv - ad) <= 1e-3 * abs(fdv), (seed, fdv, ad)

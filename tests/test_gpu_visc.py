@@ -70,11 +70,15 @@ def test_routines_with_viscosity_fp64(pa, eos):
     sim.close()
 
 
-def test_shear_cavity_3000_steps_fp64():
+@pytest.mark.parametrize("p0", [-0.1, 0.0])
+def test_shear_cavity_3000_steps_fp64(p0):
     """The lid-driven cavity (1681 fluid + 320 wall particles, 3000 steps of
-    5e-5 s) against the oracle's gather twin."""
-    c = cases.shear_cavity()
-    banner(f"shear cavity n={c['n']} 3000 steps")
+    5e-5 s) against the oracle's gather twin, with the linear EOS both as the
+    reference's eos_poly gives it, p = c^2 (rho - rho0) (p0 = -0.1 Pa), and
+    as the cavity configuration states it, p = c^2 rho (p0 = 0: with pa_sph 1
+    the background pressure c^2 rho0 enters the force)."""
+    c = cases.shear_cavity(eos_p0=p0)
+    banner(f"shear cavity n={c['n']} 3000 steps, eos_p0 {p0}")
     sim = armando_amd.Simulation(c)
     sim.time_integration(1, 3000)
     s = sim.state()
@@ -243,3 +247,37 @@ def test_torch_autograd_viscosity_input():
     assert alpha.grad.item() == pytest.approx(ref[1][0], rel=1e-12)
     assert np.allclose(v0.grad.numpy(), ref[2], rtol=1e-12, atol=0)
     loss.close()
+
+
+def test_viscous_adjoint_100k_400_steps():
+    """Config 3's third gradient at its size: the 100k dam break with the
+    physical viscosity of water (1e-3 Pa s), 400 steps, checkpoint every 20.
+    dJ/d eta of the FP64 and the FP32 adjoint against FP64 central
+    differences (eps 1e-4 relative), bound 1e-3; dJ/d alpha beside it."""
+    c, obj = cases.dam_break_adjoint_100k()
+    c["flags"] = c["flags"] | cases.VISC
+    c["visc_eta"] = 1.0e-3
+    c32, _ = cases.dam_break_adjoint_100k(precision=cases.FP32)
+    c32["flags"] = c["flags"]
+    c32["visc_eta"] = c["visc_eta"]
+    steps, ck = 400, 20
+    banner(f"viscous adjoint 100k n={c['n']} steps={steps} ckpt={ck}")
+    J, (da, _, de), _ = visc_rollout(c, obj, steps, ck)
+    J32, (da32, _, de32), _ = visc_rollout(c32, obj, steps, ck)
+    eta = c["visc_eta"]
+    e = 1e-4 * eta
+    fde = (visc_rollout(c, obj, steps, ck, eta=eta + e, backward=False)[0]
+           - visc_rollout(c, obj, steps, ck, eta=eta - e, backward=False)[0]) / (2 * e)
+    banner(f"  d/deta fd {fde:.8e} fp64 {de:.8e} fp32 {de32:.8e}")
+    assert fde != 0.0
+    assert abs(fde - de) <= 1e-3 * abs(fde), (fde, de)
+    assert abs(fde - de32) <= 1e-3 * abs(fde), (fde, de32)
+    a0 = c["av_alpha"]
+    ea = 1e-4 * a0
+    ca, cb = dict(c), dict(c)
+    ca["av_alpha"], cb["av_alpha"] = a0 + ea, a0 - ea
+    fda = (visc_rollout(ca, obj, steps, ck, backward=False)[0]
+           - visc_rollout(cb, obj, steps, ck, backward=False)[0]) / (2 * ea)
+    banner(f"  d/dalpha fd {fda:.8e} fp64 {da:.8e} fp32 {da32:.8e}")
+    assert abs(fda - da) <= 1e-3 * abs(fda), (fda, da)
+    assert abs(fda - da32) <= 1e-3 * abs(fda), (fda, da32)
