@@ -23,7 +23,7 @@ ARM_OK = 0
 ERRORS = {-1: "ARM_ERR_ARG", -2: "ARM_ERR_HIP", -3: "ARM_ERR_CAPACITY", -4: "ARM_ERR_DOMAIN",
           -5: "ARM_ERR_UNSUPPORTED", -6: "ARM_ERR_STATE"}
 ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
-CONDUCTION, PERIODIC_X, GHOSTS = 128, 256, 512
+CONDUCTION, PERIODIC_X, GHOSTS, CONTACT = 128, 256, 512, 1024
 EOS_TAIT, EOS_LINEAR = 0, 1
 FP64, FP32 = 0, 1
 PASSES = ("keys", "sort", "reorder", "density", "eos", "force", "integrate")
@@ -57,6 +57,7 @@ class ArmParams(C.Structure):
         ("hs_sigma", C.c_double),
         ("dim", C.c_int), ("oz", C.c_double), ("gz", C.c_double), ("nz", C.c_int),
         ("cond_k", C.c_double), ("cp", C.c_double), ("sle", C.c_int),
+        ("contact_hdr", C.c_double),
     ]
 
 
@@ -64,7 +65,7 @@ class ArmParams(C.Structure):
 # Tait EOS)
 PARAM_DEFAULTS = {"visc_eta": 0.0, "eos": 0, "eos_p0": 0.0, "hs_q0": 0.0, "hs_x": 0.0,
                   "hs_y": 0.0, "hs_sigma": 0.0, "dim": 2, "oz": 0.0, "gz": 0.0, "nz": 1,
-                  "cond_k": 0.0, "cp": 0.0, "sle": 0}
+                  "cond_k": 0.0, "cp": 0.0, "sle": 0, "contact_hdr": 0.0}
 OBJ_PRESSURE, OBJ_ENERGY = 0, 1
 
 

@@ -17,7 +17,7 @@ import numpy as np
 
 # flags (include/armando.h)
 ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
-CONDUCTION, PERIODIC_X, GHOSTS = 128, 256, 512
+CONDUCTION, PERIODIC_X, GHOSTS, CONTACT = 128, 256, 512, 1024
 EOS_TAIT, EOS_LINEAR = 0, 1
 FP64, FP32 = 0, 1
 
@@ -222,6 +222,25 @@ def ghost_tank(col_w=0.03, col_h=0.06, tank_w=0.16, dx=1e-3, jitter=0.01, seed=1
         xsph_eps=0.5, gx=0.0, gy=-g, dt=cfl * h / c0,
         ox=ox, oy=oy, cell=cell, nx=nxg, ny=nyg,
     )
+
+
+def contact_tank(col_w=0.02, col_h=0.03, tank_w=0.05, dx=1e-3, gap=1.0, precision=FP64, **kw):
+    """The ghost tank's column and walls with the contact displacement of
+    FORTRANsource/external_force.f90 as the wall model: the fluid lattice
+    starts `gap` dx from the bottom and left walls, inside the contact
+    distance h / hdr (hdr = 1.2 of FORTRANsource/options.inc), so the walls
+    push from the first step; the virtual particles carry their inward unit
+    normals in v and take part in the pair sums as in the ghost tank."""
+    c = ghost_tank(col_w=col_w, col_h=col_h, tank_w=tank_w, dx=dx, precision=precision, **kw)
+    nf = c["np"]
+    c["x"] = c["x"].copy()
+    c["y"] = c["y"].copy()
+    c["x"][:nf] += (gap - 0.5) * dx
+    c["y"][:nf] += (gap - 0.5) * dx
+    c["flags"] = (c["flags"] & ~GHOSTS) | CONTACT
+    c["contact_hdr"] = 1.2
+    c["name"] = "contact_tank"
+    return c
 
 
 def box_lattice(nx=20, ny=15, dx=0.01, jitter=0.05, seed=7, vel=0.0, precision=FP64, h_fac=1.3):
@@ -495,4 +514,5 @@ def dam_break3d_small(scale=0.1, dx=7.1e-3, precision=FP64, seed=4242, jitter=0.
 PARAM_KEYS = ("precision", "flags", "pa_sph", "density_method", "h", "mass", "rho0", "c0",
               "pmin", "av_alpha", "av_beta", "av_eta", "ah_g1", "ah_g2", "ah_eta", "xsph_eps",
               "gx", "gy", "dt", "ox", "oy", "cell", "nx", "ny", "visc_eta", "eos", "eos_p0",
-              "hs_q0", "hs_x", "hs_y", "hs_sigma", "dim", "oz", "gz", "nz", "cond_k", "cp")
+              "hs_q0", "hs_x", "hs_y", "hs_sigma", "dim", "oz", "gz", "nz", "cond_k", "cp",
+              "sle", "contact_hdr")

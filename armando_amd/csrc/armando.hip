@@ -81,6 +81,7 @@ struct arm_ctx {
   uint32_t* nbm3 = nullptr;  // 3D neighbour masks (M3_TOTAL words per particle)
   void* pack3_vq = nullptr;  // 3D force pass: packed neighbour data (Pack3)
   void* pack3_ap = nullptr;
+  void* d_cd = nullptr;      // ARM_CONTACT: contact displacement per slot (V2<R>)
   // sort workspace
   uint32_t *key_by_id = nullptr, *slot_by_id = nullptr;
   uint32_t *sk[2] = {nullptr, nullptr}, *sv[2] = {nullptr, nullptr};
@@ -530,6 +531,8 @@ template <typename R> int force_t(arm_ctx* c, bool press, int flags, bool xs) {
   const bool av = (flags & ARM_ART_VISC) != 0, ah = (flags & ARM_ART_HEAT) != 0;
   dep_window(c);
   const bool vi = press && (flags & ARM_VISC) != 0;
+  if (press && (flags & ARM_CONTACT) && !c->d_cd)
+    return fail(c, ARM_ERR_ARG, "ARM_CONTACT: the context was created without it");
   PassTimer tm(c, ARM_PASS_FORCE);
   if (vi) launch_tensor<R>(c);
   const int code = (vi ? 16 : 0) | (press ? 8 : 0) | (av ? 4 : 0) | (ah ? 2 : 0) | (xs ? 1 : 0);
@@ -556,6 +559,16 @@ template <typename R> int force_t(arm_ctx* c, bool press, int flags, bool xs) {
   if (press && (c->K.flags & KF_DEPOSITION))
     hipLaunchKernelGGL(k_deposition<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
                        stA<R>(c), c->K);
+  if (press && (flags & ARM_CONTACT)) {
+    // time_integration.f90:128-142: the push to the contact distance from
+    // the virtual neighbours, smoothed over the real ones, as an acceleration
+    using V = typename V2<R>::type;
+    V* cd = static_cast<V*>(c->d_cd);
+    hipLaunchKernelGGL(k_contact_dx<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                       stA<R>(c), c->K, c->skey, c->cstart, cd);
+    hipLaunchKernelGGL(k_contact_apply<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream,
+                       stA<R>(c), c->K, c->skey, c->cstart, static_cast<const V*>(cd));
+  }
   HIPCHK(c, hipGetLastError());
   return ARM_OK;
 }
@@ -2005,6 +2018,7 @@ void set_kparams(KParams& K, const arm_params& P) {
                ? 4.0 * P.cond_k * P.cond_k / (P.cond_k + P.cond_k) / P.cp
                : 0.0;
   K.f_cond2 = 2.0 * K.cond;
+  K.contact_h = (P.flags & ARM_CONTACT) ? P.h / P.contact_hdr : 0.0;
   K.dim = P.dim == 3 ? 3 : 2;
   K.nz = K.dim == 3 ? P.nz : 1;
   K.oz = P.oz;
@@ -2078,6 +2092,12 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
            (P.dim == 3 || (P.flags & (ARM_DBC | ARM_PERIODIC_X)) || P.sle))
     rc = fail(c, ARM_ERR_UNSUPPORTED,
               "ARM_GHOSTS is 2D, without ARM_DBC, ARM_PERIODIC_X or sle");
+  else if ((P.flags & ARM_CONTACT) &&
+           (P.dim == 3 || (P.flags & (ARM_DBC | ARM_GHOSTS | ARM_PERIODIC_X)) || P.sle))
+    rc = fail(c, ARM_ERR_UNSUPPORTED,
+              "ARM_CONTACT is 2D, without ARM_DBC, ARM_GHOSTS, ARM_PERIODIC_X or sle");
+  else if ((P.flags & ARM_CONTACT) && !(P.contact_hdr > 0))
+    rc = fail(c, ARM_ERR_ARG, "ARM_CONTACT needs contact_hdr > 0");
   if (rc) {
     *out = c;  // so that arm_last_error can report
     return rc;
@@ -2149,6 +2169,11 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
       if (e != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
       c->h_pin = static_cast<int*>(hp);
     }
+  }
+  if (!rc && (P.flags & ARM_CONTACT)) {
+    double2* q = nullptr;
+    rc = dalloc(c, &q, capacity);
+    c->d_cd = q;
   }
   if (!rc) rc = dalloc(c, &c->d_err, 4);
   if (!rc) c->d_ng = c->d_err + 2;
@@ -2560,6 +2585,8 @@ int arm_rollout_forward(arm_ctx* c, int nsteps, int ckpt_every, const int32_t* o
   if (c->ghosts) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint has no mirror ghosts");
   if (c->d_dep) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint has no deposition heat source");
   if (c->prm.sle) return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint keeps h fixed (sle = 0)");
+  if (c->prm.flags & ARM_CONTACT)
+    return fail(c, ARM_ERR_UNSUPPORTED, "the adjoint has no contact displacement");
   const int ck = std::min(ckpt_every, nsteps);
   int rc;
   if (is3d(c))

@@ -66,6 +66,7 @@ struct KParams {
   // outside the grid, or an h that outgrew the cell)
   int its;
   int* halt;
+  double contact_h;  // flag 1024: the contact distance h / hdr
 };
 
 // True in every launch of a step at or after the recorded failure.  The

@@ -709,6 +709,94 @@ __global__ __launch_bounds__(BLOCK) void k_deposition(St<R> S, KParams K) {
   S.du[i] = S.du[i] + (R)deposition_at(K, q.x, q.y, 0.0) * Q;
 }
 
+// ------------------------------------------------------------------------
+// contact displacement (flag 1024) of FORTRANsource/external_force.f90,
+// after the force pass of a step (time_integration.f90:128-142)
+// ------------------------------------------------------------------------
+// contact_displacement (external_force.f90:191-244): real particle i walks
+// its virtual neighbours (unit normal n_j in v) in gather order and pushes
+// its displacement d to the contact distance: pni = h/hdr - (x_i - x_j).n_j,
+// pnm = d.n_j, d += (pni - pnm) n_j when pni > 0 and pni > pnm.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_contact_dx(St<R> S, KParams K,
+                                                      const uint32_t* __restrict__ skey,
+                                                      const int* __restrict__ cstart,
+                                                      typename V2<R>::type* __restrict__ cd) {
+  if (halted(K)) return;
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= K.n) return;
+  typename V2<R>::type d;
+  d.x = (R)0;
+  d.y = (R)0;
+  if (S.id[i] < K.np) {
+    const auto pi = load_pos(S, i);
+    const R cut2 = cutoff2<R>(K);
+    const R hh = (R)K.contact_h;
+    for_candidates<false>(K, skey[i], cstart, [&](int t, int sh, int, int) {
+      const int jd = S.id[t];
+      if (jd < K.np || jd >= K.nb) return;
+      const auto pj = load_pos(S, t);
+      R dx, dy;
+      delta(pi, pj, K, dx, dy, sh);
+      const R r2 = dx * dx + dy * dy;
+      if (!(r2 < cut2)) return;
+      const auto n = S.v[t];
+      R pni = (R)0, pnm = (R)0;
+      pni = pni + dx * n.x;
+      pnm = pnm + d.x * n.x;
+      pni = pni + dy * n.y;
+      pnm = pnm + d.y * n.y;
+      pni = hh - pni;
+      if (pni > (R)0 && pni > pnm) {
+        d.x = d.x + (pni - pnm) * n.x;
+        d.y = d.y + (pni - pnm) * n.y;
+      }
+    });
+  }
+  cd[i] = d;
+}
+
+// smooth_vector (external_force.f90:247-339) and the acceleration of
+// time_integration.f90:138-142: a_i += 0.5 (W(0) m/rho_i d_i
+// + sum over real neighbours W_ij m/rho_i d_j) / dt^2.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_contact_apply(St<R> S, KParams K,
+                                                         const uint32_t* __restrict__ skey,
+                                                         const int* __restrict__ cstart,
+                                                         const typename V2<R>::type* __restrict__ cd) {
+  if (halted(K)) return;
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= K.n || S.id[i] >= K.np) return;
+  const auto pi = load_pos(S, i);
+  const R cut2 = cutoff2<R>(K);
+  const R m = (R)K.mass;
+  const R rhoi = dens(K, S.rho[i]);
+  const R w0 = (R)K.wnorm;
+  const auto di = cd[i];
+  R sx = w0 * m / rhoi * di.x;
+  R sy = w0 * m / rhoi * di.y;
+  for_candidates<false>(K, skey[i], cstart, [&](int t, int sh, int, int) {
+    if (t == i || S.id[t] >= K.np) return;
+    const auto pj = load_pos(S, t);
+    R dx, dy;
+    delta(pi, pj, K, dx, dy, sh);
+    const R r2 = dx * dx + dy * dy;
+    if (!(r2 < cut2)) return;
+    const R q = fsqrt(r2) * (R)K.invh;
+    const R s = (R)1 - (R)0.5 * q;
+    const R s2 = s * s;
+    const R w = (R)K.wnorm * (s2 * s2) * ((R)1 + (R)2 * q);
+    const auto dj = cd[t];
+    sx = sx + w * m / rhoi * dj.x;
+    sy = sy + w * m / rhoi * dj.y;
+  });
+  const R dt2 = (R)(K.dt * K.dt);
+  auto a = S.a[i];
+  a.x = a.x + (R)0.5 * sx / dt2;
+  a.y = a.y + (R)0.5 * sy / dt2;
+  S.a[i] = a;
+}
+
 template <typename R> __device__ __forceinline__ R u_of(const St<R>& S, int t) {
   if constexpr (sizeof(R) == 4) return S.uf[t]; else return (R)S.u[t];
 }

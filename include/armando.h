@@ -68,6 +68,15 @@ extern "C" {
                               cells wrap and positions re-enter on the other side
                               (nx >= 3; the reference's wrap is commented out,
                               ArmandoList/armando2Dlist.cu:856-870) */
+#define ARM_CONTACT 1024   /* contact displacement of the other tree's boundary
+                              [FORTRANsource/external_force.f90:191-339, applied in
+                              time_integration.f90:128-142]: the virtual particles
+                              carry the unit wall normal in v; after the forces each
+                              real particle gains a += 0.5 S(d)/dt^2, d the push to
+                              the contact distance h/contact_hdr from its virtual
+                              neighbours, S the kernel smoothing over its real
+                              neighbours (2D, without ARM_DBC, ARM_GHOSTS,
+                              ARM_PERIODIC_X or sle; no adjoint) */
 #define ARM_GHOSTS 512     /* mirror ghosts [CERNsource/ghost.f:1-86]: the virtual
                               particles carry the unit wall normal in v; every step
                               each real particle with a virtual neighbour gets a ghost
@@ -119,6 +128,8 @@ typedef struct arm_params {
               With 1 or 2 (2D only) h is per particle (initially prm.h, or
               arm_upload_h), every pair uses mh = (h_i + h_j)/2 and the direct
               gathers run; 2h must stay within the cell (ARM_ERR_DOMAIN otherwise). */
+  double contact_hdr; /* ARM_CONTACT: the contact distance is h / contact_hdr (hdr = 1.2 of
+                         FORTRANsource/options.inc) */
 } arm_params;
 
 typedef struct arm_ctx arm_ctx;
@@ -173,7 +184,8 @@ int arm_eos(arm_ctx* ctx);
    ARM_GRAVITY is set in the context flags, and the external heating of
    single_step.f:217-222 (ARM_HEAT_SOURCE, arm_set_deposition) to du.  With ARM_VISC in flags a shear
    tensor pass runs first and the viscous force and tds/dt are added
-   [internal_force.f:73-171, 196-430]. */
+   [internal_force.f:73-171, 196-430]; with ARM_CONTACT (a context created with it) the
+   contact acceleration [FORTRANsource/time_integration.f90:128-142] follows. */
 int arm_internal_force(arm_ctx* ctx, int flags);
 /* av_vel [CERNsource/av_vel.f:1] (XSPH velocity correction). */
 int arm_av_vel(arm_ctx* ctx);

@@ -471,6 +471,59 @@ void orc_av_vel(const orc_params* P, int niac, const int* pi, const int* pj,
   }
 }
 
+void orc_contact_displacement(const orc_params* P, int np, int nv, const double* x,
+                              const double* y, const double* vx, const double* vy, int niac,
+                              const int* pi, const int* pj, double* dx, double* dy) {
+  /* external_force.f90:218-242 */
+  const double hh = P->h / P->contact_hdr;
+  for (int k = 0; k < niac; ++k) {
+    const int i = pi[k], j = pj[k];
+    if (j >= np && j < np + nv && i < np) {
+      double pni = 0.0, pnm = 0.0;
+      pni = pni + (x[i] - x[j]) * vx[j];
+      pnm = pnm + dx[i] * vx[j];
+      pni = pni + (y[i] - y[j]) * vy[j];
+      pnm = pnm + dy[i] * vy[j];
+      pni = hh - pni;
+      if (pni > 0.0 && pni > pnm) {
+        dx[i] = dx[i] + (pni - pnm) * vx[j];
+        dy[i] = dy[i] + (pni - pnm) * vy[j];
+      }
+    }
+  }
+}
+
+void orc_smooth_vector(const orc_params* P, int np, const double* x, const double* y,
+                       const double* rho, int niac, const int* pi, const int* pj, double* vx,
+                       double* vy) {
+  /* external_force.f90:276-337 */
+  const double m = P->mass;
+  double* ox = (double*)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
+  double* oy = (double*)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
+  double wx, wy;
+  const double w0 = kern(P, 0.0, 0.0, 0.0, &wx, &wy);
+  for (int i = 0; i < np; ++i) {
+    ox[i] = w0 * m / rho[i] * vx[i];
+    oy[i] = w0 * m / rho[i] * vy[i];
+  }
+  for (int k = 0; k < niac; ++k) {
+    const int i = pi[k], j = pj[k];
+    if (i < np && j < np) {
+      const double ddx = x[i] - x[j], ddy = y[i] - y[j];
+      const double r = sqrt(ddx * ddx + ddy * ddy);
+      const double w = kern(P, r, ddx, ddy, &wx, &wy);
+      ox[j] = ox[j] + w * m / rho[j] * vx[i];
+      oy[j] = oy[j] + w * m / rho[j] * vy[i];
+      ox[i] = ox[i] + w * m / rho[i] * vx[j];
+      oy[i] = oy[i] + w * m / rho[i] * vy[j];
+    }
+  }
+  memcpy(vx, ox, sizeof(double) * (size_t)np);
+  memcpy(vy, oy, sizeof(double) * (size_t)np);
+  free(ox);
+  free(oy);
+}
+
 static inline double tait_p(const orc_params* P, double rho) {
   /* eos.f90:274: p = rho0 c0^2 / 7 ((rho/rho0)^7 - 1) */
   const double t = rho / P->rho0;
@@ -752,6 +805,21 @@ int orc_single_step(const orc_params* P, orc_state* s, int niac, const int* pi,
     if ((P->flags & ORC_HEAT_SOURCE) && i < np)
       s->du[i] = s->du[i] + heat_source(P, s->x[i], s->y[i], s->rho[i]);
     if (dep_active(P) && i < np) s->du[i] = s->du[i] + deposition(P, s->x[i], s->y[i], s->rho[i]);
+  }
+  if (P->flags & ORC_CONTACT) {
+    /* time_integration.f90:128-142: d from the virtual neighbours, smoothed,
+       a += 0.5 d / dt^2 */
+    double* cdx = (double*)calloc((size_t)(np > 0 ? np : 1), sizeof(double));
+    double* cdy = (double*)calloc((size_t)(np > 0 ? np : 1), sizeof(double));
+    orc_contact_displacement(P, np, s->nb - np, s->x, s->y, s->vx, s->vy, niac, pi, pj, cdx,
+                             cdy);
+    orc_smooth_vector(P, np, s->x, s->y, s->rho, niac, pi, pj, cdx, cdy);
+    for (int i = 0; i < np; ++i) {
+      s->ax[i] = s->ax[i] + 0.5 * cdx[i] / (P->dt * P->dt);
+      s->ay[i] = s->ay[i] + 0.5 * cdy[i] / (P->dt * P->dt);
+    }
+    free(cdx);
+    free(cdy);
   }
   free(avdv_x);
   free(avdv_y);
@@ -1074,6 +1142,78 @@ int orc_single_step_gather(const orc_params* P, orc_state* s) {
   if (hnew) {
     memcpy(s->h, hnew, sizeof(double) * (size_t)n);
     free(hnew);
+  }
+  if (P->flags & ORC_CONTACT) {
+    /* contact_displacement and smooth_vector (external_force.f90:191-339)
+       in gather order: d_i from the virtual neighbours, then
+       a_i += 0.5 (W(0) m/rho_i d_i + sum_real W_ij m/rho_i d_j) / dt^2 */
+    double* cdx = (double*)calloc((size_t)n, sizeof(double));
+    double* cdy = (double*)calloc((size_t)n, sizeof(double));
+    const double hh = P->h / P->contact_hdr, cut2 = (2.0 * P->h) * (2.0 * P->h);
+    const int nbv = s->nb;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < np; ++i) {
+      const int key = C.key[i];
+      const int cx = key % nx, cy = key / nx;
+      double ddx = 0.0, ddy = 0.0;
+      for (int ry = cy - 1; ry <= cy + 1; ++ry) {
+        if (ry < 0 || ry >= ny) continue;
+        span_t sp[2];
+        const int ns = row_spans(P, C.start, cx, ry, sp);
+        for (int q = 0; q < ns; ++q)
+        for (int t = sp[q].t0; t < sp[q].t1; ++t) {
+          const int j = C.sorted[t];
+          if (j < np || j >= nbv) continue;
+          const double rx = dx_of(P, x[i], x[j], sp[q].sh), ry_ = y[i] - y[j];
+          double r2 = rx * rx;
+          r2 = r2 + ry_ * ry_;
+          if (!(r2 < cut2)) continue;
+          double pni = 0.0, pnm = 0.0;
+          pni = pni + rx * vx[j];
+          pnm = pnm + ddx * vx[j];
+          pni = pni + ry_ * vy[j];
+          pnm = pnm + ddy * vy[j];
+          pni = hh - pni;
+          if (pni > 0.0 && pni > pnm) {
+            ddx = ddx + (pni - pnm) * vx[j];
+            ddy = ddy + (pni - pnm) * vy[j];
+          }
+        }
+      }
+      cdx[i] = ddx;
+      cdy[i] = ddy;
+    }
+    double w0x, w0y;
+    const double w0 = kern(P, 0.0, 0.0, 0.0, &w0x, &w0y);
+    const double dt2 = P->dt * P->dt;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < np; ++i) {
+      const int key = C.key[i];
+      const int cx = key % nx, cy = key / nx;
+      double sx = w0 * m / s->rho[i] * cdx[i], sy = w0 * m / s->rho[i] * cdy[i];
+      for (int ry = cy - 1; ry <= cy + 1; ++ry) {
+        if (ry < 0 || ry >= ny) continue;
+        span_t sp[2];
+        const int ns = row_spans(P, C.start, cx, ry, sp);
+        for (int q = 0; q < ns; ++q)
+        for (int t = sp[q].t0; t < sp[q].t1; ++t) {
+          const int j = C.sorted[t];
+          if (j == i || j >= np) continue;
+          const double rx = dx_of(P, x[i], x[j], sp[q].sh), ry_ = y[i] - y[j];
+          double r2 = rx * rx;
+          r2 = r2 + ry_ * ry_;
+          if (!(r2 < cut2)) continue;
+          double wx, wy;
+          const double w = kern(P, sqrt(r2), rx, ry_, &wx, &wy);
+          sx = sx + w * m / s->rho[i] * cdx[j];
+          sy = sy + w * m / s->rho[i] * cdy[j];
+        }
+      }
+      s->ax[i] = s->ax[i] + 0.5 * sx / dt2;
+      s->ay[i] = s->ay[i] + 0.5 * sy / dt2;
+    }
+    free(cdx);
+    free(cdy);
   }
   free(tt);
   free(psi);

@@ -75,10 +75,15 @@ enum {
                                    (x_ij . grad_i W)/(r^2 + (h ah_eta)^2), T = u/cp */
   ORC_PERIODIC_X = 256, /* periodic in x with period nx cell from ox: the cells
                            cx - 1 and cx + 1 wrap, x_i - x_j gains -+ nx cell */
-  ORC_GHOSTS = 512      /* mirror ghosts of ghost.f: the virtual particles (np .. nb-1)
+  ORC_GHOSTS = 512,     /* mirror ghosts of ghost.f: the virtual particles (np .. nb-1)
                            carry the wall normal in v; every step each real particle
                            with a virtual neighbour gets a ghost mirrored across the
                            plane of the nearest one (orc_ghost_particles) */
+  ORC_CONTACT = 1024    /* contact displacement of FORTRANsource/external_force.f90:
+                           the virtual particles carry the wall normal in v; after the
+                           forces every real particle gains
+                           a += 0.5 S(d)/dt^2, d from orc_contact_displacement and
+                           S = orc_smooth_vector (time_integration.f90:128-142) */
 };
 
 enum {
@@ -128,6 +133,8 @@ typedef struct {
   double dep_o[3], dep_d[3];
   double dep_tend;
   int dep_on;
+  double contact_hdr; /* ORC_CONTACT: the contact distance is h / contact_hdr
+                         (hdr of FORTRANsource/options.inc, 1.2) */
 } orc_params;
 
 /* Particle state, SoA, FP64. */
@@ -208,6 +215,19 @@ void orc_art_heat(const orc_params* P, int nt, int niac, const int* pi, const in
 void orc_av_vel(const orc_params* P, int niac, const int* pi, const int* pj,
                 const double* x, const double* y, const double* vx, const double* vy,
                 const double* rho, double* avx, double* avy);
+/* contact_displacement (FORTRANsource/external_force.f90:191-244): over the
+   pairs (i real, j virtual) in list order, pni = h/hdr - (x_i - x_j).n_j,
+   pnm = d_i.n_j and, when pni > 0 and pni > pnm, d_i += (pni - pnm) n_j,
+   with n_j = (vx_j, vy_j); d accumulates into (dx, dy). */
+void orc_contact_displacement(const orc_params* P, int np, int nv, const double* x,
+                              const double* y, const double* vx, const double* vy, int niac,
+                              const int* pi, const int* pj, double* dx, double* dy);
+/* smooth_vector (FORTRANsource/external_force.f90:247-339) on the real
+   particles: vec_i <- W(0) m/rho_i vec_i + sum over real pairs W_ij m/rho_i
+   vec_j (each pair at both ends). */
+void orc_smooth_vector(const orc_params* P, int np, const double* x, const double* y,
+                       const double* rho, int niac, const int* pi, const int* pj, double* vx,
+                       double* vy);
 /* Tait EOS on [0, n): p, c, and the pmin clamp (p = 0, rho = rho0). */
 void orc_eos_tait(const orc_params* P, int n, double* rho, double* p, double* c);
 /* The selected EOS (P->eos) with the same clamp. */

@@ -11,10 +11,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liborc.so")
 REF_PATH = os.path.join(HERE, "_ref", "libarmref.so")
+REF90_PATH = os.path.join(HERE, "_ref", "libarmref90.so")
 
 K_WENDLAND2D, K_CUBIC2D, K_CUBIC3D, K_WENDLAND3D = 0, 1, 2, 3
 ART_VISC, ART_HEAT, AV_VEL, DBC, GRAVITY, VISC, HEAT_SOURCE = 1, 2, 4, 8, 16, 32, 64
-CONDUCTION, PERIODIC_X, GHOSTS = 128, 256, 512
+CONDUCTION, PERIODIC_X, GHOSTS, CONTACT = 128, 256, 512, 1024
 EOS_TAIT, EOS_LINEAR = 0, 1
 
 
@@ -37,6 +38,7 @@ class Params(C.Structure):
         ("cond_k", C.c_double), ("cp", C.c_double), ("sle", C.c_int),
         ("dep", C.POINTER(C.c_double)), ("dep_n", C.c_int * 3), ("dep_o", C.c_double * 3),
         ("dep_d", C.c_double * 3), ("dep_tend", C.c_double), ("dep_on", C.c_int),
+        ("contact_hdr", C.c_double),
     ]
 
 
@@ -131,6 +133,9 @@ def lib():
         "orc3_predictor": (None, [_PP, C.POINTER(State3)]),
         "orc3_corrector": (None, [_PP, C.POINTER(State3), C.c_int]),
         "orc3_run": (C.c_int, [_PP, C.POINTER(State3), C.c_int, C.c_int]),
+        "orc_contact_displacement": (None, [_PP, C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_int,
+                                            _ip, _ip, _dp, _dp]),
+        "orc_smooth_vector": (None, [_PP, C.c_int, _dp, _dp, _dp, C.c_int, _ip, _ip, _dp, _dp]),
         "orc_kat_interaction_f32": (None, [C.c_int, C.c_int, _ip, _fp, _fp, _fp, _fp, _fp, _fp,
                                            C.c_float, C.c_float, C.c_float, _fp, _fp, _fp]),
     }

@@ -71,6 +71,65 @@ def hvec(h, n):
     return np.ascontiguousarray(np.broadcast_to(np.asarray(h, np.float64), (n,)))
 
 
+_lib90 = None
+
+
+def available90():
+    return os.path.exists(orc.REF90_PATH)
+
+
+def lib90():
+    """The other tree's routines (FORTRANsource/external_force.f90, kernel.f90)."""
+    global _lib90
+    if _lib90 is None:
+        _lib90 = C.CDLL(orc.REF90_PATH)
+    return _lib90
+
+
+def kernel90(r, h, skf=0):
+    """W of FORTRANsource/kernel.f90 (dim = 3) at distance r along x."""
+    w = C.c_double(0.0)
+    dx = np.array([r, 0.0, 0.0])
+    dw = np.zeros(3)
+    lib90().kernel_(C.byref(C.c_double(r)), _p(dx), C.byref(C.c_double(h)), C.byref(w), _p(dw),
+                    _i(skf))
+    return w.value
+
+
+def contact_displacement(np_, nv, x, y, vx, vy, h, pi, pj, dx0=None, dy0=None):
+    """contact_displacement of FORTRANsource/external_force.f90:191-244 on
+    the pair list (0-based pi, pj); d starts at (dx0, dy0) or zero."""
+    def f():
+        n = len(x)
+        X = vec3(x, y)
+        V = vec3(vx, vy)
+        H = hvec(h, n)
+        D = vec3(np.zeros(n) if dx0 is None else dx0, np.zeros(n) if dy0 is None else dy0)
+        P1 = np.ascontiguousarray(pi + 1, np.int32)
+        P2 = np.ascontiguousarray(pj + 1, np.int32)
+        lib90().contact_displacement_(_i(np_), _i(nv), _p(X), _p(V), _p(H), _p(D),
+                                      _i(len(pi)), _p(P1), _p(P2))
+        return D[:, 0].copy(), D[:, 1].copy()
+    return run_big_stack(f)
+
+
+def smooth_vector(np_, x, y, h, mass, rho, vx, vy, pi, pj, skf=0):
+    """smooth_vector of FORTRANsource/external_force.f90:247-339."""
+    def f():
+        n = len(x)
+        X = vec3(x, y)
+        H = hvec(h, n)
+        M = np.full(n, mass)
+        R = np.ascontiguousarray(rho, np.float64)
+        Vv = vec3(vx, vy)
+        P1 = np.ascontiguousarray(pi + 1, np.int32)
+        P2 = np.ascontiguousarray(pj + 1, np.int32)
+        lib90().smooth_vector_(_i(np_), _p(X), _p(H), _p(M), _p(R), _p(Vv), _i(len(pi)),
+                               _p(P1), _p(P2), _i(skf))
+        return Vv[:, 0].copy(), Vv[:, 1].copy()
+    return run_big_stack(f)
+
+
 def link_list(x, y, h, z=None):
     def f():
         n = len(x)
