@@ -1336,7 +1336,7 @@ template <typename R> int alloc_state3(arm_ctx* c, St3<R>& S, int cap) {
   AL(id, cap);
   AL(xyz, cap);
   if (sizeof(R) == 4) {
-    AL(pf, cap);
+    AL(pf, (size_t)cap + 8);  // the eight-wide neighbour pre-test reads up to 7 past a row
     AL(uf, cap);
   }
   AL(v, cap);
@@ -2001,6 +2001,18 @@ void set_kparams(KParams& K, const arm_params& P) {
   K.f_xs = 0.5 * K.wnorm * K.eps_m;
   K.cellf = (float)P.cell;
   K.cut2f = (float)K.cut2;
+  {
+    // x_a = fl(cx cell + off) is within u |x_a| <= u nx cell of the exact
+    // value (u = 2^-24); a difference of two, rounded, and the y and z terms
+    // (oy_i + uy) - oy_j rounded in another order, are within
+    // E = u (2 nx cell + 4 cell + 16 h) of the exact path's per axis, so
+    // r < 2h implies r_a < 2h + 2E
+    const double u = 1.0 / 16777216.0;
+    const double nmax = (double)std::max(P.nx, std::max(P.ny, P.dim == 3 ? P.nz : 1));
+    const double E = u * (2.0 * nmax * P.cell + 4.0 * P.cell + 16.0 * P.h);
+    const double b = 2.0 * P.h + 2.0 * E;
+    K.pre2f = std::nextafter((float)(b * b * (1.0 + 1e-5)), INFINITY);
+  }
   K.eos = P.eos;
   K.lin_c2 = P.c0 * P.c0;
   K.lin_p0 = P.eos_p0;

@@ -328,23 +328,33 @@ __device__ __forceinline__ void density3_direct(St3<R>& S, const KParams& K, int
   // (below) does.
   if constexpr (sizeof(R) == 4) {
     constexpr int NB = 8;
+    // pre-test positions: x = cx cell + off rounded once (K.pre2f bounds
+    // the exact test's cut-off through that rounding); the exact test runs
+    // on the survivors in the pair walk below
+    const float cellf = K.cellf;
+    const float xi = __builtin_fmaf(pi.cxf, cellf, pi.ox);
+    const float pre2 = K.pre2f;
     for (int k = 0; k < 9; ++k) {
       const int t0 = rw.t0[k], t1 = rw.t1[k];
       const R uy = row_u3<R>(K, 1 - k % 3), uz = row_u3<R>(K, 1 - k / 3);
       if (t1 - t0 > 32 * M3_WORDS) ok = false;
       const int tA = t1 < t0 + 32 * M3_WORDS ? t1 : t0 + 32 * M3_WORDS;
+      const float yi = pi.oy + uy, zi = pi.oz + uz;
       uint32_t m0 = 0, m1 = 0, m2 = 0;
       for (int c0 = t0; c0 < tA; c0 += NB) {
         // a batch starts a multiple of NB past t0, so its bits lie in one
-        // mask word: test into a byte, then place the byte once
+        // mask word: test into a byte, then place the byte once.  The
+        // position records are padded by NB - 1 past the last particle.
         uint32_t bb = 0u;
+        const float4* pb = S.pf + c0;
   #pragma unroll
         for (int q = 0; q < NB; ++q) {
-          const int t = c0 + q;
-          const auto pj = load_pos3(S, t < tA ? t : t0);
-          R dx, dy, dz;
-          const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
-          bb |= (r2 < cut2 ? 1u : 0u) << q;
+          const float4 pj = pb[q];
+          const float dx = xi - __builtin_fmaf(pj.w, cellf, pj.x);
+          const float dy = yi - pj.y;
+          const float dz = zi - pj.z;
+          const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+          bb |= (r2 < pre2 ? 1u : 0u) << q;
         }
         const int len = tA - c0;
         if (len < NB) bb &= (1u << len) - 1u;
@@ -375,9 +385,21 @@ __device__ __forceinline__ void density3_direct(St3<R>& S, const KParams& K, int
           w1 = s1 ? rest : w1;
           w2 = (s0 || s1) ? w2 : rest;
           const auto pj = load_pos3(S, t);
+          const auto vj = S.v[t];
           R dx, dy, dz;
           const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
-          const auto vj = S.v[t];
+          // both loads in flight together: keep the velocity load out of
+          // the branch below
+          asm volatile("" ::"v"(vj.x), "v"(vj.y), "v"(vj.z));
+          if (!(r2 < cut2)) {
+            // passed the pre-test only: not a neighbour
+            const int b = t - t0;
+            const uint32_t keep = ~(1u << (b & 31));
+            m0 &= b < 32 ? keep : ~0u;
+            m1 &= (b >= 32 && b < 64) ? keep : ~0u;
+            m2 &= b >= 64 ? keep : ~0u;
+            continue;
+          }
           dens3_pair<R, SUM>(K, dx, dy, dz, r2, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, o.sum,
                              o.rs);
         }

@@ -34,6 +34,9 @@ struct KParams {
   // (beta phi - alpha c0)/2 = f_avb phi + f_ava; XSPH wnorm eps m / 2; 2 cond
   double f_cm, f_ava, f_avb, f_xs, f_cond2;
   float cellf, cut2f;
+  // FP32 neighbour pre-test: a bound on (2h)^2 for r^2 computed from
+  // x = cx cell + off rounded once per particle (the exact test follows)
+  float pre2f;
   int eos;         // 0 Tait, 1 linear
   double lin_c2;   // linear EOS: p = lin_c2 rho + lin_p0
   double lin_p0;
