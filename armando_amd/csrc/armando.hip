@@ -458,8 +458,9 @@ template <typename R> int eos_t(arm_ctx* c) {
 
 // The FP32 tiled force gather of the dam-break flag set (pressure, artificial
 // viscosity and heat, XSPH) with pa_sph and same_e2 fixed at compile time.
-template <int PC> void launch_force_hot(arm_ctx* c, int nseg, const uint4* nbm) {
-  using T = TileCfg<float>;
+template <int PC> void launch_force_hot(arm_ctx* c, const uint4* nbm) {
+  using T = HotForceCfg;
+  const int nseg = (c->K.nx + T::CW - 1) / T::CW;
   hipLaunchKernelGGL(
       (k_force_tiled<float, true, true, true, true, false, false, T::BLK, T::CW, T::CAP, PC>),
       dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, stA<float>(c), c->K, c->skey, c->cstart,
@@ -476,10 +477,10 @@ void launch_force(arm_ctx* c) {
     if (sizeof(R) == 4 && P && AV && AH && XS && !VI && !c->K.periodic &&
         !(c->K.flags & ARM_CONDUCTION)) {
       switch ((c->K.pa_sph == 0 ? 1 : 0) | (c->K.same_e2 ? 2 : 0)) {
-        case 0: launch_force_hot<0>(c, nseg, nbm); break;
-        case 1: launch_force_hot<1>(c, nseg, nbm); break;
-        case 2: launch_force_hot<2>(c, nseg, nbm); break;
-        default: launch_force_hot<3>(c, nseg, nbm); break;
+        case 0: launch_force_hot<0>(c, nbm); break;
+        case 1: launch_force_hot<1>(c, nbm); break;
+        case 2: launch_force_hot<2>(c, nbm); break;
+        default: launch_force_hot<3>(c, nbm); break;
       }
     } else if (c->K.periodic) {
       hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, true, T::BLK, T::CW, T::CAP>),

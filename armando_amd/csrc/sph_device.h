@@ -277,6 +277,7 @@ __device__ __forceinline__ float dens(const KParams& K, float r) { return (float
 template <typename R> struct Acc {
   R s = 0;
   __device__ __forceinline__ void add(R x) { s = s + x; }
+  __device__ __forceinline__ void add_if(R x, bool ok) { s = ok ? s + x : s; }
   __device__ __forceinline__ R get() const { return s; }
 };
 template <> struct Acc<float> {
@@ -286,6 +287,14 @@ template <> struct Acc<float> {
     const float t = s + y;
     c = (t - s) - y;
     s = t;
+  }
+  // add(x) when ok, else nothing (both words kept), without a branch
+  __device__ __forceinline__ void add_if(float x, bool ok) {
+    const float y = x - c;
+    const float t = s + y;
+    const float cn = (t - s) - y;
+    s = ok ? t : s;
+    c = ok ? cn : c;
   }
   __device__ __forceinline__ float get() const { return s; }
 };

@@ -384,19 +384,21 @@ __device__ __forceinline__ void for_candidates(const KParams& K, uint32_t key,
 
 // density pass: con_density (density.f:166-223) and, for method 1,
 // sum_density (density.f:1-62).  dvx = v_i.x - v_j.x.
+// ok = false: the pair is no neighbour and adds nothing (the sums keep
+// every word, as if it had not been visited).
 template <typename R, bool SUM>
 __device__ __forceinline__ void dens_pair(const KParams& K, R dx, R dy, R r2, R dvx, R dvy,
-                                          Acc<R>& sum, Acc<R>& rs) {
+                                          Acc<R>& sum, Acc<R>& rs, bool ok = true) {
   const R q = fsqrt(r2) * (R)K.invh;
   const R tq = q - (R)2;
   const R F = (R)K.C1 * (tq * tq * tq);
   const R dwx = F * dx, dwy = F * dy;
   const R vcc = fmr(dvy, dwy, dvx * dwx);
-  sum.add((R)K.mass * vcc);
+  sum.add_if((R)K.mass * vcc, ok);
   if (SUM) {
     const R s = fmr((R)-0.5, q, (R)1);
     const R s2 = s * s;
-    rs.add((R)K.mass * ((R)K.wnorm * (s2 * s2) * fmr((R)2, q, (R)1)));
+    rs.add_if((R)K.mass * ((R)K.wnorm * (s2 * s2) * fmr((R)2, q, (R)1)), ok);
   }
 }
 

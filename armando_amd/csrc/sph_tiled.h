@@ -390,6 +390,81 @@ __device__ __forceinline__ void walk_rows(uint32_t m0, uint32_t m1, uint32_t m2,
   }
 }
 
+// FP32 density pass: the neighbour pre-test of candidates l0 .. l0+len-1
+// (len <= 32) of one row against K.pre2f, with x = cx cell + off rounded
+// once per record (xi, yi: the lane's x, and its y plus the row term); the
+// survivors take the exact test in dens_walk_f32.
+template <int CAP>
+__device__ __forceinline__ uint32_t row_mask_pre(const KParams& K, const DensTile<float, CAP>& T,
+                                                 float xi, float yi, int l0, int len, int self) {
+  const float pre2 = K.pre2f, cellf = K.cellf;
+  uint32_t m = 0;
+  for (int k0 = 0; k0 < len; k0 += 8) {
+    uint32_t bb = 0u;  // the batch's bits, placed once
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float4 a = T.a[l0 + k0 + q];
+      const float dx = xi - __builtin_fmaf(a.z, cellf, a.x);
+      const float dy = yi - a.y;
+      const float r2 = __builtin_fmaf(dx, dx, dy * dy);
+      bb |= (r2 < pre2 ? 1u : 0u) << q;
+    }
+    m |= bb << k0;
+  }
+  m &= len >= 32 ? 0xffffffffu : ((1u << len) - 1u);
+  const int sb = self - l0;
+  if (sb >= 0 && sb < 32) m &= ~(1u << sb);
+  return m;
+}
+
+// for_neighbours of the FP32 density pass: the pre-test, then over its
+// survivors the exact test (r^2 < (2h)^2 with the direct path's arithmetic)
+// and the pair sums, without a branch: a survivor that fails the exact test
+// adds nothing (dens_pair with ok = false) and leaves the recorded mask.
+// Same visiting order and sums as for_neighbours; the exact masks in rm.
+template <bool SUM, int CW, int CAP>
+__device__ __forceinline__ void dens_walk_f32(const KParams& K, const Segment<CW>& g,
+                                              const int (*cs)[CW + 3],
+                                              const DensTile<float, CAP>& T, const LaneF& pi,
+                                              float2 vi, int i, uint32_t key, DensOwn<float>& o,
+                                              uint32_t* rm, bool* ok) {
+  const int nx = K.nx;
+  const int cx = (int)key - g.cy * nx;
+  const int a0 = (cx > 0 ? cx - 1 : 0) - g.slo;
+  const int a1 = (cx + 1 < nx ? cx + 1 : nx - 1) - g.slo + 1;
+  const int self = g.lo[1] + (i - g.rs[1]);
+  const float cut2 = K.cut2f;
+  const float xi = __builtin_fmaf(pi.cxf, K.cellf, pi.ox);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (g.rl[k] == 0) continue;
+    const float uy = row_uy(K, k);
+    const float yi = pi.oy + uy;
+    const int ta = cs[k][a0];
+    const int tb = cs[k][a1];
+    const int base = g.lo[k] - g.rs[k];
+    if (tb - ta > 32) *ok = false;
+    for (int c0 = ta; c0 < tb; c0 += 32) {
+      const int len = min(32, tb - c0);
+      uint32_t m = row_mask_pre(K, T, xi, yi, base + c0, len, self);
+      uint32_t w = m;
+      while (w) {
+        const int b = __builtin_ctz(w);
+        w &= w - 1;
+        const int l = base + c0 + b;
+        float dx, dy;
+        T.delta(K, pi, l, uy, dx, dy);
+        const float r2 = dx * dx + dy * dy;
+        const bool in = r2 < cut2;
+        m = in ? m : (m & ~(1u << b));
+        const float2 vj = T.vel(l);
+        dens_pair<float, SUM>(K, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs, in);
+      }
+      if (c0 == ta) rm[k] = m;
+    }
+  }
+}
+
 // The same walk driven by masks recorded by the density pass.
 template <typename R, int CW, typename Visit>
 __device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& g,
@@ -403,7 +478,7 @@ __device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& 
 }
 
 template <typename R, bool SUM, bool EOS, bool PER, int BLK, int CW, int CAP>
-__global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && !PER ? 8 : 1))) void k_density_tiled(St<R> S, KParams K,
                                                        const uint32_t* __restrict__ skey,
                                                        const int* __restrict__ cstart, int nseg,
                                                        uint4* __restrict__ nbm) {
@@ -433,16 +508,20 @@ __global__ __launch_bounds__(BLK) void k_density_tiled(St<R> S, KParams K,
     dens_begin<R, SUM>(K, o);
     uint32_t rm[3] = {0u, 0u, 0u};
     bool ok = true;
-    for_neighbours<R, CW>(
-        K, g, cs, T, pi, i, skey[i],
-        [&](int l, R uy) {
-          R dx, dy;
-          T.delta(K, pi, l, uy, dx, dy);
-          const R r2 = dx * dx + dy * dy;
-          const auto vj = T.vel(l);
-          dens_pair<R, SUM>(K, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
-        },
-        rm, &ok);
+    if constexpr (sizeof(R) == 4) {
+      dens_walk_f32<SUM, CW>(K, g, cs, T, pi, vi, i, skey[i], o, rm, &ok);
+    } else {
+      for_neighbours<R, CW>(
+          K, g, cs, T, pi, i, skey[i],
+          [&](int l, R uy) {
+            R dx, dy;
+            T.delta(K, pi, l, uy, dx, dy);
+            const R r2 = dx * dx + dy * dy;
+            const auto vj = T.vel(l);
+            dens_pair<R, SUM>(K, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs);
+          },
+          rm, &ok);
+    }
     nbm[i] = make_uint4(rm[0], rm[1], rm[2], ok ? 1u : 0u);
     dens_end<R, SUM, EOS>(S, K, i, o);
   }
@@ -497,7 +576,7 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
 // (force_pair_f32), -1 to read them from K.
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI, bool PER, int BLK, int CW,
           int CAP, int PC = -1>
-__global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 6 : 1))) void k_force_tiled(St<R> S, KParams K,
                                                      const uint32_t* __restrict__ skey,
                                                      const int* __restrict__ cstart, int nseg,
                                                      const uint4* __restrict__ nbm) {
@@ -557,6 +636,12 @@ __global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
 template <typename R> struct TileCfg;
 template <> struct TileCfg<float> {
   static constexpr int BLK = 256, CW = 32, CAP = 720;
+};
+// The FP32 force instances with compile-time pair constants (PC >= 0): 28-cell
+// segments, so that the tile (640 records of 40 bytes) lets six workgroups
+// share a CU, and registers capped for six waves per SIMD.
+struct HotForceCfg {
+  static constexpr int BLK = 256, CW = 28, CAP = 640;
 };
 template <> struct TileCfg<double> {
   static constexpr int BLK = 128, CW = 16, CAP = 400;
