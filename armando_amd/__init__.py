@@ -37,7 +37,7 @@ EXPORTS = ("arm_create", "arm_destroy", "arm_last_error", "arm_domain_violations
            "arm_rollout_backward", "arm_update_params", "arm_set_objective",
            "arm_rollout_param_grads", "arm_conservation3", "arm_upload_h", "arm_download_h",
            "arm_ghost_particles", "arm_ghost_count", "arm_download_ghosts",
-           "arm_set_deposition", "arm_link_list_static")
+           "arm_set_deposition", "arm_set_deposition_binning", "arm_link_list_static")
 ARM_STATIC_MAX_PARTICLES = 1 << 20  # arm_link_list_static limits (include/armando.h)
 ARM_STATIC_MAX_CELLS = 1 << 20
 
@@ -125,6 +125,8 @@ def load_library(path=LIB_PATH):
         "arm_set_objective": (C.c_int, [vp, C.c_int]),
         "arm_rollout_param_grads": (C.c_int, [vp, dpn, C.c_int]),
         "arm_set_deposition": (C.c_int, [vp, dpn, dpn, C.c_void_p, dpn, C.c_double]),
+        "arm_set_deposition_binning": (C.c_int, [vp, C.c_int, dpn, dpn, C.c_void_p, dpn,
+                                                 C.c_double]),
         # device pointers: x, key_sorted, id_sorted, cell_start, outside, stream
         "arm_link_list_static": (C.c_int, [C.POINTER(ArmParams), C.c_int] + [vp] * 6),
     }
@@ -345,11 +347,12 @@ class Simulation:
         return da.value, g, tb.value
 
     def set_deposition(self, binning=None, t_end=float("inf")):
-        """The energy-deposition heat source of fluka_heat.f (Cartesian
-        binning): `binning` a dict with origin (3), cell (3), n (3) and data
-        (n0 n1 n2 values in W/m^3, x fastest), as `armando_amd.io.read_fluka_*`
-        return; it acts in the steps with its dt <= t_end.  None switches the
-        source off."""
+        """The energy-deposition heat source of fluka_heat.f: `binning` a
+        dict with origin (3), cell (3), n (3), data (n0 n1 n2 values in W/m^3,
+        first axis fastest) and optionally type (FLUKA's tfluka: 0 Cartesian
+        x, y, z; 1 or 7 R-phi-z, r, phi in degrees, z), as
+        `armando_amd.io.read_fluka_*` return; it acts in the steps with
+        its dt <= t_end.  None switches the source off."""
         if binning is None:
             self._check(self.L.arm_set_deposition(self.ctx, None, None, None, None, 0.0))
             return
@@ -359,9 +362,10 @@ class Simulation:
         data = np.ascontiguousarray(np.asarray(binning["data"], np.float64).reshape(-1))
         if o.size != 3 or d.size != 3 or nb.size != 3 or data.size != int(np.prod(nb.astype(np.int64))):
             raise ValueError("binning needs origin, cell and n of 3 entries and n0*n1*n2 values")
-        self._check(self.L.arm_set_deposition(self.ctx, _ptr(o), _ptr(d),
-                                               nb.ctypes.data_as(C.c_void_p), _ptr(data),
-                                               float(t_end)))
+        self._check(self.L.arm_set_deposition_binning(self.ctx, int(binning.get("type", 0)),
+                                                       _ptr(o), _ptr(d),
+                                                       nb.ctypes.data_as(C.c_void_p), _ptr(data),
+                                                       float(t_end)))
 
     def set_objective(self, kind):
         """OBJ_PRESSURE (J = sum dt sum p^2) or OBJ_ENERGY (u^2) for the next rollout."""

@@ -2655,7 +2655,14 @@ int arm_update_params(arm_ctx* c, const arm_params* prm) {
 
 int arm_set_deposition(arm_ctx* c, const double* origin, const double* cell, const int* nbin,
                        const double* data, double t_end) {
+  return arm_set_deposition_binning(c, 0, origin, cell, nbin, data, t_end);
+}
+
+int arm_set_deposition_binning(arm_ctx* c, int type, const double* origin, const double* cell,
+                               const int* nbin, const double* data, double t_end) {
   if (!c) return ARM_ERR_ARG;
+  if (data && type != 0 && type != 1 && type != 7)
+    return fail(c, ARM_ERR_UNSUPPORTED, "binning type %d: 0 (Cartesian), 1 or 7 (R-phi-z)", type);
   if (!data) {  // switch the source off
     if (c->d_dep) {
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2696,6 +2703,7 @@ int arm_set_deposition(arm_ctx* c, const double* origin, const double* cell, con
     c->K.dep_o[k] = origin[k];
     c->K.dep_d[k] = cell[k];
   }
+  c->K.dep_type = type == 0 ? 0 : 1;
   c->dep_tend = t_end;
   return ARM_OK;
 }

@@ -63,6 +63,7 @@ struct KParams {
   const double* dep;
   int dep_n[3];
   double dep_o[3], dep_d[3];
+  int dep_type;  // 0: Cartesian axes (x, y, z); 1: R-phi-z axes (r, phi in degrees, z)
   // stop at the first domain error of a fused run of steps: its is the step
   // the launch belongs to; *halt (null outside fused runs) is 0, or the
   // first step that must not run (set by the launch that found a particle
@@ -145,10 +146,23 @@ __device__ __forceinline__ void dep_axis(double s, int n, int& lo, int& hi, doub
 // (fluka_heat.f:235-337, dim = 3, tfluka = 0), 0 outside the binning.  The
 // 2D branch (:485-573) is the same at z = 0: (0 - oz)/dz is its -oz/dz, and
 // z = 0 was checked to lie within the map when it was set.
+//
+// R-phi-z binning (dep_type 1; fluka_heat.f:341-481 in 3D, :593-704 in 2D):
+// the map's axes are r = sqrt(x^2 + y^2), phi = 180/pi atan2(y, x) (pi the
+// REAL(4) literal of options.inc:21) and z; the bins and weights are those
+// of the Cartesian branch on these coordinates (the reference's branch
+// scales z by the phi bin width and, in 3D, picks the bin pairs again after
+// clamping them; neither is followed).
 __device__ __forceinline__ double deposition_at(const KParams& K, double x, double y, double z) {
   const int* n = K.dep_n;
   const double* o = K.dep_o;
   const double* d = K.dep_d;
+  if (K.dep_type == 1) {
+    const double r = sqrt(x * x + y * y);
+    const double phi = 180.0 / 3.1415927410125732 * atan2(y, x);
+    x = r;
+    y = phi;
+  }
   if (!(x >= o[0] && y >= o[1] && z >= o[2] && x <= o[0] + n[0] * d[0] &&
         y <= o[1] + n[1] * d[1] && z <= o[2] + n[2] * d[2]))
     return 0.0;

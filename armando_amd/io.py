@@ -251,8 +251,9 @@ def case_from_state(template, state, np_real):
 
 # ------------------------------------------------------------------ FLUKA
 class Binning(dict):
-    """A Cartesian deposition map: origin (3), cell (3), n (3), type, data
-    (n[0] n[1] n[2] values in W/m^3, x fastest)."""
+    """A deposition map: origin (3), cell (3), n (3), type (FLUKA's tfluka:
+    0 Cartesian x, y, z; 1 or 7 R-phi-z, r, phi in degrees, z), data (n[0]
+    n[1] n[2] values in W/m^3, first axis fastest)."""
 
 
 def _scale_binning(typ, minb, dbin, nbin, vals, translation, size_scale, energy_scale,
@@ -260,15 +261,20 @@ def _scale_binning(typ, minb, dbin, nbin, vals, translation, size_scale, energy_
     # fluka_heat.f:76-95 / :176-195: sizes scaled by sf, then translated;
     # values (GeV/cm^3 per primary in FLUKA's units) times
     # np ef / (dt sf^3).  minb, dbin and vals are REAL(4) in the reference.
-    if typ % 10 != 0:
-        raise ValueError(f"binning type {typ}: only Cartesian binning (type 0) is supported")
+    # R-phi-z (tb 1 or 7 with more than one phi bin, :56-75 / :157-176): r and
+    # z scaled by sf, phi (degrees) not; all three translated; the values
+    # scaled from the ones read (the reference's branch scales the array it
+    # was given instead of the values it read).
+    tb = typ % 10
+    if tb not in (0, 1, 7) or (tb in (1, 7) and nbin[1] < 2):
+        raise ValueError(f"binning type {typ}: Cartesian (0) and R-phi-z (1, 7) binnings only")
     sf = float(size_scale)
-    origin = [sf * float(np.float32(m)) + float(t) for m, t in zip(minb, translation)]
-    cell = [sf * float(np.float32(d)) for d in dbin]
+    sc = [sf, 1.0 if tb else sf, sf]
+    origin = [s * float(np.float32(m)) + float(t) for s, m, t in zip(sc, minb, translation)]
+    cell = [s * float(np.float32(d)) for s, d in zip(sc, dbin)]
     factor = (float(n_primaries) * float(energy_scale)) / (float(dep_time) * sf ** 3)
     data = factor * np.asarray(vals, dtype=np.float32).astype(np.float64)
-    return Binning(origin=origin, cell=cell, n=[int(k) for k in nbin], type=int(typ % 10),
-                   data=data)
+    return Binning(origin=origin, cell=cell, n=[int(k) for k in nbin], type=int(tb), data=data)
 
 
 def read_fluka_text(path, translation=(0.0, 0.0, 0.0), size_scale=1.0, energy_scale=1.0,

@@ -264,11 +264,22 @@ int arm_update_params(arm_ctx* ctx, const arm_params* prm);
    (m); in 2D z = 0 must lie within the z range.  data = NULL switches the source
    off.  The step number is the its of arm_time_integration, or for the
    routine-by-routine calls the one after the last arm_corrector (1 after
-   arm_upload).  Only the Cartesian binning is implemented (the readers of
-   armando_amd.io reject the R-phi-z binning, tfluka 1 and 7); the rollouts
-   return ARM_ERR_UNSUPPORTED while a map is set. */
+   arm_upload).  The rollouts return ARM_ERR_UNSUPPORTED while a map is
+   set. */
 int arm_set_deposition(arm_ctx* ctx, const double* origin, const double* cell, const int* nbin,
                        const double* data, double t_end);
+/* The same with the binning type `tfluka` of fluka_heat.f (its `type`
+   argument [CERNsource/fluka_heat.f:201-233, tfluka set by read_binning_*
+   :47-75]): 0 Cartesian, as arm_set_deposition; 1 or 7 R-phi-z
+   [fluka_heat.f:341-481 (3D), :593-704 (2D)]: the axes of origin, cell,
+   nbin and data are r = sqrt(x^2 + y^2) (m), phi = atan2(y, x) in degrees
+   and z (m), r fastest, with the bins, weights and clamping of the
+   Cartesian branch on those coordinates.  (The reference's R-phi-z branch
+   scales z by the phi bin width, :359 and :608, and in 3D picks the bin
+   pairs again after clamping, :398-420, which can index past the map;
+   neither is followed.)  Other types: ARM_ERR_UNSUPPORTED. */
+int arm_set_deposition_binning(arm_ctx* ctx, int type, const double* origin, const double* cell,
+                               const int* nbin, const double* data, double t_end);
 /* Objective of the next rollout: J = sum over steps of dt * sum_{obj} f_i
    with f = p^2 (ARM_OBJ_PRESSURE, the default) or u^2 (ARM_OBJ_ENERGY), taken
    after the EOS of each step.  No counterpart in the reference. */

@@ -565,6 +565,18 @@ double orc_deposition_at(const orc_params* P, double x, double y, double z, int*
   const int* n = P->dep_n;
   const double* o = P->dep_o;
   const double* d = P->dep_d;
+  if (P->dep_type == 1) {
+    /* R-phi-z binning (fluka_heat.f:341-347 in 3D, :593-595 in 2D): the
+       map's axes are r, phi in degrees and z.  The reference's z coordinate
+       divides by the phi bin width (:359, :608) and its 3D branch picks the
+       bin pairs again after clamping them (:398-420); both are taken here
+       as the Cartesian branch does them: z by the z bin width, the clamped
+       pairs kept. */
+    const double r = sqrt(x * x + y * y);
+    const double phi = 180.0 / PI_F * atan2(y, x); /* options.inc's pi, a REAL(4) literal */
+    x = r;
+    y = phi;
+  }
   *inside = x >= o[0] && y >= o[1] && z >= o[2] && x <= o[0] + n[0] * d[0] &&
             y <= o[1] + n[1] * d[1] && z <= o[2] + n[2] * d[2];
   if (!*inside) return 0.0;

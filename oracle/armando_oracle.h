@@ -135,6 +135,10 @@ typedef struct {
   int dep_on;
   double contact_hdr; /* ORC_CONTACT: the contact distance is h / contact_hdr
                          (hdr of FORTRANsource/options.inc, 1.2) */
+  int dep_type; /* the binning of the deposition map: 0 Cartesian (tfluka = 0), 1 R-phi-z
+                   (tfluka = 1 or 7; fluka_heat.f:341-481 in 3D, :593-704 in 2D): the axes
+                   are r = sqrt(x^2 + y^2), phi = atan2(y, x) in degrees and z, with the
+                   bin pairs clamped to the map as in the Cartesian branch */
 } orc_params;
 
 /* Particle state, SoA, FP64. */

@@ -38,7 +38,7 @@ class Params(C.Structure):
         ("cond_k", C.c_double), ("cp", C.c_double), ("sle", C.c_int),
         ("dep", C.POINTER(C.c_double)), ("dep_n", C.c_int * 3), ("dep_o", C.c_double * 3),
         ("dep_d", C.c_double * 3), ("dep_tend", C.c_double), ("dep_on", C.c_int),
-        ("contact_hdr", C.c_double),
+        ("contact_hdr", C.c_double), ("dep_type", C.c_int),
     ]
 
 
@@ -161,9 +161,11 @@ def params_from(case):
     return params(**{k: case[k] for k in keys if k in case})
 
 
-def set_deposition(P, origin, cell, n, data, t_end=float("inf"), on=1):
-    """Attach a Cartesian deposition map (W/m^3, shape (n[2], n[1], n[0]) or
-    flat with x fastest) to P; P keeps a reference to the array."""
+def set_deposition(P, origin, cell, n, data, t_end=float("inf"), on=1, typ=0):
+    """Attach a deposition map (W/m^3, shape (n[2], n[1], n[0]) or flat with
+    the first axis fastest) to P; P keeps a reference to the array.  typ:
+    FLUKA's binning type, 0 Cartesian (x, y, z), 1 or 7 R-phi-z (r, phi in
+    degrees, z)."""
     arr = np.ascontiguousarray(np.asarray(data, dtype=np.float64).reshape(-1))
     assert arr.size == int(n[0]) * int(n[1]) * int(n[2])
     P._dep_keep = arr
@@ -174,6 +176,9 @@ def set_deposition(P, origin, cell, n, data, t_end=float("inf"), on=1):
         P.dep_d[k] = float(cell[k])
     P.dep_tend = float(t_end)
     P.dep_on = int(on)
+    if int(typ) % 10 not in (0, 1, 7):
+        raise ValueError(f"binning type {typ}")
+    P.dep_type = 0 if int(typ) % 10 == 0 else 1
     return P
 
 

@@ -313,11 +313,12 @@ def ghost_particles(x, y, vx, vy, h, mass, rho, p, np_, nv, pi, pj):
     return run_big_stack(f)
 
 
-def fluka_heat(x, y, rho, origin, cell, nbin, data, z=None):
-    """fluka_heat.f fluka_heat (as shipped, dim = 3, Cartesian binning
-    tfluka = 0) for every particle, at z = 0 unless z is given; dedt starts
-    at zero, as single_step.f:92-96 leaves it (the 3D branch does not touch
-    particles outside the binning)."""
+def fluka_heat(x, y, rho, origin, cell, nbin, data, z=None, tfluka=0):
+    """fluka_heat.f fluka_heat (as shipped, dim = 3; Cartesian binning
+    tfluka = 0, or 1 / 7 for its R-phi-z branch) for every particle, at
+    z = 0 unless z is given; dedt starts at zero, as single_step.f:92-96
+    leaves it (the 3D branch does not touch particles outside the
+    binning)."""
     def f():
         n = len(x)
         X = vec3(x, y, z)
@@ -327,7 +328,7 @@ def fluka_heat(x, y, rho, origin, cell, nbin, data, z=None):
         D = np.ascontiguousarray(cell, np.float64)
         N = np.ascontiguousarray(nbin, np.int32)
         M = np.ascontiguousarray(np.asarray(data, np.float64).reshape(-1))
-        lib().fluka_heat_(_i(n), _p(X), _p(R), _p(out), _p(O), _p(D), _p(N), _i(0), _p(M))
+        lib().fluka_heat_(_i(n), _p(X), _p(R), _p(out), _p(O), _p(D), _p(N), _i(tfluka), _p(M))
         return out
     return run_big_stack(f)
 

@@ -1,9 +1,18 @@
 """The energy-deposition heat source of CERNsource/fluka_heat.f (Cartesian
-binning) in the oracle, and the binning readers of armando_amd.io, against
-the reference's own fluka_heat, read_binning_txt and read_binning_bin
-compiled unmodified (oracle/_ref), and against what linear interpolation
-between bin centres fixes: a constant map, a map linear in the bin centres,
-the bin values at the centres and zero outside the binning.
+and R-phi-z binnings) in the oracle, and the binning readers of
+armando_amd.io, against the reference's own fluka_heat, read_binning_txt and
+read_binning_bin compiled unmodified (oracle/_ref), and against what linear
+interpolation between bin centres fixes: a constant map, a map linear in the
+bin centres, the bin values at the centres and zero outside the binning.
+
+The reference's R-phi-z branch scales z by the phi bin width and, in 3D,
+picks the bin pairs again after clamping them (fluka_heat.f:359, :398-420,
+:608); the oracle does neither.  Against the reference that branch is
+compared on a constant map at positions whose bins are not on the map's
+edge (where its re-picked pairs stay in the map and any weights give the
+constant): the classification inside / outside, with r, phi in degrees from
+the Fortran pi, and the value.  The interpolation itself is pinned by the
+closed forms.
 """
 import os
 
@@ -117,6 +126,111 @@ def test_bin_centres_give_the_bin_values_in_x_and_y():
     np.testing.assert_allclose(got, m[1, j, i], rtol=1e-14)
 
 
+# R-phi-z binning: r in [0.02, 0.09] m, phi in [-60, 100] degrees, z in
+# [-0.03, 0.05] m (holds the slice z = 0 at the centre of z bin 1)
+RORIGIN = (0.02, -60.0, -0.03)
+RCELL = (0.01, 20.0, 0.02)
+RNBIN = (7, 8, 4)
+PI_F = float(np.float32(np.pi))  # options.inc:21's pi, a REAL(4) literal
+
+
+def rphiz_params(data):
+    P = orc.params(dt=1e-3)
+    return orc.set_deposition(P, RORIGIN, RCELL, RNBIN, data, 1.0, typ=1)
+
+
+def rcentres(k):
+    return RORIGIN[k] + (np.arange(RNBIN[k]) + 0.5) * RCELL[k]
+
+
+def polar(r, phi_deg):
+    """(x, y) whose phi, as the Fortran computes it with its pi, is phi_deg."""
+    t = np.asarray(phi_deg) * (PI_F / 180.0)
+    return r * np.cos(t), r * np.sin(t)
+
+
+def phi_of(x, y):
+    return 180.0 / PI_F * np.arctan2(y, x)
+
+
+def linear_rphiz_map(a, b, c, d):
+    Z, P_, R = np.meshgrid(rcentres(2), rcentres(1), rcentres(0), indexing="ij")
+    return (a + b * R + c * P_ + d * Z).reshape(-1)
+
+
+def test_rphiz_linear_map_is_reproduced_2d_and_3d():
+    """A map linear in (r, phi, z) at the bin centres is interpolated exactly
+    between the first and last centres: phi in degrees, r the first axis,
+    z by the z bin width (the phi width there would move the z weights)."""
+    a, b, c, d = 2.0e8, 3.0e9, 1.5e6, 4.0e9
+    data = linear_rphiz_map(a, b, c, d)
+    rng = np.random.default_rng(31)
+    rc, pc, zc = rcentres(0), rcentres(1), rcentres(2)
+    r = rng.uniform(rc[0], rc[-1], 3000)
+    ph = rng.uniform(pc[0], pc[-1], 3000)
+    x, y = polar(r, ph)
+    got = orc.fluka_heat(rphiz_params(data), x, y, np.ones_like(x))
+    want = a + b * np.hypot(x, y) + c * phi_of(x, y)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * abs(a))
+    z = rng.uniform(zc[0], zc[-1], 3000)
+    got3 = orc.fluka_heat3(rphiz_params(data), x, y, z, np.ones_like(x))
+    np.testing.assert_allclose(got3, want + d * z, rtol=1e-12, atol=1e-12 * abs(a))
+
+
+def test_rphiz_bin_centres_give_the_bin_values():
+    rng = np.random.default_rng(32)
+    data = rng.uniform(0.0, 1e9, int(np.prod(RNBIN)))
+    m = data.reshape(RNBIN[2], RNBIN[1], RNBIN[0])
+    i, j = np.meshgrid(np.arange(1, RNBIN[0] - 1), np.arange(1, RNBIN[1] - 1))
+    i, j = i.ravel(), j.ravel()
+    x, y = polar(rcentres(0)[i], rcentres(1)[j])
+    got = orc.fluka_heat(rphiz_params(data), x, y, np.ones(len(i)))
+    # the centres are met to the rounding of r and phi
+    np.testing.assert_allclose(got, m[1, j, i], rtol=1e-12)
+
+
+def test_rphiz_constant_map_and_outside():
+    rng = np.random.default_rng(33)
+    r = rng.uniform(0.0, 0.11, 4000)
+    ph = rng.uniform(-180.0, 180.0, 4000)
+    x, y = polar(r, ph)
+    rho = rng.uniform(900.0, 1100.0, len(x))
+    got = orc.fluka_heat(rphiz_params(np.full(int(np.prod(RNBIN)), 3.0e8)), x, y, rho)
+    rr, pp = np.hypot(x, y), phi_of(x, y)
+    inside = ((rr >= RORIGIN[0]) & (rr <= RORIGIN[0] + RNBIN[0] * RCELL[0]) &
+              (pp >= RORIGIN[1]) & (pp <= RORIGIN[1] + RNBIN[1] * RCELL[1]))
+    assert inside.sum() > 500 and (~inside).sum() > 500
+    np.testing.assert_allclose(got[inside], 3.0e8 / rho[inside], rtol=1e-15)
+    assert np.all(got[~inside] == 0.0)
+
+
+@pytest.mark.skipif(not refbind.available(), reason="oracle/_ref not built")
+def test_rphiz_against_reference_branch_on_a_constant_map():
+    rng = np.random.default_rng(34)
+    n = 6000
+    r = rng.uniform(0.0, 0.11, n)
+    ph = rng.uniform(-180.0, 180.0, n)
+    x, y = polar(r, ph)
+    z = rng.uniform(RORIGIN[2] - 0.02, RORIGIN[2] + RNBIN[2] * RCELL[2] + 0.02, n)
+    rr, pp = np.hypot(x, y), phi_of(x, y)
+    s = [(rr - RORIGIN[0]) / RCELL[0], (pp - RORIGIN[1]) / RCELL[1],
+         (z - RORIGIN[2]) / RCELL[2]]
+    inside = np.ones(n, bool)
+    interior = np.ones(n, bool)
+    for k in range(3):
+        inside &= (s[k] >= 0) & (s[k] <= RNBIN[k])
+        interior &= (s[k] >= 1.0) & (s[k] < RNBIN[k] - 1)
+    keep = ~inside | interior
+    x, y, z = x[keep], y[keep], z[keep]
+    rho = rng.uniform(900.0, 1100.0, len(x))
+    data = np.full(int(np.prod(RNBIN)), 3.0e8)
+    want = refbind.fluka_heat(x, y, rho, RORIGIN, RCELL, RNBIN, data, z=z, tfluka=1)
+    got = orc.fluka_heat3(rphiz_params(data), x, y, z, rho)
+    assert np.count_nonzero(want) > 300 and np.count_nonzero(want == 0) > 300
+    assert np.array_equal(got == 0, want == 0)
+    np.testing.assert_allclose(got, want, rtol=1e-14)
+
+
 def write_form(path, minb, maxb, nbin, dbin, vals, typ=10):
     with open(path, "w") as f:
         f.write(" deposition test run\n")
@@ -193,10 +307,49 @@ def test_readers_equal_reference(tmp_path, kind):
     np.testing.assert_allclose(b["data"], f * vals.astype(np.float64), rtol=1e-15)
 
 
-def test_reader_rejects_cylindrical_binning(tmp_path):
+@pytest.mark.parametrize("kind", ["txt", "bin"])
+def test_readers_rphiz_equal_reference(tmp_path, kind):
+    """Types 11 and 17 (tb 1, 7: R-phi-z): r and z scaled, phi not, all
+    translated, as read_binning_* set ofluka, dfluka, nfluka and tfluka; the
+    values scaled from the ones read (the reference's branch scales the
+    array it was given, left at zero here)."""
+    rng = np.random.default_rng(18)
+    nbin = (5, 6, 3)
+    minb = np.float32([0.5, -90.0, -20.0])
+    dbin = np.float32([0.25, 30.0, 5.0])
+    maxb = minb + dbin * np.float32(nbin)
+    vals = rng.uniform(0.0, 2e-3, int(np.prod(nbin))).astype(np.float32)
+    for typ, tb in ((11, 1), (17, 7)):
+        if kind == "txt":
+            path = os.path.join(tmp_path, "fluka_binning.FORM")
+            write_form(path, minb, maxb, nbin, dbin, vals, typ=typ)
+            b = io.read_fluka_text(path, **SCALE)
+        else:
+            path = os.path.join(tmp_path, "fluka_binning")
+            write_bin(path, minb, maxb, nbin, dbin, vals, typ=typ)
+            b = io.read_fluka_binary(path, **SCALE)
+        assert b["type"] == tb and list(b["n"]) == list(nbin)
+        if refbind.available():
+            O, D, N, T, _ = refbind.read_binning(
+                kind, str(tmp_path), SCALE["translation"], SCALE["size_scale"],
+                SCALE["energy_scale"], SCALE["n_primaries"], SCALE["dep_time"], len(vals))
+            assert T == tb and list(N) == list(nbin)
+            np.testing.assert_array_equal(np.array(b["origin"]), O)
+            np.testing.assert_array_equal(np.array(b["cell"]), D)
+        assert b["cell"][1] == float(dbin[1]) and b["origin"][1] == float(minb[1])
+        f = 3.03e13 * 1.602e-10 / (140e-9 * 1e-6)
+        np.testing.assert_allclose(b["data"], f * vals.astype(np.float64), rtol=1e-15)
+
+
+@pytest.mark.parametrize("typ, nphi", [(2, 4), (8, 4), (11, 1)])
+def test_reader_rejects_other_binnings(tmp_path, typ, nphi):
+    """Region (2), region/lattice/user (8) and R-Z (R-phi-z with one phi
+    bin) binnings: the reference's readers leave them unread."""
     rng = np.random.default_rng(16)
     minb, maxb, nbin, dbin, vals = synthetic_map(rng)
+    nbin = (nbin[0], nphi, nbin[2])
+    vals = rng.uniform(0.0, 1.0, int(np.prod(nbin))).astype(np.float32)
     path = os.path.join(tmp_path, "fluka_binning.FORM")
-    write_form(path, minb, maxb, nbin, dbin, vals, typ=11)
-    with pytest.raises(ValueError, match="Cartesian"):
+    write_form(path, minb, maxb, nbin, dbin, vals, typ=typ)
+    with pytest.raises(ValueError, match="binning type"):
         io.read_fluka_text(path)

@@ -1,6 +1,6 @@
 """The energy-deposition heat source (CERNsource/fluka_heat.f:201-704,
-Cartesian binning, applied while its dt <= dtfluka as single_step.f:219
-does) on the GPU, through the C ABI, against the oracle.
+Cartesian and R-phi-z binnings, applied while its dt <= dtfluka as
+single_step.f:219 does) on the GPU, through the C ABI, against the oracle.
 
 Bar: FP64 fields within 1e-10 of the oracle over steps that cross the end
 of the deposition time; the FP32 form within 1e-5 rel-L2 of FP64 on u; the
@@ -176,3 +176,81 @@ def test_deposition_3d_matches_oracle():
         assert np.max(np.abs(s[k] - ref)) < 1e-10 * max(1.0, np.max(np.abs(ref))), k
     heated = s["u"] > 1.0
     assert heated.sum() > 20 and (~heated[:case["np"]]).sum() > 20
+
+
+def polar_binning(seed, n=(6, 4, 2), origin=(0.004, 5.0, -0.01), cell=(0.01, 20.0, 0.01)):
+    """An R-phi-z map (r fastest; phi in degrees) over part of the column."""
+    rng = np.random.default_rng(seed)
+    return dict(origin=origin, cell=cell, n=n, type=1,
+                data=rng.uniform(1e8, 1e9, int(np.prod(n))))
+
+
+def test_rphiz_deposition_matches_oracle():
+    """The R-phi-z binning (fluka_heat.f:593-704 in 2D, tfluka = 1), 12
+    steps with the deposition time ending after step 6, against the oracle;
+    the FP32 form against FP64."""
+    case = cases.dam_break_small(scale=0.05)
+    b = polar_binning(23)
+    t_end = 6.5 * case["dt"]
+    banner(f"R-phi-z deposition fp64, 12 steps, n={case['n']}")
+    sim = armando_amd.Simulation(case)
+    sim.set_deposition(b, t_end)
+    sim.time_integration(1, 12)
+    s = sim.state()
+    sim.close()
+    P = orc.params_from(case)
+    orc.set_deposition(P, b["origin"], b["cell"], b["n"], b["data"], t_end, typ=1)
+    st = orc.OracleState(case["n"], case["np"])
+    st.set(**{k: case[k] for k in FIELDS_IN})
+    orc.run(P, st, 1, 12, gather=True)
+    for k in ("x", "y", "rho", "u", "p"):
+        e = relmax(s[k], st.field(k))
+        banner(f"  {k} relmax {e:.2e}")
+        assert e < 1e-10, k
+    for k in ("vx", "vy"):
+        assert np.max(np.abs(s[k] - st.field(k))) < 1e-10 * max(1.0, np.max(np.abs(st.field(k))))
+    heated = s["u"] > 1.0
+    assert heated.sum() > 50 and (~heated[:case["np"]]).sum() > 50
+    c32 = cases.dam_break_small(scale=0.05, precision=cases.FP32)
+    sim = armando_amd.Simulation(c32)
+    sim.set_deposition(b, t_end)
+    sim.time_integration(1, 12)
+    u32 = sim.state()["u"]
+    sim.close()
+    rel = np.linalg.norm(u32 - s["u"]) / np.linalg.norm(s["u"])
+    banner(f"  fp32 vs fp64 u rel-L2 {rel:.2e}")
+    assert rel < 1e-5
+
+
+def test_rphiz_deposition_3d_matches_oracle():
+    """The dim = 3 R-phi-z binning with real z (fluka_heat.f:341-481 with the
+    bin pairs of the Cartesian branch), 10 steps."""
+    case = cases.dam_break3d_small(scale=0.1)
+    b = polar_binning(24, n=(5, 4, 4), origin=(0.01, 10.0, 0.02), cell=(0.012, 17.5, 0.02))
+    t_end = 5.5 * case["dt"]
+    banner(f"3D R-phi-z deposition fp64 n={case['n']}, 10 steps")
+    sim = armando_amd.Simulation(case)
+    sim.set_deposition(b, t_end)
+    sim.time_integration(1, 10)
+    s = sim.state()
+    sim.close()
+    P = orc.params_from(case)
+    orc.set_deposition(P, b["origin"], b["cell"], b["n"], b["data"], t_end, typ=1)
+    st = orc.OracleState3(case["n"], case["np"])
+    st.set(**{k: case[k] for k in ("x", "y", "z", "vx", "vy", "vz", "rho", "u")})
+    orc.run3(P, st, 1, 10)
+    for k in ("x", "y", "z", "rho", "u"):
+        e = relmax(s[k], st.field(k))
+        banner(f"  {k} relmax {e:.2e}")
+        assert e < 1e-10, k
+    heated = s["u"] > 1.0
+    assert heated.sum() > 20 and (~heated[:case["np"]]).sum() > 20
+
+
+def test_binning_type_errors():
+    case = cases.dam_break_small(scale=0.05)
+    sim = armando_amd.Simulation(case)
+    with pytest.raises(armando_amd.ArmandoError) as e:
+        sim.set_deposition(dict(polar_binning(25), type=2))
+    assert e.value.code == -5
+    sim.close()
