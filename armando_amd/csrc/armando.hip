@@ -79,8 +79,7 @@ struct arm_ctx {
   St3<double> A3_64{}, B3_64{};
   St3<float> A3_32{}, B3_32{};
   uint32_t* nbm3 = nullptr;  // 3D neighbour masks (M3_TOTAL words per particle)
-  void* pack3_vq = nullptr;  // 3D force pass: packed neighbour data (Pack3)
-  void* pack3_ap = nullptr;
+  void* pack3_vq = nullptr;  // 3D force pass: packed neighbour records (Pack3)
   void* d_cd = nullptr;      // ARM_CONTACT: contact displacement per slot (V2<R>)
   // sort workspace
   uint32_t *key_by_id = nullptr, *slot_by_id = nullptr;
@@ -1424,7 +1423,7 @@ template <typename R> int eos3_t(arm_ctx* c) {
 
 template <typename R> Pack3<R> pack3(arm_ctx* c) {
   using R4 = typename V4<R>::type;
-  return Pack3<R>{static_cast<R4*>(c->pack3_vq), static_cast<R4*>(c->pack3_ap)};
+  return Pack3<R>{static_cast<R4*>(c->pack3_vq)};
 }
 
 template <typename R, bool P, bool AV, bool AH, bool XS> void launch_force3(arm_ctx* c) {
@@ -2197,10 +2196,8 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
   if (!rc && is3d(c)) {
     const size_t r4 = is32(c) ? sizeof(float4) : sizeof(double4);
     double4* q = nullptr;
-    rc = dalloc(c, &q, (r4 * capacity + sizeof(double4) - 1) / sizeof(double4));
+    rc = dalloc(c, &q, (3 * r4 * capacity + sizeof(double4) - 1) / sizeof(double4));
     c->pack3_vq = q;
-    if (!rc) rc = dalloc(c, &q, (r4 * capacity + sizeof(double4) - 1) / sizeof(double4));
-    c->pack3_ap = q;
   }
   if (!rc) {
     e = hipMemset(c->d_err, 0, 4 * sizeof(int));

@@ -576,15 +576,20 @@ template <typename R> __device__ __forceinline__ R u3_of(const St3<R>& S, int t)
   if constexpr (sizeof(R) == 4) return S.uf[t]; else return (R)S.u[t];
 }
 
-// Neighbour data of the force pass packed per particle, so that a
-// neighbour is three 16-byte (FP32) loads: the position record of the state
-// plus vq = (v, Q = 1/rho) and ap = (A = p/rho, psi, u, 0), written by
-// k3_force_pack right before the gather with q_and_a's arithmetic.
+// Neighbour data of the force pass packed per particle into one record of
+// three vectors, so that a neighbour's loads fall in one or two cache lines:
+// the position record of the state (FP32: pf; FP64: xyz), (v, Q = 1/rho) and
+// (A = p/rho, psi, u, 0), written by k3_force_pack right before the gather
+// with q_and_a's arithmetic.
 template <typename R> struct Pack3 {
   using R4 = typename V4<R>::type;
-  R4* vq;
-  R4* ap;
+  R4* rec;  // 3 per particle
 };
+
+__device__ __forceinline__ float4 pos_rec3(const St3<float>& S, int i) { return S.pf[i]; }
+__device__ __forceinline__ double4 pos_rec3(const St3<double>& S, int i) { return S.xyz[i]; }
+__device__ __forceinline__ Pos3F pos_of_rec(float4 q) { return {q.w, q.x, q.y, q.z}; }
+__device__ __forceinline__ Pos3D pos_of_rec(double4 q) { return {q.x, q.y, q.z}; }
 
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k3_force_pack(St3<R> S, KParams K, Pack3<R> P) {
@@ -604,19 +609,21 @@ __global__ __launch_bounds__(BLOCK) void k3_force_pack(St3<R> S, KParams K, Pack
   ap.y = S.psi[i];
   ap.z = u3_of(S, i);
   ap.w = (R)0;
-  P.vq[i] = vq;
-  P.ap[i] = ap;
+  P.rec[3 * (size_t)i] = pos_rec3(S, i);
+  P.rec[3 * (size_t)i + 1] = vq;
+  P.rec[3 * (size_t)i + 2] = ap;
 }
 
 // One neighbour t of particle i in the direct path.
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, typename P3>
 __device__ __forceinline__ void force3_visit(const St3<R>& S, const Pack3<R>& P, const KParams& K,
                                              const P3& pi, R uy, R uz, ForceOwn3<R>& o, int t) {
-  const auto pj = load_pos3(S, t);
+  const auto* rp = P.rec + 3 * (size_t)t;
+  const auto pj = pos_of_rec(rp[0]);
+  const auto vq = rp[1];
+  const auto ap = rp[2];
   R dx, dy, dz;
   const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
-  const auto vq = P.vq[t];
-  const auto ap = P.ap[t];
   force3_pair<R, PRESS, AV, AH, XS>(K, o, dx, dy, dz, r2, vq.x, vq.y, vq.z, vq.w, ap.x, ap.y,
                                     ap.z);
 }
