@@ -2042,6 +2042,7 @@ void set_kparams(KParams& K, const arm_params& P) {
     K.f_cm = -8.0 * K.C1 * P.mass;
     K.f_xs = 0.5 * K.wnorm * K.eps_m;
   }
+  set_kf(K);
 }
 
 int need_sorted(arm_ctx* c) {

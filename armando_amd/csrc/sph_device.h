@@ -37,6 +37,12 @@ struct KParams {
   // FP32 neighbour pre-test: a bound on (2h)^2 for r^2 computed from
   // x = cx cell + off rounded once per particle (the exact test follows)
   float pre2f;
+  // FP32 roundings of the constants the FP32 pair bodies use, so that they
+  // stay in scalar registers (a conversion in the kernel lands in a vector
+  // register); each is (float) of the double field it is named after
+  struct {
+    float invh, h, cm, ava, avb, xs, ave2, ahe2, cond2, rho0, gx, gy, C1, mass, wnorm;
+  } kf;
   int eos;         // 0 Tait, 1 linear
   double lin_c2;   // linear EOS: p = lin_c2 rho + lin_p0
   double lin_p0;
@@ -102,6 +108,24 @@ __device__ __forceinline__ int xcd_block() {
 // within the deposition time (single_step.f:219); not a caller flag.
 constexpr int KF_DEPOSITION = 1 << 24;
 
+__host__ __device__ inline void set_kf(KParams& K) {
+  K.kf.invh = (float)K.invh;
+  K.kf.h = (float)K.h;
+  K.kf.cm = (float)K.f_cm;
+  K.kf.ava = (float)K.f_ava;
+  K.kf.avb = (float)K.f_avb;
+  K.kf.xs = (float)K.f_xs;
+  K.kf.ave2 = (float)K.av_e2;
+  K.kf.ahe2 = (float)K.ah_e2;
+  K.kf.cond2 = (float)K.f_cond2;
+  K.kf.rho0 = (float)K.rho0;
+  K.kf.gx = (float)K.gx;
+  K.kf.gy = (float)K.gy;
+  K.kf.C1 = (float)K.C1;
+  K.kf.mass = (float)K.mass;
+  K.kf.wnorm = (float)K.wnorm;
+}
+
 // K with the smoothing length mh: h, 1/h, the Wendland 2D constants, the
 // cut-off (2 mh)^2, the artificial viscosity and heat regularisations
 // (mh eta)^2 and the art_heat coefficient constants of mh.
@@ -119,6 +143,7 @@ __device__ __forceinline__ KParams with_h(const KParams& K, double mh) {
   P.g2h2 = K.ah_g2 * mh * mh;
   P.f_cm = -8.0 * P.C1 * K.mass;
   P.f_xs = 0.5 * P.wnorm * K.eps_m;
+  set_kf(P);
   return P;
 }
 
@@ -285,7 +310,7 @@ __device__ __forceinline__ double fmr(double a, double b, double c) { return a *
 
 // Density in the pair arithmetic: absolute (FP64 form) or rho0 + perturbation.
 __device__ __forceinline__ double dens(const KParams&, double r) { return r; }
-__device__ __forceinline__ float dens(const KParams& K, float r) { return (float)K.rho0 + r; }
+__device__ __forceinline__ float dens(const KParams& K, float r) { return K.kf.rho0 + r; }
 
 // Compensated (Kahan) accumulator for the FP32 form; plain for FP64.
 template <typename R> struct Acc {

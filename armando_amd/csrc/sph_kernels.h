@@ -567,10 +567,10 @@ __device__ __forceinline__ void force_pair_f32(const KParams& K, ForceOwn<float>
   const bool pa0 = PC < 0 ? K.pa_sph == 0 : (PC & 1) != 0;
   const bool se = PC < 0 ? K.same_e2 != 0 : (PC & 2) != 0;
   const float r2 = __builtin_fmaf(dx, dx, dy * dy);
-  const float q = fsqrt(r2) * (float)K.invh;
+  const float q = fsqrt(r2) * K.kf.invh;
   const float s = __builtin_fmaf(-0.5f, q, 1.0f);
   const float s2 = s * s;
-  const float mF = (float)K.f_cm * (s2 * s);
+  const float mF = K.kf.cm * (s2 * s);
   const float dvx = o.v.x - vjx, dvy = o.v.y - vjy;
   const float vr = __builtin_fmaf(dvy, dy, dvx * dx);
   const float imr2 = o.q4 * Qj * frcp(o.Q + Qj);
@@ -578,9 +578,9 @@ __device__ __forceinline__ void force_pair_f32(const KParams& K, ForceOwn<float>
   if (PRESS) coef = pa0 ? __builtin_fmaf(o.A, Qj, Aj * o.Q) : __builtin_fmaf(o.A, o.Q, Aj * Qj);
   float inv_ea = 0.f;
   if (AV) {
-    inv_ea = frcp(r2 + (float)K.av_e2);
-    const float phi = __builtin_fminf(vr, 0.f) * ((float)K.h * inv_ea);
-    const float pc = __builtin_fmaf((float)K.f_avb, phi, (float)K.f_ava);
+    inv_ea = frcp(r2 + K.kf.ave2);
+    const float phi = __builtin_fminf(vr, 0.f) * (K.kf.h * inv_ea);
+    const float pc = __builtin_fmaf(K.kf.avb, phi, K.kf.ava);
     coef = __builtin_fmaf(pc * phi, imr2, coef);
   }
   if (VI) visc_pair(K, o, mF, dx, dy, Qj, wj);
@@ -591,20 +591,20 @@ __device__ __forceinline__ void force_pair_f32(const KParams& K, ForceOwn<float>
     float d = G * vr;
     float inv_eh = 0.f;
     if (AH) {
-      inv_eh = (AV && se) ? inv_ea : frcp(r2 + (float)K.ah_e2);
+      inv_eh = (AV && se) ? inv_ea : frcp(r2 + K.kf.ahe2);
       d = __builtin_fmaf((o.psi + psij) * mF * (r2 * inv_eh) * imr2, o.u - uj, d);
     }
     if (PRESS && PC < 0 && (K.flags & 128)) {  // Cleary-Monaghan conduction
-      if (!AH) inv_eh = frcp(r2 + (float)K.ah_e2);
-      d = __builtin_fmaf((float)K.f_cond2 * o.Q * Qj * mF * (r2 * inv_eh), o.u - uj, d);
+      if (!AH) inv_eh = frcp(r2 + K.kf.ahe2);
+      d = __builtin_fmaf(K.kf.cond2 * o.Q * Qj * mF * (r2 * inv_eh), o.u - uj, d);
     }
     o.du.add(d);
   } else if (AH) {
-    const float inv_eh = frcp(r2 + (float)K.ah_e2);
+    const float inv_eh = frcp(r2 + K.kf.ahe2);
     o.du.add((o.psi + psij) * mF * (r2 * inv_eh) * imr2 * (o.u - uj));
   }
   if (XS) {
-    const float wf = (float)K.f_xs * ((s2 * s2) * __builtin_fmaf(2.f, q, 1.f)) * imr2;
+    const float wf = K.kf.xs * ((s2 * s2) * __builtin_fmaf(2.f, q, 1.f)) * imr2;
     o.xsx = __builtin_fmaf(-wf, dvx, o.xsx);
     o.xsy = __builtin_fmaf(-wf, dvy, o.xsy);
   }
@@ -673,7 +673,15 @@ __device__ __forceinline__ void force_end(St<R>& S, const KParams& K, int i, con
   if (PRESS) {
     const int id = S.id[i];
     const bool grav = (K.flags & 16) && id < K.np;
-    const R ex = grav ? (R)K.gx : (R)0, ey = grav ? (R)K.gy : (R)0;
+    R gx, gy;
+    if constexpr (sizeof(R) == 4) {
+      gx = K.kf.gx;
+      gy = K.kf.gy;
+    } else {
+      gx = (R)K.gx;
+      gy = (R)K.gy;
+    }
+    const R ex = grav ? gx : (R)0, ey = grav ? gy : (R)0;
     R2 acc;
     acc.x = o.fx + ex;
     acc.y = o.fy + ey;
