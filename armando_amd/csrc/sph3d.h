@@ -618,7 +618,7 @@ __global__ __launch_bounds__(BLOCK) void k3_force_pack(St3<R> S, KParams K, Pack
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, typename P3>
 __device__ __forceinline__ void force3_visit(const St3<R>& S, const Pack3<R>& P, const KParams& K,
                                              const P3& pi, R uy, R uz, ForceOwn3<R>& o, int t) {
-  const auto* rp = P.rec + 3 * (size_t)t;
+  const auto* rp = P.rec + (((unsigned)t << 1) + (unsigned)t);  // 3 t, no 64-bit multiply
   const auto pj = pos_of_rec(rp[0]);
   const auto vq = rp[1];
   const auto ap = rp[2];
@@ -656,23 +656,27 @@ __device__ __forceinline__ void force3_direct(St3<R>& S, const Pack3<R>& P, cons
 #pragma unroll
     for (int k = 0; k < 9; ++k) lt[k][lane] = rw.t0[k];
     if (nz) {
-      int w = __builtin_ctz(nz);
-      nz &= nz - 1u;
-      uint32_t m = lw[w][lane];
+      // the word's first candidate and its row's y and z terms are found
+      // when the walk enters the word, and kept until it leaves it
+      uint32_t m = 0u;
+      int tb = 0;
+      R uy = (R)0, uz = (R)0;
       while (true) {
-        const int b = __builtin_ctz(m);
-        m &= m - 1u;
-        const int k = (w * 11) >> 5;  // w / 3 for w < 27
-        const int q = w - 3 * k;
-        const int ky = (k * 11) >> 5;  // k / 3 for k < 9
-        const R uy = row_u3<R>(K, 1 - (k - 3 * ky)), uz = row_u3<R>(K, 1 - ky);
-        force3_visit<R, PRESS, AV, AH, XS>(S, P, K, pi, uy, uz, o, lt[k][lane] + 32 * q + b);
         if (m == 0u) {
           if (nz == 0u) break;
-          w = __builtin_ctz(nz);
+          const int w = __builtin_ctz(nz);
           nz &= nz - 1u;
           m = lw[w][lane];
+          const int k = (w * 11) >> 5;  // w / 3 for w < 27
+          const int q = w - 3 * k;
+          const int ky = (k * 11) >> 5;  // k / 3 for k < 9
+          uy = row_u3<R>(K, 1 - (k - 3 * ky));
+          uz = row_u3<R>(K, 1 - ky);
+          tb = lt[k][lane] + 32 * q;
         }
+        const int b = __builtin_ctz(m);
+        m &= m - 1u;
+        force3_visit<R, PRESS, AV, AH, XS>(S, P, K, pi, uy, uz, o, tb + b);
       }
     }
   } else {
