@@ -92,7 +92,7 @@ struct arm_ctx {
   int cs_tiles = 0;
   int* cnt = nullptr;       // per-cell counters, zero between sorts
   int* tsum = nullptr;      // per-tile sums of the counters
-  uint64_t* ent = nullptr;  // (id, slot) entries grouped by cell (cell in sk[1])
+  uint64_t* ent = nullptr;  // entries grouped by cell: cap ids then cap slots (ent_of); cells in sk[1]
   uint32_t* skey = nullptr;  // sorted keys of the current order
   int* cstart = nullptr;
   // d_err[0]: particles outside the grid, [1]: h outgrew the cell, [2]: ghost
@@ -251,6 +251,13 @@ void zero_counts(arm_ctx* c) {
 }
 
 // Sorts the first n slots (c->n when n < 0).
+// The entry arrays of the counting sort: ids then slots, capacity each, in
+// the context's 8-byte-per-particle buffer.
+Ent ent_of(arm_ctx* c) {
+  uint32_t* p = reinterpret_cast<uint32_t*>(c->ent);
+  return Ent{p, p + c->cap};
+}
+
 int run_sort(arm_ctx* c, const int32_t* id_of, const uint32_t** sval_out, bool* ranges_done,
              int n_sort = -1) {
   const int n = n_sort < 0 ? c->n : n_sort;
@@ -263,8 +270,8 @@ int run_sort(arm_ctx* c, const int32_t* id_of, const uint32_t** sval_out, bool* 
     hipLaunchKernelGGL(k_cs_tile_scan, dim3(c->cs_tiles), dim3(CS_THREADS), 0, c->stream, c->cnt,
                        ncell, n, c->tsum, c->cstart, hl);
     hipLaunchKernelGGL(k_cs_scatter, dim3(nb), dim3(CS_THREADS), 0, c->stream, c->key_by_id,
-                       id_of, n, c->cstart, c->cnt, c->ent, c->sk[1], hl);
-    hipLaunchKernelGGL(k_cs_rank, dim3(nb), dim3(CS_THREADS), 0, c->stream, c->ent, c->sk[1], n,
+                       id_of, n, c->cstart, c->cnt, ent_of(c), c->sk[1], hl);
+    hipLaunchKernelGGL(k_cs_rank, dim3(nb), dim3(CS_THREADS), 0, c->stream, ent_of(c), c->sk[1], n,
                        c->cstart, c->sk[0], c->sv[0], c->ghosts ? c->K.ncell : -1, hl);
     HIPCHK(c, hipGetLastError());
     c->skey = c->sk[0];
@@ -2833,7 +2840,8 @@ extern "C" int arm_link_list_static(const arm_params* prm, int n, const double* 
   int* count = reinterpret_cast<int*>(base + ST_COUNT);
   int* tsum = reinterpret_cast<int*>(base + ST_TSUM);
   uint32_t* key = reinterpret_cast<uint32_t*>(base + ST_KEY);
-  uint64_t* ent = reinterpret_cast<uint64_t*>(base + ST_ENT);
+  uint32_t* ent_id = reinterpret_cast<uint32_t*>(base + ST_ENT);
+  const Ent ent{ent_id, ent_id + ST_N};
   uint32_t* entc = reinterpret_cast<uint32_t*>(base + ST_ENTC);
   int* out = outside ? outside : reinterpret_cast<int*>(base + ST_OUT);
   hipStream_t s = static_cast<hipStream_t>(stream);

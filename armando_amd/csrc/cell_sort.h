@@ -9,7 +9,7 @@
 //   k_cs_tile_sum   sums of CS_TILE consecutive counts
 //   k_cs_tile_scan  exclusive scan of the counts -> cell_start[0 .. ncell]
 //                   (each workgroup adds up the tile sums before its own)
-//   k_cs_scatter    every particle takes a slot of its cell range (count[key]
+//   k_cs_scatter    every particle takes a place in its cell range (count[key]
 //                   -= run restores the counters to zero); the order inside a
 //                   cell is arbitrary here
 //   k_cs_rank       every particle counts the ids of its cell below its own:
@@ -197,13 +197,19 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_tile_scan(const int* __restri
   if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
 }
 
-// entry = id << 32 | slot; entc = its cell.  Particles in slot order; with
-// id_of null the particle in slot t has id t.
+// The scattered entries: id and slot of the particle at each position of
+// its cell range (two arrays, so that the rank's loop reads 4 bytes per
+// entry); entc = its cell.
+struct Ent {
+  uint32_t* id;
+  uint32_t* slot;
+};
+
+// Particles in slot order; with id_of null the particle in slot t has id t.
 __global__ __launch_bounds__(CS_THREADS) void k_cs_scatter(const uint32_t* __restrict__ key_by_slot,
                                                            const int32_t* __restrict__ id_of,
                                                            int n, const int* __restrict__ cstart,
-                                                           int* __restrict__ count,
-                                                           uint64_t* __restrict__ ent,
+                                                           int* __restrict__ count, Ent ent,
                                                            uint32_t* __restrict__ entc, Halt hl) {
   if (hl.on()) return;
   const int t = blockIdx.x * CS_THREADS + threadIdx.x;
@@ -215,16 +221,15 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_scatter(const uint32_t* __res
   old = __shfl(old, r.hl, 64);
   if (!valid) return;
   const int pos = cstart[c] + old - 1 - r.rank;
-  const uint32_t id = id_of ? (uint32_t)id_of[t] : (uint32_t)t;
-  ent[pos] = ((uint64_t)id << 32) | (uint64_t)(uint32_t)t;
+  ent.id[pos] = id_of ? (uint32_t)id_of[t] : (uint32_t)t;
+  ent.slot[pos] = (uint32_t)t;
   entc[pos] = c;
 }
 
 // skip: a cell whose order does not matter (the extra cell of stale ghosts,
 // or -1): its entries keep their scatter positions.
-__global__ __launch_bounds__(CS_THREADS) void k_cs_rank(const uint64_t* __restrict__ ent,
-                                                        const uint32_t* __restrict__ entc, int n,
-                                                        const int* __restrict__ cstart,
+__global__ __launch_bounds__(CS_THREADS) void k_cs_rank(Ent ent, const uint32_t* __restrict__ entc,
+                                                        int n, const int* __restrict__ cstart,
                                                         uint32_t* __restrict__ skey,
                                                         uint32_t* __restrict__ sval, int skip,
                                                         Halt hl) {
@@ -232,18 +237,19 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_rank(const uint64_t* __restri
   const int p = blockIdx.x * CS_THREADS + threadIdx.x;
   if (p >= n) return;
   const uint32_t c = entc[p];
-  const uint64_t e = ent[p];
+  const uint32_t slot = ent.slot[p];
   if ((int)c == skip) {
     skey[p] = c;
-    sval[p] = (uint32_t)e;
+    sval[p] = slot;
     return;
   }
   const int a = cstart[c], b = cstart[c + 1];
-  const uint32_t id = (uint32_t)(e >> 32);
+  const uint32_t id = ent.id[p];
+  const uint32_t* __restrict__ ids = ent.id;
   int rank = 0;
-  for (int q = a; q < b; ++q) rank += (uint32_t)(ent[q] >> 32) < id ? 1 : 0;
+  for (int q = a; q < b; ++q) rank += ids[q] < id ? 1 : 0;
   skey[a + rank] = c;
-  sval[a + rank] = (uint32_t)e;
+  sval[a + rank] = slot;
 }
 
 }  // namespace arm
