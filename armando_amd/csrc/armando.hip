@@ -64,6 +64,10 @@ struct arm_ctx {
   double grads[ARM_NGRAD] = {0};   // parameter gradients of the last backward
   bool masks_valid = false;  // nbm holds the neighbour masks of the current order
   uint4* nbm = nullptr;
+  // FP32 2D tiled gathers: the row pieces of the current order (k_row_pieces)
+  int2* pieces = nullptr;
+  int* npieces = nullptr;
+  bool pieces_valid = false;
   // deposition heat source (arm_set_deposition): the map on the device, the
   // deposition time and the number of the step the next force pass belongs
   // to (its of single_step.f:219)
@@ -258,6 +262,10 @@ Ent ent_of(arm_ctx* c) {
   return Ent{p, p + c->cap};
 }
 
+RowPieces row_pieces_of(const arm_ctx* c) {
+  return c->pieces ? RowPieces{c->pieces, c->npieces, c->K.nx, c->K.ny} : RowPieces{};
+}
+
 int run_sort(arm_ctx* c, const int32_t* id_of, const uint32_t** sval_out, bool* ranges_done,
              int n_sort = -1) {
   const int n = n_sort < 0 ? c->n : n_sort;
@@ -271,15 +279,17 @@ int run_sort(arm_ctx* c, const int32_t* id_of, const uint32_t** sval_out, bool* 
                        ncell, n, c->tsum, c->cstart, hl);
     hipLaunchKernelGGL(k_cs_scatter, dim3(nb), dim3(CS_THREADS), 0, c->stream, c->key_by_id,
                        id_of, n, c->cstart, c->cnt, ent_of(c), c->sk[1], hl);
-    hipLaunchKernelGGL(k_cs_rank, dim3(nb), dim3(CS_THREADS), 0, c->stream, ent_of(c), c->sk[1], n,
-                       c->cstart, c->sk[0], c->sv[0], c->ghosts ? c->K.ncell : -1, hl);
+    hipLaunchKernelGGL(k_cs_rank, dim3(nb), dim3(CS_THREADS), 0, c->stream, ent_of(c), c->sk[1],
+                       n, c->cstart, c->sk[0], c->sv[0], c->ghosts ? c->K.ncell : -1, hl);
     HIPCHK(c, hipGetLastError());
     c->skey = c->sk[0];
     *sval_out = c->sv[0];
     *ranges_done = true;
+    c->pieces_valid = false;
     return ARM_OK;
   }
   *ranges_done = false;
+  c->pieces_valid = false;
   const int ntiles = (n + SORT_TILE - 1) / SORT_TILE;
   const uint32_t* kin = c->key_by_id;
   const uint32_t* vin = c->slot_by_id;
@@ -399,8 +409,12 @@ template <typename R> int link_list_t(arm_ctx* c) {
   c->have_ranges = true;
   {
     PassTimer tm(c, ARM_PASS_REORDER);
-    hipLaunchKernelGGL(k_reorder_all<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, B,
-                       c->K, c->skey, sval, c->cstart, ranges ? 0 : 1, c->G);
+    // with the sort's cell ranges the kernel also adds the row pieces
+    const RowPieces rp = ranges ? row_pieces_of(c) : RowPieces{};
+    hipLaunchKernelGGL(k_reorder_all<R>, dim3(grid_of(c->n) + row_blocks(rp)),
+                       dim3(BLOCK), 0, c->stream, A, B, c->K, c->skey, sval, c->cstart,
+                       ranges ? 0 : 1, c->G, rp);
+    c->pieces_valid = rp.pieces != nullptr;
   }
   HIPCHK(c, hipGetLastError());
   std::swap(A, B);
@@ -409,23 +423,41 @@ template <typename R> int link_list_t(arm_ctx* c) {
   return check_domain(c);
 }
 
-template <typename R> int nseg_of(const arm_ctx* c) {
-  return (c->K.nx + TileCfg<R>::CW - 1) / TileCfg<R>::CW;
+// Where the tiled gathers' workgroups find their segments, and how many
+// workgroups they take: row pieces (CW = 0; the piece table is rebuilt after
+// each sort, and the grid is the bound n / PIECE + ny on the piece count) or
+// runs of CW cells of every row.
+template <int CW> SegArg seg_arg(arm_ctx* c) {
+  if constexpr (CW > 0) {
+    return SegArg{(c->K.nx + CW - 1) / CW, nullptr, nullptr};
+  } else {
+    if (!c->pieces_valid) {
+      const RowPieces rp = row_pieces_of(c);
+      hipLaunchKernelGGL(k_row_pieces, dim3(row_blocks(rp)), dim3(BLOCK), 0, c->stream,
+                         c->cstart, rp, Halt{c->K.halt, c->K.its});
+      c->pieces_valid = true;
+    }
+    return SegArg{0, c->pieces, c->npieces};
+  }
+}
+template <int CW> int seg_grid(const arm_ctx* c, const SegArg& sa) {
+  return CW > 0 ? c->K.ny * sa.nseg : (c->n + PIECE - 1) / PIECE + c->K.ny;
 }
 
 template <typename R, bool SUM, bool EOS> void launch_density(arm_ctx* c) {
   St<R>& A = stA<R>(c);
   using T = TileCfg<R>;
   if (c->tiled && !c->K.sle) {
-    const int nseg = nseg_of<R>(c);
+    const SegArg sa = seg_arg<T::CW>(c);
+    const int grid = seg_grid<T::CW>(c, sa);
     if (c->K.periodic)
       hipLaunchKernelGGL((k_density_tiled<R, SUM, EOS, true, T::BLK, T::CW, T::CAP>),
-                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
-                         c->cstart, nseg, c->nbm);
+                         dim3(grid), dim3(T::BLK), 0, c->stream, A, c->K, c->skey, c->cstart, sa,
+                         c->nbm);
     else
       hipLaunchKernelGGL((k_density_tiled<R, SUM, EOS, false, T::BLK, T::CW, T::CAP>),
-                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
-                         c->cstart, nseg, c->nbm);
+                         dim3(grid), dim3(T::BLK), 0, c->stream, A, c->K, c->skey, c->cstart, sa,
+                         c->nbm);
   } else {
     hipLaunchKernelGGL((k_density<R, SUM, EOS>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
                        c->stream, A, c->K, c->skey, c->cstart, c->nbm);
@@ -466,11 +498,11 @@ template <typename R> int eos_t(arm_ctx* c) {
 // viscosity and heat, XSPH) with pa_sph and same_e2 fixed at compile time.
 template <int PC> void launch_force_hot(arm_ctx* c, const uint4* nbm) {
   using T = HotForceCfg;
-  const int nseg = (c->K.nx + T::CW - 1) / T::CW;
+  const SegArg sa = seg_arg<T::CW>(c);
   hipLaunchKernelGGL(
       (k_force_tiled<float, true, true, true, true, false, false, T::BLK, T::CW, T::CAP, PC>),
-      dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, stA<float>(c), c->K, c->skey, c->cstart,
-      nseg, nbm);
+      dim3(seg_grid<T::CW>(c, sa)), dim3(T::BLK), 0, c->stream, stA<float>(c), c->K, c->skey,
+      c->cstart, sa, nbm);
 }
 
 template <typename R, bool P, bool AV, bool AH, bool XS, bool VI = false>
@@ -478,7 +510,6 @@ void launch_force(arm_ctx* c) {
   St<R>& A = stA<R>(c);
   using T = TileCfg<R>;
   if (c->tiled && !c->K.sle) {
-    const int nseg = nseg_of<R>(c);
     const uint4* nbm = c->masks_valid ? c->nbm : nullptr;
     if (sizeof(R) == 4 && P && AV && AH && XS && !VI && !c->K.periodic &&
         !(c->K.flags & ARM_CONDUCTION)) {
@@ -488,14 +519,17 @@ void launch_force(arm_ctx* c) {
         case 2: launch_force_hot<2>(c, nbm); break;
         default: launch_force_hot<3>(c, nbm); break;
       }
-    } else if (c->K.periodic) {
-      hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, true, T::BLK, T::CW, T::CAP>),
-                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
-                         c->cstart, nseg, nbm);
     } else {
-      hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, false, T::BLK, T::CW, T::CAP>),
-                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
-                         c->cstart, nseg, nbm);
+      const SegArg sa = seg_arg<T::CW>(c);
+      const int grid = seg_grid<T::CW>(c, sa);
+      if (c->K.periodic)
+        hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, true, T::BLK, T::CW, T::CAP>),
+                           dim3(grid), dim3(T::BLK), 0, c->stream, A, c->K, c->skey, c->cstart,
+                           sa, nbm);
+      else
+        hipLaunchKernelGGL((k_force_tiled<R, P, AV, AH, XS, VI, false, T::BLK, T::CW, T::CAP>),
+                           dim3(grid), dim3(T::BLK), 0, c->stream, A, c->K, c->skey, c->cstart,
+                           sa, nbm);
     }
   } else {
     hipLaunchKernelGGL((k_force<R, P, AV, AH, XS, VI>), dim3(grid_of(c->n)), dim3(BLOCK), 0,
@@ -508,16 +542,15 @@ template <typename R> void launch_tensor(arm_ctx* c) {
   St<R>& A = stA<R>(c);
   using T = TileCfg<R>;
   if (c->tiled && !c->K.sle) {
-    const int nseg = nseg_of<R>(c);
     const uint4* nbm = c->masks_valid ? c->nbm : nullptr;
+    const SegArg sa = seg_arg<T::CW>(c);
+    const int grid = seg_grid<T::CW>(c, sa);
     if (c->K.periodic)
-      hipLaunchKernelGGL((k_tensor_tiled<R, true, T::BLK, T::CW, T::CAP>),
-                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
-                         c->cstart, nseg, nbm);
+      hipLaunchKernelGGL((k_tensor_tiled<R, true, T::BLK, T::CW, T::CAP>), dim3(grid),
+                         dim3(T::BLK), 0, c->stream, A, c->K, c->skey, c->cstart, sa, nbm);
     else
-      hipLaunchKernelGGL((k_tensor_tiled<R, false, T::BLK, T::CW, T::CAP>),
-                         dim3(c->K.ny * nseg), dim3(T::BLK), 0, c->stream, A, c->K, c->skey,
-                         c->cstart, nseg, nbm);
+      hipLaunchKernelGGL((k_tensor_tiled<R, false, T::BLK, T::CW, T::CAP>), dim3(grid),
+                         dim3(T::BLK), 0, c->stream, A, c->K, c->skey, c->cstart, sa, nbm);
   } else {
     hipLaunchKernelGGL(k_tensor<R>, dim3(grid_of(c->n)), dim3(BLOCK), 0, c->stream, A, c->K,
                        c->skey, c->cstart);
@@ -725,8 +758,12 @@ template <typename R> int fused_step(arm_ctx* c, int its, const StepHooks& hk) {
     St<R>& A = stA<R>(c);
     St<R>& B = stB<R>(c);
     PassTimer tm(c, ARM_PASS_REORDER);
-    hipLaunchKernelGGL(k_reorder_pred<R>, dim3(grid_of(n)), dim3(BLOCK), 0, c->stream, A, B,
-                       c->K, c->skey, sval, c->cstart, its != 1 ? 1 : 0, ranges ? 0 : 1, c->G);
+    // with the sort's cell ranges the kernel also adds the row pieces
+    const RowPieces rp = ranges ? row_pieces_of(c) : RowPieces{};
+    hipLaunchKernelGGL(k_reorder_pred<R>, dim3(grid_of(n) + row_blocks(rp)),
+                       dim3(BLOCK), 0, c->stream, A, B, c->K, c->skey, sval, c->cstart,
+                       its != 1 ? 1 : 0, ranges ? 0 : 1, c->G, rp);
+    c->pieces_valid = rp.pieces != nullptr;
     std::swap(A, B);
     c->masks_valid = false;
   }
@@ -2200,6 +2237,10 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
   if (!rc) rc = dalloc(c, &c->d_part, (size_t)grid_of(capacity) * 6);
   if (!rc) rc = dalloc(c, &c->d_cnt, capacity);
   if (!rc) rc = dalloc(c, &c->nbm, capacity);
+  if (!rc && !is3d(c) && is32(c)) {
+    rc = dalloc(c, &c->pieces, (size_t)capacity / PIECE + c->K.ny + 2);
+    if (!rc) rc = dalloc(c, &c->npieces, 1);
+  }
   if (!rc && is3d(c)) rc = dalloc(c, &c->nbm3, (size_t)M3_TOTAL * capacity);
   if (!rc && is3d(c)) {
     const size_t r4 = is32(c) ? sizeof(float4) : sizeof(double4);

@@ -226,6 +226,44 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_scatter(const uint32_t* __res
   entc[pos] = c;
 }
 
+// Pieces of the cell rows, for the gathers that give each workgroup a run of
+// consecutive sorted positions of one row: the sorted range of row y,
+// [cstart[y nx], cstart[(y+1) nx]), cut into runs of PIECE positions from its
+// start (the last one shorter); pieces[s] = (first position, row), in row
+// order, and *npieces their count, at most n / PIECE + ny.
+constexpr int PIECE = 128;
+struct RowPieces {
+  int2* pieces = nullptr;
+  int* npieces = nullptr;
+  int nx = 0, ny = 0;
+};
+
+// Workgroup wg of a grid of ceil(ny / THREADS) lists the pieces of rows
+// wg THREADS .. wg THREADS + THREADS-1, one lane per row.  It counts the
+// pieces of the rows before its own itself (as k_cs_tile_scan adds up the
+// tile sums before its own), so the table needs neither atomics nor a
+// cleared counter; the workgroup of the last row writes *npieces.
+template <int THREADS>
+__device__ __forceinline__ void row_pieces_block(const int* __restrict__ cstart,
+                                                 const RowPieces& rp, int wg) {
+  __shared__ int wsum[THREADS / 64];
+  const int nx = rp.nx;
+  const int y0 = wg * THREADS;
+  int before = 0;
+  for (int y = threadIdx.x; y < y0; y += THREADS)
+    before += (cstart[(y + 1) * nx] - cstart[y * nx] + PIECE - 1) / PIECE;
+  int base;
+  block_exclusive_scan<THREADS>(before, wsum, &base);
+  __syncthreads();  // wsum is reused
+  const int y = y0 + (int)threadIdx.x;
+  const int a = y < rp.ny ? cstart[y * nx] : 0;
+  const int b = y < rp.ny ? cstart[(y + 1) * nx] : 0;
+  int total;
+  int s = base + block_exclusive_scan<THREADS>((b - a + PIECE - 1) / PIECE, wsum, &total);
+  for (int p = a; p < b; p += PIECE) rp.pieces[s++] = make_int2(p, y);
+  if (threadIdx.x == 0 && y0 + THREADS >= rp.ny) *rp.npieces = base + total;
+}
+
 // skip: a cell whose order does not matter (the extra cell of stale ghosts,
 // or -1): its entries keep their scatter positions.
 __global__ __launch_bounds__(CS_THREADS) void k_cs_rank(Ent ent, const uint32_t* __restrict__ entc,

@@ -104,6 +104,15 @@ __device__ __forceinline__ int xcd_block() {
   return x * q + (x < r ? x : r) + b / NXCD;
 }
 
+// The same for the first `count` logical blocks of a grid of at least count
+// workgroups: XCD x covers one contiguous share of 0 .. count-1, the shares
+// as even as the count allows; -1 for the workgroups past them.
+__device__ __forceinline__ int xcd_share(int count) {
+  const int b = (int)blockIdx.x;
+  const int q = count / NXCD, r = count - q * NXCD, x = b - (b / NXCD) * NXCD, j = b / NXCD;
+  return j < q + (x < r ? 1 : 0) ? x * q + (x < r ? x : r) + j : -1;
+}
+
 // Set in KParams::flags by the host for the steps whose time its dt lies
 // within the deposition time (single_step.f:219); not a caller flag.
 constexpr int KF_DEPOSITION = 1 << 24;

@@ -185,6 +185,25 @@ __device__ __forceinline__ void ghost_slot(const St<R>& A, St<R>& Bs, const KPar
   G.gpid[t] = A.id[ps];
 }
 
+// A reorder kernel launched with row pieces (rp.pieces set; cell_start
+// written by the sort) has row_blocks(rp) extra workgroups in front, one
+// lane per cell row, which list the rows' pieces (they run beside the first
+// particles' workgroups).  Returns true in those; *t is the sorted position
+// of the lane in the others.
+__host__ __device__ inline int row_blocks(const RowPieces& rp) {
+  return rp.pieces ? (rp.ny + BLOCK - 1) / BLOCK : 0;
+}
+__device__ __forceinline__ bool reorder_row_pieces(const int* __restrict__ cstart,
+                                                   const RowPieces& rp, int* t) {
+  const int rb = row_blocks(rp);
+  if ((int)blockIdx.x < rb) {
+    row_pieces_block<BLOCK>(cstart, rp, blockIdx.x);
+    return true;
+  }
+  *t = ((int)blockIdx.x - rb) * BLOCK + threadIdx.x;
+  return false;
+}
+
 // Permute every field from state A to state B (per-routine link_list);
 // fill = 0 when the sort already wrote cell_start.
 template <typename R>
@@ -192,8 +211,9 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_all(St<R> A, St<R> Bs, KParam
                                                        const uint32_t* __restrict__ skey,
                                                        const uint32_t* __restrict__ sval,
                                                        int* __restrict__ cstart, int fill,
-                                                       Ghosts G) {
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
+                                                       Ghosts G, RowPieces rp) {
+  int t;
+  if (reorder_row_pieces(cstart, rp, &t)) return;
   const bool valid = t < K.n;
   if (fill) fill_cell_start(skey, t, valid, K.n, K.ncell, cstart);
   if (!valid) return;
@@ -238,10 +258,11 @@ __global__ __launch_bounds__(BLOCK) void k_reorder_pred(St<R> A, St<R> Bs, KPara
                                                         const uint32_t* __restrict__ skey,
                                                         const uint32_t* __restrict__ sval,
                                                         int* __restrict__ cstart, int predict,
-                                                        int fill, Ghosts G) {
+                                                        int fill, Ghosts G, RowPieces rp) {
   if (halted(K)) return;
+  int t;
+  if (reorder_row_pieces(cstart, rp, &t)) return;
   using R2 = typename V2<R>::type;
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
   const bool valid = t < K.n;
   if (fill) fill_cell_start(skey, t, valid, K.n, K.ncell, cstart);
   if (!valid) return;
@@ -559,19 +580,31 @@ __device__ __forceinline__ void visc_pair(const KParams& K, ForceOwn<R>& o, R mF
 // PC < 0 reads pa_sph and same_e2 from K; PC >= 0 fixes them at compile time
 // (bit 0: pa_sph == 0, bit 1: same_e2) with the same expressions, so both give
 // identical bits, and leaves out conduction (its callers run without it).
-template <bool PRESS, bool AV, bool AH, bool XS, bool VI, int PC>
-__device__ __forceinline__ void force_pair_f32(const KParams& K, ForceOwn<float>& o, float dx,
-                                               float dy, float vjx, float vjy, float Qj,
-                                               float Aj, float psij, float uj,
-                                               const Tau3<float>& wj) {
+// The pair terms apart from their accumulation (force_acc_f32): the particle
+// gains -G dx, -G dy, the doubled energy rate d and -wf dv.  mF: for the
+// physical viscosity, which force_pair_f32 adds before them.
+struct TermsF {
+  float mF, G, dx, dy, d, wf, dvx, dvy;
+};
+
+template <bool PRESS, bool AV, bool AH, bool XS, int PC>
+__device__ __forceinline__ TermsF force_terms_f32(const KParams& K, const ForceOwn<float>& o,
+                                                  float dx, float dy, float vjx, float vjy,
+                                                  float Qj, float Aj, float psij, float uj) {
   const bool pa0 = PC < 0 ? K.pa_sph == 0 : (PC & 1) != 0;
   const bool se = PC < 0 ? K.same_e2 != 0 : (PC & 2) != 0;
+  TermsF t;
+  t.dx = dx;
+  t.dy = dy;
   const float r2 = __builtin_fmaf(dx, dx, dy * dy);
   const float q = fsqrt(r2) * K.kf.invh;
   const float s = __builtin_fmaf(-0.5f, q, 1.0f);
   const float s2 = s * s;
   const float mF = K.kf.cm * (s2 * s);
+  t.mF = mF;
   const float dvx = o.v.x - vjx, dvy = o.v.y - vjy;
+  t.dvx = dvx;
+  t.dvy = dvy;
   const float vr = __builtin_fmaf(dvy, dy, dvx * dx);
   const float imr2 = o.q4 * Qj * frcp(o.Q + Qj);
   float coef = 0.f;
@@ -583,11 +616,11 @@ __device__ __forceinline__ void force_pair_f32(const KParams& K, ForceOwn<float>
     const float pc = __builtin_fmaf(K.kf.avb, phi, K.kf.ava);
     coef = __builtin_fmaf(pc * phi, imr2, coef);
   }
-  if (VI) visc_pair(K, o, mF, dx, dy, Qj, wj);
+  t.G = 0.f;
+  t.d = 0.f;
   if (PRESS || AV) {
     const float G = mF * coef;
-    o.fx = __builtin_fmaf(-G, dx, o.fx);
-    o.fy = __builtin_fmaf(-G, dy, o.fy);
+    t.G = G;
     float d = G * vr;
     float inv_eh = 0.f;
     if (AH) {
@@ -598,16 +631,39 @@ __device__ __forceinline__ void force_pair_f32(const KParams& K, ForceOwn<float>
       if (!AH) inv_eh = frcp(r2 + K.kf.ahe2);
       d = __builtin_fmaf(K.kf.cond2 * o.Q * Qj * mF * (r2 * inv_eh), o.u - uj, d);
     }
-    o.du.add(d);
+    t.d = d;
   } else if (AH) {
     const float inv_eh = frcp(r2 + K.kf.ahe2);
-    o.du.add((o.psi + psij) * mF * (r2 * inv_eh) * imr2 * (o.u - uj));
+    t.d = (o.psi + psij) * mF * (r2 * inv_eh) * imr2 * (o.u - uj);
   }
+  t.wf = 0.f;
+  if (XS) t.wf = K.kf.xs * ((s2 * s2) * __builtin_fmaf(2.f, q, 1.f)) * imr2;
+  return t;
+}
+
+// Adds the terms of one pair when ok (else nothing), without a branch.
+template <bool PRESS, bool AV, bool AH, bool XS>
+__device__ __forceinline__ void force_acc_f32(ForceOwn<float>& o, const TermsF& t, bool ok) {
+  if (PRESS || AV) {
+    o.fx = ok ? __builtin_fmaf(-t.G, t.dx, o.fx) : o.fx;
+    o.fy = ok ? __builtin_fmaf(-t.G, t.dy, o.fy) : o.fy;
+  }
+  if (PRESS || AV || AH) o.du.add_if(t.d, ok);
   if (XS) {
-    const float wf = K.kf.xs * ((s2 * s2) * __builtin_fmaf(2.f, q, 1.f)) * imr2;
-    o.xsx = __builtin_fmaf(-wf, dvx, o.xsx);
-    o.xsy = __builtin_fmaf(-wf, dvy, o.xsy);
+    o.xsx = ok ? __builtin_fmaf(-t.wf, t.dvx, o.xsx) : o.xsx;
+    o.xsy = ok ? __builtin_fmaf(-t.wf, t.dvy, o.xsy) : o.xsy;
   }
+}
+
+template <bool PRESS, bool AV, bool AH, bool XS, bool VI, int PC>
+__device__ __forceinline__ void force_pair_f32(const KParams& K, ForceOwn<float>& o, float dx,
+                                               float dy, float vjx, float vjy, float Qj,
+                                               float Aj, float psij, float uj,
+                                               const Tau3<float>& wj) {
+  const TermsF t =
+      force_terms_f32<PRESS, AV, AH, XS, PC>(K, o, dx, dy, vjx, vjy, Qj, Aj, psij, uj);
+  if (VI) visc_pair(K, o, t.mF, dx, dy, Qj, wj);
+  force_acc_f32<PRESS, AV, AH, XS>(o, t, true);
 }
 
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI = false, int PC = -1>

@@ -180,6 +180,16 @@ template <int CAP, bool VI> struct ForceTile<float, CAP, VI> {
     force_pair<float, PRESS, AV, AH, XS, VI, PC>(K, o, dx, dy, r2, q.z, q.w, w.x, w.y, w.z,
                                                  w.w, tw.get(l));
   }
+  // the terms of the pair with record l, unaccumulated (no physical viscosity)
+  template <bool PRESS, bool AV, bool AH, bool XS, int PC>
+  __device__ __forceinline__ TermsF terms(const KParams& K, const ForceOwn<float>& o,
+                                          const LaneF& p, int l) const {
+    float dx, dy;
+    delta_rec(K, p, l, dx, dy);
+    const float4 q = a[l];
+    const float4 w = b[l];
+    return force_terms_f32<PRESS, AV, AH, XS, PC>(K, o, dx, dy, q.z, q.w, w.x, w.y, w.z, w.w);
+  }
 };
 template <int CAP, bool VI> struct ForceTile<double, CAP, VI> {
   double2 x[CAP + TPAD], v[CAP + TPAD], rp[CAP + TPAD], pu[CAP + TPAD];  // rp = (Q, A), pu = (psi, u)
@@ -235,23 +245,87 @@ template <typename R, int CAP> struct TensTile {
 };
 
 // ---- segment geometry ------------------------------------------------------
-template <int CW> struct Segment {
+// A segment: the particles [O0, O1) of cell row cy, which lie in cells
+// slo .. shi of that row; the tile holds the rows cy-1 .. cy+1 over those
+// cells (rs[k], rl[k]: sorted start and length of row k's range, lo[k]: its
+// first tile entry).
+struct Segment {
   int cy, slo, O0, O1;
   int rs[3], rl[3], lo[3];
   int total;
 };
 
-// cs[k][j] = cell_start of cell (slo + j) in row cy-1+k, j = 0 .. CW+2
+// How a workgroup finds its segment.
+//  CW > 0: a fixed run of CW cells of a row (segment row * nseg + run).
+//  CW = 0: a piece of at most PIECE consecutive sorted positions of one row
+//          (RowPieces); every wavefront of a piece but the row's last is
+//          full.
+struct SegArg {
+  int nseg;
+  const int2* pieces;
+  const int* npieces;
+};
+
+// Width of the LDS copy of the cell starts (none for pieces, whose cells are
+// looked up in global memory: their span is not bounded).
+template <int CW> constexpr int csw() { return CW > 0 ? CW + 3 : 1; }
+
+// cell_start of cell c of tile row k (grid row cy-1+k): from the segment's
+// LDS copy (CW > 0) or from global memory (CW = 0).
+template <int CW> struct CellBounds {
+  const int (*cs)[csw<CW>()];
+  int slo;
+  __device__ __forceinline__ int at(int k, int c) const { return cs[k][c - slo]; }
+};
+template <> struct CellBounds<0> {
+  const int* __restrict__ cstart;
+  int base;  // (cy - 1) nx
+  int nx;
+  __device__ __forceinline__ int at(int k, int c) const { return cstart[base + k * nx + c]; }
+};
+
 template <int CW>
-__device__ __forceinline__ Segment<CW> segment_of(const KParams& K, const int* __restrict__ cstart,
-                                                  int nseg, int (*cs)[CW + 3]) {
-  Segment<CW> g;
+__device__ __forceinline__ CellBounds<CW> bounds_of(const KParams& K, const Segment& g,
+                                                    const int* __restrict__ cstart,
+                                                    const int (*cs)[csw<CW>()]) {
+  if constexpr (CW > 0)
+    return CellBounds<CW>{cs, g.slo};
+  else
+    return CellBounds<0>{cstart, (g.cy - 1) * K.nx, K.nx};
+}
+
+// The segment of this workgroup; O0 >= O1 when it has no particles.  With
+// CW > 0 the workgroup also copies the cell starts of its rows into cs.
+template <int CW>
+__device__ __forceinline__ Segment segment_of(const KParams& K, const int* __restrict__ cstart,
+                                              const uint32_t* __restrict__ skey,
+                                              const SegArg& sa, int (*cs)[csw<CW>()]) {
+  Segment g;
   const int nx = K.nx;
-  g.cy = blockIdx.x / nseg;
-  const int cx0 = (blockIdx.x - g.cy * nseg) * CW;
-  const int cx1 = min(cx0 + CW, nx);
-  g.O0 = cstart[g.cy * nx + cx0];
-  g.O1 = cstart[g.cy * nx + cx1];
+  int cx0, cx1;  // the particles' cells: cx0 .. cx1-1
+  if constexpr (CW > 0) {
+    // plain order: most cell runs of a tank are empty, and contiguous
+    // shares per XCD would leave the XCDs of the empty rows idle
+    g.cy = blockIdx.x / sa.nseg;
+    cx0 = (blockIdx.x - g.cy * sa.nseg) * CW;
+    cx1 = min(cx0 + CW, nx);
+    g.O0 = cstart[g.cy * nx + cx0];
+    g.O1 = cstart[g.cy * nx + cx1];
+  } else {
+    // pieces in even XCD-contiguous shares: neighbouring pieces, which stage
+    // the same rows, share an L2 (the grid is a bound on the piece count)
+    const int sb = xcd_share(*sa.npieces);
+    if (sb < 0) {
+      g.O0 = g.O1 = 0;
+      return g;
+    }
+    const int2 pc = sa.pieces[sb];
+    g.cy = pc.y;
+    g.O0 = pc.x;
+    g.O1 = min(pc.x + PIECE, cstart[(pc.y + 1) * nx]);
+    cx0 = (int)skey[g.O0] - g.cy * nx;
+    cx1 = (int)skey[g.O1 - 1] - g.cy * nx + 1;
+  }
   g.slo = cx0 > 0 ? cx0 - 1 : 0;
   const int shi = cx1 < nx ? cx1 : nx - 1;
   g.total = 0;
@@ -264,8 +338,10 @@ __device__ __forceinline__ Segment<CW> segment_of(const KParams& K, const int* _
     } else {
       g.rs[k] = cstart[ry * nx + g.slo];
       g.rl[k] = cstart[ry * nx + shi + 1] - g.rs[k];
-      const int ncs = shi + 2 - g.slo;
-      for (int j = threadIdx.x; j < ncs; j += blockDim.x) cs[k][j] = cstart[ry * nx + g.slo + j];
+      if constexpr (CW > 0) {
+        const int ncs = shi + 2 - g.slo;
+        for (int j = threadIdx.x; j < ncs; j += blockDim.x) cs[k][j] = cstart[ry * nx + g.slo + j];
+      }
     }
     g.lo[k] = g.total;
     g.total += g.rl[k];
@@ -273,12 +349,19 @@ __device__ __forceinline__ Segment<CW> segment_of(const KParams& K, const int* _
   return g;
 }
 
+// The row pieces on their own (after a sort that does not build them).
+__global__ __launch_bounds__(BLOCK) void k_row_pieces(const int* __restrict__ cstart,
+                                                      RowPieces rp, Halt hl) {
+  if (hl.on()) return;
+  row_pieces_block<BLOCK>(cstart, rp, blockIdx.x);
+}
+
 // A particle of the first or last cell column of a grid periodic in x: its
 // neighbours wrap to the other end, outside the segment's tile, so the
 // direct path (which walks the wrapped cells) takes it.  PER = false: the
 // kernel was launched for a grid without periodic x.
-template <bool PER, int CW>
-__device__ __forceinline__ bool seam(const KParams& K, const Segment<CW>& g, uint32_t key) {
+template <bool PER>
+__device__ __forceinline__ bool seam(const KParams& K, const Segment& g, uint32_t key) {
   if (!PER) return false;
   const int cx = (int)key - g.cy * K.nx;
   return cx == 0 || cx == K.nx - 1;
@@ -312,23 +395,22 @@ __device__ __forceinline__ uint32_t row_mask(const KParams& K, const Tile& T, co
 // row).  With rm != nullptr it also returns the neighbour masks of the three
 // rows (bit b = candidate ta_k + b) and whether they are complete (every
 // row at most 32 candidates).
-template <typename R, int CW, typename Tile, typename LP, typename Visit>
-__device__ __forceinline__ void for_neighbours(const KParams& K, const Segment<CW>& g,
-                                               const int (*cs)[CW + 3], const Tile& T,
-                                               const LP& pi, int i, uint32_t key,
+template <typename R, typename BD, typename Tile, typename LP, typename Visit>
+__device__ __forceinline__ void for_neighbours(const KParams& K, const Segment& g, const BD& B,
+                                               const Tile& T, const LP& pi, int i, uint32_t key,
                                                Visit&& visit, uint32_t* rm = nullptr,
                                                bool* ok = nullptr) {
   const int nx = K.nx;
   const int cx = (int)key - g.cy * nx;
-  const int a0 = (cx > 0 ? cx - 1 : 0) - g.slo;
-  const int a1 = (cx + 1 < nx ? cx + 1 : nx - 1) - g.slo + 1;
+  const int ca = cx > 0 ? cx - 1 : 0;
+  const int cb = (cx + 1 < nx ? cx + 1 : nx - 1) + 1;
   const int self = g.lo[1] + (i - g.rs[1]);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     if (g.rl[k] == 0) continue;
     const R uy = (R)row_uy(K, k);
-    const int ta = cs[k][a0];
-    const int tb = cs[k][a1];
+    const int ta = B.at(k, ca);
+    const int tb = B.at(k, cb);
     const int base = g.lo[k] - g.rs[k];
     if (rm && tb - ta > 32) *ok = false;
     for (int c0 = ta; c0 < tb; c0 += 32) {
@@ -390,6 +472,37 @@ __device__ __forceinline__ void walk_rows(uint32_t m0, uint32_t m1, uint32_t m2,
   }
 }
 
+// walk_rows two neighbours at a time: visit(la, lb, hb) with lb the next
+// neighbour after la in the same order when hb (else lb = la, to be
+// ignored), so that the pair arithmetic of two neighbours can overlap.
+template <typename Visit2>
+__device__ __forceinline__ void walk_rows2(uint32_t m0, uint32_t m1, uint32_t m2, int l0, int l1,
+                                           int l2, Visit2&& visit) {
+  uint32_t m = m0;
+  int base = l0;
+  auto refill = [&]() {
+    const bool t1 = m1 != 0u;
+    m = t1 ? m1 : m2;
+    base = t1 ? l1 : l2;
+    m1 = t1 ? m2 : 0u;
+    l1 = l2;
+    m2 = 0u;
+  };
+  while (true) {
+    if (m == 0u) {
+      refill();
+      if (m == 0u) break;
+    }
+    const int la = base + (int)__builtin_ctz(m);
+    m &= m - 1u;
+    if (m == 0u) refill();
+    const bool hb = m != 0u;
+    const int lb = hb ? base + (int)__builtin_ctz(m) : la;
+    m = hb ? (m & (m - 1u)) : m;
+    visit(la, lb, hb);
+  }
+}
+
 // FP32 density pass: the neighbour pre-test of candidates l0 .. l0+len-1
 // (len <= 32) of one row against K.pre2f, with x = cx cell + off rounded
 // once per record (xi, yi: the lane's x, and its y plus the row term); the
@@ -422,16 +535,15 @@ __device__ __forceinline__ uint32_t row_mask_pre(const KParams& K, const DensTil
 // and the pair sums, without a branch: a survivor that fails the exact test
 // adds nothing (dens_pair with ok = false) and leaves the recorded mask.
 // Same visiting order and sums as for_neighbours; the exact masks in rm.
-template <bool SUM, int CW, int CAP>
-__device__ __forceinline__ void dens_walk_f32(const KParams& K, const Segment<CW>& g,
-                                              const int (*cs)[CW + 3],
+template <bool SUM, int CAP, typename BD>
+__device__ __forceinline__ void dens_walk_f32(const KParams& K, const Segment& g, const BD& B,
                                               const DensTile<float, CAP>& T, const LaneF& pi,
                                               float2 vi, int i, uint32_t key, DensOwn<float>& o,
                                               uint32_t* rm, bool* ok) {
   const int nx = K.nx;
   const int cx = (int)key - g.cy * nx;
-  const int a0 = (cx > 0 ? cx - 1 : 0) - g.slo;
-  const int a1 = (cx + 1 < nx ? cx + 1 : nx - 1) - g.slo + 1;
+  const int ca = cx > 0 ? cx - 1 : 0;
+  const int cb = (cx + 1 < nx ? cx + 1 : nx - 1) + 1;
   const int self = g.lo[1] + (i - g.rs[1]);
   const float cut2 = K.cut2f;
   const float xi = __builtin_fmaf(pi.cxf, K.cellf, pi.ox);
@@ -440,8 +552,8 @@ __device__ __forceinline__ void dens_walk_f32(const KParams& K, const Segment<CW
     if (g.rl[k] == 0) continue;
     const float uy = row_uy(K, k);
     const float yi = pi.oy + uy;
-    const int ta = cs[k][a0];
-    const int tb = cs[k][a1];
+    const int ta = B.at(k, ca);
+    const int tb = B.at(k, cb);
     const int base = g.lo[k] - g.rs[k];
     if (tb - ta > 32) *ok = false;
     for (int c0 = ta; c0 < tb; c0 += 32) {
@@ -466,27 +578,29 @@ __device__ __forceinline__ void dens_walk_f32(const KParams& K, const Segment<CW
 }
 
 // The same walk driven by masks recorded by the density pass.
-template <typename R, int CW, typename Visit>
-__device__ __forceinline__ void for_masked(const KParams& K, const Segment<CW>& g,
-                                           const int (*cs)[CW + 3], uint32_t key, uint4 nm,
-                                           Visit&& visit) {
+// (Rows outside the grid have empty masks: their entries are not looked up.)
+template <typename R, typename BD, typename Visit>
+__device__ __forceinline__ void for_masked(const KParams& K, const Segment& g, const BD& B,
+                                           uint32_t key, uint4 nm, Visit&& visit) {
   const int nx = K.nx;
   const int cx = (int)key - g.cy * nx;
-  const int a0 = (cx > 0 ? cx - 1 : 0) - g.slo;
-  walk_masks<R>(K, nm.x, nm.y, nm.z, g.lo[0] - g.rs[0] + cs[0][a0],
-                g.lo[1] - g.rs[1] + cs[1][a0], g.lo[2] - g.rs[2] + cs[2][a0], visit);
+  const int ca = cx > 0 ? cx - 1 : 0;
+  walk_masks<R>(K, nm.x, nm.y, nm.z, g.rl[0] ? g.lo[0] - g.rs[0] + B.at(0, ca) : 0,
+                g.lo[1] - g.rs[1] + B.at(1, ca), g.rl[2] ? g.lo[2] - g.rs[2] + B.at(2, ca) : 0,
+                visit);
 }
 
 template <typename R, bool SUM, bool EOS, bool PER, int BLK, int CW, int CAP>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && !PER ? 8 : 1))) void k_density_tiled(St<R> S, KParams K,
                                                        const uint32_t* __restrict__ skey,
-                                                       const int* __restrict__ cstart, int nseg,
+                                                       const int* __restrict__ cstart, SegArg sa,
                                                        uint4* __restrict__ nbm) {
   if (halted(K)) return;
   __shared__ DensTile<R, CAP> T;
-  __shared__ int cs[3][CW + 3];
-  const Segment<CW> g = segment_of<CW>(K, cstart, nseg, cs);
+  __shared__ int cs[3][csw<CW>()];
+  const Segment g = segment_of<CW>(K, cstart, skey, sa, cs);
   if (g.O0 >= g.O1) return;
+  const auto B = bounds_of<CW>(K, g, cstart, cs);
   const bool tiled = g.total <= CAP;
   if (tiled) {
 #pragma unroll
@@ -509,10 +623,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
     uint32_t rm[3] = {0u, 0u, 0u};
     bool ok = true;
     if constexpr (sizeof(R) == 4) {
-      dens_walk_f32<SUM, CW>(K, g, cs, T, pi, vi, i, skey[i], o, rm, &ok);
+      dens_walk_f32<SUM>(K, g, B, T, pi, vi, i, skey[i], o, rm, &ok);
     } else {
-      for_neighbours<R, CW>(
-          K, g, cs, T, pi, i, skey[i],
+      for_neighbours<R>(
+          K, g, B, T, pi, i, skey[i],
           [&](int l, R uy) {
             R dx, dy;
             T.delta(K, pi, l, uy, dx, dy);
@@ -530,13 +644,14 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
 template <typename R, bool PER, int BLK, int CW, int CAP>
 __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
                                                       const uint32_t* __restrict__ skey,
-                                                      const int* __restrict__ cstart, int nseg,
+                                                      const int* __restrict__ cstart, SegArg sa,
                                                       const uint4* __restrict__ nbm) {
   if (halted(K)) return;
   __shared__ TensTile<R, CAP> T;
-  __shared__ int cs[3][CW + 3];
-  const Segment<CW> g = segment_of<CW>(K, cstart, nseg, cs);
+  __shared__ int cs[3][csw<CW>()];
+  const Segment g = segment_of<CW>(K, cstart, skey, sa, cs);
   if (g.O0 >= g.O1) return;
+  const auto B = bounds_of<CW>(K, g, cstart, cs);
   const bool tiled = g.total <= CAP;
   if (tiled) {
 #pragma unroll
@@ -565,9 +680,9 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
     const uint32_t key = skey[i];
     const uint4 nm = nbm ? nbm[i] : make_uint4(0u, 0u, 0u, 0u);
     if (nm.w)
-      for_masked<R, CW>(K, g, cs, key, nm, visit);
+      for_masked<R>(K, g, B, key, nm, visit);
     else
-      for_neighbours<R, CW>(K, g, cs, T, pi, i, key, visit);
+      for_neighbours<R>(K, g, B, T, pi, i, key, visit);
     tens_end(S, K, i, T.q[self], o);
   }
 }
@@ -576,15 +691,16 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
 // (force_pair_f32), -1 to read them from K.
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI, bool PER, int BLK, int CW,
           int CAP, int PC = -1>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 6 : 1))) void k_force_tiled(St<R> S, KParams K,
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 4 : 1))) void k_force_tiled(St<R> S, KParams K,
                                                      const uint32_t* __restrict__ skey,
-                                                     const int* __restrict__ cstart, int nseg,
+                                                     const int* __restrict__ cstart, SegArg sa,
                                                      const uint4* __restrict__ nbm) {
   if (halted(K)) return;
   __shared__ ForceTile<R, CAP, VI> T;
-  __shared__ int cs[3][CW + 3];
-  const Segment<CW> g = segment_of<CW>(K, cstart, nseg, cs);
+  __shared__ int cs[3][csw<CW>()];
+  const Segment g = segment_of<CW>(K, cstart, skey, sa, cs);
   if (g.O0 >= g.O1) return;
+  const auto B = bounds_of<CW>(K, g, cstart, cs);
   const bool tiled = g.total <= CAP;
   if (tiled) {
 #pragma unroll
@@ -614,34 +730,46 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 6
     const uint4 nm = nbm ? nbm[i] : make_uint4(0u, 0u, 0u, 0u);
     if (nm.w && sizeof(R) == 4) {
       // FP32: the row-refill walk, the row term from the tile record
-      const int nx = K.nx;
-      const int a0 = ((int)key - g.cy * nx > 0 ? (int)key - g.cy * nx - 1 : 0) - g.slo;
-      walk_rows(nm.x, nm.y, nm.z, g.lo[0] - g.rs[0] + cs[0][a0],
-                g.lo[1] - g.rs[1] + cs[1][a0], g.lo[2] - g.rs[2] + cs[2][a0], [&](int l) {
-                  R dx, dy;
-                  T.delta_rec(K, pi, l, dx, dy);
-                  const R r2 = dx * dx + dy * dy;
-                  T.template pair<PRESS, AV, AH, XS, PC>(K, o, l, dx, dy, r2);
-                });
+      const int cx = (int)key - g.cy * K.nx;
+      const int ca = cx > 0 ? cx - 1 : 0;
+      const int l0 = g.rl[0] ? g.lo[0] - g.rs[0] + B.at(0, ca) : 0;
+      const int l1 = g.lo[1] - g.rs[1] + B.at(1, ca);
+      const int l2 = g.rl[2] ? g.lo[2] - g.rs[2] + B.at(2, ca) : 0;
+      if constexpr (sizeof(R) == 4 && PC >= 0 && !VI) {
+        // two neighbours per step; each added in walk order
+        walk_rows2(nm.x, nm.y, nm.z, l0, l1, l2, [&](int la, int lb, bool hb) {
+          const TermsF ta = T.template terms<PRESS, AV, AH, XS, PC>(K, o, pi, la);
+          const TermsF tb = T.template terms<PRESS, AV, AH, XS, PC>(K, o, pi, lb);
+          force_acc_f32<PRESS, AV, AH, XS>(o, ta, true);
+          force_acc_f32<PRESS, AV, AH, XS>(o, tb, hb);
+        });
+      } else {
+        walk_rows(nm.x, nm.y, nm.z, l0, l1, l2, [&](int l) {
+          R dx, dy;
+          T.delta_rec(K, pi, l, dx, dy);
+          const R r2 = dx * dx + dy * dy;
+          T.template pair<PRESS, AV, AH, XS, PC>(K, o, l, dx, dy, r2);
+        });
+      }
     } else if (nm.w) {
-      for_masked<R, CW>(K, g, cs, key, nm, visit);
+      for_masked<R>(K, g, B, key, nm, visit);
     } else {
-      for_neighbours<R, CW>(K, g, cs, T, pi, i, key, visit);
+      for_neighbours<R>(K, g, B, T, pi, i, key, visit);
     }
     force_end<R, PRESS, XS, VI>(S, K, i, o);
   }
 }
 
-// Launch shapes: cells per segment, lanes, LDS record capacity.
+// Launch shapes: cells per segment (0: row pieces of PIECE particles), lanes,
+// LDS record capacity.
 template <typename R> struct TileCfg;
 template <> struct TileCfg<float> {
-  static constexpr int BLK = 256, CW = 32, CAP = 720;
+  static constexpr int BLK = PIECE, CW = 0, CAP = 640;
 };
-// The FP32 force instances with compile-time pair constants (PC >= 0): 28-cell
-// segments, so that the tile (640 records of 40 bytes) lets six workgroups
-// share a CU, and registers capped for six waves per SIMD.
+// The FP32 force instances with compile-time pair constants (PC >= 0): row
+// pieces; the tile (640 records of 40 bytes) lets six workgroups share a CU.
 struct HotForceCfg {
-  static constexpr int BLK = 256, CW = 28, CAP = 640;
+  static constexpr int BLK = PIECE, CW = 0, CAP = 640;
 };
 template <> struct TileCfg<double> {
   static constexpr int BLK = 128, CW = 16, CAP = 400;
