@@ -530,13 +530,14 @@ __device__ __forceinline__ uint32_t row_mask_pre(const KParams& K, const DensTil
   return m;
 }
 
-// for_neighbours of the FP32 density pass: the pre-test, then over its
-// survivors the exact test (r^2 < (2h)^2 with the direct path's arithmetic)
-// and the pair sums, without a branch: a survivor that fails the exact test
-// adds nothing (dens_pair with ok = false) and leaves the recorded mask.
-// Same visiting order and sums as for_neighbours; the exact masks in rm.
+// for_neighbours of the FP32 density pass, row by row: the pre-test, then over
+// its survivors the exact test (r^2 < (2h)^2 with the direct path's
+// arithmetic) and the pair sums, without a branch: a survivor that fails the
+// exact test adds nothing (dens_pair with ok = false) and leaves the recorded
+// mask.  Same visiting order and sums as for_neighbours; the exact masks in rm.
+// Taken by dens_walk_f32 for a particle with a row of more than 32 candidates.
 template <bool SUM, int CAP, typename BD>
-__device__ __forceinline__ void dens_walk_f32(const KParams& K, const Segment& g, const BD& B,
+__device__ __forceinline__ void dens_walk_rows_f32(const KParams& K, const Segment& g, const BD& B,
                                               const DensTile<float, CAP>& T, const LaneF& pi,
                                               float2 vi, int i, uint32_t key, DensOwn<float>& o,
                                               uint32_t* rm, bool* ok) {
@@ -575,6 +576,78 @@ __device__ __forceinline__ void dens_walk_f32(const KParams& K, const Segment& g
       if (c0 == ta) rm[k] = m;
     }
   }
+}
+
+// The FP32 density walk: the pre-test of all three rows first, then one loop
+// over the survivors of the three masks in row order (so that a wavefront
+// runs for its longest lane's survivor count, not for the sum over the rows
+// of the per-row maxima), each through the exact test and the pair sums
+// without a branch.  Failed survivors are collected per row (fc, flushed at
+// each row change) and taken out of the recorded masks.  Same visiting order
+// and sums as for_neighbours; the exact masks in rm.
+template <bool SUM, int CAP, typename BD>
+__device__ __forceinline__ void dens_walk_f32(const KParams& K, const Segment& g, const BD& B,
+                                              const DensTile<float, CAP>& T, const LaneF& pi,
+                                              float2 vi, int i, uint32_t key, DensOwn<float>& o,
+                                              uint32_t* rm, bool* ok) {
+  const int nx = K.nx;
+  const int cx = (int)key - g.cy * nx;
+  const int ca = cx > 0 ? cx - 1 : 0;
+  const int cb = (cx + 1 < nx ? cx + 1 : nx - 1) + 1;
+  const int self = g.lo[1] + (i - g.rs[1]);
+  const float xi = __builtin_fmaf(pi.cxf, K.cellf, pi.ox);
+  uint32_t pm[3];
+  int pb[3];
+  bool small = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    pm[k] = 0u;
+    pb[k] = 0;
+    if (g.rl[k] == 0) continue;
+    const int ta = B.at(k, ca);
+    const int tb = B.at(k, cb);
+    pb[k] = g.lo[k] - g.rs[k] + ta;
+    small = small && tb - ta <= 32;
+    pm[k] = row_mask_pre(K, T, xi, pi.oy + row_uy(K, k), pb[k], min(32, tb - ta), self);
+  }
+  if (!small) {
+    dens_walk_rows_f32<SUM>(K, g, B, T, pi, vi, i, key, o, rm, ok);
+    return;
+  }
+  const float cut2 = K.cut2f;
+  uint32_t f0 = 0u, f1 = 0u, f2 = 0u, fc = 0u;  // failed survivors: rows 0..2, current row
+  int kc = 0;
+  uint32_t m = pm[0];
+  int base = pb[0];
+  float uy = row_uy(K, 0);
+  while (true) {
+    if (m == 0u) {
+      f0 |= kc == 0 ? fc : 0u;
+      f1 |= kc == 1 ? fc : 0u;
+      f2 |= kc == 2 ? fc : 0u;
+      fc = 0u;
+      const bool t1 = kc == 0 && pm[1] != 0u;
+      const bool t2 = !t1 && kc < 2 && pm[2] != 0u;
+      if (!(t1 || t2)) break;
+      kc = t1 ? 1 : 2;
+      m = t1 ? pm[1] : pm[2];
+      base = t1 ? pb[1] : pb[2];
+      uy = t1 ? row_uy(K, 1) : row_uy(K, 2);
+    }
+    const int b = (int)__builtin_ctz(m);
+    m &= m - 1u;
+    const int l = base + b;
+    float dx, dy;
+    T.delta(K, pi, l, uy, dx, dy);
+    const float r2 = dx * dx + dy * dy;
+    const bool in = r2 < cut2;
+    fc |= in ? 0u : (1u << b);
+    const float2 vj = T.vel(l);
+    dens_pair<float, SUM>(K, dx, dy, r2, vi.x - vj.x, vi.y - vj.y, o.sum, o.rs, in);
+  }
+  rm[0] = pm[0] & ~f0;
+  rm[1] = pm[1] & ~f1;
+  rm[2] = pm[2] & ~f2;
 }
 
 // The same walk driven by masks recorded by the density pass.
