@@ -141,36 +141,28 @@ template <int CAP> struct TauTile<double, CAP, true> {
 
 template <typename R, int CAP, bool VI = false> struct ForceTile;
 template <int CAP, bool VI> struct ForceTile<float, CAP, VI> {
-  float4 a[CAP + TPAD];   // offx, offy, vx, vy
-  float4 b[CAP + TPAD];   // Q = 1/rho, A = p/rho, psi, u
-  float2 cu[CAP + TPAD];  // cell column as float, the cell-row term of the record's row
+  float4 a[CAP + TPAD];  // offx, offy, vx, vy
+  float4 b[CAP + TPAD];  // Q = 1/rho, A = p/rho, psi, u
+  float cx[CAP + TPAD];  // cell column as float
   TauTile<float, CAP + TPAD, VI> tw;
-  __device__ __forceinline__ void put(int l, const St<float>& S, const KParams& K, int t, int k) {
+  __device__ __forceinline__ void put(int l, const St<float>& S, const KParams& K, int t, int) {
     const float2 o = S.off[t];
     const float2 v = S.v[t];
     a[l] = make_float4(o.x, o.y, v.x, v.y);
     float Q, A;
     q_and_a(K, S.rho[t], S.p[t], Q, A);
     b[l] = make_float4(Q, A, S.psi[t], S.uf[t]);
-    cu[l] = make_float2((float)(S.cxy[t] & 0xffffu), row_uy(K, k));
+    cx[l] = (float)(S.cxy[t] & 0xffffu);
     tw.put(l, S, t, visc_c(K, Q));
   }
   __device__ __forceinline__ LaneF lane(int l) const {
     const float4 q = a[l];
-    return {cu[l].x, q.x, q.y};
+    return {lds_at(cx, off4(l)), q.x, q.y};
   }
   __device__ __forceinline__ void delta(const KParams& K, const LaneF& p, int l, float uy,
                                         float& dx, float& dy) const {
     const float4 q = a[l];
-    tile_delta(K, p, cu[l].x, q.x, q.y, uy, dx, dy);
-  }
-  // the same with the row term of the record's own row (lanes are particles
-  // of the segment's row)
-  __device__ __forceinline__ void delta_rec(const KParams& K, const LaneF& p, int l, float& dx,
-                                            float& dy) const {
-    const float4 q = a[l];
-    const float2 c = cu[l];
-    tile_delta(K, p, c.x, q.x, q.y, c.y, dx, dy);
+    tile_delta(K, p, lds_at(cx, off4(l)), q.x, q.y, uy, dx, dy);
   }
   template <bool PRESS, bool AV, bool AH, bool XS, int PC = -1>
   __device__ __forceinline__ void pair(const KParams& K, ForceOwn<float>& o, int l, float dx,
@@ -180,12 +172,13 @@ template <int CAP, bool VI> struct ForceTile<float, CAP, VI> {
     force_pair<float, PRESS, AV, AH, XS, VI, PC>(K, o, dx, dy, r2, q.z, q.w, w.x, w.y, w.z,
                                                  w.w, tw.get(l));
   }
-  // the terms of the pair with record l, unaccumulated (no physical viscosity)
+  // the terms of the pair with record l of the row with term uy,
+  // unaccumulated (no physical viscosity)
   template <bool PRESS, bool AV, bool AH, bool XS, int PC>
   __device__ __forceinline__ TermsF terms(const KParams& K, const ForceOwn<float>& o,
-                                          const LaneF& p, int l) const {
+                                          const LaneF& p, int l, float uy) const {
     float dx, dy;
-    delta_rec(K, p, l, dx, dy);
+    delta(K, p, l, uy, dx, dy);
     const float4 q = a[l];
     const float4 w = b[l];
     return force_terms_f32<PRESS, AV, AH, XS, PC>(K, o, dx, dy, q.z, q.w, w.x, w.y, w.z, w.w);
@@ -212,10 +205,6 @@ template <int CAP, bool VI> struct ForceTile<double, CAP, VI> {
     const double2 q = x[l];
     dx = p.x - q.x;
     dy = p.y - q.y;
-  }
-  __device__ __forceinline__ void delta_rec(const KParams& K, const PosD& p, int l, double& dx,
-                                            double& dy) const {
-    delta(K, p, l, 0.0, dx, dy);
   }
   template <bool PRESS, bool AV, bool AH, bool XS, int PC = -1>
   __device__ __forceinline__ void pair(const KParams& K, ForceOwn<double>& o, int l, double dx,
@@ -449,43 +438,49 @@ __device__ __forceinline__ void walk_masks(const KParams& K, uint32_t m0, uint32
 // The same order with less work per neighbour: a lane walks one row mask at
 // a time and takes the next non-empty row only when the current one is
 // exhausted, so the per-neighbour step is a bit scan; the row change runs
-// only in the iterations where some lane of the wavefront needs it.  The
-// visitor gets the tile entry alone (the record carries its row term).
-template <typename Visit>
+// only in the iterations where some lane of the wavefront needs it.  Row k
+// has mask m_k, first tile entry l_k and row term u_k; visit(l, u).
+template <typename R, typename Visit>
 __device__ __forceinline__ void walk_rows(uint32_t m0, uint32_t m1, uint32_t m2, int l0, int l1,
-                                          int l2, Visit&& visit) {
+                                          int l2, R u0, R u1, R u2, Visit&& visit) {
   uint32_t m = m0;
   int base = l0;
+  R uy = u0;
   while (true) {
     if (m == 0u) {
       const bool t1 = m1 != 0u;
       m = t1 ? m1 : m2;
       base = t1 ? l1 : l2;
+      uy = t1 ? u1 : u2;
       m1 = t1 ? m2 : 0u;
       l1 = l2;
+      u1 = u2;
       m2 = 0u;
       if (m == 0u) break;
     }
     const int b = (int)__builtin_ctz(m);
     m &= m - 1u;
-    visit(base + b);
+    visit(base + b, uy);
   }
 }
 
-// walk_rows two neighbours at a time: visit(la, lb, hb) with lb the next
-// neighbour after la in the same order when hb (else lb = la, to be
+// walk_rows two neighbours at a time: visit(la, ua, lb, ub, hb) with lb the
+// next neighbour after la in the same order when hb (else lb = la, to be
 // ignored), so that the pair arithmetic of two neighbours can overlap.
 template <typename Visit2>
 __device__ __forceinline__ void walk_rows2(uint32_t m0, uint32_t m1, uint32_t m2, int l0, int l1,
-                                           int l2, Visit2&& visit) {
+                                           int l2, float u0, float u1, float u2, Visit2&& visit) {
   uint32_t m = m0;
   int base = l0;
+  float uy = u0;
   auto refill = [&]() {
     const bool t1 = m1 != 0u;
     m = t1 ? m1 : m2;
     base = t1 ? l1 : l2;
+    uy = t1 ? u1 : u2;
     m1 = t1 ? m2 : 0u;
     l1 = l2;
+    u1 = u2;
     m2 = 0u;
   };
   while (true) {
@@ -494,12 +489,14 @@ __device__ __forceinline__ void walk_rows2(uint32_t m0, uint32_t m1, uint32_t m2
       if (m == 0u) break;
     }
     const int la = base + (int)__builtin_ctz(m);
+    const float ua = uy;
     m &= m - 1u;
     if (m == 0u) refill();
     const bool hb = m != 0u;
     const int lb = hb ? base + (int)__builtin_ctz(m) : la;
+    const float ub = hb ? uy : ua;
     m = hb ? (m & (m - 1u)) : m;
-    visit(la, lb, hb);
+    visit(la, ua, lb, ub, hb);
   }
 }
 
@@ -808,21 +805,18 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 4
       const int l0 = g.rl[0] ? g.lo[0] - g.rs[0] + B.at(0, ca) : 0;
       const int l1 = g.lo[1] - g.rs[1] + B.at(1, ca);
       const int l2 = g.rl[2] ? g.lo[2] - g.rs[2] + B.at(2, ca) : 0;
+      const R u0 = (R)row_uy(K, 0), u1 = (R)row_uy(K, 1), u2 = (R)row_uy(K, 2);
       if constexpr (sizeof(R) == 4 && PC >= 0 && !VI) {
         // two neighbours per step; each added in walk order
-        walk_rows2(nm.x, nm.y, nm.z, l0, l1, l2, [&](int la, int lb, bool hb) {
-          const TermsF ta = T.template terms<PRESS, AV, AH, XS, PC>(K, o, pi, la);
-          const TermsF tb = T.template terms<PRESS, AV, AH, XS, PC>(K, o, pi, lb);
-          force_acc_f32<PRESS, AV, AH, XS>(o, ta, true);
-          force_acc_f32<PRESS, AV, AH, XS>(o, tb, hb);
-        });
+        walk_rows2(nm.x, nm.y, nm.z, l0, l1, l2, u0, u1, u2,
+                   [&](int la, float ua, int lb, float ub, bool hb) {
+                     const TermsF ta = T.template terms<PRESS, AV, AH, XS, PC>(K, o, pi, la, ua);
+                     const TermsF tb = T.template terms<PRESS, AV, AH, XS, PC>(K, o, pi, lb, ub);
+                     force_acc_f32<PRESS, AV, AH, XS>(o, ta, true);
+                     force_acc_f32<PRESS, AV, AH, XS>(o, tb, hb);
+                   });
       } else {
-        walk_rows(nm.x, nm.y, nm.z, l0, l1, l2, [&](int l) {
-          R dx, dy;
-          T.delta_rec(K, pi, l, dx, dy);
-          const R r2 = dx * dx + dy * dy;
-          T.template pair<PRESS, AV, AH, XS, PC>(K, o, l, dx, dy, r2);
-        });
+        walk_rows(nm.x, nm.y, nm.z, l0, l1, l2, u0, u1, u2, visit);
       }
     } else if (nm.w) {
       for_masked<R>(K, g, B, key, nm, visit);
@@ -840,9 +834,11 @@ template <> struct TileCfg<float> {
   static constexpr int BLK = PIECE, CW = 0, CAP = 640;
 };
 // The FP32 force instances with compile-time pair constants (PC >= 0): row
-// pieces; the tile (640 records of 40 bytes) lets six workgroups share a CU.
+// pieces; the tile (600 records of 36 bytes, 21.9 KB) lets seven workgroups
+// share a CU (a piece's three rows hold about 3 x 140 records on the
+// dam-break lattice, 584 at most).
 struct HotForceCfg {
-  static constexpr int BLK = PIECE, CW = 0, CAP = 640;
+  static constexpr int BLK = PIECE, CW = 0, CAP = 600;
 };
 template <> struct TileCfg<double> {
   static constexpr int BLK = 128, CW = 16, CAP = 400;
