@@ -96,6 +96,8 @@ struct arm_ctx {
   int cs_tiles = 0;
   int* cnt = nullptr;       // per-cell counters, zero between sorts
   int* tsum = nullptr;      // per-tile sums of the counters
+  uint64_t* tstat = nullptr;  // status words of the single-pass scan (k_cs_scan1)
+  uint32_t scan_epoch = 0;    // epoch of the last single-pass scan
   uint64_t* ent = nullptr;  // entries grouped by cell: cap ids then cap slots (ent_of); cells in sk[1]
   uint32_t* skey = nullptr;  // sorted keys of the current order
   int* cstart = nullptr;
@@ -273,10 +275,15 @@ int run_sort(arm_ctx* c, const int32_t* id_of, const uint32_t** sval_out, bool* 
     const int ncell = sort_cells(c);
     const int nb = (n + CS_THREADS - 1) / CS_THREADS;
     const Halt hl{c->K.halt, c->K.its};
-    hipLaunchKernelGGL(k_cs_tile_sum, dim3(c->cs_tiles), dim3(CS_THREADS), 0, c->stream, c->cnt,
-                       ncell, c->tsum, hl);
-    hipLaunchKernelGGL(k_cs_tile_scan, dim3(c->cs_tiles), dim3(CS_THREADS), 0, c->stream, c->cnt,
-                       ncell, n, c->tsum, c->cstart, hl);
+    // epochs 1 .. SCAN_EPOCH_MAX; the cleared words (epoch 0) read as unpublished
+    if (c->scan_epoch == SCAN_EPOCH_MAX) {
+      hipLaunchKernelGGL(k_zero_words, dim3(grid_of(2 * c->cs_tiles)), dim3(BLOCK), 0, c->stream,
+                         reinterpret_cast<uint32_t*>(c->tstat), (size_t)2 * c->cs_tiles);
+      c->scan_epoch = 0;
+    }
+    ++c->scan_epoch;
+    hipLaunchKernelGGL(k_cs_scan1, dim3(c->cs_tiles), dim3(CS_THREADS), 0, c->stream, c->cnt,
+                       ncell, n, c->cstart, c->tstat, c->scan_epoch, hl);
     hipLaunchKernelGGL(k_cs_scatter, dim3(nb), dim3(CS_THREADS), 0, c->stream, c->key_by_id,
                        id_of, n, c->cstart, c->cnt, ent_of(c), c->sk[1], hl);
     hipLaunchKernelGGL(k_cs_rank, dim3(nb), dim3(CS_THREADS), 0, c->stream, ent_of(c), c->sk[1],
@@ -2198,6 +2205,11 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
     const int tiles = std::max(c->cs_tiles, (capacity + CS_TILE - 1) / CS_TILE);
     rc = dalloc(c, &c->cnt, (size_t)ncell + 1);
     if (!rc) rc = dalloc(c, &c->tsum, (size_t)tiles);
+    if (!rc) rc = dalloc(c, &c->tstat, (size_t)c->cs_tiles);
+    if (!rc) {
+      hipError_t ez = hipMemset(c->tstat, 0, sizeof(uint64_t) * (size_t)c->cs_tiles);
+      if (ez != hipSuccess) rc = fail(c, ARM_ERR_HIP, "hipMemset: %s", hipGetErrorString(ez));
+    }
     if (!rc) rc = dalloc(c, &c->ent, (size_t)capacity);
     if (!rc) {
       hipError_t ez = hipMemset(c->cnt, 0, sizeof(int) * ((size_t)ncell + 1));

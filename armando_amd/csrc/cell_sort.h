@@ -197,6 +197,67 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_tile_scan(const int* __restri
   if (blockIdx.x == 0 && threadIdx.x == 0) cstart[ncell] = n;
 }
 
+// ---- single-pass scan -----------------------------------------------------
+// k_cs_scan1 does the work of k_cs_tile_sum and k_cs_tile_scan in one launch:
+// each workgroup publishes its tile's sum in a status word, then adds up the
+// words of all the tiles before its own, read by all its lanes at once (the
+// grid's workgroups run together, so the words are there within a few
+// microseconds).  A word is (epoch << 32) | sum; a word of another epoch (an
+// earlier sort) reads as not yet published.  The host gives every sort a new
+// epoch (1 .. SCAN_EPOCH_MAX), so the words need no clearing between sorts.
+constexpr uint32_t SCAN_EPOCH_MAX = 0xffffffffu;  // the host clears the words and restarts at 1
+constexpr int SCAN_SPIN_LIMIT = 1 << 14;
+
+__global__ __launch_bounds__(CS_THREADS) void k_cs_scan1(const int* __restrict__ count, int ncell,
+                                                         int n, int* __restrict__ cstart,
+                                                         uint64_t* __restrict__ tstat,
+                                                         uint32_t epoch, Halt hl) {
+  if (hl.on()) return;
+  __shared__ int wsum[CS_THREADS / 64];
+  const int b = blockIdx.x;
+  const int c0 = b * CS_TILE + threadIdx.x * CS_PER_LANE;
+  int v[CS_PER_LANE];
+  load8(count, c0, ncell, v);
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < CS_PER_LANE; ++k) s += v[k];
+  int total;
+  const int pre = block_exclusive_scan<CS_THREADS>(s, wsum, &total);
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&tstat[b], ((uint64_t)epoch << 32) | (uint32_t)total, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  // the sums of tiles 0 .. b-1; re-read while one is missing, and after
+  // SCAN_SPIN_LIMIT rounds add up their counters instead (slower, the same)
+  int part = 0;
+  bool ready = false;
+  for (int spins = 0; !ready && spins < SCAN_SPIN_LIMIT; ++spins) {
+    part = 0;
+    bool miss = false;
+    for (int q = threadIdx.x; q < b; q += CS_THREADS) {
+      const uint64_t w = __hip_atomic_load(&tstat[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      miss = miss || (uint32_t)(w >> 32) != epoch;
+      part += (int)(uint32_t)w;
+    }
+    ready = !__syncthreads_or(miss);
+  }
+  if (!ready) {
+    part = 0;
+    for (int c = threadIdx.x; c < min(b * CS_TILE, ncell); c += CS_THREADS) part += count[c];
+  }
+  __syncthreads();  // wsum is reused
+  int excl;
+  block_exclusive_scan<CS_THREADS>(part, wsum, &excl);
+  int run = excl + pre;
+  int o[CS_PER_LANE];
+#pragma unroll
+  for (int k = 0; k < CS_PER_LANE; ++k) {
+    o[k] = run;
+    run += v[k];
+  }
+  store8(cstart, c0, ncell, o);
+  if (b == 0 && threadIdx.x == 0) cstart[ncell] = n;
+}
+
 // The scattered entries: id and slot of the particle at each position of
 // its cell range (two arrays, so that the rank's loop reads 4 bytes per
 // entry); entc = its cell.
