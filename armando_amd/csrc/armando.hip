@@ -64,7 +64,7 @@ struct arm_ctx {
   double grads[ARM_NGRAD] = {0};   // parameter gradients of the last backward
   bool masks_valid = false;  // nbm holds the neighbour masks of the current order
   uint4* nbm = nullptr;
-  // FP32 2D tiled gathers: the row pieces of the current order (k_row_pieces)
+  // 2D tiled gathers: the row pieces of the current order (RowPieces)
   int2* pieces = nullptr;
   int* npieces = nullptr;
   bool pieces_valid = false;
@@ -2249,7 +2249,7 @@ int arm_create(const arm_params* prm, int capacity, arm_ctx** out) {
   if (!rc) rc = dalloc(c, &c->d_part, (size_t)grid_of(capacity) * 6);
   if (!rc) rc = dalloc(c, &c->d_cnt, capacity);
   if (!rc) rc = dalloc(c, &c->nbm, capacity);
-  if (!rc && !is3d(c) && is32(c)) {
+  if (!rc && !is3d(c)) {
     rc = dalloc(c, &c->pieces, (size_t)capacity / PIECE + c->K.ny + 2);
     if (!rc) rc = dalloc(c, &c->npieces, 1);
   }

@@ -828,7 +828,8 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 4
 }
 
 // Launch shapes: cells per segment (0: row pieces of PIECE particles), lanes,
-// LDS record capacity.
+// LDS record capacity.  Both forms run on row pieces; the cell runs (CW > 0)
+// remain as an alternative shape.
 template <typename R> struct TileCfg;
 template <> struct TileCfg<float> {
   static constexpr int BLK = PIECE, CW = 0, CAP = 640;
@@ -841,7 +842,7 @@ struct HotForceCfg {
   static constexpr int BLK = PIECE, CW = 0, CAP = 600;
 };
 template <> struct TileCfg<double> {
-  static constexpr int BLK = 128, CW = 16, CAP = 400;
+  static constexpr int BLK = PIECE, CW = 0, CAP = 600;
 };
 
 }  // namespace arm
