@@ -515,7 +515,7 @@ template <int PC> void launch_force_hot(arm_ctx* c, const uint4* nbm) {
 template <typename R, bool P, bool AV, bool AH, bool XS, bool VI = false>
 void launch_force(arm_ctx* c) {
   St<R>& A = stA<R>(c);
-  using T = TileCfg<R>;
+  using T = ForceCfg<R, VI>;
   if (c->tiled && !c->K.sle) {
     const uint4* nbm = c->masks_valid ? c->nbm : nullptr;
     if (sizeof(R) == 4 && P && AV && AH && XS && !VI && !c->K.periodic &&
