@@ -500,6 +500,46 @@ __device__ __forceinline__ void walk_rows2(uint32_t m0, uint32_t m1, uint32_t m2
   }
 }
 
+// for_neighbours with the masks of all three rows built first and one walk
+// over them (walk_rows): the same visits in the same order, but a wavefront
+// runs for its longest lane's neighbour count rather than for the sum over
+// the rows of the per-row maxima.  A particle with a row of more than 32
+// candidates takes for_neighbours.  The masks go to rm.
+template <typename R, typename BD, typename Tile, typename LP, typename Visit>
+__device__ __forceinline__ void for_neighbours_rows(const KParams& K, const Segment& g,
+                                                    const BD& B, const Tile& T, const LP& pi,
+                                                    int i, uint32_t key, Visit&& visit,
+                                                    uint32_t* rm, bool* ok) {
+  const int nx = K.nx;
+  const int cx = (int)key - g.cy * nx;
+  const int ca = cx > 0 ? cx - 1 : 0;
+  const int cb = (cx + 1 < nx ? cx + 1 : nx - 1) + 1;
+  const int self = g.lo[1] + (i - g.rs[1]);
+  uint32_t m[3];
+  int l[3];
+  bool small = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    m[k] = 0u;
+    l[k] = 0;
+    if (g.rl[k] == 0) continue;
+    const int ta = B.at(k, ca);
+    const int tb = B.at(k, cb);
+    l[k] = g.lo[k] - g.rs[k] + ta;
+    small = small && tb - ta <= 32;
+    m[k] = row_mask<R>(K, T, pi, (R)row_uy(K, k), l[k], min(32, tb - ta), self);
+  }
+  if (!small) {
+    for_neighbours<R>(K, g, B, T, pi, i, key, visit, rm, ok);
+    return;
+  }
+  walk_rows(m[0], m[1], m[2], l[0], l[1], l[2], (R)row_uy(K, 0), (R)row_uy(K, 1),
+            (R)row_uy(K, 2), visit);
+  rm[0] = m[0];
+  rm[1] = m[1];
+  rm[2] = m[2];
+}
+
 // FP32 density pass: the neighbour pre-test of candidates l0 .. l0+len-1
 // (len <= 32) of one row against K.pre2f, with x = cx cell + off rounded
 // once per record (xi, yi: the lane's x, and its y plus the row term); the
@@ -695,7 +735,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
     if constexpr (sizeof(R) == 4) {
       dens_walk_f32<SUM>(K, g, B, T, pi, vi, i, skey[i], o, rm, &ok);
     } else {
-      for_neighbours<R>(
+      for_neighbours_rows<R>(
           K, g, B, T, pi, i, skey[i],
           [&](int l, R uy) {
             R dx, dy;
@@ -799,7 +839,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 4
     const uint32_t key = skey[i];
     const uint4 nm = nbm ? nbm[i] : make_uint4(0u, 0u, 0u, 0u);
     if (nm.w && sizeof(R) == 4) {
-      // FP32: the row-refill walk, the row term from the tile record
+      // FP32: the row-refill walk, the row term from the walk
       const int cx = (int)key - g.cy * K.nx;
       const int ca = cx > 0 ? cx - 1 : 0;
       const int l0 = g.rl[0] ? g.lo[0] - g.rs[0] + B.at(0, ca) : 0;
@@ -819,6 +859,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 4
         walk_rows(nm.x, nm.y, nm.z, l0, l1, l2, u0, u1, u2, visit);
       }
     } else if (nm.w) {
+      // FP64: walk_masks (walk_rows measured 0.6 % slower here)
       for_masked<R>(K, g, B, key, nm, visit);
     } else {
       for_neighbours<R>(K, g, B, T, pi, i, key, visit);
