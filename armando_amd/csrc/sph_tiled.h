@@ -700,6 +700,19 @@ __device__ __forceinline__ void for_masked(const KParams& K, const Segment& g, c
                 visit);
 }
 
+// The same with walk_rows (a bit scan per neighbour, the row change only
+// where a lane needs it).
+template <typename R, typename BD, typename Visit>
+__device__ __forceinline__ void for_masked_rows(const KParams& K, const Segment& g, const BD& B,
+                                                uint32_t key, uint4 nm, Visit&& visit) {
+  const int nx = K.nx;
+  const int cx = (int)key - g.cy * nx;
+  const int ca = cx > 0 ? cx - 1 : 0;
+  walk_rows(nm.x, nm.y, nm.z, g.rl[0] ? g.lo[0] - g.rs[0] + B.at(0, ca) : 0,
+            g.lo[1] - g.rs[1] + B.at(1, ca), g.rl[2] ? g.lo[2] - g.rs[2] + B.at(2, ca) : 0,
+            (R)row_uy(K, 0), (R)row_uy(K, 1), (R)row_uy(K, 2), visit);
+}
+
 template <typename R, bool SUM, bool EOS, bool PER, int BLK, int CW, int CAP>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && !PER ? 8 : 1))) void k_density_tiled(St<R> S, KParams K,
                                                        const uint32_t* __restrict__ skey,
@@ -789,7 +802,9 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
     };
     const uint32_t key = skey[i];
     const uint4 nm = nbm ? nbm[i] : make_uint4(0u, 0u, 0u, 0u);
-    if (nm.w)
+    if (nm.w && sizeof(R) == 4)
+      for_masked_rows<R>(K, g, B, key, nm, visit);
+    else if (nm.w)
       for_masked<R>(K, g, B, key, nm, visit);
     else
       for_neighbours<R>(K, g, B, T, pi, i, key, visit);
