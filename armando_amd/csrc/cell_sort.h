@@ -6,9 +6,11 @@
 //                   count[key] += run, where a run is a group of adjacent
 //                   lanes of a wavefront with the same key: one atomic per
 //                   run; in sorted order most of a cell is one run
-//   k_cs_tile_sum   sums of CS_TILE consecutive counts
-//   k_cs_tile_scan  exclusive scan of the counts -> cell_start[0 .. ncell]
-//                   (each workgroup adds up the tile sums before its own)
+//   k_cs_scan1      exclusive scan of the counts -> cell_start[0 .. ncell]
+//                   in one launch (each workgroup publishes its tile's sum
+//                   and adds up those of the tiles before its own); the
+//                   two-launch form k_cs_tile_sum + k_cs_tile_scan serves the
+//                   allocation-free entry point and the ghost numbering
 //   k_cs_scatter    every particle takes a place in its cell range (count[key]
 //                   -= run restores the counters to zero); the order inside a
 //                   cell is arbitrary here

@@ -349,8 +349,9 @@ int arm_link_list_static(const arm_params* prm, int n, const double* x, int32_t*
                          int32_t* id_sorted, int32_t* cell_start, int32_t* outside,
                          void* stream);
 
-/* Select the gather implementation: 1 (default) LDS-tiled cell segments,
-   0 direct per-particle gathers from global memory.  Both visit neighbours
+/* Select the gather implementation: 1 (default) LDS-tiled segments of cell
+   rows (pieces of 128 sorted particles of a row, or runs of cells), 0 direct
+   per-particle gathers from global memory.  Both visit neighbours
    in the same order with the same arithmetic and give identical results.
    In 3D the direct gathers are the only ones (the switch has no effect). */
 int arm_set_gather_path(arm_ctx* ctx, int tiled);
