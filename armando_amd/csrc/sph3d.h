@@ -320,26 +320,32 @@ __device__ __forceinline__ void density3_direct(St3<R>& S, const KParams& K, int
   DensOwn<R> o;
   dens_begin<R, SUM>(K, o);
   bool ok = true;
-  // FP32 form, per row: the neighbour test of the first 96 candidates,
-  // eight at a time with no branch, into three mask words; then the pair
-  // sums over the set bits (one loop over the three words); then any
-  // candidates past 96 test and sum one by one.  The pairs are summed in
-  // ascending position, as the single test-and-sum walk of the FP64 form
-  // (below) does.
+  // FP32 form: the neighbour pre-test of every row's candidates (at most 96
+  // per row), eight at a time with no branch, into three mask words per row
+  // kept in the lane's own LDS column; then one loop over the set bits of
+  // all 27 words in (row, word, bit) order, each survivor through the exact
+  // test and the pair sums (a wavefront runs for its longest lane's survivor
+  // count, not for the sum of the nine per-row maxima).  A lane with a row of
+  // more than 96 candidates tests and sums its candidates one by one
+  // instead.  Either way the pairs are summed in ascending position, row by
+  // row, as the single test-and-sum walk of the FP64 form (below) does.
   if constexpr (sizeof(R) == 4) {
     constexpr int NB = 8;
+    __shared__ uint32_t lw[M3_OK][BLOCK];
+    __shared__ int lt[9][BLOCK];
+    const int lane = threadIdx.x;
     // pre-test positions: x = cx cell + off rounded once (K.pre2f bounds
-    // the exact test's cut-off through that rounding); the exact test runs
-    // on the survivors in the pair walk below
+    // the exact test's cut-off through that rounding)
     const float cellf = K.cellf;
     const float xi = __builtin_fmaf(pi.cxf, cellf, pi.ox);
     const float pre2 = K.pre2f;
+    uint32_t nz = 0u;  // the lane's non-empty words
+#pragma unroll
     for (int k = 0; k < 9; ++k) {
       const int t0 = rw.t0[k], t1 = rw.t1[k];
-      const R uy = row_u3<R>(K, 1 - k % 3), uz = row_u3<R>(K, 1 - k / 3);
       if (t1 - t0 > 32 * M3_WORDS) ok = false;
       const int tA = t1 < t0 + 32 * M3_WORDS ? t1 : t0 + 32 * M3_WORDS;
-      const float yi = pi.oy + uy, zi = pi.oz + uz;
+      const float yi = pi.oy + row_u3<float>(K, 1 - k % 3), zi = pi.oz + row_u3<float>(K, 1 - k / 3);
       uint32_t m0 = 0, m1 = 0, m2 = 0;
       for (int c0 = t0; c0 < tA; c0 += NB) {
         // a batch starts a multiple of NB past t0, so its bits lie in one
@@ -347,7 +353,7 @@ __device__ __forceinline__ void density3_direct(St3<R>& S, const KParams& K, int
         // position records are padded by NB - 1 past the last particle.
         uint32_t bb = 0u;
         const float4* pb = S.pf + c0;
-  #pragma unroll
+#pragma unroll
         for (int q = 0; q < NB; ++q) {
           const float4 pj = pb[q];
           const float dx = xi - __builtin_fmaf(pj.w, cellf, pj.x);
@@ -374,49 +380,82 @@ __device__ __forceinline__ void density3_direct(St3<R>& S, const KParams& K, int
           m2 &= sb >= 64 ? keep : ~0u;
         }
       }
-      {
-        uint32_t w0 = m0, w1 = m1, w2 = m2;
-        while (w0 | w1 | w2) {
-          const bool s0 = w0 != 0u, s1 = !s0 && w1 != 0u;
-          const uint32_t m = s0 ? w0 : (s1 ? w1 : w2);
-          const int t = t0 + (s0 ? 0 : (s1 ? 32 : 64)) + (int)__builtin_ctz(m);
-          const uint32_t rest = m & (m - 1u);
-          w0 = s0 ? rest : w0;
-          w1 = s1 ? rest : w1;
-          w2 = (s0 || s1) ? w2 : rest;
+      lw[3 * k + 0][lane] = m0;
+      lw[3 * k + 1][lane] = m1;
+      lw[3 * k + 2][lane] = m2;
+      nz |= (m0 != 0u ? 1u : 0u) << (3 * k);
+      nz |= (m1 != 0u ? 2u : 0u) << (3 * k);
+      nz |= (m2 != 0u ? 4u : 0u) << (3 * k);
+      lt[k][lane] = t0;
+    }
+    if (ok) {
+      // the word's first candidate and its row's y and z terms are found
+      // when the walk enters the word; a survivor that fails the exact test
+      // leaves the lane's copy of the word
+      uint32_t m = 0u;
+      int tb = 0, wc = 0;
+      float uy = 0.f, uz = 0.f;
+      while (true) {
+        if (m == 0u) {
+          if (nz == 0u) break;
+          const int w = __builtin_ctz(nz);
+          nz &= nz - 1u;
+          m = lw[w][lane];
+          wc = w;
+          const int k = (w * 11) >> 5;  // w / 3 for w < 27
+          const int q = w - 3 * k;
+          const int ky = (k * 11) >> 5;  // k / 3 for k < 9
+          uy = row_u3<float>(K, 1 - (k - 3 * ky));
+          uz = row_u3<float>(K, 1 - ky);
+          tb = lt[k][lane] + 32 * q;
+        }
+        const int b = __builtin_ctz(m);
+        m &= m - 1u;
+        const int t = tb + b;
+        const auto pj = load_pos3(S, t);
+        const auto vj = S.v[t];
+        float dx, dy, dz;
+        const float r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
+        // both loads in flight together: keep the velocity load out of
+        // the branch below
+        asm volatile("" ::"v"(vj.x), "v"(vj.y), "v"(vj.z));
+        if (!(r2 < cut2)) {
+          lw[wc][lane] &= ~(1u << b);  // passed the pre-test only
+          continue;
+        }
+        dens3_pair<R, SUM>(K, dx, dy, dz, r2, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, o.sum,
+                           o.rs);
+      }
+      if (nbm.w)
+        for (int q = 0; q < M3_OK; ++q) nbm.put(q, i, lw[q][lane]);
+    } else {
+      // a row of more than 96 candidates: test and sum one by one, in the
+      // same order; masks of the first 96 candidates of each row
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const int t0 = rw.t0[k], t1 = rw.t1[k];
+        const float uy = row_u3<float>(K, 1 - k % 3), uz = row_u3<float>(K, 1 - k / 3);
+        uint32_t m0 = 0, m1 = 0, m2 = 0;
+        for (int t = t0; t < t1; ++t) {
+          if (t == i) continue;
           const auto pj = load_pos3(S, t);
+          float dx, dy, dz;
+          const float r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
+          if (!(r2 < cut2)) continue;
+          const int b = t - t0;
+          const uint32_t bit = 1u << (b & 31);
+          m0 |= b < 32 ? bit : 0u;
+          m1 |= (b >= 32 && b < 64) ? bit : 0u;
+          m2 |= (b >= 64 && b < 96) ? bit : 0u;
           const auto vj = S.v[t];
-          R dx, dy, dz;
-          const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
-          // both loads in flight together: keep the velocity load out of
-          // the branch below
-          asm volatile("" ::"v"(vj.x), "v"(vj.y), "v"(vj.z));
-          if (!(r2 < cut2)) {
-            // passed the pre-test only: not a neighbour
-            const int b = t - t0;
-            const uint32_t keep = ~(1u << (b & 31));
-            m0 &= b < 32 ? keep : ~0u;
-            m1 &= (b >= 32 && b < 64) ? keep : ~0u;
-            m2 &= b >= 64 ? keep : ~0u;
-            continue;
-          }
           dens3_pair<R, SUM>(K, dx, dy, dz, r2, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, o.sum,
                              o.rs);
         }
-      }
-      for (int t = tA; t < t1; ++t) {
-        if (t == i) continue;
-        const auto pj = load_pos3(S, t);
-        R dx, dy, dz;
-        const R r2 = delta3(pi, pj, K, uy, uz, dx, dy, dz);
-        if (!(r2 < cut2)) continue;
-        const auto vj = S.v[t];
-        dens3_pair<R, SUM>(K, dx, dy, dz, r2, vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, o.sum, o.rs);
-      }
-      if (nbm.w) {
-        nbm.put(k * M3_WORDS + 0, i, m0);
-        nbm.put(k * M3_WORDS + 1, i, m1);
-        nbm.put(k * M3_WORDS + 2, i, m2);
+        if (nbm.w) {
+          nbm.put(k * M3_WORDS + 0, i, m0);
+          nbm.put(k * M3_WORDS + 1, i, m1);
+          nbm.put(k * M3_WORDS + 2, i, m2);
+        }
       }
     }
   } else {
