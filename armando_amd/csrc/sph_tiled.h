@@ -884,8 +884,8 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 4
 }
 
 // Launch shapes: cells per segment (0: row pieces of PIECE particles), lanes,
-// LDS record capacity.  Both forms run on row pieces; the cell runs (CW > 0)
-// remain as an alternative shape.
+// LDS record capacity.  Both forms run on row pieces, except the force
+// gathers with the physical viscosity (ForceCfg), which run on cell runs.
 template <typename R> struct TileCfg;
 template <> struct TileCfg<float> {
   static constexpr int BLK = PIECE, CW = 0, CAP = 640;
