@@ -1,22 +1,24 @@
 // sph_tiled.h -- LDS-tiled gathers of the density and force passes.
 //
-// A workgroup owns a segment of CW consecutive cells of one cell row.  Its
-// particles are one contiguous sorted range, and the candidates of all of
-// them lie in three contiguous sorted ranges (the cell row below, the row
-// itself and the row above, cells cx0-1 .. cx0+CW).  The workgroup copies
-// those three ranges, and the cell_start entries that delimit their cells,
-// into LDS with coalesced loads.  Each lane then handles one particle, row
-// by row:
+// A workgroup owns a segment of one cell row: a piece of up to 128
+// consecutive sorted particles of the row (or, for the viscous force
+// gathers, a run of CW cells).  Its particles are one contiguous sorted
+// range, and the candidates of all of them lie in three contiguous sorted
+// ranges (the cell row below, the row itself and the row above, over the
+// segment's cells and one either side).  The workgroup copies those three
+// ranges into LDS with coalesced loads.  Each lane then handles one
+// particle, row by row:
 //   A  test the row's candidates (r^2 < (2h)^2) eight at a time, reading
-//      their positions from LDS with independent loads, into a 64-bit
-//      neighbour mask;
+//      their positions from LDS with independent loads, into a 32-bit row
+//      mask;
 //   B  run the pair arithmetic over the set bits in ascending order, so the
-//      wavefront does the expensive math only for real neighbours.
+//      wavefront does the expensive math only for real neighbours (one walk
+//      over the three rows' masks where they are complete).
 // The visiting order (rows bottom to top, ascending sorted position) and the
 // arithmetic are those of the direct path, so both give identical bits.
-// Rows longer than 64 candidates are handled in 64-candidate chunks; when
-// the three ranges exceed the LDS capacity the workgroup falls back to the
-// direct path.
+// Rows longer than 32 candidates are handled in 32-candidate chunks (their
+// masks are not recorded); when the three ranges exceed the LDS capacity the
+// workgroup falls back to the direct path.
 #pragma once
 
 #include "sph_kernels.h"
