@@ -206,7 +206,10 @@ __global__ __launch_bounds__(CS_THREADS) void k_cs_tile_scan(const int* __restri
 // grid's workgroups run together, so the words are there within a few
 // microseconds).  A word is (epoch << 32) | sum; a word of another epoch (an
 // earlier sort) reads as not yet published.  The host gives every sort a new
-// epoch (1 .. SCAN_EPOCH_MAX), so the words need no clearing between sorts.
+// epoch (1 .. SCAN_EPOCH_MAX), so the words need no clearing between sorts;
+// a launch replayed with the same epoch (a captured graph) would read the
+// previous replay's words, so the allocation-free entry point, which a
+// caller may capture, keeps the two-launch scan.
 constexpr uint32_t SCAN_EPOCH_MAX = 0xffffffffu;  // the host clears the words and restarts at 1
 constexpr int SCAN_SPIN_LIMIT = 1 << 14;
 
