@@ -506,10 +506,9 @@ template <typename R> int eos_t(arm_ctx* c) {
 template <int PC> void launch_force_hot(arm_ctx* c, const uint4* nbm) {
   using T = HotForceCfg;
   const SegArg sa = seg_arg<T::CW>(c);
-  hipLaunchKernelGGL(
-      (k_force_tiled<float, true, true, true, true, false, false, T::BLK, T::CW, T::CAP, PC>),
-      dim3(seg_grid<T::CW>(c, sa)), dim3(T::BLK), 0, c->stream, stA<float>(c), c->K, c->skey,
-      c->cstart, sa, nbm);
+  hipLaunchKernelGGL((k_force_hot<T::BLK, T::CAP, PC>), dim3(seg_grid<T::CW>(c, sa)),
+                     dim3(T::BLK), 0, c->stream, stA<float>(c), c->K, c->skey, c->cstart, sa,
+                     nbm);
 }
 
 template <typename R, bool P, bool AV, bool AH, bool XS, bool VI = false>

@@ -174,17 +174,6 @@ template <int CAP, bool VI> struct ForceTile<float, CAP, VI> {
     force_pair<float, PRESS, AV, AH, XS, VI, PC>(K, o, dx, dy, r2, q.z, q.w, w.x, w.y, w.z,
                                                  w.w, tw.get(l));
   }
-  // the terms of the pair with record l of the row with term uy,
-  // unaccumulated (no physical viscosity)
-  template <bool PRESS, bool AV, bool AH, bool XS, int PC>
-  __device__ __forceinline__ TermsF terms(const KParams& K, const ForceOwn<float>& o,
-                                          const LaneF& p, int l, float uy) const {
-    float dx, dy;
-    delta(K, p, l, uy, dx, dy);
-    const float4 q = a[l];
-    const float4 w = b[l];
-    return force_terms_f32<PRESS, AV, AH, XS, PC>(K, o, dx, dy, q.z, q.w, w.x, w.y, w.z, w.w);
-  }
 };
 template <int CAP, bool VI> struct ForceTile<double, CAP, VI> {
   double2 x[CAP + TPAD], v[CAP + TPAD], rp[CAP + TPAD], pu[CAP + TPAD];  // rp = (Q, A), pu = (psi, u)
@@ -463,42 +452,6 @@ __device__ __forceinline__ void walk_rows(uint32_t m0, uint32_t m1, uint32_t m2,
     const int b = (int)__builtin_ctz(m);
     m &= m - 1u;
     visit(base + b, uy);
-  }
-}
-
-// walk_rows two neighbours at a time: visit(la, ua, lb, ub, hb) with lb the
-// next neighbour after la in the same order when hb (else lb = la, to be
-// ignored), so that the pair arithmetic of two neighbours can overlap.
-template <typename Visit2>
-__device__ __forceinline__ void walk_rows2(uint32_t m0, uint32_t m1, uint32_t m2, int l0, int l1,
-                                           int l2, float u0, float u1, float u2, Visit2&& visit) {
-  uint32_t m = m0;
-  int base = l0;
-  float uy = u0;
-  auto refill = [&]() {
-    const bool t1 = m1 != 0u;
-    m = t1 ? m1 : m2;
-    base = t1 ? l1 : l2;
-    uy = t1 ? u1 : u2;
-    m1 = t1 ? m2 : 0u;
-    l1 = l2;
-    u1 = u2;
-    m2 = 0u;
-  };
-  while (true) {
-    if (m == 0u) {
-      refill();
-      if (m == 0u) break;
-    }
-    const int la = base + (int)__builtin_ctz(m);
-    const float ua = uy;
-    m &= m - 1u;
-    if (m == 0u) refill();
-    const bool hb = m != 0u;
-    const int lb = hb ? base + (int)__builtin_ctz(m) : la;
-    const float ub = hb ? uy : ua;
-    m = hb ? (m & (m - 1u)) : m;
-    visit(la, ua, lb, ub, hb);
   }
 }
 
@@ -815,10 +768,10 @@ __global__ __launch_bounds__(BLK) void k_tensor_tiled(St<R> S, KParams K,
 }
 
 // PC: pa_sph and same_e2 fixed at compile time for the FP32 pair body
-// (force_pair_f32), -1 to read them from K.
+// (force_pair_f32), -1 to read them from K (k_force_hot fixes them).
 template <typename R, bool PRESS, bool AV, bool AH, bool XS, bool VI, bool PER, int BLK, int CW,
           int CAP, int PC = -1>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 4 : 1))) void k_force_tiled(St<R> S, KParams K,
+__global__ __launch_bounds__(BLK) void k_force_tiled(St<R> S, KParams K,
                                                      const uint32_t* __restrict__ skey,
                                                      const int* __restrict__ cstart, SegArg sa,
                                                      const uint4* __restrict__ nbm) {
@@ -863,18 +816,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 4
       const int l1 = g.lo[1] - g.rs[1] + B.at(1, ca);
       const int l2 = g.rl[2] ? g.lo[2] - g.rs[2] + B.at(2, ca) : 0;
       const R u0 = (R)row_uy(K, 0), u1 = (R)row_uy(K, 1), u2 = (R)row_uy(K, 2);
-      if constexpr (sizeof(R) == 4 && PC >= 0 && !VI) {
-        // two neighbours per step; each added in walk order
-        walk_rows2(nm.x, nm.y, nm.z, l0, l1, l2, u0, u1, u2,
-                   [&](int la, float ua, int lb, float ub, bool hb) {
-                     const TermsF ta = T.template terms<PRESS, AV, AH, XS, PC>(K, o, pi, la, ua);
-                     const TermsF tb = T.template terms<PRESS, AV, AH, XS, PC>(K, o, pi, lb, ub);
-                     force_acc_f32<PRESS, AV, AH, XS>(o, ta, true);
-                     force_acc_f32<PRESS, AV, AH, XS>(o, tb, hb);
-                   });
-      } else {
-        walk_rows(nm.x, nm.y, nm.z, l0, l1, l2, u0, u1, u2, visit);
-      }
+      walk_rows(nm.x, nm.y, nm.z, l0, l1, l2, u0, u1, u2, visit);
     } else if (nm.w) {
       // FP64: walk_masks (walk_rows measured 0.6 % slower here)
       for_masked<R>(K, g, B, key, nm, visit);
@@ -885,6 +827,141 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PC >= 0 ? 4
   }
 }
 
+// ---- the hot FP32 force gather ---------------------------------------------
+// The dam-break flag set (pressure, artificial viscosity and heat, XSPH, no
+// physical viscosity, x not periodic) with pa_sph and same_e2 fixed at
+// compile time (PC), on row pieces, with 32-byte records: the record's cell
+// column is not staged.  For a neighbour in row k of a lane in cell column
+// cx, (cx_i - cx_j) cell is +cell, 0 or -cell as the neighbour's tile entry
+// lies before the row's cell cx, in it, or after it (e1_k, e2_k: the entries
+// where cells cx and cx+1 start), the exact value the FP32 delta computes.
+// The smaller tile lets eight workgroups share a CU.  Particles without
+// complete masks take the direct path.
+template <int CAP> struct HotForceTile {
+  float4 a[CAP];  // offx, offy, vx, vy
+  float4 b[CAP];  // Q = 1/rho, A = p/rho, psi, u
+  __device__ __forceinline__ void put(int l, const St<float>& S, const KParams& K, int t) {
+    const float2 o = S.off[t];
+    const float2 v = S.v[t];
+    a[l] = make_float4(o.x, o.y, v.x, v.y);
+    float Q, A;
+    q_and_a(K, S.rho[t], S.p[t], Q, A);
+    b[l] = make_float4(Q, A, S.psi[t], S.uf[t]);
+  }
+};
+
+// A row of the hot walk: first tile entry of the lane's three cells, the
+// entries where cells cx and cx+1 start, the row's cell-row term.
+struct HotRow {
+  int base, e1, e2;
+  float uy;
+};
+
+// walk_rows two neighbours at a time, with the rows' cell boundaries:
+// visit(la, ra, lb, rb, hb) with lb the next neighbour after la in the same
+// order when hb (else lb = la, to be ignored), so that the pair arithmetic
+// of two neighbours can overlap.
+template <typename Visit2>
+__device__ __forceinline__ void walk_rows2_hot(uint32_t m0, uint32_t m1, uint32_t m2, HotRow r0,
+                                               HotRow r1, HotRow r2, Visit2&& visit) {
+  uint32_t m = m0;
+  HotRow r = r0;
+  auto refill = [&]() {
+    const bool t1 = m1 != 0u;
+    m = t1 ? m1 : m2;
+    r.base = t1 ? r1.base : r2.base;
+    r.e1 = t1 ? r1.e1 : r2.e1;
+    r.e2 = t1 ? r1.e2 : r2.e2;
+    r.uy = t1 ? r1.uy : r2.uy;
+    m1 = t1 ? m2 : 0u;
+    r1 = r2;
+    m2 = 0u;
+  };
+  while (true) {
+    if (m == 0u) {
+      refill();
+      if (m == 0u) break;
+    }
+    const int la = r.base + (int)__builtin_ctz(m);
+    const HotRow ra = r;
+    m &= m - 1u;
+    if (m == 0u) refill();
+    const bool hb = m != 0u;
+    const int lb = hb ? r.base + (int)__builtin_ctz(m) : la;
+    HotRow rb;
+    rb.e1 = hb ? r.e1 : ra.e1;
+    rb.e2 = hb ? r.e2 : ra.e2;
+    rb.uy = hb ? r.uy : ra.uy;
+    m = hb ? (m & (m - 1u)) : m;
+    visit(la, ra, lb, rb, hb);
+  }
+}
+
+template <int BLK, int CAP, int PC>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k_force_hot(
+    St<float> S, KParams K, const uint32_t* __restrict__ skey, const int* __restrict__ cstart,
+    SegArg sa, const uint4* __restrict__ nbm) {
+  if (halted(K)) return;
+  __shared__ HotForceTile<CAP> T;
+  __shared__ int cs[3][csw<0>()];
+  const Segment g = segment_of<0>(K, cstart, skey, sa, cs);
+  if (g.O0 >= g.O1) return;
+  const auto B = bounds_of<0>(K, g, cstart, cs);
+  const bool tiled = g.total <= CAP;
+  if (tiled) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      for (int q = threadIdx.x; q < g.rl[k]; q += BLK) T.put(g.lo[k] + q, S, K, g.rs[k] + q);
+  }
+  __syncthreads();
+  for (int i0 = g.O0; i0 < g.O1; i0 += BLK) {
+    const int i = i0 + threadIdx.x;
+    if (i >= g.O1) continue;
+    const uint4 nm = nbm ? nbm[i] : make_uint4(0u, 0u, 0u, 0u);
+    if (!tiled || !nm.w) {
+      force_direct<float, true, true, true, true, false, false>(S, K, i, skey, cstart);
+      continue;
+    }
+    const int self = g.lo[1] + (i - g.rs[1]);
+    const float4 qs = T.a[self];
+    ForceOwn<float> o;
+    force_begin<float, false>(S, K, i, o);
+    const int cx = (int)skey[i] - g.cy * K.nx;
+    const int ca = cx > 0 ? cx - 1 : 0;
+    HotRow rw[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int d = g.lo[k] - g.rs[k];
+      const bool on = k == 1 || g.rl[k] != 0;  // rows outside the grid have empty masks
+      rw[k].base = on ? d + B.at(k, ca) : 0;
+      rw[k].e1 = on ? d + B.at(k, cx) : 0;
+      rw[k].e2 = on ? d + B.at(k, cx + 1) : 0;
+      rw[k].uy = row_uy(K, k);
+    }
+    const float cellf = K.cellf;
+    auto terms = [&](int l, const HotRow& r) {
+      const float4 q = T.a[l];
+      const float4 w = T.b[l];
+      const float ux = l < r.e1 ? cellf : (l < r.e2 ? 0.f : -cellf);
+      const float tx = qs.x - q.x;
+      const float dx = tx + ux;
+      const float ty = qs.y - q.y;
+      const float dy = ty + r.uy;
+      return force_terms_f32<true, true, true, true, PC>(K, o, dx, dy, q.z, q.w, w.x, w.y, w.z,
+                                                         w.w);
+    };
+    // two neighbours per step; each added in walk order
+    walk_rows2_hot(nm.x, nm.y, nm.z, rw[0], rw[1], rw[2],
+                   [&](int la, const HotRow& ra, int lb, const HotRow& rb, bool hb) {
+                     const TermsF ta = terms(la, ra);
+                     const TermsF tb = terms(lb, rb);
+                     force_acc_f32<true, true, true, true>(o, ta, true);
+                     force_acc_f32<true, true, true, true>(o, tb, hb);
+                   });
+    force_end<float, true, true, false>(S, K, i, o);
+  }
+}
+
 // Launch shapes: cells per segment (0: row pieces of PIECE particles), lanes,
 // LDS record capacity.  Both forms run on row pieces, except the force
 // gathers with the physical viscosity (ForceCfg), which run on cell runs.
@@ -892,12 +969,11 @@ template <typename R> struct TileCfg;
 template <> struct TileCfg<float> {
   static constexpr int BLK = PIECE, CW = 0, CAP = 640;
 };
-// The FP32 force instances with compile-time pair constants (PC >= 0): row
-// pieces; the tile (600 records of 36 bytes, 21.9 KB) lets seven workgroups
-// share a CU (a piece's three rows hold about 3 x 140 records on the
-// dam-break lattice, 584 at most).
+// The hot FP32 force gather (k_force_hot): row pieces; the tile (592 records
+// of 32 bytes, 18.9 KB) lets eight workgroups share a CU (a piece's three
+// rows hold about 3 x 140 records on the dam-break lattice, 584 at most).
 struct HotForceCfg {
-  static constexpr int BLK = PIECE, CW = 0, CAP = 600;
+  static constexpr int BLK = PIECE, CW = 0, CAP = 592;
 };
 template <> struct TileCfg<double> {
   static constexpr int BLK = PIECE, CW = 0, CAP = 600;

@@ -581,7 +581,7 @@ def main():
     t_force = pass_ms(r32, "force") * 1e-3
     bytes_force = n * BYTES[("force", 1)]
     achieved = bytes_force / t_force / 1e9
-    prof = profile_figures("arm::k_force_tiled<float") if args.scale == 1.0 else None
+    prof = profile_figures("arm::k_force_hot<") if args.scale == 1.0 else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -608,7 +608,7 @@ def main():
         "pair_interactions_per_s": 2 * r32["niac"] / t_force,
         "passes_ms": {k: pass_ms(r32, k) for k in r32["passes"]},
         "roofline": {
-            "kernel": "k_force_tiled<float,...> (internal_force + art_visc + art_heat + av_vel)",
+            "kernel": "k_force_hot<...> (internal_force + art_visc + art_heat + av_vel)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
