@@ -39,9 +39,10 @@ struct KParams {
   float pre2f;
   // FP32 roundings of the constants the FP32 pair bodies use, so that they
   // stay in scalar registers (a conversion in the kernel lands in a vector
-  // register); each is (float) of the double field it is named after
+  // register); each is (float) of the double field it is named after, apart
+  // from two products: avah = f_ava h and avbh2 = f_avb h^2
   struct {
-    float invh, h, cm, ava, avb, xs, ave2, ahe2, cond2, rho0, gx, gy, C1, mass, wnorm;
+    float invh, cm, avah, avbh2, xs, ave2, ahe2, cond2, rho0, gx, gy, C1, mass, wnorm;
   } kf;
   int eos;         // 0 Tait, 1 linear
   double lin_c2;   // linear EOS: p = lin_c2 rho + lin_p0
@@ -119,10 +120,9 @@ constexpr int KF_DEPOSITION = 1 << 24;
 
 __host__ __device__ inline void set_kf(KParams& K) {
   K.kf.invh = (float)K.invh;
-  K.kf.h = (float)K.h;
   K.kf.cm = (float)K.f_cm;
-  K.kf.ava = (float)K.f_ava;
-  K.kf.avb = (float)K.f_avb;
+  K.kf.avah = (float)(K.f_ava * K.h);
+  K.kf.avbh2 = (float)(K.f_avb * K.h * K.h);
   K.kf.xs = (float)K.f_xs;
   K.kf.ave2 = (float)K.av_e2;
   K.kf.ahe2 = (float)K.ah_e2;

@@ -503,7 +503,10 @@ template <typename R> struct ForceOwn {
   using R2 = typename V2<R>::type;
   R2 v;
   R Q, A, psi, u;
-  R q4;  // 4 Q (FP32 pair body: 2 / rho_bar = q4 Q_j / (Q + Q_j))
+  // FP32 pair body (force_terms_f32): with c4 = f_cm 4 Q, the factors
+  // cA = f_cm A, cQ = f_cm Q, cAQ = f_cm A Q, avb = c4 avbh2, ava = c4 avah,
+  // c4, c4psi = c4 psi, xw = f_xs 4 Q and xw2 = 2 xw
+  R cA, cQ, cAQ, avb, ava, c4, c4psi, xw, xw2;
   R wxx = 0, wyy = 0, wxy = 0, tds = 0;  // physical viscosity (tau of i)
   R fx = 0, fy = 0, xsx = 0, xsy = 0;
   Acc<R> du;
@@ -532,13 +535,31 @@ __device__ __forceinline__ void q_and_a(const KParams& K, R rho_s, R p, R& Q, R&
   A = p * Q;
 }
 
+// The FP32 pair body's factors of the particle and the constants of K.
+__device__ __forceinline__ void own_factors_f32(const KParams& K, ForceOwn<float>& o) {
+  const float cm = K.kf.cm, q4 = 4.f * o.Q, c4 = cm * q4;
+  o.cA = cm * o.A;
+  o.cQ = cm * o.Q;
+  o.cAQ = o.cA * o.Q;
+  o.avb = c4 * K.kf.avbh2;
+  o.ava = c4 * K.kf.avah;
+  o.c4 = c4;
+  o.c4psi = c4 * o.psi;
+  o.xw = K.kf.xs * q4;
+  o.xw2 = 2.f * o.xw;
+}
+
 template <typename R, bool VI = false>
 __device__ __forceinline__ void force_begin(const St<R>& S, const KParams& K, int i, ForceOwn<R>& o) {
   o.v = S.v[i];
   q_and_a(K, S.rho[i], S.p[i], o.Q, o.A);
-  o.q4 = (R)4 * o.Q;
   o.psi = S.psi[i];
-  if constexpr (sizeof(R) == 4) o.u = S.uf[i]; else o.u = (R)S.u[i];
+  if constexpr (sizeof(R) == 4) {
+    o.u = S.uf[i];
+    own_factors_f32(K, o);
+  } else {
+    o.u = (R)S.u[i];
+  }
   if (VI) {
     const R* w = S.tau + 4 * (size_t)i;
     const R c = visc_c(K, o.Q);
@@ -575,6 +596,13 @@ __device__ __forceinline__ void visc_pair(const KParams& K, ForceOwn<R>& o, R mF
 //    and halved once in force_end.
 //  - The viscosity term uses phi = min(vr, 0) h/(r^2 + (h eta)^2): zero for
 //    receding pairs, which then add an exact zero (the reference's vr < 0 test).
+//    (f_avb phi + f_ava) phi is evaluated as e (avbh2 e + avah) with
+//    e = phi / h, the factors of h folded into the two constants.
+//  - The particle's own factors f_cm and 4 Q are applied once per particle
+//    (ForceOwn: c4 = f_cm 4 Q and the products with it): per pair remain
+//    s^3, Q_j/(Q + Q_j) and their product M, which the viscosity, heat and
+//    XSPH terms share (m F imr2 = c4 M; f_xs s^4 (1 + 2q) imr2 =
+//    f_xs 4 Q (1 + 2q) s M).
 //  - r^2 of the arithmetic is one fused multiply-add (the neighbour test, done
 //    before, uses the unfused form).
 // PC < 0 reads pa_sph and same_e2 from K; PC >= 0 fixes them at compile time
@@ -600,44 +628,43 @@ __device__ __forceinline__ TermsF force_terms_f32(const KParams& K, const ForceO
   const float q = fsqrt(r2) * K.kf.invh;
   const float s = __builtin_fmaf(-0.5f, q, 1.0f);
   const float s2 = s * s;
-  const float mF = K.kf.cm * (s2 * s);
-  t.mF = mF;
+  const float s3 = s2 * s;
+  t.mF = K.kf.cm * s3;
   const float dvx = o.v.x - vjx, dvy = o.v.y - vjy;
   t.dvx = dvx;
   t.dvy = dvy;
   const float vr = __builtin_fmaf(dvy, dy, dvx * dx);
-  const float imr2 = o.q4 * Qj * frcp(o.Q + Qj);
-  float coef = 0.f;
-  if (PRESS) coef = pa0 ? __builtin_fmaf(o.A, Qj, Aj * o.Q) : __builtin_fmaf(o.A, o.Q, Aj * Qj);
+  const float M = s3 * (Qj * frcp(o.Q + Qj));
+  float G = 0.f;
+  if (PRESS)
+    G = s3 * (pa0 ? __builtin_fmaf(o.cA, Qj, Aj * o.cQ) : __builtin_fmaf(Aj * Qj, K.kf.cm, o.cAQ));
   float inv_ea = 0.f;
   if (AV) {
     inv_ea = frcp(r2 + K.kf.ave2);
-    const float phi = __builtin_fminf(vr, 0.f) * (K.kf.h * inv_ea);
-    const float pc = __builtin_fmaf(K.kf.avb, phi, K.kf.ava);
-    coef = __builtin_fmaf(pc * phi, imr2, coef);
+    const float e = __builtin_fminf(vr, 0.f) * inv_ea;
+    G = __builtin_fmaf(e * __builtin_fmaf(o.avb, e, o.ava), M, G);
   }
   t.G = 0.f;
   t.d = 0.f;
   if (PRESS || AV) {
-    const float G = mF * coef;
     t.G = G;
     float d = G * vr;
     float inv_eh = 0.f;
     if (AH) {
       inv_eh = (AV && se) ? inv_ea : frcp(r2 + K.kf.ahe2);
-      d = __builtin_fmaf((o.psi + psij) * mF * (r2 * inv_eh) * imr2, o.u - uj, d);
+      d = __builtin_fmaf(__builtin_fmaf(o.c4, psij, o.c4psi) * M * (r2 * inv_eh), o.u - uj, d);
     }
     if (PRESS && PC < 0 && (K.flags & 128)) {  // Cleary-Monaghan conduction
       if (!AH) inv_eh = frcp(r2 + K.kf.ahe2);
-      d = __builtin_fmaf(K.kf.cond2 * o.Q * Qj * mF * (r2 * inv_eh), o.u - uj, d);
+      d = __builtin_fmaf(K.kf.cond2 * o.Q * Qj * t.mF * (r2 * inv_eh), o.u - uj, d);
     }
     t.d = d;
   } else if (AH) {
     const float inv_eh = frcp(r2 + K.kf.ahe2);
-    t.d = (o.psi + psij) * mF * (r2 * inv_eh) * imr2 * (o.u - uj);
+    t.d = __builtin_fmaf(o.c4, psij, o.c4psi) * M * (r2 * inv_eh) * (o.u - uj);
   }
   t.wf = 0.f;
-  if (XS) t.wf = K.kf.xs * ((s2 * s2) * __builtin_fmaf(2.f, q, 1.f)) * imr2;
+  if (XS) t.wf = __builtin_fmaf(o.xw2, q, o.xw) * s * M;
   return t;
 }
 
@@ -1027,6 +1054,7 @@ __device__ __forceinline__ void force_direct_h(St<R>& S, const KParams& K, int i
     const auto vj = S.v[t];
     R Qj, Aj;
     q_and_a(K, S.rho[t], S.p[t], Qj, Aj);
+    if constexpr (sizeof(R) == 4) own_factors_f32(Kp, o);  // the pair's h
     force_pair<R, PRESS, AV, AH, XS, VI>(Kp, o, dx, dy, r2, vj.x, vj.y, Qj, Aj, S.psi[t],
                                          u_of(S, t),
                                          VI ? tau_of(S, t, visc_c(K, Qj)) : Tau3<R>{});

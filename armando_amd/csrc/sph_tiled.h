@@ -956,7 +956,15 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
                      const TermsF ta = terms(la, ra);
                      const TermsF tb = terms(lb, rb);
                      force_acc_f32<true, true, true, true>(o, ta, true);
-                     force_acc_f32<true, true, true, true>(o, tb, hb);
+                     // without a second neighbour G and wf are zeroed (adding
+                     // -0 dx to a sum that is never -0 leaves its bits), the
+                     // energy rate is skipped
+                     const float gb = hb ? tb.G : 0.f, wb = hb ? tb.wf : 0.f;
+                     o.fx = __builtin_fmaf(-gb, tb.dx, o.fx);
+                     o.fy = __builtin_fmaf(-gb, tb.dy, o.fy);
+                     o.du.add_if(tb.d, hb);
+                     o.xsx = __builtin_fmaf(-wb, tb.dvx, o.xsx);
+                     o.xsy = __builtin_fmaf(-wb, tb.dvy, o.xsy);
                    });
     force_end<float, true, true, false>(S, K, i, o);
   }
