@@ -40,9 +40,10 @@ struct KParams {
   // FP32 roundings of the constants the FP32 pair bodies use, so that they
   // stay in scalar registers (a conversion in the kernel lands in a vector
   // register); each is (float) of the double field it is named after, apart
-  // from two products: avah = f_ava h and avbh2 = f_avb h^2
+  // from four products: avah = f_ava h, avbh2 = f_avb h^2, mC1 = mass C1 and
+  // mw = mass wnorm
   struct {
-    float invh, cm, avah, avbh2, xs, ave2, ahe2, cond2, rho0, gx, gy, C1, mass, wnorm;
+    float invh, cm, avah, avbh2, xs, ave2, ahe2, cond2, rho0, gx, gy, C1, mass, wnorm, mC1, mw;
   } kf;
   int eos;         // 0 Tait, 1 linear
   double lin_c2;   // linear EOS: p = lin_c2 rho + lin_p0
@@ -133,6 +134,8 @@ __host__ __device__ inline void set_kf(KParams& K) {
   K.kf.C1 = (float)K.C1;
   K.kf.mass = (float)K.mass;
   K.kf.wnorm = (float)K.wnorm;
+  K.kf.mC1 = (float)(K.mass * K.C1);
+  K.kf.mw = (float)(K.mass * K.wnorm);
 }
 
 // K with the smoothing length mh: h, 1/h, the Wendland 2D constants, the

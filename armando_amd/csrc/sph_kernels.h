@@ -410,6 +410,19 @@ __device__ __forceinline__ void for_candidates(const KParams& K, uint32_t key,
 template <typename R, bool SUM>
 __device__ __forceinline__ void dens_pair(const KParams& K, R dx, R dy, R r2, R dvx, R dvy,
                                           Acc<R>& sum, Acc<R>& rs, bool ok = true) {
+  if constexpr (sizeof(R) == 4) {
+    // m dW.dv = m C1 (q - 2)^3 (dx dvx + dy dvy), m C1 one constant
+    const float q = fsqrt(r2) * K.kf.invh;
+    const float tq = q - 2.f;
+    const float vr = __builtin_fmaf(dvy, dy, dvx * dx);
+    sum.add_if(K.kf.mC1 * ((tq * tq) * tq * vr), ok);
+    if (SUM) {
+      const float s = __builtin_fmaf(-0.5f, q, 1.f);
+      const float s2 = s * s;
+      rs.add_if(K.kf.mw * ((s2 * s2) * __builtin_fmaf(2.f, q, 1.f)), ok);
+    }
+    return;
+  }
   const R q = fsqrt(r2) * (R)K.invh;
   const R tq = q - (R)2;
   const R F = (R)K.C1 * (tq * tq * tq);
